@@ -1,0 +1,548 @@
+// hprqp_capi.hip -- extern "C" entry points of libhprqp (include/hprqp.h).
+// Each entry catches every exception and turns it into a status code plus a
+// per-context message; nothing is thrown across the ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "hprqp_internal.hpp"
+
+using hprqp::DevCsr;
+using hprqp::Error;
+using hprqp::HostCsr;
+
+namespace hprqp {
+
+HostCsr host_csr_from(int32_t nrow, int32_t ncol, int64_t nnz, const int32_t *rowptr, const int32_t *colidx,
+                      const double *vals, int index_base) {
+    HPRQP_REQUIRE(nrow >= 0 && ncol >= 0 && nnz >= 0, HPRQP_E_ARG, "negative CSR dimension");
+    HPRQP_REQUIRE(index_base == 0 || index_base == 1, HPRQP_E_ARG, "index_base must be 0 or 1");
+    HPRQP_REQUIRE(nnz < INT32_MAX, HPRQP_E_ARG, "nnz exceeds int32 indexing");
+    HostCsr h;
+    h.nrow = nrow;
+    h.ncol = ncol;
+    h.rp.assign((size_t)nrow + 1, 0);
+    if (nrow > 0) HPRQP_REQUIRE(rowptr != nullptr, HPRQP_E_ARG, "NULL rowptr");
+    for (int32_t i = 0; i <= nrow && nrow > 0; ++i) h.rp[i] = rowptr[i] - index_base;
+    HPRQP_REQUIRE(nrow == 0 || (h.rp[0] == 0 && h.rp[nrow] == nnz), HPRQP_E_ARG, "rowptr inconsistent with nnz");
+    h.ci.resize((size_t)nnz);
+    h.v.resize((size_t)nnz);
+    for (int32_t i = 0; i < nrow; ++i) {
+        HPRQP_REQUIRE(h.rp[i] <= h.rp[i + 1], HPRQP_E_ARG, "rowptr not monotone");
+        std::vector<std::pair<int32_t, double>> row;
+        row.reserve((size_t)(h.rp[i + 1] - h.rp[i]));
+        for (int32_t k = h.rp[i]; k < h.rp[i + 1]; ++k) {
+            const int32_t j = colidx[k] - index_base;
+            HPRQP_REQUIRE(j >= 0 && j < ncol, HPRQP_E_ARG, "column index out of range");
+            HPRQP_REQUIRE(std::isfinite(vals[k]), HPRQP_E_DATA, "matrix contains NaN or Inf values");
+            row.emplace_back(j, vals[k]);
+        }
+        std::sort(row.begin(), row.end(), [](auto &a, auto &b) { return a.first < b.first; });
+        for (size_t t = 1; t < row.size(); ++t)
+            HPRQP_REQUIRE(row[t].first != row[t - 1].first, HPRQP_E_ARG, "duplicate entry in CSR row");
+        for (size_t t = 0; t < row.size(); ++t) {
+            h.ci[h.rp[i] + t] = row[t].first;
+            h.v[h.rp[i] + t] = row[t].second;
+        }
+    }
+    return h;
+}
+
+HostCsr host_csr_transpose(const HostCsr &a) {
+    HostCsr t;
+    t.nrow = a.ncol;
+    t.ncol = a.nrow;
+    const int64_t nnz = a.v.size();
+    t.rp.assign((size_t)t.nrow + 1, 0);
+    t.ci.resize((size_t)nnz);
+    t.v.resize((size_t)nnz);
+    for (int64_t k = 0; k < nnz; ++k) t.rp[a.ci[k] + 1]++;
+    for (int32_t i = 0; i < t.nrow; ++i) t.rp[i + 1] += t.rp[i];
+    std::vector<int32_t> next(t.rp.begin(), t.rp.end());
+    for (int32_t i = 0; i < a.nrow; ++i)
+        for (int32_t k = a.rp[i]; k < a.rp[i + 1]; ++k) {
+            const int32_t p = next[a.ci[k]]++;
+            t.ci[p] = i;
+            t.v[p] = a.v[k];
+        }
+    return t;
+}
+
+int default_lanes(const HostCsr &h) {
+    if (h.nrow <= 0) return 1;
+    const double mean = (double)h.v.size() / (double)h.nrow;
+    int L = 1;
+    while (L * 2 <= mean && L < 64) L *= 2;
+    return L;
+}
+
+void upload_csr(hipStream_t s, const HostCsr &h, DevCsr &d) {
+    d.nrow = h.nrow;
+    d.ncol = h.ncol;
+    d.nnz = (int64_t)h.v.size();
+    d.rp.alloc((int64_t)h.rp.size());
+    if (!h.rp.empty())
+        HPRQP_HIP(hipMemcpyAsync(d.rp.p, h.rp.data(), sizeof(int32_t) * h.rp.size(), hipMemcpyHostToDevice, s));
+    d.ci.alloc(d.nnz);
+    d.v.alloc(d.nnz);
+    d.v32.release();
+    if (d.nnz > 0) {
+        HPRQP_HIP(hipMemcpyAsync(d.ci.p, h.ci.data(), sizeof(int32_t) * d.nnz, hipMemcpyHostToDevice, s));
+        HPRQP_HIP(hipMemcpyAsync(d.v.p, h.v.data(), sizeof(double) * d.nnz, hipMemcpyHostToDevice, s));
+    }
+    d.lanes = default_lanes(h);
+    HPRQP_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace hprqp
+
+// ---------------------------------------------------------------------
+template <class F>
+static int guarded(hprqp_ctx *ctx, F &&f) {
+    try {
+        f();
+        return HPRQP_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->err = "host allocation failed";
+        return HPRQP_E_ALLOC;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        return HPRQP_E_STATE;
+    } catch (...) {
+        if (ctx) ctx->err = "unknown error";
+        return HPRQP_E_STATE;
+    }
+}
+
+#define CTX_CHECK(ctx)                      \
+    do {                                    \
+        if (!(ctx)) return HPRQP_E_ARG;     \
+        (void)hipSetDevice((ctx)->device);  \
+    } while (0)
+
+extern "C" {
+
+int hprqp_abi_version(void) { return HPRQP_ABI_VERSION; }
+
+void hprqp_params_default(hprqp_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->stoptol = 1e-6;
+    p->sigma = -1;
+    p->max_iter = 2147483647LL;
+    p->sigma_fixed = 0;
+    p->problem_type = HPRQP_PROBLEM_QP;
+    p->time_limit = 3600.0;
+    p->eig_factor = 1.05;
+    p->check_iter = 100;
+    p->print_frequency = -1;
+    p->use_Ruiz_scaling = 1;
+    p->use_Pock_Chambolle_scaling = 1;
+    p->use_bc_scaling = 0;
+    p->precision = HPRQP_PREC_FP64;
+    p->verbose = 0;
+    p->spmv_lanes = 0;
+}
+
+int hprqp_ctx_create(int device, hprqp_ctx **out) {
+    if (!out) return HPRQP_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return HPRQP_E_HIP;
+    if (device < 0 || device >= count) return HPRQP_E_ARG;
+    hprqp_ctx *c = new (std::nothrow) hprqp_ctx();
+    if (!c) return HPRQP_E_ALLOC;
+    c->device = device;
+    int rc = guarded(c, [&] {
+        HPRQP_HIP(hipSetDevice(device));
+        HPRQP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HPRQP_HIP(hipHostMalloc((void **)&c->pinned, sizeof(double) * 256, hipHostMallocDefault));
+        c->pinned_count = 256;
+    });
+    if (rc != HPRQP_OK) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return HPRQP_OK;
+}
+
+int hprqp_ctx_destroy(hprqp_ctx *ctx) {
+    if (!ctx) return HPRQP_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    delete ctx;
+    return HPRQP_OK;
+}
+
+const char *hprqp_last_error(const hprqp_ctx *ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
+
+int hprqp_ctx_sync(hprqp_ctx *ctx) {
+    CTX_CHECK(ctx);
+    return guarded(ctx, [&] { HPRQP_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+// ---- buffers ---------------------------------------------------------
+int hprqp_buf_alloc(hprqp_ctx *ctx, size_t bytes, hprqp_buf **out) {
+    CTX_CHECK(ctx);
+    if (!out) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        hprqp_buf *b = new hprqp_buf();
+        b->bytes = bytes;
+        if (bytes) {
+            hipError_t e = hipMalloc(&b->p, bytes);
+            if (e != hipSuccess) {
+                delete b;
+                throw Error(HPRQP_E_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+            }
+        }
+        *out = b;
+    });
+}
+
+int hprqp_buf_free(hprqp_ctx *ctx, hprqp_buf *buf) {
+    CTX_CHECK(ctx);
+    if (!buf) return HPRQP_OK;
+    return guarded(ctx, [&] {
+        HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+        if (buf->p) HPRQP_HIP(hipFree(buf->p));
+        delete buf;
+    });
+}
+
+int hprqp_buf_upload(hprqp_ctx *ctx, hprqp_buf *buf, const void *host, size_t bytes) {
+    CTX_CHECK(ctx);
+    if (!buf || (!host && bytes) || bytes > buf->bytes) return HPRQP_E_ARG;
+    return guarded(ctx, [&] {
+        if (bytes) HPRQP_HIP(hipMemcpyAsync(buf->p, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+        HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int hprqp_buf_download(hprqp_ctx *ctx, const hprqp_buf *buf, void *host, size_t bytes) {
+    CTX_CHECK(ctx);
+    if (!buf || (!host && bytes) || bytes > buf->bytes) return HPRQP_E_ARG;
+    return guarded(ctx, [&] {
+        if (bytes) HPRQP_HIP(hipMemcpyAsync(host, buf->p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+void *hprqp_buf_device_ptr(hprqp_buf *buf) { return buf ? buf->p : nullptr; }
+size_t hprqp_buf_size(const hprqp_buf *buf) { return buf ? buf->bytes : 0; }
+
+// ---- CSR -------------------------------------------------------------
+int hprqp_csr_create(hprqp_ctx *ctx, int32_t nrow, int32_t ncol, int64_t nnz, const int32_t *rowptr,
+                     const int32_t *colidx, const double *vals, int index_base, int build_transpose,
+                     hprqp_csr **out) {
+    CTX_CHECK(ctx);
+    if (!out) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        HostCsr h = hprqp::host_csr_from(nrow, ncol, nnz, rowptr, colidx, vals, index_base);
+        hprqp_csr *c = new hprqp_csr();
+        try {
+            hprqp::upload_csr(ctx->stream, h, c->a);
+            if (build_transpose) {
+                hprqp::upload_csr(ctx->stream, hprqp::host_csr_transpose(h), c->at);
+                c->has_t = true;
+            }
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int hprqp_csr_destroy(hprqp_ctx *ctx, hprqp_csr *a) {
+    CTX_CHECK(ctx);
+    return guarded(ctx, [&] {
+        HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+        delete a;
+    });
+}
+
+int hprqp_csr_spmv(hprqp_ctx *ctx, const hprqp_csr *a, int transpose, const hprqp_buf *x, hprqp_buf *y) {
+    CTX_CHECK(ctx);
+    if (!a || !x || !y) return HPRQP_E_ARG;
+    return guarded(ctx, [&] {
+        const DevCsr &m = transpose ? a->at : a->a;
+        HPRQP_REQUIRE(!transpose || a->has_t, HPRQP_E_STATE, "matrix was created without its transpose");
+        HPRQP_REQUIRE(x->bytes >= sizeof(double) * (size_t)m.ncol, HPRQP_E_ARG, "x buffer too small");
+        HPRQP_REQUIRE(y->bytes >= sizeof(double) * (size_t)m.nrow, HPRQP_E_ARG, "y buffer too small");
+        hprqp::csr_spmv_fp64(ctx, m, (const double *)x->p, (double *)y->p);
+    });
+}
+
+// ---- Q operators -------------------------------------------------------
+int hprqp_qop_create_csr(hprqp_ctx *ctx, int32_t n, int64_t nnz, const int32_t *rowptr, const int32_t *colidx,
+                         const double *vals, int index_base, hprqp_qop **out) {
+    CTX_CHECK(ctx);
+    if (!out) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        hprqp_qop *q = new hprqp_qop();
+        try {
+            q->kind = QOP_CSR;
+            q->n = n;
+            q->name = "CSR";
+            q->hq = hprqp::host_csr_from(n, n, nnz, rowptr, colidx, vals, index_base);
+            hprqp::upload_csr(ctx->stream, q->hq, q->q);
+        } catch (...) {
+            delete q;
+            throw;
+        }
+        *out = q;
+    });
+}
+
+int hprqp_qop_create_lasso(hprqp_ctx *ctx, int32_t rows, int32_t n, int64_t nnz, const int32_t *rowptr,
+                           const int32_t *colidx, const double *vals, int index_base, hprqp_qop **out) {
+    CTX_CHECK(ctx);
+    if (!out) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        hprqp_qop *q = new hprqp_qop();
+        try {
+            q->kind = QOP_LASSO;
+            q->n = n;
+            q->name = "LASSO";
+            HostCsr h = hprqp::host_csr_from(rows, n, nnz, rowptr, colidx, vals, index_base);
+            hprqp::upload_csr(ctx->stream, h, q->la);
+            hprqp::upload_csr(ctx->stream, hprqp::host_csr_transpose(h), q->lat);
+            q->ltemp.alloc(rows);
+        } catch (...) {
+            delete q;
+            throw;
+        }
+        *out = q;
+    });
+}
+
+int hprqp_qop_create_qap(hprqp_ctx *ctx, int32_t nq, const double *A, const double *B, const double *S,
+                         const double *T, hprqp_qop **out) {
+    CTX_CHECK(ctx);
+    if (!out || nq <= 0 || !A || !B || !S || !T) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        hprqp_qop *q = new hprqp_qop();
+        try {
+            q->kind = QOP_QAP;
+            q->nq = nq;
+            q->n = nq * nq;
+            q->name = "QAP";
+            const int64_t nn = (int64_t)nq * nq;
+            hprqp::DevVec<double> *dst[4] = {&q->qa, &q->qb, &q->qs, &q->qt};
+            const double *src[4] = {A, B, S, T};
+            for (int t = 0; t < 4; ++t) {
+                for (int64_t k = 0; k < nn; ++k)
+                    HPRQP_REQUIRE(std::isfinite(src[t][k]), HPRQP_E_DATA, "QAP matrix has NaN/Inf");
+                dst[t]->alloc(nn);
+                HPRQP_HIP(hipMemcpyAsync(dst[t]->p, src[t], sizeof(double) * nn, hipMemcpyHostToDevice,
+                                         ctx->stream));
+            }
+            q->qtemp.alloc(nn);
+            HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete q;
+            throw;
+        }
+        *out = q;
+    });
+}
+
+int hprqp_qop_create_user(hprqp_ctx *ctx, const hprqp_qop_vtable *vt, void *user, hprqp_qop **out) {
+    CTX_CHECK(ctx);
+    if (!out || !vt || !vt->apply || !vt->problem_size) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        hprqp_qop *q = new hprqp_qop();
+        q->kind = QOP_USER;
+        q->vt = *vt;
+        q->user = user;
+        const int64_t n = vt->problem_size(user);
+        if (n <= 0 || n >= INT32_MAX) {
+            delete q;
+            throw Error(HPRQP_E_ARG, "user operator problem_size out of range");
+        }
+        q->n = (int32_t)n;
+        q->name = vt->name ? vt->name(user) : "USER";
+        *out = q;
+    });
+}
+
+int hprqp_qop_destroy(hprqp_ctx *ctx, hprqp_qop *q) {
+    CTX_CHECK(ctx);
+    return guarded(ctx, [&] {
+        HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+        delete q;
+    });
+}
+
+int hprqp_qop_apply(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x, hprqp_buf *Qx) {
+    CTX_CHECK(ctx);
+    if (!q || !x || !Qx) return HPRQP_E_ARG;
+    return guarded(ctx, [&] {
+        HPRQP_REQUIRE(x->bytes >= sizeof(double) * (size_t)q->n && Qx->bytes >= sizeof(double) * (size_t)q->n,
+                      HPRQP_E_ARG, "buffer smaller than the operator size");
+        hprqp::qop_apply_fp64(ctx, q, (const double *)x->p, (double *)Qx->p);
+    });
+}
+
+int hprqp_qop_info(const hprqp_qop *q, int64_t *n, int64_t *temp_size, const char **name) {
+    if (!q) return HPRQP_E_ARG;
+    if (n) *n = q->n;
+    if (temp_size) {
+        if (q->kind == QOP_LASSO)
+            *temp_size = q->la.nrow;
+        else if (q->kind == QOP_QAP)
+            *temp_size = (int64_t)q->nq * q->nq;
+        else if (q->kind == QOP_USER && q->vt.temp_size)
+            *temp_size = q->vt.temp_size(q->user);
+        else
+            *temp_size = 0;
+    }
+    if (name) *name = q->name.c_str();
+    return HPRQP_OK;
+}
+
+// ---- model -------------------------------------------------------------
+int hprqp_model_create(hprqp_ctx *ctx, int32_t n, int32_t m, hprqp_qop *Q, int64_t a_nnz, const int32_t *a_rowptr,
+                       const int32_t *a_colidx, const double *a_vals, int index_base, const double *c,
+                       const double *AL, const double *AU, const double *l, const double *u, double obj_constant,
+                       double lambda, hprqp_model **out) {
+    CTX_CHECK(ctx);
+    if (!out || !Q || n <= 0 || m < 0 || !c || !l || !u || (m > 0 && (!AL || !AU))) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        HPRQP_REQUIRE(Q->n == n, HPRQP_E_ARG, "Q operator size differs from n");
+        hprqp_model *md = new hprqp_model();
+        try {
+            md->n = n;
+            md->m = m;
+            md->Q = Q;
+            if (m > 0) {
+                md->A = hprqp::host_csr_from(m, n, a_nnz, a_rowptr, a_colidx, a_vals, index_base);
+            } else {
+                md->A.nrow = 0;
+                md->A.ncol = n;
+                md->A.rp.assign(1, 0);
+            }
+            md->c.assign(c, c + n);
+            md->l.assign(l, l + n);
+            md->u.assign(u, u + n);
+            md->AL.assign(AL ? AL : c, AL ? AL + m : c);
+            md->AU.assign(AU ? AU : c, AU ? AU + m : c);
+            if (m == 0) {
+                md->AL.clear();
+                md->AU.clear();
+            }
+            // validation (qp_formulation, src/utils.jl:78-146)
+            for (int32_t i = 0; i < n; ++i) {
+                HPRQP_REQUIRE(std::isfinite(md->c[i]), HPRQP_E_DATA, "c vector contains NaN or Inf values");
+                HPRQP_REQUIRE(!std::isnan(md->l[i]) && !std::isnan(md->u[i]), HPRQP_E_DATA,
+                              "variable bounds l or u contain NaN values");
+                HPRQP_REQUIRE(md->l[i] <= md->u[i], HPRQP_E_DATA, "infeasible variable bounds: l > u");
+            }
+            for (int32_t i = 0; i < m; ++i) {
+                HPRQP_REQUIRE(!std::isnan(md->AL[i]) && !std::isnan(md->AU[i]), HPRQP_E_DATA,
+                              "constraint bounds AL or AU contain NaN values");
+                HPRQP_REQUIRE(md->AL[i] <= md->AU[i], HPRQP_E_DATA, "infeasible constraint bounds: AL > AU");
+            }
+            md->obj_constant = obj_constant;
+            md->lambda = lambda;
+            if (Q->kind == QOP_CSR) {
+                // symmetric pattern completion (explicit zeros) and the map
+                // from each entry to its transpose partner
+                const HostCsr &q = Q->hq;
+                HostCsr qt = hprqp::host_csr_transpose(q);
+                HostCsr &s = md->Qsym;
+                s.nrow = s.ncol = n;
+                s.rp.assign((size_t)n + 1, 0);
+                s.ci.clear();
+                s.v.clear();
+                for (int32_t i = 0; i < n; ++i) {
+                    int32_t a = q.rp[i], ae = q.rp[i + 1], b = qt.rp[i], be = qt.rp[i + 1];
+                    while (a < ae || b < be) {
+                        const int32_t ca = a < ae ? q.ci[a] : INT32_MAX, cb = b < be ? qt.ci[b] : INT32_MAX;
+                        if (ca == cb) {
+                            s.ci.push_back(ca);
+                            s.v.push_back(q.v[a]);
+                            ++a;
+                            ++b;
+                        } else if (ca < cb) {
+                            s.ci.push_back(ca);
+                            s.v.push_back(q.v[a]);
+                            ++a;
+                        } else {
+                            s.ci.push_back(cb);
+                            s.v.push_back(0.0);
+                            ++b;
+                        }
+                    }
+                    s.rp[i + 1] = (int32_t)s.ci.size();
+                }
+                // transpose positions: entry (i,j) at k <-> entry (j,i)
+                md->q_tpos.resize(s.ci.size());
+                std::vector<int32_t> next(s.rp.begin(), s.rp.end() - 1);
+                for (int32_t i = 0; i < n; ++i)
+                    for (int32_t k = s.rp[i]; k < s.rp[i + 1]; ++k) {
+                        // rows are visited in increasing i, so the entries
+                        // (j, i) of row j appear in the same increasing order
+                        const int32_t j = s.ci[k];
+                        md->q_tpos[k] = next[j]++;
+                    }
+            }
+        } catch (...) {
+            delete md;
+            throw;
+        }
+        *out = md;
+    });
+}
+
+int hprqp_model_destroy(hprqp_ctx *ctx, hprqp_model *model) {
+    if (!ctx) return HPRQP_E_ARG;
+    delete model;
+    return HPRQP_OK;
+}
+
+int hprqp_solve(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, const double *x0,
+                const double *y0, double *x_out, double *y_out, double *z_out, double *w_out, double *hist,
+                int64_t hist_cap, hprqp_result *result) {
+    CTX_CHECK(ctx);
+    if (!model || !result) return HPRQP_E_ARG;
+    hprqp_params def;
+    if (!params) {
+        hprqp_params_default(&def);
+        params = &def;
+    }
+    return guarded(ctx, [&] {
+        hprqp::solve(ctx, model, params, x0, y0, x_out, y_out, z_out, w_out, hist, hist_cap, result);
+    });
+}
+
+int hprqp_profile_step(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, int reps,
+                       double *ms_out, int *n_out) {
+    CTX_CHECK(ctx);
+    if (!model || !ms_out || reps <= 0) return HPRQP_E_ARG;
+    hprqp_params def;
+    if (!params) {
+        hprqp_params_default(&def);
+        params = &def;
+    }
+    return guarded(ctx, [&] { hprqp::profile_step(ctx, model, params, reps, ms_out, n_out); });
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+// hprqp_internal.hpp -- host-side objects behind the C ABI (include/hprqp.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hprqp.h"
+
+namespace hprqp {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HPRQP_HIP(expr)                                                              \
+    do {                                                                             \
+        hipError_t _e = (expr);                                                      \
+        if (_e != hipSuccess)                                                        \
+            throw ::hprqp::Error(HPRQP_E_HIP, std::string(#expr) + " failed: " +     \
+                                                  hipGetErrorString(_e));            \
+    } while (0)
+
+#define HPRQP_REQUIRE(cond, code, msg)                           \
+    do {                                                         \
+        if (!(cond)) throw ::hprqp::Error((code), (msg));        \
+    } while (0)
+
+// RAII device array
+template <class T>
+struct DevVec {
+    T *p = nullptr;
+    int64_t n = 0;
+    DevVec() = default;
+    explicit DevVec(int64_t n_) { alloc(n_); }
+    void alloc(int64_t n_) {
+        release();
+        n = n_;
+        if (n > 0) {
+            hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)n);
+            if (e != hipSuccess) {
+                p = nullptr;
+                n = 0;
+                throw Error(HPRQP_E_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+            }
+        }
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevVec() { release(); }
+    DevVec(const DevVec &) = delete;
+    DevVec &operator=(const DevVec &) = delete;
+    DevVec(DevVec &&o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DevVec &operator=(DevVec &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+};
+
+// Device CSR (0-based, int32 indices).  vals: FP64; vals32: optional FP32
+// copy for mixed precision.  lanes: rows-per-group choice of the SpMV.
+struct DevCsr {
+    int32_t nrow = 0, ncol = 0;
+    int64_t nnz = 0;
+    DevVec<int32_t> rp, ci;
+    DevVec<double> v;
+    DevVec<float> v32;
+    int lanes = 1;
+};
+
+// Host-side CSR (sorted column indices), used while building models.
+struct HostCsr {
+    int32_t nrow = 0, ncol = 0;
+    std::vector<int32_t> rp, ci;
+    std::vector<double> v;
+};
+
+HostCsr host_csr_from(int32_t nrow, int32_t ncol, int64_t nnz, const int32_t *rowptr,
+                      const int32_t *colidx, const double *vals, int index_base);
+HostCsr host_csr_transpose(const HostCsr &a);
+void upload_csr(hipStream_t s, const HostCsr &h, DevCsr &d);
+int default_lanes(const HostCsr &h);
+
+}  // namespace hprqp
+
+struct hprqp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double *pinned = nullptr;  // small pinned staging area for scalar readback
+    size_t pinned_count = 0;
+};
+
+struct hprqp_buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct hprqp_csr {
+    hprqp::DevCsr a;
+    hprqp::DevCsr at;  // stored transpose (may be empty)
+    bool has_t = false;
+};
+
+enum QopKind { QOP_CSR = 0, QOP_LASSO = 1, QOP_QAP = 2, QOP_USER = 3 };
+
+struct hprqp_qop {
+    int kind = QOP_CSR;
+    int32_t n = 0;
+    // CSR: Q (n x n) with the host copy of its pattern kept for scaling
+    hprqp::HostCsr hq;
+    hprqp::DevCsr q;
+    // LASSO: data matrix and its transpose, temp of length rows
+    hprqp::DevCsr la, lat;
+    hprqp::DevVec<double> ltemp;
+    hprqp::DevVec<float> ltemp32;
+    // QAP: dense column-major nq x nq
+    int32_t nq = 0;
+    hprqp::DevVec<double> qa, qb, qs, qt, qtemp;
+    hprqp::DevVec<float> qtemp32;
+    // user plugin
+    hprqp_qop_vtable vt{};
+    void *user = nullptr;
+    std::string name;
+};
+
+struct hprqp_model {
+    int32_t n = 0, m = 0;
+    hprqp_qop *Q = nullptr;
+    hprqp::HostCsr A;  // host copy (m x n), sorted
+    std::vector<double> c, AL, AU, l, u;
+    double obj_constant = 0.0, lambda = 0.0;
+    // Q pattern symmetric completion and transpose position map (CSR Q only)
+    hprqp::HostCsr Qsym;
+    std::vector<int32_t> q_tpos;
+    // SpMV lanes chosen by the autotuner (0 = not tuned yet): Q, A, A^T
+    mutable int tuned_lanes[3] = {0, 0, 0};
+};
+
+namespace hprqp {
+// solver entry (hprqp_solver.hip)
+void solve(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, const double *x0,
+           const double *y0, double *x_out, double *y_out, double *z_out, double *w_out,
+           double *hist, int64_t hist_cap, hprqp_result *res);
+// per-kernel timing of the iteration (hprqp_profile_step)
+void profile_step(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, int reps,
+                  double *ms_out, int *n_out);
+// operator applies exposed through the ABI (FP64)
+void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx);
+void csr_spmv_fp64(hprqp_ctx *ctx, const DevCsr &a, const double *x, double *y);
+// QAP dense products (hprqp_qap.hip)
+void qap_apply_fp64(hipStream_t s, int32_t nq, const double *A, const double *B, const double *S,
+                    const double *T, const double *X, double *tmp, double *QX);
+void qap_apply_fp32(hipStream_t s, int32_t nq, const double *A, const double *B, const double *S,
+                    const double *T, const float *X, float *tmp, float *QX);
+}  // namespace hprqp

@@ -1,0 +1,436 @@
+// hprqp_kernels.hpp -- gfx950 device kernels of the HPR-QP iteration.
+//
+// Design (CDNA4, wave64):
+//  * CSR SpMV: a group of L lanes (L = 1..64, a power of two, chosen per
+//    matrix by the autotuner) owns one row; lanes stride the row's nonzeros
+//    and combine with xor-shuffles inside the group.  The transpose product
+//    uses a stored transpose, never atomics, so every kernel is race-free and
+//    deterministic.
+//  * Every HPR half-step is ONE kernel: the SpMV's epilogue (run by the
+//    group's lane 0) applies the projection / prox / Halpern update for that
+//    row, so the product vector never makes a round trip through HBM unless a
+//    later phase needs it.  Equivalent of the reference's "customized" fused
+//    kernels (src/kernels.jl:422-991) with a wave-cooperative row product.
+//  * Block index remap: the hardware deals workgroups round-robin over the
+//    8 XCDs; xcd_block() gives each XCD a contiguous range of rows so the
+//    gathered vector segments stay in that XCD's L2.
+//  * Reductions are two-pass (per-block partials, then one block) with a
+//    fixed grid, so results are bitwise reproducible run to run.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hprqp {
+
+constexpr int kBlock = 256;
+constexpr int kMaxReduceBlocks = 1024;
+
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+    constexpr int X = 8;
+    const int per = nblocks / X, rem = nblocks % X;
+    const int xcd = b % X, k = b / X;
+    return xcd * per + (xcd < rem ? xcd : rem) + k;
+}
+
+template <class T>
+__device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
+template <class T>
+__device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
+// Julia min(max(z, lo), hi)
+template <class T>
+__device__ __forceinline__ T proj(T z, T lo, T hi) { return dmin(dmax(z, lo), hi); }
+
+// ---------------------------------------------------------------------
+// CSR row product with fused epilogue
+// ---------------------------------------------------------------------
+template <int L, class MT, class XT, class AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_csr(int32_t nrows,
+                                                const int32_t *__restrict__ rp,
+                                                const int32_t *__restrict__ ci,
+                                                const MT *__restrict__ v,
+                                                const XT *__restrict__ x, Epi epi) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t t = (int64_t)b * kBlock + threadIdx.x;
+    const int32_t row = (int32_t)(t / L);
+    const int lane = threadIdx.x & (L - 1);
+    if (row >= nrows) return;  // whole L-group leaves together
+    const int32_t s = rp[row], e = rp[row + 1];
+    AT acc0 = 0, acc1 = 0;
+    int32_t k = s + lane;
+    for (; k + L < e; k += 2 * L) {
+        acc0 += (AT)v[k] * (AT)x[ci[k]];
+        acc1 += (AT)v[k + L] * (AT)x[ci[k + L]];
+    }
+    if (k < e) acc0 += (AT)v[k] * (AT)x[ci[k]];
+    AT acc = acc0 + acc1;
+    if constexpr (L > 1) {
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+    }
+    if (lane == 0) epi(row, acc);
+}
+
+// elementwise form of the same epilogue (operator Q: product precomputed)
+template <class ST, class AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_vec(int32_t n, const ST *__restrict__ src, Epi epi) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) epi((int32_t)i, (AT)src[i]);
+}
+
+// plain store epilogue: y = A x
+template <class YT>
+struct EpiStore {
+    YT *y;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT acc) const { y[i] = (YT)acc; }
+};
+
+// y = (A x) * s  (s read from device memory; used by the power method)
+template <class YT>
+struct EpiStoreScaled {
+    YT *y;
+    const double *s;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT acc) const { y[i] = (YT)(acc * (AT)(*s)); }
+};
+
+// ---------------------------------------------------------------------
+// HPR half-step epilogues.  VT = bulk vector type, CT = compute type.
+// Anchors (last_*) are always FP64.
+// ---------------------------------------------------------------------
+
+// z/x/w1 step: update_zxw1 (src/kernels.jl:422-537, CPU :2301-2397)
+template <class VT, bool FULL, bool DIAG>
+struct EpiZxw1 {
+    VT *Qw;  // written when non-null (the product Q w)
+    const VT *ATy, *c, *l, *u, *w, *fact1, *fact2;
+    const double *last_x;
+    VT *x, *x_hat, *w_bar, *dx, *x_bar, *z_bar;
+    double sigma, f1s, f2s, h1, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT qw_) const {
+        const double qw = (double)qw_;
+        if (Qw) Qw[i] = (VT)qw_;
+        const double xi = (double)x[i], wi = (double)w[i];
+        const double z_raw = xi + sigma * (-qw + (double)ATy[i] - (double)c[i]);
+        const double xb = proj(z_raw, (double)l[i], (double)u[i]);
+        const double xh = 2.0 * xb - xi;
+        const double xn = fma(h2, xh, h1 * last_x[i]);
+        const double wb = DIAG ? fma((double)fact1[i], wi, (double)fact2[i] * xh)
+                               : fma(f1s, wi, f2s * xh);
+        if constexpr (FULL) {
+            dx[i] = (VT)(xb - xi);
+            x_bar[i] = (VT)xb;
+            z_bar[i] = (VT)((xb - z_raw) / sigma);
+        }
+        x[i] = (VT)xn;
+        x_hat[i] = (VT)xh;
+        w_bar[i] = (VT)wb;
+    }
+};
+
+// tempv = x_hat + sigma (Qw - Qw_bar)  (src/kernels.jl:674-705)
+template <class VT>
+struct EpiTempv {
+    const VT *x_hat, *Qw;
+    VT *tempv, *Qw_bar;  // Qw_bar written when non-null
+    double sigma;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT qwb_) const {
+        const double qwb = (double)qwb_;
+        if (Qw_bar) Qw_bar[i] = (VT)qwb_;
+        tempv[i] = (VT)((double)x_hat[i] + sigma * ((double)Qw[i] - qwb));
+    }
+};
+
+// dual step: update_y (src/kernels.jl:760-862, CPU :2682-2743)
+template <class VT, bool FULL>
+struct EpiY {
+    const VT *AL, *AU;
+    const double *last_y;
+    VT *y, *y_bar, *s, *dy;
+    double fact1, fact2, h1, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT ax_) const {
+        const double ax = (double)ax_, yi = (double)y[i];
+        const double s_raw = ax - fact1 * yi;
+        const double sp = proj(s_raw, (double)AL[i], (double)AU[i]);
+        const double yb = fact2 * (sp - s_raw);
+        const double yh = 2.0 * yb - yi;
+        const double yn = fma(h2, yh, h1 * last_y[i]);
+        if constexpr (FULL) {
+            s[i] = (VT)sp;
+            dy[i] = (VT)(yb - yi);
+        }
+        y_bar[i] = (VT)yb;
+        y[i] = (VT)yn;
+    }
+};
+
+// second w step: update_w2 (src/kernels.jl:878-991, CPU :2747-2832)
+template <class VT, bool FULL, bool DIAG>
+struct EpiW2 {
+    const VT *fact;
+    const double *last_w, *last_ATy;
+    VT *w, *w_bar, *ATy, *dw, *ATdy, *ATy_bar;
+    double fs, h1, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT atyb_) const {
+        const double atyb = (double)atyb_;
+        const double wi = (double)w[i], wbi = (double)w_bar[i], atyi = (double)ATy[i];
+        const double f = DIAG ? (double)fact[i] : fs;
+        const double wbn = wbi + f * (atyb - atyi);
+        const double wn = h1 * last_w[i] + h2 * (2.0 * wbn - wi);
+        const double atyn = h1 * last_ATy[i] + h2 * (2.0 * atyb - atyi);
+        w[i] = (VT)wn;
+        ATy[i] = (VT)atyn;
+        if constexpr (FULL) {
+            w_bar[i] = (VT)wbn;
+            dw[i] = (VT)(wbn - wi);
+            ATdy[i] = (VT)(atyb - atyi);
+            ATy_bar[i] = (VT)atyb;
+        }
+    }
+};
+
+// noC z/x/w step: product is A^T y (src/kernels.jl:540-669, CPU :2117-2209)
+template <class VT, bool FULL, bool DIAG>
+struct EpiNocZxw {
+    VT *ATy;
+    const VT *Qw, *c, *l, *u, *fact1, *fact2;
+    const double *last_x, *last_w;
+    VT *x, *x_hat, *w, *w_bar, *dx, *x_bar, *z_bar, *dw;
+    double sigma, f1s, f2s, h1, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT aty_) const {
+        const double aty = (double)aty_;
+        ATy[i] = (VT)aty_;
+        const double xi = (double)x[i], wi = (double)w[i];
+        const double z_raw = xi + sigma * (-(double)Qw[i] + aty - (double)c[i]);
+        const double xb = proj(z_raw, (double)l[i], (double)u[i]);
+        const double xh = 2.0 * xb - xi;
+        const double xn = fma(h2, xh, h1 * last_x[i]);
+        const double wb = DIAG ? fma((double)fact1[i], wi, (double)fact2[i] * xh)
+                               : fma(f1s, wi, f2s * xh);
+        const double wn = fma(h2, 2.0 * wb - wi, h1 * last_w[i]);
+        if constexpr (FULL) {
+            dx[i] = (VT)(xb - xi);
+            x_bar[i] = (VT)xb;
+            z_bar[i] = (VT)((xb - z_raw) / sigma);
+            dw[i] = (VT)(wb - wi);
+        }
+        x[i] = (VT)xn;
+        x_hat[i] = (VT)xh;
+        w_bar[i] = (VT)wb;
+        w[i] = (VT)wn;
+    }
+};
+
+// noC tempv/Qw Halpern step: product is Q w_bar (src/kernels.jl:710-748)
+template <class VT>
+struct EpiNocTempv {
+    const VT *x_hat;
+    const double *last_Qw;
+    VT *Qw, *Qw_bar, *tempv;
+    double sigma, h1, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT qwb_) const {
+        const double qwb = (double)qwb_, qw = (double)Qw[i];
+        Qw_bar[i] = (VT)qwb_;
+        Qw[i] = (VT)fma(h2, 2.0 * qwb - qw, h1 * last_Qw[i]);
+        tempv[i] = (VT)((double)x_hat[i] + sigma * (qw - qwb));
+    }
+};
+
+// LASSO step: product is Q w = A^T (A w) (src/kernels.jl:261-376)
+template <class VT, bool FULL>
+struct EpiLasso {
+    VT *Qw;
+    const VT *ATy, *c;
+    const double *last_x, *last_w;
+    VT *x, *x_hat, *w, *w_bar, *dx, *x_bar, *z_bar, *dw;
+    double sigma, lambda_sigma, f1s, f2s, h1, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT qw_) const {
+        const double qw = (double)qw_;
+        if (Qw) Qw[i] = (VT)qw_;
+        const double xi = (double)x[i], wi = (double)w[i];
+        const double z_raw = xi + sigma * (-qw + (double)ATy[i] - (double)c[i]);
+        const double shrink = dmax(fabs(z_raw) - lambda_sigma, 0.0);
+        const double xb = copysign(shrink, z_raw);
+        const double xh = 2.0 * xb - xi;
+        const double xn = fma(h2, xh, h1 * last_x[i]);
+        const double wb = fma(f1s, wi, f2s * xh);
+        const double wn = fma(h2, 2.0 * wb - wi, h1 * last_w[i]);
+        if constexpr (FULL) {
+            dx[i] = (VT)(xb - xi);
+            x_bar[i] = (VT)xb;
+            z_bar[i] = (VT)((xb - z_raw) / sigma);
+            dw[i] = (VT)(wb - wi);
+        }
+        x[i] = (VT)xn;
+        x_hat[i] = (VT)xh;
+        w_bar[i] = (VT)wb;
+        w[i] = (VT)wn;
+    }
+};
+
+// LP z/x step: product is A^T y (src/kernels.jl:1243-1351)
+template <class VT, bool FULL>
+struct EpiLpZx {
+    VT *ATy;
+    const VT *c, *l, *u;
+    const double *last_x;
+    VT *x, *x_hat, *dx, *x_bar, *z_bar;
+    double sigma, h1, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT aty_) const {
+        const double aty = (double)aty_;
+        ATy[i] = (VT)aty_;
+        const double xi = (double)x[i];
+        const double z_raw = xi + sigma * (aty - (double)c[i]);
+        const double xb = proj(z_raw, (double)l[i], (double)u[i]);
+        const double xh = 2.0 * xb - xi;
+        const double xn = fma(h2, xh, h1 * last_x[i]);
+        if constexpr (FULL) {
+            dx[i] = (VT)(xb - xi);
+            x_bar[i] = (VT)xb;
+            z_bar[i] = (VT)((xb - z_raw) / sigma);
+        }
+        x[i] = (VT)xn;
+        x_hat[i] = (VT)xh;
+    }
+};
+
+// ---------------------------------------------------------------------
+// multi-value reductions: NS sums and NM maxima, two passes
+// ---------------------------------------------------------------------
+template <int NS, int NM, class F>
+__global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, F f, double *__restrict__ part) {
+    constexpr int NT = NS + NM;
+    double s[NS > 0 ? NS : 1], mx[NM > 0 ? NM : 1];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) mx[k] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        f(i, s, mx);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) s[k] += __shfl_xor(s[k], off);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) mx[k] = dmax(mx[k], __shfl_xor(mx[k], off));
+    }
+    __shared__ double sh[kBlock / 64][NT > 0 ? NT : 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k) sh[wave][k] = s[k];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) sh[wave][NS + k] = mx[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < NT) {
+        const int k = threadIdx.x;
+        double r = sh[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) r = k < NS ? r + sh[w][k] : dmax(r, sh[w][k]);
+        part[(int64_t)blockIdx.x * NT + k] = r;
+    }
+}
+
+// final pass: one block; out[k] for k < NS + NM
+template <int NS, int NM>
+__global__ __launch_bounds__(kBlock) void k_reduce_final(int nb, const double *__restrict__ part,
+                                                         double *__restrict__ out) {
+    constexpr int NT = NS + NM;
+    __shared__ double sh[kBlock / 64][NT > 0 ? NT : 1];
+    double r[NT > 0 ? NT : 1];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) r[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const double v = part[(int64_t)b * NT + k];
+            r[k] = k < NS ? r[k] + v : dmax(r[k], v);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const double o = __shfl_xor(r[k], off);
+            r[k] = k < NS ? r[k] + o : dmax(r[k], o);
+        }
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NT; ++k) sh[wave][k] = r[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < NT) {
+        const int k = threadIdx.x;
+        double a = sh[0][k];
+        for (int w = 1; w < kBlock / 64; ++w) a = k < NS ? a + sh[w][k] : dmax(a, sh[w][k]);
+        out[k] = a;
+    }
+}
+
+// ---------------------------------------------------------------------
+// generic elementwise kernel
+// ---------------------------------------------------------------------
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_each(int64_t n, F f) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) f(i);
+}
+
+// ---------------------------------------------------------------------
+// scaling kernels (src/kernels.jl:1746-2048)
+// ---------------------------------------------------------------------
+// column statistic over A's columns (rows of the stored transpose) and Q's
+// rows: sqrt(max) (Ruiz) or sqrt(sum) (Pock-Chambolle), 0 -> 1.
+__global__ __launch_bounds__(kBlock) void k_col_stat(int32_t n, const int32_t *__restrict__ at_rp,
+                                                     const double *__restrict__ at_v,
+                                                     const int32_t *__restrict__ q_rp,
+                                                     const double *__restrict__ q_v, int ruiz,
+                                                     double *__restrict__ out) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    double sa = 0.0, sq = 0.0;
+    if (at_rp)
+        for (int32_t k = at_rp[j]; k < at_rp[j + 1]; ++k)
+            sa = ruiz ? dmax(sa, fabs(at_v[k])) : sa + fabs(at_v[k]);
+    if (q_rp)
+        for (int32_t k = q_rp[j]; k < q_rp[j + 1]; ++k)
+            sq = ruiz ? dmax(sq, fabs(q_v[k])) : sq + fabs(q_v[k]);
+    const double t = ruiz ? dmax(sq, sa) : sq + sa;
+    out[j] = t > 0.0 ? sqrt(t) : 1.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_row_stat(int32_t m, const int32_t *__restrict__ rp,
+                                                     const double *__restrict__ v, int ruiz,
+                                                     double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s = ruiz ? dmax(s, fabs(v[k])) : s + fabs(v[k]);
+    out[i] = s > 0.0 ? sqrt(s) : 1.0;
+}
+
+// v_ij = (v_ij * (1/r_i)) / c_j
+__global__ __launch_bounds__(kBlock) void k_scale_csr(int32_t nrow, const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci,
+                                                      double *__restrict__ v,
+                                                      const double *__restrict__ r,
+                                                      const double *__restrict__ c) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nrow) return;
+    const double inv = 1.0 / r[i];
+    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) v[k] = (v[k] * inv) / c[ci[k]];
+}
+
+}  // namespace hprqp

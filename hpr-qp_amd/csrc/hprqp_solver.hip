@@ -1,0 +1,1610 @@
+// hprqp_solver.hip -- the HPR-QP solve loop on one MI355X.
+//
+// Host side restates solve() (src/algorithm.jl:1904-2039): scaling, power
+// method, warm start, then the iteration loop whose per-step device work is
+// the fused kernels of hprqp_kernels.hpp.  Restart and sigma logic stay on
+// the host and read back a few scalars per check step, as in the reference.
+//
+// Precision: Solver<double> is the plain FP64 form.  Solver<float> is the
+// mixed form: FP32 matrices, FP32 bulk vectors and FP32 accumulation in the
+// SpMVs and fused updates; FP64 Halpern anchors (last_*); FP64 matrices and
+// FP64 accumulation for the residual/KKT evaluation and all dot products of
+// the sigma/restart tests.  A mixed solve finishes with an FP64 polish that
+// continues the same iteration in Solver<double> from the FP32 state.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <type_traits>
+
+#include "hprqp_internal.hpp"
+#include "hprqp_kernels.hpp"
+
+namespace hprqp {
+
+static double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+static inline int nblk(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------
+template <class AT, class MT, class XT, class Epi>
+static void launch_csr(hipStream_t s, int lanes, int32_t nrows, const int32_t *rp, const int32_t *ci,
+                       const MT *v, const XT *x, const Epi &epi) {
+    if (nrows <= 0) return;
+    const int64_t threads = (int64_t)nrows * lanes;
+    const int grid = (int)((threads + kBlock - 1) / kBlock);
+    switch (lanes) {
+        case 1: k_csr<1, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
+        case 2: k_csr<2, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
+        case 4: k_csr<4, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
+        case 8: k_csr<8, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
+        case 16: k_csr<16, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
+        case 32: k_csr<32, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
+        default: k_csr<64, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
+    }
+    HPRQP_HIP(hipGetLastError());
+}
+
+template <class MT>
+static const MT *csr_vals(const DevCsr &a);
+template <>
+const double *csr_vals<double>(const DevCsr &a) { return a.v.p; }
+template <>
+const float *csr_vals<float>(const DevCsr &a) { return a.v32.p; }
+
+template <class AT, class MT, class XT, class Epi>
+static void csr_apply(hipStream_t s, const DevCsr &a, const XT *x, const Epi &epi) {
+    launch_csr<AT>(s, a.lanes, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), x, epi);
+}
+
+template <class F>
+static void each(hipStream_t s, int64_t n, const F &f) {
+    if (n <= 0) return;
+    k_each<F><<<nblk(n), kBlock, 0, s>>>(n, f);
+    HPRQP_HIP(hipGetLastError());
+}
+
+template <class ST, class AT, class Epi>
+static void vec_apply(hipStream_t s, int32_t n, const ST *src, const Epi &epi) {
+    if (n <= 0) return;
+    k_vec<ST, AT, Epi><<<nblk(n), kBlock, 0, s>>>(n, src, epi);
+    HPRQP_HIP(hipGetLastError());
+}
+
+static void ensure_v32(DevCsr &a) {
+    if (a.nnz > 0 && a.v32.n != a.nnz) {
+        a.v32.alloc(a.nnz);
+        const double *src = a.v.p;
+        float *dst = a.v32.p;
+        each(nullptr, a.nnz, [=] __device__(int64_t k) { dst[k] = (float)src[k]; });
+        HPRQP_HIP(hipDeviceSynchronize());
+    }
+}
+
+// Reduction scratch and pinned readback
+struct Reducer {
+    hipStream_t s;
+    DevVec<double> part, out;
+    double *host;
+    explicit Reducer(hprqp_ctx *ctx) : s(ctx->stream), host(ctx->pinned) {
+        part.alloc((int64_t)kMaxReduceBlocks * 16);
+        out.alloc(64);
+    }
+    // enqueue; results land in out[off .. off+NS+NM)
+    template <int NS, int NM, class F>
+    void enqueue(int64_t n, const F &f, int off) {
+        const int nb = (int)std::min<int64_t>(kMaxReduceBlocks, std::max<int64_t>(1, (n + kBlock - 1) / kBlock));
+        k_reduce<NS, NM, F><<<nb, kBlock, 0, s>>>(n, f, part.p);
+        HPRQP_HIP(hipGetLastError());
+        k_reduce_final<NS, NM><<<1, kBlock, 0, s>>>(nb, part.p, out.p + off);
+        HPRQP_HIP(hipGetLastError());
+    }
+    // copy out[0..cnt) to host and wait
+    const double *fetch(int cnt) {
+        HPRQP_HIP(hipMemcpyAsync(host, out.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+        HPRQP_HIP(hipStreamSynchronize(s));
+        return host;
+    }
+};
+
+// ---------------------------------------------------------------------
+// scaled FP64 problem (output of scaling!, src/utils.jl:350-804)
+// ---------------------------------------------------------------------
+struct Scaled {
+    int32_t n = 0, m = 0;
+    int qkind = QOP_CSR;
+    hprqp_qop *qop = nullptr;
+    bool q_empty = true;
+    DevCsr Q, A, AT;
+    DevVec<double> c, l, u, AL, AU, row_norm, col_norm, diag_Q;
+    std::vector<double> diag_Q_host;
+    double b_scale = 1.0, c_scale = 1.0, norm_b = 0, norm_c = 0, norm_b_org = 0, norm_c_org = 0;
+    bool Q_is_diag = false;
+    double q_max_val = 0.0;
+};
+
+static void upload_vec(hipStream_t s, DevVec<double> &d, const std::vector<double> &h) {
+    d.alloc((int64_t)h.size());
+    if (!h.empty())
+        HPRQP_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
+}
+
+static void fill(hipStream_t s, double *p, int64_t n, double v) {
+    each(s, n, [=] __device__(int64_t i) { p[i] = v; });
+}
+
+// bound statistic max(|AL_nInf|, |AU_nInf|): inf-norm (k=0) and 2-norm^2 (k=1)
+struct BoundNorm {
+    const double *AL, *AU;
+    __device__ void operator()(int64_t i, double *s, double *mx) const {
+        const double a = AL[i] == -INFINITY ? 0.0 : AL[i];
+        const double b = AU[i] == INFINITY ? 0.0 : AU[i];
+        const double t = dmax(fabs(a), fabs(b));
+        s[0] += t * t;
+        mx[0] = dmax(mx[0], t);
+    }
+};
+struct VecNorm {
+    const double *v;
+    __device__ void operator()(int64_t i, double *s, double *mx) const {
+        s[0] += v[i] * v[i];
+        mx[0] = dmax(mx[0], fabs(v[i]));
+    }
+};
+
+static void build_scaled(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, Reducer &red,
+                         Scaled &S) {
+    hipStream_t s = ctx->stream;
+    const int32_t n = md->n, m = md->m;
+    S.n = n;
+    S.m = m;
+    S.qop = md->Q;
+    S.qkind = md->Q->kind;
+    // raw data
+    upload_csr(s, md->A, S.A);
+    upload_csr(s, host_csr_transpose(md->A), S.AT);
+    if (S.qkind == QOP_CSR) {
+        upload_csr(s, md->Qsym, S.Q);
+        S.q_empty = md->Qsym.v.empty();
+        S.Q.lanes = md->Q->q.lanes;
+    } else {
+        S.q_empty = false;
+    }
+    upload_vec(s, S.c, md->c);
+    upload_vec(s, S.l, md->l);
+    upload_vec(s, S.u, md->u);
+    upload_vec(s, S.AL, md->AL);
+    upload_vec(s, S.AU, md->AU);
+    S.row_norm.alloc(m);
+    S.col_norm.alloc(n);
+    fill(s, S.row_norm.p, m, 1.0);
+    fill(s, S.col_norm.p, n, 1.0);
+    // original norms (src/utils.jl:404-408)
+    red.enqueue<1, 1>(m, BoundNorm{S.AL.p, S.AU.p}, 0);
+    red.enqueue<1, 1>(n, VecNorm{S.c.p}, 2);
+    const double *h = red.fetch(4);
+    S.norm_b_org = m > 0 ? h[1] : 0.0;
+    S.norm_c_org = h[3];
+    if (S.qkind != QOP_CSR) {
+        S.norm_b = m > 0 ? std::sqrt(h[0]) : 0.0;
+        S.norm_c = std::sqrt(h[2]);
+        return;
+    }
+    DevVec<double> tr(m), tc(n);
+    const int nruiz = p->use_Ruiz_scaling ? 10 : 0;
+    const int total = nruiz + (p->use_Pock_Chambolle_scaling ? 1 : 0);
+    for (int pass = 0; pass < total; ++pass) {
+        const int ruiz = pass < nruiz;
+        k_col_stat<<<nblk(n), kBlock, 0, s>>>(n, m > 0 ? S.AT.rp.p : nullptr, S.AT.v.p,
+                                               S.Q.nnz > 0 ? S.Q.rp.p : nullptr, S.Q.v.p, ruiz, tc.p);
+        HPRQP_HIP(hipGetLastError());
+        if (m > 0) {
+            k_row_stat<<<nblk(m), kBlock, 0, s>>>(m, S.A.rp.p, S.A.v.p, ruiz, tr.p);
+            HPRQP_HIP(hipGetLastError());
+        }
+        {
+            double *cn = S.col_norm.p, *c = S.c.p, *l = S.l.p, *u = S.u.p;
+            const double *t = tc.p;
+            each(s, n, [=] __device__(int64_t j) {
+                cn[j] *= t[j];
+                c[j] /= t[j];
+                l[j] *= t[j];
+                u[j] *= t[j];
+            });
+        }
+        if (m > 0) {
+            double *rn = S.row_norm.p, *al = S.AL.p, *au = S.AU.p;
+            const double *t = tr.p;
+            each(s, m, [=] __device__(int64_t i) {
+                rn[i] *= t[i];
+                al[i] /= t[i];
+                au[i] /= t[i];
+            });
+        }
+        if (S.Q.nnz > 0) {
+            k_scale_csr<<<nblk(n), kBlock, 0, s>>>(n, S.Q.rp.p, S.Q.ci.p, S.Q.v.p, tc.p, tc.p);
+            HPRQP_HIP(hipGetLastError());
+        }
+        if (m > 0 && S.A.nnz > 0) {
+            k_scale_csr<<<nblk(m), kBlock, 0, s>>>(m, S.A.rp.p, S.A.ci.p, S.A.v.p, tr.p, tc.p);
+            k_scale_csr<<<nblk(n), kBlock, 0, s>>>(n, S.AT.rp.p, S.AT.ci.p, S.AT.v.p, tc.p, tr.p);
+            HPRQP_HIP(hipGetLastError());
+        }
+    }
+    // Q = (Q + Q')/2 (src/utils.jl:772-777) over the symmetric pattern
+    if (S.Q.nnz > 0) {
+        DevVec<int32_t> tpos(S.Q.nnz);
+        HPRQP_HIP(hipMemcpyAsync(tpos.p, md->q_tpos.data(), sizeof(int32_t) * S.Q.nnz,
+                                 hipMemcpyHostToDevice, s));
+        DevVec<double> v2(S.Q.nnz);
+        const double *v = S.Q.v.p;
+        const int32_t *tp = tpos.p;
+        double *o = v2.p;
+        each(s, S.Q.nnz, [=] __device__(int64_t k) { o[k] = (v[k] + v[tp[k]]) / 2.0; });
+        HPRQP_HIP(hipStreamSynchronize(s));
+        std::swap(S.Q.v, v2);
+    }
+    red.enqueue<1, 1>(m, BoundNorm{S.AL.p, S.AU.p}, 0);
+    red.enqueue<1, 1>(n, VecNorm{S.c.p}, 2);
+    h = red.fetch(4);
+    S.norm_b = m > 0 ? std::sqrt(h[0]) : 0.0;
+    S.norm_c = std::sqrt(h[2]);
+}
+
+// check_Q_diagonal after scaling (src/utils.jl:827-855)
+struct DiagCheck {
+    const int32_t *rp, *ci;
+    const double *v;
+    double *diag;
+    __device__ void operator()(int64_t i, double *s, double *mx) const {
+        double sum = 0.0, d = 0.0;
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            sum += fabs(v[k]);
+            if (ci[k] == i) d = v[k];
+            mx[1] = dmax(mx[1], v[k]);
+        }
+        diag[i] = d;
+        const double ad = fabs(d);
+        const bool ok = sum == ad || (isfinite(sum) && isfinite(ad) &&
+                                      fabs(sum - ad) <= 1.4901161193847656e-8 * dmax(sum, ad));
+        if (!ok) mx[0] = 1.0;
+    }
+};
+
+static void check_Q_diagonal(Reducer &red, Scaled &S) {
+    S.diag_Q.alloc(S.n);
+    fill(red.s, S.diag_Q.p, S.n, 0.0);
+    if (S.qkind != QOP_CSR) {
+        S.Q_is_diag = false;
+        S.diag_Q_host.assign(S.n, 0.0);
+        return;
+    }
+    red.enqueue<0, 2>(S.n, DiagCheck{S.Q.rp.p, S.Q.ci.p, S.Q.v.p, S.diag_Q.p}, 0);
+    const double *h = red.fetch(2);
+    S.Q_is_diag = h[0] == 0.0;
+    S.q_max_val = h[1];
+    S.diag_Q_host.resize(S.n);
+    HPRQP_HIP(hipMemcpy(S.diag_Q_host.data(), S.diag_Q.p, sizeof(double) * S.n, hipMemcpyDeviceToHost));
+}
+
+// deterministic N(0,1) stream for the power-method start vector; the same
+// counter-based generator is used by the CPU oracle so both start alike.
+static uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static void randn_stream(uint64_t seed, int64_t count, double *out) {
+    const double two_pi = 6.283185307179586476925286766559;
+    const uint64_t ks = mix64(seed + 0x9E3779B97F4A7C15ULL);
+    for (int64_t i = 0; i < count; ++i) {
+        const uint64_t k0 = 2 * (uint64_t)i, k1 = k0 + 1;
+        const double a = (double)(mix64(ks ^ (k0 * 0xD1B54A32D192ED03ULL + 0x632BE59BD9B4E019ULL)) >> 11) *
+                         (1.0 / 9007199254740992.0);
+        const double b = (double)(mix64(ks ^ (k1 * 0xD1B54A32D192ED03ULL + 0x632BE59BD9B4E019ULL)) >> 11) *
+                         (1.0 / 9007199254740992.0);
+        out[i] = std::sqrt(-2.0 * std::log(1.0 - a)) * std::cos(two_pi * b);
+    }
+}
+
+// ---------------------------------------------------------------------
+// Solver<VT>
+// ---------------------------------------------------------------------
+template <class VT>
+struct Solver {
+    hprqp_ctx *ctx;
+    hipStream_t s;
+    Scaled &S;
+    Reducer &red;
+    const hprqp_params *p;
+    int32_t n, m;
+    // bulk vectors
+    DevVec<VT> w, w_bar, dw, x, x_hat, x_bar, dx, y, y_bar, dy, z_bar;
+    DevVec<VT> ATy, ATy_bar, ATdy, QATdy, s_, Qw, Qw_bar, dQw, tempv;
+    DevVec<VT> fact1, fact2, fact, fact_M;
+    DevVec<VT> qtemp;  // operator temp (LASSO rows / QAP n)
+    // anchors (FP64)
+    DevVec<double> last_x, last_y, last_Qw, last_w, last_ATy;
+    // FP64 residual temporaries
+    DevVec<double> Qx_d, ATdy_d, Ax_d, xd_tmp, qtemp_d;
+    // data in VT (aliases of the FP64 arrays when VT == double)
+    DevVec<VT> c_own, l_own, u_own, AL_own, AU_own;
+    const VT *cV, *lV, *uV, *ALV, *AUV;
+    double sigma = 1.0, lambda_max_A = 0.0, lambda_max_Q = 0.0, lambda = 0.0;
+    bool noC = false;
+
+    Solver(hprqp_ctx *c, Scaled &sc, Reducer &r, const hprqp_params *pp)
+        : ctx(c), s(c->stream), S(sc), red(r), p(pp), n(sc.n), m(sc.m) {}
+
+    void alloc() {
+        for (DevVec<VT> *v : {&w, &w_bar, &dw, &x, &x_hat, &x_bar, &dx, &z_bar, &ATy, &ATy_bar, &ATdy,
+                              &QATdy, &Qw, &Qw_bar, &dQw, &tempv, &fact1, &fact2, &fact, &fact_M}) {
+            v->alloc(n);
+            HPRQP_HIP(hipMemsetAsync(v->p, 0, sizeof(VT) * n, s));
+        }
+        for (DevVec<VT> *v : {&y, &y_bar, &dy, &s_}) {
+            v->alloc(m);
+            if (m) HPRQP_HIP(hipMemsetAsync(v->p, 0, sizeof(VT) * m, s));
+        }
+        for (DevVec<double> *v : {&last_x, &last_Qw, &last_w, &last_ATy, &Qx_d, &ATdy_d, &xd_tmp}) {
+            v->alloc(n);
+            HPRQP_HIP(hipMemsetAsync(v->p, 0, sizeof(double) * n, s));
+        }
+        for (DevVec<double> *v : {&last_y, &Ax_d}) {
+            v->alloc(m);
+            if (m) HPRQP_HIP(hipMemsetAsync(v->p, 0, sizeof(double) * m, s));
+        }
+        if (S.qkind == QOP_LASSO) {
+            qtemp.alloc(S.qop->la.nrow);
+            qtemp_d.alloc(S.qop->la.nrow);
+        } else if (S.qkind == QOP_QAP) {
+            qtemp.alloc(n);
+            qtemp_d.alloc(n);
+        }
+        if constexpr (std::is_same<VT, double>::value) {
+            cV = S.c.p;
+            lV = S.l.p;
+            uV = S.u.p;
+            ALV = S.AL.p;
+            AUV = S.AU.p;
+        } else {
+            auto cvt = [&](DevVec<VT> &dst, const DevVec<double> &src) {
+                dst.alloc(src.n);
+                const double *a = src.p;
+                VT *b = dst.p;
+                each(s, src.n, [=] __device__(int64_t i) { b[i] = (VT)a[i]; });
+            };
+            cvt(c_own, S.c);
+            cvt(l_own, S.l);
+            cvt(u_own, S.u);
+            cvt(AL_own, S.AL);
+            cvt(AU_own, S.AU);
+            cV = c_own.p;
+            lV = l_own.p;
+            uV = u_own.p;
+            ALV = AL_own.p;
+            AUV = AU_own.p;
+            if (S.qkind == QOP_CSR) ensure_v32(S.Q);
+            ensure_v32(S.A);
+            ensure_v32(S.AT);
+            if (S.qkind == QOP_LASSO) {
+                ensure_v32(S.qop->la);
+                ensure_v32(S.qop->lat);
+            }
+        }
+    }
+
+    // ---------------- Q applies -------------------------------------
+    // Qx = Q x in VT precision
+    void q_apply(const VT *xin, VT *out) {
+        switch (S.qkind) {
+            case QOP_CSR:
+                if (S.q_empty) {
+                    HPRQP_HIP(hipMemsetAsync(out, 0, sizeof(VT) * n, s));
+                } else {
+                    csr_apply<VT, VT>(s, S.Q, xin, EpiStore<VT>{out});
+                }
+                break;
+            case QOP_LASSO:
+                csr_apply<VT, VT>(s, S.qop->la, xin, EpiStore<VT>{qtemp.p});
+                csr_apply<VT, VT>(s, S.qop->lat, (const VT *)qtemp.p, EpiStore<VT>{out});
+                break;
+            case QOP_QAP:
+                if constexpr (std::is_same<VT, double>::value)
+                    qap_apply_fp64(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xin,
+                                   qtemp.p, out);
+                else
+                    qap_apply_fp32(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xin,
+                                   qtemp.p, out);
+                break;
+            default: {
+                if constexpr (std::is_same<VT, double>::value) {
+                    int rc = S.qop->vt.apply(S.qop->user, xin, out, (void *)s);
+                    HPRQP_REQUIRE(rc == 0, HPRQP_E_ARG, "user Q operator apply failed");
+                } else {
+                    double *xd = xd_tmp.p, *od = Qx_d.p;
+                    each(s, n, [=] __device__(int64_t i) { xd[i] = (double)xin[i]; });
+                    int rc = S.qop->vt.apply(S.qop->user, xd, od, (void *)s);
+                    HPRQP_REQUIRE(rc == 0, HPRQP_E_ARG, "user Q operator apply failed");
+                    each(s, n, [=] __device__(int64_t i) { out[i] = (VT)od[i]; });
+                }
+            }
+        }
+    }
+
+    // Q x with a fused per-row epilogue; qbuf receives the product when Q
+    // is a dense/user operator (then the epilogue runs elementwise).
+    template <class Epi>
+    void q_fused(const VT *xin, const Epi &epi, VT *qbuf) {
+        if (S.qkind == QOP_CSR && !S.q_empty) {
+            csr_apply<VT, VT>(s, S.Q, xin, epi);
+        } else if (S.qkind == QOP_LASSO) {
+            csr_apply<VT, VT>(s, S.qop->la, xin, EpiStore<VT>{qtemp.p});
+            csr_apply<VT, VT>(s, S.qop->lat, (const VT *)qtemp.p, epi);
+        } else {
+            q_apply(xin, qbuf);
+            vec_apply<VT, VT>(s, n, (const VT *)qbuf, epi);
+        }
+    }
+
+    // Qx = Q x in FP64 from a VT input (residuals)
+    void q_apply_d(const VT *xin, double *out) {
+        switch (S.qkind) {
+            case QOP_CSR:
+                if (S.q_empty)
+                    HPRQP_HIP(hipMemsetAsync(out, 0, sizeof(double) * n, s));
+                else
+                    csr_apply<double, double>(s, S.Q, xin, EpiStore<double>{out});
+                break;
+            case QOP_LASSO:
+                csr_apply<double, double>(s, S.qop->la, xin, EpiStore<double>{qtemp_d.p});
+                csr_apply<double, double>(s, S.qop->lat, (const double *)qtemp_d.p, EpiStore<double>{out});
+                break;
+            default: {
+                const double *xd;
+                if constexpr (std::is_same<VT, double>::value) {
+                    xd = xin;
+                } else {
+                    double *t = xd_tmp.p;
+                    each(s, n, [=] __device__(int64_t i) { t[i] = (double)xin[i]; });
+                    xd = t;
+                }
+                if (S.qkind == QOP_QAP) {
+                    qap_apply_fp64(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xd,
+                                   qtemp_d.p, out);
+                } else {
+                    int rc = S.qop->vt.apply(S.qop->user, xd, out, (void *)s);
+                    HPRQP_REQUIRE(rc == 0, HPRQP_E_ARG, "user Q operator apply failed");
+                }
+            }
+        }
+    }
+
+    // ---------------- power method (src/utils.jl:1335-1461) ---------
+    struct SumSq {
+        const double *v;
+        __device__ void operator()(int64_t i, double *s, double *) const { s[0] += v[i] * v[i]; }
+    };
+    struct Dot2 {
+        const double *a, *b;
+        __device__ void operator()(int64_t i, double *s, double *) const { s[0] += a[i] * b[i]; }
+    };
+
+    // which = 0: A A^T (size m); which = 1: Q (size n).  FP64 throughout.
+    double power_iteration(int which, int64_t max_it, double tol, int64_t *its) {
+        const int64_t len = which == 0 ? m : n;
+        std::vector<double> z0(len);
+        randn_stream(1, len, z0.data());
+        for (auto &v : z0) v += 1e-8;
+        DevVec<double> z(len), q(len), ATq(which == 0 ? n : 0), qd_tmp(which == 1 ? n : 0);
+        HPRQP_HIP(hipMemcpyAsync(z.p, z0.data(), sizeof(double) * len, hipMemcpyHostToDevice, s));
+        double lambda_max = 1.0;
+        for (int64_t it = 1; it <= max_it; ++it) {
+            red.enqueue<1, 0>(len, SumSq{z.p}, 0);
+            double nz = std::sqrt(red.fetch(1)[0]);
+            {
+                double *qq = q.p;
+                const double *zz = z.p;
+                each(s, len, [=] __device__(int64_t i) { qq[i] = zz[i] / nz; });
+            }
+            if (which == 0) {
+                csr_apply<double, double>(s, S.AT, (const double *)q.p, EpiStore<double>{ATq.p});
+                csr_apply<double, double>(s, S.A, (const double *)ATq.p, EpiStore<double>{z.p});
+            } else {
+                qop_fp64(q.p, z.p);
+            }
+            red.enqueue<1, 0>(len, Dot2{q.p, z.p}, 0);
+            lambda_max = red.fetch(1)[0];
+            {
+                double *qq = q.p;
+                const double *zz = z.p;
+                const double lm = lambda_max;
+                each(s, len, [=] __device__(int64_t i) { qq[i] = zz[i] - lm * qq[i]; });
+            }
+            red.enqueue<1, 0>(len, SumSq{q.p}, 0);
+            red.enqueue<1, 0>(len, SumSq{z.p}, 1);
+            const double *h = red.fetch(2);
+            const double err = std::sqrt(h[0]) / (std::sqrt(h[1]) + lambda_max);
+            *its = it;
+            if (err < tol) return lambda_max;
+        }
+        if (p->verbose) std::printf("Power iteration did not converge within %lld iterations\n", (long long)max_it);
+        return lambda_max;
+    }
+
+    // FP64 operator apply on FP64 vectors (power method)
+    void qop_fp64(const double *xin, double *out) {
+        switch (S.qkind) {
+            case QOP_CSR:
+                csr_apply<double, double>(s, S.Q, xin, EpiStore<double>{out});
+                break;
+            case QOP_LASSO:
+                csr_apply<double, double>(s, S.qop->la, xin, EpiStore<double>{qtemp_d.p});
+                csr_apply<double, double>(s, S.qop->lat, (const double *)qtemp_d.p, EpiStore<double>{out});
+                break;
+            case QOP_QAP:
+                qap_apply_fp64(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xin,
+                               qtemp_d.p, out);
+                break;
+            default: {
+                int rc = S.qop->vt.apply(S.qop->user, xin, out, (void *)s);
+                HPRQP_REQUIRE(rc == 0, HPRQP_E_ARG, "user Q operator apply failed");
+            }
+        }
+    }
+
+    // ---------------- steps ------------------------------------------
+    template <bool FULL, bool DIAG>
+    void step_zxw1(double h1, double h2) {
+        EpiZxw1<VT, FULL, DIAG> e;
+        e.Qw = Qw.p;
+        e.ATy = ATy.p;
+        e.c = cV;
+        e.l = lV;
+        e.u = uV;
+        e.w = w.p;
+        e.fact1 = fact1.p;
+        e.fact2 = fact2.p;
+        e.last_x = last_x.p;
+        e.x = x.p;
+        e.x_hat = x_hat.p;
+        e.w_bar = w_bar.p;
+        e.dx = dx.p;
+        e.x_bar = x_bar.p;
+        e.z_bar = z_bar.p;
+        e.sigma = sigma;
+        e.f2s = 1.0 / (1.0 + sigma * lambda_max_Q);
+        e.f1s = 1.0 - e.f2s;
+        e.h1 = h1;
+        e.h2 = h2;
+        q_fused(w.p, e, Qw.p);
+    }
+    void step_tempv() {
+        EpiTempv<VT> e{x_hat.p, Qw.p, tempv.p, nullptr, sigma};
+        q_fused(w_bar.p, e, Qw_bar.p);
+    }
+    template <bool FULL>
+    void step_y(const VT *t, double h1, double h2) {
+        if (m == 0) return;
+        EpiY<VT, FULL> e;
+        e.AL = ALV;
+        e.AU = AUV;
+        e.last_y = last_y.p;
+        e.y = y.p;
+        e.y_bar = y_bar.p;
+        e.s = s_.p;
+        e.dy = dy.p;
+        e.fact1 = lambda_max_A * sigma;
+        e.fact2 = 1.0 / e.fact1;
+        e.h1 = h1;
+        e.h2 = h2;
+        csr_apply<VT, VT>(s, S.A, t, e);
+    }
+    template <bool FULL, bool DIAG>
+    void step_w2(double h1, double h2) {
+        EpiW2<VT, FULL, DIAG> e;
+        e.fact = fact.p;
+        e.last_w = last_w.p;
+        e.last_ATy = last_ATy.p;
+        e.w = w.p;
+        e.w_bar = w_bar.p;
+        e.ATy = ATy.p;
+        e.dw = dw.p;
+        e.ATdy = ATdy.p;
+        e.ATy_bar = ATy_bar.p;
+        e.fs = sigma / (1.0 + sigma * lambda_max_Q);
+        e.h1 = h1;
+        e.h2 = h2;
+        csr_apply<VT, VT>(s, S.AT, (const VT *)y_bar.p, e);
+    }
+    template <bool FULL, bool DIAG>
+    void step_noc(double h1, double h2) {
+        EpiNocZxw<VT, FULL, DIAG> e;
+        e.ATy = ATy.p;
+        e.Qw = Qw.p;
+        e.c = cV;
+        e.l = lV;
+        e.u = uV;
+        e.fact1 = fact1.p;
+        e.fact2 = fact2.p;
+        e.last_x = last_x.p;
+        e.last_w = last_w.p;
+        e.x = x.p;
+        e.x_hat = x_hat.p;
+        e.w = w.p;
+        e.w_bar = w_bar.p;
+        e.dx = dx.p;
+        e.x_bar = x_bar.p;
+        e.z_bar = z_bar.p;
+        e.dw = dw.p;
+        e.sigma = sigma;
+        e.f2s = 1.0 / (1.0 + sigma * lambda_max_Q);
+        e.f1s = 1.0 - e.f2s;
+        e.h1 = h1;
+        e.h2 = h2;
+        csr_apply<VT, VT>(s, S.AT, (const VT *)y.p, e);
+        EpiNocTempv<VT> e2{x_hat.p, last_Qw.p, Qw.p, Qw_bar.p, tempv.p, sigma, h1, h2};
+        q_fused(w_bar.p, e2, Qw_bar.p);
+        step_y<FULL>(tempv.p, h1, h2);
+    }
+    template <bool FULL>
+    void step_lasso(double h1, double h2) {
+        EpiLasso<VT, FULL> e;
+        e.Qw = Qw.p;
+        e.ATy = ATy.p;
+        e.c = cV;
+        e.last_x = last_x.p;
+        e.last_w = last_w.p;
+        e.x = x.p;
+        e.x_hat = x_hat.p;
+        e.w = w.p;
+        e.w_bar = w_bar.p;
+        e.dx = dx.p;
+        e.x_bar = x_bar.p;
+        e.z_bar = z_bar.p;
+        e.dw = dw.p;
+        e.sigma = sigma;
+        e.lambda_sigma = lambda * sigma;
+        e.f2s = 1.0 / (1.0 + sigma * lambda_max_Q);
+        e.f1s = 1.0 - e.f2s;
+        e.h1 = h1;
+        e.h2 = h2;
+        q_fused(w.p, e, Qw.p);
+    }
+    template <bool FULL>
+    void step_lp(double h1, double h2) {
+        EpiLpZx<VT, FULL> e;
+        e.ATy = ATy.p;
+        e.c = cV;
+        e.l = lV;
+        e.u = uV;
+        e.last_x = last_x.p;
+        e.x = x.p;
+        e.x_hat = x_hat.p;
+        e.dx = dx.p;
+        e.x_bar = x_bar.p;
+        e.z_bar = z_bar.p;
+        e.sigma = sigma;
+        e.h1 = h1;
+        e.h2 = h2;
+        csr_apply<VT, VT>(s, S.AT, (const VT *)y.p, e);
+        step_y<FULL>(x_hat.p, h1, h2);
+    }
+
+    template <bool FULL, bool DIAG>
+    void update_impl(double h1, double h2) {
+        if (S.qkind == QOP_LASSO) {
+            step_lasso<FULL>(h1, h2);
+            return;
+        }
+        if (!S.q_empty) {
+            if (noC) {
+                step_noc<FULL, DIAG>(h1, h2);
+            } else {
+                step_zxw1<FULL, DIAG>(h1, h2);
+                if (m > 0) step_tempv();
+                step_y<FULL>(tempv.p, h1, h2);
+                step_w2<FULL, DIAG>(h1, h2);
+            }
+        } else {
+            step_lp<FULL>(h1, h2);
+        }
+    }
+
+    // main_update_gpu! (src/algorithm.jl:1250-1279)
+    void main_update(int64_t inner, bool full) {
+        const double h1 = 1.0 / ((double)inner + 2.0);
+        const double h2 = 1.0 - h1;
+        const bool diag = S.Q_is_diag;
+        if (full)
+            diag ? update_impl<true, true>(h1, h2) : update_impl<true, false>(h1, h2);
+        else
+            diag ? update_impl<false, true>(h1, h2) : update_impl<false, false>(h1, h2);
+    }
+
+    // ---------------- residuals (src/algorithm.jl:37-133) ------------
+    struct ResN {
+        const double *c, *Qx, *ATdy, *cn, *l, *u;
+        const VT *xb, *zb;
+        double c_scale, b_scale, lam;
+        int has_m, iter0;
+        __device__ void operator()(int64_t i, double *s, double *mx) const {
+            const double x = (double)xb[i], q = Qx[i], z = (double)zb[i], ci = c[i];
+            s[0] += ci * x;
+            s[1] += x * q;
+            s[2] += z * x;
+            s[3] += fabs(lam * x);
+            const double sf = cn[i] * c_scale;
+            const double a = has_m ? ATdy[i] : 0.0;
+            mx[0] = dmax(mx[0], fabs((q + ci - a - z) * sf));
+            mx[1] = dmax(mx[1], fabs(a * sf));
+            mx[2] = dmax(mx[2], fabs(q * sf));
+            if (iter0) {
+                const double v = (dmax(l[i] - x, 0.0) + dmax(x - u[i], 0.0)) * (b_scale / cn[i]);
+                mx[3] = dmax(mx[3], fabs(v));
+            }
+        }
+    };
+    struct ResM {
+        const double *AL, *AU, *Ax, *rn;
+        const VT *yb, *sv;
+        double b_scale;
+        __device__ void operator()(int64_t i, double *s, double *mx) const {
+            s[0] += (double)yb[i] * (double)sv[i];
+            const double ax = Ax[i];
+            const double corr = proj(ax, AL[i], AU[i]) - ax;
+            const double sf = rn[i] * b_scale;
+            mx[0] = dmax(mx[0], fabs(corr * sf));
+            mx[1] = dmax(mx[1], fabs(ax * sf));
+        }
+    };
+
+    struct Resid {
+        double err_Rp = 0, err_Rd = 0, kkt = 0, primal_obj = 0, rel_gap = 0, dual_obj = 0;
+    };
+
+    void compute_residuals(int64_t iter, double obj_constant, bool lasso, Resid &r) {
+        q_apply_d(x_bar.p, Qx_d.p);
+        if (m > 0) {
+            csr_apply<double, double>(s, S.AT, (const VT *)y_bar.p, EpiStore<double>{ATdy_d.p});
+            csr_apply<double, double>(s, S.A, (const VT *)x_bar.p, EpiStore<double>{Ax_d.p});
+        }
+        red.enqueue<4, 4>(n, ResN{S.c.p, Qx_d.p, ATdy_d.p, S.col_norm.p, S.l.p, S.u.p, x_bar.p, z_bar.p,
+                                  S.c_scale, S.b_scale, lambda, m > 0, iter == 0},
+                          0);
+        red.enqueue<1, 2>(m, ResM{S.AL.p, S.AU.p, Ax_d.p, S.row_norm.p, y_bar.p, s_.p, S.b_scale}, 8);
+        const double *h = red.fetch(11);
+        const double bc = S.b_scale * S.c_scale;
+        r.primal_obj = bc * (h[0] + 0.5 * h[1]) + obj_constant;
+        r.dual_obj = bc * (-0.5 * h[1] + h[2]) + obj_constant;
+        if (m > 0) r.dual_obj += bc * h[8];
+        if (lasso) {
+            r.primal_obj += bc * h[3];
+            r.dual_obj += bc * h[3];
+        }
+        r.rel_gap = std::fabs(r.primal_obj - r.dual_obj) /
+                    (1.0 + std::max(std::fabs(r.primal_obj), std::fabs(r.dual_obj)));
+        r.err_Rd = h[4] / (1.0 + std::max(std::max(S.norm_c_org, h[5]), h[6]));
+        r.err_Rp = m > 0 ? h[9] / (1.0 + std::max(S.norm_b_org, h[10])) : 0.0;
+        if (iter == 0) r.err_Rp = std::max(r.err_Rp, h[7]);
+        r.kkt = std::max(std::max(r.err_Rp, r.err_Rd), r.rel_gap);
+    }
+
+    // ---------------- M-norm (src/algorithm.jl:689-739) --------------
+    struct MnN {
+        const VT *dx, *dQw, *ATdy, *QATdy, *dw, *fM;
+        int diag;
+        __device__ void operator()(int64_t i, double *s, double *) const {
+            const double a = (double)dx[i], b = (double)dQw[i], c = ATdy ? (double)ATdy[i] : 0.0,
+                         d = QATdy ? (double)QATdy[i] : 0.0, e = (double)dw[i];
+            s[0] += a * a;
+            s[1] += b * a;
+            s[2] += b * c;
+            s[3] += c * a;
+            s[4] += (diag ? c * (double)fM[i] : c) * d;
+            s[5] += b * b;
+            s[6] += e * b;
+        }
+    };
+    struct SumSqV {
+        const VT *v;
+        __device__ void operator()(int64_t i, double *s, double *) const {
+            const double a = (double)v[i];
+            s[0] += a * a;
+        }
+    };
+
+    double compute_M_norm() {
+        q_apply(dw.p, dQw.p);
+        if (m > 0) {
+            csr_apply<VT, VT>(s, S.AT, (const VT *)dy.p, EpiStore<VT>{ATdy.p});
+            q_apply(ATdy.p, QATdy.p);
+        }
+        red.enqueue<7, 0>(n,
+                          MnN{dx.p, dQw.p, m > 0 ? ATdy.p : nullptr, m > 0 ? QATdy.p : nullptr, dw.p, fact_M.p,
+                              S.Q_is_diag},
+                          0);
+        red.enqueue<1, 0>(m, SumSqV{dy.p}, 7);
+        const double *h = red.fetch(8);
+        double M1 = 0.0, M3 = 0.0;
+        double M2 = (1.0 / sigma) * h[0];
+        M2 -= 2.0 * h[1];
+        if (m > 0) {
+            M1 = sigma * lambda_max_A * h[7];
+            M1 -= 2.0 * sigma * h[2];
+            M2 += 2.0 * h[3];
+            if (S.Q_is_diag) {
+                M3 = h[4];
+                M1 += sigma * h[5];
+            } else {
+                M3 = (sigma * sigma) / (1.0 + sigma * lambda_max_Q) * h[4];
+                M1 += sigma * lambda_max_Q * h[6];
+            }
+        } else if (!S.Q_is_diag) {
+            M1 = sigma * lambda_max_Q * h[6];
+        }
+        M2 += std::max(M1, 0.0);
+        const double Mn = std::max(M2, 0.0) + std::max(M3, 0.0);
+        return std::sqrt(Mn);
+    }
+
+    // ---------------- diagonal-Q factors ------------------------------
+    void update_Q_factors() {
+        const double sg = sigma, s2 = sigma * sigma;
+        const double *dq = S.diag_Q.p;
+        VT *f2 = fact2.p, *f = fact.p, *f1 = fact1.p, *fm = fact_M.p;
+        each(s, n, [=] __device__(int64_t i) {
+            const double t = 1.0 / (1.0 + sg * dq[i]);
+            f2[i] = (VT)t;
+            f[i] = (VT)(sg * t);
+            f1[i] = (VT)(sg * dq[i] * t);
+            fm[i] = (VT)(s2 * t);
+        });
+    }
+};
+
+// ---------------------------------------------------------------------
+// restart bookkeeping (host, src/algorithm.jl:441-555, 1154-1171)
+// ---------------------------------------------------------------------
+struct Restart {
+    int restart_flag = 0;
+    bool first_restart = true;
+    double last_gap = INFINITY, current_gap = INFINITY, save_gap = INFINITY;
+    int64_t inner = 0, sufficient = 0, necessary = 0, long_ = 0, times = 0;
+    double weighted_norm = INFINITY, best_gap = INFINITY, best_sigma = 1.0;
+
+    void check(int64_t iter, int64_t check_iter, double sigma) {
+        restart_flag = 0;
+        if (first_restart) {
+            if (iter == check_iter) {
+                first_restart = false;
+                restart_flag = 1;
+                weighted_norm = current_gap;
+                best_gap = current_gap;
+                best_sigma = sigma;
+            }
+        } else if (iter % check_iter == 0) {
+            if (current_gap < 0) current_gap = 1e-6;
+            if (current_gap <= 0.2 * last_gap) {
+                sufficient++;
+                restart_flag = 1;
+            }
+            if (current_gap <= 0.8 * last_gap && current_gap > 1.00 * save_gap) {
+                necessary++;
+                restart_flag = 2;
+            }
+            const double fact = (current_gap / weighted_norm > 1e-1) ? 0.5 : 0.2;
+            if ((double)inner >= fact * (double)iter) {
+                long_++;
+                restart_flag = 3;
+            }
+            if (best_gap > current_gap) {
+                best_gap = current_gap;
+                best_sigma = sigma;
+            }
+            save_gap = current_gap;
+        }
+    }
+};
+
+// golden-section searches (src/kernels.jl:1449-1486; src/unified_operations.jl:348-386)
+static double golden(double ap, double bp, double cp, double dp, double lo, double hi, double tol, int maxiter) {
+    const double phi = (std::sqrt(5.0) - 1.0) / 2.0;
+    auto f = [&](double x) { return ap * x + bp / x + cp * x * x / (1 + dp * x); };
+    double a = lo, b = hi;
+    double c = b - phi * (b - a), d = a + phi * (b - a);
+    double fc = f(c), fd = f(d);
+    for (int i = 1; i <= maxiter; ++i) {
+        if (fd < fc) {
+            a = c;
+            c = d;
+            fc = fd;
+            d = a + phi * (b - a);
+            fd = f(d);
+        } else {
+            b = d;
+            d = c;
+            fd = fc;
+            c = b - phi * (b - a);
+            fc = f(c);
+        }
+        if ((b - a) < tol) break;
+    }
+    return 0.5 * (a + b);
+}
+
+static double golden_Q_diag(double a, double b, const std::vector<double> &Q, const std::vector<double> &c,
+                            const std::vector<double> &d, double lo, double hi, double tol, int maxiter) {
+    const double phi = (std::sqrt(5.0) - 1.0) / 2.0;
+    auto f = [&](double x) {
+        double acc = 0.0;
+        for (size_t i = 0; i < Q.size(); ++i) acc += c[i] * (d[i] / (1.0 + x * Q[i]));
+        return a * x + b / x + x * x * acc;
+    };
+    double x1 = hi - phi * (hi - lo), x2 = lo + phi * (hi - lo);
+    double f1 = f(x1), f2 = f(x2);
+    int iter = 0;
+    while (std::fabs(hi - lo) > tol * std::max(1.0, std::fabs(lo)) && iter < maxiter) {
+        if (f1 > f2) {
+            lo = x1;
+            x1 = x2;
+            f1 = f2;
+            x2 = lo + phi * (hi - lo);
+            f2 = f(x2);
+        } else {
+            hi = x2;
+            x2 = x1;
+            f2 = f1;
+            x1 = hi - phi * (hi - lo);
+            f1 = f(x1);
+        }
+        iter++;
+    }
+    return (f1 < f2) ? x1 : x2;
+}
+
+template <class VT>
+struct SigmaDots {
+    const VT *dx, *dQw, *ATdy, *QATdy, *dw;
+    __device__ void operator()(int64_t i, double *s, double *) const {
+        const double a = (double)dx[i], b = (double)dQw[i], c = ATdy ? (double)ATdy[i] : 0.0,
+                     d = QATdy ? (double)QATdy[i] : 0.0, e = (double)dw[i];
+        s[0] += a * a;
+        s[1] += b * c;
+        s[2] += b * b;
+        s[3] += e * b;
+        s[4] += c * d;
+    }
+};
+
+// update_sigma! (src/algorithm.jl:323-423)
+template <class VT>
+static void update_sigma(Solver<VT> &W, Restart &r, double err_Rd, double err_Rp, double rel_gap) {
+    const hprqp_params *p = W.p;
+    if (p->sigma_fixed || r.restart_flag < 1) return;
+    hipStream_t s = W.s;
+    const int32_t n = W.n, m = W.m;
+    const double sigma_old = W.sigma;
+    {
+        VT *dx = W.dx.p, *dw = W.dw.p;
+        const VT *xb = W.x_bar.p, *wb = W.w_bar.p;
+        const double *lx = W.last_x.p, *lw = W.last_w.p;
+        each(s, n, [=] __device__(int64_t i) {
+            dx[i] = (VT)((double)xb[i] - lx[i]);
+            dw[i] = (VT)((double)wb[i] - lw[i]);
+        });
+    }
+    W.q_apply(W.dw.p, W.dQw.p);
+    if (m > 0) {
+        VT *dy = W.dy.p, *atdy = W.ATdy.p;
+        const VT *yb = W.y_bar.p, *atyb = W.ATy_bar.p;
+        const double *ly = W.last_y.p, *laty = W.last_ATy.p;
+        each(s, m, [=] __device__(int64_t i) { dy[i] = (VT)((double)yb[i] - ly[i]); });
+        each(s, n, [=] __device__(int64_t i) { atdy[i] = (VT)((double)atyb[i] - laty[i]); });
+        W.q_apply(W.ATdy.p, W.QATdy.p);
+    }
+    W.red.template enqueue<5, 0>(n,
+                                 SigmaDots<VT>{W.dx.p, W.dQw.p, m > 0 ? W.ATdy.p : nullptr,
+                                               m > 0 ? W.QATdy.p : nullptr, W.dw.p},
+                                 0);
+    W.red.template enqueue<1, 0>(m, typename Solver<VT>::SumSqV{W.dy.p}, 5);
+    const double *h = W.red.fetch(6);
+    const double dxdx = h[0], dQwATdy = h[1], dQwdQw = h[2], dwdQw = h[3], ATdyQATdy = h[4], dydy = h[5];
+    double a = 0.0, b = dxdx, c = 0.0, d = 0.0;
+    if (m > 0) a = W.lambda_max_A * dydy - 2 * dQwATdy;
+    if (W.S.Q_is_diag) {
+        a += dQwdQw;
+    } else {
+        a += W.lambda_max_Q * dwdQw;
+        if (m > 0) {
+            c = ATdyQATdy;
+            d = W.lambda_max_Q;
+        }
+    }
+    a = std::max(a, 1e-12);
+    b = std::max(b, 1e-12);
+    double est;
+    if (W.S.Q_is_diag) {
+        if (m > 0) {
+            std::vector<VT> hc(n), hd(n);
+            HPRQP_HIP(hipMemcpyAsync(hc.data(), W.ATdy.p, sizeof(VT) * n, hipMemcpyDeviceToHost, s));
+            HPRQP_HIP(hipMemcpyAsync(hd.data(), W.QATdy.p, sizeof(VT) * n, hipMemcpyDeviceToHost, s));
+            HPRQP_HIP(hipStreamSynchronize(s));
+            std::vector<double> cc(hc.begin(), hc.end()), dd(hd.begin(), hd.end());
+            est = golden_Q_diag(a, b, W.S.diag_Q_host, cc, dd, 1e-12, 1e12, 1e-13, 200);
+        } else {
+            est = std::sqrt(b / a);
+        }
+    } else {
+        est = m > 0 ? golden(a, b, c, d, 1e-12, 1e12, 1e-13, 200) : std::sqrt(b / a);
+    }
+    const double fact = std::exp(-0.05 * (r.current_gap / r.best_gap));
+    const double temp_1 = std::max(std::min(err_Rd, err_Rp), std::min(rel_gap, r.current_gap));
+    const double cand = std::exp(fact * std::log(est) + (1 - fact) * std::log(r.best_sigma));
+    auto clampj = [](double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); };
+    double kappa;
+    if (temp_1 > 9e-10)
+        kappa = 1.0;
+    else if (temp_1 > 5e-10)
+        kappa = clampj(std::sqrt(err_Rd / err_Rp), 1e-2, 100.0);
+    else
+        kappa = clampj(err_Rd / err_Rp, 1e-2, 100.0);
+    W.sigma = kappa * cand;
+    if (W.S.Q_is_diag && std::fabs(sigma_old - W.sigma) > 1e-15) W.update_Q_factors();
+}
+
+// do_restart! (src/algorithm.jl:522-555)
+template <class VT>
+static void do_restart(Solver<VT> &W, Restart &r) {
+    if (r.restart_flag <= 0) return;
+    hipStream_t s = W.s;
+    const int32_t n = W.n, m = W.m;
+    if (m > 0) {
+        csr_apply<VT, VT>(s, W.S.AT, (const VT *)W.y_bar.p, EpiStore<VT>{W.ATy_bar.p});
+        VT *y = W.y.p;
+        const VT *yb = W.y_bar.p;
+        double *ly = W.last_y.p;
+        each(s, m, [=] __device__(int64_t i) {
+            y[i] = yb[i];
+            ly[i] = (double)yb[i];
+        });
+    }
+    {
+        VT *x = W.x.p, *w = W.w.p, *aty = W.ATy.p, *qw = W.Qw.p;
+        const VT *xb = W.x_bar.p, *wb = W.w_bar.p, *atyb = W.ATy_bar.p, *qwb = W.Qw_bar.p;
+        double *lx = W.last_x.p, *lw = W.last_w.p, *laty = W.last_ATy.p, *lqw = W.last_Qw.p;
+        const bool hm = m > 0, noc = W.noC;
+        each(s, n, [=] __device__(int64_t i) {
+            x[i] = xb[i];
+            w[i] = wb[i];
+            lx[i] = (double)xb[i];
+            lw[i] = (double)wb[i];
+            if (hm) {
+                laty[i] = (double)atyb[i];
+                aty[i] = atyb[i];
+            }
+            if (noc) {
+                qw[i] = qwb[i];
+                lqw[i] = (double)qwb[i];
+            }
+        });
+    }
+    r.last_gap = r.current_gap;
+    r.save_gap = INFINITY;
+    r.times++;
+    r.inner = 0;
+}
+
+// process_initial_points! (src/algorithm.jl:1783-1836)
+template <class VT>
+static void process_initial_points(Solver<VT> &W, const double *x0, const double *y0) {
+    hipStream_t s = W.s;
+    const int32_t n = W.n, m = W.m;
+    const double bs = W.S.b_scale, cs = W.S.c_scale;
+    if (x0) {
+        DevVec<double> t(n);
+        HPRQP_HIP(hipMemcpyAsync(t.p, x0, sizeof(double) * n, hipMemcpyHostToDevice, s));
+        const double *tp = t.p, *cn = W.S.col_norm.p;
+        VT *x = W.x.p, *xb = W.x_bar.p, *w = W.w.p, *wb = W.w_bar.p;
+        double *lx = W.last_x.p, *lw = W.last_w.p;
+        each(s, n, [=] __device__(int64_t i) {
+            const double v = tp[i] * cn[i] / bs;
+            x[i] = xb[i] = w[i] = wb[i] = (VT)v;
+            lx[i] = lw[i] = (double)(VT)v;
+        });
+        HPRQP_HIP(hipStreamSynchronize(s));
+    }
+    if (y0) {
+        if (m > 0) {
+            DevVec<double> t(m);
+            HPRQP_HIP(hipMemcpyAsync(t.p, y0, sizeof(double) * m, hipMemcpyHostToDevice, s));
+            const double *tp = t.p, *rn = W.S.row_norm.p;
+            VT *y = W.y.p, *yb = W.y_bar.p;
+            double *ly = W.last_y.p;
+            each(s, m, [=] __device__(int64_t i) {
+                const double v = tp[i] * rn[i] / cs;
+                y[i] = yb[i] = (VT)v;
+                ly[i] = (double)(VT)v;
+            });
+            csr_apply<VT, VT>(s, W.S.AT, (const VT *)W.y_bar.p, EpiStore<VT>{W.ATy_bar.p});
+            VT *aty = W.ATy.p;
+            const VT *atyb = W.ATy_bar.p;
+            double *laty = W.last_ATy.p;
+            each(s, n, [=] __device__(int64_t i) {
+                aty[i] = atyb[i];
+                laty[i] = (double)atyb[i];
+            });
+            HPRQP_HIP(hipStreamSynchronize(s));
+        }
+        W.q_apply_d(W.x_bar.p, W.Qx_d.p);
+        {
+            const double *qx = W.Qx_d.p;
+            const VT *xb = W.x_bar.p, *atyb = W.ATy_bar.p, *c = W.cV;
+            VT *zb = W.z_bar.p;
+            const double sg = W.sigma;
+            each(s, n, [=] __device__(int64_t i) {
+                const double z_raw = (double)xb[i] + sg * (-qx[i] + (double)atyb[i] - (double)c[i]);
+                zb[i] = (VT)(((double)xb[i] - z_raw) / sg);
+            });
+        }
+        if (m > 0) {
+            csr_apply<double, double>(s, W.S.A, (const VT *)W.x_bar.p, EpiStore<double>{W.Ax_d.p});
+            const double *ax = W.Ax_d.p;
+            const VT *y = W.y.p, *al = W.ALV, *au = W.AUV;
+            VT *sv = W.s_.p;
+            const double f1 = W.lambda_max_A * W.sigma;
+            each(s, m, [=] __device__(int64_t i) {
+                sv[i] = (VT)dmin(dmax(ax[i] - f1 * (double)y[i], (double)al[i]), (double)au[i]);
+            });
+        }
+        HPRQP_HIP(hipStreamSynchronize(s));
+    }
+}
+
+static int64_t print_step(int64_t k) {
+    if (k <= 0) return 10;
+    int64_t p = 1;
+    while (p <= k / 10) p *= 10;
+    return std::max<int64_t>(p / 10, 10);
+}
+
+// copy the FP32 state of a mixed solve into the FP64 solver (polish start)
+static void promote_state(Solver<float> &F, Solver<double> &D) {
+    hipStream_t s = D.s;
+    auto cp = [&](DevVec<double> &dst, const DevVec<float> &src) {
+        const float *a = src.p;
+        double *b = dst.p;
+        each(s, src.n, [=] __device__(int64_t i) { b[i] = (double)a[i]; });
+    };
+    cp(D.w, F.w);
+    cp(D.w_bar, F.w_bar);
+    cp(D.dw, F.dw);
+    cp(D.x, F.x);
+    cp(D.x_hat, F.x_hat);
+    cp(D.x_bar, F.x_bar);
+    cp(D.dx, F.dx);
+    cp(D.y, F.y);
+    cp(D.y_bar, F.y_bar);
+    cp(D.dy, F.dy);
+    cp(D.z_bar, F.z_bar);
+    cp(D.ATy, F.ATy);
+    cp(D.ATy_bar, F.ATy_bar);
+    cp(D.s_, F.s_);
+    cp(D.Qw, F.Qw);
+    cp(D.Qw_bar, F.Qw_bar);
+    cp(D.tempv, F.tempv);
+    auto cpd = [&](DevVec<double> &dst, const DevVec<double> &src) {
+        if (src.n) HPRQP_HIP(hipMemcpyAsync(dst.p, src.p, sizeof(double) * src.n, hipMemcpyDeviceToDevice, s));
+    };
+    cpd(D.last_x, F.last_x);
+    cpd(D.last_y, F.last_y);
+    cpd(D.last_w, F.last_w);
+    cpd(D.last_ATy, F.last_ATy);
+    cpd(D.last_Qw, F.last_Qw);
+    D.sigma = F.sigma;
+    D.lambda_max_A = F.lambda_max_A;
+    D.lambda_max_Q = F.lambda_max_Q;
+    D.lambda = F.lambda;
+    D.noC = F.noC;
+    if (D.S.Q_is_diag) D.update_Q_factors();
+    HPRQP_HIP(hipStreamSynchronize(s));
+}
+
+// The iteration loop of solve() (src/algorithm.jl:1977-2038).  Returns the
+// status, or -1 when a mixed-precision run hands over to the FP64 polish.
+struct LoopState {
+    Restart r;
+    int64_t iter = 0;
+    int64_t iter_4 = 0, iter_6 = 0;
+    double time_4 = 0.0, time_6 = 0.0;
+    bool first_4 = true, first_6 = true;
+    int64_t nh = 0;
+    double best_kkt = INFINITY;
+    int stall = 0;
+};
+
+template <class VT>
+static int run_loop(Solver<VT> &W, LoopState &L, int64_t iter_begin, double t_start, double obj_constant,
+                    bool lasso, double *hist, int64_t hist_cap, bool mixed, typename Solver<VT>::Resid &res,
+                    bool has_res) {
+    const hprqp_params *p = W.p;
+    Restart &r = L.r;
+    for (int64_t iter = iter_begin; iter <= p->max_iter; ++iter) {
+        L.iter = iter;
+        const double tnow = now_s() - t_start;
+        const int64_t ps = p->print_frequency == -1 ? print_step(iter) : p->print_frequency;
+        const bool print_yes = (iter % ps == 0) || iter == p->max_iter || tnow > p->time_limit;
+        const bool do_res = (iter % p->check_iter == 0) || print_yes || !has_res;
+        if (do_res) {
+            W.compute_residuals(iter, obj_constant, lasso, res);
+            has_res = true;
+            // mixed precision: hand over to the FP64 polish once the FP32
+            // iteration stops improving the FP64-measured KKT residual
+            if (mixed && res.kkt >= p->stoptol && (iter % p->check_iter == 0)) {
+                if (res.kkt < 0.5 * L.best_kkt) {
+                    L.best_kkt = res.kkt;
+                    L.stall = 0;
+                } else if (++L.stall >= 6 && res.kkt < 1e-3) {
+                    L.iter = iter;
+                    return -1;
+                }
+            }
+        }
+        int st = -2;
+        if (res.kkt < p->stoptol)
+            st = HPRQP_STATUS_OPTIMAL;
+        else if (iter == p->max_iter)
+            st = HPRQP_STATUS_MAX_ITER;
+        else if (now_s() - t_start > p->time_limit)
+            st = HPRQP_STATUS_TIME_LIMIT;
+        r.check(iter, p->check_iter, W.sigma);
+        update_sigma(W, r, res.err_Rd, res.err_Rp, res.rel_gap);
+        do_restart(W, r);
+        if (do_res && hist && L.nh < hist_cap) {
+            double *h = hist + L.nh * 8;
+            h[0] = (double)iter;
+            h[1] = res.err_Rp;
+            h[2] = res.err_Rd;
+            h[3] = res.rel_gap;
+            h[4] = res.primal_obj;
+            h[5] = res.dual_obj;
+            h[6] = W.sigma;
+            h[7] = (double)r.restart_flag;
+            L.nh++;
+        }
+        if (p->verbose && (print_yes || st != -2))
+            std::printf("%7lld  %3.2e  %3.2e  %+.8e  %+.8e  %3.2e  %3.2e  %7.2f\n", (long long)iter, res.err_Rp,
+                        res.err_Rd, res.primal_obj, res.dual_obj, res.rel_gap, W.sigma, now_s() - t_start);
+        if (res.kkt < 1e-4 && L.first_4) {
+            L.time_4 = now_s() - t_start;
+            L.iter_4 = iter;
+            L.first_4 = false;
+        }
+        if (res.kkt < 1e-6 && L.first_6) {
+            L.time_6 = now_s() - t_start;
+            L.iter_6 = iter;
+            L.first_6 = false;
+        }
+        if (st != -2) return st;
+        const int64_t next = iter + 1;
+        const int64_t ps2 = p->print_frequency == -1 ? print_step(next) : p->print_frequency;
+        const bool to_check = (next % p->check_iter == 0) || (r.restart_flag > 0) || (next % ps2 == 0);
+        // perform_iteration_step! (src/algorithm.jl:1881-1900)
+        W.main_update(r.inner, to_check);
+        if (r.restart_flag > 0) r.last_gap = W.compute_M_norm();
+        if ((iter + 1) % p->check_iter == 0) r.current_gap = W.compute_M_norm();
+        r.inner += 1;
+    }
+    return HPRQP_STATUS_MAX_ITER;
+}
+
+template <class VT>
+static void collect(Solver<VT> &W, double *x_out, double *y_out, double *z_out, double *w_out) {
+    hipStream_t s = W.s;
+    const int32_t n = W.n, m = W.m;
+    const double bs = W.S.b_scale, cs = W.S.c_scale;
+    DevVec<double> t(std::max(n, m));
+    auto dl = [&](const VT *src, const double *nrm, int64_t len, double sc, int mode, double *out) {
+        if (!out || len == 0) return;
+        double *tp = t.p;
+        each(s, len, [=] __device__(int64_t i) {
+            const double v = (double)src[i];
+            tp[i] = mode == 0 ? sc * (v / nrm[i]) : sc * (v * nrm[i]);
+        });
+        HPRQP_HIP(hipMemcpyAsync(out, tp, sizeof(double) * len, hipMemcpyDeviceToHost, s));
+        HPRQP_HIP(hipStreamSynchronize(s));
+    };
+    dl(W.x_bar.p, W.S.col_norm.p, n, bs, 0, x_out);
+    dl(W.w_bar.p, W.S.col_norm.p, n, bs, 0, w_out);
+    dl(W.z_bar.p, W.S.col_norm.p, n, cs, 1, z_out);
+    dl(W.y_bar.p, W.S.row_norm.p, m, cs, 0, y_out);
+}
+
+template <class VT>
+static void init_solver(Solver<VT> &W, const hprqp_model *md) {
+    W.alloc();
+    W.lambda = md->lambda;
+}
+
+static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool mixed);
+
+void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const double *x0, const double *y0,
+           double *x_out, double *y_out, double *z_out, double *w_out, double *hist, int64_t hist_cap,
+           hprqp_result *res) {
+    std::memset(res, 0, sizeof(*res));
+    HPRQP_REQUIRE(p->check_iter > 0, HPRQP_E_ARG, "check_iter must be positive");
+    HPRQP_REQUIRE(p->print_frequency == -1 || p->print_frequency > 0, HPRQP_E_ARG,
+                  "print_frequency must be -1 or positive");
+    HPRQP_REQUIRE(p->sigma == -1 || p->sigma > 0, HPRQP_E_ARG, "sigma must be positive or -1");
+    HPRQP_REQUIRE(p->precision == HPRQP_PREC_FP64 || p->precision == HPRQP_PREC_MIXED, HPRQP_E_ARG,
+                  "unknown precision mode");
+    const double setup_start = now_s();
+    Reducer red(ctx);
+    Scaled S;
+    build_scaled(ctx, md, p, red, S);
+    check_Q_diagonal(red, S);
+    const bool mixed = p->precision == HPRQP_PREC_MIXED;
+    apply_lanes(ctx->stream, md, p, S, mixed);
+    const bool lasso = p->problem_type == HPRQP_PROBLEM_LASSO;
+    Solver<double> D(ctx, S, red, p);
+    init_solver(D, md);
+    // sigma_0 (src/algorithm.jl:839-851)
+    double sigma0;
+    if (p->sigma == -1) {
+        if (S.norm_c > 1e-16 && S.norm_b > 1e-16 && S.norm_b < 1e16 && S.norm_c < 1e16)
+            sigma0 = S.norm_b / S.norm_c;
+        else
+            sigma0 = 1.0;
+    } else {
+        sigma0 = p->sigma;
+    }
+    D.sigma = sigma0;
+    HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+    res->setup_time = now_s() - setup_start;
+
+    const double t_start = now_s();
+    // estimate_eigenvalues (src/algorithm.jl:1407-1440)
+    int64_t *pa = &res->power_iter_A, *pq = &res->power_iter_Q;
+    D.lambda_max_A = md->m > 0 ? D.power_iteration(0, 5000, 1e-4, pa) * p->eig_factor : 0.0;
+    if (S.qkind != QOP_CSR)
+        D.lambda_max_Q = D.power_iteration(1, 5000, 1e-4, pq) * p->eig_factor;
+    else if (!S.q_empty)
+        D.lambda_max_Q = S.Q_is_diag ? S.q_max_val : D.power_iteration(1, 5000, 1e-4, pq) * p->eig_factor;
+    else
+        D.lambda_max_Q = 0.0;
+    res->power_time = now_s() - t_start;
+    // noC decision uses the unscaled bounds (src/algorithm.jl:1969-1974)
+    int64_t nfree = 0;
+    for (int32_t i = 0; i < md->n; ++i)
+        if (md->l[i] == -INFINITY && md->u[i] == INFINITY) nfree++;
+    const bool noC = (double)nfree > 0.8 * (double)md->n;
+    D.noC = noC;
+
+    LoopState L;
+    L.r.best_sigma = sigma0;
+    Solver<double>::Resid R;
+    int status;
+    int64_t polish_from = -1;
+    if (!mixed) {
+        process_initial_points(D, x0, y0);
+        if (S.Q_is_diag) D.update_Q_factors();
+        status = run_loop(D, L, 0, t_start, md->obj_constant, lasso, hist, hist_cap, false, R, false);
+    } else {
+        Solver<float> F(ctx, S, red, p);
+        init_solver(F, md);
+        F.sigma = sigma0;
+        F.lambda_max_A = D.lambda_max_A;
+        F.lambda_max_Q = D.lambda_max_Q;
+        F.noC = noC;
+        process_initial_points(F, x0, y0);
+        if (S.Q_is_diag) F.update_Q_factors();
+        Solver<float>::Resid RF;
+        status = run_loop(F, L, 0, t_start, md->obj_constant, lasso, hist, hist_cap, true, RF, false);
+        if (status == -1) {
+            // FP64 polish: continue the same iteration from the FP32 state
+            promote_state(F, D);
+            polish_from = L.iter;
+            status = run_loop(D, L, L.iter, t_start, md->obj_constant, lasso, hist, hist_cap, false, R, false);
+        } else {
+            R.err_Rp = RF.err_Rp;
+            R.err_Rd = RF.err_Rd;
+            R.kkt = RF.kkt;
+            R.primal_obj = RF.primal_obj;
+            R.rel_gap = RF.rel_gap;
+            R.dual_obj = RF.dual_obj;
+            collect(F, x_out, y_out, z_out, w_out);
+            res->sigma = F.sigma;
+        }
+    }
+    HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+    res->time = now_s() - t_start;
+    if (!mixed || polish_from >= 0) {
+        collect(D, x_out, y_out, z_out, w_out);
+        res->sigma = D.sigma;
+    }
+    res->status = status;
+    res->iter = L.iter;
+    res->iter_4 = L.iter_4 == 0 ? L.iter : L.iter_4;
+    res->iter_6 = L.iter_6 == 0 ? L.iter : L.iter_6;
+    res->time_4 = L.time_4 == 0.0 ? res->time : L.time_4;
+    res->time_6 = L.time_6 == 0.0 ? res->time : L.time_6;
+    res->primal_obj = R.primal_obj;
+    res->dual_obj = R.dual_obj;
+    res->residuals = R.kkt;
+    res->gap = R.rel_gap;
+    res->err_Rp = R.err_Rp;
+    res->err_Rd = R.err_Rd;
+    res->lambda_max_A = D.lambda_max_A;
+    res->lambda_max_Q = D.lambda_max_Q;
+    res->sigma0 = sigma0;
+    res->restarts = (int32_t)L.r.times;
+    res->Q_is_diag = S.Q_is_diag;
+    res->noC = noC;
+    res->n_history = L.nh;
+    res->polish_iter = polish_from >= 0 ? L.iter - polish_from : 0;
+}
+
+// ---------------------------------------------------------------------
+// measurement and autotuning
+// ---------------------------------------------------------------------
+struct EventTimer {
+    hipEvent_t a, b;
+    hipStream_t s;
+    explicit EventTimer(hipStream_t st) : s(st) {
+        HPRQP_HIP(hipEventCreate(&a));
+        HPRQP_HIP(hipEventCreate(&b));
+    }
+    ~EventTimer() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    template <class F>
+    double avg_ms(int reps, F &&f) {
+        f();  // warm
+        HPRQP_HIP(hipEventRecord(a, s));
+        for (int r = 0; r < reps; ++r) f();
+        HPRQP_HIP(hipEventRecord(b, s));
+        HPRQP_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        HPRQP_HIP(hipEventElapsedTime(&ms, a, b));
+        return (double)ms / reps;
+    }
+};
+
+// Rows-per-group autotuner for one matrix: times the plain product for every
+// lane count (replaces determine_spmv_mode's cuSPARSE-vs-custom choice).
+template <class VT>
+static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
+    DevVec<VT> x(std::max<int64_t>(1, a.ncol)), y(std::max<int64_t>(1, a.nrow));
+    {
+        VT *xp = x.p;
+        each(s, a.ncol, [=] __device__(int64_t i) { xp[i] = (VT)1; });
+    }
+    EventTimer tm(s);
+    int best = a.lanes;
+    double best_ms = INFINITY;
+    for (int L = 1; L <= 64; L *= 2) {
+        const double ms = tm.avg_ms(reps, [&] {
+            launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<VT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
+        });
+        if (ms < best_ms * 0.98) {
+            best_ms = ms;
+            best = L;
+        }
+    }
+    return best;
+}
+
+static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool mixed) {
+    if (p->spmv_lanes > 0) {
+        HPRQP_REQUIRE((p->spmv_lanes & (p->spmv_lanes - 1)) == 0 && p->spmv_lanes <= 64, HPRQP_E_ARG,
+                      "spmv_lanes must be a power of two <= 64");
+        S.Q.lanes = S.A.lanes = S.AT.lanes = p->spmv_lanes;
+        return;
+    }
+    DevCsr *mats[3] = {&S.Q, &S.A, &S.AT};
+    for (int k = 0; k < 3; ++k) {
+        DevCsr &a = *mats[k];
+        if (a.nnz < (1 << 18)) continue;  // heuristic choice is already close
+        if (md->tuned_lanes[k] == 0) {
+            if (mixed) {
+                ensure_v32(a);
+                md->tuned_lanes[k] = tune_lanes<float>(s, a, 10);
+            } else {
+                md->tuned_lanes[k] = tune_lanes<double>(s, a, 10);
+            }
+        }
+        a.lanes = md->tuned_lanes[k];
+    }
+}
+
+template <class VT>
+static void profile_impl(Solver<VT> &W, int reps, double *ms, int *nout) {
+    EventTimer tm(W.s);
+    const bool standard = W.S.qkind != QOP_LASSO && !W.S.q_empty && !W.noC;
+    for (int k = 0; k < 6; ++k) ms[k] = 0.0;
+    if (standard) {
+        const double h1 = 0.5, h2 = 0.5;
+        ms[0] = tm.avg_ms(reps, [&] { W.template step_zxw1<false, false>(h1, h2); });
+        if (W.m > 0) {
+            ms[1] = tm.avg_ms(reps, [&] { W.step_tempv(); });
+            ms[2] = tm.avg_ms(reps, [&] { W.template step_y<false>(W.tempv.p, h1, h2); });
+        }
+        ms[3] = tm.avg_ms(reps, [&] { W.template step_w2<false, false>(h1, h2); });
+    }
+    int64_t inner = 0;
+    ms[4] = tm.avg_ms(reps, [&] { W.main_update(inner++, false); });
+    typename Solver<VT>::Resid r;
+    ms[5] = tm.avg_ms(std::max(1, reps / 10), [&] { W.compute_residuals(1, 0.0, false, r); });
+    if (nout) *nout = 6;
+}
+
+void profile_step(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, int reps, double *ms_out,
+                  int *n_out) {
+    Reducer red(ctx);
+    Scaled S;
+    build_scaled(ctx, md, p, red, S);
+    check_Q_diagonal(red, S);
+    const bool mixed = p->precision == HPRQP_PREC_MIXED;
+    apply_lanes(ctx->stream, md, p, S, mixed);
+    int64_t nfree = 0;
+    for (int32_t i = 0; i < md->n; ++i)
+        if (md->l[i] == -INFINITY && md->u[i] == INFINITY) nfree++;
+    const bool noC = (double)nfree > 0.8 * (double)md->n;
+    auto setup = [&](auto &W) {
+        init_solver(W, md);
+        W.sigma = 1.0;
+        W.lambda_max_A = 1.0;
+        W.lambda_max_Q = 1.0;
+        W.noC = noC;
+        if (S.Q_is_diag) W.update_Q_factors();
+    };
+    if (mixed) {
+        Solver<float> F(ctx, S, red, p);
+        setup(F);
+        profile_impl(F, reps, ms_out, n_out);
+    } else {
+        Solver<double> D(ctx, S, red, p);
+        setup(D);
+        profile_impl(D, reps, ms_out, n_out);
+    }
+    HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// ---------------------------------------------------------------------
+// ABI helpers
+// ---------------------------------------------------------------------
+void csr_spmv_fp64(hprqp_ctx *ctx, const DevCsr &a, const double *x, double *y) {
+    csr_apply<double, double>(ctx->stream, a, x, EpiStore<double>{y});
+}
+
+void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx) {
+    hipStream_t s = ctx->stream;
+    switch (q->kind) {
+        case QOP_CSR:
+            if (q->q.nnz == 0)
+                HPRQP_HIP(hipMemsetAsync(Qx, 0, sizeof(double) * q->n, s));
+            else
+                csr_apply<double, double>(s, q->q, x, EpiStore<double>{Qx});
+            break;
+        case QOP_LASSO:
+            csr_apply<double, double>(s, q->la, x, EpiStore<double>{q->ltemp.p});
+            csr_apply<double, double>(s, q->lat, (const double *)q->ltemp.p, EpiStore<double>{Qx});
+            break;
+        case QOP_QAP:
+            qap_apply_fp64(s, q->nq, q->qa.p, q->qb.p, q->qs.p, q->qt.p, x, q->qtemp.p, Qx);
+            break;
+        default: {
+            int rc = q->vt.apply(q->user, x, Qx, (void *)s);
+            HPRQP_REQUIRE(rc == 0, HPRQP_E_ARG, "user Q operator apply failed");
+        }
+    }
+}
+
+}  // namespace hprqp

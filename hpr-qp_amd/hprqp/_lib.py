@@ -1,0 +1,246 @@
+"""ctypes binding of libhprqp (include/hprqp.h).
+
+The shared library is built in-tree next to this file by
+``hprqp._build.build_library()``.  There is no CPU fallback: importing the
+binding without the built library, or calling it on a machine without a
+GPU, raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libhprqp.so"
+
+HPRQP_OK = 0
+ERRORS = {
+    -1: "invalid argument",
+    -2: "HIP runtime error",
+    -3: "allocation failed",
+    -4: "invalid state",
+    -5: "unsupported",
+    -6: "invalid problem data",
+}
+
+
+class hprqp_params(C.Structure):
+    _fields_ = [
+        ("stoptol", C.c_double),
+        ("sigma", C.c_double),
+        ("max_iter", C.c_int64),
+        ("sigma_fixed", C.c_int32),
+        ("problem_type", C.c_int32),
+        ("time_limit", C.c_double),
+        ("eig_factor", C.c_double),
+        ("check_iter", C.c_int64),
+        ("print_frequency", C.c_int64),
+        ("use_Ruiz_scaling", C.c_int32),
+        ("use_Pock_Chambolle_scaling", C.c_int32),
+        ("use_bc_scaling", C.c_int32),
+        ("precision", C.c_int32),
+        ("verbose", C.c_int32),
+        ("spmv_lanes", C.c_int32),
+    ]
+
+
+class hprqp_result(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("restarts", C.c_int32),
+        ("iter", C.c_int64),
+        ("iter_4", C.c_int64),
+        ("iter_6", C.c_int64),
+        ("time", C.c_double),
+        ("time_4", C.c_double),
+        ("time_6", C.c_double),
+        ("power_time", C.c_double),
+        ("setup_time", C.c_double),
+        ("primal_obj", C.c_double),
+        ("dual_obj", C.c_double),
+        ("residuals", C.c_double),
+        ("gap", C.c_double),
+        ("err_Rp", C.c_double),
+        ("err_Rd", C.c_double),
+        ("lambda_max_A", C.c_double),
+        ("lambda_max_Q", C.c_double),
+        ("sigma", C.c_double),
+        ("sigma0", C.c_double),
+        ("Q_is_diag", C.c_int32),
+        ("noC", C.c_int32),
+        ("n_history", C.c_int64),
+        ("polish_iter", C.c_int64),
+        ("power_iter_A", C.c_int64),
+        ("power_iter_Q", C.c_int64),
+    ]
+
+
+class hprqp_qop_vtable(C.Structure):
+    _fields_ = [
+        ("apply", C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)),
+        ("problem_size", C.CFUNCTYPE(C.c_int64, C.c_void_p)),
+        ("temp_size", C.CFUNCTYPE(C.c_int64, C.c_void_p)),
+        ("name", C.CFUNCTYPE(C.c_char_p, C.c_void_p)),
+    ]
+
+
+_P = C.c_void_p
+_I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+# (name, restype, argtypes)
+SIGNATURES = [
+    ("hprqp_abi_version", C.c_int, []),
+    ("hprqp_params_default", None, [C.POINTER(hprqp_params)]),
+    ("hprqp_ctx_create", C.c_int, [C.c_int, C.POINTER(_P)]),
+    ("hprqp_ctx_destroy", C.c_int, [_P]),
+    ("hprqp_last_error", C.c_char_p, [_P]),
+    ("hprqp_ctx_sync", C.c_int, [_P]),
+    ("hprqp_buf_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    ("hprqp_buf_free", C.c_int, [_P, _P]),
+    ("hprqp_buf_upload", C.c_int, [_P, _P, _P, C.c_size_t]),
+    ("hprqp_buf_download", C.c_int, [_P, _P, _P, C.c_size_t]),
+    ("hprqp_buf_device_ptr", _P, [_P]),
+    ("hprqp_buf_size", C.c_size_t, [_P]),
+    ("hprqp_csr_create", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int64, _P, _P, _P, C.c_int, C.c_int,
+                                   C.POINTER(_P)]),
+    ("hprqp_csr_destroy", C.c_int, [_P, _P]),
+    ("hprqp_csr_spmv", C.c_int, [_P, _P, C.c_int, _P, _P]),
+    ("hprqp_qop_create_csr", C.c_int, [_P, C.c_int32, C.c_int64, _P, _P, _P, C.c_int, C.POINTER(_P)]),
+    ("hprqp_qop_create_lasso", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int64, _P, _P, _P, C.c_int,
+                                         C.POINTER(_P)]),
+    ("hprqp_qop_create_qap", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, C.POINTER(_P)]),
+    ("hprqp_qop_create_user", C.c_int, [_P, C.POINTER(hprqp_qop_vtable), _P, C.POINTER(_P)]),
+    ("hprqp_qop_destroy", C.c_int, [_P, _P]),
+    ("hprqp_qop_apply", C.c_int, [_P, _P, _P, _P]),
+    ("hprqp_qop_info", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_char_p)]),
+    ("hprqp_model_create", C.c_int, [_P, C.c_int32, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int,
+                                     _P, _P, _P, _P, _P, C.c_double, C.c_double, C.POINTER(_P)]),
+    ("hprqp_model_destroy", C.c_int, [_P, _P]),
+    ("hprqp_solve", C.c_int, [_P, _P, C.POINTER(hprqp_params), _P, _P, _P, _P, _P, _P, _P, C.c_int64,
+                              C.POINTER(hprqp_result)]),
+    ("hprqp_profile_step", C.c_int, [_P, _P, C.POINTER(hprqp_params), C.c_int, _P, C.POINTER(C.c_int)]),
+]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libhprqp.so (built in-tree); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"libhprqp.so not found at {LIB_PATH}; build it with __graft_entry__.build() "
+            "(hipcc --offload-arch=gfx950). There is no CPU fallback.")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """Host pointer of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class HPRQPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, 'error')} ({code}): {msg}")
+        self.code = code
+
+
+def check(ctx, rc):
+    if rc != HPRQP_OK:
+        lib = load()
+        msg = lib.hprqp_last_error(ctx).decode() if ctx else ""
+        raise HPRQPError(rc, msg)
+
+
+class Context:
+    """One HIP device + stream (hprqp_ctx)."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        rc = lib.hprqp_ctx_create(int(device), C.byref(h))
+        if rc != HPRQP_OK:
+            raise HPRQPError(rc, f"cannot create a context on device {device}")
+        self.h = h
+        self.device = device
+        self.lib = lib
+
+    def close(self):
+        if self.h:
+            self.lib.hprqp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts = {}
+
+
+def context(device: int = 0) -> Context:
+    if device not in _contexts:
+        _contexts[device] = Context(device)
+    return _contexts[device]
+
+
+class DeviceBuffer:
+    """FP64 device vector (hprqp_buf)."""
+
+    def __init__(self, ctx: Context, n: int):
+        self.ctx = ctx
+        self.n = int(n)
+        h = C.c_void_p()
+        check(ctx.h, ctx.lib.hprqp_buf_alloc(ctx.h, 8 * max(self.n, 1), C.byref(h)))
+        self.h = h
+
+    @classmethod
+    def from_array(cls, ctx, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = cls(ctx, a.size)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == self.n
+        check(self.ctx.h, self.ctx.lib.hprqp_buf_upload(self.ctx.h, self.h, ptr(a), 8 * self.n))
+
+    def download(self):
+        out = np.empty(self.n, dtype=np.float64)
+        check(self.ctx.h, self.ctx.lib.hprqp_buf_download(self.ctx.h, self.h, ptr(out), 8 * self.n))
+        return out
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.hprqp_buf_free(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def exported_symbols():
+    return [s[0] for s in SIGNATURES]
+
+
+def lib_env_summary():
+    return {"path": str(LIB_PATH), "exists": LIB_PATH.exists(), "pid": os.getpid()}

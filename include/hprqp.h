@@ -1,0 +1,224 @@
+/*
+ * hprqp.h -- C ABI of libhprqp, the MI355X (gfx950) implementation of the
+ * HPR-QP iteration (dual Halpern Peaceman-Rachford for convex composite QP):
+ *
+ *     min 1/2 <x,Qx> + <c,x> + phi(x)   s.t.  AL <= A x <= AU,  l <= x <= u
+ *
+ * phi is the box indicator, or lambda*||x||_1 for LASSO.
+ *
+ * Every entry point returns 0 on success or a negative HPRQP_E* code; the
+ * message of the last failure on a context is hprqp_last_error(ctx).  No C++
+ * exception crosses this boundary.  A context owns one HIP stream; work on a
+ * context is asynchronous on that stream unless the entry point returns host
+ * data.  A context is used by one host thread at a time.
+ *
+ * Sparse matrices are CSR with int32 indices (0- or 1-based on input, the
+ * latter being Julia's SparseMatrixCSC transposed/CuSparseMatrixCSR 'O'
+ * layout).  Host arrays are copied; nothing is retained after a call.
+ *
+ * Reference interfaces replaced (HPRQP.jl, Julia):
+ *   optimize / solve ................. src/algorithm.jl:1511-1554, 1904-2039
+ *   Qmap! plugin interface ........... src/Q_operators/Q_operator_interface.jl:5-92
+ *   CSR/LASSO/QAP operators .......... src/Q_operators/{sparse_matrix,LASSO,QAP}_operator.jl
+ *   A*x, A'*y (cuSPARSE) ............. src/kernels.jl:33-99, src/unified_operations.jl:117-143
+ *   fused half-step kernels .......... src/kernels.jl:261-1426
+ *   residual/KKT reductions .......... src/algorithm.jl:37-133, src/kernels.jl:114-194
+ *   M-norm, sigma, restart ........... src/algorithm.jl:323-555, 689-739
+ *   power method ..................... src/utils.jl:1335-1461
+ *   scaling .......................... src/utils.jl:350-804
+ */
+#ifndef HPRQP_H
+#define HPRQP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPRQP_ABI_VERSION 1
+
+/* ---- error codes ---------------------------------------------------- */
+#define HPRQP_OK 0
+#define HPRQP_E_ARG (-1)     /* invalid argument (dimension, NULL, range) */
+#define HPRQP_E_HIP (-2)     /* HIP runtime error */
+#define HPRQP_E_ALLOC (-3)   /* device or host allocation failed */
+#define HPRQP_E_STATE (-4)   /* object used in the wrong state */
+#define HPRQP_E_UNSUP (-5)   /* unsupported combination */
+#define HPRQP_E_DATA (-6)    /* invalid problem data (NaN/Inf, l > u, ...) */
+
+/* ---- status (HPRQP_results.status) ---------------------------------- */
+#define HPRQP_STATUS_OPTIMAL 0
+#define HPRQP_STATUS_MAX_ITER 1
+#define HPRQP_STATUS_TIME_LIMIT 2
+
+/* ---- problem types (HPRQP_parameters.problem_type) ------------------ */
+#define HPRQP_PROBLEM_QP 0
+#define HPRQP_PROBLEM_LASSO 1
+#define HPRQP_PROBLEM_QAP 2
+
+/* ---- precision modes ------------------------------------------------- */
+#define HPRQP_PREC_FP64 0   /* everything FP64 (plain form) */
+#define HPRQP_PREC_MIXED 1  /* FP32 bulk, FP64 anchors/KKT, FP64 polish */
+
+typedef struct hprqp_ctx hprqp_ctx;
+typedef struct hprqp_buf hprqp_buf;
+typedef struct hprqp_csr hprqp_csr;
+typedef struct hprqp_qop hprqp_qop;
+typedef struct hprqp_model hprqp_model;
+
+/* Solver parameters.  Field meanings and defaults follow HPRQP_parameters
+ * (src/structs.jl:195-275); hprqp_params_default fills the defaults. */
+typedef struct {
+    double stoptol;             /* 1e-6 */
+    double sigma;               /* -1 = automatic */
+    int64_t max_iter;           /* 2^31-1 */
+    int32_t sigma_fixed;        /* 0 */
+    int32_t problem_type;       /* HPRQP_PROBLEM_QP */
+    double time_limit;          /* 3600 s */
+    double eig_factor;          /* 1.05 */
+    int64_t check_iter;         /* 100 */
+    int64_t print_frequency;    /* -1 = adaptive cadence */
+    int32_t use_Ruiz_scaling;   /* 1 */
+    int32_t use_Pock_Chambolle_scaling; /* 1 */
+    int32_t use_bc_scaling;     /* 0 (disabled in the reference as well) */
+    int32_t precision;          /* HPRQP_PREC_FP64 */
+    int32_t verbose;            /* 0 */
+    int32_t spmv_lanes;         /* 0 = autotune, else lanes per row (1..64) */
+} hprqp_params;
+
+/* Results (HPRQP_results, src/structs.jl:292-309) plus diagnostics. */
+typedef struct {
+    int32_t status;
+    int32_t restarts;
+    int64_t iter;
+    int64_t iter_4;
+    int64_t iter_6;
+    double time;          /* seconds, the reference's results.time window */
+    double time_4;
+    double time_6;
+    double power_time;
+    double setup_time;    /* scaling + allocation, outside results.time */
+    double primal_obj;
+    double dual_obj;
+    double residuals;     /* max(err_Rp, err_Rd, rel_gap) */
+    double gap;
+    double err_Rp;
+    double err_Rd;
+    double lambda_max_A;
+    double lambda_max_Q;
+    double sigma;
+    double sigma0;
+    int32_t Q_is_diag;
+    int32_t noC;
+    int64_t n_history;
+    int64_t polish_iter;  /* FP64 polish iterations (mixed mode) */
+    int64_t power_iter_A; /* power-method iterations for lambda_max(A A^T) */
+    int64_t power_iter_Q; /* power-method iterations for lambda_max(Q) */
+} hprqp_result;
+
+/* User Q operator plugin (AbstractQOperator, Q_operator_interface.jl:33-92).
+ * apply must enqueue Qx = Q x on `stream` (a hipStream_t) for device
+ * pointers x, Qx of length n (FP64). */
+typedef struct {
+    int (*apply)(void *user, const double *x_dev, double *Qx_dev, void *stream);
+    int64_t (*problem_size)(void *user);
+    int64_t (*temp_size)(void *user);
+    const char *(*name)(void *user);
+} hprqp_qop_vtable;
+
+/* ---- context -------------------------------------------------------- */
+int hprqp_ctx_create(int device, hprqp_ctx **out);
+int hprqp_ctx_destroy(hprqp_ctx *ctx);
+const char *hprqp_last_error(const hprqp_ctx *ctx);
+int hprqp_ctx_sync(hprqp_ctx *ctx);
+int hprqp_abi_version(void);
+void hprqp_params_default(hprqp_params *p);
+
+/* ---- device buffers (workspace vectors are FP64 device arrays) ------- */
+int hprqp_buf_alloc(hprqp_ctx *ctx, size_t bytes, hprqp_buf **out);
+int hprqp_buf_free(hprqp_ctx *ctx, hprqp_buf *buf);
+int hprqp_buf_upload(hprqp_ctx *ctx, hprqp_buf *buf, const void *host, size_t bytes);
+int hprqp_buf_download(hprqp_ctx *ctx, const hprqp_buf *buf, void *host, size_t bytes);
+void *hprqp_buf_device_ptr(hprqp_buf *buf);
+size_t hprqp_buf_size(const hprqp_buf *buf);
+
+/* ---- CSR matrices (CuSparseMatrixCSR + the stored transpose) ---------- */
+/* replaces CuSparseMatrixCSR(A) / CuSparseMatrixCSR(A') built in
+ * transfer_to_gpu (src/algorithm.jl:1284-1322). */
+int hprqp_csr_create(hprqp_ctx *ctx, int32_t nrow, int32_t ncol, int64_t nnz,
+                     const int32_t *rowptr, const int32_t *colidx,
+                     const double *vals, int index_base, int build_transpose,
+                     hprqp_csr **out);
+int hprqp_csr_destroy(hprqp_ctx *ctx, hprqp_csr *a);
+/* y = A x (transpose=0) or A^T x via the stored transpose (transpose=1);
+ * replaces Amap!/ATmap!/unified_mul! (src/kernels.jl:33-99). */
+int hprqp_csr_spmv(hprqp_ctx *ctx, const hprqp_csr *a, int transpose,
+                   const hprqp_buf *x, hprqp_buf *y);
+
+/* ---- Q operators (Qmap! plugin interface) ----------------------------- */
+/* sparse Q (n x n CSR), src/Q_operators/sparse_matrix_operator.jl */
+int hprqp_qop_create_csr(hprqp_ctx *ctx, int32_t n, int64_t nnz,
+                         const int32_t *rowptr, const int32_t *colidx,
+                         const double *vals, int index_base, hprqp_qop **out);
+/* Q = A^T A for a rows x n data matrix, src/Q_operators/LASSO_operator.jl */
+int hprqp_qop_create_lasso(hprqp_ctx *ctx, int32_t rows, int32_t n, int64_t nnz,
+                           const int32_t *rowptr, const int32_t *colidx,
+                           const double *vals, int index_base, hprqp_qop **out);
+/* Q(X) = 2(A X B - S X - X T), dense nq x nq column-major inputs,
+ * src/Q_operators/QAP_operator.jl */
+int hprqp_qop_create_qap(hprqp_ctx *ctx, int32_t nq, const double *A,
+                         const double *B, const double *S, const double *T,
+                         hprqp_qop **out);
+/* user operator through a vtable of C function pointers */
+int hprqp_qop_create_user(hprqp_ctx *ctx, const hprqp_qop_vtable *vt,
+                          void *user, hprqp_qop **out);
+int hprqp_qop_destroy(hprqp_ctx *ctx, hprqp_qop *q);
+/* Qmap!(x, Qx, Q): FP64 in, FP64 out, on the context stream */
+int hprqp_qop_apply(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x,
+                    hprqp_buf *Qx);
+/* get_problem_size / get_temp_size / get_operator_name */
+int hprqp_qop_info(const hprqp_qop *q, int64_t *n, int64_t *temp_size,
+                   const char **name);
+
+/* ---- model (QP_info_cpu after qp_formulation, src/utils.jl:67-188) ---- */
+/* The model keeps its own device copy of the raw (unscaled) data; Q is
+ * referenced, not copied, and must outlive the model.  A is m x n CSR. */
+int hprqp_model_create(hprqp_ctx *ctx, int32_t n, int32_t m, hprqp_qop *Q,
+                       int64_t a_nnz, const int32_t *a_rowptr,
+                       const int32_t *a_colidx, const double *a_vals,
+                       int index_base, const double *c, const double *AL,
+                       const double *AU, const double *l, const double *u,
+                       double obj_constant, double lambda, hprqp_model **out);
+int hprqp_model_destroy(hprqp_ctx *ctx, hprqp_model *model);
+
+/* ---- solve (optimize -> solve, src/algorithm.jl:1511-1554, 1904-2039) -- */
+/* x0 (length n) and y0 (length m) are optional warm starts in the original
+ * space (params.initial_x / initial_y).  x/y/z/w_out are optional host
+ * outputs in the original space.  hist (optional) receives up to hist_cap
+ * rows of 8 doubles per residual evaluation: iter, err_Rp, err_Rd, rel_gap,
+ * primal_obj, dual_obj, sigma, restart_flag. */
+int hprqp_solve(hprqp_ctx *ctx, const hprqp_model *model,
+                const hprqp_params *params, const double *x0, const double *y0,
+                double *x_out, double *y_out, double *z_out, double *w_out,
+                double *hist, int64_t hist_cap, hprqp_result *result);
+
+/* ---- measurement (determine_spmv_mode's timing loop,
+ *      src/algorithm.jl:1557-1746) ------------------------------------------
+ * Prepares the solver state for `model` exactly as hprqp_solve does (scaling,
+ * power method, autotuned SpMV lanes), then times each fused kernel of the
+ * standard step with HIP events on the context stream, `reps` back-to-back
+ * launches per kernel.  ms_out[0..5] = average ms of: z/x/w1 step (Q w),
+ * tempv step (Q w_bar), y step (A tempv), w2 step (A^T y_bar), one whole
+ * partial iteration, one residual evaluation.  *n_out = number of slots
+ * filled (4 per-kernel slots are 0 when the problem does not take the
+ * standard path).  Iterate values are not meaningful afterwards. */
+int hprqp_profile_step(hprqp_ctx *ctx, const hprqp_model *model,
+                       const hprqp_params *params, int reps, double *ms_out,
+                       int *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HPRQP_H */

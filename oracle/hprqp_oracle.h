@@ -1,0 +1,132 @@
+/*
+ * hprqp_oracle.h -- CPU FP64 restatement of the HPR-QP reference CPU path
+ * (use_gpu=false).  TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load this; the product library never does.
+ *
+ * Sparse matrices are CSC (column pointers, row indices, values), 0-based,
+ * which is the layout of Julia's SparseMatrixCSC that the reference CPU path
+ * multiplies with mul! (a column scatter; for each output entry it sums in
+ * increasing index order, exactly like a sequential CSR row gather).
+ */
+#ifndef HPRQP_ORACLE_H
+#define HPRQP_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_Q_SPARSE = 0, ORACLE_Q_LASSO = 1, ORACLE_Q_QAP = 2 };
+enum { ORACLE_OPTIMAL = 0, ORACLE_MAX_ITER = 1, ORACLE_TIME_LIMIT = 2 };
+
+typedef struct {
+    int32_t kind;
+    /* ORACLE_Q_SPARSE: n x n CSC */
+    int64_t nnz;
+    const int64_t *colptr;
+    const int32_t *rowidx;
+    const double *val;
+    /* ORACLE_Q_LASSO: data matrix, rows x n, CSC */
+    int32_t rows;
+    int64_t l_nnz;
+    const int64_t *l_colptr;
+    const int32_t *l_rowidx;
+    const double *l_val;
+    /* ORACLE_Q_QAP: dense nq x nq column-major, Q(X) = 2(A X B - S X - X T) */
+    int32_t nq;
+    const double *qa, *qb, *qs, *qt;
+} oracle_qop;
+
+typedef struct {
+    double stoptol;
+    double sigma;          /* -1 = automatic */
+    int64_t max_iter;
+    int32_t sigma_fixed;
+    double time_limit;
+    double eig_factor;
+    int64_t check_iter;
+    int64_t print_frequency; /* -1 = adaptive */
+    int32_t use_Ruiz_scaling;
+    int32_t use_Pock_Chambolle_scaling;
+    int32_t problem_type;    /* 0 QP, 1 LASSO, 2 QAP */
+    int32_t pad;
+    int64_t max_power_iter;  /* 0 = the reference's 5000; >0 bounds a timing sample */
+} oracle_params;
+
+typedef struct {
+    int32_t status;
+    int32_t Q_is_diag;
+    int32_t noC;
+    int32_t restarts;
+    int64_t iter;
+    int64_t iter_4;
+    int64_t iter_6;
+    double time;
+    double power_time;
+    double primal_obj;
+    double dual_obj;
+    double residuals;
+    double gap;
+    double err_Rp;
+    double err_Rd;
+    double lambda_max_A;
+    double lambda_max_Q;
+    double sigma;
+    double sigma0;
+    int64_t n_history;
+    int64_t power_iter_A;
+    int64_t power_iter_Q;
+    double setup_time;
+    double power_time_A;
+} oracle_result;
+
+/* History row written at every residual evaluation:
+ * iter, err_Rp, err_Rd, rel_gap, primal_obj, dual_obj, sigma, restart_flag */
+#define ORACLE_HIST_COLS 8
+
+/*
+ * Solve min 1/2<x,Qx> + <c,x> (+ lambda||x||_1 for LASSO)
+ *   s.t. AL <= A x <= AU, l <= x <= u.
+ * A is m x n CSC.  x0/y0 are optional warm starts in the original space.
+ * x_out,y_out,z_out,w_out (may be NULL) receive unscaled results.
+ * hist (may be NULL) receives up to hist_cap rows of ORACLE_HIST_COLS doubles.
+ */
+int oracle_solve(int32_t n, int32_t m, const oracle_qop *q,
+                 int64_t a_nnz, const int64_t *a_colptr, const int32_t *a_rowidx,
+                 const double *a_val, const double *c, const double *AL,
+                 const double *AU, const double *l, const double *u,
+                 double obj_constant, double lambda, const oracle_params *p,
+                 const double *x0, const double *y0, double *x_out,
+                 double *y_out, double *z_out, double *w_out, double *hist,
+                 int64_t hist_cap, oracle_result *res);
+
+/* y = M x for an nrow x ncol CSC matrix (the reference's CPU mul!). */
+void oracle_csc_mul(int32_t nrow, int32_t ncol, const int64_t *colptr,
+                    const int32_t *rowidx, const double *val, const double *x,
+                    double *y);
+
+/* Qx = Q(x) for any oracle_qop (the reference's CPU Qmap!). */
+int oracle_qmap(int32_t n, const oracle_qop *q, const double *x, double *Qx);
+
+/* Largest eigenvalue estimate by the reference's power method on A A^T
+ * (which=0) or Q (which=1); start vector from oracle_randn(seed). */
+double oracle_power_iteration(int32_t n, int32_t m, const oracle_qop *q,
+                              int64_t a_nnz, const int64_t *a_colptr,
+                              const int32_t *a_rowidx, const double *a_val,
+                              int which, int64_t max_it, double tol,
+                              int64_t *iters_out);
+
+/* Deterministic N(0,1) stream used for the power-method start vector. */
+void oracle_randn(uint64_t seed, int64_t count, double *out);
+
+/* Golden-section searches (host sigma update). */
+double oracle_golden(double a, double b, double c, double d, double lo,
+                     double hi, double tol, int maxiter);
+double oracle_golden_Q_diag(double a, double b, int32_t n, const double *Q,
+                            const double *c, const double *d, double lo,
+                            double hi, double tol, int maxiter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,194 @@
+"""GPU parity: libhprqp (FP64 plain form) against the CPU oracle.
+
+Bar (from the reference's CPU path, restated by the oracle): status OPTIMAL,
+relative KKT < 1e-6, objective within 1e-6 relative, iteration count within
+10% (at least one residual-check step, since termination is only tested at
+check/print iterations: every 10 iterations below 1000, every 100 below
+10^4).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import fixture_path
+
+pytestmark = pytest.mark.gpu
+
+
+def _iter_ok(a, b):
+    step = 10 if b < 1000 else (100 if b < 10_000 else 1000)
+    return abs(a - b) <= max(0.10 * b, step)
+
+
+def _solve_both(model, problem_type="QP", **kw):
+    from hprqp import HPRQP_parameters, optimize
+    from oracle import oracle
+
+    params = HPRQP_parameters(verbose=False, problem_type=problem_type, **kw)
+    res = optimize(model, params)
+    ref = oracle.solve(model, problem_type=problem_type,
+                       **{k: v for k, v in kw.items() if k in ("max_iter", "stoptol", "check_iter")})
+    print(f"gpu: {res.status} iter={res.iter} obj={res.primal_obj:.12e} kkt={res.residuals:.3e} "
+          f"lamA={res.lambda_max_A:.6e} lamQ={res.lambda_max_Q:.6e} restarts={res.restarts}")
+    print(f"cpu: {ref.status} iter={ref.iter} obj={ref.primal_obj:.12e} kkt={ref.residuals:.3e} "
+          f"lamA={ref.lambda_max_A:.6e} lamQ={ref.lambda_max_Q:.6e} restarts={ref.restarts}")
+    return res, ref
+
+
+def _check(res, ref, obj_tol=1e-6):
+    assert res.status == "OPTIMAL"
+    assert res.residuals < 1e-6
+    assert ref.status == "OPTIMAL"
+    rel = abs(res.primal_obj - ref.primal_obj) / (1.0 + abs(ref.primal_obj))
+    assert rel < obj_tol, rel
+    assert _iter_ok(res.iter, ref.iter), (res.iter, ref.iter)
+    assert res.Q_is_diag == ref.Q_is_diag
+    assert res.noC == ref.noC
+    # the power method is seeded identically: lambda_max agrees to its tolerance
+    assert abs(res.lambda_max_A - ref.lambda_max_A) <= 1e-3 * max(1.0, ref.lambda_max_A)
+    assert abs(res.lambda_max_Q - ref.lambda_max_Q) <= 1e-3 * max(1.0, ref.lambda_max_Q)
+
+
+def test_model_mps():
+    from hprqp import build_from_mps
+
+    model = build_from_mps(fixture_path("model.mps"), verbose=False)
+    res, ref = _solve_both(model)
+    _check(res, ref)
+    assert abs(res.primal_obj - (-8.5)) < 1e-5
+    np.testing.assert_allclose(res.x, [1.5, 2.5], atol=1e-4)
+
+
+def test_aug2d_noC_diagonal_Q():
+    from hprqp import build_from_mps
+
+    model = build_from_mps(fixture_path("AUG2D.mps"), verbose=False)
+    res, ref = _solve_both(model)
+    _check(res, ref)
+    assert res.noC and res.Q_is_diag
+    # known optimum from an equality-KKT solve (and Maros-Meszaros 1.6874118e+06)
+    assert abs(res.primal_obj - 1687411.752897) / 1687411.752897 < 1e-6
+
+
+def test_aug3dcqp_diagonal_Q():
+    from hprqp import build_from_mps
+
+    model = build_from_mps(fixture_path("AUG3DCQP.mps"), verbose=False)
+    res, ref = _solve_both(model)
+    _check(res, ref)
+
+
+def test_random_sparse_qp_small():
+    from hprqp import instances
+
+    model = instances.random_sparse_qp(2000, 1000, seed=0)
+    res, ref = _solve_both(model)
+    _check(res, ref)
+    assert not res.Q_is_diag and not res.noC
+
+
+def test_grid_qp_small():
+    from hprqp import instances
+
+    model = instances.grid_qp(60, seed=0)
+    res, ref = _solve_both(model)
+    _check(res, ref)
+
+
+def test_lasso_small():
+    from hprqp import instances
+
+    model = instances.lasso(200, 4000, mean_nnz=60, seed=1)
+    res, ref = _solve_both(model, problem_type="LASSO")
+    _check(res, ref)
+
+
+def test_qap_esc64a():
+    from hprqp import build_from_mat
+
+    model = build_from_mat(fixture_path("esc64a.mat"), "QAP", verbose=False)
+    res, ref = _solve_both(model, problem_type="QAP")
+    _check(res, ref)
+
+
+def test_lp_empty_Q():
+    from hprqp import build_from_QAbc
+
+    rng = np.random.default_rng(3)
+    n, m = 60, 30
+    A = sp.random(m, n, density=0.2, random_state=4, format="csr") + sp.eye(m, n)
+    x0 = rng.uniform(0, 1, n)
+    b = A @ x0
+    c = rng.uniform(0.1, 1.0, n)
+    model = build_from_QAbc(sp.csr_matrix((n, n)), c, A, b, b + 1.0, np.zeros(n), np.full(n, 5.0), verbose=False)
+    res, ref = _solve_both(model)
+    _check(res, ref)
+
+
+def test_portfolio_small():
+    from hprqp import instances
+
+    model, _ = instances.portfolio(300, 10, seed=0)
+    res, ref = _solve_both(model)
+    _check(res, ref)
+
+
+def test_warm_start_at_optimum_gives_iter_0():
+    """test/runtests.jl:209-246: x and y warm start at the optimum -> iter == 0."""
+    from hprqp import HPRQP_parameters, build_from_mps, optimize
+
+    model = build_from_mps(fixture_path("model.mps"), verbose=False)
+    cold = optimize(model, HPRQP_parameters(verbose=False))
+    warm = optimize(model, HPRQP_parameters(verbose=False, initial_x=cold.x, initial_y=cold.y))
+    assert warm.status == "OPTIMAL"
+    assert warm.iter == 0
+    xonly = optimize(model, HPRQP_parameters(verbose=False, initial_x=cold.x))
+    assert xonly.status == "OPTIMAL" and xonly.iter <= cold.iter
+
+
+def test_csr_spmv_matches_scipy():
+    import ctypes as C
+
+    from hprqp import _lib
+
+    ctx = _lib.context(0)
+    rng = np.random.default_rng(0)
+    for (m, n, dens) in ((1, 1, 1.0), (37, 53, 0.2), (1000, 700, 0.01), (300, 5000, 0.3)):
+        M = sp.random(m, n, density=dens, random_state=rng, format="csr")
+        M.sort_indices()
+        rp = M.indptr.astype(np.int32)
+        ci = M.indices.astype(np.int32)
+        v = M.data.astype(np.float64)
+        h = C.c_void_p()
+        _lib.check(ctx.h, ctx.lib.hprqp_csr_create(ctx.h, m, n, M.nnz, _lib.ptr(rp), _lib.ptr(ci), _lib.ptr(v), 0,
+                                                   1, C.byref(h)))
+        x = rng.standard_normal(n)
+        yv = rng.standard_normal(m)
+        xb = _lib.DeviceBuffer.from_array(ctx, x)
+        yb = _lib.DeviceBuffer(ctx, m)
+        _lib.check(ctx.h, ctx.lib.hprqp_csr_spmv(ctx.h, h, 0, xb.h, yb.h))
+        np.testing.assert_allclose(yb.download(), M @ x, rtol=1e-12, atol=1e-12)
+        yb2 = _lib.DeviceBuffer.from_array(ctx, yv)
+        xo = _lib.DeviceBuffer(ctx, n)
+        _lib.check(ctx.h, ctx.lib.hprqp_csr_spmv(ctx.h, h, 1, yb2.h, xo.h))
+        np.testing.assert_allclose(xo.download(), M.T @ yv, rtol=1e-12, atol=1e-12)
+        ctx.lib.hprqp_csr_destroy(ctx.h, h)
+
+
+def test_qop_apply_matches_oracle():
+    from hprqp import build_from_mat, instances, to_gpu
+    from oracle import oracle
+
+    rng = np.random.default_rng(1)
+    qap = build_from_mat(fixture_path("esc64a.mat"), "QAP", verbose=False)
+    las = instances.lasso(50, 800, mean_nnz=30, seed=2)
+    rq = instances.random_sparse_qp(500, 100, seed=3)
+    for model in (qap, las, rq):
+        x = rng.standard_normal(model.n)
+        dev = to_gpu(model.Q, model.n)
+        got = dev.Qmap(x)
+        want = oracle.qmap(model.Q, x)
+        np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-9 * np.max(np.abs(want)))
+        n, t, name = dev.info()
+        assert n == model.n
+        dev.close()
