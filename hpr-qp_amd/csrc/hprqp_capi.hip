@@ -72,6 +72,10 @@ HostCsr host_csr_transpose(const HostCsr &a) {
 int default_lanes(const HostCsr &h) {
     if (h.nrow <= 0) return 1;
     const double mean = (double)h.v.size() / (double)h.nrow;
+    // short rows: CSR-stream with 256/128/64 rows per block (negative code)
+    if (mean <= 6.0) return -256;
+    if (mean <= 12.0) return -128;
+    if (mean <= 24.0) return -64;
     int L = 1;
     while (L * 2 <= mean && L < 64) L *= 2;
     return L;

@@ -71,6 +71,43 @@ __global__ __launch_bounds__(kBlock) void k_csr(int32_t nrows,
     if (lane == 0) epi(row, acc);
 }
 
+// CSR-stream row product for short rows: a block owns RB consecutive rows;
+// its nonzeros are read once, fully coalesced, and their products staged in
+// LDS; then thread t sums row r0+t in order and runs the epilogue, so every
+// vector access of the epilogue is coalesced as well.  A block whose rows
+// hold more than kStreamCap nonzeros falls back to one thread per row.
+constexpr int kStreamCap = 2048;
+
+template <class MT, class XT, class AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_csr_stream(int32_t nrows, int rb, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci,
+                                                       const MT *__restrict__ v, const XT *__restrict__ x,
+                                                       Epi epi) {
+    __shared__ AT prod[kStreamCap];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t r0 = b * rb;
+    if (r0 >= nrows) return;
+    const int32_t r1 = min(r0 + rb, nrows);
+    const int32_t base = rp[r0];
+    const int32_t cnt = rp[r1] - base;
+    const int t = threadIdx.x;
+    const int32_t row = r0 + t;
+    if (cnt <= kStreamCap) {
+        for (int32_t k = t; k < cnt; k += kBlock) prod[k] = (AT)v[base + k] * (AT)x[ci[base + k]];
+        __syncthreads();
+        if (row < r1) {
+            const int32_t s = rp[row] - base, e = rp[row + 1] - base;
+            AT acc = 0;
+            for (int32_t k = s; k < e; ++k) acc += prod[k];
+            epi(row, acc);
+        }
+    } else if (row < r1) {
+        AT acc = 0;
+        for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += (AT)v[k] * (AT)x[ci[k]];
+        epi(row, acc);
+    }
+}
+
 // elementwise form of the same epilogue (operator Q: product precomputed)
 template <class ST, class AT, class Epi>
 __global__ __launch_bounds__(kBlock) void k_vec(int32_t n, const ST *__restrict__ src, Epi epi) {

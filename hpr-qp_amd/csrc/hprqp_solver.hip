@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <type_traits>
 
 #include "hprqp_internal.hpp"
@@ -37,6 +38,13 @@ template <class AT, class MT, class XT, class Epi>
 static void launch_csr(hipStream_t s, int lanes, int32_t nrows, const int32_t *rp, const int32_t *ci,
                        const MT *v, const XT *x, const Epi &epi) {
     if (nrows <= 0) return;
+    if (lanes < 0) {  // CSR-stream: -lanes rows per block
+        const int rb = -lanes;
+        const int grid = (int)(((int64_t)nrows + rb - 1) / rb);
+        k_csr_stream<MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rb, rp, ci, v, x, epi);
+        HPRQP_HIP(hipGetLastError());
+        return;
+    }
     const int64_t threads = (int64_t)nrows * lanes;
     const int grid = (int)((threads + kBlock - 1) / kBlock);
     switch (lanes) {
@@ -110,6 +118,41 @@ struct Reducer {
         HPRQP_HIP(hipMemcpyAsync(host, out.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
         HPRQP_HIP(hipStreamSynchronize(s));
         return host;
+    }
+};
+
+// HIP-event timing of one kernel class during a solve: events bracket each
+// launch on the solve's stream; pairs are drained every `pairs` launches.
+struct KernelTimer {
+    hipStream_t s;
+    std::vector<hipEvent_t> ev;
+    int used = 0;
+    double total_ms = 0.0;
+    int64_t count = 0;
+    explicit KernelTimer(hipStream_t st, int pairs = 512) : s(st), ev(2 * pairs) {
+        for (auto &e : ev) HPRQP_HIP(hipEventCreate(&e));
+    }
+    ~KernelTimer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    void begin() {
+        if (used == (int)ev.size()) flush();
+        HPRQP_HIP(hipEventRecord(ev[used], s));
+    }
+    void end() {
+        HPRQP_HIP(hipEventRecord(ev[used + 1], s));
+        used += 2;
+    }
+    void flush() {
+        if (!used) return;
+        HPRQP_HIP(hipEventSynchronize(ev[used - 1]));
+        for (int k = 0; k < used; k += 2) {
+            float ms = 0.f;
+            HPRQP_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            total_ms += ms;
+            ++count;
+        }
+        used = 0;
     }
 };
 
@@ -338,6 +381,14 @@ struct Solver {
     const VT *cV, *lV, *uV, *ALV, *AUV;
     double sigma = 1.0, lambda_max_A = 0.0, lambda_max_Q = 0.0, lambda = 0.0;
     bool noC = false;
+    KernelTimer *kt = nullptr;  // times the first fused kernel of partial steps
+    bool timing_now = false;
+    void kbegin() {
+        if (timing_now) kt->begin();
+    }
+    void kend() {
+        if (timing_now) kt->end();
+    }
 
     Solver(hprqp_ctx *c, Scaled &sc, Reducer &r, const hprqp_params *pp)
         : ctx(c), s(c->stream), S(sc), red(r), p(pp), n(sc.n), m(sc.m) {}
@@ -583,7 +634,9 @@ struct Solver {
         e.f1s = 1.0 - e.f2s;
         e.h1 = h1;
         e.h2 = h2;
+        kbegin();
         q_fused(w.p, e, Qw.p);
+        kend();
     }
     void step_tempv() {
         EpiTempv<VT> e{x_hat.p, Qw.p, tempv.p, nullptr, sigma};
@@ -648,7 +701,9 @@ struct Solver {
         e.f1s = 1.0 - e.f2s;
         e.h1 = h1;
         e.h2 = h2;
+        kbegin();
         csr_apply<VT, VT>(s, S.AT, (const VT *)y.p, e);
+        kend();
         EpiNocTempv<VT> e2{x_hat.p, last_Qw.p, Qw.p, Qw_bar.p, tempv.p, sigma, h1, h2};
         q_fused(w_bar.p, e2, Qw_bar.p);
         step_y<FULL>(tempv.p, h1, h2);
@@ -675,7 +730,9 @@ struct Solver {
         e.f1s = 1.0 - e.f2s;
         e.h1 = h1;
         e.h2 = h2;
+        kbegin();
         q_fused(w.p, e, Qw.p);
+        kend();
     }
     template <bool FULL>
     void step_lp(double h1, double h2) {
@@ -693,7 +750,9 @@ struct Solver {
         e.sigma = sigma;
         e.h1 = h1;
         e.h2 = h2;
+        kbegin();
         csr_apply<VT, VT>(s, S.AT, (const VT *)y.p, e);
+        kend();
         step_y<FULL>(x_hat.p, h1, h2);
     }
 
@@ -722,6 +781,7 @@ struct Solver {
         const double h1 = 1.0 / ((double)inner + 2.0);
         const double h2 = 1.0 - h1;
         const bool diag = S.Q_is_diag;
+        timing_now = kt != nullptr && !full;
         if (full)
             diag ? update_impl<true, true>(h1, h2) : update_impl<true, false>(h1, h2);
         else
@@ -1352,8 +1412,11 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     const bool mixed = p->precision == HPRQP_PREC_MIXED;
     apply_lanes(ctx->stream, md, p, S, mixed);
     const bool lasso = p->problem_type == HPRQP_PROBLEM_LASSO;
+    std::unique_ptr<KernelTimer> ktimer;
+    if (p->time_kernels) ktimer.reset(new KernelTimer(ctx->stream));
     Solver<double> D(ctx, S, red, p);
     init_solver(D, md);
+    D.kt = ktimer.get();
     // sigma_0 (src/algorithm.jl:839-851)
     double sigma0;
     if (p->sigma == -1) {
@@ -1398,6 +1461,7 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     } else {
         Solver<float> F(ctx, S, red, p);
         init_solver(F, md);
+        F.kt = ktimer.get();
         F.sigma = sigma0;
         F.lambda_max_A = D.lambda_max_A;
         F.lambda_max_Q = D.lambda_max_Q;
@@ -1448,6 +1512,11 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     res->noC = noC;
     res->n_history = L.nh;
     res->polish_iter = polish_from >= 0 ? L.iter - polish_from : 0;
+    if (ktimer) {
+        ktimer->flush();
+        res->kernel_calls = ktimer->count;
+        res->kernel_ms = ktimer->count ? ktimer->total_ms / (double)ktimer->count : 0.0;
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -1489,7 +1558,12 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
     EventTimer tm(s);
     int best = a.lanes;
     double best_ms = INFINITY;
-    for (int L = 1; L <= 64; L *= 2) {
+    const double mean = a.nrow > 0 ? (double)a.nnz / (double)a.nrow : 0.0;
+    std::vector<int> cand;
+    for (int rb : {256, 128, 64})  // CSR-stream when a block's nonzeros fit in LDS
+        if (mean * rb <= 0.75 * kStreamCap) cand.push_back(-rb);
+    for (int L = 1; L <= 64; L *= 2) cand.push_back(L);
+    for (int L : cand) {
         const double ms = tm.avg_ms(reps, [&] {
             launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<VT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
         });
@@ -1502,9 +1576,12 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
 }
 
 static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool mixed) {
-    if (p->spmv_lanes > 0) {
-        HPRQP_REQUIRE((p->spmv_lanes & (p->spmv_lanes - 1)) == 0 && p->spmv_lanes <= 64, HPRQP_E_ARG,
-                      "spmv_lanes must be a power of two <= 64");
+    if (p->spmv_lanes != 0) {
+        const int L = p->spmv_lanes;
+        const bool vec = L > 0 && (L & (L - 1)) == 0 && L <= 64;
+        const bool stream = L == -64 || L == -128 || L == -256;
+        HPRQP_REQUIRE(vec || stream, HPRQP_E_ARG,
+                      "spmv_lanes must be 0, a power of two <= 64, or -64/-128/-256 (rows per block)");
         S.Q.lanes = S.A.lanes = S.AT.lanes = p->spmv_lanes;
         return;
     }

@@ -43,6 +43,7 @@ class hprqp_params(C.Structure):
         ("precision", C.c_int32),
         ("verbose", C.c_int32),
         ("spmv_lanes", C.c_int32),
+        ("time_kernels", C.c_int32),
     ]
 
 
@@ -74,6 +75,8 @@ class hprqp_result(C.Structure):
         ("polish_iter", C.c_int64),
         ("power_iter_A", C.c_int64),
         ("power_iter_Q", C.c_int64),
+        ("kernel_ms", C.c_double),
+        ("kernel_calls", C.c_int64),
     ]
 
 

@@ -50,6 +50,7 @@ class HPRQP_parameters:
     # MI355X-specific
     precision: str = "fp64"      # "fp64" (plain) or "mixed"
     spmv_lanes: int = 0          # 0 = automatic
+    time_kernels: bool = False   # HIP-event timing of the first fused step kernel
 
     def to_c(self) -> _lib.hprqp_params:
         p = _lib.hprqp_params()
@@ -76,6 +77,7 @@ class HPRQP_parameters:
         p.precision = PRECISIONS[self.precision]
         p.verbose = int(bool(self.verbose))
         p.spmv_lanes = int(self.spmv_lanes)
+        p.time_kernels = int(bool(self.time_kernels))
         return p
 
 
@@ -111,6 +113,8 @@ class HPRQP_results:
     polish_iter: int = 0
     power_iter_A: int = 0
     power_iter_Q: int = 0
+    kernel_ms: float = 0.0
+    kernel_calls: int = 0
     history: np.ndarray = field(default_factory=lambda: np.zeros((0, 8)))
 
 
@@ -228,7 +232,8 @@ class DeviceModel:
             err_Rd=r.err_Rd, lambda_max_A=r.lambda_max_A, lambda_max_Q=r.lambda_max_Q, sigma=r.sigma,
             sigma0=r.sigma0, restarts=r.restarts, Q_is_diag=bool(r.Q_is_diag), noC=bool(r.noC),
             setup_time=r.setup_time, polish_iter=r.polish_iter, power_iter_A=r.power_iter_A,
-            power_iter_Q=r.power_iter_Q, history=hist[: r.n_history].copy())
+            power_iter_Q=r.power_iter_Q, kernel_ms=r.kernel_ms, kernel_calls=r.kernel_calls,
+            history=hist[: r.n_history].copy())
 
     def profile(self, params: HPRQP_parameters, reps: int = 50):
         """Average ms of the fused kernels (hprqp_profile_step)."""

@@ -85,7 +85,11 @@ typedef struct {
     int32_t use_bc_scaling;     /* 0 (disabled in the reference as well) */
     int32_t precision;          /* HPRQP_PREC_FP64 */
     int32_t verbose;            /* 0 */
-    int32_t spmv_lanes;         /* 0 = autotune, else lanes per row (1..64) */
+    int32_t spmv_lanes;         /* 0 = autotune; 1..64 (power of two) = lanes
+                                   per row; -64/-128/-256 = CSR-stream with
+                                   that many rows per block */
+    int32_t time_kernels;       /* 1 = time the dominant fused kernel with
+                                   HIP events during the solve (result.kernel_ms) */
 } hprqp_params;
 
 /* Results (HPRQP_results, src/structs.jl:292-309) plus diagnostics. */
@@ -116,6 +120,9 @@ typedef struct {
     int64_t polish_iter;  /* FP64 polish iterations (mixed mode) */
     int64_t power_iter_A; /* power-method iterations for lambda_max(A A^T) */
     int64_t power_iter_Q; /* power-method iterations for lambda_max(Q) */
+    double kernel_ms;     /* average ms of the first fused step kernel when
+                             params.time_kernels = 1 (0 otherwise) */
+    int64_t kernel_calls; /* launches averaged into kernel_ms */
 } hprqp_result;
 
 /* User Q operator plugin (AbstractQOperator, Q_operator_interface.jl:33-92).

@@ -95,6 +95,17 @@ def test_grid_qp_small():
     _check(res, ref)
 
 
+@pytest.mark.parametrize("lanes", [1, 4, 64, -64, -256])
+def test_every_spmv_kernel_shape_gives_the_same_solution(lanes):
+    """Vector kernels (1..64 lanes per row) and CSR-stream kernels (rows per
+    block) change only the summation order inside a row."""
+    from hprqp import instances
+
+    model = instances.random_sparse_qp(1500, 700, seed=11)
+    res, ref = _solve_both(model, spmv_lanes=lanes)
+    _check(res, ref)
+
+
 def test_lasso_small():
     from hprqp import instances
 
