@@ -150,6 +150,8 @@ void hprqp_params_default(hprqp_params *p) {
     p->precision = HPRQP_PREC_FP64;
     p->verbose = 0;
     p->spmv_lanes = 0;
+    p->time_kernels = 0;
+    p->polish_stall_checks = 40;
 }
 
 int hprqp_ctx_create(int device, hprqp_ctx **out) {

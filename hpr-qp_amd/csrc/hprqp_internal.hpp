@@ -148,8 +148,9 @@ struct hprqp_model {
     // Q pattern symmetric completion and transpose position map (CSR Q only)
     hprqp::HostCsr Qsym;
     std::vector<int32_t> q_tpos;
-    // SpMV lanes chosen by the autotuner (0 = not tuned yet): Q, A, A^T
-    mutable int tuned_lanes[3] = {0, 0, 0};
+    // SpMV lanes chosen by the autotuner (0 = not tuned yet), per matrix
+    // value precision (FP64, FP32) for Q, A, A^T
+    mutable int tuned_lanes[2][3] = {{0, 0, 0}, {0, 0, 0}};
 };
 
 namespace hprqp {

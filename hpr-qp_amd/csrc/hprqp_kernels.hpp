@@ -138,10 +138,11 @@ struct EpiStoreScaled {
 // ---------------------------------------------------------------------
 
 // z/x/w1 step: update_zxw1 (src/kernels.jl:422-537, CPU :2301-2397)
-template <class VT, bool FULL, bool DIAG>
+template <class VT, class DT, bool FULL, bool DIAG>
 struct EpiZxw1 {
     VT *Qw;  // written when non-null (the product Q w)
-    const VT *ATy, *c, *l, *u, *w, *fact1, *fact2;
+    const VT *ATy, *w, *fact1, *fact2;
+    const DT *c, *l, *u;
     const double *last_x;
     VT *x, *x_hat, *w_bar, *dx, *x_bar, *z_bar;
     double sigma, f1s, f2s, h1, h2;
@@ -182,9 +183,9 @@ struct EpiTempv {
 };
 
 // dual step: update_y (src/kernels.jl:760-862, CPU :2682-2743)
-template <class VT, bool FULL>
+template <class VT, class DT, bool FULL>
 struct EpiY {
-    const VT *AL, *AU;
+    const DT *AL, *AU;
     const double *last_y;
     VT *y, *y_bar, *s, *dy;
     double fact1, fact2, h1, h2;
@@ -232,10 +233,11 @@ struct EpiW2 {
 };
 
 // noC z/x/w step: product is A^T y (src/kernels.jl:540-669, CPU :2117-2209)
-template <class VT, bool FULL, bool DIAG>
+template <class VT, class DT, bool FULL, bool DIAG>
 struct EpiNocZxw {
     VT *ATy;
-    const VT *Qw, *c, *l, *u, *fact1, *fact2;
+    const VT *Qw, *fact1, *fact2;
+    const DT *c, *l, *u;
     const double *last_x, *last_w;
     VT *x, *x_hat, *w, *w_bar, *dx, *x_bar, *z_bar, *dw;
     double sigma, f1s, f2s, h1, h2;
@@ -281,10 +283,11 @@ struct EpiNocTempv {
 };
 
 // LASSO step: product is Q w = A^T (A w) (src/kernels.jl:261-376)
-template <class VT, bool FULL>
+template <class VT, class DT, bool FULL>
 struct EpiLasso {
     VT *Qw;
-    const VT *ATy, *c;
+    const VT *ATy;
+    const DT *c;
     const double *last_x, *last_w;
     VT *x, *x_hat, *w, *w_bar, *dx, *x_bar, *z_bar, *dw;
     double sigma, lambda_sigma, f1s, f2s, h1, h2;
@@ -314,10 +317,10 @@ struct EpiLasso {
 };
 
 // LP z/x step: product is A^T y (src/kernels.jl:1243-1351)
-template <class VT, bool FULL>
+template <class VT, class DT, bool FULL>
 struct EpiLpZx {
     VT *ATy;
-    const VT *c, *l, *u;
+    const DT *c, *l, *u;
     const double *last_x;
     VT *x, *x_hat, *dx, *x_bar, *z_bar;
     double sigma, h1, h2;

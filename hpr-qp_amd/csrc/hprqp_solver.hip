@@ -359,7 +359,7 @@ static void randn_stream(uint64_t seed, int64_t count, double *out) {
 // ---------------------------------------------------------------------
 // Solver<VT>
 // ---------------------------------------------------------------------
-template <class VT>
+template <class VT, class MT = VT>
 struct Solver {
     hprqp_ctx *ctx;
     hipStream_t s;
@@ -377,8 +377,8 @@ struct Solver {
     // FP64 residual temporaries
     DevVec<double> Qx_d, ATdy_d, Ax_d, xd_tmp, qtemp_d;
     // data in VT (aliases of the FP64 arrays when VT == double)
-    DevVec<VT> c_own, l_own, u_own, AL_own, AU_own;
-    const VT *cV, *lV, *uV, *ALV, *AUV;
+    DevVec<MT> c_own, l_own, u_own, AL_own, AU_own;
+    const MT *cV, *lV, *uV, *ALV, *AUV;
     double sigma = 1.0, lambda_max_A = 0.0, lambda_max_Q = 0.0, lambda = 0.0;
     bool noC = false;
     KernelTimer *kt = nullptr;  // times the first fused kernel of partial steps
@@ -418,18 +418,18 @@ struct Solver {
             qtemp.alloc(n);
             qtemp_d.alloc(n);
         }
-        if constexpr (std::is_same<VT, double>::value) {
+        if constexpr (std::is_same<MT, double>::value) {
             cV = S.c.p;
             lV = S.l.p;
             uV = S.u.p;
             ALV = S.AL.p;
             AUV = S.AU.p;
         } else {
-            auto cvt = [&](DevVec<VT> &dst, const DevVec<double> &src) {
+            auto cvt = [&](DevVec<MT> &dst, const DevVec<double> &src) {
                 dst.alloc(src.n);
                 const double *a = src.p;
-                VT *b = dst.p;
-                each(s, src.n, [=] __device__(int64_t i) { b[i] = (VT)a[i]; });
+                MT *b = dst.p;
+                each(s, src.n, [=] __device__(int64_t i) { b[i] = (MT)a[i]; });
             };
             cvt(c_own, S.c);
             cvt(l_own, S.l);
@@ -459,12 +459,12 @@ struct Solver {
                 if (S.q_empty) {
                     HPRQP_HIP(hipMemsetAsync(out, 0, sizeof(VT) * n, s));
                 } else {
-                    csr_apply<VT, VT>(s, S.Q, xin, EpiStore<VT>{out});
+                    csr_apply<VT, MT>(s, S.Q, xin, EpiStore<VT>{out});
                 }
                 break;
             case QOP_LASSO:
-                csr_apply<VT, VT>(s, S.qop->la, xin, EpiStore<VT>{qtemp.p});
-                csr_apply<VT, VT>(s, S.qop->lat, (const VT *)qtemp.p, EpiStore<VT>{out});
+                csr_apply<VT, MT>(s, S.qop->la, xin, EpiStore<VT>{qtemp.p});
+                csr_apply<VT, MT>(s, S.qop->lat, (const VT *)qtemp.p, EpiStore<VT>{out});
                 break;
             case QOP_QAP:
                 if constexpr (std::is_same<VT, double>::value)
@@ -494,10 +494,10 @@ struct Solver {
     template <class Epi>
     void q_fused(const VT *xin, const Epi &epi, VT *qbuf) {
         if (S.qkind == QOP_CSR && !S.q_empty) {
-            csr_apply<VT, VT>(s, S.Q, xin, epi);
+            csr_apply<VT, MT>(s, S.Q, xin, epi);
         } else if (S.qkind == QOP_LASSO) {
-            csr_apply<VT, VT>(s, S.qop->la, xin, EpiStore<VT>{qtemp.p});
-            csr_apply<VT, VT>(s, S.qop->lat, (const VT *)qtemp.p, epi);
+            csr_apply<VT, MT>(s, S.qop->la, xin, EpiStore<VT>{qtemp.p});
+            csr_apply<VT, MT>(s, S.qop->lat, (const VT *)qtemp.p, epi);
         } else {
             q_apply(xin, qbuf);
             vec_apply<VT, VT>(s, n, (const VT *)qbuf, epi);
@@ -555,14 +555,22 @@ struct Solver {
         for (auto &v : z0) v += 1e-8;
         DevVec<double> z(len), q(len), ATq(which == 0 ? n : 0), qd_tmp(which == 1 ? n : 0);
         HPRQP_HIP(hipMemcpyAsync(z.p, z0.data(), sizeof(double) * len, hipMemcpyHostToDevice, s));
+        // device scalars: 0 |z|^2, 1 lambda, 2 |q|^2, 3 |z|^2, 4 done, 5 lambda
+        // at the stop, 6 stop iteration, 7 error.  Every kernel after the
+        // converging iteration is a no-op, so the result is the reference's
+        // (first iteration with err < tol) while the host polls rarely.
+        double *d = red.out.p + 32;
+        HPRQP_HIP(hipMemsetAsync(d, 0, sizeof(double) * 8, s));
+        constexpr int kPoll = 16;
         double lambda_max = 1.0;
         for (int64_t it = 1; it <= max_it; ++it) {
-            red.enqueue<1, 0>(len, SumSq{z.p}, 0);
-            double nz = std::sqrt(red.fetch(1)[0]);
+            red.enqueue<1, 0>(len, SumSq{z.p}, 32);
             {
                 double *qq = q.p;
                 const double *zz = z.p;
-                each(s, len, [=] __device__(int64_t i) { qq[i] = zz[i] / nz; });
+                each(s, len, [=] __device__(int64_t i) {
+                    if (d[4] == 0.0) qq[i] = zz[i] / sqrt(d[0]);
+                });
             }
             if (which == 0) {
                 csr_apply<double, double>(s, S.AT, (const double *)q.p, EpiStore<double>{ATq.p});
@@ -570,24 +578,43 @@ struct Solver {
             } else {
                 qop_fp64(q.p, z.p);
             }
-            red.enqueue<1, 0>(len, Dot2{q.p, z.p}, 0);
-            lambda_max = red.fetch(1)[0];
-            {
-                double *qq = q.p;
-                const double *zz = z.p;
-                const double lm = lambda_max;
-                each(s, len, [=] __device__(int64_t i) { qq[i] = zz[i] - lm * qq[i]; });
+            red.enqueue<1, 0>(len, Dot2{q.p, z.p}, 33);
+            red.enqueue<2, 0>(len, PowerResid{q.p, z.p, d}, 34);
+            const double itd = (double)it, tl = tol;
+            k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) {
+                if (d[4] != 0.0) return;
+                const double err = sqrt(d[2]) / (sqrt(d[3]) + d[1]);
+                d[5] = d[1];
+                d[6] = itd;
+                d[7] = err;
+                if (err < tl) d[4] = 1.0;
+            });
+            HPRQP_HIP(hipGetLastError());
+            if (it % kPoll == 0 || it == max_it) {
+                HPRQP_HIP(hipMemcpyAsync(red.host, d, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+                HPRQP_HIP(hipStreamSynchronize(s));
+                lambda_max = red.host[5];
+                *its = (int64_t)red.host[6];
+                if (red.host[4] != 0.0) return lambda_max;
             }
-            red.enqueue<1, 0>(len, SumSq{q.p}, 0);
-            red.enqueue<1, 0>(len, SumSq{z.p}, 1);
-            const double *h = red.fetch(2);
-            const double err = std::sqrt(h[0]) / (std::sqrt(h[1]) + lambda_max);
-            *its = it;
-            if (err < tol) return lambda_max;
         }
         if (p->verbose) std::printf("Power iteration did not converge within %lld iterations\n", (long long)max_it);
         return lambda_max;
     }
+    // q = z - lambda q (skipped after convergence), and |q|^2, |z|^2
+    struct PowerResid {
+        double *q;
+        const double *z, *d;
+        __device__ void operator()(int64_t i, double *s, double *) const {
+            double qi = q[i];
+            if (d[4] == 0.0) {
+                qi = z[i] - d[1] * qi;
+                q[i] = qi;
+            }
+            s[0] += qi * qi;
+            s[1] += z[i] * z[i];
+        }
+    };
 
     // FP64 operator apply on FP64 vectors (power method)
     void qop_fp64(const double *xin, double *out) {
@@ -613,7 +640,7 @@ struct Solver {
     // ---------------- steps ------------------------------------------
     template <bool FULL, bool DIAG>
     void step_zxw1(double h1, double h2) {
-        EpiZxw1<VT, FULL, DIAG> e;
+        EpiZxw1<VT, MT, FULL, DIAG> e;
         e.Qw = Qw.p;
         e.ATy = ATy.p;
         e.c = cV;
@@ -645,7 +672,7 @@ struct Solver {
     template <bool FULL>
     void step_y(const VT *t, double h1, double h2) {
         if (m == 0) return;
-        EpiY<VT, FULL> e;
+        EpiY<VT, MT, FULL> e;
         e.AL = ALV;
         e.AU = AUV;
         e.last_y = last_y.p;
@@ -657,7 +684,7 @@ struct Solver {
         e.fact2 = 1.0 / e.fact1;
         e.h1 = h1;
         e.h2 = h2;
-        csr_apply<VT, VT>(s, S.A, t, e);
+        csr_apply<VT, MT>(s, S.A, t, e);
     }
     template <bool FULL, bool DIAG>
     void step_w2(double h1, double h2) {
@@ -674,11 +701,11 @@ struct Solver {
         e.fs = sigma / (1.0 + sigma * lambda_max_Q);
         e.h1 = h1;
         e.h2 = h2;
-        csr_apply<VT, VT>(s, S.AT, (const VT *)y_bar.p, e);
+        csr_apply<VT, MT>(s, S.AT, (const VT *)y_bar.p, e);
     }
     template <bool FULL, bool DIAG>
     void step_noc(double h1, double h2) {
-        EpiNocZxw<VT, FULL, DIAG> e;
+        EpiNocZxw<VT, MT, FULL, DIAG> e;
         e.ATy = ATy.p;
         e.Qw = Qw.p;
         e.c = cV;
@@ -702,7 +729,7 @@ struct Solver {
         e.h1 = h1;
         e.h2 = h2;
         kbegin();
-        csr_apply<VT, VT>(s, S.AT, (const VT *)y.p, e);
+        csr_apply<VT, MT>(s, S.AT, (const VT *)y.p, e);
         kend();
         EpiNocTempv<VT> e2{x_hat.p, last_Qw.p, Qw.p, Qw_bar.p, tempv.p, sigma, h1, h2};
         q_fused(w_bar.p, e2, Qw_bar.p);
@@ -710,7 +737,7 @@ struct Solver {
     }
     template <bool FULL>
     void step_lasso(double h1, double h2) {
-        EpiLasso<VT, FULL> e;
+        EpiLasso<VT, MT, FULL> e;
         e.Qw = Qw.p;
         e.ATy = ATy.p;
         e.c = cV;
@@ -736,7 +763,7 @@ struct Solver {
     }
     template <bool FULL>
     void step_lp(double h1, double h2) {
-        EpiLpZx<VT, FULL> e;
+        EpiLpZx<VT, MT, FULL> e;
         e.ATy = ATy.p;
         e.c = cV;
         e.l = lV;
@@ -751,7 +778,7 @@ struct Solver {
         e.h1 = h1;
         e.h2 = h2;
         kbegin();
-        csr_apply<VT, VT>(s, S.AT, (const VT *)y.p, e);
+        csr_apply<VT, MT>(s, S.AT, (const VT *)y.p, e);
         kend();
         step_y<FULL>(x_hat.p, h1, h2);
     }
@@ -883,7 +910,7 @@ struct Solver {
     double compute_M_norm() {
         q_apply(dw.p, dQw.p);
         if (m > 0) {
-            csr_apply<VT, VT>(s, S.AT, (const VT *)dy.p, EpiStore<VT>{ATdy.p});
+            csr_apply<VT, MT>(s, S.AT, (const VT *)dy.p, EpiStore<VT>{ATdy.p});
             q_apply(ATdy.p, QATdy.p);
         }
         red.enqueue<7, 0>(n,
@@ -1044,8 +1071,8 @@ struct SigmaDots {
 };
 
 // update_sigma! (src/algorithm.jl:323-423)
-template <class VT>
-static void update_sigma(Solver<VT> &W, Restart &r, double err_Rd, double err_Rp, double rel_gap) {
+template <class VT, class MT>
+static void update_sigma(Solver<VT, MT> &W, Restart &r, double err_Rd, double err_Rp, double rel_gap) {
     const hprqp_params *p = W.p;
     if (p->sigma_fixed || r.restart_flag < 1) return;
     hipStream_t s = W.s;
@@ -1073,7 +1100,7 @@ static void update_sigma(Solver<VT> &W, Restart &r, double err_Rd, double err_Rp
                                  SigmaDots<VT>{W.dx.p, W.dQw.p, m > 0 ? W.ATdy.p : nullptr,
                                                m > 0 ? W.QATdy.p : nullptr, W.dw.p},
                                  0);
-    W.red.template enqueue<1, 0>(m, typename Solver<VT>::SumSqV{W.dy.p}, 5);
+    W.red.template enqueue<1, 0>(m, typename Solver<VT, MT>::SumSqV{W.dy.p}, 5);
     const double *h = W.red.fetch(6);
     const double dxdx = h[0], dQwATdy = h[1], dQwdQw = h[2], dwdQw = h[3], ATdyQATdy = h[4], dydy = h[5];
     double a = 0.0, b = dxdx, c = 0.0, d = 0.0;
@@ -1120,13 +1147,13 @@ static void update_sigma(Solver<VT> &W, Restart &r, double err_Rd, double err_Rp
 }
 
 // do_restart! (src/algorithm.jl:522-555)
-template <class VT>
-static void do_restart(Solver<VT> &W, Restart &r) {
+template <class VT, class MT>
+static void do_restart(Solver<VT, MT> &W, Restart &r) {
     if (r.restart_flag <= 0) return;
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
     if (m > 0) {
-        csr_apply<VT, VT>(s, W.S.AT, (const VT *)W.y_bar.p, EpiStore<VT>{W.ATy_bar.p});
+        csr_apply<VT, MT>(s, W.S.AT, (const VT *)W.y_bar.p, EpiStore<VT>{W.ATy_bar.p});
         VT *y = W.y.p;
         const VT *yb = W.y_bar.p;
         double *ly = W.last_y.p;
@@ -1162,8 +1189,8 @@ static void do_restart(Solver<VT> &W, Restart &r) {
 }
 
 // process_initial_points! (src/algorithm.jl:1783-1836)
-template <class VT>
-static void process_initial_points(Solver<VT> &W, const double *x0, const double *y0) {
+template <class VT, class MT>
+static void process_initial_points(Solver<VT, MT> &W, const double *x0, const double *y0) {
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
     const double bs = W.S.b_scale, cs = W.S.c_scale;
@@ -1192,7 +1219,7 @@ static void process_initial_points(Solver<VT> &W, const double *x0, const double
                 y[i] = yb[i] = (VT)v;
                 ly[i] = (double)(VT)v;
             });
-            csr_apply<VT, VT>(s, W.S.AT, (const VT *)W.y_bar.p, EpiStore<VT>{W.ATy_bar.p});
+            csr_apply<VT, MT>(s, W.S.AT, (const VT *)W.y_bar.p, EpiStore<VT>{W.ATy_bar.p});
             VT *aty = W.ATy.p;
             const VT *atyb = W.ATy_bar.p;
             double *laty = W.last_ATy.p;
@@ -1205,7 +1232,8 @@ static void process_initial_points(Solver<VT> &W, const double *x0, const double
         W.q_apply_d(W.x_bar.p, W.Qx_d.p);
         {
             const double *qx = W.Qx_d.p;
-            const VT *xb = W.x_bar.p, *atyb = W.ATy_bar.p, *c = W.cV;
+            const VT *xb = W.x_bar.p, *atyb = W.ATy_bar.p;
+            const MT *c = W.cV;
             VT *zb = W.z_bar.p;
             const double sg = W.sigma;
             each(s, n, [=] __device__(int64_t i) {
@@ -1216,7 +1244,8 @@ static void process_initial_points(Solver<VT> &W, const double *x0, const double
         if (m > 0) {
             csr_apply<double, double>(s, W.S.A, (const VT *)W.x_bar.p, EpiStore<double>{W.Ax_d.p});
             const double *ax = W.Ax_d.p;
-            const VT *y = W.y.p, *al = W.ALV, *au = W.AUV;
+            const VT *y = W.y.p;
+            const MT *al = W.ALV, *au = W.AUV;
             VT *sv = W.s_.p;
             const double f1 = W.lambda_max_A * W.sigma;
             each(s, m, [=] __device__(int64_t i) {
@@ -1235,10 +1264,11 @@ static int64_t print_step(int64_t k) {
 }
 
 // copy the FP32 state of a mixed solve into the FP64 solver (polish start)
-static void promote_state(Solver<float> &F, Solver<double> &D) {
+template <class VT, class MT>
+static void promote_state(Solver<VT, MT> &F, Solver<double> &D) {
     hipStream_t s = D.s;
-    auto cp = [&](DevVec<double> &dst, const DevVec<float> &src) {
-        const float *a = src.p;
+    auto cp = [&](DevVec<double> &dst, const DevVec<VT> &src) {
+        const VT *a = src.p;
         double *b = dst.p;
         each(s, src.n, [=] __device__(int64_t i) { b[i] = (double)a[i]; });
     };
@@ -1289,9 +1319,9 @@ struct LoopState {
     int stall = 0;
 };
 
-template <class VT>
-static int run_loop(Solver<VT> &W, LoopState &L, int64_t iter_begin, double t_start, double obj_constant,
-                    bool lasso, double *hist, int64_t hist_cap, bool mixed, typename Solver<VT>::Resid &res,
+template <class VT, class MT>
+static int run_loop(Solver<VT, MT> &W, LoopState &L, int64_t iter_begin, double t_start, double obj_constant,
+                    bool lasso, double *hist, int64_t hist_cap, bool mixed, typename Solver<VT, MT>::Resid &res,
                     bool has_res) {
     const hprqp_params *p = W.p;
     Restart &r = L.r;
@@ -1310,7 +1340,7 @@ static int run_loop(Solver<VT> &W, LoopState &L, int64_t iter_begin, double t_st
                 if (res.kkt < 0.5 * L.best_kkt) {
                     L.best_kkt = res.kkt;
                     L.stall = 0;
-                } else if (++L.stall >= 6 && res.kkt < 1e-3) {
+                } else if (++L.stall >= std::max(1, p->polish_stall_checks) && res.kkt < 1e-3) {
                     L.iter = iter;
                     return -1;
                 }
@@ -1364,8 +1394,8 @@ static int run_loop(Solver<VT> &W, LoopState &L, int64_t iter_begin, double t_st
     return HPRQP_STATUS_MAX_ITER;
 }
 
-template <class VT>
-static void collect(Solver<VT> &W, double *x_out, double *y_out, double *z_out, double *w_out) {
+template <class VT, class MT>
+static void collect(Solver<VT, MT> &W, double *x_out, double *y_out, double *z_out, double *w_out) {
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
     const double bs = W.S.b_scale, cs = W.S.c_scale;
@@ -1386,8 +1416,8 @@ static void collect(Solver<VT> &W, double *x_out, double *y_out, double *z_out, 
     dl(W.y_bar.p, W.S.row_norm.p, m, cs, 0, y_out);
 }
 
-template <class VT>
-static void init_solver(Solver<VT> &W, const hprqp_model *md) {
+template <class VT, class MT>
+static void init_solver(Solver<VT, MT> &W, const hprqp_model *md) {
     W.alloc();
     W.lambda = md->lambda;
 }
@@ -1402,14 +1432,15 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     HPRQP_REQUIRE(p->print_frequency == -1 || p->print_frequency > 0, HPRQP_E_ARG,
                   "print_frequency must be -1 or positive");
     HPRQP_REQUIRE(p->sigma == -1 || p->sigma > 0, HPRQP_E_ARG, "sigma must be positive or -1");
-    HPRQP_REQUIRE(p->precision == HPRQP_PREC_FP64 || p->precision == HPRQP_PREC_MIXED, HPRQP_E_ARG,
-                  "unknown precision mode");
+    HPRQP_REQUIRE(p->precision == HPRQP_PREC_FP64 || p->precision == HPRQP_PREC_MIXED ||
+                      p->precision == HPRQP_PREC_FP32,
+                  HPRQP_E_ARG, "unknown precision mode");
     const double setup_start = now_s();
     Reducer red(ctx);
     Scaled S;
     build_scaled(ctx, md, p, red, S);
     check_Q_diagonal(red, S);
-    const bool mixed = p->precision == HPRQP_PREC_MIXED;
+    const bool mixed = p->precision != HPRQP_PREC_FP64;
     apply_lanes(ctx->stream, md, p, S, mixed);
     const bool lasso = p->problem_type == HPRQP_PROBLEM_LASSO;
     std::unique_ptr<KernelTimer> ktimer;
@@ -1459,31 +1490,40 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
         if (S.Q_is_diag) D.update_Q_factors();
         status = run_loop(D, L, 0, t_start, md->obj_constant, lasso, hist, hist_cap, false, R, false);
     } else {
-        Solver<float> F(ctx, S, red, p);
-        init_solver(F, md);
-        F.kt = ktimer.get();
-        F.sigma = sigma0;
-        F.lambda_max_A = D.lambda_max_A;
-        F.lambda_max_Q = D.lambda_max_Q;
-        F.noC = noC;
-        process_initial_points(F, x0, y0);
-        if (S.Q_is_diag) F.update_Q_factors();
-        Solver<float>::Resid RF;
-        status = run_loop(F, L, 0, t_start, md->obj_constant, lasso, hist, hist_cap, true, RF, false);
-        if (status == -1) {
-            // FP64 polish: continue the same iteration from the FP32 state
-            promote_state(F, D);
-            polish_from = L.iter;
-            status = run_loop(D, L, L.iter, t_start, md->obj_constant, lasso, hist, hist_cap, false, R, false);
+        // reduced-precision run, then (only if it stalls) the FP64 polish
+        // continuing the same iteration from its state
+        auto lowp = [&](auto &F) {
+            init_solver(F, md);
+            F.kt = ktimer.get();
+            F.sigma = sigma0;
+            F.lambda_max_A = D.lambda_max_A;
+            F.lambda_max_Q = D.lambda_max_Q;
+            F.noC = noC;
+            process_initial_points(F, x0, y0);
+            if (S.Q_is_diag) F.update_Q_factors();
+            typename std::remove_reference<decltype(F)>::type::Resid RF;
+            status = run_loop(F, L, 0, t_start, md->obj_constant, lasso, hist, hist_cap, true, RF, false);
+            if (status == -1) {
+                promote_state(F, D);
+                polish_from = L.iter;
+                status = run_loop(D, L, L.iter, t_start, md->obj_constant, lasso, hist, hist_cap, false, R, false);
+            } else {
+                R.err_Rp = RF.err_Rp;
+                R.err_Rd = RF.err_Rd;
+                R.kkt = RF.kkt;
+                R.primal_obj = RF.primal_obj;
+                R.rel_gap = RF.rel_gap;
+                R.dual_obj = RF.dual_obj;
+                collect(F, x_out, y_out, z_out, w_out);
+                res->sigma = F.sigma;
+            }
+        };
+        if (p->precision == HPRQP_PREC_MIXED) {
+            Solver<double, float> F(ctx, S, red, p);
+            lowp(F);
         } else {
-            R.err_Rp = RF.err_Rp;
-            R.err_Rd = RF.err_Rd;
-            R.kkt = RF.kkt;
-            R.primal_obj = RF.primal_obj;
-            R.rel_gap = RF.rel_gap;
-            R.dual_obj = RF.dual_obj;
-            collect(F, x_out, y_out, z_out, w_out);
-            res->sigma = F.sigma;
+            Solver<float, float> F(ctx, S, red, p);
+            lowp(F);
         }
     }
     HPRQP_HIP(hipStreamSynchronize(ctx->stream));
@@ -1589,20 +1629,21 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
     for (int k = 0; k < 3; ++k) {
         DevCsr &a = *mats[k];
         if (a.nnz < (1 << 18)) continue;  // heuristic choice is already close
-        if (md->tuned_lanes[k] == 0) {
+        int &tuned = md->tuned_lanes[mixed ? 1 : 0][k];
+        if (tuned == 0) {
             if (mixed) {
                 ensure_v32(a);
-                md->tuned_lanes[k] = tune_lanes<float>(s, a, 10);
+                tuned = tune_lanes<float>(s, a, 10);
             } else {
-                md->tuned_lanes[k] = tune_lanes<double>(s, a, 10);
+                tuned = tune_lanes<double>(s, a, 10);
             }
         }
-        a.lanes = md->tuned_lanes[k];
+        a.lanes = tuned;
     }
 }
 
-template <class VT>
-static void profile_impl(Solver<VT> &W, int reps, double *ms, int *nout) {
+template <class VT, class MT>
+static void profile_impl(Solver<VT, MT> &W, int reps, double *ms, int *nout) {
     EventTimer tm(W.s);
     const bool standard = W.S.qkind != QOP_LASSO && !W.S.q_empty && !W.noC;
     for (int k = 0; k < 6; ++k) ms[k] = 0.0;
@@ -1617,7 +1658,7 @@ static void profile_impl(Solver<VT> &W, int reps, double *ms, int *nout) {
     }
     int64_t inner = 0;
     ms[4] = tm.avg_ms(reps, [&] { W.main_update(inner++, false); });
-    typename Solver<VT>::Resid r;
+    typename Solver<VT, MT>::Resid r;
     ms[5] = tm.avg_ms(std::max(1, reps / 10), [&] { W.compute_residuals(1, 0.0, false, r); });
     if (nout) *nout = 6;
 }
@@ -1628,7 +1669,7 @@ void profile_step(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, 
     Scaled S;
     build_scaled(ctx, md, p, red, S);
     check_Q_diagonal(red, S);
-    const bool mixed = p->precision == HPRQP_PREC_MIXED;
+    const bool mixed = p->precision != HPRQP_PREC_FP64;
     apply_lanes(ctx->stream, md, p, S, mixed);
     int64_t nfree = 0;
     for (int32_t i = 0; i < md->n; ++i)
@@ -1642,8 +1683,12 @@ void profile_step(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, 
         W.noC = noC;
         if (S.Q_is_diag) W.update_Q_factors();
     };
-    if (mixed) {
-        Solver<float> F(ctx, S, red, p);
+    if (p->precision == HPRQP_PREC_MIXED) {
+        Solver<double, float> F(ctx, S, red, p);
+        setup(F);
+        profile_impl(F, reps, ms_out, n_out);
+    } else if (p->precision == HPRQP_PREC_FP32) {
+        Solver<float, float> F(ctx, S, red, p);
         setup(F);
         profile_impl(F, reps, ms_out, n_out);
     } else {

@@ -44,6 +44,7 @@ class hprqp_params(C.Structure):
         ("verbose", C.c_int32),
         ("spmv_lanes", C.c_int32),
         ("time_kernels", C.c_int32),
+        ("polish_stall_checks", C.c_int32),
     ]
 
 

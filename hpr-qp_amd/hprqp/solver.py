@@ -18,7 +18,7 @@ from .model import AbstractQOperator, LASSO_Q_operator, QAP_Q_operator, QP_info_
 
 STATUS = {0: "OPTIMAL", 1: "MAX_ITER", 2: "TIME_LIMIT"}
 PROBLEM_TYPES = {"QP": 0, "LASSO": 1, "QAP": 2}
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp32": 2}
 
 
 @dataclass
@@ -48,9 +48,10 @@ class HPRQP_parameters:
     verbose: bool = True
     use_gpu: bool = True
     # MI355X-specific
-    precision: str = "fp64"      # "fp64" (plain) or "mixed"
+    precision: str = "fp64"      # "fp64" (plain), "mixed" or "fp32"
     spmv_lanes: int = 0          # 0 = automatic
     time_kernels: bool = False   # HIP-event timing of the first fused step kernel
+    polish_stall_checks: int = 40  # mixed: checks without KKT halving before FP64 polish
 
     def to_c(self) -> _lib.hprqp_params:
         p = _lib.hprqp_params()
@@ -78,6 +79,7 @@ class HPRQP_parameters:
         p.verbose = int(bool(self.verbose))
         p.spmv_lanes = int(self.spmv_lanes)
         p.time_kernels = int(bool(self.time_kernels))
+        p.polish_stall_checks = int(self.polish_stall_checks)
         return p
 
 

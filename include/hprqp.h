@@ -60,7 +60,11 @@ extern "C" {
 
 /* ---- precision modes ------------------------------------------------- */
 #define HPRQP_PREC_FP64 0   /* everything FP64 (plain form) */
-#define HPRQP_PREC_MIXED 1  /* FP32 bulk, FP64 anchors/KKT, FP64 polish */
+#define HPRQP_PREC_MIXED 1  /* FP32 matrix values and problem data (the bulk
+                               byte streams); FP64 iterates, anchors,
+                               accumulation and KKT */
+#define HPRQP_PREC_FP32 2   /* FP32 matrices, iterates and accumulation; FP64
+                               anchors and KKT; FP64 polish when FP32 stalls */
 
 typedef struct hprqp_ctx hprqp_ctx;
 typedef struct hprqp_buf hprqp_buf;
@@ -90,6 +94,9 @@ typedef struct {
                                    that many rows per block */
     int32_t time_kernels;       /* 1 = time the dominant fused kernel with
                                    HIP events during the solve (result.kernel_ms) */
+    int32_t polish_stall_checks; /* mixed mode: residual checks without the KKT
+                                    residual halving before the FP64 polish
+                                    takes over (default 40) */
 } hprqp_params;
 
 /* Results (HPRQP_results, src/structs.jl:292-309) plus diagnostics. */

@@ -106,6 +106,35 @@ def test_every_spmv_kernel_shape_gives_the_same_solution(lanes):
     _check(res, ref)
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp32"])
+@pytest.mark.parametrize("case", ["grid", "random", "aug3dcqp", "lasso"])
+def test_reduced_precision_reaches_fp64_accuracy(precision, case):
+    """Mixed (FP32 matrices and data, FP64 iterates) and FP32 (+FP64 polish)
+    forms: the KKT residual is evaluated in FP64 against the FP64 data, so
+    the same 1e-6 relative KKT bar applies; the objective must agree with
+    the FP64 oracle within 1e-6 relative."""
+    from hprqp import HPRQP_parameters, build_from_mps, instances, optimize
+    from oracle import oracle
+
+    ptype = "QP"
+    if case == "grid":
+        model = instances.grid_qp(80, seed=2)
+    elif case == "random":
+        model = instances.random_sparse_qp(2000, 1000, seed=4)
+    elif case == "aug3dcqp":
+        model = build_from_mps(fixture_path("AUG3DCQP.mps"), verbose=False)
+    else:
+        model = instances.lasso(200, 4000, mean_nnz=60, seed=1)
+        ptype = "LASSO"
+    res = optimize(model, HPRQP_parameters(verbose=False, precision=precision, problem_type=ptype))
+    ref = oracle.solve(model, problem_type=ptype)
+    rel = abs(res.primal_obj - ref.primal_obj) / (1.0 + abs(ref.primal_obj))
+    print(f"{precision} {case}: {res.status} iter={res.iter} polish={res.polish_iter} kkt={res.residuals:.2e} "
+          f"rel_obj_err_vs_fp64={rel:.2e} (oracle iter={ref.iter})")
+    assert res.status == "OPTIMAL" and res.residuals < 1e-6
+    assert rel < 1e-6
+
+
 def test_lasso_small():
     from hprqp import instances
 
