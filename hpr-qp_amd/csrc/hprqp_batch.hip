@@ -1,0 +1,984 @@
+// hprqp_batch.hip -- many HPR-QP instances of uneven sizes solved together.
+//
+// Each instance i:  min 1/2 x'(diag(d_i) + U_i U_i')x + c_i'x
+//                   s.t. AL_i <= A_i x <= AU_i,  l_i <= x <= u_i
+// with U_i dense n_i x k_i (column-major) and A_i sparse (CSR, with its
+// stored transpose).  Q_i is an operator, so -- as for every operator Q in
+// the reference (src/utils.jl:364-406) -- no scaling is applied.
+//
+// Device side: one 256-thread workgroup owns one instance for a whole check
+// interval (the iterations between two residual evaluations of the
+// reference's cadence, src/algorithm.jl:1977-2038): it runs the standard
+// z/x/w1 - tempv - y - w2 steps (src/kernels.jl:2301-2832) with block-level
+// reductions, the M-norm(s) the interval needs, the FP64 residuals at the
+// interval's end and the dot products a sigma update would need.  All
+// instances share the cadence, so one launch covers one interval for the
+// whole batch; retired instances return at once.
+// Host side: per-instance check_break / check_restart / update_sigma /
+// do_restart decisions from a 32-double record per instance
+// (src/algorithm.jl:323-576), exactly as the single-instance loop.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "hprqp_internal.hpp"
+#include "hprqp_kernels.hpp"
+
+namespace hprqp {
+
+static double bnow() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// per-instance control (host -> device) and record (device -> host)
+struct BCtl {
+    double sigma, lamA, lamQ, h_inner;  // h_inner: Halpern counter at interval start
+    int32_t active, K, restart, check_iter;
+    int64_t iter0;
+    int32_t want_last_gap, want_cur_gap, init, pad;
+};
+enum {
+    R_CX = 0, R_XQX, R_ZX, R_YS, R_RD, R_ATY, R_QX, R_RP, R_AX, R_BND,  // residual parts
+    R_LASTGAP, R_CURGAP,                                                // M-norms
+    R_DXDX, R_DQWATDY, R_DQWDQW, R_DWDQW, R_ATDYQATDY, R_DYDY,          // sigma dots
+    R_POW_A, R_POW_Q, R_POW_ITA, R_POW_ITQ,                             // power method
+    R_COUNT = 24
+};
+
+template <class MT>
+struct BArgs {
+    int32_t count;
+    const int32_t *n, *m, *k;
+    const int64_t *noff, *moff, *uoff, *arp_off, *atrp_off;
+    const MT *U, *d;            // iteration precision
+    const double *U64, *d64;    // FP64 (residuals, power method)
+    const int32_t *a_rp, *a_ci, *at_rp, *at_ci;  // packed CSR, row pointers local to the instance
+    const MT *a_v, *at_v;
+    const double *a_v64, *at_v64;
+    const double *c, *l, *u, *AL, *AU;
+    const MT *cM, *lM, *uM, *ALM, *AUM;
+    double *x, *w, *w_bar, *x_hat, *x_bar, *z_bar, *dx, *dw, *Qw, *Qw_bar, *tempv, *ATy, *ATy_bar, *ATdy, *QATdy,
+        *dQw;
+    double *y, *y_bar, *s, *dy;
+    double *last_x, *last_w, *last_y, *last_ATy;
+    double *pq, *pz;  // power method scratch (n-packed and m-packed)
+    BCtl *ctl;
+    double *rec;
+};
+
+// ---------------- block primitives (256 threads, 4 waves) ----------------
+__device__ __forceinline__ double bsum(double v, double *sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    const double r = sh[0] + sh[1] + sh[2] + sh[3];
+    return r;
+}
+__device__ __forceinline__ double bmax(double v, double *sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = dmax(v, __shfl_xor(v, off));
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    return dmax(dmax(sh[0], sh[1]), dmax(sh[2], sh[3]));
+}
+
+// t[j] = sum_i U[i + j n] v[i], j < k  (t in LDS).  Columns go in groups of
+// kG: each thread keeps kG partial sums, so one round of shuffles and one
+// barrier pair serve kG columns and the loads of a group are independent.
+constexpr int kG = 8;
+template <class UT, class VT>
+__device__ void lr_tvec(int32_t n, int32_t k, const UT *__restrict__ U, const VT *__restrict__ v, double *t,
+                        double *sh) {
+    __shared__ double part[kBlock / 64][kG];
+    for (int32_t j0 = 0; j0 < k; j0 += kG) {
+        double acc[kG];
+#pragma unroll
+        for (int g = 0; g < kG; ++g) acc[g] = 0.0;
+        const int gmax = min(kG, k - j0);
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+            const double vi = (double)v[i];
+            const UT *col = U + (int64_t)j0 * n + i;
+#pragma unroll
+            for (int g = 0; g < kG; ++g)
+                if (g < gmax) acc[g] += (double)col[(int64_t)g * n] * vi;
+        }
+#pragma unroll
+        for (int g = 0; g < kG; ++g) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) acc[g] += __shfl_xor(acc[g], off);
+        }
+        if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+            for (int g = 0; g < kG; ++g) part[threadIdx.x >> 6][g] = acc[g];
+        }
+        __syncthreads();
+        if (threadIdx.x < gmax) {
+            const int g = threadIdx.x;
+            t[j0 + g] = part[0][g] + part[1][g] + part[2][g] + part[3][g];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+}
+template <class UT>
+__device__ __forceinline__ double lr_row(int32_t n, int32_t k, const UT *__restrict__ U, int32_t i, const double *t) {
+    double acc = 0.0;
+    for (int32_t j = 0; j < k; ++j) acc += (double)U[i + (int64_t)j * n] * t[j];
+    return acc;
+}
+// Qv for all i into out: out_i = d_i v_i + U_i. t  (t = U'v already in LDS)
+template <class UT, class VT>
+__device__ void lr_apply(int32_t n, int32_t k, const UT *U, const UT *d, const VT *v, double *out, double *t,
+                         double *sh) {
+    lr_tvec(n, k, U, v, t, sh);
+    for (int32_t i = threadIdx.x; i < n; i += kBlock) out[i] = (double)d[i] * (double)v[i] + lr_row(n, k, U, i, t);
+    __syncthreads();
+}
+// y_r = sum A_r. v  for the instance's m rows: one wave per row
+template <class AT_, class VT>
+__device__ void csr_rows_wave(int32_t m, const int32_t *rp, const int32_t *ci, const AT_ *v, const VT *x,
+                              double *out) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int32_t r = wave; r < m; r += kBlock / 64) {
+        double acc = 0.0;
+        for (int32_t q = rp[r] + lane; q < rp[r + 1]; q += 64) acc += (double)v[q] * (double)x[ci[q]];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        if (lane == 0) out[r] = acc;
+    }
+    __syncthreads();
+}
+// transpose product, one thread per column j (rows of the stored transpose)
+template <class AT_, class VT>
+__device__ __forceinline__ double csr_row(const int32_t *rp, const int32_t *ci, const AT_ *v, const VT *x, int32_t j) {
+    double acc = 0.0;
+    for (int32_t q = rp[j]; q < rp[j + 1]; ++q) acc += (double)v[q] * (double)x[ci[q]];
+    return acc;
+}
+
+template <class MT>
+struct Inst {
+    int32_t n, m, k;
+    const MT *U, *d;
+    const double *U64, *d64;
+    const int32_t *arp, *aci, *atrp, *atci;
+    const MT *av, *atv;
+    const double *av64, *atv64;
+    const double *c, *l, *u, *AL, *AU;
+    const MT *cM, *lM, *uM, *ALM, *AUM;
+    double *x, *w, *w_bar, *x_hat, *x_bar, *z_bar, *dx, *dw, *Qw, *Qw_bar, *tempv, *ATy, *ATy_bar, *ATdy, *QATdy,
+        *dQw;
+    double *y, *y_bar, *s, *dy, *last_x, *last_w, *last_y, *last_ATy, *pq, *pz;
+};
+
+template <class MT>
+__device__ Inst<MT> inst_of(const BArgs<MT> &a, int b) {
+    Inst<MT> I;
+    I.n = a.n[b];
+    I.m = a.m[b];
+    I.k = a.k[b];
+    const int64_t no = a.noff[b], mo = a.moff[b], uo = a.uoff[b];
+    I.U = a.U + uo;
+    I.d = a.d + no;
+    I.U64 = a.U64 + uo;
+    I.d64 = a.d64 + no;
+    I.arp = a.a_rp + a.arp_off[b];
+    I.atrp = a.at_rp + a.atrp_off[b];
+    I.aci = a.a_ci;
+    I.atci = a.at_ci;
+    I.av = a.a_v;
+    I.atv = a.at_v;
+    I.av64 = a.a_v64;
+    I.atv64 = a.at_v64;
+    I.c = a.c + no;
+    I.l = a.l + no;
+    I.u = a.u + no;
+    I.AL = a.AL + mo;
+    I.AU = a.AU + mo;
+    I.cM = a.cM + no;
+    I.lM = a.lM + no;
+    I.uM = a.uM + no;
+    I.ALM = a.ALM + mo;
+    I.AUM = a.AUM + mo;
+#define P(f) I.f = a.f + no
+    P(x); P(w); P(w_bar); P(x_hat); P(x_bar); P(z_bar); P(dx); P(dw); P(Qw); P(Qw_bar); P(tempv); P(ATy);
+    P(ATy_bar); P(ATdy); P(QATdy); P(dQw); P(last_x); P(last_w); P(last_ATy); P(pq);
+#undef P
+    I.y = a.y + mo;
+    I.y_bar = a.y_bar + mo;
+    I.s = a.s + mo;
+    I.dy = a.dy + mo;
+    I.last_y = a.last_y + mo;
+    I.pz = a.pz + mo;
+    return I;
+}
+
+// one HPR step of the standard path (Q non-diagonal operator, noC false)
+template <class MT>
+__device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, double h1, double h2, bool full,
+                      double *t, double *sh) {
+    const int32_t n = I.n, m = I.m, k = I.k;
+    const double f2s = 1.0 / (1.0 + sigma * lamQ), f1s = 1.0 - f2s;
+    // z/x/w1: Qw = Q w
+    lr_tvec(n, k, I.U, I.w, t, sh);
+    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        const double wi = I.w[i], xi = I.x[i];
+        const double qw = (double)I.d[i] * wi + lr_row(n, k, I.U, i, t);
+        I.Qw[i] = qw;
+        const double z_raw = xi + sigma * (-qw + I.ATy[i] - (double)I.cM[i]);
+        const double xb = proj(z_raw, (double)I.lM[i], (double)I.uM[i]);
+        const double xh = 2.0 * xb - xi;
+        if (full) {
+            I.dx[i] = xb - xi;
+            I.x_bar[i] = xb;
+            I.z_bar[i] = (xb - z_raw) / sigma;
+        }
+        I.x[i] = fma(h2, xh, h1 * I.last_x[i]);
+        I.x_hat[i] = xh;
+        I.w_bar[i] = fma(f1s, wi, f2s * xh);
+    }
+    __syncthreads();
+    if (m > 0) {
+        // tempv = x_hat + sigma (Qw - Q w_bar)
+        lr_tvec(n, k, I.U, I.w_bar, t, sh);
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+            const double qwb = (double)I.d[i] * I.w_bar[i] + lr_row(n, k, I.U, i, t);
+            I.tempv[i] = I.x_hat[i] + sigma * (I.Qw[i] - qwb);
+        }
+        __syncthreads();
+        // y step: A tempv
+        csr_rows_wave(m, I.arp, I.aci, I.av, I.tempv, I.pz);
+        const double fa = lamA * sigma, fb = 1.0 / fa;
+        for (int32_t r = threadIdx.x; r < m; r += kBlock) {
+            const double yi = I.y[r];
+            const double s_raw = I.pz[r] - fa * yi;
+            const double sp = proj(s_raw, (double)I.ALM[r], (double)I.AUM[r]);
+            const double yb = fb * (sp - s_raw);
+            if (full) {
+                I.s[r] = sp;
+                I.dy[r] = yb - yi;
+            }
+            I.y_bar[r] = yb;
+            I.y[r] = fma(h2, 2.0 * yb - yi, h1 * I.last_y[r]);
+        }
+        __syncthreads();
+    }
+    // w2 step: A^T y_bar
+    const double fs = sigma / (1.0 + sigma * lamQ);
+    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        const double atyb = m > 0 ? csr_row(I.atrp, I.atci, I.atv, I.y_bar, i) : 0.0;
+        const double wi = I.w[i], wbi = I.w_bar[i], atyi = I.ATy[i];
+        const double wbn = wbi + fs * (atyb - atyi);
+        I.w[i] = h1 * I.last_w[i] + h2 * (2.0 * wbn - wi);
+        I.ATy[i] = h1 * I.last_ATy[i] + h2 * (2.0 * atyb - atyi);
+        if (full) {
+            I.w_bar[i] = wbn;
+            I.dw[i] = wbn - wi;
+            I.ATdy[i] = atyb - atyi;
+            I.ATy_bar[i] = atyb;
+        }
+    }
+    __syncthreads();
+}
+
+// compute_M_norm! (src/algorithm.jl:689-739), non-diagonal Q
+template <class MT>
+__device__ double bmnorm(Inst<MT> &I, double sigma, double lamA, double lamQ, double *t, double *sh) {
+    const int32_t n = I.n, m = I.m, k = I.k;
+    lr_apply(n, k, I.U, I.d, I.dw, I.dQw, t, sh);
+    if (m > 0) {
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) I.ATdy[i] = csr_row(I.atrp, I.atci, I.atv, I.dy, i);
+        __syncthreads();
+        lr_apply(n, k, I.U, I.d, I.ATdy, I.QATdy, t, sh);
+    }
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s6 = 0, s7 = 0;
+    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        const double a = I.dx[i], b = I.dQw[i], c = m > 0 ? I.ATdy[i] : 0.0, d = m > 0 ? I.QATdy[i] : 0.0,
+                     e = I.dw[i];
+        s0 += a * a;
+        s1 += b * a;
+        s2 += b * c;
+        s3 += c * a;
+        s4 += c * d;
+        s6 += e * b;
+    }
+    for (int32_t r = threadIdx.x; r < m; r += kBlock) s7 += I.dy[r] * I.dy[r];
+    s0 = bsum(s0, sh);
+    s1 = bsum(s1, sh);
+    s2 = bsum(s2, sh);
+    s3 = bsum(s3, sh);
+    s4 = bsum(s4, sh);
+    s6 = bsum(s6, sh);
+    s7 = bsum(s7, sh);
+    double M1 = 0.0, M3 = 0.0, M2 = (1.0 / sigma) * s0 - 2.0 * s1;
+    if (m > 0) {
+        M1 = sigma * lamA * s7 - 2.0 * sigma * s2;
+        M2 += 2.0 * s3;
+        M3 = (sigma * sigma) / (1.0 + sigma * lamQ) * s4;
+        M1 += sigma * lamQ * s6;
+    } else {
+        M1 = sigma * lamQ * s6;
+    }
+    M2 += dmax(M1, 0.0);
+    return sqrt(dmax(M2, 0.0) + dmax(M3, 0.0));
+}
+
+// residual parts in FP64 against the FP64 data (src/algorithm.jl:37-133)
+template <class MT>
+__device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *sh) {
+    const int32_t n = I.n, m = I.m, k = I.k;
+    // Qx (into dQw as scratch), A^T y_bar (into ATdy), A x_bar (into pz)
+    lr_apply(n, k, I.U64, I.d64, I.x_bar, I.dQw, t, sh);
+    if (m > 0) {
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) I.ATdy[i] = csr_row(I.atrp, I.atci, I.atv64, I.y_bar, i);
+        csr_rows_wave(m, I.arp, I.aci, I.av64, I.x_bar, I.pz);
+    }
+    double cx = 0, xqx = 0, zx = 0, rd = 0, aty = 0, qx = 0, bnd = 0, ys = 0, rp = 0, ax = 0;
+    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        const double x = I.x_bar[i], q = I.dQw[i], z = I.z_bar[i], ci = I.c[i], a = m > 0 ? I.ATdy[i] : 0.0;
+        cx += ci * x;
+        xqx += x * q;
+        zx += z * x;
+        rd = dmax(rd, fabs(q + ci - a - z));
+        aty = dmax(aty, fabs(a));
+        qx = dmax(qx, fabs(q));
+        if (iter0) bnd = dmax(bnd, dmax(I.l[i] - x, 0.0) + dmax(x - I.u[i], 0.0));
+    }
+    for (int32_t r = threadIdx.x; r < m; r += kBlock) {
+        ys += I.y_bar[r] * I.s[r];
+        const double v = I.pz[r];
+        rp = dmax(rp, fabs(proj(v, I.AL[r], I.AU[r]) - v));
+        ax = dmax(ax, fabs(v));
+    }
+    cx = bsum(cx, sh);
+    xqx = bsum(xqx, sh);
+    zx = bsum(zx, sh);
+    ys = bsum(ys, sh);
+    rd = bmax(rd, sh);
+    aty = bmax(aty, sh);
+    qx = bmax(qx, sh);
+    rp = bmax(rp, sh);
+    ax = bmax(ax, sh);
+    bnd = bmax(bnd, sh);
+    if (threadIdx.x == 0) {
+        rec[R_CX] = cx;
+        rec[R_XQX] = xqx;
+        rec[R_ZX] = zx;
+        rec[R_YS] = ys;
+        rec[R_RD] = rd;
+        rec[R_ATY] = aty;
+        rec[R_QX] = qx;
+        rec[R_RP] = rp;
+        rec[R_AX] = ax;
+        rec[R_BND] = bnd;
+    }
+    __syncthreads();
+}
+
+// the dot products of update_sigma! (src/algorithm.jl:332-364)
+template <class MT>
+__device__ void bsigma_dots(Inst<MT> &I, double *rec, double *t, double *sh) {
+    const int32_t n = I.n, m = I.m, k = I.k;
+    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        I.dx[i] = I.x_bar[i] - I.last_x[i];
+        I.dw[i] = I.w_bar[i] - I.last_w[i];
+        if (m > 0) I.ATdy[i] = I.ATy_bar[i] - I.last_ATy[i];
+    }
+    for (int32_t r = threadIdx.x; r < m; r += kBlock) I.dy[r] = I.y_bar[r] - I.last_y[r];
+    __syncthreads();
+    lr_apply(n, k, I.U, I.d, I.dw, I.dQw, t, sh);
+    if (m > 0) lr_apply(n, k, I.U, I.d, I.ATdy, I.QATdy, t, sh);
+    double a = 0, b = 0, c = 0, d = 0, e = 0, f = 0;
+    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        const double dxi = I.dx[i], dq = I.dQw[i], at = m > 0 ? I.ATdy[i] : 0.0, qat = m > 0 ? I.QATdy[i] : 0.0;
+        a += dxi * dxi;
+        b += dq * at;
+        c += dq * dq;
+        d += I.dw[i] * dq;
+        e += at * qat;
+    }
+    for (int32_t r = threadIdx.x; r < m; r += kBlock) f += I.dy[r] * I.dy[r];
+    a = bsum(a, sh);
+    b = bsum(b, sh);
+    c = bsum(c, sh);
+    d = bsum(d, sh);
+    e = bsum(e, sh);
+    f = bsum(f, sh);
+    if (threadIdx.x == 0) {
+        rec[R_DXDX] = a;
+        rec[R_DQWATDY] = b;
+        rec[R_DQWDQW] = c;
+        rec[R_DWDQW] = d;
+        rec[R_ATDYQATDY] = e;
+        rec[R_DYDY] = f;
+    }
+    __syncthreads();
+}
+
+// do_restart! (src/algorithm.jl:522-555)
+template <class MT>
+__device__ void brestart(Inst<MT> &I) {
+    const int32_t n = I.n, m = I.m;
+    for (int32_t r = threadIdx.x; r < m; r += kBlock) {
+        I.y[r] = I.y_bar[r];
+        I.last_y[r] = I.y_bar[r];
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        I.x[i] = I.x_bar[i];
+        I.w[i] = I.w_bar[i];
+        I.last_x[i] = I.x_bar[i];
+        I.last_w[i] = I.w_bar[i];
+        if (m > 0) {
+            const double a = csr_row(I.atrp, I.atci, I.atv, I.y_bar, i);
+            I.ATy_bar[i] = a;
+            I.last_ATy[i] = a;
+            I.ATy[i] = a;
+        }
+    }
+    __syncthreads();
+}
+
+static __device__ int64_t bprint_step(int64_t k) {
+    if (k <= 0) return 10;
+    int64_t p = 1;
+    while (p <= k / 10) p *= 10;
+    return p / 10 > 10 ? p / 10 : 10;
+}
+
+// one check interval for every active instance
+template <class MT>
+__global__ __launch_bounds__(kBlock) void k_batch_interval(BArgs<MT> a) {
+    __shared__ double t[64];
+    __shared__ double sh[4];
+    const int b = blockIdx.x;
+    const BCtl c = a.ctl[b];
+    if (!c.active) return;
+    Inst<MT> I = inst_of(a, b);
+    double *rec = a.rec + (int64_t)b * R_COUNT;
+    if (c.restart) brestart(I);
+    double inner = c.h_inner;
+    for (int s = 0; s < c.K; ++s) {
+        const int64_t it = c.iter0 + s, next = it + 1;
+        const bool full = (s == c.K - 1) || (s == 0 && c.restart) || (next % c.check_iter == 0) ||
+                          (next % bprint_step(next) == 0);
+        const double h1 = 1.0 / (inner + 2.0);
+        bstep(I, c.sigma, c.lamA, c.lamQ, h1, 1.0 - h1, full, t, sh);
+        if (s == 0 && c.want_last_gap) {
+            const double g = bmnorm(I, c.sigma, c.lamA, c.lamQ, t, sh);
+            if (threadIdx.x == 0) rec[R_LASTGAP] = g;
+        }
+        if (next % c.check_iter == 0) {
+            const double g = bmnorm(I, c.sigma, c.lamA, c.lamQ, t, sh);
+            if (threadIdx.x == 0) rec[R_CURGAP] = g;
+        }
+        inner += 1.0;
+    }
+    bresid(I, false, rec, t, sh);
+    bsigma_dots(I, rec, t, sh);
+}
+
+// iteration-0 residuals (with the bound-violation term) and warm start
+template <class MT>
+__global__ __launch_bounds__(kBlock) void k_batch_init(BArgs<MT> a, int has_x0, int has_y0) {
+    __shared__ double t[64];
+    __shared__ double sh[4];
+    const int b = blockIdx.x;
+    const BCtl c = a.ctl[b];
+    if (!c.active) return;
+    Inst<MT> I = inst_of(a, b);
+    double *rec = a.rec + (int64_t)b * R_COUNT;
+    const int32_t n = I.n, m = I.m;
+    // process_initial_points! (src/algorithm.jl:1783-1836): x0/y0 were
+    // copied into x_bar / y_bar by the host
+    if (has_x0) {
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+            const double v = I.x_bar[i];
+            I.x[i] = I.w[i] = I.w_bar[i] = I.last_x[i] = I.last_w[i] = v;
+        }
+    }
+    __syncthreads();
+    if (has_y0) {
+        for (int32_t r = threadIdx.x; r < m; r += kBlock) I.y[r] = I.last_y[r] = I.y_bar[r];
+        __syncthreads();
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+            const double v = m > 0 ? csr_row(I.atrp, I.atci, I.atv, I.y_bar, i) : 0.0;
+            I.ATy_bar[i] = I.ATy[i] = I.last_ATy[i] = v;
+        }
+        __syncthreads();
+        lr_apply(n, I.k, I.U, I.d, I.x_bar, I.dQw, t, sh);
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+            const double z_raw = I.x_bar[i] + c.sigma * (-I.dQw[i] + I.ATy_bar[i] - (double)I.cM[i]);
+            I.z_bar[i] = (I.x_bar[i] - z_raw) / c.sigma;
+        }
+        if (m > 0) {
+            csr_rows_wave(m, I.arp, I.aci, I.av, I.x_bar, I.pz);
+            for (int32_t r = threadIdx.x; r < m; r += kBlock)
+                I.s[r] = dmin(dmax(I.pz[r] - c.lamA * c.sigma * I.y[r], (double)I.ALM[r]), (double)I.AUM[r]);
+        }
+        __syncthreads();
+    }
+    bresid(I, true, rec, t, sh);
+}
+
+// per-instance power method (src/utils.jl:1335-1461) in one launch: each
+// block iterates to its own tolerance; start vectors were written by the host
+template <class MT>
+__global__ __launch_bounds__(kBlock) void k_batch_power(BArgs<MT> a, double tol, int max_it) {
+    __shared__ double t[64];
+    __shared__ double sh[4];
+    const int b = blockIdx.x;
+    if (!a.ctl[b].active) return;
+    Inst<MT> I = inst_of(a, b);
+    double *rec = a.rec + (int64_t)b * R_COUNT;
+    const int32_t n = I.n, m = I.m, k = I.k;
+    // lambda_max(A A^T): z (m) in pz, q (m) in s, A^T q (n) in pq
+    double lam = 0.0;
+    int it = 0;
+    if (m > 0) {
+        lam = 1.0;
+        for (it = 1; it <= max_it; ++it) {
+            double nz = 0.0;
+            for (int32_t r = threadIdx.x; r < m; r += kBlock) nz += I.pz[r] * I.pz[r];
+            nz = sqrt(bsum(nz, sh));
+            for (int32_t r = threadIdx.x; r < m; r += kBlock) I.s[r] = I.pz[r] / nz;
+            __syncthreads();
+            for (int32_t i = threadIdx.x; i < n; i += kBlock) I.pq[i] = csr_row(I.atrp, I.atci, I.atv64, I.s, i);
+            __syncthreads();
+            csr_rows_wave(m, I.arp, I.aci, I.av64, I.pq, I.pz);
+            double qz = 0.0;
+            for (int32_t r = threadIdx.x; r < m; r += kBlock) qz += I.s[r] * I.pz[r];
+            lam = bsum(qz, sh);
+            double e1 = 0.0, e2 = 0.0;
+            for (int32_t r = threadIdx.x; r < m; r += kBlock) {
+                const double q = I.pz[r] - lam * I.s[r];
+                I.s[r] = q;
+                e1 += q * q;
+                e2 += I.pz[r] * I.pz[r];
+            }
+            e1 = bsum(e1, sh);
+            e2 = bsum(e2, sh);
+            if (sqrt(e1) / (sqrt(e2) + lam) < tol) break;
+        }
+    }
+    if (threadIdx.x == 0) {
+        rec[R_POW_A] = lam;
+        rec[R_POW_ITA] = (double)(it > max_it ? max_it : it);
+    }
+    // lambda_max(Q): z (n) in dQw, q (n) in pq
+    double lq = 1.0;
+    for (it = 1; it <= max_it; ++it) {
+        double nz = 0.0;
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) nz += I.dQw[i] * I.dQw[i];
+        nz = sqrt(bsum(nz, sh));
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) I.pq[i] = I.dQw[i] / nz;
+        __syncthreads();
+        lr_apply(n, k, I.U64, I.d64, I.pq, I.dQw, t, sh);
+        double qz = 0.0;
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) qz += I.pq[i] * I.dQw[i];
+        lq = bsum(qz, sh);
+        double e1 = 0.0, e2 = 0.0;
+        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+            const double q = I.dQw[i] - lq * I.pq[i];
+            I.pq[i] = q;
+            e1 += q * q;
+            e2 += I.dQw[i] * I.dQw[i];
+        }
+        e1 = bsum(e1, sh);
+        e2 = bsum(e2, sh);
+        if (sqrt(e1) / (sqrt(e2) + lq) < tol) break;
+    }
+    if (threadIdx.x == 0) {
+        rec[R_POW_Q] = lq;
+        rec[R_POW_ITQ] = (double)(it > max_it ? max_it : it);
+    }
+}
+
+// ----------------------------------------------------------------------
+// host side
+// ----------------------------------------------------------------------
+static uint64_t bmix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+// same counter-based N(0,1) stream as the single-instance power method
+static void brandn(uint64_t seed, int64_t count, double *out) {
+    const double two_pi = 6.283185307179586476925286766559;
+    const uint64_t ks = bmix64(seed + 0x9E3779B97F4A7C15ULL);
+    for (int64_t i = 0; i < count; ++i) {
+        const uint64_t k0 = 2 * (uint64_t)i, k1 = k0 + 1;
+        const double a = (double)(bmix64(ks ^ (k0 * 0xD1B54A32D192ED03ULL + 0x632BE59BD9B4E019ULL)) >> 11) *
+                         (1.0 / 9007199254740992.0);
+        const double b = (double)(bmix64(ks ^ (k1 * 0xD1B54A32D192ED03ULL + 0x632BE59BD9B4E019ULL)) >> 11) *
+                         (1.0 / 9007199254740992.0);
+        out[i] = std::sqrt(-2.0 * std::log(1.0 - a)) * std::cos(two_pi * b);
+    }
+}
+
+static int64_t hprint_step(int64_t k) {
+    if (k <= 0) return 10;
+    int64_t p = 1;
+    while (p <= k / 10) p *= 10;
+    return std::max<int64_t>(p / 10, 10);
+}
+
+static double bgolden(double ap, double bp, double cp, double dp) {
+    const double phi = (std::sqrt(5.0) - 1.0) / 2.0;
+    auto f = [&](double x) { return ap * x + bp / x + cp * x * x / (1 + dp * x); };
+    double a = 1e-12, b = 1e12;
+    double c = b - phi * (b - a), d = a + phi * (b - a);
+    double fc = f(c), fd = f(d);
+    for (int i = 1; i <= 200; ++i) {
+        if (fd < fc) {
+            a = c;
+            c = d;
+            fc = fd;
+            d = a + phi * (b - a);
+            fd = f(d);
+        } else {
+            b = d;
+            d = c;
+            fd = fc;
+            c = b - phi * (b - a);
+            fc = f(c);
+        }
+        if ((b - a) < 1e-13) break;
+    }
+    return 0.5 * (a + b);
+}
+
+struct BatchWork {
+    DevVec<double> vn[20], vm[6];
+    DevVec<BCtl> ctl;
+    DevVec<double> rec;
+};
+
+template <class MT>
+static BArgs<MT> make_args(hprqp_batch *B, BatchWork &W) {
+    BArgs<MT> a{};
+    a.count = B->count;
+    a.n = B->dn.p;
+    a.m = B->dm.p;
+    a.k = B->dk.p;
+    a.noff = B->dnoff.p;
+    a.moff = B->dmoff.p;
+    a.uoff = B->duoff.p;
+    a.arp_off = B->darp_off.p;
+    a.atrp_off = B->datrp_off.p;
+    a.U64 = B->U.p;
+    a.d64 = B->d.p;
+    a.a_rp = B->a_rp.p;
+    a.a_ci = B->a_ci.p;
+    a.at_rp = B->at_rp.p;
+    a.at_ci = B->at_ci.p;
+    a.a_v64 = B->a_v.p;
+    a.at_v64 = B->at_v.p;
+    a.c = B->c.p;
+    a.l = B->l.p;
+    a.u = B->u.p;
+    a.AL = B->AL.p;
+    a.AU = B->AU.p;
+    if constexpr (std::is_same<MT, double>::value) {
+        a.U = B->U.p;
+        a.d = B->d.p;
+        a.a_v = B->a_v.p;
+        a.at_v = B->at_v.p;
+        a.cM = B->c.p;
+        a.lM = B->l.p;
+        a.uM = B->u.p;
+        a.ALM = B->AL.p;
+        a.AUM = B->AU.p;
+    } else {
+        a.U = B->U32.p;
+        a.d = B->d32.p;
+        a.a_v = B->a_v32.p;
+        a.at_v = B->at_v32.p;
+        a.cM = B->c32.p;
+        a.lM = B->l32.p;
+        a.uM = B->u32.p;
+        a.ALM = B->AL32.p;
+        a.AUM = B->AU32.p;
+    }
+    double **np[] = {&a.x, &a.w, &a.w_bar, &a.x_hat, &a.x_bar, &a.z_bar, &a.dx, &a.dw, &a.Qw, &a.Qw_bar,
+                     &a.tempv, &a.ATy, &a.ATy_bar, &a.ATdy, &a.QATdy, &a.dQw, &a.last_x, &a.last_w, &a.last_ATy,
+                     &a.pq};
+    for (int i = 0; i < 20; ++i) *np[i] = W.vn[i].p;
+    double **mp[] = {&a.y, &a.y_bar, &a.s, &a.dy, &a.last_y, &a.pz};
+    for (int i = 0; i < 6; ++i) *mp[i] = W.vm[i].p;
+    a.ctl = W.ctl.p;
+    a.rec = W.rec.p;
+    return a;
+}
+
+struct BInstState {
+    int64_t restart_flag = 0;
+    bool first_restart = true;
+    double last_gap = INFINITY, current_gap = INFINITY, save_gap = INFINITY;
+    int64_t inner = 0, times = 0;
+    double weighted_norm = INFINITY, best_gap = INFINITY, best_sigma = 1.0;
+    double sigma = 1.0, lamA = 0.0, lamQ = 0.0;
+    int status = -2;
+    int64_t iter = 0, power_it = 0;
+    double kkt = INFINITY, pobj = 0, dobj = 0, gap = 0, errp = 0, errd = 0;
+};
+
+template <class MT>
+static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params *p, const double *x0,
+                             const double *y0, double *x_out, double *y_out, int32_t *status_out, int64_t *iter_out,
+                             double *obj_out, double *kkt_out, hprqp_batch_result *res) {
+    hipStream_t s = ctx->stream;
+    const int32_t cnt = B->count;
+    const int64_t N = B->Ntot, M = B->Mtot;
+    BatchWork W;
+    for (auto &v : W.vn) {
+        v.alloc(std::max<int64_t>(N, 1));
+        HPRQP_HIP(hipMemsetAsync(v.p, 0, sizeof(double) * v.n, s));
+    }
+    for (auto &v : W.vm) {
+        v.alloc(std::max<int64_t>(M, 1));
+        HPRQP_HIP(hipMemsetAsync(v.p, 0, sizeof(double) * v.n, s));
+    }
+    W.ctl.alloc(cnt);
+    W.rec.alloc((int64_t)cnt * R_COUNT);
+    HPRQP_HIP(hipMemsetAsync(W.rec.p, 0, sizeof(double) * W.rec.n, s));
+    BArgs<MT> a = make_args<MT>(B, W);
+    // pinned staging for the per-instance control and records
+    BCtl *hctl = nullptr;
+    double *hrec = nullptr;
+    HPRQP_HIP(hipHostMalloc((void **)&hctl, sizeof(BCtl) * cnt, hipHostMallocDefault));
+    HPRQP_HIP(hipHostMalloc((void **)&hrec, sizeof(double) * R_COUNT * cnt, hipHostMallocDefault));
+    struct PinnedFree {
+        void *a, *b;
+        ~PinnedFree() {
+            (void)hipHostFree(a);
+            (void)hipHostFree(b);
+        }
+    } pf{hctl, hrec};
+    std::vector<BInstState> st(cnt);
+    std::memset(hctl, 0, sizeof(BCtl) * cnt);
+    for (int b = 0; b < cnt; ++b) {
+        hctl[b].active = 1;
+        hctl[b].check_iter = (int32_t)p->check_iter;
+    }
+    auto put_ctl = [&] {
+        HPRQP_HIP(hipMemcpyAsync(W.ctl.p, hctl, sizeof(BCtl) * cnt, hipMemcpyHostToDevice, s));
+    };
+    auto get_rec = [&] {
+        HPRQP_HIP(hipMemcpyAsync(hrec, W.rec.p, sizeof(double) * R_COUNT * cnt, hipMemcpyDeviceToHost, s));
+        HPRQP_HIP(hipStreamSynchronize(s));
+    };
+    const double t_start = bnow();
+    // power method start vectors: z (m) in pz, z (n) in dQw
+    {
+        std::vector<double> zm(std::max<int64_t>(M, 1)), zn(std::max<int64_t>(N, 1));
+        for (int b = 0; b < cnt; ++b) {
+            brandn(1, B->m[b], zm.data() + B->moff[b]);
+            for (int32_t r = 0; r < B->m[b]; ++r) zm[B->moff[b] + r] += 1e-8;
+            brandn(1, B->n[b], zn.data() + B->noff[b]);
+            for (int32_t i = 0; i < B->n[b]; ++i) zn[B->noff[b] + i] += 1e-8;
+        }
+        HPRQP_HIP(hipMemcpyAsync(a.pz, zm.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
+        HPRQP_HIP(hipMemcpyAsync(a.dQw, zn.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+        put_ctl();
+        k_batch_power<MT><<<cnt, kBlock, 0, s>>>(a, 1e-4, 5000);
+        HPRQP_HIP(hipGetLastError());
+        get_rec();
+    }
+    for (int b = 0; b < cnt; ++b) {
+        BInstState &S = st[b];
+        const double *r = hrec + (int64_t)b * R_COUNT;
+        S.lamA = B->m[b] > 0 ? r[R_POW_A] * p->eig_factor : 0.0;
+        S.lamQ = r[R_POW_Q] * p->eig_factor;
+        S.power_it = (int64_t)(r[R_POW_ITA] + r[R_POW_ITQ]);
+        // sigma_0 (src/algorithm.jl:839-851), unscaled data
+        const double nb = B->norm_b2[b], nc = B->norm_c2[b];
+        S.sigma = (p->sigma == -1) ? ((nc > 1e-16 && nb > 1e-16 && nb < 1e16 && nc < 1e16) ? nb / nc : 1.0)
+                                   : p->sigma;
+        S.best_sigma = S.sigma;
+        hctl[b].sigma = S.sigma;
+        hctl[b].lamA = S.lamA;
+        hctl[b].lamQ = S.lamQ;
+    }
+    res->power_time = bnow() - t_start;
+    // warm start: x0 / y0 go to x_bar / y_bar, the init kernel spreads them
+    if (x0) HPRQP_HIP(hipMemcpyAsync(a.x_bar, x0, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    if (y0 && M) HPRQP_HIP(hipMemcpyAsync(a.y_bar, y0, sizeof(double) * M, hipMemcpyHostToDevice, s));
+    put_ctl();
+    k_batch_init<MT><<<cnt, kBlock, 0, s>>>(a, x0 != nullptr, y0 != nullptr);
+    HPRQP_HIP(hipGetLastError());
+    get_rec();
+    // one check interval = ctl upload + interval kernel + record download,
+    // captured once in a HIP graph and replayed (ctl/rec live in pinned
+    // memory at fixed addresses; the interval length travels in ctl)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    HPRQP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    HPRQP_HIP(hipMemcpyAsync(W.ctl.p, hctl, sizeof(BCtl) * cnt, hipMemcpyHostToDevice, s));
+    k_batch_interval<MT><<<cnt, kBlock, 0, s>>>(a);
+    HPRQP_HIP(hipMemcpyAsync(hrec, W.rec.p, sizeof(double) * R_COUNT * cnt, hipMemcpyDeviceToHost, s));
+    HPRQP_HIP(hipStreamEndCapture(s, &graph));
+    HPRQP_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    struct GraphFree {
+        hipGraph_t g;
+        hipGraphExec_t e;
+        ~GraphFree() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } gf{graph, gexec};
+
+    int64_t it = 0, intervals = 0, solved = 0;
+    int active = cnt;
+    const double tol = p->stoptol;
+    while (active > 0) {
+        // decisions at residual iteration `it` for every active instance
+        for (int b = 0; b < cnt; ++b) {
+            BInstState &S = st[b];
+            if (!hctl[b].active) continue;
+            const double *r = hrec + (int64_t)b * R_COUNT;
+            // residuals (operator Q: no scaling, b_scale = c_scale = 1)
+            S.pobj = r[R_CX] + 0.5 * r[R_XQX] + B->obj_constant[b];
+            S.dobj = -0.5 * r[R_XQX] + r[R_ZX] + B->obj_constant[b];
+            if (B->m[b] > 0) S.dobj += r[R_YS];
+            S.gap = std::fabs(S.pobj - S.dobj) / (1.0 + std::max(std::fabs(S.pobj), std::fabs(S.dobj)));
+            S.errd = r[R_RD] / (1.0 + std::max(std::max(B->norm_c_inf[b], r[R_ATY]), r[R_QX]));
+            S.errp = B->m[b] > 0 ? r[R_RP] / (1.0 + std::max(B->norm_b_inf[b], r[R_AX])) : 0.0;
+            if (it == 0) S.errp = std::max(S.errp, r[R_BND]);
+            S.kkt = std::max(std::max(S.errp, S.errd), S.gap);
+            S.iter = it;
+            int stat = -2;
+            if (S.kkt < tol)
+                stat = HPRQP_STATUS_OPTIMAL;
+            else if (it >= p->max_iter)
+                stat = HPRQP_STATUS_MAX_ITER;
+            else if (bnow() - t_start > p->time_limit)
+                stat = HPRQP_STATUS_TIME_LIMIT;
+            if (stat != -2) {
+                S.status = stat;
+                hctl[b].active = 0;
+                --active;
+                if (stat == HPRQP_STATUS_OPTIMAL) ++solved;
+                continue;
+            }
+            // check_restart (src/algorithm.jl:441-492)
+            S.restart_flag = 0;
+            if (S.first_restart) {
+                if (it == p->check_iter) {
+                    S.first_restart = false;
+                    S.restart_flag = 1;
+                    S.weighted_norm = S.current_gap;
+                    S.best_gap = S.current_gap;
+                    S.best_sigma = S.sigma;
+                }
+            } else if (it % p->check_iter == 0) {
+                if (S.current_gap < 0) S.current_gap = 1e-6;
+                if (S.current_gap <= 0.2 * S.last_gap) S.restart_flag = 1;
+                if (S.current_gap <= 0.8 * S.last_gap && S.current_gap > S.save_gap) S.restart_flag = 2;
+                const double fact = (S.current_gap / S.weighted_norm > 1e-1) ? 0.5 : 0.2;
+                if ((double)S.inner >= fact * (double)it) S.restart_flag = 3;
+                if (S.best_gap > S.current_gap) {
+                    S.best_gap = S.current_gap;
+                    S.best_sigma = S.sigma;
+                }
+                S.save_gap = S.current_gap;
+            }
+            // update_sigma! (src/algorithm.jl:323-423), non-diagonal operator Q
+            if (!p->sigma_fixed && S.restart_flag >= 1) {
+                const int32_t m = B->m[b];
+                double aa = 0.0, bb = std::max(r[R_DXDX], 1e-12), cc = 0.0, dd = 0.0;
+                if (m > 0) aa = S.lamA * r[R_DYDY] - 2 * r[R_DQWATDY];
+                aa += S.lamQ * r[R_DWDQW];
+                if (m > 0) {
+                    cc = r[R_ATDYQATDY];
+                    dd = S.lamQ;
+                }
+                aa = std::max(aa, 1e-12);
+                const double est = m > 0 ? bgolden(aa, bb, cc, dd) : std::sqrt(bb / aa);
+                const double fact = std::exp(-0.05 * (S.current_gap / S.best_gap));
+                const double temp_1 = std::max(std::min(S.errd, S.errp), std::min(S.gap, S.current_gap));
+                const double cand = std::exp(fact * std::log(est) + (1 - fact) * std::log(S.best_sigma));
+                auto clampj = [](double x, double lo, double hi) { return x > hi ? hi : (x < lo ? lo : x); };
+                double kappa = 1.0;
+                if (temp_1 <= 9e-10)
+                    kappa = temp_1 > 5e-10 ? clampj(std::sqrt(S.errd / S.errp), 1e-2, 100.0)
+                                           : clampj(S.errd / S.errp, 1e-2, 100.0);
+                S.sigma = kappa * cand;
+            }
+            // do_restart! bookkeeping (the vectors are reset on the device)
+            hctl[b].restart = S.restart_flag > 0;
+            if (S.restart_flag > 0) {
+                S.last_gap = S.current_gap;
+                S.save_gap = INFINITY;
+                S.times++;
+                S.inner = 0;
+            }
+            hctl[b].sigma = S.sigma;
+            hctl[b].h_inner = (double)S.inner;
+            hctl[b].want_last_gap = S.restart_flag > 0;
+            hctl[b].iter0 = it;
+        }
+        if (active == 0) break;
+        // interval to the next residual iteration (shared cadence)
+        int64_t K = 1;
+        while (!(((it + K) % p->check_iter) == 0 ||
+                 (p->print_frequency == -1 ? ((it + K) % hprint_step(it + K) == 0)
+                                           : ((it + K) % p->print_frequency == 0)) ||
+                 it + K >= p->max_iter))
+            ++K;
+        for (int b = 0; b < cnt; ++b)
+            if (hctl[b].active) hctl[b].K = (int32_t)K;
+        HPRQP_HIP(hipGraphLaunch(gexec, s));
+        HPRQP_HIP(hipStreamSynchronize(s));
+        ++intervals;
+        for (int b = 0; b < cnt; ++b) {
+            if (!hctl[b].active) continue;
+            BInstState &S = st[b];
+            const double *r = hrec + (int64_t)b * R_COUNT;
+            if (hctl[b].want_last_gap) S.last_gap = r[R_LASTGAP];
+            if ((it + K) % p->check_iter == 0) S.current_gap = r[R_CURGAP];
+            S.inner += K;
+            hctl[b].restart = 0;
+            hctl[b].want_last_gap = 0;
+        }
+        it += K;
+    }
+    HPRQP_HIP(hipStreamSynchronize(s));
+    res->time = bnow() - t_start;
+    res->solved = solved;
+    res->intervals = intervals;
+    res->max_iter_reached = it;
+    int64_t tot_it = 0;
+    for (int b = 0; b < cnt; ++b) {
+        const BInstState &S = st[b];
+        tot_it += S.iter;
+        if (status_out) status_out[b] = S.status;
+        if (iter_out) iter_out[b] = S.iter;
+        if (obj_out) obj_out[b] = S.pobj;
+        if (kkt_out) kkt_out[b] = S.kkt;
+    }
+    res->total_iter = tot_it;
+    if (x_out) HPRQP_HIP(hipMemcpy(x_out, a.x_bar, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (y_out && M) HPRQP_HIP(hipMemcpy(y_out, a.y_bar, sizeof(double) * M, hipMemcpyDeviceToHost));
+}
+
+void batch_solve(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params *p, const double *x0, const double *y0,
+                 double *x_out, double *y_out, int32_t *status_out, int64_t *iter_out, double *obj_out,
+                 double *kkt_out, hprqp_batch_result *res) {
+    std::memset(res, 0, sizeof(*res));
+    HPRQP_REQUIRE(p->check_iter > 0, HPRQP_E_ARG, "check_iter must be positive");
+    if (p->precision == HPRQP_PREC_FP64)
+        batch_solve_impl<double>(ctx, B, p, x0, y0, x_out, y_out, status_out, iter_out, obj_out, kkt_out, res);
+    else
+        batch_solve_impl<float>(ctx, B, p, x0, y0, x_out, y_out, status_out, iter_out, obj_out, kkt_out, res);
+}
+
+}  // namespace hprqp

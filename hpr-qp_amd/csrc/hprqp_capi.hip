@@ -368,6 +368,38 @@ int hprqp_qop_create_qap(hprqp_ctx *ctx, int32_t nq, const double *A, const doub
     });
 }
 
+int hprqp_qop_create_lowrank(hprqp_ctx *ctx, int32_t n, int32_t k, const double *U, const double *diag,
+                             hprqp_qop **out) {
+    CTX_CHECK(ctx);
+    if (!out || n <= 0 || k < 0 || !diag || (k > 0 && !U)) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        hprqp_qop *q = new hprqp_qop();
+        try {
+            q->kind = QOP_LOWRANK;
+            q->n = n;
+            q->lr_k = k;
+            q->name = "DiagonalPlusLowRank";
+            const int64_t nk = (int64_t)n * k;
+            for (int64_t i = 0; i < nk; ++i)
+                HPRQP_REQUIRE(std::isfinite(U[i]), HPRQP_E_DATA, "U has NaN/Inf");
+            for (int32_t i = 0; i < n; ++i)
+                HPRQP_REQUIRE(std::isfinite(diag[i]), HPRQP_E_DATA, "diag has NaN/Inf");
+            q->lr_U.alloc(nk);
+            q->lr_d.alloc(n);
+            q->lr_t.alloc(std::max(1, k));
+            if (nk)
+                HPRQP_HIP(hipMemcpyAsync(q->lr_U.p, U, sizeof(double) * nk, hipMemcpyHostToDevice, ctx->stream));
+            HPRQP_HIP(hipMemcpyAsync(q->lr_d.p, diag, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+            HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete q;
+            throw;
+        }
+        *out = q;
+    });
+}
+
 int hprqp_qop_create_user(hprqp_ctx *ctx, const hprqp_qop_vtable *vt, void *user, hprqp_qop **out) {
     CTX_CHECK(ctx);
     if (!out || !vt || !vt->apply || !vt->problem_size) return HPRQP_E_ARG;
@@ -414,6 +446,8 @@ int hprqp_qop_info(const hprqp_qop *q, int64_t *n, int64_t *temp_size, const cha
             *temp_size = q->la.nrow;
         else if (q->kind == QOP_QAP)
             *temp_size = (int64_t)q->nq * q->nq;
+        else if (q->kind == QOP_LOWRANK)
+            *temp_size = q->lr_k;
         else if (q->kind == QOP_USER && q->vt.temp_size)
             *temp_size = q->vt.temp_size(q->user);
         else
@@ -536,6 +570,170 @@ int hprqp_solve(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *pa
     }
     return guarded(ctx, [&] {
         hprqp::solve(ctx, model, params, x0, y0, x_out, y_out, z_out, w_out, hist, hist_cap, result);
+    });
+}
+
+int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const int32_t *m, const int32_t *k,
+                       const double *U, const double *diag, const int32_t *a_rowptr, const int32_t *a_colidx,
+                       const double *a_vals, const double *c, const double *l, const double *u, const double *AL,
+                       const double *AU, const double *obj_constant, hprqp_batch **out) {
+    CTX_CHECK(ctx);
+    if (!out || count <= 0 || !n || !m || !k || !diag || !c || !l || !u) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        hprqp_batch *B = new hprqp_batch();
+        try {
+            hipStream_t s = ctx->stream;
+            B->count = count;
+            B->n.assign(n, n + count);
+            B->m.assign(m, m + count);
+            B->k.assign(k, k + count);
+            B->noff.resize(count);
+            B->moff.resize(count);
+            B->uoff.resize(count);
+            std::vector<int64_t> arp_off(count), atrp_off(count), nnz_off(count);
+            int64_t N = 0, M = 0, UT = 0, NZ = 0, ARP = 0, ATRP = 0;
+            for (int32_t b = 0; b < count; ++b) {
+                HPRQP_REQUIRE(n[b] > 0 && m[b] >= 0 && k[b] >= 0 && k[b] <= 64, HPRQP_E_ARG,
+                              "instance dimensions out of range (n > 0, m >= 0, 0 <= k <= 64)");
+                B->noff[b] = N;
+                B->moff[b] = M;
+                B->uoff[b] = UT;
+                arp_off[b] = ARP;
+                atrp_off[b] = ATRP;
+                nnz_off[b] = NZ;
+                const int32_t nnz_b = m[b] > 0 ? a_rowptr[ARP + m[b]] : 0;
+                N += n[b];
+                M += m[b];
+                UT += (int64_t)n[b] * k[b];
+                NZ += nnz_b;
+                ARP += m[b] + 1;
+                ATRP += n[b] + 1;
+            }
+            HPRQP_REQUIRE(NZ < INT32_MAX && N < INT32_MAX, HPRQP_E_ARG, "batch exceeds int32 indexing");
+            B->Ntot = N;
+            B->Mtot = M;
+            B->Utot = UT;
+            B->nnzA = NZ;
+            // packed CSR with row pointers made global, and the per-instance transposes
+            std::vector<int32_t> arp(ARP), aci(NZ), atrp(ATRP), atci(NZ);
+            std::vector<double> av(NZ), atv(NZ);
+            for (int32_t b = 0; b < count; ++b) {
+                if (m[b] > 0) {
+                    HostCsr h = hprqp::host_csr_from(m[b], n[b], a_rowptr[arp_off[b] + m[b]],
+                                                     a_rowptr + arp_off[b], a_colidx + nnz_off[b],
+                                                     a_vals + nnz_off[b], 0);
+                    HostCsr t = hprqp::host_csr_transpose(h);
+                    for (int32_t r = 0; r <= m[b]; ++r) arp[arp_off[b] + r] = (int32_t)(h.rp[r] + nnz_off[b]);
+                    for (int32_t j = 0; j <= n[b]; ++j) atrp[atrp_off[b] + j] = (int32_t)(t.rp[j] + nnz_off[b]);
+                    std::copy(h.ci.begin(), h.ci.end(), aci.begin() + nnz_off[b]);
+                    std::copy(h.v.begin(), h.v.end(), av.begin() + nnz_off[b]);
+                    std::copy(t.ci.begin(), t.ci.end(), atci.begin() + nnz_off[b]);
+                    std::copy(t.v.begin(), t.v.end(), atv.begin() + nnz_off[b]);
+                } else {
+                    arp[arp_off[b]] = (int32_t)nnz_off[b];
+                    for (int32_t j = 0; j <= n[b]; ++j) atrp[atrp_off[b] + j] = (int32_t)nnz_off[b];
+                }
+            }
+            // per-instance norms (operator Q: no scaling, src/utils.jl:364-406)
+            B->obj_constant.assign(count, 0.0);
+            B->norm_b2.assign(count, 0.0);
+            B->norm_c2.assign(count, 0.0);
+            B->norm_b_inf.assign(count, 0.0);
+            B->norm_c_inf.assign(count, 0.0);
+            for (int32_t b = 0; b < count; ++b) {
+                if (obj_constant) B->obj_constant[b] = obj_constant[b];
+                double s2 = 0, si = 0;
+                for (int32_t i = 0; i < n[b]; ++i) {
+                    const int64_t g = B->noff[b] + i;
+                    HPRQP_REQUIRE(std::isfinite(c[g]) && std::isfinite(diag[g]), HPRQP_E_DATA,
+                                  "c or diag has NaN/Inf");
+                    HPRQP_REQUIRE(!(l[g] > u[g]), HPRQP_E_DATA, "infeasible variable bounds: l > u");
+                    s2 += c[g] * c[g];
+                    si = std::max(si, std::fabs(c[g]));
+                }
+                B->norm_c2[b] = std::sqrt(s2);
+                B->norm_c_inf[b] = si;
+                s2 = si = 0;
+                for (int32_t r = 0; r < m[b]; ++r) {
+                    const int64_t g = B->moff[b] + r;
+                    HPRQP_REQUIRE(!(AL[g] > AU[g]), HPRQP_E_DATA, "infeasible constraint bounds: AL > AU");
+                    const double a = AL[g] == -INFINITY ? 0.0 : AL[g], bb = AU[g] == INFINITY ? 0.0 : AU[g];
+                    const double t = std::max(std::fabs(a), std::fabs(bb));
+                    s2 += t * t;
+                    si = std::max(si, t);
+                }
+                B->norm_b2[b] = std::sqrt(s2);
+                B->norm_b_inf[b] = si;
+            }
+            auto up = [&](auto &dst, const auto *src, int64_t len) {
+                dst.alloc(std::max<int64_t>(len, 1));
+                if (len) HPRQP_HIP(hipMemcpyAsync(dst.p, src, sizeof(*src) * len, hipMemcpyHostToDevice, s));
+            };
+            auto up32 = [&](hprqp::DevVec<float> &dst, const double *src, int64_t len) {
+                std::vector<float> f(src, src + len);
+                dst.alloc(std::max<int64_t>(len, 1));
+                if (len) HPRQP_HIP(hipMemcpy(dst.p, f.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+            };
+            up(B->dn, n, count);
+            up(B->dm, m, count);
+            up(B->dk, k, count);
+            up(B->dnoff, B->noff.data(), count);
+            up(B->dmoff, B->moff.data(), count);
+            up(B->duoff, B->uoff.data(), count);
+            up(B->darp_off, arp_off.data(), count);
+            up(B->datrp_off, atrp_off.data(), count);
+            up(B->U, U, UT);
+            up(B->d, diag, N);
+            up(B->a_rp, arp.data(), ARP);
+            up(B->a_ci, aci.data(), NZ);
+            up(B->a_v, av.data(), NZ);
+            up(B->at_rp, atrp.data(), ATRP);
+            up(B->at_ci, atci.data(), NZ);
+            up(B->at_v, atv.data(), NZ);
+            up(B->c, c, N);
+            up(B->l, l, N);
+            up(B->u, u, N);
+            up(B->AL, AL, M);
+            up(B->AU, AU, M);
+            HPRQP_HIP(hipStreamSynchronize(s));
+            up32(B->U32, U, UT);
+            up32(B->d32, diag, N);
+            up32(B->a_v32, av.data(), NZ);
+            up32(B->at_v32, atv.data(), NZ);
+            up32(B->c32, c, N);
+            up32(B->l32, l, N);
+            up32(B->u32, u, N);
+            up32(B->AL32, AL, M);
+            up32(B->AU32, AU, M);
+        } catch (...) {
+            delete B;
+            throw;
+        }
+        *out = B;
+    });
+}
+
+int hprqp_batch_destroy(hprqp_ctx *ctx, hprqp_batch *batch) {
+    CTX_CHECK(ctx);
+    return guarded(ctx, [&] {
+        HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+        delete batch;
+    });
+}
+
+int hprqp_batch_solve(hprqp_ctx *ctx, hprqp_batch *batch, const hprqp_params *params, const double *x0,
+                      const double *y0, double *x_out, double *y_out, int32_t *status_out, int64_t *iter_out,
+                      double *obj_out, double *kkt_out, hprqp_batch_result *result) {
+    CTX_CHECK(ctx);
+    if (!batch || !result) return HPRQP_E_ARG;
+    hprqp_params def;
+    if (!params) {
+        hprqp_params_default(&def);
+        params = &def;
+    }
+    return guarded(ctx, [&] {
+        hprqp::batch_solve(ctx, batch, params, x0, y0, x_out, y_out, status_out, iter_out, obj_out, kkt_out, result);
     });
 }
 

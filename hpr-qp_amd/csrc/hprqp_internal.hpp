@@ -117,7 +117,7 @@ struct hprqp_csr {
     bool has_t = false;
 };
 
-enum QopKind { QOP_CSR = 0, QOP_LASSO = 1, QOP_QAP = 2, QOP_USER = 3 };
+enum QopKind { QOP_CSR = 0, QOP_LASSO = 1, QOP_QAP = 2, QOP_USER = 3, QOP_LOWRANK = 4 };
 
 struct hprqp_qop {
     int kind = QOP_CSR;
@@ -133,6 +133,10 @@ struct hprqp_qop {
     int32_t nq = 0;
     hprqp::DevVec<double> qa, qb, qs, qt, qtemp;
     hprqp::DevVec<float> qtemp32;
+    // diagonal plus low rank: Q = diag(d) + U U^T, U n x k column-major
+    int32_t lr_k = 0;
+    hprqp::DevVec<double> lr_U, lr_d, lr_t;
+    hprqp::DevVec<float> lr_U32, lr_d32;
     // user plugin
     hprqp_qop_vtable vt{};
     void *user = nullptr;
@@ -153,7 +157,23 @@ struct hprqp_model {
     mutable int tuned_lanes[2][3] = {{0, 0, 0}, {0, 0, 0}};
 };
 
+struct hprqp_batch {
+    int32_t count = 0;
+    int64_t Ntot = 0, Mtot = 0, Utot = 0, nnzA = 0;
+    std::vector<int32_t> n, m, k;
+    std::vector<int64_t> noff, moff, uoff;
+    std::vector<double> obj_constant, norm_b2, norm_c2, norm_b_inf, norm_c_inf;
+    hprqp::DevVec<int32_t> dn, dm, dk;
+    hprqp::DevVec<int64_t> dnoff, dmoff, duoff, darp_off, datrp_off;
+    hprqp::DevVec<double> U, d, a_v, at_v, c, l, u, AL, AU;
+    hprqp::DevVec<float> U32, d32, a_v32, at_v32, c32, l32, u32, AL32, AU32;
+    hprqp::DevVec<int32_t> a_rp, a_ci, at_rp, at_ci;
+};
+
 namespace hprqp {
+void batch_solve(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params *p, const double *x0, const double *y0,
+                 double *x_out, double *y_out, int32_t *status_out, int64_t *iter_out, double *obj_out,
+                 double *kkt_out, hprqp_batch_result *res);
 // solver entry (hprqp_solver.hip)
 void solve(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, const double *x0,
            const double *y0, double *x_out, double *y_out, double *z_out, double *w_out,
@@ -169,4 +189,7 @@ void qap_apply_fp64(hipStream_t s, int32_t nq, const double *A, const double *B,
                     const double *T, const double *X, double *tmp, double *QX);
 void qap_apply_fp32(hipStream_t s, int32_t nq, const double *A, const double *B, const double *S,
                     const double *T, const float *X, float *tmp, float *QX);
+// diagonal plus low rank Qx = d.*x + U (U^T x)  (hprqp_lowrank.hip)
+template <class MT, class XT, class YT>
+void lowrank_apply(hipStream_t s, int32_t n, int32_t k, const MT *U, const MT *d, const XT *x, double *t, YT *Qx);
 }  // namespace hprqp

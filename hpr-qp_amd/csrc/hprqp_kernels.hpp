@@ -433,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_each(int64_t n, F f) {
 // ---------------------------------------------------------------------
 // column statistic over A's columns (rows of the stored transpose) and Q's
 // rows: sqrt(max) (Ruiz) or sqrt(sum) (Pock-Chambolle), 0 -> 1.
-__global__ __launch_bounds__(kBlock) void k_col_stat(int32_t n, const int32_t *__restrict__ at_rp,
+static __global__ __launch_bounds__(kBlock) void k_col_stat(int32_t n, const int32_t *__restrict__ at_rp,
                                                      const double *__restrict__ at_v,
                                                      const int32_t *__restrict__ q_rp,
                                                      const double *__restrict__ q_v, int ruiz,
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_col_stat(int32_t n, const int32_t *_
     out[j] = t > 0.0 ? sqrt(t) : 1.0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_row_stat(int32_t m, const int32_t *__restrict__ rp,
+static __global__ __launch_bounds__(kBlock) void k_row_stat(int32_t m, const int32_t *__restrict__ rp,
                                                      const double *__restrict__ v, int ruiz,
                                                      double *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -462,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_row_stat(int32_t m, const int32_t *_
 }
 
 // v_ij = (v_ij * (1/r_i)) / c_j
-__global__ __launch_bounds__(kBlock) void k_scale_csr(int32_t nrow, const int32_t *__restrict__ rp,
+static __global__ __launch_bounds__(kBlock) void k_scale_csr(int32_t nrow, const int32_t *__restrict__ rp,
                                                       const int32_t *__restrict__ ci,
                                                       double *__restrict__ v,
                                                       const double *__restrict__ r,

@@ -375,7 +375,7 @@ struct Solver {
     // anchors (FP64)
     DevVec<double> last_x, last_y, last_Qw, last_w, last_ATy;
     // FP64 residual temporaries
-    DevVec<double> Qx_d, ATdy_d, Ax_d, xd_tmp, qtemp_d;
+    DevVec<double> Qx_d, ATdy_d, Ax_d, xd_tmp, qtemp_d, lr_t;
     // data in VT (aliases of the FP64 arrays when VT == double)
     DevVec<MT> c_own, l_own, u_own, AL_own, AU_own;
     const MT *cV, *lV, *uV, *ALV, *AUV;
@@ -417,6 +417,20 @@ struct Solver {
         } else if (S.qkind == QOP_QAP) {
             qtemp.alloc(n);
             qtemp_d.alloc(n);
+        } else if (S.qkind == QOP_LOWRANK) {
+            lr_t.alloc(std::max(1, S.qop->lr_k));
+            if constexpr (!std::is_same<MT, double>::value) {
+                hprqp_qop *q = S.qop;
+                const int64_t nk = (int64_t)n * q->lr_k;
+                if (q->lr_U32.n != nk) {
+                    q->lr_U32.alloc(nk);
+                    q->lr_d32.alloc(n);
+                    const double *u = q->lr_U.p, *d = q->lr_d.p;
+                    float *u32 = q->lr_U32.p, *d32 = q->lr_d32.p;
+                    each(s, nk, [=] __device__(int64_t i) { u32[i] = (float)u[i]; });
+                    each(s, n, [=] __device__(int64_t i) { d32[i] = (float)d[i]; });
+                }
+            }
         }
         if constexpr (std::is_same<MT, double>::value) {
             cV = S.c.p;
@@ -474,6 +488,14 @@ struct Solver {
                     qap_apply_fp32(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xin,
                                    qtemp.p, out);
                 break;
+            case QOP_LOWRANK:
+                if constexpr (std::is_same<MT, double>::value)
+                    lowrank_apply<double, VT, VT>(s, n, S.qop->lr_k, S.qop->lr_U.p, S.qop->lr_d.p, xin,
+                                                  lr_t.p, out);
+                else
+                    lowrank_apply<float, VT, VT>(s, n, S.qop->lr_k, S.qop->lr_U32.p, S.qop->lr_d32.p, xin,
+                                                 lr_t.p, out);
+                break;
             default: {
                 if constexpr (std::is_same<VT, double>::value) {
                     int rc = S.qop->vt.apply(S.qop->user, xin, out, (void *)s);
@@ -529,6 +551,9 @@ struct Solver {
                 if (S.qkind == QOP_QAP) {
                     qap_apply_fp64(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xd,
                                    qtemp_d.p, out);
+                } else if (S.qkind == QOP_LOWRANK) {
+                    lowrank_apply<double, double, double>(s, n, S.qop->lr_k, S.qop->lr_U.p, S.qop->lr_d.p, xd,
+                                                          lr_t.p, out);
                 } else {
                     int rc = S.qop->vt.apply(S.qop->user, xd, out, (void *)s);
                     HPRQP_REQUIRE(rc == 0, HPRQP_E_ARG, "user Q operator apply failed");
@@ -629,6 +654,10 @@ struct Solver {
             case QOP_QAP:
                 qap_apply_fp64(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xin,
                                qtemp_d.p, out);
+                break;
+            case QOP_LOWRANK:
+                lowrank_apply<double, double, double>(s, n, S.qop->lr_k, S.qop->lr_U.p, S.qop->lr_d.p, xin,
+                                                      lr_t.p, out);
                 break;
             default: {
                 int rc = S.qop->vt.apply(S.qop->user, xin, out, (void *)s);
@@ -1721,6 +1750,9 @@ void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx) {
             break;
         case QOP_QAP:
             qap_apply_fp64(s, q->nq, q->qa.p, q->qb.p, q->qs.p, q->qt.p, x, q->qtemp.p, Qx);
+            break;
+        case QOP_LOWRANK:
+            lowrank_apply<double, double, double>(s, q->n, q->lr_k, q->lr_U.p, q->lr_d.p, x, q->lr_t.p, Qx);
             break;
         default: {
             int rc = q->vt.apply(q->user, x, Qx, (void *)s);

@@ -7,6 +7,7 @@ runs in libhprqp (hand-written HIP kernels behind a C ABI, include/hprqp.h).
 """
 from .model import (
     AbstractQOperator,
+    DiagonalPlusLowRank_Q_operator,
     LASSO_Q_operator,
     QAP_Q_operator,
     QP_info_cpu,
@@ -21,6 +22,7 @@ from .solver import DeviceModel, DeviceQOperator, HPRQP_parameters, HPRQP_result
 
 __all__ = [
     "AbstractQOperator",
+    "DiagonalPlusLowRank_Q_operator",
     "LASSO_Q_operator",
     "QAP_Q_operator",
     "QP_info_cpu",

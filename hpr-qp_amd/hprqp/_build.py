@@ -9,7 +9,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent.parent
 CSRC = PKG.parent / "csrc"
 INCLUDE = ROOT / "include"
-SOURCES = ["hprqp_solver.hip", "hprqp_capi.hip", "hprqp_qap.hip"]
+SOURCES = ["hprqp_solver.hip", "hprqp_capi.hip", "hprqp_qap.hip", "hprqp_lowrank.hip", "hprqp_batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -81,6 +81,17 @@ class hprqp_result(C.Structure):
     ]
 
 
+class hprqp_batch_result(C.Structure):
+    _fields_ = [
+        ("time", C.c_double),
+        ("power_time", C.c_double),
+        ("solved", C.c_int64),
+        ("intervals", C.c_int64),
+        ("max_iter_reached", C.c_int64),
+        ("total_iter", C.c_int64),
+    ]
+
+
 class hprqp_qop_vtable(C.Structure):
     _fields_ = [
         ("apply", C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)),
@@ -116,6 +127,7 @@ SIGNATURES = [
     ("hprqp_qop_create_lasso", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int64, _P, _P, _P, C.c_int,
                                          C.POINTER(_P)]),
     ("hprqp_qop_create_qap", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, C.POINTER(_P)]),
+    ("hprqp_qop_create_lowrank", C.c_int, [_P, C.c_int32, C.c_int32, _P, _P, C.POINTER(_P)]),
     ("hprqp_qop_create_user", C.c_int, [_P, C.POINTER(hprqp_qop_vtable), _P, C.POINTER(_P)]),
     ("hprqp_qop_destroy", C.c_int, [_P, _P]),
     ("hprqp_qop_apply", C.c_int, [_P, _P, _P, _P]),
@@ -126,6 +138,11 @@ SIGNATURES = [
     ("hprqp_solve", C.c_int, [_P, _P, C.POINTER(hprqp_params), _P, _P, _P, _P, _P, _P, _P, C.c_int64,
                               C.POINTER(hprqp_result)]),
     ("hprqp_profile_step", C.c_int, [_P, _P, C.POINTER(hprqp_params), C.c_int, _P, C.POINTER(C.c_int)]),
+    ("hprqp_batch_create", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     C.POINTER(_P)]),
+    ("hprqp_batch_destroy", C.c_int, [_P, _P]),
+    ("hprqp_batch_solve", C.c_int, [_P, _P, C.POINTER(hprqp_params), _P, _P, _P, _P, _P, _P, _P, _P,
+                                    C.POINTER(hprqp_batch_result)]),
 ]
 
 _lib = None

@@ -15,7 +15,8 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 
-from .model import QAP_Q_operator, QP_info_cpu, build_from_ABST, build_from_Ab_lambda, build_from_QAbc
+from .model import (DiagonalPlusLowRank_Q_operator, QAP_Q_operator, QP_info_cpu, build_from_ABST,
+                    build_from_Ab_lambda, build_from_QAbc)
 
 
 def _distinct_cols(rng, rows, per_row, n):
@@ -147,15 +148,30 @@ def qap(nq=128, seed=0, density=0.1, grid=32, verbose=False):
     return model, dict(F=F, D=D, shift=shift)
 
 
-def portfolio(n, k, seed=0):
+def portfolio(n, k, seed=0, explicit=True):
     """min 1/2 x^T (F F^T + D) x - mu^T x  s.t. 1^T x = 1, 0 <= x <= 0.1.
     F: n x k N(0, 0.1^2); D ~ U[0.01, 0.1]; mu ~ N(0.05, 0.02^2).
-    Returns the explicit-Q model and the factors (F, D, mu)."""
+    explicit=True stores Q = F F^T + D as a sparse matrix (scaled like any
+    sparse QP); explicit=False keeps it as the diagonal-plus-low-rank
+    operator (unscaled, as every operator Q).  Returns (model, factors)."""
     rng = np.random.default_rng(seed)
     F = rng.normal(0.0, 0.1, (n, k))
     Dg = rng.uniform(0.01, 0.1, n)
     mu = rng.normal(0.05, 0.02, n)
-    Q = sp.csc_matrix(F @ F.T + np.diag(Dg))
     A = sp.csr_matrix(np.ones((1, n)))
-    model = build_from_QAbc(Q, -mu, A, np.ones(1), np.ones(1), np.zeros(n), np.full(n, 0.1), verbose=False)
+    if explicit:
+        Q = sp.csc_matrix(F @ F.T + np.diag(Dg))
+        model = build_from_QAbc(Q, -mu, A, np.ones(1), np.ones(1), np.zeros(n), np.full(n, 0.1), verbose=False)
+    else:
+        model = QP_info_cpu(DiagonalPlusLowRank_Q_operator(Dg, F), -mu, sp.csc_matrix(A), np.ones(1), np.ones(1),
+                            np.zeros(n), np.full(n, 0.1))
     return model, dict(F=F, D=Dg, mu=mu)
+
+
+def portfolio_batch(count=4096, seed=0, n_range=(200, 5000), k_range=(10, 50)):
+    """The batch of long-only factor-model portfolio QPs: instance i has
+    n ~ U{n_range}, k ~ U{k_range} drawn from seed, data from seed + 1 + i."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_range[0], n_range[1] + 1, count)
+    ks = rng.integers(k_range[0], k_range[1] + 1, count)
+    return [portfolio(int(ns[i]), int(ks[i]), seed=seed + 1 + i, explicit=False) for i in range(count)]

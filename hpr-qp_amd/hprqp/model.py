@@ -85,6 +85,28 @@ class QAP_Q_operator(AbstractQOperator):
         return QX.reshape(-1, order="F")
 
 
+class DiagonalPlusLowRank_Q_operator(AbstractQOperator):
+    """Q = Diag(diag) + U U^T, U n x k: the custom operator of the reference's
+    operator guide (docs/src/guide/q_operators_overview.md:121-139)."""
+
+    name = "DiagonalPlusLowRank"
+
+    def __init__(self, diag, U):
+        self.diag = np.asarray(diag, dtype=np.float64)
+        self.U = np.asarray(U, dtype=np.float64)
+        if self.U.ndim != 2 or self.U.shape[0] != self.diag.size:
+            raise AssertionError("U must be n x k with n = length(diag)")
+
+    def get_problem_size(self):
+        return self.diag.size
+
+    def get_temp_size(self):
+        return self.U.shape[1]
+
+    def Qmap(self, x):
+        return self.diag * x + self.U @ (self.U.T @ x)
+
+
 QType = Union[sp.spmatrix, AbstractQOperator]
 
 

@@ -14,7 +14,8 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib
-from .model import AbstractQOperator, LASSO_Q_operator, QAP_Q_operator, QP_info_cpu
+from .model import (AbstractQOperator, DiagonalPlusLowRank_Q_operator, LASSO_Q_operator, QAP_Q_operator,
+                    QP_info_cpu)
 
 STATUS = {0: "OPTIMAL", 1: "MAX_ITER", 2: "TIME_LIMIT"}
 PROBLEM_TYPES = {"QP": 0, "LASSO": 1, "QAP": 2}
@@ -146,6 +147,12 @@ class DeviceQOperator:
             mats = [np.ascontiguousarray(M) for M in mats]
             keep += mats
             _lib.check(ctx.h, lib.hprqp_qop_create_qap(ctx.h, Q.n, *(_lib.ptr(M) for M in mats), C.byref(h)))
+        elif isinstance(Q, DiagonalPlusLowRank_Q_operator):
+            U = np.ascontiguousarray(Q.U.ravel(order="F"))
+            d = np.ascontiguousarray(Q.diag)
+            keep += [U, d]
+            _lib.check(ctx.h, lib.hprqp_qop_create_lowrank(ctx.h, Q.diag.size, Q.U.shape[1], _lib.ptr(U),
+                                                           _lib.ptr(d), C.byref(h)))
         elif isinstance(Q, AbstractQOperator):
             raise NotImplementedError("Python-side user operators are bound through hprqp_qop_create_user "
                                       "from native code; see INTEGRATION.md")

@@ -185,6 +185,12 @@ int hprqp_qop_create_lasso(hprqp_ctx *ctx, int32_t rows, int32_t n, int64_t nnz,
 int hprqp_qop_create_qap(hprqp_ctx *ctx, int32_t nq, const double *A,
                          const double *B, const double *S, const double *T,
                          hprqp_qop **out);
+/* Q = diag(d) + U U^T with U n x k column-major: the "diagonal plus low
+ * rank" custom operator of the reference's operator guide
+ * (docs/src/guide/q_operators_overview.md:121-139), here built in */
+int hprqp_qop_create_lowrank(hprqp_ctx *ctx, int32_t n, int32_t k,
+                             const double *U, const double *diag,
+                             hprqp_qop **out);
 /* user operator through a vtable of C function pointers */
 int hprqp_qop_create_user(hprqp_ctx *ctx, const hprqp_qop_vtable *vt,
                           void *user, hprqp_qop **out);
@@ -217,6 +223,44 @@ int hprqp_solve(hprqp_ctx *ctx, const hprqp_model *model,
                 const hprqp_params *params, const double *x0, const double *y0,
                 double *x_out, double *y_out, double *z_out, double *w_out,
                 double *hist, int64_t hist_cap, hprqp_result *result);
+
+/* ---- batched solves (many instances, one launch per check interval) ----
+ * Instance i: Q_i = diag(d_i) + U_i U_i^T (U_i n_i x k_i column-major),
+ * sparse A_i (m_i x n_i CSR), c_i, l_i, u_i, AL_i, AU_i.  Arrays are packed
+ * instance after instance: U at offsets sum_j n_j k_j, n-vectors at sum_j n_j,
+ * m-vectors at sum_j m_j, row pointers at sum_j (m_j + 1) holding offsets
+ * local to the instance's nonzeros (which are packed at sum_j nnz_j), column
+ * indices local to the instance.  Each instance runs the reference's
+ * iteration (solve, src/algorithm.jl:1904-2039) with its own restart and
+ * sigma state and retires when its relative KKT residual < stoptol.  No
+ * counterpart in the reference, which solves one instance per call. */
+typedef struct hprqp_batch hprqp_batch;
+typedef struct {
+    double time;              /* wall seconds, power method to last retirement */
+    double power_time;
+    int64_t solved;           /* instances that reached stoptol */
+    int64_t intervals;        /* check intervals launched */
+    int64_t max_iter_reached; /* iteration count of the last interval */
+    int64_t total_iter;       /* sum of per-instance iteration counts */
+} hprqp_batch_result;
+
+int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n,
+                       const int32_t *m, const int32_t *k, const double *U,
+                       const double *diag, const int32_t *a_rowptr,
+                       const int32_t *a_colidx, const double *a_vals,
+                       const double *c, const double *l, const double *u,
+                       const double *AL, const double *AU,
+                       const double *obj_constant, hprqp_batch **out);
+int hprqp_batch_destroy(hprqp_ctx *ctx, hprqp_batch *batch);
+/* precision: HPRQP_PREC_FP64 (all FP64) or HPRQP_PREC_MIXED (FP32 U, d, A,
+ * c, l, u, AL, AU; FP64 iterates, anchors, accumulation and KKT).  x0/y0
+ * optional packed warm starts; outputs optional, packed. */
+int hprqp_batch_solve(hprqp_ctx *ctx, hprqp_batch *batch,
+                      const hprqp_params *params, const double *x0,
+                      const double *y0, double *x_out, double *y_out,
+                      int32_t *status_out, int64_t *iter_out,
+                      double *obj_out, double *kkt_out,
+                      hprqp_batch_result *result);
 
 /* ---- measurement (determine_spmv_mode's timing loop,
  *      src/algorithm.jl:1557-1746) ------------------------------------------

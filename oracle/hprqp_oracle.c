@@ -195,6 +195,9 @@ typedef struct {
     int32_t nq;        /* QAP */
     const double *qa, *qb, *qs, *qt;
     double *qtemp;     /* QAP temp (nq*nq) */
+    int32_t lr_k;      /* diagonal plus low rank */
+    const double *lr_U, *lr_d;
+    double *lr_t;
 } qop_t;
 
 /* C = alpha * X * Y + beta * C, dense column-major nq x nq (BLAS dgemm). */
@@ -220,12 +223,22 @@ static void qmap(qop_t *q, const double *x, double *Qx) {
     } else if (q->kind == ORACLE_Q_LASSO) {
         csc_mul(&q->LA, x, q->ltemp);
         csc_mul(&q->LAT, q->ltemp, Qx);
-    } else {
+    } else if (q->kind == ORACLE_Q_QAP) {
         int32_t nq = q->nq;
         gemm_nn(nq, 1.0, q->qa, x, 0.0, q->qtemp);   /* TMP = A*X */
         gemm_nn(nq, 2.0, q->qtemp, q->qb, 0.0, Qx);  /* QX = 2*TMP*B */
         gemm_nn(nq, -2.0, q->qs, x, 1.0, Qx);        /* QX -= 2*S*X */
         gemm_nn(nq, -2.0, x, q->qt, 1.0, Qx);        /* QX -= 2*X*T */
+    } else {
+        /* Qx = diag .* x + U * (U' * x) (docs/src/guide/q_operators_overview.md:135-137) */
+        int32_t n = q->n, k = q->lr_k;
+        for (int32_t j = 0; j < k; ++j) q->lr_t[j] = dot(n, q->lr_U + (int64_t)j * n, x);
+        for (int32_t i = 0; i < n; ++i) Qx[i] = q->lr_d[i] * x[i];
+        for (int32_t j = 0; j < k; ++j) {
+            const double tj = q->lr_t[j];
+            const double *col = q->lr_U + (int64_t)j * n;
+            for (int32_t i = 0; i < n; ++i) Qx[i] += col[i] * tj;
+        }
     }
 }
 
@@ -247,6 +260,11 @@ static int qop_build(int32_t n, const oracle_qop *in, qop_t *q) {
         q->qs = in->qs;
         q->qt = in->qt;
         q->qtemp = dalloc((int64_t)n);
+    } else if (in->kind == ORACLE_Q_LOWRANK) {
+        q->lr_k = in->lr_k;
+        q->lr_U = in->lr_U;
+        q->lr_d = in->lr_d;
+        q->lr_t = dalloc(in->lr_k);
     } else {
         return -1;
     }
@@ -261,6 +279,7 @@ static void qop_free(qop_t *q) {
         free(q->ltemp);
     }
     if (q->kind == ORACLE_Q_QAP) free(q->qtemp);
+    if (q->kind == ORACLE_Q_LOWRANK) free(q->lr_t);
 }
 
 int oracle_qmap(int32_t n, const oracle_qop *in, const double *x, double *Qx) {

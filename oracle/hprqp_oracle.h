@@ -16,7 +16,7 @@
 extern "C" {
 #endif
 
-enum { ORACLE_Q_SPARSE = 0, ORACLE_Q_LASSO = 1, ORACLE_Q_QAP = 2 };
+enum { ORACLE_Q_SPARSE = 0, ORACLE_Q_LASSO = 1, ORACLE_Q_QAP = 2, ORACLE_Q_LOWRANK = 3 };
 enum { ORACLE_OPTIMAL = 0, ORACLE_MAX_ITER = 1, ORACLE_TIME_LIMIT = 2 };
 
 typedef struct {
@@ -35,6 +35,10 @@ typedef struct {
     /* ORACLE_Q_QAP: dense nq x nq column-major, Q(X) = 2(A X B - S X - X T) */
     int32_t nq;
     const double *qa, *qb, *qs, *qt;
+    /* ORACLE_Q_LOWRANK: Q = diag(d) + U U^T, U n x k column-major (the
+     * reference operator guide's DiagonalPlusLowRank custom operator) */
+    int32_t lr_k;
+    const double *lr_U, *lr_d;
 } oracle_qop;
 
 typedef struct {

@@ -16,7 +16,7 @@ import scipy.sparse as sp
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
 
-ORACLE_Q_SPARSE, ORACLE_Q_LASSO, ORACLE_Q_QAP = 0, 1, 2
+ORACLE_Q_SPARSE, ORACLE_Q_LASSO, ORACLE_Q_QAP, ORACLE_Q_LOWRANK = 0, 1, 2, 3
 STATUS = {0: "OPTIMAL", 1: "MAX_ITER", 2: "TIME_LIMIT"}
 
 
@@ -37,6 +37,9 @@ class oracle_qop(C.Structure):
         ("qb", C.c_void_p),
         ("qs", C.c_void_p),
         ("qt", C.c_void_p),
+        ("lr_k", C.c_int32),
+        ("lr_U", C.c_void_p),
+        ("lr_d", C.c_void_p),
     ]
 
 
@@ -144,6 +147,13 @@ def make_qop(Q, n, keep):
         q.kind = ORACLE_Q_QAP
         q.nq = Q.n
         q.qa, q.qb, q.qs, q.qt = (_p(M) for M in mats)
+    elif name == "DiagonalPlusLowRank_Q_operator":
+        U = np.ascontiguousarray(np.asarray(Q.U, dtype=np.float64).ravel(order="F"))
+        d = np.ascontiguousarray(Q.diag, dtype=np.float64)
+        keep += [U, d]
+        q.kind = ORACLE_Q_LOWRANK
+        q.lr_k = Q.U.shape[1]
+        q.lr_U, q.lr_d = _p(U), _p(d)
     else:
         cp, ri, v, M = _csc(Q)
         assert M.shape == (n, n)
