@@ -9,7 +9,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent.parent
 CSRC = PKG.parent / "csrc"
 INCLUDE = ROOT / "include"
-SOURCES = ["hprqp_solver.hip", "hprqp_capi.hip", "hprqp_qap.hip", "hprqp_lowrank.hip", "hprqp_batch.hip"]
+SOURCES = ["hprqp_solver.hip", "hprqp_capi.hip", "hprqp_qap.hip", "hprqp_lowrank.hip", "hprqp_batch.hip",
+           "hprqp_mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

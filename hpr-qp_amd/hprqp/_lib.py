@@ -141,6 +141,16 @@ SIGNATURES = [
     ("hprqp_batch_create", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
                                      C.POINTER(_P)]),
     ("hprqp_batch_destroy", C.c_int, [_P, _P]),
+    ("hprqp_mlp_create", C.c_int, [_P, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_P)]),
+    ("hprqp_mlp_destroy", C.c_int, [_P, _P]),
+    ("hprqp_mlp_param_count", C.c_int64, [_P]),
+    ("hprqp_mlp_get_params", C.c_int, [_P, _P, _P]),
+    ("hprqp_mlp_set_params", C.c_int, [_P, _P, _P]),
+    ("hprqp_mlp_forward", C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    ("hprqp_mlp_loss_grad", C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, C.c_float,
+                                      C.POINTER(C.c_double), _P]),
+    ("hprqp_mlp_train", C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P, _P, C.c_float, C.c_int32, C.c_int32,
+                                  C.c_float, C.c_uint64, _P]),
     ("hprqp_batch_solve", C.c_int, [_P, _P, C.POINTER(hprqp_params), _P, _P, _P, _P, _P, _P, _P, _P,
                                     C.POINTER(hprqp_batch_result)]),
 ]

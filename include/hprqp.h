@@ -262,6 +262,38 @@ int hprqp_batch_solve(hprqp_ctx *ctx, hprqp_batch *batch,
                       double *obj_out, double *kkt_out,
                       hprqp_batch_result *result);
 
+/* ---- learned warm start: per-variable MLP n_in -> h -> h -> h -> 2 (ReLU,
+ * FP32), shared across the variables of an instance.  Its predictions are
+ * passed as initial_x / initial_y (process_initial_points!,
+ * src/algorithm.jl:1783-1836); no counterpart in the reference, which only
+ * takes user-supplied warm starts.  Training data are (groups x group) rows:
+ * X row-major rows x n_in, tx one target per row (output 0), ty one target
+ * per group of `group` consecutive rows (mean of output 1 over the group).
+ * loss = mean_r (o_r0 - tx_r)^2 + wy * mean_g (mean_{r in g} o_r1 - ty_g)^2. */
+typedef struct hprqp_mlp hprqp_mlp;
+int hprqp_mlp_create(hprqp_ctx *ctx, int32_t n_in, int32_t hidden,
+                     uint64_t seed, hprqp_mlp **out);
+int hprqp_mlp_destroy(hprqp_ctx *ctx, hprqp_mlp *mlp);
+int64_t hprqp_mlp_param_count(const hprqp_mlp *mlp);
+/* parameters flattened as W1 (n_in x h, row-major), b1, W2, b2, W3, b3,
+ * W4 (h x 2), b4 */
+int hprqp_mlp_get_params(hprqp_ctx *ctx, const hprqp_mlp *mlp, float *host);
+int hprqp_mlp_set_params(hprqp_ctx *ctx, hprqp_mlp *mlp, const float *host);
+/* out: rows x 2 */
+int hprqp_mlp_forward(hprqp_ctx *ctx, hprqp_mlp *mlp, int64_t rows,
+                      const float *X, float *out);
+/* loss and its gradient (flattened like the parameters) on one batch */
+int hprqp_mlp_loss_grad(hprqp_ctx *ctx, hprqp_mlp *mlp, int64_t groups,
+                        int32_t group, const float *X, const float *tx,
+                        const float *ty, float wy, double *loss, float *grad);
+/* Adam over shuffled minibatches of batch_groups groups; loss_history
+ * (optional) receives the mean loss of each epoch */
+int hprqp_mlp_train(hprqp_ctx *ctx, hprqp_mlp *mlp, int64_t groups,
+                    int32_t group, const float *X, const float *tx,
+                    const float *ty, float wy, int32_t batch_groups,
+                    int32_t epochs, float lr, uint64_t seed,
+                    double *loss_history);
+
 /* ---- measurement (determine_spmv_mode's timing loop,
  *      src/algorithm.jl:1557-1746) ------------------------------------------
  * Prepares the solver state for `model` exactly as hprqp_solve does (scaling,
