@@ -40,6 +40,63 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def batched_leg(count, plain_sample):
+    """The batched workload: `count` long-only factor-model portfolio QPs
+    (n ~ U{200..5000}, k ~ U{10..50}, instance seeds 1..count).  Reports
+    solves/s of (a) the plain FP64 form (one instance per call, cold,
+    sequential, model resident in HBM; timed on the first `plain_sample`
+    instances), (b) batched mixed cold, (c) batched mixed warm-started by the
+    learned MLP (inference time included).  Every solve must reach 1e-6
+    relative KKT."""
+    import numpy as np
+
+    from hprqp import DeviceModel, HPRQP_parameters, instances
+    from hprqp.batch import BatchModel
+    from hprqp.warmstart import MLP
+
+    t = time.time()
+    insts = instances.portfolio_batch(count, seed=0)
+    models = [m for m, _ in insts]
+    log(f"batched: {count} instances generated in {time.time() - t:.1f}s")
+    plain = HPRQP_parameters(verbose=False)
+    dms = [DeviceModel(m) for m in models[:plain_sample]]
+    dms[0].solve(plain)
+    t = time.perf_counter()
+    ok = 0
+    for dm in dms:
+        r = dm.solve(plain)
+        ok += r.status == "OPTIMAL" and r.residuals < 1e-6
+    plain_sps = ok / (time.perf_counter() - t)
+    for dm in dms:
+        dm.close()
+    bm = BatchModel(models)
+    mixed = HPRQP_parameters(verbose=False, precision="mixed")
+    bm.solve(mixed)  # warm-up (code objects, graph)
+    t = time.perf_counter()
+    cold = bm.solve(mixed)
+    cold_s = time.perf_counter() - t
+    net = MLP.load(str(ROOT / "hpr-qp_amd" / "hprqp" / "data" / "warm_mlp.npz"))
+    t = time.perf_counter()
+    x0, y0 = net.predict_batch(models)
+    t_pred = time.perf_counter() - t
+    warm = bm.solve(mixed, x0=x0, y0=y0)
+    warm_s = time.perf_counter() - t
+    cold_ok = int(np.sum((np.array(cold.status) == "OPTIMAL") & (cold.residuals < 1e-6)))
+    warm_ok = int(np.sum((np.array(warm.status) == "OPTIMAL") & (warm.residuals < 1e-6)))
+    out = {"workload": f"{count} portfolio QPs, n~U{{200..5000}}, k~U{{10..50}}, 1e-6 relative KKT",
+           "plain_fp64_solves_per_s": plain_sps, "plain_sample": plain_sample,
+           "batched_mixed_cold_solves_per_s": cold_ok / cold_s,
+           "batched_mixed_warm_solves_per_s": warm_ok / warm_s,
+           "warm_vs_plain": (warm_ok / warm_s) / plain_sps,
+           "cold_mean_iter": float(cold.iter.mean()), "warm_mean_iter": float(warm.iter.mean()),
+           "cold_max_iter": int(cold.iter.max()), "warm_max_iter": int(warm.iter.max()),
+           "mlp_inference_s": t_pred, "all_solved": cold_ok == count and warm_ok == count}
+    log("batched: " + json.dumps(out))
+    bm.close()
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,6 +108,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=40)
     ap.add_argument("--cpu-power-iters", type=int, default=100)
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",
@@ -139,6 +199,7 @@ def main():
                    "primal_obj": r0.primal_obj, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": None if args.no_batch else batched_leg(args.batch_count, args.plain_sample),
     }
     print(json.dumps(line), flush=True)
 
