@@ -51,6 +51,7 @@ enum {
 template <class MT>
 struct BArgs {
     int32_t count;
+    const int32_t *order;  // blockIdx -> instance (largest first)
     const int32_t *n, *m, *k;
     const int64_t *noff, *moff, *uoff, *arp_off, *atrp_off;
     const MT *U, *d;            // iteration precision
@@ -459,7 +460,7 @@ template <class MT>
 __global__ __launch_bounds__(kBlock) void k_batch_interval(BArgs<MT> a) {
     __shared__ double t[64];
     __shared__ double sh[4];
-    const int b = blockIdx.x;
+    const int b = a.order[blockIdx.x];
     const BCtl c = a.ctl[b];
     if (!c.active) return;
     Inst<MT> I = inst_of(a, b);
@@ -491,7 +492,7 @@ template <class MT>
 __global__ __launch_bounds__(kBlock) void k_batch_init(BArgs<MT> a, int has_x0, int has_y0) {
     __shared__ double t[64];
     __shared__ double sh[4];
-    const int b = blockIdx.x;
+    const int b = a.order[blockIdx.x];
     const BCtl c = a.ctl[b];
     if (!c.active) return;
     Inst<MT> I = inst_of(a, b);
@@ -535,7 +536,7 @@ template <class MT>
 __global__ __launch_bounds__(kBlock) void k_batch_power(BArgs<MT> a, double tol, int max_it) {
     __shared__ double t[64];
     __shared__ double sh[4];
-    const int b = blockIdx.x;
+    const int b = a.order[blockIdx.x];
     if (!a.ctl[b].active) return;
     Inst<MT> I = inst_of(a, b);
     double *rec = a.rec + (int64_t)b * R_COUNT;
@@ -666,6 +667,7 @@ template <class MT>
 static BArgs<MT> make_args(hprqp_batch *B, BatchWork &W) {
     BArgs<MT> a{};
     a.count = B->count;
+    a.order = B->dorder.p;
     a.n = B->dn.p;
     a.m = B->dm.p;
     a.k = B->dk.p;
@@ -733,7 +735,7 @@ struct BInstState {
 
 template <class MT>
 static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params *p, const double *x0,
-                             const double *y0, double *x_out, double *y_out, int32_t *status_out, int64_t *iter_out,
+                             const double *y0, const double *x0_dev, const double *y0_dev, double *x_out, double *y_out, int32_t *status_out, int64_t *iter_out,
                              double *obj_out, double *kkt_out, hprqp_batch_result *res) {
     hipStream_t s = ctx->stream;
     const int32_t cnt = B->count;
@@ -810,10 +812,16 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
     }
     res->power_time = bnow() - t_start;
     // warm start: x0 / y0 go to x_bar / y_bar, the init kernel spreads them
-    if (x0) HPRQP_HIP(hipMemcpyAsync(a.x_bar, x0, sizeof(double) * N, hipMemcpyHostToDevice, s));
-    if (y0 && M) HPRQP_HIP(hipMemcpyAsync(a.y_bar, y0, sizeof(double) * M, hipMemcpyHostToDevice, s));
+    if (x0_dev)
+        HPRQP_HIP(hipMemcpyAsync(a.x_bar, x0_dev, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    else if (x0)
+        HPRQP_HIP(hipMemcpyAsync(a.x_bar, x0, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    if (y0_dev && M)
+        HPRQP_HIP(hipMemcpyAsync(a.y_bar, y0_dev, sizeof(double) * M, hipMemcpyDeviceToDevice, s));
+    else if (y0 && M)
+        HPRQP_HIP(hipMemcpyAsync(a.y_bar, y0, sizeof(double) * M, hipMemcpyHostToDevice, s));
     put_ctl();
-    k_batch_init<MT><<<cnt, kBlock, 0, s>>>(a, x0 != nullptr, y0 != nullptr);
+    k_batch_init<MT><<<cnt, kBlock, 0, s>>>(a, x0 != nullptr || x0_dev != nullptr, y0 != nullptr || y0_dev != nullptr);
     HPRQP_HIP(hipGetLastError());
     get_rec();
     // one check interval = ctl upload + interval kernel + record download,
@@ -971,14 +979,16 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
 }
 
 void batch_solve(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params *p, const double *x0, const double *y0,
-                 double *x_out, double *y_out, int32_t *status_out, int64_t *iter_out, double *obj_out,
-                 double *kkt_out, hprqp_batch_result *res) {
+                 const double *x0_dev, const double *y0_dev, double *x_out, double *y_out, int32_t *status_out,
+                 int64_t *iter_out, double *obj_out, double *kkt_out, hprqp_batch_result *res) {
     std::memset(res, 0, sizeof(*res));
     HPRQP_REQUIRE(p->check_iter > 0, HPRQP_E_ARG, "check_iter must be positive");
     if (p->precision == HPRQP_PREC_FP64)
-        batch_solve_impl<double>(ctx, B, p, x0, y0, x_out, y_out, status_out, iter_out, obj_out, kkt_out, res);
+        batch_solve_impl<double>(ctx, B, p, x0, y0, x0_dev, y0_dev, x_out, y_out, status_out, iter_out, obj_out,
+                                 kkt_out, res);
     else
-        batch_solve_impl<float>(ctx, B, p, x0, y0, x_out, y_out, status_out, iter_out, obj_out, kkt_out, res);
+        batch_solve_impl<float>(ctx, B, p, x0, y0, x0_dev, y0_dev, x_out, y_out, status_out, iter_out, obj_out,
+                                kkt_out, res);
 }
 
 }  // namespace hprqp

@@ -2,6 +2,7 @@
 // Each entry catches every exception and turns it into a status code plus a
 // per-context message; nothing is thrown across the ABI.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -696,6 +697,16 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
             up(B->u, u, N);
             up(B->AL, AL, M);
             up(B->AU, AU, M);
+            // largest-first dispatch: per-iteration work of an instance is
+            // about four passes over U plus the A and A^T products
+            std::vector<int32_t> order(count);
+            std::vector<double> cost(count);
+            for (int32_t b = 0; b < count; ++b) {
+                order[b] = b;
+                cost[b] = 4.0 * (double)n[b] * (k[b] + 1) + 2.0 * (arp[arp_off[b] + m[b]] - nnz_off[b]) + m[b];
+            }
+            std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+            up(B->dorder, order.data(), count);
             HPRQP_HIP(hipStreamSynchronize(s));
             up32(B->U32, U, UT);
             up32(B->d32, diag, N);
@@ -733,7 +744,43 @@ int hprqp_batch_solve(hprqp_ctx *ctx, hprqp_batch *batch, const hprqp_params *pa
         params = &def;
     }
     return guarded(ctx, [&] {
-        hprqp::batch_solve(ctx, batch, params, x0, y0, x_out, y_out, status_out, iter_out, obj_out, kkt_out, result);
+        hprqp::batch_solve(ctx, batch, params, x0, y0, nullptr, nullptr, x_out, y_out, status_out, iter_out, obj_out,
+                           kkt_out, result);
+    });
+}
+
+int hprqp_batch_predict(hprqp_ctx *ctx, hprqp_batch *batch, hprqp_mlp *mlp, const float *feat_mean,
+                        const float *feat_std, double y_scale, double *x0_out, double *y0_out) {
+    CTX_CHECK(ctx);
+    if (!batch || !mlp || !feat_mean || !feat_std) return HPRQP_E_ARG;
+    return guarded(ctx, [&] {
+        hprqp::DevVec<double> x0(std::max<int64_t>(batch->Ntot, 1)), y0(std::max<int64_t>(batch->Mtot, 1));
+        hprqp::batch_predict(ctx, batch, mlp, feat_mean, feat_std, y_scale, x0.p, y0.p);
+        if (x0_out) HPRQP_HIP(hipMemcpy(x0_out, x0.p, sizeof(double) * batch->Ntot, hipMemcpyDeviceToHost));
+        if (y0_out && batch->Mtot)
+            HPRQP_HIP(hipMemcpy(y0_out, y0.p, sizeof(double) * batch->Mtot, hipMemcpyDeviceToHost));
+    });
+}
+
+int hprqp_batch_solve_warm(hprqp_ctx *ctx, hprqp_batch *batch, hprqp_mlp *mlp, const float *feat_mean,
+                           const float *feat_std, double y_scale, const hprqp_params *params, double *x_out,
+                           double *y_out, int32_t *status_out, int64_t *iter_out, double *obj_out, double *kkt_out,
+                           hprqp_batch_result *result) {
+    CTX_CHECK(ctx);
+    if (!batch || !mlp || !feat_mean || !feat_std || !result) return HPRQP_E_ARG;
+    hprqp_params def;
+    if (!params) {
+        hprqp_params_default(&def);
+        params = &def;
+    }
+    return guarded(ctx, [&] {
+        const auto t0 = std::chrono::steady_clock::now();
+        hprqp::DevVec<double> x0(std::max<int64_t>(batch->Ntot, 1)), y0(std::max<int64_t>(batch->Mtot, 1));
+        hprqp::batch_predict(ctx, batch, mlp, feat_mean, feat_std, y_scale, x0.p, y0.p);
+        const double tp = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        hprqp::batch_solve(ctx, batch, params, nullptr, nullptr, x0.p, y0.p, x_out, y_out, status_out, iter_out,
+                           obj_out, kkt_out, result);
+        result->predict_time = tp;
     });
 }
 

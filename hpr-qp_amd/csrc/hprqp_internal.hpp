@@ -168,12 +168,37 @@ struct hprqp_batch {
     hprqp::DevVec<double> U, d, a_v, at_v, c, l, u, AL, AU;
     hprqp::DevVec<float> U32, d32, a_v32, at_v32, c32, l32, u32, AL32, AU32;
     hprqp::DevVec<int32_t> a_rp, a_ci, at_rp, at_ci;
+    // dispatch order of the per-instance workgroups: instances sorted by
+    // estimated per-iteration cost, largest first, so the big ones start
+    // early and the small ones fill in behind them
+    hprqp::DevVec<int32_t> dorder;
+};
+
+struct hprqp_mlp {
+    int32_t nin = 0, hid = 0, nout = 2;
+    int64_t np = 0;            // parameter count
+    int64_t off_W[4], off_b[4];
+    int32_t in_w[4], out_w[4];
+    hprqp::DevVec<float> p, g, am, av;
+    int64_t adam_t = 0;
+    int64_t cap = 0;           // activation capacity in rows
+    hprqp::DevVec<float> act[5], dact[5];
+    hprqp::DevVec<double> part;
 };
 
 namespace hprqp {
+// x0_dev / y0_dev: optional packed warm starts already on the device
+// (taken instead of the host x0 / y0 when given)
 void batch_solve(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params *p, const double *x0, const double *y0,
-                 double *x_out, double *y_out, int32_t *status_out, int64_t *iter_out, double *obj_out,
-                 double *kkt_out, hprqp_batch_result *res);
+                 const double *x0_dev, const double *y0_dev, double *x_out, double *y_out, int32_t *status_out,
+                 int64_t *iter_out, double *obj_out, double *kkt_out, hprqp_batch_result *res);
+// MLP forward of `rows` device rows X (rows x nin) into device out (rows x 2)
+void mlp_forward_dev(hipStream_t s, hprqp_mlp *M, const float *X, int64_t rows, float *out);
+// learned warm start of every instance of a batch, computed on the device
+// (hprqp_warm.hip); x0 (Ntot) and y0 (Mtot) are device arrays
+void batch_predict(hprqp_ctx *ctx, hprqp_batch *B, hprqp_mlp *M, const float *feat_mean, const float *feat_std,
+                   double y_scale, double *x0, double *y0);
+constexpr int kWarmFeatures = 12;
 // solver entry (hprqp_solver.hip)
 void solve(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, const double *x0,
            const double *y0, double *x_out, double *y_out, double *z_out, double *w_out,

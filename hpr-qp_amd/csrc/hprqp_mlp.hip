@@ -182,18 +182,6 @@ static inline int nb256(int64_t n) { return (int)((n + 255) / 256); }
 
 using hprqp::DevVec;
 
-struct hprqp_mlp {
-    int32_t nin = 0, hid = 0, nout = 2;
-    int64_t np = 0;            // parameter count
-    int64_t off_W[4], off_b[4];
-    int32_t in_w[4], out_w[4];
-    DevVec<float> p, g, am, av;
-    int64_t adam_t = 0;
-    int64_t cap = 0;           // activation capacity in rows
-    DevVec<float> act[5], dact[5];
-    DevVec<double> part;
-};
-
 namespace hprqp {
 
 static void mlp_layout(hprqp_mlp *M) {
@@ -221,14 +209,25 @@ static void mlp_reserve(hprqp_mlp *M, int64_t rows) {
     M->cap = rows;
 }
 
-// forward on rows already in act[0]
-static void mlp_forward(hipStream_t s, hprqp_mlp *M, int64_t rows) {
+// forward on rows already in act[0] (or in X when given)
+static void mlp_forward(hipStream_t s, hprqp_mlp *M, int64_t rows, const float *X = nullptr) {
     for (int l = 0; l < 4; ++l) {
-        sgemm<false, false>(s, (int)rows, M->out_w[l], M->in_w[l], M->act[l].p, M->in_w[l], M->p.p + M->off_W[l],
+        const float *in = (l == 0 && X) ? X : M->act[l].p;
+        sgemm<false, false>(s, (int)rows, M->out_w[l], M->in_w[l], in, M->in_w[l], M->p.p + M->off_W[l],
                             M->out_w[l], M->act[l + 1].p, M->out_w[l], 0.f);
         k_bias_act<<<nb256(rows * M->out_w[l]), 256, 0, s>>>(rows, M->out_w[l], M->act[l + 1].p,
                                                              M->p.p + M->off_b[l], l < 3);
         HPRQP_HIP(hipGetLastError());
+    }
+}
+
+void mlp_forward_dev(hipStream_t s, hprqp_mlp *M, const float *X, int64_t rows, float *out) {
+    const int64_t chunk = 1 << 20;
+    mlp_reserve(M, std::min(rows, chunk));
+    for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+        const int64_t r = std::min(chunk, rows - r0);
+        mlp_forward(s, M, r, X + r0 * M->nin);
+        HPRQP_HIP(hipMemcpyAsync(out + 2 * r0, M->act[4].p, sizeof(float) * 2 * r, hipMemcpyDeviceToDevice, s));
     }
 }
 

@@ -10,7 +10,7 @@ ROOT = PKG.parent.parent
 CSRC = PKG.parent / "csrc"
 INCLUDE = ROOT / "include"
 SOURCES = ["hprqp_solver.hip", "hprqp_capi.hip", "hprqp_qap.hip", "hprqp_lowrank.hip", "hprqp_batch.hip",
-           "hprqp_mlp.hip"]
+           "hprqp_mlp.hip", "hprqp_warm.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

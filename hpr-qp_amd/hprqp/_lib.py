@@ -89,6 +89,7 @@ class hprqp_batch_result(C.Structure):
         ("intervals", C.c_int64),
         ("max_iter_reached", C.c_int64),
         ("total_iter", C.c_int64),
+        ("predict_time", C.c_double),
     ]
 
 
@@ -153,6 +154,9 @@ SIGNATURES = [
                                   C.c_float, C.c_uint64, _P]),
     ("hprqp_batch_solve", C.c_int, [_P, _P, C.POINTER(hprqp_params), _P, _P, _P, _P, _P, _P, _P, _P,
                                     C.POINTER(hprqp_batch_result)]),
+    ("hprqp_batch_predict", C.c_int, [_P, _P, _P, _P, _P, C.c_double, _P, _P]),
+    ("hprqp_batch_solve_warm", C.c_int, [_P, _P, _P, _P, _P, C.c_double, C.POINTER(hprqp_params), _P, _P, _P, _P,
+                                         _P, _P, C.POINTER(hprqp_batch_result)]),
 ]
 
 _lib = None

@@ -31,6 +31,7 @@ class BatchResults:
     solved: int
     intervals: int
     total_iter: int
+    predict_time: float = 0.0
 
 
 class BatchModel:
@@ -78,11 +79,34 @@ class BatchModel:
     def pack_y(self, ys):
         return np.ascontiguousarray(np.concatenate(ys) if len(ys) else np.zeros(0), dtype=np.float64)
 
-    def solve(self, params: HPRQP_parameters = None, x0=None, y0=None) -> BatchResults:
+    def predict(self, net):
+        """Learned warm start of every instance, computed on the device by
+        hprqp_batch_predict; returns lists (x0_i, y0_i) like
+        MLP.predict_batch does on the host."""
+        from .warmstart import Y_SCALE
+
+        N, M = int(self.noff[-1]), int(self.moff[-1])
+        x = np.zeros(max(N, 1))
+        y = np.zeros(max(M, 1))
+        mean = np.ascontiguousarray(net.feat_mean, dtype=np.float32)
+        std = np.ascontiguousarray(net.feat_std, dtype=np.float32)
+        P = _lib.ptr
+        _lib.check(self.ctx.h, self.ctx.lib.hprqp_batch_predict(self.ctx.h, self.h, net.h, P(mean), P(std),
+                                                                Y_SCALE, P(x), P(y)))
+        return ([x[self.noff[i]:self.noff[i + 1]].copy() for i in range(self.count)],
+                [y[self.moff[i]:self.moff[i + 1]].copy() for i in range(self.count)])
+
+    def solve(self, params: HPRQP_parameters = None, x0=None, y0=None, warm=None) -> BatchResults:
+        """Solve every instance.  x0 / y0: optional warm starts (lists per
+        instance or packed arrays).  warm: an MLP whose prediction, computed
+        on the device, is the warm start (hprqp_batch_solve_warm); the
+        prediction time is in BatchResults.predict_time."""
         params = params or HPRQP_parameters(verbose=False)
         p = params.to_c()
         cnt = self.count
         N, M = int(self.noff[-1]), int(self.moff[-1])
+        if warm is not None and (x0 is not None or y0 is not None):
+            raise ValueError("give either x0 / y0 or warm, not both")
         x0p = None if x0 is None else (self.pack_x(x0) if isinstance(x0, (list, tuple)) else np.ascontiguousarray(x0))
         y0p = None if y0 is None else (self.pack_y(y0) if isinstance(y0, (list, tuple)) else np.ascontiguousarray(y0))
         if x0p is not None and x0p.size != N:
@@ -97,12 +121,22 @@ class BatchModel:
         kkt = np.zeros(cnt)
         r = _lib.hprqp_batch_result()
         P = _lib.ptr
-        _lib.check(self.ctx.h, self.ctx.lib.hprqp_batch_solve(
-            self.ctx.h, self.h, C.byref(p), P(x0p), P(y0p), P(x), P(y), P(st), P(it), P(obj), P(kkt), C.byref(r)))
+        if warm is None:
+            _lib.check(self.ctx.h, self.ctx.lib.hprqp_batch_solve(
+                self.ctx.h, self.h, C.byref(p), P(x0p), P(y0p), P(x), P(y), P(st), P(it), P(obj), P(kkt),
+                C.byref(r)))
+        else:
+            from .warmstart import Y_SCALE
+
+            mean = np.ascontiguousarray(warm.feat_mean, dtype=np.float32)
+            std = np.ascontiguousarray(warm.feat_std, dtype=np.float32)
+            _lib.check(self.ctx.h, self.ctx.lib.hprqp_batch_solve_warm(
+                self.ctx.h, self.h, warm.h, P(mean), P(std), Y_SCALE, C.byref(p), P(x), P(y), P(st), P(it), P(obj),
+                P(kkt), C.byref(r)))
         xs = [x[self.noff[i]:self.noff[i + 1]].copy() for i in range(cnt)]
         ys = [y[self.moff[i]:self.moff[i + 1]].copy() for i in range(cnt)]
         return BatchResults([STATUS.get(int(s), str(int(s))) for s in st], it, obj, kkt, xs, ys, r.time,
-                            r.power_time, r.solved, r.intervals, r.total_iter)
+                            r.power_time, r.solved, r.intervals, r.total_iter, r.predict_time)
 
     def close(self):
         if self.h:

@@ -242,6 +242,8 @@ typedef struct {
     int64_t intervals;        /* check intervals launched */
     int64_t max_iter_reached; /* iteration count of the last interval */
     int64_t total_iter;       /* sum of per-instance iteration counts */
+    double predict_time;      /* hprqp_batch_solve_warm: seconds computing the
+                                 learned warm start (not part of time) */
 } hprqp_batch_result;
 
 int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n,
@@ -293,6 +295,31 @@ int hprqp_mlp_train(hprqp_ctx *ctx, hprqp_mlp *mlp, int64_t groups,
                     const float *ty, float wy, int32_t batch_groups,
                     int32_t epochs, float lr, uint64_t seed,
                     double *loss_history);
+
+/* Learned warm start of every instance of a batch, computed on the device:
+ * the 12 per-variable features below, normalised as (f - feat_mean) /
+ * feat_std, through the MLP (n_in = 12), then x0_i = clamp(o_i0, 0, 1) u_i
+ * and y0 = mean_i(o_i1) / y_scale on each of the instance's rows (u_i := 1
+ * where infinite).  Features of variable i (cm, cs: mean and population
+ * standard deviation of the instance's c, cs += 1e-12; ubar: column means of
+ * U; q_ii = d_i + |U_i.|^2; r_i = -c_i / q_ii; rank(v)_i = #{j: v_j < v_i}
+ * + #{j < i: v_j == v_i}, over max(n - 1, 1)):
+ *   (c_i - cm)/cs, 20 c_i, 10 d_i, 3 (q_ii - d_i), 30 U_i.ubar, 0.5 r_i,
+ *   rank(c)_i, rank(-r)_i, 100/n, k/50, min(n u_i, 100)/100, 50 cs.
+ * hprqp_batch_predict returns the packed (x0, y0) to the host (either may be
+ * NULL); hprqp_batch_solve_warm feeds them to the solve without leaving the
+ * device and reports the prediction time in result->predict_time.  No
+ * counterpart in the reference. */
+int hprqp_batch_predict(hprqp_ctx *ctx, hprqp_batch *batch, hprqp_mlp *mlp,
+                        const float *feat_mean, const float *feat_std,
+                        double y_scale, double *x0_out, double *y0_out);
+int hprqp_batch_solve_warm(hprqp_ctx *ctx, hprqp_batch *batch,
+                           hprqp_mlp *mlp, const float *feat_mean,
+                           const float *feat_std, double y_scale,
+                           const hprqp_params *params, double *x_out,
+                           double *y_out, int32_t *status_out,
+                           int64_t *iter_out, double *obj_out,
+                           double *kkt_out, hprqp_batch_result *result);
 
 /* ---- measurement (determine_spmv_mode's timing loop,
  *      src/algorithm.jl:1557-1746) ------------------------------------------

@@ -70,3 +70,51 @@ def test_learned_warm_start_gain_on_held_out_batch():
     for i in range(len(test)):
         assert abs(warm.primal_obj[i] - cold.primal_obj[i]) / (1 + abs(cold.primal_obj[i])) < 1e-5
     assert warm.iter.sum() < cold.iter.sum()
+
+
+def _trained():
+    from pathlib import Path
+
+    from hprqp.warmstart import MLP
+
+    return MLP.load(str(Path(__file__).resolve().parents[1] / "hpr-qp_amd" / "hprqp" / "data" / "warm_mlp.npz"))
+
+
+def test_device_prediction_matches_host_features_path():
+    """hprqp_batch_predict (features, MLP and output map on the device) against
+    the host path (numpy features, then the HIP MLP)."""
+    from hprqp import instances
+    from hprqp.batch import BatchModel
+
+    net = _trained()
+    insts = instances.portfolio_batch(12, seed=321, n_range=(30, 900), k_range=(2, 40))
+    models = [m for m, _ in insts]
+    bm = BatchModel(models)
+    hx, hy = net.predict_batch(models)
+    dx, dy = bm.predict(net)
+    for i in range(len(models)):
+        np.testing.assert_allclose(dx[i], hx[i], rtol=0, atol=2e-6 * max(1.0, np.abs(hx[i]).max()))
+        np.testing.assert_allclose(dy[i], hy[i], rtol=1e-5, atol=1e-7)
+
+
+def test_solve_warm_on_device_matches_host_warm_start():
+    from hprqp import HPRQP_parameters, instances
+    from hprqp.batch import BatchModel
+
+    net = _trained()
+    insts = instances.portfolio_batch(48, seed=654, n_range=(200, 1500), k_range=(10, 50))
+    models = [m for m, _ in insts]
+    bm = BatchModel(models)
+    params = HPRQP_parameters(verbose=False, precision="mixed")
+    cold = bm.solve(params)
+    x0, y0 = net.predict_batch(models)
+    host = bm.solve(params, x0=x0, y0=y0)
+    dev = bm.solve(params, warm=net)
+    print(f"cold iters {cold.iter.sum()} host-warm {host.iter.sum()} device-warm {dev.iter.sum()} "
+          f"predict {dev.predict_time:.3f}s")
+    assert dev.solved == len(models) and np.all(dev.residuals < 1e-6)
+    assert dev.predict_time > 0
+    assert abs(int(dev.iter.sum()) - int(host.iter.sum())) <= 0.02 * host.iter.sum()
+    assert dev.iter.sum() < cold.iter.sum()
+    for i in range(len(models)):
+        assert abs(dev.primal_obj[i] - cold.primal_obj[i]) / (1 + abs(cold.primal_obj[i])) < 1e-5
