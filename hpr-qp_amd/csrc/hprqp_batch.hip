@@ -65,7 +65,7 @@ struct BArgs {
         *dQw;
     double *y, *y_bar, *s, *dy;
     double *last_x, *last_w, *last_y, *last_ATy;
-    double *pq, *pz;  // power method scratch (n-packed and m-packed)
+    double *pq, *pz, *pq2, *pz2;  // eigenvalue scratch (n-packed and m-packed)
     BCtl *ctl;
     double *rec;
 };
@@ -177,7 +177,7 @@ struct Inst {
     const MT *cM, *lM, *uM, *ALM, *AUM;
     double *x, *w, *w_bar, *x_hat, *x_bar, *z_bar, *dx, *dw, *Qw, *Qw_bar, *tempv, *ATy, *ATy_bar, *ATdy, *QATdy,
         *dQw;
-    double *y, *y_bar, *s, *dy, *last_x, *last_w, *last_y, *last_ATy, *pq, *pz;
+    double *y, *y_bar, *s, *dy, *last_x, *last_w, *last_y, *last_ATy, *pq, *pz, *pq2, *pz2;
 };
 
 template <class MT>
@@ -211,7 +211,7 @@ __device__ Inst<MT> inst_of(const BArgs<MT> &a, int b) {
     I.AUM = a.AUM + mo;
 #define P(f) I.f = a.f + no
     P(x); P(w); P(w_bar); P(x_hat); P(x_bar); P(z_bar); P(dx); P(dw); P(Qw); P(Qw_bar); P(tempv); P(ATy);
-    P(ATy_bar); P(ATdy); P(QATdy); P(dQw); P(last_x); P(last_w); P(last_ATy); P(pq);
+    P(ATy_bar); P(ATdy); P(QATdy); P(dQw); P(last_x); P(last_w); P(last_ATy); P(pq); P(pq2);
 #undef P
     I.y = a.y + mo;
     I.y_bar = a.y_bar + mo;
@@ -219,6 +219,7 @@ __device__ Inst<MT> inst_of(const BArgs<MT> &a, int b) {
     I.dy = a.dy + mo;
     I.last_y = a.last_y + mo;
     I.pz = a.pz + mo;
+    I.pz2 = a.pz2 + mo;
     return I;
 }
 
@@ -530,8 +531,139 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(BArgs<MT> a, int has_x0, 
     bresid(I, true, rec, t, sh);
 }
 
-// per-instance power method (src/utils.jl:1335-1461) in one launch: each
-// block iterates to its own tolerance; start vectors were written by the host
+// ---------------- lambda_max by Lanczos ----------------------------------
+// The reference estimates lambda_max(A A^T) and lambda_max(Q) with the power
+// method (src/utils.jl:1335-1461: start randn + 1e-8, stop when
+// |Op q - lambda q| / (|Op q| + lambda) < 1e-4, at most 5000 steps).  Batched
+// instances are often factor models whose top eigenvalues are close, where
+// the power method needs thousands of steps; here each block runs Lanczos
+// (no reorthogonalisation) from the same start vector and applies the same
+// test to the top Ritz pair (theta, y): |Op y - theta y| = beta_m |s_m|, s
+// the eigenvector of the tridiagonal T_m.  theta is found by multisection
+// with Sturm counts (all threads), s_m by the three-term recurrence.
+constexpr int kLzMax = 512;  // Lanczos steps kept (LDS)
+
+__device__ __forceinline__ int sturm_below(const double *al, const double *be, int m, double x) {
+    double d = al[0] - x;
+    int c = d < 0.0;
+    for (int i = 1; i < m; ++i) {
+        if (d == 0.0) d = -1e-300;
+        d = al[i] - x - be[i - 1] * be[i - 1] / d;
+        c += d < 0.0;
+    }
+    return c;
+}
+
+// largest eigenvalue of the m x m tridiagonal (al, be), lo a lower bound
+__device__ double tri_max(const double *al, const double *be, int m, double lo, int *imin) {
+    double hi = -INFINITY;
+    for (int i = 0; i < m; ++i) {
+        lo = fmax(lo, al[i]);
+        hi = fmax(hi, al[i] + (i > 0 ? fabs(be[i - 1]) : 0.0) + (i + 1 < m ? fabs(be[i]) : 0.0));
+    }
+    for (int round = 0; round < 10 && hi - lo > 1e-15 * fmax(fabs(hi), fabs(lo)); ++round) {
+        const double x = lo + (hi - lo) * (double)(threadIdx.x + 1) / (double)(kBlock + 1);
+        const bool above = sturm_below(al, be, m, x) == m;
+        __syncthreads();
+        if (threadIdx.x == 0) *imin = kBlock;
+        __syncthreads();
+        if (above) atomicMin(imin, (int)threadIdx.x);
+        __syncthreads();
+        const int t = *imin;
+        const double w = (hi - lo) / (double)(kBlock + 1);
+        const double nlo = lo + w * (double)t;
+        const double nhi = t < kBlock ? lo + w * (double)(t + 1) : hi;
+        lo = nlo;
+        hi = nhi;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// |last component| of the unit eigenvector of T_m for eigenvalue theta
+__device__ double tri_last(const double *al, const double *be, int m, double theta) {
+    double xp = 0.0, xc = 1.0, ss = 1.0;
+    for (int i = 0; i + 1 < m; ++i) {
+        if (be[i] == 0.0) return 0.0;
+        const double xn = -((al[i] - theta) * xc + (i > 0 ? be[i - 1] * xp : 0.0)) / be[i];
+        xp = xc;
+        xc = xn;
+        ss += xn * xn;
+        if (fabs(xc) > 1e100) {
+            xp *= 1e-100;
+            xc *= 1e-100;
+            ss *= 1e-200;
+        }
+    }
+    return fabs(xc) / sqrt(ss);
+}
+
+__device__ __forceinline__ bool lz_check(int m) {
+    if (m <= 24) return m % 4 == 0;
+    if (m <= 48) return m % 8 == 0;
+    if (m <= 96) return m % 16 == 0;
+    return m % 32 == 0;
+}
+
+// Lanczos on a dim-dimensional operator; v0 holds the start vector (it is
+// overwritten), v1 and v2 are scratch of the same length
+template <class Op>
+__device__ double blanczos(int32_t dim, Op op, double *v0, double *v1, double *v2, double tol, int max_steps,
+                           int *steps, double *sh) {
+    __shared__ double al[kLzMax], be[kLzMax];
+    __shared__ double bc[2];
+    __shared__ int imin;
+    max_steps = min(max_steps, kLzMax);
+    double nz = 0.0;
+    for (int32_t i = threadIdx.x; i < dim; i += kBlock) nz += v0[i] * v0[i];
+    nz = sqrt(bsum(nz, sh));
+    for (int32_t i = threadIdx.x; i < dim; i += kBlock) v0[i] /= nz;
+    __syncthreads();
+    double *qp = v1, *qc = v0, *w = v2;
+    double beta_prev = 0.0, theta = -INFINITY;
+    int j = 0;
+    for (; j < max_steps; ++j) {
+        op(qc, w);
+        double a = 0.0;
+        for (int32_t i = threadIdx.x; i < dim; i += kBlock) a += qc[i] * w[i];
+        const double alpha = bsum(a, sh);
+        double ww = 0.0;
+        for (int32_t i = threadIdx.x; i < dim; i += kBlock) {
+            const double v = w[i] - alpha * qc[i] - (j > 0 ? beta_prev * qp[i] : 0.0);
+            w[i] = v;
+            ww += v * v;
+        }
+        const double beta = sqrt(bsum(ww, sh));
+        if (threadIdx.x == 0) {
+            al[j] = alpha;
+            be[j] = beta;
+        }
+        __syncthreads();
+        const int m = j + 1;
+        const bool tiny = !(beta > 1e-13 * (fabs(alpha) + beta_prev));
+        if (lz_check(m) || m == max_steps || tiny) {
+            theta = tri_max(al, be, m, theta, &imin);
+            if (threadIdx.x == 0) {
+                const double res = tiny ? 0.0 : beta * tri_last(al, be, m, theta);
+                const double den = sqrt(theta * theta + res * res) + fabs(theta);
+                bc[0] = (res == 0.0 || (den > 0.0 && res / den < tol)) ? 1.0 : 0.0;
+            }
+            __syncthreads();
+            if (bc[0] != 0.0) break;
+        }
+        for (int32_t i = threadIdx.x; i < dim; i += kBlock) w[i] /= beta;
+        __syncthreads();
+        double *tmp = qp;
+        qp = qc;
+        qc = w;
+        w = tmp;
+        beta_prev = beta;
+    }
+    *steps = j < max_steps ? j + 1 : max_steps;
+    return theta;
+}
+
+// per-instance lambda_max of A A^T and Q in one launch; start vectors were
+// written by the host (m-part in pz, n-part in dQw)
 template <class MT>
 __global__ __launch_bounds__(kBlock) void k_batch_power(BArgs<MT> a, double tol, int max_it) {
     __shared__ double t[64];
@@ -541,65 +673,26 @@ __global__ __launch_bounds__(kBlock) void k_batch_power(BArgs<MT> a, double tol,
     Inst<MT> I = inst_of(a, b);
     double *rec = a.rec + (int64_t)b * R_COUNT;
     const int32_t n = I.n, m = I.m, k = I.k;
-    // lambda_max(A A^T): z (m) in pz, q (m) in s, A^T q (n) in pq
     double lam = 0.0;
-    int it = 0;
+    int steps = 0;
     if (m > 0) {
-        lam = 1.0;
-        for (it = 1; it <= max_it; ++it) {
-            double nz = 0.0;
-            for (int32_t r = threadIdx.x; r < m; r += kBlock) nz += I.pz[r] * I.pz[r];
-            nz = sqrt(bsum(nz, sh));
-            for (int32_t r = threadIdx.x; r < m; r += kBlock) I.s[r] = I.pz[r] / nz;
+        // A A^T v = A (A^T v), A^T v into pq (n)
+        auto opA = [&](const double *v, double *out) {
+            for (int32_t i = threadIdx.x; i < n; i += kBlock) I.pq[i] = csr_row(I.atrp, I.atci, I.atv, v, i);
             __syncthreads();
-            for (int32_t i = threadIdx.x; i < n; i += kBlock) I.pq[i] = csr_row(I.atrp, I.atci, I.atv64, I.s, i);
-            __syncthreads();
-            csr_rows_wave(m, I.arp, I.aci, I.av64, I.pq, I.pz);
-            double qz = 0.0;
-            for (int32_t r = threadIdx.x; r < m; r += kBlock) qz += I.s[r] * I.pz[r];
-            lam = bsum(qz, sh);
-            double e1 = 0.0, e2 = 0.0;
-            for (int32_t r = threadIdx.x; r < m; r += kBlock) {
-                const double q = I.pz[r] - lam * I.s[r];
-                I.s[r] = q;
-                e1 += q * q;
-                e2 += I.pz[r] * I.pz[r];
-            }
-            e1 = bsum(e1, sh);
-            e2 = bsum(e2, sh);
-            if (sqrt(e1) / (sqrt(e2) + lam) < tol) break;
-        }
+            csr_rows_wave(m, I.arp, I.aci, I.av, I.pq, out);
+        };
+        lam = blanczos(m, opA, I.pz, I.s, I.pz2, tol, max_it, &steps, sh);
     }
     if (threadIdx.x == 0) {
         rec[R_POW_A] = lam;
-        rec[R_POW_ITA] = (double)(it > max_it ? max_it : it);
+        rec[R_POW_ITA] = (double)steps;
     }
-    // lambda_max(Q): z (n) in dQw, q (n) in pq
-    double lq = 1.0;
-    for (it = 1; it <= max_it; ++it) {
-        double nz = 0.0;
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) nz += I.dQw[i] * I.dQw[i];
-        nz = sqrt(bsum(nz, sh));
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) I.pq[i] = I.dQw[i] / nz;
-        __syncthreads();
-        lr_apply(n, k, I.U64, I.d64, I.pq, I.dQw, t, sh);
-        double qz = 0.0;
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) qz += I.pq[i] * I.dQw[i];
-        lq = bsum(qz, sh);
-        double e1 = 0.0, e2 = 0.0;
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
-            const double q = I.dQw[i] - lq * I.pq[i];
-            I.pq[i] = q;
-            e1 += q * q;
-            e2 += I.dQw[i] * I.dQw[i];
-        }
-        e1 = bsum(e1, sh);
-        e2 = bsum(e2, sh);
-        if (sqrt(e1) / (sqrt(e2) + lq) < tol) break;
-    }
+    auto opQ = [&](const double *v, double *out) { lr_apply(n, k, I.U, I.d, v, out, t, sh); };
+    const double lq = blanczos(n, opQ, I.dQw, I.pq, I.pq2, tol, max_it, &steps, sh);
     if (threadIdx.x == 0) {
         rec[R_POW_Q] = lq;
-        rec[R_POW_ITQ] = (double)(it > max_it ? max_it : it);
+        rec[R_POW_ITQ] = (double)steps;
     }
 }
 
@@ -658,7 +751,7 @@ static double bgolden(double ap, double bp, double cp, double dp) {
 }
 
 struct BatchWork {
-    DevVec<double> vn[20], vm[6];
+    DevVec<double> vn[21], vm[7];
     DevVec<BCtl> ctl;
     DevVec<double> rec;
 };
@@ -712,10 +805,10 @@ static BArgs<MT> make_args(hprqp_batch *B, BatchWork &W) {
     }
     double **np[] = {&a.x, &a.w, &a.w_bar, &a.x_hat, &a.x_bar, &a.z_bar, &a.dx, &a.dw, &a.Qw, &a.Qw_bar,
                      &a.tempv, &a.ATy, &a.ATy_bar, &a.ATdy, &a.QATdy, &a.dQw, &a.last_x, &a.last_w, &a.last_ATy,
-                     &a.pq};
-    for (int i = 0; i < 20; ++i) *np[i] = W.vn[i].p;
-    double **mp[] = {&a.y, &a.y_bar, &a.s, &a.dy, &a.last_y, &a.pz};
-    for (int i = 0; i < 6; ++i) *mp[i] = W.vm[i].p;
+                     &a.pq, &a.pq2};
+    for (int i = 0; i < 21; ++i) *np[i] = W.vn[i].p;
+    double **mp[] = {&a.y, &a.y_bar, &a.s, &a.dy, &a.last_y, &a.pz, &a.pz2};
+    for (int i = 0; i < 7; ++i) *mp[i] = W.vm[i].p;
     a.ctl = W.ctl.p;
     a.rec = W.rec.p;
     return a;

@@ -360,7 +360,9 @@ int hprqp_qop_create_qap(hprqp_ctx *ctx, int32_t nq, const double *A, const doub
                                          ctx->stream));
             }
             q->qtemp.alloc(nn);
+            q->qtemp32.alloc(nn);
             HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+            hprqp::qap_prepare(ctx->stream, q, A, B, S, T);
         } catch (...) {
             delete q;
             throw;
@@ -436,6 +438,20 @@ int hprqp_qop_apply(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x, hprqp_buf 
         HPRQP_REQUIRE(x->bytes >= sizeof(double) * (size_t)q->n && Qx->bytes >= sizeof(double) * (size_t)q->n,
                       HPRQP_E_ARG, "buffer smaller than the operator size");
         hprqp::qop_apply_fp64(ctx, q, (const double *)x->p, (double *)Qx->p);
+    });
+}
+
+int hprqp_qop_apply_prec(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x, hprqp_buf *Qx, int precision) {
+    CTX_CHECK(ctx);
+    if (!q || !x || !Qx) return HPRQP_E_ARG;
+    if (precision != HPRQP_PREC_FP64 && precision != HPRQP_PREC_MIXED) return HPRQP_E_ARG;
+    return guarded(ctx, [&] {
+        HPRQP_REQUIRE(x->bytes >= sizeof(double) * (size_t)q->n && Qx->bytes >= sizeof(double) * (size_t)q->n,
+                      HPRQP_E_ARG, "buffer smaller than the operator size");
+        if (precision == HPRQP_PREC_FP64)
+            hprqp::qop_apply_fp64(ctx, q, (const double *)x->p, (double *)Qx->p);
+        else
+            hprqp::qop_apply_mixed(ctx, q, (const double *)x->p, (double *)Qx->p);
     });
 }
 

@@ -133,6 +133,16 @@ struct hprqp_qop {
     int32_t nq = 0;
     hprqp::DevVec<double> qa, qb, qs, qt, qtemp;
     hprqp::DevVec<float> qtemp32;
+    // QAP operands of the MFMA kernels (built by qap_prepare): row-major
+    // copies of B and T for the FP64 products; split-BF16 pieces padded to
+    // qnp = nq rounded up to 32 (A, S row-major; B, T column-major); the
+    // diagonals of S / T when those are diagonal (then the full matrices are
+    // not used by the MFMA kernels)
+    int32_t qnp = 0;
+    bool qs_diag = false, qt_diag = false;
+    bool qa_exact = false, qb_exact = false, qs_exact = false, qt_exact = false;
+    hprqp::DevVec<double> qbt, qtt, qsd, qtd;
+    hprqp::DevVec<uint16_t> qa0, qa1, qb0, qb1, qs0, qs1, qt0, qt1;
     // diagonal plus low rank: Q = diag(d) + U U^T, U n x k column-major
     int32_t lr_k = 0;
     hprqp::DevVec<double> lr_U, lr_d, lr_t;
@@ -209,11 +219,16 @@ void profile_step(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *
 // operator applies exposed through the ABI (FP64)
 void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx);
 void csr_spmv_fp64(hprqp_ctx *ctx, const DevCsr &a, const double *x, double *y);
-// QAP dense products (hprqp_qap.hip)
-void qap_apply_fp64(hipStream_t s, int32_t nq, const double *A, const double *B, const double *S,
-                    const double *T, const double *X, double *tmp, double *QX);
-void qap_apply_fp32(hipStream_t s, int32_t nq, const double *A, const double *B, const double *S,
-                    const double *T, const float *X, float *tmp, float *QX);
+// QAP dense products (hprqp_qap.hip).  qap_prepare builds the MFMA operands
+// from qa/qb/qs/qt (host copies given); qap_apply_f64 is the FP64 product,
+// qap_apply_bf16x3 the split-BF16 three-term product accumulated in FP32.
+void qap_prepare(hipStream_t s, hprqp_qop *q, const double *A, const double *B, const double *S, const double *T);
+void qap_apply_f64(hipStream_t s, const hprqp_qop *q, const double *X, double *QX);
+template <class IT, class OT>
+void qap_apply_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, OT *QX);
+// the reduced-precision operator apply of the MIXED path (FP32 matrix data;
+// split-BF16 x3 MFMA for QAP), FP64 in and out
+void qop_apply_mixed(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx);
 // diagonal plus low rank Qx = d.*x + U (U^T x)  (hprqp_lowrank.hip)
 template <class MT, class XT, class YT>
 void lowrank_apply(hipStream_t s, int32_t n, int32_t k, const MT *U, const MT *d, const XT *x, double *t, YT *Qx);

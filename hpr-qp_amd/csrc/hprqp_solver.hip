@@ -481,12 +481,11 @@ struct Solver {
                 csr_apply<VT, MT>(s, S.qop->lat, (const VT *)qtemp.p, EpiStore<VT>{out});
                 break;
             case QOP_QAP:
-                if constexpr (std::is_same<VT, double>::value)
-                    qap_apply_fp64(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xin,
-                                   qtemp.p, out);
+                // FP64 solves: FP64 MFMA; reduced precision: split-BF16 x3 MFMA
+                if constexpr (std::is_same<MT, double>::value)
+                    qap_apply_f64(s, S.qop, xin, out);
                 else
-                    qap_apply_fp32(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xin,
-                                   qtemp.p, out);
+                    qap_apply_bf16x3<VT, VT>(s, S.qop, xin, out);
                 break;
             case QOP_LOWRANK:
                 if constexpr (std::is_same<MT, double>::value)
@@ -549,8 +548,7 @@ struct Solver {
                     xd = t;
                 }
                 if (S.qkind == QOP_QAP) {
-                    qap_apply_fp64(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xd,
-                                   qtemp_d.p, out);
+                    qap_apply_f64(s, S.qop, xd, out);
                 } else if (S.qkind == QOP_LOWRANK) {
                     lowrank_apply<double, double, double>(s, n, S.qop->lr_k, S.qop->lr_U.p, S.qop->lr_d.p, xd,
                                                           lr_t.p, out);
@@ -652,8 +650,7 @@ struct Solver {
                 csr_apply<double, double>(s, S.qop->lat, (const double *)qtemp_d.p, EpiStore<double>{out});
                 break;
             case QOP_QAP:
-                qap_apply_fp64(s, S.qop->nq, S.qop->qa.p, S.qop->qb.p, S.qop->qs.p, S.qop->qt.p, xin,
-                               qtemp_d.p, out);
+                qap_apply_f64(s, S.qop, xin, out);
                 break;
             case QOP_LOWRANK:
                 lowrank_apply<double, double, double>(s, n, S.qop->lr_k, S.qop->lr_U.p, S.qop->lr_d.p, xin,
@@ -1749,10 +1746,40 @@ void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx) {
             csr_apply<double, double>(s, q->lat, (const double *)q->ltemp.p, EpiStore<double>{Qx});
             break;
         case QOP_QAP:
-            qap_apply_fp64(s, q->nq, q->qa.p, q->qb.p, q->qs.p, q->qt.p, x, q->qtemp.p, Qx);
+            qap_apply_f64(s, q, x, Qx);
             break;
         case QOP_LOWRANK:
             lowrank_apply<double, double, double>(s, q->n, q->lr_k, q->lr_U.p, q->lr_d.p, x, q->lr_t.p, Qx);
+            break;
+        default: {
+            int rc = q->vt.apply(q->user, x, Qx, (void *)s);
+            HPRQP_REQUIRE(rc == 0, HPRQP_E_ARG, "user Q operator apply failed");
+        }
+    }
+}
+
+void qop_apply_mixed(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx) {
+    hipStream_t s = ctx->stream;
+    switch (q->kind) {
+        case QOP_CSR:
+            if (q->q.nnz == 0) {
+                HPRQP_HIP(hipMemsetAsync(Qx, 0, sizeof(double) * q->n, s));
+            } else {
+                ensure_v32(q->q);
+                csr_apply<double, float>(s, q->q, x, EpiStore<double>{Qx});
+            }
+            break;
+        case QOP_LASSO:
+            ensure_v32(q->la);
+            ensure_v32(q->lat);
+            csr_apply<double, float>(s, q->la, x, EpiStore<double>{q->ltemp.p});
+            csr_apply<double, float>(s, q->lat, (const double *)q->ltemp.p, EpiStore<double>{Qx});
+            break;
+        case QOP_QAP:
+            qap_apply_bf16x3<double, double>(s, q, x, Qx);
+            break;
+        case QOP_LOWRANK:
+            lowrank_apply<float, double, double>(s, q->n, q->lr_k, q->lr_U32.p, q->lr_d32.p, x, q->lr_t.p, Qx);
             break;
         default: {
             int rc = q->vt.apply(q->user, x, Qx, (void *)s);

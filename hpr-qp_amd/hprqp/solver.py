@@ -166,11 +166,18 @@ class DeviceQOperator:
         self.h = h
         self.n = n
 
-    def Qmap(self, x: np.ndarray) -> np.ndarray:
-        """Qmap!(x, Qx, Q) through the device (FP64)."""
+    def Qmap(self, x: np.ndarray, precision: str = "fp64") -> np.ndarray:
+        """Qmap!(x, Qx, Q) through the device.  precision "fp64" (the
+        reference's arithmetic) or "mixed" (the product the mixed-precision
+        iteration uses: FP32 matrix data; split-BF16 x3 MFMA for QAP)."""
         xb = _lib.DeviceBuffer.from_array(self.ctx, x)
         qb = _lib.DeviceBuffer(self.ctx, self.n)
-        _lib.check(self.ctx.h, self.ctx.lib.hprqp_qop_apply(self.ctx.h, self.h, xb.h, qb.h))
+        if precision == "fp64":
+            _lib.check(self.ctx.h, self.ctx.lib.hprqp_qop_apply(self.ctx.h, self.h, xb.h, qb.h))
+        elif precision == "mixed":
+            _lib.check(self.ctx.h, self.ctx.lib.hprqp_qop_apply_prec(self.ctx.h, self.h, xb.h, qb.h, 1))
+        else:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         return qb.download()
 
     def info(self):

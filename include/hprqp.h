@@ -198,6 +198,13 @@ int hprqp_qop_destroy(hprqp_ctx *ctx, hprqp_qop *q);
 /* Qmap!(x, Qx, Q): FP64 in, FP64 out, on the context stream */
 int hprqp_qop_apply(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x,
                     hprqp_buf *Qx);
+/* The same product in the precision the solver's iteration uses:
+ * HPRQP_PREC_FP64 as hprqp_qop_apply; HPRQP_PREC_MIXED with FP32 matrix
+ * data (CSR, LASSO, low rank) and FP64 accumulation, and for QAP the
+ * split-BF16 three-term MFMA product accumulated in FP32.  x and Qx are FP64
+ * buffers of n doubles.  No counterpart in the reference (FP64 only). */
+int hprqp_qop_apply_prec(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x,
+                         hprqp_buf *Qx, int precision);
 /* get_problem_size / get_temp_size / get_operator_name */
 int hprqp_qop_info(const hprqp_qop *q, int64_t *n, int64_t *temp_size,
                    const char **name);

@@ -232,3 +232,88 @@ def test_qop_apply_matches_oracle():
         n, t, name = dev.info()
         assert n == model.n
         dev.close()
+
+
+def _qap_dense(Q, X):
+    return 2.0 * (Q.A @ X @ Q.B - Q.S @ X - X @ Q.T)
+
+
+def _qap_cases():
+    """QAP operators: esc64a (full S, T; integer data), the 128 x 128
+    flow/distance instance (diagonal S, T; F/2 and D exact in BF16) and two
+    sizes that are not multiples of the 16 x 16 tile with Gaussian (not
+    BF16-exact) matrices, one with full and one with diagonal shifts."""
+    from hprqp import build_from_mat, instances
+    from hprqp.model import QAP_Q_operator
+
+    rng = np.random.default_rng(11)
+    out = [("esc64a", build_from_mat(fixture_path("esc64a.mat"), "QAP", verbose=False).Q),
+           ("flow128", instances.qap(128, seed=0)[0].Q)]
+    for nq, full in ((45, True), (70, False)):
+        A = rng.standard_normal((nq, nq))
+        B = rng.standard_normal((nq, nq))
+        if full:
+            S, T = rng.standard_normal((nq, nq)), rng.standard_normal((nq, nq))
+        else:
+            S, T = np.diag(rng.standard_normal(nq)), np.diag(rng.standard_normal(nq))
+        out.append((f"gauss{nq}", QAP_Q_operator(A, B, S, T)))
+    return out
+
+
+def test_qap_fp64_mfma_matches_closed_form():
+    """FP64 MFMA operator (f64 16x16x4) against vec(2(AXB - SX - XT))."""
+    from hprqp import to_gpu
+
+    rng = np.random.default_rng(2)
+    for name, Q in _qap_cases():
+        nq = Q.A.shape[0]
+        X = rng.standard_normal((nq, nq))
+        dev = to_gpu(Q, nq * nq)
+        got = dev.Qmap(X.ravel(order="F"))
+        want = _qap_dense(Q, X).ravel(order="F")
+        scale = np.max(np.abs(want))
+        err = np.max(np.abs(got - want)) / scale
+        print(f"{name}: fp64 max rel err {err:.2e}")
+        assert err < 1e-13
+        dev.close()
+
+
+def test_qap_split_bf16x3_accuracy():
+    """Split-BF16 three-term MFMA product (FP32 accumulation) against the FP64
+    closed form, error relative to the magnitude M = 2(|A||X||B| + |S||X| +
+    |X||T|): BF16-exact coefficients (three-piece data split) reach FP32
+    accuracy; general coefficients (two-piece splits of both) 16 bits."""
+    from hprqp import to_gpu
+
+    rng = np.random.default_rng(3)
+    for name, Q in _qap_cases():
+        nq = Q.A.shape[0]
+        X = rng.uniform(0.0, 1.0, (nq, nq)) / nq
+        dev = to_gpu(Q, nq * nq)
+        got = dev.Qmap(X.ravel(order="F"), precision="mixed")
+        want = _qap_dense(Q, X).ravel(order="F")
+        aX = np.abs(X)
+        M = (2.0 * (np.abs(Q.A) @ aX @ np.abs(Q.B) + np.abs(Q.S) @ aX + aX @ np.abs(Q.T))).ravel(order="F")
+        err = np.max(np.abs(got - want) / M)
+        exact = name in ("esc64a", "flow128")
+        print(f"{name}: split-bf16x3 max err/M {err:.2e} ({'exact' if exact else 'general'} coefficients)")
+        assert err < (1e-5 if exact else 2e-4)
+        dev.close()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp32"])
+def test_qap_reduced_precision_solve(precision):
+    """QAP relaxations with the split-BF16 x3 operator in the iteration: the
+    FP64 KKT residual still decides termination (1e-6) and the objective
+    agrees with the FP64 oracle within 1e-6 relative."""
+    from hprqp import HPRQP_parameters, build_from_mat, instances, optimize
+    from oracle import oracle
+
+    for model in (build_from_mat(fixture_path("esc64a.mat"), "QAP", verbose=False), instances.qap(48, seed=1)[0]):
+        res = optimize(model, HPRQP_parameters(verbose=False, precision=precision, problem_type="QAP"))
+        ref = oracle.solve(model, problem_type="QAP")
+        rel = abs(res.primal_obj - ref.primal_obj) / (1.0 + abs(ref.primal_obj))
+        print(f"{precision} QAP n={model.n}: {res.status} iter={res.iter} polish={res.polish_iter} "
+              f"kkt={res.residuals:.2e} rel_obj_err_vs_fp64={rel:.2e} (oracle iter={ref.iter})")
+        assert res.status == "OPTIMAL" and res.residuals < 1e-6
+        assert rel < 1e-6
