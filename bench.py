@@ -8,9 +8,15 @@ seed 0 -- the largest configuration that is quoted single-instance.
 
 Also reported: `roofline` for the dominant kernel (the fused z/x/w step
 whose row product is Q w), measured live with HIP events inside libhprqp on
-the context stream, and `cpu_baseline` -- the CPU oracle (a restatement of
-the reference CPU path, 1 thread) timed on a bounded sample of the same
-workload and extrapolated to one solve.
+the context stream; `cpu_baseline` -- the CPU oracle (a restatement of the
+reference CPU path, 1 thread) timed on a bounded sample of the same
+workload and extrapolated to one solve; and `batched` -- 4,096 portfolio QPs
+solved by the batched mixed-precision solver, cold and warm-started by the
+learned MLP, against the plain FP64 form solving them one per call.
+
+--workload random|lasso|qap runs the same single-instance measurement on
+the other workloads (random sparse QP n=10,000; LASSO 3,308 x 150,360; QAP
+128 x 128), --workload batch only the batched leg; these are for profiling.
 """
 from __future__ import annotations
 
@@ -34,6 +40,34 @@ def zxw1_bytes(nnz_q, n, prec):
     FP64 anchor last_x, written Qw x x_hat w_bar."""
     v = 8 if prec == "fp64" else 4
     return nnz_q * (v + 4) + 4 * (n + 1) + v * n + 5 * v * n + 8 * n + 4 * v * n
+
+
+def lasso_step_bytes(nnz, m, n, prec):
+    """Algorithmic HBM bytes of the timed LASSO step (Q w = A^T (A w), two
+    launches): A and A^T values+indices and row pointers, gathered w and
+    temp, temp written; epilogue reads ATy c x w and the FP64 anchors
+    last_x last_w, writes Qw x x_hat w_bar w."""
+    v = 8 if prec == "fp64" else 4
+    return 2 * nnz * (v + 4) + 4 * (m + n + 2) + v * n + 2 * v * m + 9 * v * n + 16 * n
+
+
+def workload(name, grid):
+    """(model, problem_type, description) of a single-instance workload."""
+    from hprqp import instances
+
+    if name == "grid":
+        m = instances.grid_qp(grid, seed=0)
+        return m, "QP", f"grid QP {grid}x{grid} (n={m.n}, m={m.m})"
+    if name == "random":
+        m = instances.random_sparse_qp(10_000, 5_000, seed=0)
+        return m, "QP", f"random sparse QP (n={m.n}, m={m.m}, nnz(Q)={m.Q.nnz})"
+    if name == "lasso":
+        m = instances.lasso(seed=0)
+        return m, "LASSO", f"sparse LASSO, A {m.Q.A.shape[0]}x{m.Q.A.shape[1]} with {m.Q.A.nnz} nnz"
+    if name == "qap":
+        m = instances.qap(128, seed=0)[0]
+        return m, "QAP", "QAP relaxation 128x128 (n=16384, m=256), flow/distance"
+    raise ValueError(name)
 
 
 def log(msg):
@@ -110,20 +144,29 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--workload", default="grid", choices=["grid", "random", "lasso", "qap", "batch"],
+                    help="grid (default): the headline single-instance workload; random / lasso / qap: the "
+                         "other single-instance workloads (for profiling); batch: only the batched leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from hprqp import DeviceModel, HPRQP_parameters, instances
+    from hprqp import DeviceModel, HPRQP_parameters
 
+    if args.workload == "batch":
+        out = batched_leg(args.batch_count, args.plain_sample)
+        print(json.dumps({"metric": "batched solves/sec to 1e-6 rel KKT", "value":
+                          out["batched_mixed_warm_solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+                          "higher_is_better": True, "data": "synthetic (hprqp.instances.portfolio_batch)",
+                          "batched": out}), flush=True)
+        return
     t0 = time.time()
-    model = instances.grid_qp(args.grid, seed=0)
-    log(f"grid {args.grid}: n={model.n} m={model.m} nnzQ={model.Q.nnz} nnzA={model.A.nnz} "
-        f"built in {time.time() - t0:.1f}s")
+    model, ptype, desc = workload(args.workload, args.grid)
+    log(f"{desc}: n={model.n} m={model.m} built in {time.time() - t0:.1f}s")
     dm = DeviceModel(model, 0)
-    params = HPRQP_parameters(verbose=False, precision=args.precision, time_kernels=True)
+    params = HPRQP_parameters(verbose=False, precision=args.precision, time_kernels=True, problem_type=ptype)
     for w in range(args.warmup):
         r = dm.solve(params, hist_cap=1000)
         log(f"warmup {w}: {r.status} iter={r.iter} kkt={r.residuals:.2e} time={r.time:.3f}s "
@@ -147,13 +190,29 @@ def main():
     prof = dm.profile(params, reps=args.profile_reps)
     log(f"in-solve z/x/w1 kernel: {kms * 1e3:.2f} us avg over {calls} launches; isolated kernel ms: "
         + ", ".join(f"{k}={v:.4f}" for k, v in prof.items()))
-    nb = zxw1_bytes(model.Q.nnz, model.n, args.precision)
-    achieved = nb / (kms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "k_csr<EpiZxw1> (z/x/w1 step, row product Q w)",
-                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": nb, "avg_ms": kms, "launches": calls,
-                "isolated_kernels_ms": prof}
+    if ptype == "QAP":
+        nq = int(round(model.n ** 0.5))
+        fl = 4.0 * nq ** 3  # two n^3 GEMM pairs per apply (diagonal S, T)
+        peak = 78.6 if args.precision == "fp64" else 2500.0 / 3.0
+        achieved = fl / (kms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": "k_qap_f64 / k_qap_bf16x3 + z/x/w1 epilogue (timed together)",
+                    "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                    "traffic": None, "flops_per_launch": fl, "avg_ms": kms, "launches": calls,
+                    "isolated_kernels_ms": prof}
+    else:
+        if ptype == "LASSO":
+            A = model.Q.A
+            nb = lasso_step_bytes(A.nnz, A.shape[0], A.shape[1], args.precision)
+            kname = "LASSO step: k_csr (A w) + k_csr<EpiLasso> (A^T temp), timed together"
+        else:
+            nb = zxw1_bytes(model.Q.nnz, model.n, args.precision)
+            kname = "k_csr<EpiZxw1> (z/x/w1 step, row product Q w)"
+        achieved = nb / (kms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": kname,
+                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_launch": nb, "avg_ms": kms, "launches": calls,
+                    "isolated_kernels_ms": prof}
 
     cpu = None
     if not args.no_cpu:
@@ -161,14 +220,14 @@ def main():
 
         ncores = 1
         t = time.time()
-        ref = oracle.solve(model, max_iter=args.cpu_iters, max_power_iter=args.cpu_power_iters)
+        ref = oracle.solve(model, problem_type=ptype, max_iter=args.cpu_iters, max_power_iter=args.cpu_power_iters)
         wall = time.time() - t
         t_pa = ref.power_time_A / max(1, ref.power_iter_A)
         t_pq = (ref.power_time - ref.power_time_A) / max(1, ref.power_iter_Q)
         t_it = (ref.time - ref.power_time) / max(1, ref.iter)
         est = ref.setup_time + t_pa * r0.power_iter_A + t_pq * r0.power_iter_Q + t_it * r0.iter
         cpu = {"value": 1.0 / est, "unit": "solves/s", "cores": ncores, "kind": "port",
-               "sample": (f"oracle (C restatement of the reference CPU path, 1 thread) on the same grid QP: "
+               "sample": (f"oracle (C restatement of the reference CPU path, 1 thread) on the same workload: "
                           f"scaling + {ref.power_iter_A}+{ref.power_iter_Q} power iterations + {ref.iter} HPR "
                           f"iterations in {wall:.1f}s wall; extrapolated to the GPU run's "
                           f"{r0.power_iter_A}+{r0.power_iter_Q} power and {r0.iter} HPR iterations = "
@@ -189,16 +248,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-        "data": "synthetic (seeded generator, hprqp.instances.grid_qp)",
-        "config": {"workload": f"grid QP {args.grid}x{args.grid} (n={model.n}, m={model.m}), "
-                               f"{args.precision}, cold start, 1e-6 relative KKT",
-                   "n": model.n, "m": model.m, "nnz_Q": int(model.Q.nnz), "nnz_A": int(model.A.nnz),
+        "data": f"synthetic (seeded generator, hprqp.instances, workload {args.workload})",
+        "config": {"workload": f"{desc}, {args.precision}, cold start, 1e-6 relative KKT",
+                   "n": model.n, "m": model.m, "nnz_A": int(model.A.nnz),
+                   "nnz_Q": int(model.Q.nnz) if hasattr(model.Q, "nnz") else None,
                    "precision": args.precision, "iterations": r0.iter, "kkt": r0.residuals,
                    "solve_time_s": r0.time, "power_time_s": r0.power_time, "setup_time_s": r0.setup_time,
                    "primal_obj": r0.primal_obj, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "batched": None if args.no_batch else batched_leg(args.batch_count, args.plain_sample),
+        "batched": (None if args.no_batch or args.workload != "grid"
+                    else batched_leg(args.batch_count, args.plain_sample)),
     }
     print(json.dumps(line), flush=True)
 

@@ -133,15 +133,15 @@ struct hprqp_qop {
     int32_t nq = 0;
     hprqp::DevVec<double> qa, qb, qs, qt, qtemp;
     hprqp::DevVec<float> qtemp32;
-    // QAP operands of the MFMA kernels (built by qap_prepare): row-major
-    // copies of B and T for the FP64 products; split-BF16 pieces padded to
-    // qnp = nq rounded up to 32 (A, S row-major; B, T column-major); the
-    // diagonals of S / T when those are diagonal (then the full matrices are
-    // not used by the MFMA kernels)
+    // QAP operands of the MFMA kernels (built by qap_prepare): split-BF16
+    // pieces padded to qnp = nq rounded up to 32 (A, S row-major; B, T
+    // column-major); the diagonals of S / T when those are diagonal (the
+    // full matrices are then not read); qtemp / qtemp32 hold the two n x n
+    // intermediates of the FP64 / split-BF16 passes
     int32_t qnp = 0;
     bool qs_diag = false, qt_diag = false;
     bool qa_exact = false, qb_exact = false, qs_exact = false, qt_exact = false;
-    hprqp::DevVec<double> qbt, qtt, qsd, qtd;
+    hprqp::DevVec<double> qsd, qtd;
     hprqp::DevVec<uint16_t> qa0, qa1, qb0, qb1, qs0, qs1, qt0, qt1;
     // diagonal plus low rank: Q = diag(d) + U U^T, U n x k column-major
     int32_t lr_k = 0;

@@ -21,6 +21,7 @@
 
 #include "hprqp_internal.hpp"
 #include "hprqp_kernels.hpp"
+#include "hprqp_qap.hpp"
 
 namespace hprqp {
 
@@ -519,6 +520,12 @@ struct Solver {
         } else if (S.qkind == QOP_LASSO) {
             csr_apply<VT, MT>(s, S.qop->la, xin, EpiStore<VT>{qtemp.p});
             csr_apply<VT, MT>(s, S.qop->lat, (const VT *)qtemp.p, epi);
+        } else if (S.qkind == QOP_QAP) {
+            // the half-step epilogue runs in the second MFMA pass
+            if constexpr (std::is_same<MT, double>::value)
+                qap_launch_f64(s, S.qop, xin, epi);
+            else
+                qap_launch_bf16x3<VT>(s, S.qop, xin, epi);
         } else {
             q_apply(xin, qbuf);
             vec_apply<VT, VT>(s, n, (const VT *)qbuf, epi);
