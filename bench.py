@@ -51,6 +51,19 @@ def lasso_step_bytes(nnz, m, n, prec):
     return 2 * nnz * (v + 4) + 4 * (m + n + 2) + v * n + 2 * v * m + 9 * v * n + 16 * n
 
 
+def measured_traffic(workload_name, grid, prec):
+    """HBM bytes per launch of the dominant kernel, from the FETCH_SIZE /
+    WRITE_SIZE counter runs summarised under profiles/ (None when that
+    workload has not been measured)."""
+    f = ROOT / "profiles" / f"grid{grid}_{prec}_zxw1_traffic.md"
+    if workload_name != "grid" or not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def workload(name, grid):
     """(model, problem_type, description) of a single-instance workload."""
     from hprqp import instances
@@ -144,7 +157,9 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--workload", default="grid", choices=["grid", "random", "lasso", "qap", "batch"],
+    ap.add_argument("--copy-mib", type=int, default=1024,
+                    help="--workload copy: bytes per copy (MiB) for calibrating the HBM byte counters")
+    ap.add_argument("--workload", default="grid", choices=["grid", "random", "lasso", "qap", "batch", "copy"],
                     help="grid (default): the headline single-instance workload; random / lasso / qap: the "
                          "other single-instance workloads (for profiling); batch: only the batched leg")
     args = ap.parse_args()
@@ -155,6 +170,25 @@ def main():
 
     from hprqp import DeviceModel, HPRQP_parameters
 
+    if args.workload == "copy":
+        # a plain device-to-device copy of known size: what FETCH_SIZE /
+        # WRITE_SIZE read for it calibrates the counters of the other runs
+        import torch
+
+        nb = args.copy_mib << 20
+        a = torch.empty(nb // 4, dtype=torch.float32, device="cuda:0").uniform_()
+        b = torch.empty_like(a)
+        for _ in range(args.warmup):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        print(json.dumps({"metric": "device copy GB/s (read + write)", "value": 2 * nb / dt / 1e9, "unit": "GB/s",
+                          "bytes_read_per_copy": nb, "bytes_written_per_copy": nb, "steps": args.steps}), flush=True)
+        return
     if args.workload == "batch":
         out = batched_leg(args.batch_count, args.plain_sample)
         print(json.dumps({"metric": "batched solves/sec to 1e-6 rel KKT", "value":
@@ -206,11 +240,12 @@ def main():
             kname = "LASSO step: k_csr (A w) + k_csr<EpiLasso> (A^T temp), timed together"
         else:
             nb = zxw1_bytes(model.Q.nnz, model.n, args.precision)
-            kname = "k_csr<EpiZxw1> (z/x/w1 step, row product Q w)"
+            kname = "z/x/w1 step: CSR row product Q w with the fused EpiZxw1 epilogue (k_csr / k_csr_stream, lanes autotuned)"
         achieved = nb / (kms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": kname,
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(args.workload, args.grid, args.precision),
                     "bytes_per_launch": nb, "avg_ms": kms, "launches": calls,
                     "isolated_kernels_ms": prof}
 

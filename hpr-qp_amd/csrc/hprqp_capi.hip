@@ -73,6 +73,7 @@ HostCsr host_csr_transpose(const HostCsr &a) {
 int default_lanes(const HostCsr &h) {
     if (h.nrow <= 0) return 1;
     const double mean = (double)h.v.size() / (double)h.nrow;
+    if (mean >= 256.0) return 256;  // long rows: one block per row
     // short rows: CSR-stream with 256/128/64 rows per block (negative code)
     if (mean <= 6.0) return -256;
     if (mean <= 12.0) return -128;

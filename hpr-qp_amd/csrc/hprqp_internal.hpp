@@ -1,5 +1,6 @@
 // hprqp_internal.hpp -- host-side objects behind the C ABI (include/hprqp.h).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -90,6 +91,29 @@ struct HostCsr {
     std::vector<double> v;
 };
 
+// Kernel timing carried by the launches themselves (hipExtLaunchKernel):
+// while armed, the first launch records `start` as it begins and every
+// launch records `stop` as it ends, so start..stop spans exactly the kernels
+// of one timed step -- the durations a kernel trace reports -- without the
+// gaps of separately recorded events.
+struct LaunchTiming {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool armed = false, first = true;
+};
+inline thread_local LaunchTiming g_launch_timing;
+
+template <class... KA, class... PA>
+inline void launch_k(void (*kernel)(KA...), dim3 grid, dim3 block, uint32_t shm, hipStream_t s, PA &&...args) {
+    LaunchTiming &t = g_launch_timing;
+    if (t.armed) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shm, s, t.first ? t.start : nullptr, t.stop, 0u,
+                              static_cast<KA>(args)...);
+        t.first = false;
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shm, s, static_cast<KA>(args)...);
+    }
+}
+
 HostCsr host_csr_from(int32_t nrow, int32_t ncol, int64_t nnz, const int32_t *rowptr,
                       const int32_t *colidx, const double *vals, int index_base);
 HostCsr host_csr_transpose(const HostCsr &a);
@@ -163,8 +187,9 @@ struct hprqp_model {
     hprqp::HostCsr Qsym;
     std::vector<int32_t> q_tpos;
     // SpMV lanes chosen by the autotuner (0 = not tuned yet), per matrix
-    // value precision (FP64, FP32) for Q, A, A^T
-    mutable int tuned_lanes[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    // value precision (FP64, FP32) for Q, A, A^T and a LASSO operator's
+    // data matrix and its transpose
+    mutable int tuned_lanes[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
 };
 
 struct hprqp_batch {

@@ -71,6 +71,35 @@ __global__ __launch_bounds__(kBlock) void k_csr(int32_t nrows,
     if (lane == 0) epi(row, acc);
 }
 
+// long rows (hundreds of nonzeros and up): one block per row, every thread
+// strides the row with four independent accumulators, so a row's loads are
+// in flight together; wave shuffles and one LDS exchange finish the sum in a
+// fixed order
+template <class MT, class XT, class AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_csr_rowblock(int32_t nrows, const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ ci,
+                                                         const MT *__restrict__ v, const XT *__restrict__ x,
+                                                         Epi epi) {
+    __shared__ AT part[kBlock / 64];
+    const int32_t row = blockIdx.x;
+    const int32_t s = rp[row], e = rp[row + 1];
+    AT a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int32_t k = s + threadIdx.x;
+    for (; k + 3 * kBlock < e; k += 4 * kBlock) {
+        a0 += (AT)v[k] * (AT)x[ci[k]];
+        a1 += (AT)v[k + kBlock] * (AT)x[ci[k + kBlock]];
+        a2 += (AT)v[k + 2 * kBlock] * (AT)x[ci[k + 2 * kBlock]];
+        a3 += (AT)v[k + 3 * kBlock] * (AT)x[ci[k + 3 * kBlock]];
+    }
+    for (; k < e; k += kBlock) a0 += (AT)v[k] * (AT)x[ci[k]];
+    AT acc = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) epi(row, (part[0] + part[1]) + (part[2] + part[3]));
+}
+
 // CSR-stream row product for short rows: a block owns RB consecutive rows;
 // its nonzeros are read once, fully coalesced, and their products staged in
 // LDS; then thread t sums row r0+t in order and runs the epilogue, so every

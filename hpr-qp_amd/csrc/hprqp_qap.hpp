@@ -406,9 +406,9 @@ void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Ep
     const int64_t nn = (int64_t)g.n * g.n;
     double *Y = q->qtemp.p, *Z = q->qtemp.p + nn;
     const dim3 grid(nt, nt);
-    k_qap64_p1<<<grid, 64, 0, s>>>(g, X, Y, Z);
+    launch_k(k_qap64_p1, grid, dim3(64), 0, s, g, X, Y, Z);
     HPRQP_HIP(hipGetLastError());
-    k_qap64_p2<Epi><<<grid, 64, 0, s>>>(g, X, Y, Z, epi);
+    launch_k(k_qap64_p2<Epi>, grid, dim3(64), 0, s, g, X, (const double *)Y, (const double *)Z, epi);
     HPRQP_HIP(hipGetLastError());
 }
 
@@ -419,9 +419,9 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
     const int64_t nn = (int64_t)g.n * g.n;
     float *Y = q->qtemp32.p, *Z = q->qtemp32.p + nn;
     const dim3 grid(nt, nt);
-    k_qapbf_p1<IT><<<grid, 64, 0, s>>>(g, X, Y, Z);
+    launch_k(k_qapbf_p1<IT>, grid, dim3(64), 0, s, g, X, Y, Z);
     HPRQP_HIP(hipGetLastError());
-    k_qapbf_p2<IT, Epi><<<grid, 64, 0, s>>>(g, X, Y, Z, epi);
+    launch_k(k_qapbf_p2<IT, Epi>, grid, dim3(64), 0, s, g, X, (const float *)Y, (const float *)Z, epi);
     HPRQP_HIP(hipGetLastError());
 }
 

@@ -39,23 +39,28 @@ template <class AT, class MT, class XT, class Epi>
 static void launch_csr(hipStream_t s, int lanes, int32_t nrows, const int32_t *rp, const int32_t *ci,
                        const MT *v, const XT *x, const Epi &epi) {
     if (nrows <= 0) return;
+    if (lanes > 64) {  // one block per row
+        launch_k(k_csr_rowblock<MT, XT, AT, Epi>, dim3(nrows), dim3(kBlock), 0, s, nrows, rp, ci, v, x, epi);
+        HPRQP_HIP(hipGetLastError());
+        return;
+    }
     if (lanes < 0) {  // CSR-stream: -lanes rows per block
         const int rb = -lanes;
         const int grid = (int)(((int64_t)nrows + rb - 1) / rb);
-        k_csr_stream<MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rb, rp, ci, v, x, epi);
+        launch_k(k_csr_stream<MT, XT, AT, Epi>, dim3(grid), dim3(kBlock), 0, s, nrows, rb, rp, ci, v, x, epi);
         HPRQP_HIP(hipGetLastError());
         return;
     }
     const int64_t threads = (int64_t)nrows * lanes;
-    const int grid = (int)((threads + kBlock - 1) / kBlock);
+    const dim3 grid((unsigned)((threads + kBlock - 1) / kBlock)), blk(kBlock);
     switch (lanes) {
-        case 1: k_csr<1, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
-        case 2: k_csr<2, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
-        case 4: k_csr<4, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
-        case 8: k_csr<8, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
-        case 16: k_csr<16, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
-        case 32: k_csr<32, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
-        default: k_csr<64, MT, XT, AT, Epi><<<grid, kBlock, 0, s>>>(nrows, rp, ci, v, x, epi); break;
+        case 1: launch_k(k_csr<1, MT, XT, AT, Epi>, grid, blk, 0, s, nrows, rp, ci, v, x, epi); break;
+        case 2: launch_k(k_csr<2, MT, XT, AT, Epi>, grid, blk, 0, s, nrows, rp, ci, v, x, epi); break;
+        case 4: launch_k(k_csr<4, MT, XT, AT, Epi>, grid, blk, 0, s, nrows, rp, ci, v, x, epi); break;
+        case 8: launch_k(k_csr<8, MT, XT, AT, Epi>, grid, blk, 0, s, nrows, rp, ci, v, x, epi); break;
+        case 16: launch_k(k_csr<16, MT, XT, AT, Epi>, grid, blk, 0, s, nrows, rp, ci, v, x, epi); break;
+        case 32: launch_k(k_csr<32, MT, XT, AT, Epi>, grid, blk, 0, s, nrows, rp, ci, v, x, epi); break;
+        default: launch_k(k_csr<64, MT, XT, AT, Epi>, grid, blk, 0, s, nrows, rp, ci, v, x, epi); break;
     }
     HPRQP_HIP(hipGetLastError());
 }
@@ -82,7 +87,7 @@ static void each(hipStream_t s, int64_t n, const F &f) {
 template <class ST, class AT, class Epi>
 static void vec_apply(hipStream_t s, int32_t n, const ST *src, const Epi &epi) {
     if (n <= 0) return;
-    k_vec<ST, AT, Epi><<<nblk(n), kBlock, 0, s>>>(n, src, epi);
+    launch_k(k_vec<ST, AT, Epi>, dim3(nblk(n)), dim3(kBlock), 0, s, n, src, epi);
     HPRQP_HIP(hipGetLastError());
 }
 
@@ -131,17 +136,24 @@ struct KernelTimer {
     double total_ms = 0.0;
     int64_t count = 0;
     explicit KernelTimer(hipStream_t st, int pairs = 512) : s(st), ev(2 * pairs) {
-        for (auto &e : ev) HPRQP_HIP(hipEventCreate(&e));
+        // timing-only events: no system-scope fence when they are recorded
+        for (auto &e : ev) HPRQP_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     }
     ~KernelTimer() {
         for (auto e : ev) (void)hipEventDestroy(e);
     }
+    // the kernels launched between begin() and end() carry the event pair
     void begin() {
         if (used == (int)ev.size()) flush();
-        HPRQP_HIP(hipEventRecord(ev[used], s));
+        g_launch_timing = LaunchTiming{ev[used], ev[used + 1], true, true};
     }
     void end() {
-        HPRQP_HIP(hipEventRecord(ev[used + 1], s));
+        LaunchTiming &t = g_launch_timing;
+        if (t.first) {  // nothing was launched: an empty span
+            HPRQP_HIP(hipEventRecord(ev[used], s));
+            HPRQP_HIP(hipEventRecord(ev[used + 1], s));
+        }
+        t = LaunchTiming{};
         used += 2;
     }
     void flush() {
@@ -1599,8 +1611,8 @@ struct EventTimer {
     hipEvent_t a, b;
     hipStream_t s;
     explicit EventTimer(hipStream_t st) : s(st) {
-        HPRQP_HIP(hipEventCreate(&a));
-        HPRQP_HIP(hipEventCreate(&b));
+        HPRQP_HIP(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+        HPRQP_HIP(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
     }
     ~EventTimer() {
         (void)hipEventDestroy(a);
@@ -1636,6 +1648,7 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
     for (int rb : {256, 128, 64})  // CSR-stream when a block's nonzeros fit in LDS
         if (mean * rb <= 0.75 * kStreamCap) cand.push_back(-rb);
     for (int L = 1; L <= 64; L *= 2) cand.push_back(L);
+    if (mean >= 64.0) cand.push_back(kBlock);  // one block per row
     for (int L : cand) {
         const double ms = tm.avg_ms(reps, [&] {
             launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<VT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
@@ -1651,15 +1664,24 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
 static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool mixed) {
     if (p->spmv_lanes != 0) {
         const int L = p->spmv_lanes;
-        const bool vec = L > 0 && (L & (L - 1)) == 0 && L <= 64;
+        const bool vec = L > 0 && (L & (L - 1)) == 0 && (L <= 64 || L == kBlock);
         const bool stream = L == -64 || L == -128 || L == -256;
         HPRQP_REQUIRE(vec || stream, HPRQP_E_ARG,
-                      "spmv_lanes must be 0, a power of two <= 64, or -64/-128/-256 (rows per block)");
+                      "spmv_lanes must be 0, a power of two <= 64, 256 (a block per row) or -64/-128/-256 "
+                      "(rows per block)");
         S.Q.lanes = S.A.lanes = S.AT.lanes = p->spmv_lanes;
+        if (S.qkind == QOP_LASSO) S.qop->la.lanes = S.qop->lat.lanes = p->spmv_lanes;
         return;
     }
-    DevCsr *mats[3] = {&S.Q, &S.A, &S.AT};
-    for (int k = 0; k < 3; ++k) {
+    // Q, A, A^T of the scaled problem, and the LASSO operator's data matrix
+    // and its transpose
+    DevCsr *mats[5] = {&S.Q, &S.A, &S.AT, nullptr, nullptr};
+    if (S.qkind == QOP_LASSO) {
+        mats[3] = &S.qop->la;
+        mats[4] = &S.qop->lat;
+    }
+    for (int k = 0; k < 5; ++k) {
+        if (!mats[k]) continue;
         DevCsr &a = *mats[k];
         if (a.nnz < (1 << 18)) continue;  // heuristic choice is already close
         int &tuned = md->tuned_lanes[mixed ? 1 : 0][k];

@@ -90,8 +90,9 @@ typedef struct {
     int32_t precision;          /* HPRQP_PREC_FP64 */
     int32_t verbose;            /* 0 */
     int32_t spmv_lanes;         /* 0 = autotune; 1..64 (power of two) = lanes
-                                   per row; -64/-128/-256 = CSR-stream with
-                                   that many rows per block */
+                                   per row; 256 = one block per row;
+                                   -64/-128/-256 = CSR-stream with that many
+                                   rows per block */
     int32_t time_kernels;       /* 1 = time the dominant fused kernel with
                                    HIP events during the solve (result.kernel_ms) */
     int32_t polish_stall_checks; /* mixed mode: residual checks without the KKT

@@ -95,7 +95,7 @@ def test_grid_qp_small():
     _check(res, ref)
 
 
-@pytest.mark.parametrize("lanes", [1, 4, 64, -64, -256])
+@pytest.mark.parametrize("lanes", [1, 4, 64, 256, -64, -256])
 def test_every_spmv_kernel_shape_gives_the_same_solution(lanes):
     """Vector kernels (1..64 lanes per row) and CSR-stream kernels (rows per
     block) change only the summation order inside a row."""
