@@ -127,6 +127,13 @@ def batched_leg(count, plain_sample):
     warm = bm.solve(mixed, warm=net)  # prediction on the device, then the solve
     warm_s = time.perf_counter() - t
     t_pred = warm.predict_time
+    # algorithmic HBM bytes of the batched iteration (mixed: FP32 U, d, c, l,
+    # u, A; FP64 iterates): four passes over U (Q w, Q w_bar, each a U^T v
+    # and a U t pass) = 16 n k, plus 216 n of vector streams per iteration
+    nk = np.array([m.n * m.Q.U.shape[1] for m in models], dtype=np.float64)
+    nn = np.array([m.n for m in models], dtype=np.float64)
+    per_it = 16.0 * nk + 216.0 * nn
+    cold_bytes = float(np.sum(per_it * cold.iter))
     cold_ok = int(np.sum((np.array(cold.status) == "OPTIMAL") & (cold.residuals < 1e-6)))
     warm_ok = int(np.sum((np.array(warm.status) == "OPTIMAL") & (warm.residuals < 1e-6)))
     out = {"workload": f"{count} portfolio QPs, n~U{{200..5000}}, k~U{{10..50}}, 1e-6 relative KKT",
@@ -136,7 +143,9 @@ def batched_leg(count, plain_sample):
            "warm_vs_plain": (warm_ok / warm_s) / plain_sps,
            "cold_mean_iter": float(cold.iter.mean()), "warm_mean_iter": float(warm.iter.mean()),
            "cold_max_iter": int(cold.iter.max()), "warm_max_iter": int(warm.iter.max()),
-           "mlp_inference_s": t_pred, "all_solved": cold_ok == count and warm_ok == count}
+           "mlp_inference_s": t_pred, "all_solved": cold_ok == count and warm_ok == count,
+           "cold_iteration_hbm_gbs": cold_bytes / (cold.time - cold.power_time) / 1e9,
+           "cold_iteration_bytes": cold_bytes, "cold_lambda_max_s": cold.power_time}
     log("batched: " + json.dumps(out))
     bm.close()
     net.close()

@@ -92,24 +92,28 @@ __device__ __forceinline__ double bmax(double v, double *sh) {
 }
 
 // t[j] = sum_i U[i + j n] v[i], j < k  (t in LDS).  Columns go in groups of
-// kG: each thread keeps kG partial sums, so one round of shuffles and one
-// barrier pair serve kG columns and the loads of a group are independent.
+// kG: each thread keeps kG partial sums over its rows, so one round of
+// shuffles and one barrier pair serve kG columns and the loads of a group
+// are independent.  Columns past k in the last group read column k-1 again
+// (cache hits) and are dropped, so no load sits behind a branch.
 constexpr int kG = 8;
 template <class UT, class VT>
 __device__ void lr_tvec(int32_t n, int32_t k, const UT *__restrict__ U, const VT *__restrict__ v, double *t,
                         double *sh) {
     __shared__ double part[kBlock / 64][kG];
+    (void)sh;
     for (int32_t j0 = 0; j0 < k; j0 += kG) {
         double acc[kG];
+        int32_t co[kG];  // column offsets (an instance's U has < 2^31 entries)
 #pragma unroll
-        for (int g = 0; g < kG; ++g) acc[g] = 0.0;
-        const int gmax = min(kG, k - j0);
+        for (int g = 0; g < kG; ++g) {
+            acc[g] = 0.0;
+            co[g] = min(j0 + g, k - 1) * n;
+        }
         for (int32_t i = threadIdx.x; i < n; i += kBlock) {
             const double vi = (double)v[i];
-            const UT *col = U + (int64_t)j0 * n + i;
 #pragma unroll
-            for (int g = 0; g < kG; ++g)
-                if (g < gmax) acc[g] += (double)col[(int64_t)g * n] * vi;
+            for (int g = 0; g < kG; ++g) acc[g] += (double)U[co[g] + i] * vi;
         }
 #pragma unroll
         for (int g = 0; g < kG; ++g) {
@@ -121,6 +125,7 @@ __device__ void lr_tvec(int32_t n, int32_t k, const UT *__restrict__ U, const VT
             for (int g = 0; g < kG; ++g) part[threadIdx.x >> 6][g] = acc[g];
         }
         __syncthreads();
+        const int gmax = min(kG, k - j0);
         if (threadIdx.x < gmax) {
             const int g = threadIdx.x;
             t[j0 + g] = part[0][g] + part[1][g] + part[2][g] + part[3][g];
@@ -129,11 +134,20 @@ __device__ void lr_tvec(int32_t n, int32_t k, const UT *__restrict__ U, const VT
     }
     __syncthreads();
 }
+// U_i. t with four independent accumulators
 template <class UT>
 __device__ __forceinline__ double lr_row(int32_t n, int32_t k, const UT *__restrict__ U, int32_t i, const double *t) {
-    double acc = 0.0;
-    for (int32_t j = 0; j < k; ++j) acc += (double)U[i + (int64_t)j * n] * t[j];
-    return acc;
+    const UT *p = U + i;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int32_t j = 0;
+    for (; j + 4 <= k; j += 4) {
+        a0 += (double)p[(int64_t)j * n] * t[j];
+        a1 += (double)p[(int64_t)(j + 1) * n] * t[j + 1];
+        a2 += (double)p[(int64_t)(j + 2) * n] * t[j + 2];
+        a3 += (double)p[(int64_t)(j + 3) * n] * t[j + 3];
+    }
+    for (; j < k; ++j) a0 += (double)p[(int64_t)j * n] * t[j];
+    return (a0 + a1) + (a2 + a3);
 }
 // Qv for all i into out: out_i = d_i v_i + U_i. t  (t = U'v already in LDS)
 template <class UT, class VT>
