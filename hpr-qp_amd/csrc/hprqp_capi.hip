@@ -153,7 +153,8 @@ void hprqp_params_default(hprqp_params *p) {
     p->verbose = 0;
     p->spmv_lanes = 0;
     p->time_kernels = 0;
-    p->polish_stall_checks = 40;
+    p->polish_stall_checks = 0;
+    p->polish_switch = 5.0;
 }
 
 int hprqp_ctx_create(int device, hprqp_ctx **out) {

@@ -1379,13 +1379,22 @@ static int run_loop(Solver<VT, MT> &W, LoopState &L, int64_t iter_begin, double 
         if (do_res) {
             W.compute_residuals(iter, obj_constant, lasso, res);
             has_res = true;
-            // mixed precision: hand over to the FP64 polish once the FP32
-            // iteration stops improving the FP64-measured KKT residual
+            // reduced precision: hand over to the FP64 polish once the
+            // FP64-measured KKT residual is within polish_switch of the
+            // tolerance (the FP32 data / iterates floor lies near there), or
+            // once the FP32 iteration stops improving it
             if (mixed && res.kkt >= p->stoptol && (iter % p->check_iter == 0)) {
+                if (res.kkt < p->polish_switch * p->stoptol) {
+                    L.iter = iter;
+                    return -1;
+                }
                 if (res.kkt < 0.5 * L.best_kkt) {
                     L.best_kkt = res.kkt;
                     L.stall = 0;
-                } else if (++L.stall >= std::max(1, p->polish_stall_checks) && res.kkt < 1e-3) {
+                } else if (++L.stall >= (p->polish_stall_checks > 0
+                                             ? p->polish_stall_checks
+                                             : (p->precision == HPRQP_PREC_MIXED ? 100 : 40)) &&
+                           res.kkt < 1e-3) {
                     L.iter = iter;
                     return -1;
                 }

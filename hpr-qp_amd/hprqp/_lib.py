@@ -45,6 +45,7 @@ class hprqp_params(C.Structure):
         ("spmv_lanes", C.c_int32),
         ("time_kernels", C.c_int32),
         ("polish_stall_checks", C.c_int32),
+        ("polish_switch", C.c_double),
     ]
 
 

@@ -52,7 +52,8 @@ class HPRQP_parameters:
     precision: str = "fp64"      # "fp64" (plain), "mixed" or "fp32"
     spmv_lanes: int = 0          # 0 = automatic
     time_kernels: bool = False   # HIP-event timing of the first fused step kernel
-    polish_stall_checks: int = 40  # mixed: checks without KKT halving before FP64 polish
+    polish_stall_checks: int = 0  # checks without KKT halving before FP64 polish (0: 100 mixed, 40 fp32)
+    polish_switch: float = 5.0  # reduced precision: FP64 polish once KKT < polish_switch * stoptol
 
     def to_c(self) -> _lib.hprqp_params:
         p = _lib.hprqp_params()
@@ -81,6 +82,7 @@ class HPRQP_parameters:
         p.spmv_lanes = int(self.spmv_lanes)
         p.time_kernels = int(bool(self.time_kernels))
         p.polish_stall_checks = int(self.polish_stall_checks)
+        p.polish_switch = float(self.polish_switch)
         return p
 
 

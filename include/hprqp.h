@@ -95,9 +95,14 @@ typedef struct {
                                    rows per block */
     int32_t time_kernels;       /* 1 = time the dominant fused kernel with
                                    HIP events during the solve (result.kernel_ms) */
-    int32_t polish_stall_checks; /* mixed mode: residual checks without the KKT
-                                    residual halving before the FP64 polish
-                                    takes over (default 40) */
+    int32_t polish_stall_checks; /* reduced precision: residual checks without
+                                    the KKT residual halving before the FP64
+                                    polish takes over (0 = automatic: 100 for
+                                    HPRQP_PREC_MIXED, 40 for HPRQP_PREC_FP32) */
+    double polish_switch;        /* reduced precision: the FP64 polish also
+                                    takes over once the KKT residual drops
+                                    below polish_switch * stoptol (default 5;
+                                    0 = only on a stall) */
 } hprqp_params;
 
 /* Results (HPRQP_results, src/structs.jl:292-309) plus diagnostics. */
