@@ -28,6 +28,10 @@
 
 namespace hprqp {
 
+// threads per instance workgroup (a multiple of 64)
+constexpr int kBB = 256;
+constexpr int kBW = kBB / 64;
+
 static double bnow() {
     using namespace std::chrono;
     return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -45,6 +49,7 @@ enum {
     R_LASTGAP, R_CURGAP,                                                // M-norms
     R_DXDX, R_DQWATDY, R_DQWDQW, R_DWDQW, R_ATDYQATDY, R_DYDY,          // sigma dots
     R_POW_A, R_POW_Q, R_POW_ITA, R_POW_ITQ,                             // power method
+    R_DONE,  // iteration at which the instance met the tolerance inside an interval, -1 if not
     R_COUNT = 24
 };
 
@@ -60,6 +65,9 @@ struct BArgs {
     const MT *a_v, *at_v;
     const double *a_v64, *at_v64;
     const double *c, *l, *u, *AL, *AU;
+    const double *c0, *nbi, *nci;  // objective constant, |b|_inf, |c|_inf (residual norms)
+    double tol;
+    int64_t print_freq;  // -1: the adaptive print cadence
     const MT *cM, *lM, *uM, *ALM, *AUM;
     double *x, *w, *w_bar, *x_hat, *x_bar, *z_bar, *dx, *dw, *Qw, *Qw_bar, *tempv, *ATy, *ATy_bar, *ATdy, *QATdy,
         *dQw;
@@ -78,7 +86,9 @@ __device__ __forceinline__ double bsum(double v, double *sh) {
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[w] = v;
     __syncthreads();
-    const double r = sh[0] + sh[1] + sh[2] + sh[3];
+    double r = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBW; ++q) r += sh[q];
     return r;
 }
 __device__ __forceinline__ double bmax(double v, double *sh) {
@@ -88,7 +98,10 @@ __device__ __forceinline__ double bmax(double v, double *sh) {
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[w] = v;
     __syncthreads();
-    return dmax(dmax(sh[0], sh[1]), dmax(sh[2], sh[3]));
+    double r = sh[0];
+#pragma unroll
+    for (int q = 1; q < kBW; ++q) r = dmax(r, sh[q]);
+    return r;
 }
 
 // t[j] = sum_i U[i + j n] v[i], j < k  (t in LDS).  Columns go in groups of
@@ -100,7 +113,7 @@ constexpr int kG = 8;
 template <class UT, class VT>
 __device__ void lr_tvec(int32_t n, int32_t k, const UT *__restrict__ U, const VT *__restrict__ v, double *t,
                         double *sh) {
-    __shared__ double part[kBlock / 64][kG];
+    __shared__ double part[kBB / 64][kG];
     (void)sh;
     for (int32_t j0 = 0; j0 < k; j0 += kG) {
         double acc[kG];
@@ -110,7 +123,7 @@ __device__ void lr_tvec(int32_t n, int32_t k, const UT *__restrict__ U, const VT
             acc[g] = 0.0;
             co[g] = min(j0 + g, k - 1) * n;
         }
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < n; i += kBB) {
             const double vi = (double)v[i];
 #pragma unroll
             for (int g = 0; g < kG; ++g) acc[g] += (double)U[co[g] + i] * vi;
@@ -128,7 +141,10 @@ __device__ void lr_tvec(int32_t n, int32_t k, const UT *__restrict__ U, const VT
         const int gmax = min(kG, k - j0);
         if (threadIdx.x < gmax) {
             const int g = threadIdx.x;
-            t[j0 + g] = part[0][g] + part[1][g] + part[2][g] + part[3][g];
+            double r = 0.0;
+#pragma unroll
+            for (int q = 0; q < kBW; ++q) r += part[q][g];
+            t[j0 + g] = r;
         }
         __syncthreads();
     }
@@ -154,7 +170,7 @@ template <class UT, class VT>
 __device__ void lr_apply(int32_t n, int32_t k, const UT *U, const UT *d, const VT *v, double *out, double *t,
                          double *sh) {
     lr_tvec(n, k, U, v, t, sh);
-    for (int32_t i = threadIdx.x; i < n; i += kBlock) out[i] = (double)d[i] * (double)v[i] + lr_row(n, k, U, i, t);
+    for (int32_t i = threadIdx.x; i < n; i += kBB) out[i] = (double)d[i] * (double)v[i] + lr_row(n, k, U, i, t);
     __syncthreads();
 }
 // y_r = sum A_r. v  for the instance's m rows: one wave per row
@@ -162,7 +178,7 @@ template <class AT_, class VT>
 __device__ void csr_rows_wave(int32_t m, const int32_t *rp, const int32_t *ci, const AT_ *v, const VT *x,
                               double *out) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int32_t r = wave; r < m; r += kBlock / 64) {
+    for (int32_t r = wave; r < m; r += kBB / 64) {
         double acc = 0.0;
         for (int32_t q = rp[r] + lane; q < rp[r + 1]; q += 64) acc += (double)v[q] * (double)x[ci[q]];
 #pragma unroll
@@ -245,7 +261,7 @@ __device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, doubl
     const double f2s = 1.0 / (1.0 + sigma * lamQ), f1s = 1.0 - f2s;
     // z/x/w1: Qw = Q w
     lr_tvec(n, k, I.U, I.w, t, sh);
-    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
         const double wi = I.w[i], xi = I.x[i];
         const double qw = (double)I.d[i] * wi + lr_row(n, k, I.U, i, t);
         I.Qw[i] = qw;
@@ -265,7 +281,7 @@ __device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, doubl
     if (m > 0) {
         // tempv = x_hat + sigma (Qw - Q w_bar)
         lr_tvec(n, k, I.U, I.w_bar, t, sh);
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < n; i += kBB) {
             const double qwb = (double)I.d[i] * I.w_bar[i] + lr_row(n, k, I.U, i, t);
             I.tempv[i] = I.x_hat[i] + sigma * (I.Qw[i] - qwb);
         }
@@ -273,7 +289,7 @@ __device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, doubl
         // y step: A tempv
         csr_rows_wave(m, I.arp, I.aci, I.av, I.tempv, I.pz);
         const double fa = lamA * sigma, fb = 1.0 / fa;
-        for (int32_t r = threadIdx.x; r < m; r += kBlock) {
+        for (int32_t r = threadIdx.x; r < m; r += kBB) {
             const double yi = I.y[r];
             const double s_raw = I.pz[r] - fa * yi;
             const double sp = proj(s_raw, (double)I.ALM[r], (double)I.AUM[r]);
@@ -289,7 +305,7 @@ __device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, doubl
     }
     // w2 step: A^T y_bar
     const double fs = sigma / (1.0 + sigma * lamQ);
-    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
         const double atyb = m > 0 ? csr_row(I.atrp, I.atci, I.atv, I.y_bar, i) : 0.0;
         const double wi = I.w[i], wbi = I.w_bar[i], atyi = I.ATy[i];
         const double wbn = wbi + fs * (atyb - atyi);
@@ -311,12 +327,12 @@ __device__ double bmnorm(Inst<MT> &I, double sigma, double lamA, double lamQ, do
     const int32_t n = I.n, m = I.m, k = I.k;
     lr_apply(n, k, I.U, I.d, I.dw, I.dQw, t, sh);
     if (m > 0) {
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) I.ATdy[i] = csr_row(I.atrp, I.atci, I.atv, I.dy, i);
+        for (int32_t i = threadIdx.x; i < n; i += kBB) I.ATdy[i] = csr_row(I.atrp, I.atci, I.atv, I.dy, i);
         __syncthreads();
         lr_apply(n, k, I.U, I.d, I.ATdy, I.QATdy, t, sh);
     }
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s6 = 0, s7 = 0;
-    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
         const double a = I.dx[i], b = I.dQw[i], c = m > 0 ? I.ATdy[i] : 0.0, d = m > 0 ? I.QATdy[i] : 0.0,
                      e = I.dw[i];
         s0 += a * a;
@@ -326,7 +342,7 @@ __device__ double bmnorm(Inst<MT> &I, double sigma, double lamA, double lamQ, do
         s4 += c * d;
         s6 += e * b;
     }
-    for (int32_t r = threadIdx.x; r < m; r += kBlock) s7 += I.dy[r] * I.dy[r];
+    for (int32_t r = threadIdx.x; r < m; r += kBB) s7 += I.dy[r] * I.dy[r];
     s0 = bsum(s0, sh);
     s1 = bsum(s1, sh);
     s2 = bsum(s2, sh);
@@ -354,11 +370,11 @@ __device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *
     // Qx (into dQw as scratch), A^T y_bar (into ATdy), A x_bar (into pz)
     lr_apply(n, k, I.U64, I.d64, I.x_bar, I.dQw, t, sh);
     if (m > 0) {
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) I.ATdy[i] = csr_row(I.atrp, I.atci, I.atv64, I.y_bar, i);
+        for (int32_t i = threadIdx.x; i < n; i += kBB) I.ATdy[i] = csr_row(I.atrp, I.atci, I.atv64, I.y_bar, i);
         csr_rows_wave(m, I.arp, I.aci, I.av64, I.x_bar, I.pz);
     }
     double cx = 0, xqx = 0, zx = 0, rd = 0, aty = 0, qx = 0, bnd = 0, ys = 0, rp = 0, ax = 0;
-    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
         const double x = I.x_bar[i], q = I.dQw[i], z = I.z_bar[i], ci = I.c[i], a = m > 0 ? I.ATdy[i] : 0.0;
         cx += ci * x;
         xqx += x * q;
@@ -368,7 +384,7 @@ __device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *
         qx = dmax(qx, fabs(q));
         if (iter0) bnd = dmax(bnd, dmax(I.l[i] - x, 0.0) + dmax(x - I.u[i], 0.0));
     }
-    for (int32_t r = threadIdx.x; r < m; r += kBlock) {
+    for (int32_t r = threadIdx.x; r < m; r += kBB) {
         ys += I.y_bar[r] * I.s[r];
         const double v = I.pz[r];
         rp = dmax(rp, fabs(proj(v, I.AL[r], I.AU[r]) - v));
@@ -403,17 +419,17 @@ __device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *
 template <class MT>
 __device__ void bsigma_dots(Inst<MT> &I, double *rec, double *t, double *sh) {
     const int32_t n = I.n, m = I.m, k = I.k;
-    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
         I.dx[i] = I.x_bar[i] - I.last_x[i];
         I.dw[i] = I.w_bar[i] - I.last_w[i];
         if (m > 0) I.ATdy[i] = I.ATy_bar[i] - I.last_ATy[i];
     }
-    for (int32_t r = threadIdx.x; r < m; r += kBlock) I.dy[r] = I.y_bar[r] - I.last_y[r];
+    for (int32_t r = threadIdx.x; r < m; r += kBB) I.dy[r] = I.y_bar[r] - I.last_y[r];
     __syncthreads();
     lr_apply(n, k, I.U, I.d, I.dw, I.dQw, t, sh);
     if (m > 0) lr_apply(n, k, I.U, I.d, I.ATdy, I.QATdy, t, sh);
     double a = 0, b = 0, c = 0, d = 0, e = 0, f = 0;
-    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
         const double dxi = I.dx[i], dq = I.dQw[i], at = m > 0 ? I.ATdy[i] : 0.0, qat = m > 0 ? I.QATdy[i] : 0.0;
         a += dxi * dxi;
         b += dq * at;
@@ -421,7 +437,7 @@ __device__ void bsigma_dots(Inst<MT> &I, double *rec, double *t, double *sh) {
         d += I.dw[i] * dq;
         e += at * qat;
     }
-    for (int32_t r = threadIdx.x; r < m; r += kBlock) f += I.dy[r] * I.dy[r];
+    for (int32_t r = threadIdx.x; r < m; r += kBB) f += I.dy[r] * I.dy[r];
     a = bsum(a, sh);
     b = bsum(b, sh);
     c = bsum(c, sh);
@@ -443,12 +459,12 @@ __device__ void bsigma_dots(Inst<MT> &I, double *rec, double *t, double *sh) {
 template <class MT>
 __device__ void brestart(Inst<MT> &I) {
     const int32_t n = I.n, m = I.m;
-    for (int32_t r = threadIdx.x; r < m; r += kBlock) {
+    for (int32_t r = threadIdx.x; r < m; r += kBB) {
         I.y[r] = I.y_bar[r];
         I.last_y[r] = I.y_bar[r];
     }
     __syncthreads();
-    for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
         I.x[i] = I.x_bar[i];
         I.w[i] = I.w_bar[i];
         I.last_x[i] = I.x_bar[i];
@@ -470,22 +486,35 @@ static __device__ int64_t bprint_step(int64_t k) {
     return p / 10 > 10 ? p / 10 : 10;
 }
 
+// relative KKT residual from a residual record, the host's formula
+// (compute_residuals!, src/algorithm.jl:37-133, operator Q: no scaling)
+__device__ __forceinline__ double bkkt(const double *r, double c0, double nbi, double nci, int32_t m) {
+    const double pobj = r[R_CX] + 0.5 * r[R_XQX] + c0;
+    double dobj = -0.5 * r[R_XQX] + r[R_ZX] + c0;
+    if (m > 0) dobj += r[R_YS];
+    const double gap = fabs(pobj - dobj) / (1.0 + fmax(fabs(pobj), fabs(dobj)));
+    const double errd = r[R_RD] / (1.0 + fmax(fmax(nci, r[R_ATY]), r[R_QX]));
+    const double errp = m > 0 ? r[R_RP] / (1.0 + fmax(nbi, r[R_AX])) : 0.0;
+    return fmax(fmax(errp, errd), gap);
+}
+
 // one check interval for every active instance
 template <class MT>
-__global__ __launch_bounds__(kBlock) void k_batch_interval(BArgs<MT> a) {
+__global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
     __shared__ double t[64];
-    __shared__ double sh[4];
+    __shared__ double sh[kBW];
     const int b = a.order[blockIdx.x];
     const BCtl c = a.ctl[b];
     if (!c.active) return;
     Inst<MT> I = inst_of(a, b);
     double *rec = a.rec + (int64_t)b * R_COUNT;
     if (c.restart) brestart(I);
+    if (threadIdx.x == 0) rec[R_DONE] = -1.0;
     double inner = c.h_inner;
     for (int s = 0; s < c.K; ++s) {
         const int64_t it = c.iter0 + s, next = it + 1;
-        const bool full = (s == c.K - 1) || (s == 0 && c.restart) || (next % c.check_iter == 0) ||
-                          (next % bprint_step(next) == 0);
+        const bool print_yes = next % (a.print_freq > 0 ? a.print_freq : bprint_step(next)) == 0;
+        const bool full = (s == c.K - 1) || (s == 0 && c.restart) || (next % c.check_iter == 0) || print_yes;
         const double h1 = 1.0 / (inner + 2.0);
         bstep(I, c.sigma, c.lamA, c.lamQ, h1, 1.0 - h1, full, t, sh);
         if (s == 0 && c.want_last_gap) {
@@ -497,6 +526,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_interval(BArgs<MT> a) {
             if (threadIdx.x == 0) rec[R_CURGAP] = g;
         }
         inner += 1.0;
+        // a residual iteration before the interval's end (a print step between
+        // two check iterations): only the termination test can act there, so
+        // it runs here and a converged instance retires without the host
+        if (print_yes && s < c.K - 1) {
+            bresid(I, false, rec, t, sh);
+            __syncthreads();
+            if (bkkt(rec, a.c0[b], a.nbi[b], a.nci[b], I.m) < a.tol) {
+                if (threadIdx.x == 0) rec[R_DONE] = (double)next;
+                return;
+            }
+        }
     }
     bresid(I, false, rec, t, sh);
     bsigma_dots(I, rec, t, sh);
@@ -504,9 +544,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_interval(BArgs<MT> a) {
 
 // iteration-0 residuals (with the bound-violation term) and warm start
 template <class MT>
-__global__ __launch_bounds__(kBlock) void k_batch_init(BArgs<MT> a, int has_x0, int has_y0) {
+__global__ __launch_bounds__(kBB) void k_batch_init(BArgs<MT> a, int has_x0, int has_y0) {
     __shared__ double t[64];
-    __shared__ double sh[4];
+    __shared__ double sh[kBW];
     const int b = a.order[blockIdx.x];
     const BCtl c = a.ctl[b];
     if (!c.active) return;
@@ -516,28 +556,28 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(BArgs<MT> a, int has_x0, 
     // process_initial_points! (src/algorithm.jl:1783-1836): x0/y0 were
     // copied into x_bar / y_bar by the host
     if (has_x0) {
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < n; i += kBB) {
             const double v = I.x_bar[i];
             I.x[i] = I.w[i] = I.w_bar[i] = I.last_x[i] = I.last_w[i] = v;
         }
     }
     __syncthreads();
     if (has_y0) {
-        for (int32_t r = threadIdx.x; r < m; r += kBlock) I.y[r] = I.last_y[r] = I.y_bar[r];
+        for (int32_t r = threadIdx.x; r < m; r += kBB) I.y[r] = I.last_y[r] = I.y_bar[r];
         __syncthreads();
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < n; i += kBB) {
             const double v = m > 0 ? csr_row(I.atrp, I.atci, I.atv, I.y_bar, i) : 0.0;
             I.ATy_bar[i] = I.ATy[i] = I.last_ATy[i] = v;
         }
         __syncthreads();
         lr_apply(n, I.k, I.U, I.d, I.x_bar, I.dQw, t, sh);
-        for (int32_t i = threadIdx.x; i < n; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < n; i += kBB) {
             const double z_raw = I.x_bar[i] + c.sigma * (-I.dQw[i] + I.ATy_bar[i] - (double)I.cM[i]);
             I.z_bar[i] = (I.x_bar[i] - z_raw) / c.sigma;
         }
         if (m > 0) {
             csr_rows_wave(m, I.arp, I.aci, I.av, I.x_bar, I.pz);
-            for (int32_t r = threadIdx.x; r < m; r += kBlock)
+            for (int32_t r = threadIdx.x; r < m; r += kBB)
                 I.s[r] = dmin(dmax(I.pz[r] - c.lamA * c.sigma * I.y[r], (double)I.ALM[r]), (double)I.AUM[r]);
         }
         __syncthreads();
@@ -576,17 +616,17 @@ __device__ double tri_max(const double *al, const double *be, int m, double lo, 
         hi = fmax(hi, al[i] + (i > 0 ? fabs(be[i - 1]) : 0.0) + (i + 1 < m ? fabs(be[i]) : 0.0));
     }
     for (int round = 0; round < 10 && hi - lo > 1e-15 * fmax(fabs(hi), fabs(lo)); ++round) {
-        const double x = lo + (hi - lo) * (double)(threadIdx.x + 1) / (double)(kBlock + 1);
+        const double x = lo + (hi - lo) * (double)(threadIdx.x + 1) / (double)(kBB + 1);
         const bool above = sturm_below(al, be, m, x) == m;
         __syncthreads();
-        if (threadIdx.x == 0) *imin = kBlock;
+        if (threadIdx.x == 0) *imin = kBB;
         __syncthreads();
         if (above) atomicMin(imin, (int)threadIdx.x);
         __syncthreads();
         const int t = *imin;
-        const double w = (hi - lo) / (double)(kBlock + 1);
+        const double w = (hi - lo) / (double)(kBB + 1);
         const double nlo = lo + w * (double)t;
-        const double nhi = t < kBlock ? lo + w * (double)(t + 1) : hi;
+        const double nhi = t < kBB ? lo + w * (double)(t + 1) : hi;
         lo = nlo;
         hi = nhi;
     }
@@ -628,9 +668,9 @@ __device__ double blanczos(int32_t dim, Op op, double *v0, double *v1, double *v
     __shared__ int imin;
     max_steps = min(max_steps, kLzMax);
     double nz = 0.0;
-    for (int32_t i = threadIdx.x; i < dim; i += kBlock) nz += v0[i] * v0[i];
+    for (int32_t i = threadIdx.x; i < dim; i += kBB) nz += v0[i] * v0[i];
     nz = sqrt(bsum(nz, sh));
-    for (int32_t i = threadIdx.x; i < dim; i += kBlock) v0[i] /= nz;
+    for (int32_t i = threadIdx.x; i < dim; i += kBB) v0[i] /= nz;
     __syncthreads();
     double *qp = v1, *qc = v0, *w = v2;
     double beta_prev = 0.0, theta = -INFINITY;
@@ -638,10 +678,10 @@ __device__ double blanczos(int32_t dim, Op op, double *v0, double *v1, double *v
     for (; j < max_steps; ++j) {
         op(qc, w);
         double a = 0.0;
-        for (int32_t i = threadIdx.x; i < dim; i += kBlock) a += qc[i] * w[i];
+        for (int32_t i = threadIdx.x; i < dim; i += kBB) a += qc[i] * w[i];
         const double alpha = bsum(a, sh);
         double ww = 0.0;
-        for (int32_t i = threadIdx.x; i < dim; i += kBlock) {
+        for (int32_t i = threadIdx.x; i < dim; i += kBB) {
             const double v = w[i] - alpha * qc[i] - (j > 0 ? beta_prev * qp[i] : 0.0);
             w[i] = v;
             ww += v * v;
@@ -664,7 +704,7 @@ __device__ double blanczos(int32_t dim, Op op, double *v0, double *v1, double *v
             __syncthreads();
             if (bc[0] != 0.0) break;
         }
-        for (int32_t i = threadIdx.x; i < dim; i += kBlock) w[i] /= beta;
+        for (int32_t i = threadIdx.x; i < dim; i += kBB) w[i] /= beta;
         __syncthreads();
         double *tmp = qp;
         qp = qc;
@@ -679,9 +719,9 @@ __device__ double blanczos(int32_t dim, Op op, double *v0, double *v1, double *v
 // per-instance lambda_max of A A^T and Q in one launch; start vectors were
 // written by the host (m-part in pz, n-part in dQw)
 template <class MT>
-__global__ __launch_bounds__(kBlock) void k_batch_power(BArgs<MT> a, double tol, int max_it) {
+__global__ __launch_bounds__(kBB) void k_batch_power(BArgs<MT> a, double tol, int max_it) {
     __shared__ double t[64];
-    __shared__ double sh[4];
+    __shared__ double sh[kBW];
     const int b = a.order[blockIdx.x];
     if (!a.ctl[b].active) return;
     Inst<MT> I = inst_of(a, b);
@@ -692,7 +732,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_power(BArgs<MT> a, double tol,
     if (m > 0) {
         // A A^T v = A (A^T v), A^T v into pq (n)
         auto opA = [&](const double *v, double *out) {
-            for (int32_t i = threadIdx.x; i < n; i += kBlock) I.pq[i] = csr_row(I.atrp, I.atci, I.atv, v, i);
+            for (int32_t i = threadIdx.x; i < n; i += kBB) I.pq[i] = csr_row(I.atrp, I.atci, I.atv, v, i);
             __syncthreads();
             csr_rows_wave(m, I.arp, I.aci, I.av, I.pq, out);
         };
@@ -730,13 +770,6 @@ static void brandn(uint64_t seed, int64_t count, double *out) {
                          (1.0 / 9007199254740992.0);
         out[i] = std::sqrt(-2.0 * std::log(1.0 - a)) * std::cos(two_pi * b);
     }
-}
-
-static int64_t hprint_step(int64_t k) {
-    if (k <= 0) return 10;
-    int64_t p = 1;
-    while (p <= k / 10) p *= 10;
-    return std::max<int64_t>(p / 10, 10);
 }
 
 static double bgolden(double ap, double bp, double cp, double dp) {
@@ -796,6 +829,9 @@ static BArgs<MT> make_args(hprqp_batch *B, BatchWork &W) {
     a.u = B->u.p;
     a.AL = B->AL.p;
     a.AU = B->AU.p;
+    a.c0 = B->dc0.p;
+    a.nbi = B->dnbi.p;
+    a.nci = B->dnci.p;
     if constexpr (std::is_same<MT, double>::value) {
         a.U = B->U.p;
         a.d = B->d.p;
@@ -860,6 +896,8 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
     W.rec.alloc((int64_t)cnt * R_COUNT);
     HPRQP_HIP(hipMemsetAsync(W.rec.p, 0, sizeof(double) * W.rec.n, s));
     BArgs<MT> a = make_args<MT>(B, W);
+    a.tol = p->stoptol;
+    a.print_freq = p->print_frequency;
     // pinned staging for the per-instance control and records
     BCtl *hctl = nullptr;
     double *hrec = nullptr;
@@ -898,7 +936,7 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
         HPRQP_HIP(hipMemcpyAsync(a.pz, zm.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
         HPRQP_HIP(hipMemcpyAsync(a.dQw, zn.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
         put_ctl();
-        k_batch_power<MT><<<cnt, kBlock, 0, s>>>(a, 1e-4, 5000);
+        k_batch_power<MT><<<cnt, kBB, 0, s>>>(a, 1e-4, 5000);
         HPRQP_HIP(hipGetLastError());
         get_rec();
     }
@@ -928,7 +966,7 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
     else if (y0 && M)
         HPRQP_HIP(hipMemcpyAsync(a.y_bar, y0, sizeof(double) * M, hipMemcpyHostToDevice, s));
     put_ctl();
-    k_batch_init<MT><<<cnt, kBlock, 0, s>>>(a, x0 != nullptr || x0_dev != nullptr, y0 != nullptr || y0_dev != nullptr);
+    k_batch_init<MT><<<cnt, kBB, 0, s>>>(a, x0 != nullptr || x0_dev != nullptr, y0 != nullptr || y0_dev != nullptr);
     HPRQP_HIP(hipGetLastError());
     get_rec();
     // one check interval = ctl upload + interval kernel + record download,
@@ -938,7 +976,7 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
     hipGraphExec_t gexec = nullptr;
     HPRQP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     HPRQP_HIP(hipMemcpyAsync(W.ctl.p, hctl, sizeof(BCtl) * cnt, hipMemcpyHostToDevice, s));
-    k_batch_interval<MT><<<cnt, kBlock, 0, s>>>(a);
+    k_batch_interval<MT><<<cnt, kBB, 0, s>>>(a);
     HPRQP_HIP(hipMemcpyAsync(hrec, W.rec.p, sizeof(double) * R_COUNT * cnt, hipMemcpyDeviceToHost, s));
     HPRQP_HIP(hipStreamEndCapture(s, &graph));
     HPRQP_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -960,6 +998,9 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
             BInstState &S = st[b];
             if (!hctl[b].active) continue;
             const double *r = hrec + (int64_t)b * R_COUNT;
+            // converged at a residual iteration inside the last interval
+            const bool done_early = it > 0 && r[R_DONE] >= 0.0;
+            const int64_t rit = done_early ? (int64_t)r[R_DONE] : it;
             // residuals (operator Q: no scaling, b_scale = c_scale = 1)
             S.pobj = r[R_CX] + 0.5 * r[R_XQX] + B->obj_constant[b];
             S.dobj = -0.5 * r[R_XQX] + r[R_ZX] + B->obj_constant[b];
@@ -969,9 +1010,9 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
             S.errp = B->m[b] > 0 ? r[R_RP] / (1.0 + std::max(B->norm_b_inf[b], r[R_AX])) : 0.0;
             if (it == 0) S.errp = std::max(S.errp, r[R_BND]);
             S.kkt = std::max(std::max(S.errp, S.errd), S.gap);
-            S.iter = it;
+            S.iter = rit;
             int stat = -2;
-            if (S.kkt < tol)
+            if (S.kkt < tol || done_early)
                 stat = HPRQP_STATUS_OPTIMAL;
             else if (it >= p->max_iter)
                 stat = HPRQP_STATUS_MAX_ITER;
@@ -1042,13 +1083,10 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
             hctl[b].iter0 = it;
         }
         if (active == 0) break;
-        // interval to the next residual iteration (shared cadence)
+        // interval to the next check iteration (the print steps in between
+        // are handled on the device: only termination acts there)
         int64_t K = 1;
-        while (!(((it + K) % p->check_iter) == 0 ||
-                 (p->print_frequency == -1 ? ((it + K) % hprint_step(it + K) == 0)
-                                           : ((it + K) % p->print_frequency == 0)) ||
-                 it + K >= p->max_iter))
-            ++K;
+        while (!(((it + K) % p->check_iter) == 0 || it + K >= p->max_iter)) ++K;
         for (int b = 0; b < cnt; ++b)
             if (hctl[b].active) hctl[b].K = (int32_t)K;
         HPRQP_HIP(hipGraphLaunch(gexec, s));

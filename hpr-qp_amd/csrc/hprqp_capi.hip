@@ -725,6 +725,9 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
             }
             std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
             up(B->dorder, order.data(), count);
+            up(B->dc0, B->obj_constant.data(), count);
+            up(B->dnbi, B->norm_b_inf.data(), count);
+            up(B->dnci, B->norm_c_inf.data(), count);
             HPRQP_HIP(hipStreamSynchronize(s));
             up32(B->U32, U, UT);
             up32(B->d32, diag, N);

@@ -207,6 +207,7 @@ struct hprqp_batch {
     // estimated per-iteration cost, largest first, so the big ones start
     // early and the small ones fill in behind them
     hprqp::DevVec<int32_t> dorder;
+    hprqp::DevVec<double> dc0, dnbi, dnci;  // objective constants and residual norms
 };
 
 struct hprqp_mlp {
