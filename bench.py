@@ -34,12 +34,25 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def zxw1_bytes(nnz_q, n, prec):
+def zxw1_bytes(nnz_q, n, prec, bound_streams=2):
     """Algorithmic HBM bytes of one z/x/w1 launch (partial write set):
-    Q values+indices, row pointers, gathered w, streamed ATy c x l u and the
-    FP64 anchor last_x, written Qw x x_hat w_bar."""
+    Q values+indices, row pointers, gathered w, streamed ATy c x, the bound
+    vectors l u that are not a single value (0 or +-inf for every variable
+    is passed as a scalar), the FP64 anchor last_x; written Qw x x_hat w_bar."""
     v = 8 if prec == "fp64" else 4
-    return nnz_q * (v + 4) + 4 * (n + 1) + v * n + 5 * v * n + 8 * n + 4 * v * n
+    return nnz_q * (v + 4) + 4 * (n + 1) + v * n + (3 + bound_streams) * v * n + 8 * n + 4 * v * n
+
+
+def bound_streams(model):
+    """How many of l, u the z/x/w1 kernel streams (the rule of the solver:
+    one value for every variable, 0 or +-inf, is passed as a scalar)."""
+    import numpy as np
+
+    def uniform(v):
+        v = np.asarray(v)
+        return v.size > 0 and (v[0] == 0 or np.isinf(v[0])) and bool(np.all(v == v[0]))
+
+    return 2 - int(uniform(model.l)) - int(uniform(model.u))
 
 
 def lasso_step_bytes(nnz, m, n, prec):
@@ -166,6 +179,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="--workload copy: bytes per copy (MiB) for calibrating the HBM byte counters")
     ap.add_argument("--workload", default="grid", choices=["grid", "random", "lasso", "qap", "batch", "copy"],
@@ -176,6 +190,8 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU runs are not part of this build",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    import numpy as np
 
     from hprqp import DeviceModel, HPRQP_parameters
 
@@ -248,7 +264,7 @@ def main():
             nb = lasso_step_bytes(A.nnz, A.shape[0], A.shape[1], args.precision)
             kname = "LASSO step: k_csr (A w) + k_csr<EpiLasso> (A^T temp), timed together"
         else:
-            nb = zxw1_bytes(model.Q.nnz, model.n, args.precision)
+            nb = zxw1_bytes(model.Q.nnz, model.n, args.precision, bound_streams(model))
             kname = "z/x/w1 step: CSR row product Q w with the fused EpiZxw1 epilogue (k_csr / k_csr_stream, lanes autotuned)"
         achieved = nb / (kms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": kname,
@@ -257,6 +273,25 @@ def main():
                     "traffic": measured_traffic(args.workload, args.grid, args.precision),
                     "bytes_per_launch": nb, "avg_ms": kms, "launches": calls,
                     "isolated_kernels_ms": prof}
+
+    # the same workload in the mixed form (FP32 matrices and data, FP64
+    # iterates, anchors and KKT, FP64 polish near the tolerance): solves/s and
+    # its error against the FP64 solves above
+    other = None
+    if args.precision == "fp64" and not args.no_mixed:
+        mp = HPRQP_parameters(verbose=False, precision="mixed", problem_type=ptype)
+        dm.solve(mp, hist_cap=16)
+        t = time.perf_counter()
+        mr = [dm.solve(mp, hist_cap=16) for _ in range(max(1, args.steps))]
+        mt = time.perf_counter() - t
+        ok = sum(1 for r in mr if r.status == "OPTIMAL" and r.residuals < 1e-6)
+        m0 = mr[-1]
+        other = {"precision": "mixed", "solves_per_s": ok / mt, "iterations": m0.iter,
+                 "fp64_polish_iterations": m0.polish_iter, "kkt": m0.residuals,
+                 "rel_obj_err_vs_fp64": abs(m0.primal_obj - r0.primal_obj) / (1.0 + abs(r0.primal_obj)),
+                 "rel_x_err_vs_fp64": float(np.max(np.abs(m0.x - r0.x)) / (1.0 + np.max(np.abs(r0.x))))}
+        log(f"mixed: {ok}/{len(mr)} solved, {other['solves_per_s']:.3f} solves/s, iter={m0.iter} "
+            f"(polish {m0.polish_iter}), rel obj err vs fp64 {other['rel_obj_err_vs_fp64']:.2e}")
 
     cpu = None
     if not args.no_cpu:
@@ -298,9 +333,11 @@ def main():
                    "nnz_Q": int(model.Q.nnz) if hasattr(model.Q, "nnz") else None,
                    "precision": args.precision, "iterations": r0.iter, "kkt": r0.residuals,
                    "solve_time_s": r0.time, "power_time_s": r0.power_time, "setup_time_s": r0.setup_time,
-                   "primal_obj": r0.primal_obj, "parallelism": "single GPU"},
+                   "primal_obj": r0.primal_obj, "row_product_shapes_Q_A_AT": list(r0.lanes),
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": other,
         "batched": (None if args.no_batch or args.workload != "grid"
                     else batched_leg(args.batch_count, args.plain_sample)),
     }

@@ -175,13 +175,14 @@ struct EpiZxw1 {
     const double *last_x;
     VT *x, *x_hat, *w_bar, *dx, *x_bar, *z_bar;
     double sigma, f1s, f2s, h1, h2;
+    double lS = 0.0, uS = 0.0;  // the bounds when l / u are null (one value for all)
     template <class AT>
     __device__ void operator()(int32_t i, AT qw_) const {
         const double qw = (double)qw_;
         if (Qw) Qw[i] = (VT)qw_;
         const double xi = (double)x[i], wi = (double)w[i];
         const double z_raw = xi + sigma * (-qw + (double)ATy[i] - (double)c[i]);
-        const double xb = proj(z_raw, (double)l[i], (double)u[i]);
+        const double xb = proj(z_raw, l ? (double)l[i] : lS, u ? (double)u[i] : uS);
         const double xh = 2.0 * xb - xi;
         const double xn = fma(h2, xh, h1 * last_x[i]);
         const double wb = DIAG ? fma((double)fact1[i], wi, (double)fact2[i] * xh)
@@ -222,7 +223,8 @@ struct EpiY {
     __device__ void operator()(int32_t i, AT ax_) const {
         const double ax = (double)ax_, yi = (double)y[i];
         const double s_raw = ax - fact1 * yi;
-        const double sp = proj(s_raw, (double)AL[i], (double)AU[i]);
+        const double lo = (double)AL[i];
+        const double sp = proj(s_raw, lo, AU ? (double)AU[i] : lo);  // AU null: AL = AU
         const double yb = fact2 * (sp - s_raw);
         const double yh = 2.0 * yb - yi;
         const double yn = fma(h2, yh, h1 * last_y[i]);
@@ -270,13 +272,14 @@ struct EpiNocZxw {
     const double *last_x, *last_w;
     VT *x, *x_hat, *w, *w_bar, *dx, *x_bar, *z_bar, *dw;
     double sigma, f1s, f2s, h1, h2;
+    double lS = 0.0, uS = 0.0;
     template <class AT>
     __device__ void operator()(int32_t i, AT aty_) const {
         const double aty = (double)aty_;
         ATy[i] = (VT)aty_;
         const double xi = (double)x[i], wi = (double)w[i];
         const double z_raw = xi + sigma * (-(double)Qw[i] + aty - (double)c[i]);
-        const double xb = proj(z_raw, (double)l[i], (double)u[i]);
+        const double xb = proj(z_raw, l ? (double)l[i] : lS, u ? (double)u[i] : uS);
         const double xh = 2.0 * xb - xi;
         const double xn = fma(h2, xh, h1 * last_x[i]);
         const double wb = DIAG ? fma((double)fact1[i], wi, (double)fact2[i] * xh)
@@ -353,13 +356,14 @@ struct EpiLpZx {
     const double *last_x;
     VT *x, *x_hat, *dx, *x_bar, *z_bar;
     double sigma, h1, h2;
+    double lS = 0.0, uS = 0.0;
     template <class AT>
     __device__ void operator()(int32_t i, AT aty_) const {
         const double aty = (double)aty_;
         ATy[i] = (VT)aty_;
         const double xi = (double)x[i];
         const double z_raw = xi + sigma * (aty - (double)c[i]);
-        const double xb = proj(z_raw, (double)l[i], (double)u[i]);
+        const double xb = proj(z_raw, l ? (double)l[i] : lS, u ? (double)u[i] : uS);
         const double xh = 2.0 * xb - xi;
         const double xn = fma(h2, xh, h1 * last_x[i]);
         if constexpr (FULL) {

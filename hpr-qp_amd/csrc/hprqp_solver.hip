@@ -183,6 +183,10 @@ struct Scaled {
     double b_scale = 1.0, c_scale = 1.0, norm_b = 0, norm_c = 0, norm_b_org = 0, norm_c_org = 0;
     bool Q_is_diag = false;
     double q_max_val = 0.0;
+    // one bound value for every variable (0 or +-inf, which scaling keeps),
+    // every constraint row an equality
+    bool l_uniform = false, u_uniform = false, all_equal_rows = false;
+    double l_value = 0.0, u_value = 0.0;
 };
 
 static void upload_vec(hipStream_t s, DevVec<double> &d, const std::vector<double> &h) {
@@ -222,6 +226,19 @@ static void build_scaled(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_para
     S.m = m;
     S.qop = md->Q;
     S.qkind = md->Q->kind;
+    auto uniform = [&](const std::vector<double> &v, double &val) {
+        if (v.empty()) return false;
+        const double a = v[0];
+        if (!(a == 0.0 || std::isinf(a))) return false;
+        for (double b : v)
+            if (b != a) return false;
+        val = a;
+        return true;
+    };
+    S.l_uniform = uniform(md->l, S.l_value);
+    S.u_uniform = uniform(md->u, S.u_value);
+    S.all_equal_rows = m > 0;
+    for (int32_t r = 0; r < m && S.all_equal_rows; ++r) S.all_equal_rows = md->AL[r] == md->AU[r];
     // raw data
     upload_csr(s, md->A, S.A);
     upload_csr(s, host_csr_transpose(md->A), S.AT);
@@ -392,6 +409,10 @@ struct Solver {
     // data in VT (aliases of the FP64 arrays when VT == double)
     DevVec<MT> c_own, l_own, u_own, AL_own, AU_own;
     const MT *cV, *lV, *uV, *ALV, *AUV;
+    // what the half-step epilogues read: null when every variable has the
+    // same bound (then lS / uS) or every row is an equality (AU = AL)
+    const MT *lE = nullptr, *uE = nullptr, *AUE = nullptr;
+    double lS = 0.0, uS = 0.0;
     double sigma = 1.0, lambda_max_A = 0.0, lambda_max_Q = 0.0, lambda = 0.0;
     bool noC = false;
     KernelTimer *kt = nullptr;  // times the first fused kernel of partial steps
@@ -476,6 +497,15 @@ struct Solver {
                 ensure_v32(S.qop->lat);
             }
         }
+        set_epilogue_bounds();
+    }
+
+    void set_epilogue_bounds() {
+        lE = S.l_uniform ? nullptr : lV;
+        uE = S.u_uniform ? nullptr : uV;
+        AUE = S.all_equal_rows ? nullptr : AUV;
+        lS = S.l_value;
+        uS = S.u_value;
     }
 
     // ---------------- Q applies -------------------------------------
@@ -689,8 +719,10 @@ struct Solver {
         e.Qw = Qw.p;
         e.ATy = ATy.p;
         e.c = cV;
-        e.l = lV;
-        e.u = uV;
+        e.l = lE;
+        e.u = uE;
+        e.lS = lS;
+        e.uS = uS;
         e.w = w.p;
         e.fact1 = fact1.p;
         e.fact2 = fact2.p;
@@ -719,7 +751,7 @@ struct Solver {
         if (m == 0) return;
         EpiY<VT, MT, FULL> e;
         e.AL = ALV;
-        e.AU = AUV;
+        e.AU = AUE;
         e.last_y = last_y.p;
         e.y = y.p;
         e.y_bar = y_bar.p;
@@ -754,8 +786,10 @@ struct Solver {
         e.ATy = ATy.p;
         e.Qw = Qw.p;
         e.c = cV;
-        e.l = lV;
-        e.u = uV;
+        e.l = lE;
+        e.u = uE;
+        e.lS = lS;
+        e.uS = uS;
         e.fact1 = fact1.p;
         e.fact2 = fact2.p;
         e.last_x = last_x.p;
@@ -811,8 +845,10 @@ struct Solver {
         EpiLpZx<VT, MT, FULL> e;
         e.ATy = ATy.p;
         e.c = cV;
-        e.l = lV;
-        e.u = uV;
+        e.l = lE;
+        e.u = uE;
+        e.lS = lS;
+        e.uS = uS;
         e.last_x = last_x.p;
         e.x = x.p;
         e.x_hat = x_hat.p;
@@ -1606,6 +1642,13 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     res->noC = noC;
     res->n_history = L.nh;
     res->polish_iter = polish_from >= 0 ? L.iter - polish_from : 0;
+    res->lanes_Q = S.Q.lanes;
+    res->lanes_A = S.A.lanes;
+    res->lanes_AT = S.AT.lanes;
+    if (S.qkind == QOP_LASSO) {
+        res->lanes_Q = S.qop->la.lanes;  // the LASSO operator's A (its A^T in lanes_AT)
+        res->lanes_AT = S.qop->lat.lanes;
+    }
     if (ktimer) {
         ktimer->flush();
         res->kernel_calls = ktimer->count;
@@ -1650,19 +1693,25 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
         each(s, a.ncol, [=] __device__(int64_t i) { xp[i] = (VT)1; });
     }
     EventTimer tm(s);
-    int best = a.lanes;
-    double best_ms = INFINITY;
     const double mean = a.nrow > 0 ? (double)a.nnz / (double)a.nrow : 0.0;
     std::vector<int> cand;
     for (int rb : {256, 128, 64})  // CSR-stream when a block's nonzeros fit in LDS
         if (mean * rb <= 0.75 * kStreamCap) cand.push_back(-rb);
     for (int L = 1; L <= 64; L *= 2) cand.push_back(L);
     if (mean >= 64.0) cand.push_back(kBlock);  // one block per row
-    for (int L : cand) {
-        const double ms = tm.avg_ms(reps, [&] {
+    auto time_of = [&](int L) {
+        return tm.avg_ms(reps, [&] {
             launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<VT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
         });
-        if (ms < best_ms * 0.98) {
+    };
+    // the heuristic shape stays unless another is clearly faster (10 %), so
+    // shapes within timing noise of each other do not flip from run to run
+    int best = a.lanes;
+    double best_ms = time_of(best);
+    for (int L : cand) {
+        if (L == a.lanes) continue;
+        const double ms = time_of(L);
+        if (ms < best_ms * 0.90) {
             best_ms = ms;
             best = L;
         }
@@ -1697,9 +1746,9 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
         if (tuned == 0) {
             if (mixed) {
                 ensure_v32(a);
-                tuned = tune_lanes<float>(s, a, 10);
+                tuned = tune_lanes<float>(s, a, 20);
             } else {
-                tuned = tune_lanes<double>(s, a, 10);
+                tuned = tune_lanes<double>(s, a, 20);
             }
         }
         a.lanes = tuned;

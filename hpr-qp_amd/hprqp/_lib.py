@@ -79,6 +79,10 @@ class hprqp_result(C.Structure):
         ("power_iter_Q", C.c_int64),
         ("kernel_ms", C.c_double),
         ("kernel_calls", C.c_int64),
+        ("lanes_Q", C.c_int32),
+        ("lanes_A", C.c_int32),
+        ("lanes_AT", C.c_int32),
+        ("lanes_pad", C.c_int32),
     ]
 
 

@@ -120,6 +120,7 @@ class HPRQP_results:
     power_iter_Q: int = 0
     kernel_ms: float = 0.0
     kernel_calls: int = 0
+    lanes: tuple = (0, 0, 0)  # row-product shapes used for Q, A, A^T
     history: np.ndarray = field(default_factory=lambda: np.zeros((0, 8)))
 
 
@@ -251,6 +252,7 @@ class DeviceModel:
             sigma0=r.sigma0, restarts=r.restarts, Q_is_diag=bool(r.Q_is_diag), noC=bool(r.noC),
             setup_time=r.setup_time, polish_iter=r.polish_iter, power_iter_A=r.power_iter_A,
             power_iter_Q=r.power_iter_Q, kernel_ms=r.kernel_ms, kernel_calls=r.kernel_calls,
+            lanes=(r.lanes_Q, r.lanes_A, r.lanes_AT),
             history=hist[: r.n_history].copy())
 
     def profile(self, params: HPRQP_parameters, reps: int = 50):

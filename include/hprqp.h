@@ -136,6 +136,10 @@ typedef struct {
     double kernel_ms;     /* average ms of the first fused step kernel when
                              params.time_kernels = 1 (0 otherwise) */
     int64_t kernel_calls; /* launches averaged into kernel_ms */
+    int32_t lanes_Q;      /* row-product shape used for Q, A, A^T (spmv_lanes */
+    int32_t lanes_A;      /* codes: 1..64 lanes per row, 256 a block per row, */
+    int32_t lanes_AT;     /* -64/-128/-256 CSR-stream rows per block) */
+    int32_t lanes_pad;
 } hprqp_result;
 
 /* User Q operator plugin (AbstractQOperator, Q_operator_interface.jl:33-92).
