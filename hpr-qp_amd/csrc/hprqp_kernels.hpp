@@ -106,6 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rowblock(int32_t nrows, const in
 // vector access of the epilogue is coalesced as well.  A block whose rows
 // hold more than kStreamCap nonzeros falls back to one thread per row.
 constexpr int kStreamCap = 2048;
+constexpr int kStreamU = 4;  // nonzeros per thread per pass of the product phase
 
 template <class MT, class XT, class AT, class Epi>
 __global__ __launch_bounds__(kBlock) void k_csr_stream(int32_t nrows, int rb, const int32_t *__restrict__ rp,
@@ -122,7 +123,29 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(int32_t nrows, int rb, co
     const int t = threadIdx.x;
     const int32_t row = r0 + t;
     if (cnt <= kStreamCap) {
-        for (int32_t k = t; k < cnt; k += kBlock) prod[k] = (AT)v[base + k] * (AT)x[ci[base + k]];
+        // kStreamU nonzeros per thread per pass, all loads issued before the
+        // first use; lanes past cnt load the block's first nonzero (cnt >= 1
+        // inside the loop, so it exists) and discard it, so no load waits
+        // behind a branch
+        for (int32_t k0 = 0; k0 < cnt; k0 += kStreamU * kBlock) {
+            MT vv[kStreamU];
+            int32_t cc[kStreamU];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) {
+                const int32_t k = k0 + j * kBlock + t;
+                const int32_t at = base + (k < cnt ? k : 0);
+                vv[j] = v[at];
+                cc[j] = ci[at];
+            }
+            AT pr[kStreamU];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) pr[j] = (AT)vv[j] * (AT)x[cc[j]];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) {
+                const int32_t k = k0 + j * kBlock + t;
+                if (k < cnt) prod[k] = pr[j];
+            }
+        }
         __syncthreads();
         if (row < r1) {
             const int32_t s = rp[row] - base, e = rp[row + 1] - base;
