@@ -94,7 +94,8 @@ typedef struct {
                                    -64/-128/-256 = CSR-stream with that many
                                    rows per block */
     int32_t time_kernels;       /* 1 = time the dominant fused kernel with
-                                   HIP events during the solve (result.kernel_ms) */
+                                   HIP events during the solve (result.kernel_ms);
+                                   one launch in eight carries the events */
     int32_t polish_stall_checks; /* reduced precision: residual checks without
                                     the KKT residual halving before the FP64
                                     polish takes over (0 = automatic: 100 for
@@ -135,7 +136,7 @@ typedef struct {
     int64_t power_iter_Q; /* power-method iterations for lambda_max(Q) */
     double kernel_ms;     /* average ms of the first fused step kernel when
                              params.time_kernels = 1 (0 otherwise) */
-    int64_t kernel_calls; /* launches averaged into kernel_ms */
+    int64_t kernel_calls; /* launches averaged into kernel_ms (every eighth) */
     int32_t lanes_Q;      /* row-product shape used for Q, A, A^T (spmv_lanes */
     int32_t lanes_A;      /* codes: 1..64 lanes per row, 256 a block per row, */
     int32_t lanes_AT;     /* -64/-128/-256 CSR-stream rows per block) */
