@@ -16,7 +16,8 @@ learned MLP, against the plain FP64 form solving them one per call.
 
 --workload random|lasso|qap runs the same single-instance measurement on
 the other workloads (random sparse QP n=10,000; LASSO 3,308 x 150,360; QAP
-128 x 128), --workload batch only the batched leg; these are for profiling.
+128 x 128), --workload batch only the batched leg, --workload qap-apply the
+QAP operator apply alone at several sizes; these are for profiling.
 """
 from __future__ import annotations
 
@@ -165,6 +166,50 @@ def batched_leg(count, plain_sample):
     return out
 
 
+def qap_apply_leg(sizes, reps):
+    """The QAP operator apply Q(X) = 2(A X B - S X - X T) alone, buffers
+    resident in HBM, at several sizes: FP64 MFMA and split-BF16 x3 MFMA,
+    diagonal S and T and integer A, B (the QAP-relaxation form: 4 n^3 flops
+    per apply).  Times reps applies back to back per size and precision."""
+    import numpy as np
+
+    from hprqp import _lib
+    from hprqp.model import QAP_Q_operator
+    from hprqp.solver import DeviceQOperator
+
+    ctx = _lib.Context(0)
+    lib = ctx.lib
+    out = {"metric": "QAP operator applies/sec and MFMA TFLOP/s", "unit": "TFLOP/s", "sizes": []}
+    for nq in sizes:
+        rng = np.random.default_rng(nq)
+        F = np.triu(rng.integers(0, 11, (nq, nq)) * (rng.random((nq, nq)) < 0.1), 1).astype(np.float64)
+        F = F + F.T
+        pts = rng.integers(0, 32, (nq, 2))
+        D = (np.abs(pts[:, None, 0] - pts[None, :, 0]) + np.abs(pts[:, None, 1] - pts[None, :, 1])).astype(float)
+        I = np.eye(nq)
+        op = DeviceQOperator(ctx, QAP_Q_operator(F / 2.0, D, -1.0 * I, -1.0 * I), nq * nq)
+        xb = _lib.DeviceBuffer.from_array(ctx, rng.random(nq * nq))
+        qb = _lib.DeviceBuffer(ctx, nq * nq)
+        fl = 4.0 * nq ** 3
+        row = {"n": nq, "flops_per_apply": fl}
+        for prec, code, peak in (("fp64", 0, 78.6), ("bf16x3", 1, 2500.0 / 3.0)):
+            def run(k):
+                for _ in range(k):
+                    _lib.check(ctx.h, lib.hprqp_qop_apply_prec(ctx.h, op.h, xb.h, qb.h, code))
+                qb.download()  # waits for the stream
+            run(3)
+            t = time.perf_counter()
+            run(reps)
+            dt = (time.perf_counter() - t) / reps
+            tf = fl / dt / 1e12
+            row[prec] = {"us_per_apply": dt * 1e6, "tflops": tf, "peak_tflops": peak, "frac": tf / peak}
+            log(f"QAP apply n={nq} ({nq * nq} variables) {prec}: {dt * 1e6:.1f} us, {tf:.2f} TFLOP/s "
+                f"({100 * tf / peak:.1f} % of {peak:.1f})")
+        out["sizes"].append(row)
+        op.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,7 +227,10 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="--workload copy: bytes per copy (MiB) for calibrating the HBM byte counters")
-    ap.add_argument("--workload", default="grid", choices=["grid", "random", "lasso", "qap", "batch", "copy"],
+    ap.add_argument("--qap-sizes", default="128,256,512,1024,2048",
+                    help="--workload qap-apply: QAP sizes n (X is n x n)")
+    ap.add_argument("--workload", default="grid",
+                    choices=["grid", "random", "lasso", "qap", "batch", "copy", "qap-apply"],
                     help="grid (default): the headline single-instance workload; random / lasso / qap: the "
                          "other single-instance workloads (for profiling); batch: only the batched leg")
     args = ap.parse_args()
@@ -228,6 +276,9 @@ def main():
         print(json.dumps({"metric": "device copy GB/s (read + write)", "value": 2 * nb / dt / 1e9, "unit": "GB/s",
                           "bytes_read_per_copy": nb, "bytes_written_per_copy": nb, "steps": args.steps,
                           "read_only_gbs": read_gbs, "write_only_gbs": write_gbs}), flush=True)
+        return
+    if args.workload == "qap-apply":
+        print(json.dumps(qap_apply_leg([int(v) for v in args.qap_sizes.split(",")], args.steps)), flush=True)
         return
     if args.workload == "batch":
         out = batched_leg(args.batch_count, args.plain_sample)

@@ -167,6 +167,10 @@ struct hprqp_qop {
     bool qa_exact = false, qb_exact = false, qs_exact = false, qt_exact = false;
     hprqp::DevVec<double> qsd, qtd;
     hprqp::DevVec<uint16_t> qa0, qa1, qb0, qb1, qs0, qs1, qt0, qt1;
+    // large-n split-BF16 passes: BF16 pieces of the data operands, each
+    // qnp x qnp with zero padding, three pieces back to back: X transposed
+    // (qxp), Y row-major (qyp, written by pass 1), X row-major (qxr, full T)
+    hprqp::DevVec<uint16_t> qxp, qyp, qxr;
     // diagonal plus low rank: Q = diag(d) + U U^T, U n x k column-major
     int32_t lr_k = 0;
     hprqp::DevVec<double> lr_U, lr_d, lr_t;

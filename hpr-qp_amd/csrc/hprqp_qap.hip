@@ -69,7 +69,19 @@ void qap_prepare(hipStream_t s, hprqp_qop *q, const double *A, const double *B, 
     const int64_t nn = (int64_t)n * n;
     q->qtemp.alloc(2 * nn);
     q->qtemp32.alloc(2 * nn);
-    (void)s;
+    // the large-n split-BF16 passes read their data operands as BF16 pieces;
+    // the padding stays zero (the kernels write only the n x n part)
+    if (n >= kQapBlockedMin) {
+        q->qxp.alloc(3 * pp);
+        q->qyp.alloc(3 * pp);
+        HPRQP_HIP(hipMemsetAsync(q->qxp.p, 0, sizeof(uint16_t) * 3 * pp, s));
+        HPRQP_HIP(hipMemsetAsync(q->qyp.p, 0, sizeof(uint16_t) * 3 * pp, s));
+        if (!q->qt_diag) {
+            q->qxr.alloc(3 * pp);
+            HPRQP_HIP(hipMemsetAsync(q->qxr.p, 0, sizeof(uint16_t) * 3 * pp, s));
+        }
+        HPRQP_HIP(hipStreamSynchronize(s));
+    }
 }
 
 void qap_apply_f64(hipStream_t s, const hprqp_qop *q, const double *X, double *QX) {

@@ -271,6 +271,166 @@ __global__ __launch_bounds__(64) void k_qap64_p2(QapArgs g, const double *__rest
     }
 }
 
+// ---------------- FP64 passes, large n ----------------------------------------
+// 256-thread blocks; a block owns a (16 kBT) x (16 kBT) output block held as
+// kBT x kBT MFMA tiles with independent accumulators (kBT^2 MFMA chains in
+// flight per wave instead of one), and its four waves split the K range
+// (wave w takes the 4-wide K steps w, w+4, w+8, ...), so a block is four
+// times the parallelism of one wave per block and no wave repeats another's
+// loads.  The partial accumulators are summed through LDS in wave order
+// (deterministic).  Operands are read straight from global memory: the
+// n x n operands stay in L2 at these sizes.  Rows and columns past n are
+// clamped to n-1 (their results are not stored); the last, partial K step
+// zeroes the entries past n.
+constexpr int kBT = 2;                 // MFMA tiles per block in each direction
+constexpr int kBE = 16 * kBT;          // block tile edge
+constexpr int kQapBlockedMin = 384;    // n from which the blocked passes run
+
+// sum of the four waves' accumulators; wave w then owns components
+// q = w, w+4, ... of the kBT x kBT x 4 per-lane results
+__device__ __forceinline__ void wave_sum(doublex4 (&acc)[kBT][kBT], double *red, double (&out)[kBT * kBT]) {
+    constexpr int NC = kBT * kBT * 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < kBT; ++a)
+#pragma unroll
+        for (int b = 0; b < kBT; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) red[((w * NC) + (a * kBT + b) * 4 + e) * 64 + lane] = acc[a][b][e];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NC / 4; ++t) {
+        const int q = w + 4 * t;
+        double v = red[q * 64 + lane];
+#pragma unroll
+        for (int u = 1; u < 4; ++u) v += red[((u * NC) + q) * 64 + lane];
+        out[t] = v;
+    }
+    __syncthreads();
+}
+
+static __global__ __launch_bounds__(256) void k_qap64b_p1(QapArgs g, const double *__restrict__ x,
+                                                          double *__restrict__ Y, double *__restrict__ Z) {
+    __shared__ double red[4 * kBT * kBT * 4 * 64];
+    const int n = g.n, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int r0 = blockIdx.x * kBE, c0 = blockIdx.y * kBE;
+    const double *pa[kBT], *ps[kBT], *px[kBT];
+#pragma unroll
+    for (int t = 0; t < kBT; ++t) {
+        const int r = min(r0 + 16 * t + li, n - 1), c = min(c0 + 16 * t + li, n - 1);
+        pa[t] = g.A + r;  // A[r + k n]
+        ps[t] = g.S + r;
+        px[t] = x + (int64_t)c * n;  // X[k + c n]
+    }
+    doublex4 ay[kBT][kBT], az[kBT][kBT];
+#pragma unroll
+    for (int a = 0; a < kBT; ++a)
+#pragma unroll
+        for (int b = 0; b < kBT; ++b) {
+            ay[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+            az[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+        }
+    const int nk = n & ~3;
+    auto step = [&](int k, bool tail) {
+        const bool in = !tail || k < n;
+        const int kk = in ? k : 0;
+        const int64_t kn = (int64_t)kk * n;
+        double fa[kBT], fs[kBT], fx[kBT];
+#pragma unroll
+        for (int t = 0; t < kBT; ++t) {
+            fa[t] = in ? pa[t][kn] : 0.0;
+            fs[t] = in && g.s_full ? ps[t][kn] : 0.0;
+            fx[t] = in ? px[t][kk] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < kBT; ++a)
+#pragma unroll
+            for (int b = 0; b < kBT; ++b) {
+                ay[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a], fx[b], ay[a][b], 0, 0, 0);
+                if (g.s_full) az[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fs[a], fx[b], az[a][b], 0, 0, 0);
+            }
+    };
+#pragma unroll 4
+    for (int k0 = 4 * w; k0 < nk; k0 += 16) step(k0 + lg, false);
+    if (nk < n && w == ((nk >> 2) & 3)) step(nk + lg, true);
+    double vy[kBT * kBT], vz[kBT * kBT];
+    wave_sum(ay, red, vy);
+    if (g.s_full) wave_sum(az, red, vz);
+#pragma unroll
+    for (int t = 0; t < kBT * kBT; ++t) {
+        const int q = w + 4 * t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
+        const int r = r0 + 16 * a + lg + 4 * e, j = c0 + 16 * b + li;
+        if (r >= n || j >= n) continue;
+        Y[r + (int64_t)j * n] = vy[t];
+        if (g.s_full) Z[r + (int64_t)j * n] = vz[t];
+    }
+}
+
+template <class Epi>
+__global__ __launch_bounds__(256) void k_qap64b_p2(QapArgs g, const double *__restrict__ x,
+                                                   const double *__restrict__ Y, const double *__restrict__ Z,
+                                                   Epi epi) {
+    __shared__ double red[4 * kBT * kBT * 4 * 64];
+    const int n = g.n, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int r0 = blockIdx.x * kBE, c0 = blockIdx.y * kBE;
+    const double *py[kBT], *pxr[kBT], *pb[kBT], *pt[kBT];
+#pragma unroll
+    for (int t = 0; t < kBT; ++t) {
+        const int r = min(r0 + 16 * t + li, n - 1), c = min(c0 + 16 * t + li, n - 1);
+        py[t] = Y + r;  // Y[r + k n]
+        pxr[t] = x + r;
+        pb[t] = g.B + (int64_t)c * n;  // B[k + c n]
+        pt[t] = g.T + (int64_t)c * n;
+    }
+    doublex4 acc[kBT][kBT];
+#pragma unroll
+    for (int a = 0; a < kBT; ++a)
+#pragma unroll
+        for (int b = 0; b < kBT; ++b) acc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+    const int nk = n & ~3;
+    auto step = [&](int k, bool tail) {
+        const bool in = !tail || k < n;
+        const int kk = in ? k : 0;
+        const int64_t kn = (int64_t)kk * n;
+        double fy[kBT], fx[kBT], fb[kBT], ft[kBT];
+#pragma unroll
+        for (int t = 0; t < kBT; ++t) {
+            fy[t] = in ? py[t][kn] : 0.0;
+            fx[t] = in && g.t_full ? -pxr[t][kn] : 0.0;
+            fb[t] = in ? pb[t][kk] : 0.0;
+            ft[t] = in && g.t_full ? pt[t][kk] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < kBT; ++a)
+#pragma unroll
+            for (int b = 0; b < kBT; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fy[a], fb[b], acc[a][b], 0, 0, 0);
+                if (g.t_full) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fx[a], ft[b], acc[a][b], 0, 0, 0);
+            }
+    };
+#pragma unroll 4
+    for (int k0 = 4 * w; k0 < nk; k0 += 16) step(k0 + lg, false);
+    if (nk < n && w == ((nk >> 2) & 3)) step(nk + lg, true);
+    double v[kBT * kBT];
+    wave_sum(acc, red, v);
+#pragma unroll
+    for (int t = 0; t < kBT * kBT; ++t) {
+        const int q = w + 4 * t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
+        const int r = r0 + 16 * a + lg + 4 * e, j = c0 + 16 * b + li;
+        if (r >= n || j >= n) continue;
+        const int64_t o = r + (int64_t)j * n;
+        double u = v[t];
+        if (g.s_full)
+            u -= Z[o];
+        else
+            u -= g.sd[r] * x[o];
+        if (!g.t_full) u -= x[o] * g.td[j];
+        epi((int32_t)o, 2.0 * u);
+    }
+}
+
 // ---------------- split-BF16 x3 passes ---------------------------------------
 constexpr int kLdB = kKS + 8;  // u16 row stride of staged coefficient pieces
 constexpr int kLdF = kKS + 4;  // float row stride of staged data
@@ -398,6 +558,205 @@ __global__ __launch_bounds__(64) void k_qapbf_p2(QapArgs g, const IT *__restrict
     }
 }
 
+// ---------------- split-BF16 x3 passes, large n ------------------------------
+// The blocking of the large-n FP64 passes (a block owns a 32 x 32 output as
+// 2 x 2 MFMA tiles; its four waves split K and are summed through LDS), with
+// both operands read as BF16 pieces: the data operand is split once per
+// apply (X by k_qap_split, Y written as pieces by pass 1), so the MFMA loop
+// is 16-byte fragment loads and MFMAs only.  Pass 1 forms the transpose
+// Y^T = X^T A^T, whose result tiles store as row-major Y -- the layout pass 2
+// reads as its left operand.  Every piece array is qnp x qnp with zero
+// padding, so no load needs a bound check (qnp is a multiple of the 32-wide
+// block edge and of the 32-deep K step).
+__device__ __forceinline__ bf16x8 ld_frag(const unsigned short *__restrict__ P, int64_t o) {
+    return __builtin_bit_cast(bf16x8, *(const u16x8 *)(P + o));
+}
+// acc += D C for data pieces d (d[2] used when c is exact) and coefficient c
+__device__ __forceinline__ floatx4 mma_dc(floatx4 acc, const bf16x8 (&d)[3], bf16x8 c0, bf16x8 c1, bool exact) {
+    if (exact) {
+        acc = mfma_bf16(d[2], c0, acc);
+        acc = mfma_bf16(d[1], c0, acc);
+        acc = mfma_bf16(d[0], c0, acc);
+    } else {
+        acc = mfma_bf16(d[0], c1, acc);
+        acc = mfma_bf16(d[1], c0, acc);
+        acc = mfma_bf16(d[0], c0, acc);
+    }
+    return acc;
+}
+__device__ __forceinline__ void wave_sum_f(floatx4 (&acc)[kBT][kBT], float *red, float (&out)[kBT * kBT]) {
+    constexpr int NC = kBT * kBT * 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < kBT; ++a)
+#pragma unroll
+        for (int b = 0; b < kBT; ++b)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) red[((w * NC) + (a * kBT + b) * 4 + e) * 64 + lane] = acc[a][b][e];
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NC / 4; ++t) {
+        const int q = w + 4 * t;
+        float v = red[q * 64 + lane];
+#pragma unroll
+        for (int u = 1; u < 4; ++u) v += red[((u * NC) + q) * 64 + lane];
+        out[t] = v;
+    }
+    __syncthreads();
+}
+
+// X (column-major n x n) into three BF16 pieces of X^T: P[j np + k] =
+// piece(X[k + j n]); with R, also of X row-major: R[k np + j]
+template <class IT>
+__global__ __launch_bounds__(256) void k_qap_split(int n, int np, const IT *__restrict__ x,
+                                                   unsigned short *__restrict__ P, unsigned short *__restrict__ R) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x, nn = (int64_t)n * n, pp = (int64_t)np * np;
+    if (idx >= nn) return;
+    const int k = (int)(idx % n), j = (int)(idx / n);
+    const float v = (float)x[idx];
+    const unsigned short h0 = bf16_rn(v);
+    const float r1 = v - bf16_f(h0);
+    const unsigned short h1 = bf16_rn(r1);
+    const unsigned short h2 = bf16_rn(r1 - bf16_f(h1));
+    const int64_t o = (int64_t)j * np + k;
+    P[o] = h0;
+    P[pp + o] = h1;
+    P[2 * pp + o] = h2;
+    if (R) {
+        const int64_t t = (int64_t)k * np + j;
+        R[t] = h0;
+        R[pp + t] = h1;
+        R[2 * pp + t] = h2;
+    }
+}
+
+// pass 1: Y^T = X^T A^T (and Z^T = X^T S^T for a full S); Y stored as
+// pieces, Z as FP32 column-major
+static __global__ __launch_bounds__(256) void k_qapbfb_p1(QapArgs g, const unsigned short *__restrict__ XP,
+                                                          unsigned short *__restrict__ YP, float *__restrict__ Z) {
+    __shared__ float red[4 * kBT * kBT * 4 * 64];
+    const int n = g.n, np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int64_t pp = (int64_t)np * np;
+    const int j0 = blockIdx.x * kBE, i0 = blockIdx.y * kBE;  // Y^T rows j, columns i
+    floatx4 ay[kBT][kBT], az[kBT][kBT];
+#pragma unroll
+    for (int a = 0; a < kBT; ++a)
+#pragma unroll
+        for (int b = 0; b < kBT; ++b) {
+            ay[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+            az[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+    for (int ks = w; ks * 32 < np; ks += 4) {
+        const int k = ks * 32 + 8 * lg;
+        bf16x8 d[kBT][3], ca0[kBT], ca1[kBT], cs0[kBT], cs1[kBT];
+#pragma unroll
+        for (int t = 0; t < kBT; ++t) {
+            const int64_t od = (int64_t)(j0 + 16 * t + li) * np + k, oc = (int64_t)(i0 + 16 * t + li) * np + k;
+            d[t][0] = ld_frag(XP, od);
+            d[t][1] = ld_frag(XP, pp + od);
+            d[t][2] = ld_frag(XP, 2 * pp + od);
+            ca0[t] = ld_frag(g.A0, oc);
+            ca1[t] = g.a_exact ? ca0[t] : ld_frag(g.A1, oc);
+            if (g.s_full) {
+                cs0[t] = ld_frag(g.S0, oc);
+                cs1[t] = g.s_exact ? cs0[t] : ld_frag(g.S1, oc);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kBT; ++a)
+#pragma unroll
+            for (int b = 0; b < kBT; ++b) {
+                ay[a][b] = mma_dc(ay[a][b], d[a], ca0[b], ca1[b], g.a_exact);
+                if (g.s_full) az[a][b] = mma_dc(az[a][b], d[a], cs0[b], cs1[b], g.s_exact);
+            }
+    }
+    float vy[kBT * kBT], vz[kBT * kBT];
+    wave_sum_f(ay, red, vy);
+    if (g.s_full) wave_sum_f(az, red, vz);
+#pragma unroll
+    for (int t = 0; t < kBT * kBT; ++t) {
+        const int q = w + 4 * t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
+        const int j = j0 + 16 * a + 4 * lg + e, i = i0 + 16 * b + li;
+        if (i >= n || j >= n) continue;
+        const float v = vy[t];
+        const unsigned short h0 = bf16_rn(v);
+        const float r1 = v - bf16_f(h0);
+        const unsigned short h1 = bf16_rn(r1);
+        const int64_t o = (int64_t)i * np + j;
+        YP[o] = h0;
+        YP[pp + o] = h1;
+        YP[2 * pp + o] = bf16_rn(r1 - bf16_f(h1));
+        if (g.s_full) Z[i + (int64_t)j * n] = vz[t];
+    }
+}
+
+// pass 2: R = Y B - X T (full T), the diagonal terms and the epilogue
+template <class IT, class Epi>
+__global__ __launch_bounds__(256) void k_qapbfb_p2(QapArgs g, const IT *__restrict__ x,
+                                                   const unsigned short *__restrict__ YP,
+                                                   const unsigned short *__restrict__ XR,
+                                                   const float *__restrict__ Z, Epi epi) {
+    __shared__ float red[4 * kBT * kBT * 4 * 64];
+    const int n = g.n, np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lg = lane >> 4;
+    const int64_t pp = (int64_t)np * np;
+    const int i0 = blockIdx.x * kBE, j0 = blockIdx.y * kBE;
+    floatx4 acc[kBT][kBT], at[kBT][kBT];
+#pragma unroll
+    for (int a = 0; a < kBT; ++a)
+#pragma unroll
+        for (int b = 0; b < kBT; ++b) {
+            acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+            at[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+    for (int ks = w; ks * 32 < np; ks += 4) {
+        const int k = ks * 32 + 8 * lg;
+        bf16x8 d[kBT][3], dx[kBT][3], cb0[kBT], cb1[kBT], ct0[kBT], ct1[kBT];
+#pragma unroll
+        for (int t = 0; t < kBT; ++t) {
+            const int64_t od = (int64_t)(i0 + 16 * t + li) * np + k, oc = (int64_t)(j0 + 16 * t + li) * np + k;
+            d[t][0] = ld_frag(YP, od);
+            d[t][1] = ld_frag(YP, pp + od);
+            d[t][2] = ld_frag(YP, 2 * pp + od);
+            cb0[t] = ld_frag(g.B0, oc);
+            cb1[t] = g.b_exact ? cb0[t] : ld_frag(g.B1, oc);
+            if (g.t_full) {
+                dx[t][0] = ld_frag(XR, od);
+                dx[t][1] = ld_frag(XR, pp + od);
+                dx[t][2] = ld_frag(XR, 2 * pp + od);
+                ct0[t] = ld_frag(g.T0, oc);
+                ct1[t] = g.t_exact ? ct0[t] : ld_frag(g.T1, oc);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kBT; ++a)
+#pragma unroll
+            for (int b = 0; b < kBT; ++b) {
+                acc[a][b] = mma_dc(acc[a][b], d[a], cb0[b], cb1[b], g.b_exact);
+                if (g.t_full) at[a][b] = mma_dc(at[a][b], dx[a], ct0[b], ct1[b], g.t_exact);
+            }
+    }
+    float v[kBT * kBT], vt[kBT * kBT];
+    wave_sum_f(acc, red, v);
+    if (g.t_full) wave_sum_f(at, red, vt);
+#pragma unroll
+    for (int t = 0; t < kBT * kBT; ++t) {
+        const int q = w + 4 * t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
+        const int r = i0 + 16 * a + 4 * lg + e, j = j0 + 16 * b + li;
+        if (r >= n || j >= n) continue;
+        const int64_t o = r + (int64_t)j * n;
+        double u = (double)v[t];
+        if (g.t_full) u -= (double)vt[t];
+        if (g.s_full)
+            u -= (double)Z[o];
+        else
+            u -= g.sd[r] * (double)x[o];
+        if (!g.t_full) u -= (double)x[o] * g.td[j];
+        epi((int32_t)o, 2.0 * u);
+    }
+}
+
 // Q x with the result handed to epi(i, (Q x)_i): FP64 / split-BF16 x3
 template <class Epi>
 void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Epi &epi) {
@@ -405,6 +764,14 @@ void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Ep
     const int nt = (g.n + 15) / 16;
     const int64_t nn = (int64_t)g.n * g.n;
     double *Y = q->qtemp.p, *Z = q->qtemp.p + nn;
+    if (g.n >= kQapBlockedMin) {
+        const int nb = (g.n + kBE - 1) / kBE;
+        launch_k(k_qap64b_p1, dim3(nb, nb), dim3(256), 0, s, g, X, Y, Z);
+        HPRQP_HIP(hipGetLastError());
+        launch_k(k_qap64b_p2<Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const double *)Y, (const double *)Z, epi);
+        HPRQP_HIP(hipGetLastError());
+        return;
+    }
     const dim3 grid(nt, nt);
     launch_k(k_qap64_p1, grid, dim3(64), 0, s, g, X, Y, Z);
     HPRQP_HIP(hipGetLastError());
@@ -418,6 +785,18 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
     const int nt = (g.n + 15) / 16;
     const int64_t nn = (int64_t)g.n * g.n;
     float *Y = q->qtemp32.p, *Z = q->qtemp32.p + nn;
+    if (g.n >= kQapBlockedMin) {
+        const int nb = g.np / kBE;
+        launch_k(k_qap_split<IT>, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, g.n, g.np, X, q->qxp.p,
+                 g.t_full ? q->qxr.p : (unsigned short *)nullptr);
+        HPRQP_HIP(hipGetLastError());
+        launch_k(k_qapbfb_p1, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p, Z);
+        HPRQP_HIP(hipGetLastError());
+        launch_k(k_qapbfb_p2<IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
+                 (const unsigned short *)q->qxr.p, (const float *)Z, epi);
+        HPRQP_HIP(hipGetLastError());
+        return;
+    }
     const dim3 grid(nt, nt);
     launch_k(k_qapbf_p1<IT>, grid, dim3(64), 0, s, g, X, Y, Z);
     HPRQP_HIP(hipGetLastError());
