@@ -170,7 +170,9 @@ def qap_apply_leg(sizes, reps):
     """The QAP operator apply Q(X) = 2(A X B - S X - X T) alone, buffers
     resident in HBM, at several sizes: FP64 MFMA and split-BF16 x3 MFMA,
     diagonal S and T and integer A, B (the QAP-relaxation form: 4 n^3 flops
-    per apply).  Times reps applies back to back per size and precision."""
+    per apply).  Times back-to-back applies per size and precision (at least
+    reps, and at least ~2 GFLOP of them) between two stream synchronisations;
+    at small n the figure includes the host's per-call cost."""
     import numpy as np
 
     from hprqp import _lib
@@ -196,11 +198,13 @@ def qap_apply_leg(sizes, reps):
             def run(k):
                 for _ in range(k):
                     _lib.check(ctx.h, lib.hprqp_qop_apply_prec(ctx.h, op.h, xb.h, qb.h, code))
-                qb.download()  # waits for the stream
+                _lib.check(ctx.h, lib.hprqp_ctx_sync(ctx.h))
+
+            k = max(reps, int(2e9 / fl) + 1)  # at least ~2 GFLOP of applies per size
             run(3)
             t = time.perf_counter()
-            run(reps)
-            dt = (time.perf_counter() - t) / reps
+            run(k)
+            dt = (time.perf_counter() - t) / k
             tf = fl / dt / 1e12
             row[prec] = {"us_per_apply": dt * 1e6, "tflops": tf, "peak_tflops": peak, "frac": tf / peak}
             log(f"QAP apply n={nq} ({nq * nq} variables) {prec}: {dt * 1e6:.1f} us, {tf:.2f} TFLOP/s "
@@ -227,7 +231,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024,
                     help="--workload copy: bytes per copy (MiB) for calibrating the HBM byte counters")
-    ap.add_argument("--qap-sizes", default="128,256,512,1024,2048",
+    ap.add_argument("--qap-sizes", default="128,256,512,1024,2048,4096",
                     help="--workload qap-apply: QAP sizes n (X is n x n)")
     ap.add_argument("--workload", default="grid",
                     choices=["grid", "random", "lasso", "qap", "batch", "copy", "qap-apply"],

@@ -11,7 +11,9 @@ namespace hprqp {
 
 void qap_prepare(hipStream_t s, hprqp_qop *q, const double *A, const double *B, const double *S, const double *T) {
     const int n = q->nq;
-    const int np = (n + 31) / 32 * 32;
+    // pieces padded to a multiple of 32 (the K step), of 128 (the largest
+    // block edge) where the blocked passes run
+    const int np = n >= kQapBlockedMin ? (n + 127) / 128 * 128 : (n + 31) / 32 * 32;
     q->qnp = np;
     const int64_t pp = (int64_t)np * np;
     auto is_diag = [&](const double *M) {

@@ -287,7 +287,8 @@ constexpr int kBE = 16 * kBT;          // block tile edge
 constexpr int kQapBlockedMin = 384;    // n from which the blocked passes run
 
 // sum of the four waves' accumulators; wave w then owns components
-// q = w, w+4, ... of the kBT x kBT x 4 per-lane results
+// q = 4w .. 4w+3 of the kBT x kBT x 4 per-lane results: the four elements
+// of tile w (kBT = 2: one tile per wave)
 __device__ __forceinline__ void wave_sum(doublex4 (&acc)[kBT][kBT], double *red, double (&out)[kBT * kBT]) {
     constexpr int NC = kBT * kBT * 4;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -300,7 +301,7 @@ __device__ __forceinline__ void wave_sum(doublex4 (&acc)[kBT][kBT], double *red,
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < NC / 4; ++t) {
-        const int q = w + 4 * t;
+        const int q = 4 * w + t;
         double v = red[q * 64 + lane];
 #pragma unroll
         for (int u = 1; u < 4; ++u) v += red[((u * NC) + q) * 64 + lane];
@@ -359,7 +360,7 @@ static __global__ __launch_bounds__(256) void k_qap64b_p1(QapArgs g, const doubl
     if (g.s_full) wave_sum(az, red, vz);
 #pragma unroll
     for (int t = 0; t < kBT * kBT; ++t) {
-        const int q = w + 4 * t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
+        const int q = 4 * w + t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
         const int r = r0 + 16 * a + lg + 4 * e, j = c0 + 16 * b + li;
         if (r >= n || j >= n) continue;
         Y[r + (int64_t)j * n] = vy[t];
@@ -417,7 +418,7 @@ __global__ __launch_bounds__(256) void k_qap64b_p2(QapArgs g, const double *__re
     wave_sum(acc, red, v);
 #pragma unroll
     for (int t = 0; t < kBT * kBT; ++t) {
-        const int q = w + 4 * t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
+        const int q = 4 * w + t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
         const int r = r0 + 16 * a + lg + 4 * e, j = c0 + 16 * b + li;
         if (r >= n || j >= n) continue;
         const int64_t o = r + (int64_t)j * n;
@@ -584,6 +585,33 @@ __device__ __forceinline__ floatx4 mma_dc(floatx4 acc, const bf16x8 (&d)[3], bf1
     }
     return acc;
 }
+// acc[a][b] += D_a C_b for all a, b as three rounds, each issuing the
+// TW x TW independent MFMAs back to back (rather than each accumulator's
+// three dependent MFMAs in a row); per accumulator the terms are added in
+// the order of mma_dc
+template <int TW, bool EXACT>
+__device__ __forceinline__ void mma_rounds_(floatx4 (&acc)[TW][TW], const bf16x8 (&d)[TW][3],
+                                            const bf16x8 (&c0)[TW], const bf16x8 (&c1)[TW]) {
+#pragma unroll
+    for (int rd = 0; rd < 3; ++rd)
+#pragma unroll
+        for (int a = 0; a < TW; ++a)
+#pragma unroll
+            for (int b = 0; b < TW; ++b) {
+                if constexpr (EXACT)
+                    acc[a][b] = mfma_bf16(d[a][2 - rd], c0[b], acc[a][b]);
+                else
+                    acc[a][b] = mfma_bf16(d[a][rd == 1 ? 1 : 0], rd == 0 ? c1[b] : c0[b], acc[a][b]);
+            }
+}
+template <int TW>
+__device__ __forceinline__ void mma_rounds(floatx4 (&acc)[TW][TW], const bf16x8 (&d)[TW][3], const bf16x8 (&c0)[TW],
+                                           const bf16x8 (&c1)[TW], bool exact) {
+    if (exact)
+        mma_rounds_<TW, true>(acc, d, c0, c1);
+    else
+        mma_rounds_<TW, false>(acc, d, c0, c1);
+}
 __device__ __forceinline__ void wave_sum_f(floatx4 (&acc)[kBT][kBT], float *red, float (&out)[kBT * kBT]) {
     constexpr int NC = kBT * kBT * 4;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -596,13 +624,32 @@ __device__ __forceinline__ void wave_sum_f(floatx4 (&acc)[kBT][kBT], float *red,
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < NC / 4; ++t) {
-        const int q = w + 4 * t;
+        const int q = 4 * w + t;
         float v = red[q * 64 + lane];
 #pragma unroll
         for (int u = 1; u < 4; ++u) v += red[((u * NC) + q) * 64 + lane];
         out[t] = v;
     }
     __syncthreads();
+}
+
+// v[0..3] as three BF16 pieces at P[o .. o+3] (+ pp, + 2 pp): one 8-byte
+// store per piece (o is a multiple of 4)
+typedef __attribute__((ext_vector_type(4))) unsigned short u16x4;
+__device__ __forceinline__ void store_pieces4(unsigned short *__restrict__ P, int64_t pp, int64_t o, const float *v) {
+    u16x4 p0, p1, p2;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const unsigned short h0 = bf16_rn(v[e]);
+        const float r1 = v[e] - bf16_f(h0);
+        const unsigned short h1 = bf16_rn(r1);
+        p0[e] = h0;
+        p1[e] = h1;
+        p2[e] = bf16_rn(r1 - bf16_f(h1));
+    }
+    *(u16x4 *)(P + o) = p0;
+    *(u16x4 *)(P + pp + o) = p1;
+    *(u16x4 *)(P + 2 * pp + o) = p2;
 }
 
 // X (column-major n x n) into three BF16 pieces of X^T: P[j np + k] =
@@ -663,31 +710,22 @@ static __global__ __launch_bounds__(256) void k_qapbfb_p1(QapArgs g, const unsig
                 cs1[t] = g.s_exact ? cs0[t] : ld_frag(g.S1, oc);
             }
         }
-#pragma unroll
-        for (int a = 0; a < kBT; ++a)
-#pragma unroll
-            for (int b = 0; b < kBT; ++b) {
-                ay[a][b] = mma_dc(ay[a][b], d[a], ca0[b], ca1[b], g.a_exact);
-                if (g.s_full) az[a][b] = mma_dc(az[a][b], d[a], cs0[b], cs1[b], g.s_exact);
-            }
+        mma_rounds<kBT>(ay, d, ca0, ca1, g.a_exact);
+        if (g.s_full) mma_rounds<kBT>(az, d, cs0, cs1, g.s_exact);
     }
     float vy[kBT * kBT], vz[kBT * kBT];
     wave_sum_f(ay, red, vy);
     if (g.s_full) wave_sum_f(az, red, vz);
+    // wave w holds tile w: four consecutive j per lane, stored as one 8-byte
+    // write per piece.  Rows and columns past n are exact zeros (zero-padded
+    // operands), so the whole padded tile is stored without a bound check.
+    const int a = w / kBT, b = w % kBT;
+    const int jb = j0 + 16 * a + 4 * lg, i = i0 + 16 * b + li;
+    store_pieces4(YP, pp, (int64_t)i * np + jb, vy);
+    if (g.s_full) {
 #pragma unroll
-    for (int t = 0; t < kBT * kBT; ++t) {
-        const int q = w + 4 * t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
-        const int j = j0 + 16 * a + 4 * lg + e, i = i0 + 16 * b + li;
-        if (i >= n || j >= n) continue;
-        const float v = vy[t];
-        const unsigned short h0 = bf16_rn(v);
-        const float r1 = v - bf16_f(h0);
-        const unsigned short h1 = bf16_rn(r1);
-        const int64_t o = (int64_t)i * np + j;
-        YP[o] = h0;
-        YP[pp + o] = h1;
-        YP[2 * pp + o] = bf16_rn(r1 - bf16_f(h1));
-        if (g.s_full) Z[i + (int64_t)j * n] = vz[t];
+        for (int e = 0; e < 4; ++e)
+            if (i < n && jb + e < n) Z[i + (int64_t)(jb + e) * n] = vz[e];
     }
 }
 
@@ -729,20 +767,15 @@ __global__ __launch_bounds__(256) void k_qapbfb_p2(QapArgs g, const IT *__restri
                 ct1[t] = g.t_exact ? ct0[t] : ld_frag(g.T1, oc);
             }
         }
-#pragma unroll
-        for (int a = 0; a < kBT; ++a)
-#pragma unroll
-            for (int b = 0; b < kBT; ++b) {
-                acc[a][b] = mma_dc(acc[a][b], d[a], cb0[b], cb1[b], g.b_exact);
-                if (g.t_full) at[a][b] = mma_dc(at[a][b], dx[a], ct0[b], ct1[b], g.t_exact);
-            }
+        mma_rounds<kBT>(acc, d, cb0, cb1, g.b_exact);
+        if (g.t_full) mma_rounds<kBT>(at, dx, ct0, ct1, g.t_exact);
     }
     float v[kBT * kBT], vt[kBT * kBT];
     wave_sum_f(acc, red, v);
     if (g.t_full) wave_sum_f(at, red, vt);
 #pragma unroll
     for (int t = 0; t < kBT * kBT; ++t) {
-        const int q = w + 4 * t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
+        const int q = 4 * w + t, e = q & 3, ab = q >> 2, a = ab / kBT, b = ab % kBT;
         const int r = i0 + 16 * a + 4 * lg + e, j = j0 + 16 * b + li;
         if (r >= n || j >= n) continue;
         const int64_t o = r + (int64_t)j * n;
@@ -755,6 +788,158 @@ __global__ __launch_bounds__(256) void k_qapbfb_p2(QapArgs g, const IT *__restri
         if (!g.t_full) u -= (double)x[o] * g.td[j];
         epi((int32_t)o, 2.0 * u);
     }
+}
+
+// ---------------- split-BF16 x3 passes, n >= 1024, diagonal S and T ---------
+// LDS-tiled GEMM form of the two passes for the QAP relaxation's shifts
+// (diagonal S, T): a block of 2 x 2 waves owns a (32 TW) x (32 TW) output,
+// each wave a (16 TW) x (16 TW) part as TW x TW MFMA tiles.  Per 32-deep K
+// step the block stages its data rows (three pieces) and coefficient rows
+// (one or two pieces) in LDS with 16-byte loads -- each staged fragment then
+// feeds TW MFMA tiles of two waves, so global traffic per MFMA falls by the
+// tile edge -- and every wave reads its fragments from LDS.  Rows are padded
+// to kLS halfwords so the 16 lanes reading one fragment column hit distinct
+// banks.  Piece arrays are qnp x qnp with qnp a multiple of 128, so no
+// access needs a bound check; results outside n x n are not stored.
+constexpr int kLK = 32;                // K step
+constexpr int kLS = kLK + 8;           // LDS row stride (halfwords)
+constexpr int kQapLdsMin = 1024;       // n from which these passes run
+
+// The staging of one K step, split into its global loads (into registers,
+// issued before the previous step's MFMAs so they are in flight meanwhile)
+// and its LDS stores: three data pieces of R rows and the coefficient's one
+// or two pieces of R rows, 16-byte chunks, R * kLK / 8 chunks per piece.
+template <int TW>
+struct LdsStage {
+    static constexpr int R = 32 * TW;
+    static constexpr int CP = R * (kLK / 8) / 256;  // chunks per thread per piece
+    static_assert(R * (kLK / 8) % 256 == 0, "chunks per piece must be a multiple of the block");
+    u16x8 d[3][CP], c[2][CP];
+    __device__ __forceinline__ void load(const unsigned short *__restrict__ D, int64_t pp, const unsigned short *C0,
+                                         const unsigned short *C1, bool two, int np, int drow0, int crow0, int k0) {
+#pragma unroll
+        for (int t = 0; t < CP; ++t) {
+            const int ch = t * 256 + threadIdx.x, row = ch >> 2, kc = (ch & 3) * 8;
+            const int64_t od = (int64_t)(drow0 + row) * np + k0 + kc, oc = (int64_t)(crow0 + row) * np + k0 + kc;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) d[p][t] = *(const u16x8 *)(D + p * pp + od);
+            c[0][t] = *(const u16x8 *)(C0 + oc);
+            if (two) c[1][t] = *(const u16x8 *)(C1 + oc);
+        }
+    }
+    __device__ __forceinline__ void store(unsigned short *Ds, unsigned short *Cs, bool two) const {
+#pragma unroll
+        for (int t = 0; t < CP; ++t) {
+            const int ch = t * 256 + threadIdx.x, row = ch >> 2, kc = (ch & 3) * 8;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) *(u16x8 *)(Ds + (p * R + row) * kLS + kc) = d[p][t];
+            *(u16x8 *)(Cs + row * kLS + kc) = c[0][t];
+            if (two) *(u16x8 *)(Cs + (R + row) * kLS + kc) = c[1][t];
+        }
+    }
+};
+
+// one K step of a wave: acc[a][b] += D_a C_b over the staged pieces
+template <int TW>
+__device__ __forceinline__ void lds_step(floatx4 (&acc)[TW][TW], const unsigned short *Ds, const unsigned short *Cs,
+                                         bool exact) {
+    constexpr int R = 32 * TW;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
+    const int dr = (w & 1) * 16 * TW, cr = (w >> 1) * 16 * TW;
+    bf16x8 c0[TW], c1[TW];
+#pragma unroll
+    for (int b = 0; b < TW; ++b) {
+        const int o = (cr + 16 * b + li) * kLS + 8 * lg;
+        c0[b] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + o));
+        c1[b] = exact ? c0[b] : __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + R * kLS + o));
+    }
+    bf16x8 d[TW][3];
+#pragma unroll
+    for (int a = 0; a < TW; ++a) {
+        const int o = (dr + 16 * a + li) * kLS + 8 * lg;
+#pragma unroll
+        for (int p = 0; p < 3; ++p) d[a][p] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Ds + p * R * kLS + o));
+    }
+    mma_rounds<TW>(acc, d, c0, c1, exact);
+}
+
+// pass 1: Y^T = X^T A^T, stored as row-major Y pieces
+template <int TW>
+__global__ __launch_bounds__(256) void k_qapbfl_p1(QapArgs g, const unsigned short *__restrict__ XP,
+                                                   unsigned short *__restrict__ YP) {
+    constexpr int R = 32 * TW;
+    __shared__ __attribute__((aligned(16))) unsigned short Ds[3 * R * kLS], Cs[2 * R * kLS];
+    const int n = g.n, np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
+    const int64_t pp = (int64_t)np * np;
+    const int j0 = blockIdx.x * R, i0 = blockIdx.y * R;
+    floatx4 acc[TW][TW];
+#pragma unroll
+    for (int a = 0; a < TW; ++a)
+#pragma unroll
+        for (int b = 0; b < TW; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const bool two = !g.a_exact;
+    LdsStage<TW> st;
+    st.load(XP, pp, g.A0, g.A1, two, np, j0, i0, 0);
+    for (int k0 = 0; k0 < np; k0 += kLK) {
+        st.store(Ds, Cs, two);
+        __syncthreads();
+        if (k0 + kLK < np) st.load(XP, pp, g.A0, g.A1, two, np, j0, i0, k0 + kLK);
+        lds_step<TW>(acc, Ds, Cs, g.a_exact);
+        __syncthreads();
+    }
+    // four consecutive j per lane: one 8-byte store per piece; rows and
+    // columns past n are exact zeros, so the padded tile is stored whole
+    const int jr = j0 + (w & 1) * 16 * TW, ic = i0 + (w >> 1) * 16 * TW;
+#pragma unroll
+    for (int a = 0; a < TW; ++a)
+#pragma unroll
+        for (int b = 0; b < TW; ++b) {
+            const float v[4] = {acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
+            store_pieces4(YP, pp, (int64_t)(ic + 16 * b + li) * np + jr + 16 * a + 4 * lg, v);
+        }
+    (void)n;
+}
+
+// pass 2: R = Y B, then - s_i X_ij - X_ij t_j and the epilogue
+template <int TW, class IT, class Epi>
+__global__ __launch_bounds__(256) void k_qapbfl_p2(QapArgs g, const IT *__restrict__ x,
+                                                   const unsigned short *__restrict__ YP, Epi epi) {
+    constexpr int R = 32 * TW;
+    __shared__ __attribute__((aligned(16))) unsigned short Ds[3 * R * kLS], Cs[2 * R * kLS];
+    const int n = g.n, np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
+    const int64_t pp = (int64_t)np * np;
+    const int i0 = blockIdx.x * R, j0 = blockIdx.y * R;
+    floatx4 acc[TW][TW];
+#pragma unroll
+    for (int a = 0; a < TW; ++a)
+#pragma unroll
+        for (int b = 0; b < TW; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const bool two = !g.b_exact;
+    LdsStage<TW> st;
+    st.load(YP, pp, g.B0, g.B1, two, np, i0, j0, 0);
+    for (int k0 = 0; k0 < np; k0 += kLK) {
+        st.store(Ds, Cs, two);
+        __syncthreads();
+        if (k0 + kLK < np) st.load(YP, pp, g.B0, g.B1, two, np, i0, j0, k0 + kLK);
+        lds_step<TW>(acc, Ds, Cs, g.b_exact);
+        __syncthreads();
+    }
+    const int ir = i0 + (w & 1) * 16 * TW, jc = j0 + (w >> 1) * 16 * TW;
+#pragma unroll
+    for (int a = 0; a < TW; ++a)
+#pragma unroll
+        for (int b = 0; b < TW; ++b) {
+            const int j = jc + 16 * b + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = ir + 16 * a + 4 * lg + e;
+                if (r >= n || j >= n) continue;
+                const int64_t o = r + (int64_t)j * n;
+                const double xo = (double)x[o];
+                const double u = (double)acc[a][b][e] - g.sd[r] * xo - xo * g.td[j];
+                epi((int32_t)o, 2.0 * u);
+            }
+        }
 }
 
 // Q x with the result handed to epi(i, (Q x)_i): FP64 / split-BF16 x3
@@ -785,6 +970,27 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
     const int nt = (g.n + 15) / 16;
     const int64_t nn = (int64_t)g.n * g.n;
     float *Y = q->qtemp32.p, *Z = q->qtemp32.p + nn;
+    if (g.n >= kQapLdsMin && !g.s_full && !g.t_full) {
+        launch_k(k_qap_split<IT>, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, g.n, g.np, X, q->qxp.p,
+                 (unsigned short *)nullptr);
+        HPRQP_HIP(hipGetLastError());
+        // 128-wide blocks once they give every CU one, 64-wide below
+        if (g.np >= 2048) {
+            const int nb = g.np / 128;
+            launch_k(k_qapbfl_p1<4>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
+            HPRQP_HIP(hipGetLastError());
+            launch_k(k_qapbfl_p2<4, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
+                     epi);
+        } else {
+            const int nb = g.np / 64;
+            launch_k(k_qapbfl_p1<2>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
+            HPRQP_HIP(hipGetLastError());
+            launch_k(k_qapbfl_p2<2, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
+                     epi);
+        }
+        HPRQP_HIP(hipGetLastError());
+        return;
+    }
     if (g.n >= kQapBlockedMin) {
         const int nb = g.np / kBE;
         launch_k(k_qap_split<IT>, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, g.n, g.np, X, q->qxp.p,

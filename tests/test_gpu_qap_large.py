@@ -1,13 +1,16 @@
-"""QAP operator at sizes where the register-blocked MFMA passes run
-(n >= 384), against the FP64 closed form vec(2(AXB - SX - XT)), plus a
-QAP-relaxation solve at n = 400 whose returned (x, y, z) is checked in the
-original space with the CPU operator.
+"""QAP operator at sizes where the blocked MFMA passes run, against the FP64
+closed form vec(2(AXB - SX - XT)), plus a QAP-relaxation solve at n = 400
+whose returned (x, y, z) is checked in the original space with the CPU
+operator.
 
-Sizes are chosen ragged against the 64 x 64 block tile and the 16 x 16 MFMA
-tile (400, 517) as well as aligned (448, 512); S and T are full on one case
-and diagonal on the others; coefficients are Gaussian (two-piece BF16 split
-of both operands) or integer flows and distances (BF16-exact, three-piece
-data split)."""
+Sizes cover every large-n path: 384 <= n < 1024 (register-blocked passes,
+four waves splitting K), n >= 1024 with diagonal S and T (LDS-tiled
+split-BF16 passes with 64-wide blocks, and 128-wide at n = 2048), and a full
+S, T at n = 1100 (which stays on the register-blocked passes).  Sizes are
+ragged against the block and MFMA tiles (400, 517, 1100) as well as aligned
+(448, 512, 2048); coefficients are Gaussian (two-piece BF16 split of both
+operands) or integer flows and distances (BF16-exact, three-piece data
+split)."""
 import numpy as np
 import pytest
 
@@ -24,7 +27,7 @@ def _cases():
 
     rng = np.random.default_rng(21)
     out = []
-    for nq, full in ((400, True), (517, False)):
+    for nq, full in ((400, True), (517, False), (1100, False), (1100, True)):
         A = rng.standard_normal((nq, nq))
         B = rng.standard_normal((nq, nq))
         if full:
@@ -32,7 +35,7 @@ def _cases():
         else:
             S, T = np.diag(rng.standard_normal(nq)), np.diag(rng.standard_normal(nq))
         out.append((f"gauss{nq}{'full' if full else 'diag'}", QAP_Q_operator(A, B, S, T), False))
-    for nq in (448, 512):
+    for nq in (448, 512, 2048):
         out.append((f"flow{nq}", instances.qap(nq, seed=nq)[0].Q, True))
     return out
 
