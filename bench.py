@@ -264,8 +264,8 @@ def main():
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t) / args.steps
 
-        # read-only and write-only streams of the same size beside it: the
-        # rates a plain library kernel reaches on this card's HBM
+        # torch's own read-only (sum) and write-only (fill) kernels over the
+        # same buffers beside it, for reference
         def rate(op):
             op()
             torch.cuda.synchronize()
@@ -279,7 +279,7 @@ def main():
         write_gbs = rate(lambda: b.fill_(1.0))
         print(json.dumps({"metric": "device copy GB/s (read + write)", "value": 2 * nb / dt / 1e9, "unit": "GB/s",
                           "bytes_read_per_copy": nb, "bytes_written_per_copy": nb, "steps": args.steps,
-                          "read_only_gbs": read_gbs, "write_only_gbs": write_gbs}), flush=True)
+                          "torch_sum_read_gbs": read_gbs, "torch_fill_write_gbs": write_gbs}), flush=True)
         return
     if args.workload == "qap-apply":
         print(json.dumps(qap_apply_leg([int(v) for v in args.qap_sizes.split(",")], args.steps)), flush=True)
