@@ -109,3 +109,26 @@ def test_qap400_solve_fp64_and_mixed():
     print(f"mixed: {mix.status} iter={mix.iter} polish={mix.polish_iter} kkt={mix.residuals:.2e} rel obj {rel:.2e}")
     assert mix.status == "OPTIMAL" and mix.residuals < 1e-6
     assert rel < 1e-6
+
+
+def test_qap1024_solve_mixed_lds_passes():
+    """n = 1024 (1,048,576 variables): the mixed solve runs the LDS-tiled
+    split-BF16 x3 passes with the solver's fused half-step epilogues; it must
+    reach 1e-6, satisfy the KKT conditions in the original space by the CPU
+    operator, and agree with the FP64 solve's objective within 1e-6."""
+    from hprqp import HPRQP_parameters, instances, optimize
+
+    model = instances.qap(1024, seed=9)[0]
+    ref = optimize(model, HPRQP_parameters(verbose=False, problem_type="QAP"))
+    mix = optimize(model, HPRQP_parameters(verbose=False, problem_type="QAP", precision="mixed"))
+    ep, ed = _kkt_original(model, mix.x, mix.y, mix.z)
+    ep0, ed0 = _kkt_original(model, ref.x, ref.y, ref.z)
+    rel = abs(mix.primal_obj - ref.primal_obj) / (1.0 + abs(ref.primal_obj))
+    print(f"fp64: {ref.status} iter={ref.iter} kkt={ref.residuals:.2e} time={ref.time:.2f}s original-space "
+          f"ep={ep0:.2e} ed={ed0:.2e} | mixed: {mix.status} iter={mix.iter} polish={mix.polish_iter} "
+          f"kkt={mix.residuals:.2e} time={mix.time:.2f}s rel obj {rel:.2e} original-space ep={ep:.2e} ed={ed:.2e}")
+    assert ref.status == "OPTIMAL" and ref.residuals < 1e-6
+    assert mix.status == "OPTIMAL" and mix.residuals < 1e-6
+    assert ep0 < 1e-5 and ed0 < 1e-5
+    assert ep < 1e-5 and ed < 1e-5
+    assert rel < 1e-6
