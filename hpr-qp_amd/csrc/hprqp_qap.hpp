@@ -432,6 +432,143 @@ __global__ __launch_bounds__(256) void k_qap64b_p2(QapArgs g, const double *__re
     }
 }
 
+// ---------------- FP64 passes, n >= 1024, diagonal S and T -------------------
+// LDS-tiled GEMM form of the FP64 passes: a block of 2 x 2 waves owns a
+// (32 TW) x (32 TW) output, each wave TW x TW MFMA tiles.  Per DK-deep K step
+// (32 for 64-wide blocks, 16 for 128-wide, within 64 KB of LDS) the block
+// stages R = 32 TW rows of the left operand (column-major, so each k is R
+// contiguous doubles) and R columns of the right operand (16 contiguous k
+// per load group) k-major in LDS, rows padded by one double so both the
+// transposing stores and the fragment reads are conflict-free; the next K
+// step's loads are issued before this step's MFMAs.  Rows and columns past
+// n are clamped to n-1 (results not stored); K past n reads zero.
+template <int TW, int DK>
+struct LdsStage64 {
+    static constexpr int R = 32 * TW;
+    static constexpr int KQ = 256 / R;           // threads per left-operand row across k
+    static constexpr int NL = DK / KQ;           // left-operand loads per thread
+    static constexpr int NR = (R / 16) * (DK / 16);  // right-operand loads per thread
+    double l[NL], r[NR];
+    __device__ __forceinline__ void load(const double *__restrict__ L, const double *__restrict__ Rm, int n,
+                                         int lrow0, int rcol0, int k0) {
+        const int t = threadIdx.x;
+        const int lr = min(lrow0 + t % R, n - 1), kq = t / R;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const int k = k0 + kq + KQ * i;
+            const double v = L[lr + (int64_t)(k < n ? k : 0) * n];
+            l[i] = k < n ? v : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            const int k = k0 + (i / (R / 16)) * 16 + t % 16;
+            const int c = min(rcol0 + t / 16 + 16 * (i % (R / 16)), n - 1);
+            const double v = Rm[(k < n ? k : 0) + (int64_t)c * n];
+            r[i] = k < n ? v : 0.0;
+        }
+    }
+    __device__ __forceinline__ void store(double *Ls, double *Rs) const {
+        const int t = threadIdx.x;
+        const int lr = t % R, kq = t / R;
+#pragma unroll
+        for (int i = 0; i < NL; ++i) Ls[(kq + KQ * i) * (R + 1) + lr] = l[i];
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+            Rs[((i / (R / 16)) * 16 + t % 16) * (R + 1) + t / 16 + 16 * (i % (R / 16))] = r[i];
+    }
+};
+
+template <int TW, int DK>
+__device__ __forceinline__ void lds_step64(doublex4 (&acc)[TW][TW], const double *Ls, const double *Rs) {
+    constexpr int R = 32 * TW;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
+    const int lr = (w & 1) * 16 * TW + li, rc = (w >> 1) * 16 * TW + li;
+#pragma unroll
+    for (int kq = 0; kq < DK; kq += 4) {
+        double fl[TW], fr[TW];
+#pragma unroll
+        for (int a = 0; a < TW; ++a) fl[a] = Ls[(kq + lg) * (R + 1) + lr + 16 * a];
+#pragma unroll
+        for (int b = 0; b < TW; ++b) fr[b] = Rs[(kq + lg) * (R + 1) + rc + 16 * b];
+#pragma unroll
+        for (int a = 0; a < TW; ++a)
+#pragma unroll
+            for (int b = 0; b < TW; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fl[a], fr[b], acc[a][b], 0, 0, 0);
+    }
+}
+
+template <int TW, int DK>
+__global__ __launch_bounds__(256) void k_qap64l_p1(QapArgs g, const double *__restrict__ x, double *__restrict__ Y) {
+    constexpr int R = 32 * TW;
+    __shared__ double Ls[DK * (R + 1)], Rs[DK * (R + 1)];
+    const int n = g.n, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
+    const int r0 = blockIdx.x * R, c0 = blockIdx.y * R;
+    doublex4 acc[TW][TW];
+#pragma unroll
+    for (int a = 0; a < TW; ++a)
+#pragma unroll
+        for (int b = 0; b < TW; ++b) acc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+    LdsStage64<TW, DK> st;
+    st.load(g.A, x, n, r0, c0, 0);
+    for (int k0 = 0; k0 < n; k0 += DK) {
+        st.store(Ls, Rs);
+        __syncthreads();
+        if (k0 + DK < n) st.load(g.A, x, n, r0, c0, k0 + DK);
+        lds_step64<TW, DK>(acc, Ls, Rs);
+        __syncthreads();
+    }
+    const int rr = r0 + (w & 1) * 16 * TW, cc = c0 + (w >> 1) * 16 * TW;
+#pragma unroll
+    for (int a = 0; a < TW; ++a)
+#pragma unroll
+        for (int b = 0; b < TW; ++b) {
+            const int j = cc + 16 * b + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = rr + 16 * a + lg + 4 * e;
+                if (r < n && j < n) Y[r + (int64_t)j * n] = acc[a][b][e];
+            }
+        }
+}
+
+template <int TW, int DK, class Epi>
+__global__ __launch_bounds__(256) void k_qap64l_p2(QapArgs g, const double *__restrict__ x,
+                                                   const double *__restrict__ Y, Epi epi) {
+    constexpr int R = 32 * TW;
+    __shared__ double Ls[DK * (R + 1)], Rs[DK * (R + 1)];
+    const int n = g.n, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
+    const int r0 = blockIdx.x * R, c0 = blockIdx.y * R;
+    doublex4 acc[TW][TW];
+#pragma unroll
+    for (int a = 0; a < TW; ++a)
+#pragma unroll
+        for (int b = 0; b < TW; ++b) acc[a][b] = doublex4{0.0, 0.0, 0.0, 0.0};
+    LdsStage64<TW, DK> st;
+    st.load(Y, g.B, n, r0, c0, 0);
+    for (int k0 = 0; k0 < n; k0 += DK) {
+        st.store(Ls, Rs);
+        __syncthreads();
+        if (k0 + DK < n) st.load(Y, g.B, n, r0, c0, k0 + DK);
+        lds_step64<TW, DK>(acc, Ls, Rs);
+        __syncthreads();
+    }
+    const int rr = r0 + (w & 1) * 16 * TW, cc = c0 + (w >> 1) * 16 * TW;
+#pragma unroll
+    for (int a = 0; a < TW; ++a)
+#pragma unroll
+        for (int b = 0; b < TW; ++b) {
+            const int j = cc + 16 * b + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = rr + 16 * a + lg + 4 * e;
+                if (r >= n || j >= n) continue;
+                const int64_t o = r + (int64_t)j * n;
+                const double xo = x[o];
+                epi((int32_t)o, 2.0 * (acc[a][b][e] - g.sd[r] * xo - xo * g.td[j]));
+            }
+        }
+}
+
 // ---------------- split-BF16 x3 passes ---------------------------------------
 constexpr int kLdB = kKS + 8;  // u16 row stride of staged coefficient pieces
 constexpr int kLdF = kKS + 4;  // float row stride of staged data
@@ -793,33 +930,32 @@ __global__ __launch_bounds__(256) void k_qapbfb_p2(QapArgs g, const IT *__restri
 // ---------------- split-BF16 x3 passes, n >= 1024, diagonal S and T ---------
 // LDS-tiled GEMM form of the two passes for the QAP relaxation's shifts
 // (diagonal S, T): a block of 2 x 2 waves owns a (32 TW) x (32 TW) output,
-// each wave a (16 TW) x (16 TW) part as TW x TW MFMA tiles.  Per 32-deep K
-// step the block stages its data rows (three pieces) and coefficient rows
-// (one or two pieces) in LDS with 16-byte loads -- each staged fragment then
-// feeds TW MFMA tiles of two waves, so global traffic per MFMA falls by the
-// tile edge -- and every wave reads its fragments from LDS.  Rows are padded
-// to kLS halfwords so the 16 lanes reading one fragment column hit distinct
-// banks.  Piece arrays are qnp x qnp with qnp a multiple of 128, so no
+// each wave a (16 TW) x (16 TW) part as TW x TW MFMA tiles.  Per K step of
+// KS (64 for 64-wide blocks, 32 for 128-wide, so LDS stays within 64 KB) the
+// block stages its data rows (three pieces) and coefficient rows (one or two
+// pieces) in LDS with 16-byte loads -- each staged fragment then feeds TW
+// MFMA tiles of two waves, so global traffic per MFMA falls by the tile
+// edge -- and every wave reads its fragments from LDS.  Rows are padded to
+// KS + 8 halfwords to spread the 16 lanes reading one fragment column over
+// the banks.  Piece arrays are qnp x qnp with qnp a multiple of 128, so no
 // access needs a bound check; results outside n x n are not stored.
-constexpr int kLK = 32;                // K step
-constexpr int kLS = kLK + 8;           // LDS row stride (halfwords)
 constexpr int kQapLdsMin = 1024;       // n from which these passes run
 
 // The staging of one K step, split into its global loads (into registers,
 // issued before the previous step's MFMAs so they are in flight meanwhile)
 // and its LDS stores: three data pieces of R rows and the coefficient's one
-// or two pieces of R rows, 16-byte chunks, R * kLK / 8 chunks per piece.
-template <int TW>
+// or two pieces of R rows, 16-byte chunks, R * KS / 8 chunks per piece.
+template <int TW, int KS>
 struct LdsStage {
-    static constexpr int R = 32 * TW;
-    static constexpr int CP = R * (kLK / 8) / 256;  // chunks per thread per piece
-    static_assert(R * (kLK / 8) % 256 == 0, "chunks per piece must be a multiple of the block");
+    static constexpr int R = 32 * TW, CR = KS / 8, LS = KS + 8;
+    static constexpr int CP = R * CR / 256;  // chunks per thread per piece
+    static_assert(R * CR % 256 == 0, "chunks per piece must be a multiple of the block");
     u16x8 d[3][CP], c[2][CP];
     __device__ __forceinline__ void load(const unsigned short *__restrict__ D, int64_t pp, const unsigned short *C0,
                                          const unsigned short *C1, bool two, int np, int drow0, int crow0, int k0) {
 #pragma unroll
         for (int t = 0; t < CP; ++t) {
-            const int ch = t * 256 + threadIdx.x, row = ch >> 2, kc = (ch & 3) * 8;
+            const int ch = t * 256 + threadIdx.x, row = ch / CR, kc = (ch % CR) * 8;
             const int64_t od = (int64_t)(drow0 + row) * np + k0 + kc, oc = (int64_t)(crow0 + row) * np + k0 + kc;
 #pragma unroll
             for (int p = 0; p < 3; ++p) d[p][t] = *(const u16x8 *)(D + p * pp + od);
@@ -830,45 +966,49 @@ struct LdsStage {
     __device__ __forceinline__ void store(unsigned short *Ds, unsigned short *Cs, bool two) const {
 #pragma unroll
         for (int t = 0; t < CP; ++t) {
-            const int ch = t * 256 + threadIdx.x, row = ch >> 2, kc = (ch & 3) * 8;
+            const int ch = t * 256 + threadIdx.x, row = ch / CR, kc = (ch % CR) * 8;
 #pragma unroll
-            for (int p = 0; p < 3; ++p) *(u16x8 *)(Ds + (p * R + row) * kLS + kc) = d[p][t];
-            *(u16x8 *)(Cs + row * kLS + kc) = c[0][t];
-            if (two) *(u16x8 *)(Cs + (R + row) * kLS + kc) = c[1][t];
+            for (int p = 0; p < 3; ++p) *(u16x8 *)(Ds + (p * R + row) * LS + kc) = d[p][t];
+            *(u16x8 *)(Cs + row * LS + kc) = c[0][t];
+            if (two) *(u16x8 *)(Cs + (R + row) * LS + kc) = c[1][t];
         }
     }
 };
 
-// one K step of a wave: acc[a][b] += D_a C_b over the staged pieces
-template <int TW>
+// one K step of a wave: acc[a][b] += D_a C_b over the staged pieces, in
+// 32-deep MFMA steps
+template <int TW, int KS>
 __device__ __forceinline__ void lds_step(floatx4 (&acc)[TW][TW], const unsigned short *Ds, const unsigned short *Cs,
                                          bool exact) {
-    constexpr int R = 32 * TW;
+    constexpr int R = 32 * TW, LS = KS + 8;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
     const int dr = (w & 1) * 16 * TW, cr = (w >> 1) * 16 * TW;
-    bf16x8 c0[TW], c1[TW];
 #pragma unroll
-    for (int b = 0; b < TW; ++b) {
-        const int o = (cr + 16 * b + li) * kLS + 8 * lg;
-        c0[b] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + o));
-        c1[b] = exact ? c0[b] : __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + R * kLS + o));
+    for (int kq = 0; kq < KS; kq += 32) {
+        bf16x8 c0[TW], c1[TW];
+#pragma unroll
+        for (int b = 0; b < TW; ++b) {
+            const int o = (cr + 16 * b + li) * LS + kq + 8 * lg;
+            c0[b] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + o));
+            c1[b] = exact ? c0[b] : __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + R * LS + o));
+        }
+        bf16x8 d[TW][3];
+#pragma unroll
+        for (int a = 0; a < TW; ++a) {
+            const int o = (dr + 16 * a + li) * LS + kq + 8 * lg;
+#pragma unroll
+            for (int p = 0; p < 3; ++p) d[a][p] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Ds + p * R * LS + o));
+        }
+        mma_rounds<TW>(acc, d, c0, c1, exact);
     }
-    bf16x8 d[TW][3];
-#pragma unroll
-    for (int a = 0; a < TW; ++a) {
-        const int o = (dr + 16 * a + li) * kLS + 8 * lg;
-#pragma unroll
-        for (int p = 0; p < 3; ++p) d[a][p] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Ds + p * R * kLS + o));
-    }
-    mma_rounds<TW>(acc, d, c0, c1, exact);
 }
 
 // pass 1: Y^T = X^T A^T, stored as row-major Y pieces
-template <int TW>
+template <int TW, int KS>
 __global__ __launch_bounds__(256) void k_qapbfl_p1(QapArgs g, const unsigned short *__restrict__ XP,
                                                    unsigned short *__restrict__ YP) {
     constexpr int R = 32 * TW;
-    __shared__ __attribute__((aligned(16))) unsigned short Ds[3 * R * kLS], Cs[2 * R * kLS];
+    __shared__ __attribute__((aligned(16))) unsigned short Ds[3 * R * (KS + 8)], Cs[2 * R * (KS + 8)];
     const int n = g.n, np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
     const int64_t pp = (int64_t)np * np;
     const int j0 = blockIdx.x * R, i0 = blockIdx.y * R;
@@ -878,13 +1018,13 @@ __global__ __launch_bounds__(256) void k_qapbfl_p1(QapArgs g, const unsigned sho
 #pragma unroll
         for (int b = 0; b < TW; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
     const bool two = !g.a_exact;
-    LdsStage<TW> st;
+    LdsStage<TW, KS> st;
     st.load(XP, pp, g.A0, g.A1, two, np, j0, i0, 0);
-    for (int k0 = 0; k0 < np; k0 += kLK) {
+    for (int k0 = 0; k0 < np; k0 += KS) {
         st.store(Ds, Cs, two);
         __syncthreads();
-        if (k0 + kLK < np) st.load(XP, pp, g.A0, g.A1, two, np, j0, i0, k0 + kLK);
-        lds_step<TW>(acc, Ds, Cs, g.a_exact);
+        if (k0 + KS < np) st.load(XP, pp, g.A0, g.A1, two, np, j0, i0, k0 + KS);
+        lds_step<TW, KS>(acc, Ds, Cs, g.a_exact);
         __syncthreads();
     }
     // four consecutive j per lane: one 8-byte store per piece; rows and
@@ -901,11 +1041,11 @@ __global__ __launch_bounds__(256) void k_qapbfl_p1(QapArgs g, const unsigned sho
 }
 
 // pass 2: R = Y B, then - s_i X_ij - X_ij t_j and the epilogue
-template <int TW, class IT, class Epi>
+template <int TW, int KS, class IT, class Epi>
 __global__ __launch_bounds__(256) void k_qapbfl_p2(QapArgs g, const IT *__restrict__ x,
                                                    const unsigned short *__restrict__ YP, Epi epi) {
     constexpr int R = 32 * TW;
-    __shared__ __attribute__((aligned(16))) unsigned short Ds[3 * R * kLS], Cs[2 * R * kLS];
+    __shared__ __attribute__((aligned(16))) unsigned short Ds[3 * R * (KS + 8)], Cs[2 * R * (KS + 8)];
     const int n = g.n, np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
     const int64_t pp = (int64_t)np * np;
     const int i0 = blockIdx.x * R, j0 = blockIdx.y * R;
@@ -915,13 +1055,13 @@ __global__ __launch_bounds__(256) void k_qapbfl_p2(QapArgs g, const IT *__restri
 #pragma unroll
         for (int b = 0; b < TW; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
     const bool two = !g.b_exact;
-    LdsStage<TW> st;
+    LdsStage<TW, KS> st;
     st.load(YP, pp, g.B0, g.B1, two, np, i0, j0, 0);
-    for (int k0 = 0; k0 < np; k0 += kLK) {
+    for (int k0 = 0; k0 < np; k0 += KS) {
         st.store(Ds, Cs, two);
         __syncthreads();
-        if (k0 + kLK < np) st.load(YP, pp, g.B0, g.B1, two, np, i0, j0, k0 + kLK);
-        lds_step<TW>(acc, Ds, Cs, g.b_exact);
+        if (k0 + KS < np) st.load(YP, pp, g.B0, g.B1, two, np, i0, j0, k0 + KS);
+        lds_step<TW, KS>(acc, Ds, Cs, g.b_exact);
         __syncthreads();
     }
     const int ir = i0 + (w & 1) * 16 * TW, jc = j0 + (w >> 1) * 16 * TW;
@@ -949,6 +1089,22 @@ void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Ep
     const int nt = (g.n + 15) / 16;
     const int64_t nn = (int64_t)g.n * g.n;
     double *Y = q->qtemp.p, *Z = q->qtemp.p + nn;
+    if (g.n >= kQapLdsMin && !g.s_full && !g.t_full) {
+        // 128-wide blocks once they give every CU one, 64-wide below
+        if (g.n >= 2048) {
+            const int nb = (g.n + 127) / 128;
+            launch_k(k_qap64l_p1<4, 16>, dim3(nb, nb), dim3(256), 0, s, g, X, Y);
+            HPRQP_HIP(hipGetLastError());
+            launch_k(k_qap64l_p2<4, 16, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const double *)Y, epi);
+        } else {
+            const int nb = (g.n + 63) / 64;
+            launch_k(k_qap64l_p1<2, 32>, dim3(nb, nb), dim3(256), 0, s, g, X, Y);
+            HPRQP_HIP(hipGetLastError());
+            launch_k(k_qap64l_p2<2, 32, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const double *)Y, epi);
+        }
+        HPRQP_HIP(hipGetLastError());
+        return;
+    }
     if (g.n >= kQapBlockedMin) {
         const int nb = (g.n + kBE - 1) / kBE;
         launch_k(k_qap64b_p1, dim3(nb, nb), dim3(256), 0, s, g, X, Y, Z);
@@ -977,15 +1133,15 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
         // 128-wide blocks once they give every CU one, 64-wide below
         if (g.np >= 2048) {
             const int nb = g.np / 128;
-            launch_k(k_qapbfl_p1<4>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
+            launch_k(k_qapbfl_p1<4, 32>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
             HPRQP_HIP(hipGetLastError());
-            launch_k(k_qapbfl_p2<4, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
+            launch_k(k_qapbfl_p2<4, 32, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
                      epi);
         } else {
             const int nb = g.np / 64;
-            launch_k(k_qapbfl_p1<2>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
+            launch_k(k_qapbfl_p1<2, 64>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
             HPRQP_HIP(hipGetLastError());
-            launch_k(k_qapbfl_p2<2, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
+            launch_k(k_qapbfl_p2<2, 64, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
                      epi);
         }
         HPRQP_HIP(hipGetLastError());
