@@ -195,9 +195,13 @@ __device__ __forceinline__ double csr_row(const int32_t *rp, const int32_t *ci, 
     return acc;
 }
 
+// An instance's arrays.  The table lives in LDS (filled once per block) and
+// the step code reads each pointer from it where it is used: kept in
+// registers, the fifty pointers overflowed the scalar registers into vector
+// registers and scratch and capped the interval kernel at three waves per
+// SIMD.
 template <class MT>
-struct Inst {
-    int32_t n, m, k;
+struct IPtr {
     const MT *U, *d;
     const double *U64, *d64;
     const int32_t *arp, *aci, *atrp, *atci;
@@ -209,47 +213,59 @@ struct Inst {
         *dQw;
     double *y, *y_bar, *s, *dy, *last_x, *last_w, *last_y, *last_ATy, *pq, *pz, *pq2, *pz2;
 };
-
 template <class MT>
-__device__ Inst<MT> inst_of(const BArgs<MT> &a, int b) {
+struct Inst {
+    int32_t n, m, k;
+    const IPtr<MT> *P;
+};
+
+// fills the block's table (thread 0) and returns the instance; every thread
+// of the block must call it
+template <class MT>
+__device__ Inst<MT> inst_of(const BArgs<MT> &a, int b, IPtr<MT> *tab) {
     Inst<MT> I;
     I.n = a.n[b];
     I.m = a.m[b];
     I.k = a.k[b];
-    const int64_t no = a.noff[b], mo = a.moff[b], uo = a.uoff[b];
-    I.U = a.U + uo;
-    I.d = a.d + no;
-    I.U64 = a.U64 + uo;
-    I.d64 = a.d64 + no;
-    I.arp = a.a_rp + a.arp_off[b];
-    I.atrp = a.at_rp + a.atrp_off[b];
-    I.aci = a.a_ci;
-    I.atci = a.at_ci;
-    I.av = a.a_v;
-    I.atv = a.at_v;
-    I.av64 = a.a_v64;
-    I.atv64 = a.at_v64;
-    I.c = a.c + no;
-    I.l = a.l + no;
-    I.u = a.u + no;
-    I.AL = a.AL + mo;
-    I.AU = a.AU + mo;
-    I.cM = a.cM + no;
-    I.lM = a.lM + no;
-    I.uM = a.uM + no;
-    I.ALM = a.ALM + mo;
-    I.AUM = a.AUM + mo;
-#define P(f) I.f = a.f + no
-    P(x); P(w); P(w_bar); P(x_hat); P(x_bar); P(z_bar); P(dx); P(dw); P(Qw); P(Qw_bar); P(tempv); P(ATy);
-    P(ATy_bar); P(ATdy); P(QATdy); P(dQw); P(last_x); P(last_w); P(last_ATy); P(pq); P(pq2);
-#undef P
-    I.y = a.y + mo;
-    I.y_bar = a.y_bar + mo;
-    I.s = a.s + mo;
-    I.dy = a.dy + mo;
-    I.last_y = a.last_y + mo;
-    I.pz = a.pz + mo;
-    I.pz2 = a.pz2 + mo;
+    I.P = tab;
+    if (threadIdx.x == 0) {
+        IPtr<MT> &T = *tab;
+        const int64_t no = a.noff[b], mo = a.moff[b], uo = a.uoff[b];
+        T.U = a.U + uo;
+        T.d = a.d + no;
+        T.U64 = a.U64 + uo;
+        T.d64 = a.d64 + no;
+        T.arp = a.a_rp + a.arp_off[b];
+        T.atrp = a.at_rp + a.atrp_off[b];
+        T.aci = a.a_ci;
+        T.atci = a.at_ci;
+        T.av = a.a_v;
+        T.atv = a.at_v;
+        T.av64 = a.a_v64;
+        T.atv64 = a.at_v64;
+        T.c = a.c + no;
+        T.l = a.l + no;
+        T.u = a.u + no;
+        T.AL = a.AL + mo;
+        T.AU = a.AU + mo;
+        T.cM = a.cM + no;
+        T.lM = a.lM + no;
+        T.uM = a.uM + no;
+        T.ALM = a.ALM + mo;
+        T.AUM = a.AUM + mo;
+#define P_(f) T.f = a.f + no
+        P_(x); P_(w); P_(w_bar); P_(x_hat); P_(x_bar); P_(z_bar); P_(dx); P_(dw); P_(Qw); P_(Qw_bar); P_(tempv);
+        P_(ATy); P_(ATy_bar); P_(ATdy); P_(QATdy); P_(dQw); P_(last_x); P_(last_w); P_(last_ATy); P_(pq); P_(pq2);
+#undef P_
+        T.y = a.y + mo;
+        T.y_bar = a.y_bar + mo;
+        T.s = a.s + mo;
+        T.dy = a.dy + mo;
+        T.last_y = a.last_y + mo;
+        T.pz = a.pz + mo;
+        T.pz2 = a.pz2 + mo;
+    }
+    __syncthreads();
     return I;
 }
 
@@ -260,62 +276,62 @@ __device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, doubl
     const int32_t n = I.n, m = I.m, k = I.k;
     const double f2s = 1.0 / (1.0 + sigma * lamQ), f1s = 1.0 - f2s;
     // z/x/w1: Qw = Q w
-    lr_tvec(n, k, I.U, I.w, t, sh);
+    lr_tvec(n, k, I.P->U, I.P->w, t, sh);
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
-        const double wi = I.w[i], xi = I.x[i];
-        const double qw = (double)I.d[i] * wi + lr_row(n, k, I.U, i, t);
-        I.Qw[i] = qw;
-        const double z_raw = xi + sigma * (-qw + I.ATy[i] - (double)I.cM[i]);
-        const double xb = proj(z_raw, (double)I.lM[i], (double)I.uM[i]);
+        const double wi = I.P->w[i], xi = I.P->x[i];
+        const double qw = (double)I.P->d[i] * wi + lr_row(n, k, I.P->U, i, t);
+        I.P->Qw[i] = qw;
+        const double z_raw = xi + sigma * (-qw + I.P->ATy[i] - (double)I.P->cM[i]);
+        const double xb = proj(z_raw, (double)I.P->lM[i], (double)I.P->uM[i]);
         const double xh = 2.0 * xb - xi;
         if (full) {
-            I.dx[i] = xb - xi;
-            I.x_bar[i] = xb;
-            I.z_bar[i] = (xb - z_raw) / sigma;
+            I.P->dx[i] = xb - xi;
+            I.P->x_bar[i] = xb;
+            I.P->z_bar[i] = (xb - z_raw) / sigma;
         }
-        I.x[i] = fma(h2, xh, h1 * I.last_x[i]);
-        I.x_hat[i] = xh;
-        I.w_bar[i] = fma(f1s, wi, f2s * xh);
+        I.P->x[i] = fma(h2, xh, h1 * I.P->last_x[i]);
+        I.P->x_hat[i] = xh;
+        I.P->w_bar[i] = fma(f1s, wi, f2s * xh);
     }
     __syncthreads();
     if (m > 0) {
         // tempv = x_hat + sigma (Qw - Q w_bar)
-        lr_tvec(n, k, I.U, I.w_bar, t, sh);
+        lr_tvec(n, k, I.P->U, I.P->w_bar, t, sh);
         for (int32_t i = threadIdx.x; i < n; i += kBB) {
-            const double qwb = (double)I.d[i] * I.w_bar[i] + lr_row(n, k, I.U, i, t);
-            I.tempv[i] = I.x_hat[i] + sigma * (I.Qw[i] - qwb);
+            const double qwb = (double)I.P->d[i] * I.P->w_bar[i] + lr_row(n, k, I.P->U, i, t);
+            I.P->tempv[i] = I.P->x_hat[i] + sigma * (I.P->Qw[i] - qwb);
         }
         __syncthreads();
         // y step: A tempv
-        csr_rows_wave(m, I.arp, I.aci, I.av, I.tempv, I.pz);
+        csr_rows_wave(m, I.P->arp, I.P->aci, I.P->av, I.P->tempv, I.P->pz);
         const double fa = lamA * sigma, fb = 1.0 / fa;
         for (int32_t r = threadIdx.x; r < m; r += kBB) {
-            const double yi = I.y[r];
-            const double s_raw = I.pz[r] - fa * yi;
-            const double sp = proj(s_raw, (double)I.ALM[r], (double)I.AUM[r]);
+            const double yi = I.P->y[r];
+            const double s_raw = I.P->pz[r] - fa * yi;
+            const double sp = proj(s_raw, (double)I.P->ALM[r], (double)I.P->AUM[r]);
             const double yb = fb * (sp - s_raw);
             if (full) {
-                I.s[r] = sp;
-                I.dy[r] = yb - yi;
+                I.P->s[r] = sp;
+                I.P->dy[r] = yb - yi;
             }
-            I.y_bar[r] = yb;
-            I.y[r] = fma(h2, 2.0 * yb - yi, h1 * I.last_y[r]);
+            I.P->y_bar[r] = yb;
+            I.P->y[r] = fma(h2, 2.0 * yb - yi, h1 * I.P->last_y[r]);
         }
         __syncthreads();
     }
     // w2 step: A^T y_bar
     const double fs = sigma / (1.0 + sigma * lamQ);
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
-        const double atyb = m > 0 ? csr_row(I.atrp, I.atci, I.atv, I.y_bar, i) : 0.0;
-        const double wi = I.w[i], wbi = I.w_bar[i], atyi = I.ATy[i];
+        const double atyb = m > 0 ? csr_row(I.P->atrp, I.P->atci, I.P->atv, I.P->y_bar, i) : 0.0;
+        const double wi = I.P->w[i], wbi = I.P->w_bar[i], atyi = I.P->ATy[i];
         const double wbn = wbi + fs * (atyb - atyi);
-        I.w[i] = h1 * I.last_w[i] + h2 * (2.0 * wbn - wi);
-        I.ATy[i] = h1 * I.last_ATy[i] + h2 * (2.0 * atyb - atyi);
+        I.P->w[i] = h1 * I.P->last_w[i] + h2 * (2.0 * wbn - wi);
+        I.P->ATy[i] = h1 * I.P->last_ATy[i] + h2 * (2.0 * atyb - atyi);
         if (full) {
-            I.w_bar[i] = wbn;
-            I.dw[i] = wbn - wi;
-            I.ATdy[i] = atyb - atyi;
-            I.ATy_bar[i] = atyb;
+            I.P->w_bar[i] = wbn;
+            I.P->dw[i] = wbn - wi;
+            I.P->ATdy[i] = atyb - atyi;
+            I.P->ATy_bar[i] = atyb;
         }
     }
     __syncthreads();
@@ -325,16 +341,16 @@ __device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, doubl
 template <class MT>
 __device__ double bmnorm(Inst<MT> &I, double sigma, double lamA, double lamQ, double *t, double *sh) {
     const int32_t n = I.n, m = I.m, k = I.k;
-    lr_apply(n, k, I.U, I.d, I.dw, I.dQw, t, sh);
+    lr_apply(n, k, I.P->U, I.P->d, I.P->dw, I.P->dQw, t, sh);
     if (m > 0) {
-        for (int32_t i = threadIdx.x; i < n; i += kBB) I.ATdy[i] = csr_row(I.atrp, I.atci, I.atv, I.dy, i);
+        for (int32_t i = threadIdx.x; i < n; i += kBB) I.P->ATdy[i] = csr_row(I.P->atrp, I.P->atci, I.P->atv, I.P->dy, i);
         __syncthreads();
-        lr_apply(n, k, I.U, I.d, I.ATdy, I.QATdy, t, sh);
+        lr_apply(n, k, I.P->U, I.P->d, I.P->ATdy, I.P->QATdy, t, sh);
     }
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s6 = 0, s7 = 0;
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
-        const double a = I.dx[i], b = I.dQw[i], c = m > 0 ? I.ATdy[i] : 0.0, d = m > 0 ? I.QATdy[i] : 0.0,
-                     e = I.dw[i];
+        const double a = I.P->dx[i], b = I.P->dQw[i], c = m > 0 ? I.P->ATdy[i] : 0.0, d = m > 0 ? I.P->QATdy[i] : 0.0,
+                     e = I.P->dw[i];
         s0 += a * a;
         s1 += b * a;
         s2 += b * c;
@@ -342,7 +358,7 @@ __device__ double bmnorm(Inst<MT> &I, double sigma, double lamA, double lamQ, do
         s4 += c * d;
         s6 += e * b;
     }
-    for (int32_t r = threadIdx.x; r < m; r += kBB) s7 += I.dy[r] * I.dy[r];
+    for (int32_t r = threadIdx.x; r < m; r += kBB) s7 += I.P->dy[r] * I.P->dy[r];
     s0 = bsum(s0, sh);
     s1 = bsum(s1, sh);
     s2 = bsum(s2, sh);
@@ -368,26 +384,26 @@ template <class MT>
 __device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *sh) {
     const int32_t n = I.n, m = I.m, k = I.k;
     // Qx (into dQw as scratch), A^T y_bar (into ATdy), A x_bar (into pz)
-    lr_apply(n, k, I.U64, I.d64, I.x_bar, I.dQw, t, sh);
+    lr_apply(n, k, I.P->U64, I.P->d64, I.P->x_bar, I.P->dQw, t, sh);
     if (m > 0) {
-        for (int32_t i = threadIdx.x; i < n; i += kBB) I.ATdy[i] = csr_row(I.atrp, I.atci, I.atv64, I.y_bar, i);
-        csr_rows_wave(m, I.arp, I.aci, I.av64, I.x_bar, I.pz);
+        for (int32_t i = threadIdx.x; i < n; i += kBB) I.P->ATdy[i] = csr_row(I.P->atrp, I.P->atci, I.P->atv64, I.P->y_bar, i);
+        csr_rows_wave(m, I.P->arp, I.P->aci, I.P->av64, I.P->x_bar, I.P->pz);
     }
     double cx = 0, xqx = 0, zx = 0, rd = 0, aty = 0, qx = 0, bnd = 0, ys = 0, rp = 0, ax = 0;
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
-        const double x = I.x_bar[i], q = I.dQw[i], z = I.z_bar[i], ci = I.c[i], a = m > 0 ? I.ATdy[i] : 0.0;
+        const double x = I.P->x_bar[i], q = I.P->dQw[i], z = I.P->z_bar[i], ci = I.P->c[i], a = m > 0 ? I.P->ATdy[i] : 0.0;
         cx += ci * x;
         xqx += x * q;
         zx += z * x;
         rd = dmax(rd, fabs(q + ci - a - z));
         aty = dmax(aty, fabs(a));
         qx = dmax(qx, fabs(q));
-        if (iter0) bnd = dmax(bnd, dmax(I.l[i] - x, 0.0) + dmax(x - I.u[i], 0.0));
+        if (iter0) bnd = dmax(bnd, dmax(I.P->l[i] - x, 0.0) + dmax(x - I.P->u[i], 0.0));
     }
     for (int32_t r = threadIdx.x; r < m; r += kBB) {
-        ys += I.y_bar[r] * I.s[r];
-        const double v = I.pz[r];
-        rp = dmax(rp, fabs(proj(v, I.AL[r], I.AU[r]) - v));
+        ys += I.P->y_bar[r] * I.P->s[r];
+        const double v = I.P->pz[r];
+        rp = dmax(rp, fabs(proj(v, I.P->AL[r], I.P->AU[r]) - v));
         ax = dmax(ax, fabs(v));
     }
     cx = bsum(cx, sh);
@@ -420,24 +436,24 @@ template <class MT>
 __device__ void bsigma_dots(Inst<MT> &I, double *rec, double *t, double *sh) {
     const int32_t n = I.n, m = I.m, k = I.k;
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
-        I.dx[i] = I.x_bar[i] - I.last_x[i];
-        I.dw[i] = I.w_bar[i] - I.last_w[i];
-        if (m > 0) I.ATdy[i] = I.ATy_bar[i] - I.last_ATy[i];
+        I.P->dx[i] = I.P->x_bar[i] - I.P->last_x[i];
+        I.P->dw[i] = I.P->w_bar[i] - I.P->last_w[i];
+        if (m > 0) I.P->ATdy[i] = I.P->ATy_bar[i] - I.P->last_ATy[i];
     }
-    for (int32_t r = threadIdx.x; r < m; r += kBB) I.dy[r] = I.y_bar[r] - I.last_y[r];
+    for (int32_t r = threadIdx.x; r < m; r += kBB) I.P->dy[r] = I.P->y_bar[r] - I.P->last_y[r];
     __syncthreads();
-    lr_apply(n, k, I.U, I.d, I.dw, I.dQw, t, sh);
-    if (m > 0) lr_apply(n, k, I.U, I.d, I.ATdy, I.QATdy, t, sh);
+    lr_apply(n, k, I.P->U, I.P->d, I.P->dw, I.P->dQw, t, sh);
+    if (m > 0) lr_apply(n, k, I.P->U, I.P->d, I.P->ATdy, I.P->QATdy, t, sh);
     double a = 0, b = 0, c = 0, d = 0, e = 0, f = 0;
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
-        const double dxi = I.dx[i], dq = I.dQw[i], at = m > 0 ? I.ATdy[i] : 0.0, qat = m > 0 ? I.QATdy[i] : 0.0;
+        const double dxi = I.P->dx[i], dq = I.P->dQw[i], at = m > 0 ? I.P->ATdy[i] : 0.0, qat = m > 0 ? I.P->QATdy[i] : 0.0;
         a += dxi * dxi;
         b += dq * at;
         c += dq * dq;
-        d += I.dw[i] * dq;
+        d += I.P->dw[i] * dq;
         e += at * qat;
     }
-    for (int32_t r = threadIdx.x; r < m; r += kBB) f += I.dy[r] * I.dy[r];
+    for (int32_t r = threadIdx.x; r < m; r += kBB) f += I.P->dy[r] * I.P->dy[r];
     a = bsum(a, sh);
     b = bsum(b, sh);
     c = bsum(c, sh);
@@ -460,20 +476,20 @@ template <class MT>
 __device__ void brestart(Inst<MT> &I) {
     const int32_t n = I.n, m = I.m;
     for (int32_t r = threadIdx.x; r < m; r += kBB) {
-        I.y[r] = I.y_bar[r];
-        I.last_y[r] = I.y_bar[r];
+        I.P->y[r] = I.P->y_bar[r];
+        I.P->last_y[r] = I.P->y_bar[r];
     }
     __syncthreads();
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
-        I.x[i] = I.x_bar[i];
-        I.w[i] = I.w_bar[i];
-        I.last_x[i] = I.x_bar[i];
-        I.last_w[i] = I.w_bar[i];
+        I.P->x[i] = I.P->x_bar[i];
+        I.P->w[i] = I.P->w_bar[i];
+        I.P->last_x[i] = I.P->x_bar[i];
+        I.P->last_w[i] = I.P->w_bar[i];
         if (m > 0) {
-            const double a = csr_row(I.atrp, I.atci, I.atv, I.y_bar, i);
-            I.ATy_bar[i] = a;
-            I.last_ATy[i] = a;
-            I.ATy[i] = a;
+            const double a = csr_row(I.P->atrp, I.P->atci, I.P->atv, I.P->y_bar, i);
+            I.P->ATy_bar[i] = a;
+            I.P->last_ATy[i] = a;
+            I.P->ATy[i] = a;
         }
     }
     __syncthreads();
@@ -506,7 +522,8 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
     const int b = a.order[blockIdx.x];
     const BCtl c = a.ctl[b];
     if (!c.active) return;
-    Inst<MT> I = inst_of(a, b);
+    __shared__ IPtr<MT> tab;
+    Inst<MT> I = inst_of(a, b, &tab);
     double *rec = a.rec + (int64_t)b * R_COUNT;
     if (c.restart) brestart(I);
     if (threadIdx.x == 0) rec[R_DONE] = -1.0;
@@ -550,35 +567,36 @@ __global__ __launch_bounds__(kBB) void k_batch_init(BArgs<MT> a, int has_x0, int
     const int b = a.order[blockIdx.x];
     const BCtl c = a.ctl[b];
     if (!c.active) return;
-    Inst<MT> I = inst_of(a, b);
+    __shared__ IPtr<MT> tab;
+    Inst<MT> I = inst_of(a, b, &tab);
     double *rec = a.rec + (int64_t)b * R_COUNT;
     const int32_t n = I.n, m = I.m;
     // process_initial_points! (src/algorithm.jl:1783-1836): x0/y0 were
     // copied into x_bar / y_bar by the host
     if (has_x0) {
         for (int32_t i = threadIdx.x; i < n; i += kBB) {
-            const double v = I.x_bar[i];
-            I.x[i] = I.w[i] = I.w_bar[i] = I.last_x[i] = I.last_w[i] = v;
+            const double v = I.P->x_bar[i];
+            I.P->x[i] = I.P->w[i] = I.P->w_bar[i] = I.P->last_x[i] = I.P->last_w[i] = v;
         }
     }
     __syncthreads();
     if (has_y0) {
-        for (int32_t r = threadIdx.x; r < m; r += kBB) I.y[r] = I.last_y[r] = I.y_bar[r];
+        for (int32_t r = threadIdx.x; r < m; r += kBB) I.P->y[r] = I.P->last_y[r] = I.P->y_bar[r];
         __syncthreads();
         for (int32_t i = threadIdx.x; i < n; i += kBB) {
-            const double v = m > 0 ? csr_row(I.atrp, I.atci, I.atv, I.y_bar, i) : 0.0;
-            I.ATy_bar[i] = I.ATy[i] = I.last_ATy[i] = v;
+            const double v = m > 0 ? csr_row(I.P->atrp, I.P->atci, I.P->atv, I.P->y_bar, i) : 0.0;
+            I.P->ATy_bar[i] = I.P->ATy[i] = I.P->last_ATy[i] = v;
         }
         __syncthreads();
-        lr_apply(n, I.k, I.U, I.d, I.x_bar, I.dQw, t, sh);
+        lr_apply(n, I.k, I.P->U, I.P->d, I.P->x_bar, I.P->dQw, t, sh);
         for (int32_t i = threadIdx.x; i < n; i += kBB) {
-            const double z_raw = I.x_bar[i] + c.sigma * (-I.dQw[i] + I.ATy_bar[i] - (double)I.cM[i]);
-            I.z_bar[i] = (I.x_bar[i] - z_raw) / c.sigma;
+            const double z_raw = I.P->x_bar[i] + c.sigma * (-I.P->dQw[i] + I.P->ATy_bar[i] - (double)I.P->cM[i]);
+            I.P->z_bar[i] = (I.P->x_bar[i] - z_raw) / c.sigma;
         }
         if (m > 0) {
-            csr_rows_wave(m, I.arp, I.aci, I.av, I.x_bar, I.pz);
+            csr_rows_wave(m, I.P->arp, I.P->aci, I.P->av, I.P->x_bar, I.P->pz);
             for (int32_t r = threadIdx.x; r < m; r += kBB)
-                I.s[r] = dmin(dmax(I.pz[r] - c.lamA * c.sigma * I.y[r], (double)I.ALM[r]), (double)I.AUM[r]);
+                I.P->s[r] = dmin(dmax(I.P->pz[r] - c.lamA * c.sigma * I.P->y[r], (double)I.P->ALM[r]), (double)I.P->AUM[r]);
         }
         __syncthreads();
     }
@@ -724,7 +742,8 @@ __global__ __launch_bounds__(kBB) void k_batch_power(BArgs<MT> a, double tol, in
     __shared__ double sh[kBW];
     const int b = a.order[blockIdx.x];
     if (!a.ctl[b].active) return;
-    Inst<MT> I = inst_of(a, b);
+    __shared__ IPtr<MT> tab;
+    Inst<MT> I = inst_of(a, b, &tab);
     double *rec = a.rec + (int64_t)b * R_COUNT;
     const int32_t n = I.n, m = I.m, k = I.k;
     double lam = 0.0;
@@ -732,18 +751,18 @@ __global__ __launch_bounds__(kBB) void k_batch_power(BArgs<MT> a, double tol, in
     if (m > 0) {
         // A A^T v = A (A^T v), A^T v into pq (n)
         auto opA = [&](const double *v, double *out) {
-            for (int32_t i = threadIdx.x; i < n; i += kBB) I.pq[i] = csr_row(I.atrp, I.atci, I.atv, v, i);
+            for (int32_t i = threadIdx.x; i < n; i += kBB) I.P->pq[i] = csr_row(I.P->atrp, I.P->atci, I.P->atv, v, i);
             __syncthreads();
-            csr_rows_wave(m, I.arp, I.aci, I.av, I.pq, out);
+            csr_rows_wave(m, I.P->arp, I.P->aci, I.P->av, I.P->pq, out);
         };
-        lam = blanczos(m, opA, I.pz, I.s, I.pz2, tol, max_it, &steps, sh);
+        lam = blanczos(m, opA, I.P->pz, I.P->s, I.P->pz2, tol, max_it, &steps, sh);
     }
     if (threadIdx.x == 0) {
         rec[R_POW_A] = lam;
         rec[R_POW_ITA] = (double)steps;
     }
-    auto opQ = [&](const double *v, double *out) { lr_apply(n, k, I.U, I.d, v, out, t, sh); };
-    const double lq = blanczos(n, opQ, I.dQw, I.pq, I.pq2, tol, max_it, &steps, sh);
+    auto opQ = [&](const double *v, double *out) { lr_apply(n, k, I.P->U, I.P->d, v, out, t, sh); };
+    const double lq = blanczos(n, opQ, I.P->dQw, I.P->pq, I.P->pq2, tol, max_it, &steps, sh);
     if (threadIdx.x == 0) {
         rec[R_POW_Q] = lq;
         rec[R_POW_ITQ] = (double)steps;
