@@ -435,7 +435,8 @@ __global__ __launch_bounds__(256) void k_qap64b_p2(QapArgs g, const double *__re
 // ---------------- FP64 passes, n >= 1024, diagonal S and T -------------------
 // LDS-tiled GEMM form of the FP64 passes: a block of 2 x 2 waves owns a
 // (32 TW) x (32 TW) output, each wave TW x TW MFMA tiles.  Per DK-deep K step
-// (32 for 64-wide blocks, 16 for 128-wide, within 64 KB of LDS) the block
+// (32 for 64-wide blocks, TW = 2, the form the launcher uses; 16 for
+// 128-wide, within 64 KB of LDS) the block
 // stages R = 32 TW rows of the left operand (column-major, so each k is R
 // contiguous doubles) and R columns of the right operand (16 contiguous k
 // per load group) k-major in LDS, rows padded by one double so both the
@@ -931,7 +932,8 @@ __global__ __launch_bounds__(256) void k_qapbfb_p2(QapArgs g, const IT *__restri
 // LDS-tiled GEMM form of the two passes for the QAP relaxation's shifts
 // (diagonal S, T): a block of 2 x 2 waves owns a (32 TW) x (32 TW) output,
 // each wave a (16 TW) x (16 TW) part as TW x TW MFMA tiles.  Per K step of
-// KS (64 for 64-wide blocks, 32 for 128-wide, so LDS stays within 64 KB) the
+// KS (64 for 64-wide blocks, TW = 2, the form the launcher uses; 32 for
+// 128-wide, so LDS stays within 64 KB) the
 // block stages its data rows (three pieces) and coefficient rows (one or two
 // pieces) in LDS with 16-byte loads -- each staged fragment then feeds TW
 // MFMA tiles of two waves, so global traffic per MFMA falls by the tile
@@ -1090,18 +1092,12 @@ void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Ep
     const int64_t nn = (int64_t)g.n * g.n;
     double *Y = q->qtemp.p, *Z = q->qtemp.p + nn;
     if (g.n >= kQapLdsMin && !g.s_full && !g.t_full) {
-        // 128-wide blocks once they give every CU one, 64-wide below
-        if (g.n >= 2048) {
-            const int nb = (g.n + 127) / 128;
-            launch_k(k_qap64l_p1<4, 16>, dim3(nb, nb), dim3(256), 0, s, g, X, Y);
-            HPRQP_HIP(hipGetLastError());
-            launch_k(k_qap64l_p2<4, 16, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const double *)Y, epi);
-        } else {
-            const int nb = (g.n + 63) / 64;
-            launch_k(k_qap64l_p1<2, 32>, dim3(nb, nb), dim3(256), 0, s, g, X, Y);
-            HPRQP_HIP(hipGetLastError());
-            launch_k(k_qap64l_p2<2, 32, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const double *)Y, epi);
-        }
+        // 64-wide blocks (measured faster than 128-wide up to n = 4096:
+        // four blocks per CU hide each other's K-step barriers)
+        const int nb = (g.n + 63) / 64;
+        launch_k(k_qap64l_p1<2, 32>, dim3(nb, nb), dim3(256), 0, s, g, X, Y);
+        HPRQP_HIP(hipGetLastError());
+        launch_k(k_qap64l_p2<2, 32, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const double *)Y, epi);
         HPRQP_HIP(hipGetLastError());
         return;
     }
@@ -1130,20 +1126,12 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
         launch_k(k_qap_split<IT>, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, g.n, g.np, X, q->qxp.p,
                  (unsigned short *)nullptr);
         HPRQP_HIP(hipGetLastError());
-        // 128-wide blocks once they give every CU one, 64-wide below
-        if (g.np >= 2048) {
-            const int nb = g.np / 128;
-            launch_k(k_qapbfl_p1<4, 32>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
-            HPRQP_HIP(hipGetLastError());
-            launch_k(k_qapbfl_p2<4, 32, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
-                     epi);
-        } else {
-            const int nb = g.np / 64;
-            launch_k(k_qapbfl_p1<2, 64>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
-            HPRQP_HIP(hipGetLastError());
-            launch_k(k_qapbfl_p2<2, 64, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
-                     epi);
-        }
+        // 64-wide blocks (measured faster than 128-wide up to n = 4096)
+        const int nb = g.np / 64;
+        launch_k(k_qapbfl_p1<2, 64>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
+        HPRQP_HIP(hipGetLastError());
+        launch_k(k_qapbfl_p2<2, 64, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
+                 epi);
         HPRQP_HIP(hipGetLastError());
         return;
     }
