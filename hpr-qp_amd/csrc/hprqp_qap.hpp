@@ -284,7 +284,7 @@ __global__ __launch_bounds__(64) void k_qap64_p2(QapArgs g, const double *__rest
 // zeroes the entries past n.
 constexpr int kBT = 2;                 // MFMA tiles per block in each direction
 constexpr int kBE = 16 * kBT;          // block tile edge
-constexpr int kQapBlockedMin = 384;    // n from which the blocked passes run
+constexpr int kQapBlockedMin = 256;    // n from which the blocked passes run (measured faster at 256)
 
 // sum of the four waves' accumulators; wave w then owns components
 // q = 4w .. 4w+3 of the kBT x kBT x 4 per-lane results: the four elements

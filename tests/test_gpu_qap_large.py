@@ -3,12 +3,12 @@ closed form vec(2(AXB - SX - XT)), plus a QAP-relaxation solve at n = 400
 whose returned (x, y, z) is checked in the original space with the CPU
 operator.
 
-Sizes cover every large-n path: 384 <= n < 1024 (register-blocked passes,
+Sizes cover every large-n path: 256 <= n < 1024 (register-blocked passes,
 four waves splitting K), n >= 1024 with diagonal S and T (LDS-tiled
 split-BF16 passes with 64-wide blocks, and 128-wide at n = 2048), and a full
 S, T at n = 1100 (which stays on the register-blocked passes).  Sizes are
-ragged against the block and MFMA tiles (400, 517, 1100) as well as aligned
-(448, 512, 2048); coefficients are Gaussian (two-piece BF16 split of both
+ragged against the block and MFMA tiles (300, 400, 517, 1100) as well as
+aligned (256, 448, 512, 2048); coefficients are Gaussian (two-piece BF16 split of both
 operands) or integer flows and distances (BF16-exact, three-piece data
 split)."""
 import numpy as np
@@ -27,7 +27,7 @@ def _cases():
 
     rng = np.random.default_rng(21)
     out = []
-    for nq, full in ((400, True), (517, False), (1100, False), (1100, True)):
+    for nq, full in ((300, False), (400, True), (517, False), (1100, False), (1100, True)):
         A = rng.standard_normal((nq, nq))
         B = rng.standard_normal((nq, nq))
         if full:
@@ -35,7 +35,7 @@ def _cases():
         else:
             S, T = np.diag(rng.standard_normal(nq)), np.diag(rng.standard_normal(nq))
         out.append((f"gauss{nq}{'full' if full else 'diag'}", QAP_Q_operator(A, B, S, T), False))
-    for nq in (448, 512, 2048):
+    for nq in (256, 448, 512, 2048):
         out.append((f"flow{nq}", instances.qap(nq, seed=nq)[0].Q, True))
     return out
 
