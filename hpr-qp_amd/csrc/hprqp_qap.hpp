@@ -14,13 +14,17 @@
 //     a*b0 + a*b1 + a*b2; otherwise both are split in two and the terms are
 //     a0*b0 + a0*b1 + a1*b0 (16 significant bits).
 //
-// Both run the operator as two tiled passes, one wavefront per 16 x 16
-// result tile, the K dimension in slabs of kKS staged through LDS by
-// coalesced loads:
+// Both run the operator as two passes:
 //   pass 1: Y = A X and, for a full S, Z = S X              -> temporaries
 //   pass 2: R = Y B - X T (full T);
 //           out = 2 (R - Z, or - s_i X_ij for diagonal S, - X_ij t_j for
-//           diagonal T)
+//           diagonal T), handed to the caller's epilogue
+// in one of three kernel families by size: n < 256, one 16 x 16 tile per
+// block with its K range split over the block's four waves; 256 <= n < 1024
+// (and any full S / T), register-blocked 32 x 32 blocks; n >= 1024 with
+// diagonal S and T, LDS-tiled 64 x 64 blocks.  Staging loads are issued all
+// together before their first use, with out-of-range lanes reading element 0
+// and discarding it, so that no load waits behind a branch.
 #pragma once
 #include <cmath>
 #include <cstring>
@@ -40,7 +44,6 @@ typedef __attribute__((ext_vector_type(4))) double doublex4;
 //   l%16, same k; result element e (0..3) at row 4g + e, column l%16.
 //   f64 16x16x4: left row l%16, k = g; right column l%16, k = g; result
 //   element e at row g + 4e, column l%16.
-constexpr int kKS = 128;  // K slab
 
 // ---------------- BF16 splitting ---------------------------------------
 __device__ __forceinline__ unsigned short bf16_rn(float f) {
@@ -153,73 +156,94 @@ static inline QapArgs qap_args(const hprqp_qop *q) {
     return g;
 }
 
-// ---------------- FP64 passes ----------------------------------------------
-constexpr int kLd64 = kKS + 1;  // double stride of staged column blocks
-
-// Staging: every lane issues all of its loads of a slab before the first
-// LDS store (fully unrolled, so the loads are in flight together), then
-// stores.  A slab is 16 rows or columns x kKS; entries outside the matrix
-// are zero.  Out-of-range lanes load element 0 and discard it, so that no
-// load sits behind a branch (a branch around each load would serialise
-// them).
-constexpr int kSlabIt = 16 * kKS / 64;  // elements per lane per slab
-
-// rows r0..r0+15 of column-major M over columns k0..k0+kKS-1 into L[k][i]
-// (ld 16), negated when neg
-template <class T, class D>
-__device__ __forceinline__ void stage_rows(const T *M, int n, int r0, int k0, D *L, bool neg, int ldk, bool kmajor) {
-    const int lane = threadIdx.x, i = lane & 15, r = r0 + i;
-    D v[kSlabIt];
+// ---------------- n < 256: split-K one-tile passes ---------------------------
+// The one-tile passes with the tile's K range split over four waves of a
+// 256-thread block: wave w stages and multiplies the K slabs w, w+4, ... of
+// width KS (16 FP64, 32 split-BF16) in its own LDS region, so a tile's loads
+// are spread over four waves and each MFMA chain is a quarter as long; the
+// four partial tiles are summed through LDS in wave order and wave 0 stores
+// (pass 1) or runs the epilogue (pass 2).  Every wave makes the same number
+// of slab rounds (a slab past n stages zeros and skips its MFMAs), so the
+// block-wide barriers are uniform.
+template <int KS>
+__device__ __forceinline__ void stage_rows_k(const double *__restrict__ M, int n, int r0, int k0, double *L, bool neg) {
+    constexpr int IT = 16 * KS / 64;
+    const int lane = threadIdx.x & 63, i = lane & 15, r = r0 + i;
+    double v[IT];
 #pragma unroll
-    for (int t = 0; t < kSlabIt; ++t) {
+    for (int t = 0; t < IT; ++t) {
         const int k = k0 + t * 4 + (lane >> 4);
         const bool in = r < n && k < n;
-        const D a = (D)M[in ? r + (int64_t)k * n : 0];
-        v[t] = in ? a : (D)0;
+        const double a = M[in ? r + (int64_t)k * n : 0];
+        v[t] = in ? a : 0.0;
     }
 #pragma unroll
-    for (int t = 0; t < kSlabIt; ++t) {
-        const int kk = t * 4 + (lane >> 4);
-        L[kmajor ? kk * 16 + i : i * ldk + kk] = neg ? -v[t] : v[t];
-    }
+    for (int t = 0; t < IT; ++t) L[(t * 4 + (lane >> 4)) * 16 + i] = neg ? -v[t] : v[t];
 }
-// columns c0..c0+15 of column-major M over rows k0..k0+kKS-1 into L[j][k]
-template <class T, class D>
-__device__ __forceinline__ void stage_cols(const T *M, int n, int c0, int k0, D *L, int ldk) {
-    const int lane = threadIdx.x;
-    D v[kSlabIt];
+// columns c0..c0+15 of column-major M over rows k0..k0+KS-1 into L[j * ld + k]
+template <int KS, class T, class D>
+__device__ __forceinline__ void stage_cols_k(const T *__restrict__ M, int n, int c0, int k0, D *L, int ld) {
+    constexpr int IT = 16 * KS / 64, CW = 64 / KS;  // columns per pass of the wave
+    const int lane = threadIdx.x & 63, kk = lane % KS;
+    D v[IT];
 #pragma unroll
-    for (int t = 0; t < kSlabIt; ++t) {
-        const int kk = (t & 1) * 64 + lane, c = c0 + (t >> 1), k = k0 + kk;
+    for (int t = 0; t < IT; ++t) {
+        const int c = c0 + t * CW + lane / KS, k = k0 + kk;
         const bool in = c < n && k < n;
         const D a = (D)M[in ? k + (int64_t)c * n : 0];
         v[t] = in ? a : (D)0;
     }
 #pragma unroll
-    for (int t = 0; t < kSlabIt; ++t) L[(t >> 1) * ldk + (t & 1) * 64 + lane] = v[t];
+    for (int t = 0; t < IT; ++t) L[(t * CW + lane / KS) * ld + kk] = v[t];
 }
 
-static __global__ __launch_bounds__(64) void k_qap64_p1(QapArgs g, const double *__restrict__ x, double *__restrict__ Y,
-                                                  double *__restrict__ Z) {
-    __shared__ double As[kKS * 16], Ss[kKS * 16], Xs[16 * kLd64];
+constexpr int kSK64 = 16;          // FP64 K slab per wave
+constexpr int kSLd64 = kSK64 + 1;  // double stride of staged columns
+
+__device__ __forceinline__ doublex4 wave_sum1(doublex4 a, double *red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[(w * 4 + e) * 64 + lane] = a[e];
+    __syncthreads();
+    doublex4 r = {0.0, 0.0, 0.0, 0.0};
+    if (w == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            double v = red[e * 64 + lane];
+#pragma unroll
+            for (int u = 1; u < 4; ++u) v += red[(u * 4 + e) * 64 + lane];
+            r[e] = v;
+        }
+    }
+    __syncthreads();
+    return r;
+}
+
+static __global__ __launch_bounds__(256) void k_qap64s_p1(QapArgs g, const double *__restrict__ x,
+                                                          double *__restrict__ Y, double *__restrict__ Z) {
+    __shared__ double As[4][kSK64 * 16], Ss[4][kSK64 * 16], Xs[4][16 * kSLd64];
+    __shared__ double red[4 * 4 * 64];
     const int n = g.n, r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
-    const int li = threadIdx.x & 15, lg = threadIdx.x >> 4;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
     doublex4 ay = {0.0, 0.0, 0.0, 0.0}, az = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < n; k0 += kKS) {
-        const int kwp = (min(kKS, n - k0) + 3) & ~3;
-        stage_rows(g.A, n, r0, k0, As, false, 0, true);
-        if (g.s_full) stage_rows(g.S, n, r0, k0, Ss, false, 0, true);
-        stage_cols(x, n, c0, k0, Xs, kLd64);
+    for (int kb = 0; kb < n; kb += 4 * kSK64) {
+        const int k0 = kb + w * kSK64;
+        const int kwp = k0 < n ? (min(kSK64, n - k0) + 3) & ~3 : 0;
+        stage_rows_k<kSK64>(g.A, n, r0, k0, As[w], false);
+        if (g.s_full) stage_rows_k<kSK64>(g.S, n, r0, k0, Ss[w], false);
+        stage_cols_k<kSK64>(x, n, c0, k0, Xs[w], kSLd64);
         __syncthreads();
         for (int kq = 0; kq < kwp; kq += 4) {
-            const double b = Xs[li * kLd64 + kq + lg];
-            ay = __builtin_amdgcn_mfma_f64_16x16x4f64(As[(kq + lg) * 16 + li], b, ay, 0, 0, 0);
-            if (g.s_full) az = __builtin_amdgcn_mfma_f64_16x16x4f64(Ss[(kq + lg) * 16 + li], b, az, 0, 0, 0);
+            const double b = Xs[w][li * kSLd64 + kq + lg];
+            ay = __builtin_amdgcn_mfma_f64_16x16x4f64(As[w][(kq + lg) * 16 + li], b, ay, 0, 0, 0);
+            if (g.s_full) az = __builtin_amdgcn_mfma_f64_16x16x4f64(Ss[w][(kq + lg) * 16 + li], b, az, 0, 0, 0);
         }
         __syncthreads();
     }
+    ay = wave_sum1(ay, red);
+    if (g.s_full) az = wave_sum1(az, red);
     const int j = c0 + li;
-    if (j >= n) return;
+    if (w != 0 || j >= n) return;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int r = r0 + lg + 4 * e;
@@ -230,32 +254,36 @@ static __global__ __launch_bounds__(64) void k_qap64_p1(QapArgs g, const double 
 }
 
 template <class Epi>
-__global__ __launch_bounds__(64) void k_qap64_p2(QapArgs g, const double *__restrict__ x,
-                                                  const double *__restrict__ Y, const double *__restrict__ Z, Epi epi) {
-    __shared__ double Ls[kKS * 16], Xr[kKS * 16], Bs[16 * kLd64], Ts[16 * kLd64];
+__global__ __launch_bounds__(256) void k_qap64s_p2(QapArgs g, const double *__restrict__ x,
+                                                   const double *__restrict__ Y, const double *__restrict__ Z,
+                                                   Epi epi) {
+    __shared__ double Ls[4][kSK64 * 16], Xr[4][kSK64 * 16], Bs[4][16 * kSLd64], Ts[4][16 * kSLd64];
+    __shared__ double red[4 * 4 * 64];
     const int n = g.n, r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
-    const int li = threadIdx.x & 15, lg = threadIdx.x >> 4;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
     doublex4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < n; k0 += kKS) {
-        const int kwp = (min(kKS, n - k0) + 3) & ~3;
-        stage_rows(Y, n, r0, k0, Ls, false, 0, true);
-        stage_cols(g.B, n, c0, k0, Bs, kLd64);
+    for (int kb = 0; kb < n; kb += 4 * kSK64) {
+        const int k0 = kb + w * kSK64;
+        const int kwp = k0 < n ? (min(kSK64, n - k0) + 3) & ~3 : 0;
+        stage_rows_k<kSK64>(Y, n, r0, k0, Ls[w], false);
+        stage_cols_k<kSK64>(g.B, n, c0, k0, Bs[w], kSLd64);
         if (g.t_full) {
-            stage_rows(x, n, r0, k0, Xr, true, 0, true);
-            stage_cols(g.T, n, c0, k0, Ts, kLd64);
+            stage_rows_k<kSK64>(x, n, r0, k0, Xr[w], true);
+            stage_cols_k<kSK64>(g.T, n, c0, k0, Ts[w], kSLd64);
         }
         __syncthreads();
         for (int kq = 0; kq < kwp; kq += 4) {
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ls[(kq + lg) * 16 + li], Bs[li * kLd64 + kq + lg], acc, 0, 0,
-                                                       0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ls[w][(kq + lg) * 16 + li], Bs[w][li * kSLd64 + kq + lg], acc,
+                                                       0, 0, 0);
             if (g.t_full)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xr[(kq + lg) * 16 + li], Ts[li * kLd64 + kq + lg], acc,
-                                                           0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xr[w][(kq + lg) * 16 + li], Ts[w][li * kSLd64 + kq + lg],
+                                                           acc, 0, 0, 0);
         }
         __syncthreads();
     }
+    acc = wave_sum1(acc, red);
     const int j = c0 + li;
-    if (j >= n) return;
+    if (w != 0 || j >= n) return;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int r = r0 + lg + 4 * e;
@@ -570,73 +598,96 @@ __global__ __launch_bounds__(256) void k_qap64l_p2(QapArgs g, const double *__re
         }
 }
 
-// ---------------- split-BF16 x3 passes ---------------------------------------
-constexpr int kLdB = kKS + 8;  // u16 row stride of staged coefficient pieces
-constexpr int kLdF = kKS + 4;  // float row stride of staged data
+// ---------------- split-BF16 x3, n < 256: split-K one-tile passes ------------
+constexpr int kSKB = 32;           // K slab per wave: one MFMA K step
+constexpr int kSLdB = kSKB + 8;    // u16 row stride of staged pieces
+constexpr int kSLdF = kSKB + 4;    // float row stride of staged data
 
-// rows b0..b0+15 of a padded np x np piece stored P[b * np + k] (the row-major
-// left pieces A, S, and the column-major right pieces B, T, whose column b
-// is contiguous) over k0..k0+kKS-1 into L[i][k], 16-byte loads
-__device__ __forceinline__ void stage_piece(const unsigned short *P, int np, int b0, int k0, unsigned short *L) {
-    constexpr int kChunks = 16 * kKS / 8 / 64;  // 16-byte chunks per lane
-    const int lane = threadIdx.x;
-    u16x8 v[kChunks];
+// rows b0..b0+15 of a padded piece over k0..k0+31 into L[i * kSLdB + k]
+// (one 16-byte chunk per lane; zero past np)
+__device__ __forceinline__ void stage_piece_s(const unsigned short *__restrict__ P, int np, int b0, int k0,
+                                              unsigned short *L) {
+    const int lane = threadIdx.x & 63, i = lane >> 2, kc = (lane & 3) * 8, k = k0 + kc;
+    const bool in = k < np;
+    const u16x8 a = *(const u16x8 *)(P + (in ? (int64_t)(b0 + i) * np + k : 0));
+    const u16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+    *(u16x8 *)(L + i * kSLdB + kc) = in ? a : z;
+}
+// rows r0..r0+15 of column-major M over columns k0..k0+31 into L[i * ld + k]
+template <class T>
+__device__ __forceinline__ void stage_rows_f(const T *__restrict__ M, int n, int r0, int k0, float *L, bool neg) {
+    const int lane = threadIdx.x & 63, i = lane & 15, r = r0 + i;
+    float v[8];
 #pragma unroll
-    for (int t = 0; t < kChunks; ++t) {
-        const int e = t * 64 + lane, i = e / (kKS / 8), k = k0 + (e % (kKS / 8)) * 8;
-        const bool in = k < np;
-        const u16x8 a = *(const u16x8 *)(P + (in ? (int64_t)(b0 + i) * np + k : 0));
-        const u16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
-        v[t] = in ? a : z;
+    for (int t = 0; t < 8; ++t) {
+        const int k = k0 + t * 4 + (lane >> 4);
+        const bool in = r < n && k < n;
+        const float a = (float)M[in ? r + (int64_t)k * n : 0];
+        v[t] = in ? a : 0.f;
     }
 #pragma unroll
-    for (int t = 0; t < kChunks; ++t) {
-        const int e = t * 64 + lane, i = e / (kKS / 8), kk = (e % (kKS / 8)) * 8;
-        *(u16x8 *)(L + i * kLdB + kk) = v[t];
-    }
+    for (int t = 0; t < 8; ++t) L[i * kSLdF + t * 4 + (lane >> 4)] = neg ? -v[t] : v[t];
 }
-__device__ __forceinline__ void frag_f(const float *L, int li, int k, float v[8]) {
+__device__ __forceinline__ floatx4 wave_sum1f(floatx4 a, float *red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = L[li * kLdF + k + e];
-}
-__device__ __forceinline__ bf16x8 frag_b(const unsigned short *L, int li, int k) {
-    return __builtin_bit_cast(bf16x8, *(const u16x8 *)(L + li * kLdB + k));
+    for (int e = 0; e < 4; ++e) red[(w * 4 + e) * 64 + lane] = a[e];
+    __syncthreads();
+    floatx4 r = {0.f, 0.f, 0.f, 0.f};
+    if (w == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float v = red[e * 64 + lane];
+#pragma unroll
+            for (int u = 1; u < 4; ++u) v += red[(u * 4 + e) * 64 + lane];
+            r[e] = v;
+        }
+    }
+    __syncthreads();
+    return r;
 }
 
 template <class IT>
-__global__ __launch_bounds__(64) void k_qapbf_p1(QapArgs g, const IT *__restrict__ x, float *__restrict__ Y,
-                                                  float *__restrict__ Z) {
-    __shared__ __attribute__((aligned(16))) unsigned short A0s[16 * kLdB], A1s[16 * kLdB], S0s[16 * kLdB],
-        S1s[16 * kLdB];
-    __shared__ float Xs[16 * kLdF];
+__global__ __launch_bounds__(256) void k_qapbfs_p1(QapArgs g, const IT *__restrict__ x, float *__restrict__ Y,
+                                                   float *__restrict__ Z) {
+    __shared__ __attribute__((aligned(16))) unsigned short A0s[4][16 * kSLdB], A1s[4][16 * kSLdB],
+        S0s[4][16 * kSLdB], S1s[4][16 * kSLdB];
+    __shared__ float Xs[4][16 * kSLdF];
+    __shared__ float red[4 * 4 * 64];
     const int n = g.n, np = g.np, r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
-    const int li = threadIdx.x & 15, lg = threadIdx.x >> 4;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
     floatx4 ay = {0.f, 0.f, 0.f, 0.f}, az = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < np; k0 += kKS) {
-        stage_piece(g.A0, np, r0, k0, A0s);
-        if (!g.a_exact) stage_piece(g.A1, np, r0, k0, A1s);
+    for (int kb = 0; kb < np; kb += 4 * kSKB) {
+        const int k0 = kb + w * kSKB;
+        stage_piece_s(g.A0, np, r0, k0, A0s[w]);
+        if (!g.a_exact) stage_piece_s(g.A1, np, r0, k0, A1s[w]);
         if (g.s_full) {
-            stage_piece(g.S0, np, r0, k0, S0s);
-            if (!g.s_exact) stage_piece(g.S1, np, r0, k0, S1s);
+            stage_piece_s(g.S0, np, r0, k0, S0s[w]);
+            if (!g.s_exact) stage_piece_s(g.S1, np, r0, k0, S1s[w]);
         }
-        stage_cols(x, n, c0, k0, Xs, kLdF);
+        stage_cols_k<kSKB>(x, n, c0, k0, Xs[w], kSLdF);
         __syncthreads();
-        const int kend = min(kKS, np - k0);
-        for (int kq = 0; kq < kend; kq += 32) {
-            const int k = kq + 8 * lg;
+        if (k0 < np) {
             float v[8];
-            frag_f(Xs, li, k, v);
-            const bf16x8 a0 = frag_b(A0s, li, k);
-            ay = mma3_cl(ay, a0, g.a_exact ? a0 : frag_b(A1s, li, k), g.a_exact, v);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = Xs[w][li * kSLdF + 8 * lg + e];
+            const bf16x8 a0 = __builtin_bit_cast(bf16x8, *(const u16x8 *)(A0s[w] + li * kSLdB + 8 * lg));
+            const bf16x8 a1 =
+                g.a_exact ? a0 : __builtin_bit_cast(bf16x8, *(const u16x8 *)(A1s[w] + li * kSLdB + 8 * lg));
+            ay = mma3_cl(ay, a0, a1, g.a_exact, v);
             if (g.s_full) {
-                const bf16x8 s0 = frag_b(S0s, li, k);
-                az = mma3_cl(az, s0, g.s_exact ? s0 : frag_b(S1s, li, k), g.s_exact, v);
+                const bf16x8 s0 = __builtin_bit_cast(bf16x8, *(const u16x8 *)(S0s[w] + li * kSLdB + 8 * lg));
+                const bf16x8 s1 =
+                    g.s_exact ? s0 : __builtin_bit_cast(bf16x8, *(const u16x8 *)(S1s[w] + li * kSLdB + 8 * lg));
+                az = mma3_cl(az, s0, s1, g.s_exact, v);
             }
         }
         __syncthreads();
     }
+    ay = wave_sum1f(ay, red);
+    if (g.s_full) az = wave_sum1f(az, red);
     const int j = c0 + li;
-    if (j >= n) return;
+    if (w != 0 || j >= n) return;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int r = r0 + 4 * lg + e;
@@ -647,41 +698,48 @@ __global__ __launch_bounds__(64) void k_qapbf_p1(QapArgs g, const IT *__restrict
 }
 
 template <class IT, class Epi>
-__global__ __launch_bounds__(64) void k_qapbf_p2(QapArgs g, const IT *__restrict__ x, const float *__restrict__ Y,
-                                                  const float *__restrict__ Z, Epi epi) {
-    __shared__ __attribute__((aligned(16))) unsigned short B0s[16 * kLdB], B1s[16 * kLdB], T0s[16 * kLdB],
-        T1s[16 * kLdB];
-    __shared__ float Ys[16 * kLdF], Xr[16 * kLdF];
+__global__ __launch_bounds__(256) void k_qapbfs_p2(QapArgs g, const IT *__restrict__ x, const float *__restrict__ Y,
+                                                   const float *__restrict__ Z, Epi epi) {
+    __shared__ __attribute__((aligned(16))) unsigned short B0s[4][16 * kSLdB], B1s[4][16 * kSLdB],
+        T0s[4][16 * kSLdB], T1s[4][16 * kSLdB];
+    __shared__ float Ys[4][16 * kSLdF], Xr[4][16 * kSLdF];
+    __shared__ float red[4 * 4 * 64];
     const int n = g.n, np = g.np, r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
-    const int li = threadIdx.x & 15, lg = threadIdx.x >> 4;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < np; k0 += kKS) {
-        stage_rows(Y, n, r0, k0, Ys, false, kLdF, false);
-        stage_piece(g.B0, np, c0, k0, B0s);
-        if (!g.b_exact) stage_piece(g.B1, np, c0, k0, B1s);
+    for (int kb = 0; kb < np; kb += 4 * kSKB) {
+        const int k0 = kb + w * kSKB;
+        stage_rows_f(Y, n, r0, k0, Ys[w], false);
+        stage_piece_s(g.B0, np, c0, k0, B0s[w]);
+        if (!g.b_exact) stage_piece_s(g.B1, np, c0, k0, B1s[w]);
         if (g.t_full) {
-            stage_rows(x, n, r0, k0, Xr, true, kLdF, false);
-            stage_piece(g.T0, np, c0, k0, T0s);
-            if (!g.t_exact) stage_piece(g.T1, np, c0, k0, T1s);
+            stage_rows_f(x, n, r0, k0, Xr[w], true);
+            stage_piece_s(g.T0, np, c0, k0, T0s[w]);
+            if (!g.t_exact) stage_piece_s(g.T1, np, c0, k0, T1s[w]);
         }
         __syncthreads();
-        const int kend = min(kKS, np - k0);
-        for (int kq = 0; kq < kend; kq += 32) {
-            const int k = kq + 8 * lg;
+        if (k0 < np) {
             float v[8];
-            frag_f(Ys, li, k, v);
-            const bf16x8 b0 = frag_b(B0s, li, k);
-            acc = mma3_cr(acc, b0, g.b_exact ? b0 : frag_b(B1s, li, k), g.b_exact, v);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = Ys[w][li * kSLdF + 8 * lg + e];
+            const bf16x8 b0 = __builtin_bit_cast(bf16x8, *(const u16x8 *)(B0s[w] + li * kSLdB + 8 * lg));
+            const bf16x8 b1 =
+                g.b_exact ? b0 : __builtin_bit_cast(bf16x8, *(const u16x8 *)(B1s[w] + li * kSLdB + 8 * lg));
+            acc = mma3_cr(acc, b0, b1, g.b_exact, v);
             if (g.t_full) {
-                frag_f(Xr, li, k, v);
-                const bf16x8 t0 = frag_b(T0s, li, k);
-                acc = mma3_cr(acc, t0, g.t_exact ? t0 : frag_b(T1s, li, k), g.t_exact, v);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = Xr[w][li * kSLdF + 8 * lg + e];
+                const bf16x8 t0 = __builtin_bit_cast(bf16x8, *(const u16x8 *)(T0s[w] + li * kSLdB + 8 * lg));
+                const bf16x8 t1 =
+                    g.t_exact ? t0 : __builtin_bit_cast(bf16x8, *(const u16x8 *)(T1s[w] + li * kSLdB + 8 * lg));
+                acc = mma3_cr(acc, t0, t1, g.t_exact, v);
             }
         }
         __syncthreads();
     }
+    acc = wave_sum1f(acc, red);
     const int j = c0 + li;
-    if (j >= n) return;
+    if (w != 0 || j >= n) return;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int r = r0 + 4 * lg + e;
@@ -1110,9 +1168,9 @@ void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Ep
         return;
     }
     const dim3 grid(nt, nt);
-    launch_k(k_qap64_p1, grid, dim3(64), 0, s, g, X, Y, Z);
+    launch_k(k_qap64s_p1, grid, dim3(256), 0, s, g, X, Y, Z);
     HPRQP_HIP(hipGetLastError());
-    launch_k(k_qap64_p2<Epi>, grid, dim3(64), 0, s, g, X, (const double *)Y, (const double *)Z, epi);
+    launch_k(k_qap64s_p2<Epi>, grid, dim3(256), 0, s, g, X, (const double *)Y, (const double *)Z, epi);
     HPRQP_HIP(hipGetLastError());
 }
 
@@ -1148,9 +1206,9 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
         return;
     }
     const dim3 grid(nt, nt);
-    launch_k(k_qapbf_p1<IT>, grid, dim3(64), 0, s, g, X, Y, Z);
+    launch_k(k_qapbfs_p1<IT>, grid, dim3(256), 0, s, g, X, Y, Z);
     HPRQP_HIP(hipGetLastError());
-    launch_k(k_qapbf_p2<IT, Epi>, grid, dim3(64), 0, s, g, X, (const float *)Y, (const float *)Z, epi);
+    launch_k(k_qapbfs_p2<IT, Epi>, grid, dim3(256), 0, s, g, X, (const float *)Y, (const float *)Z, epi);
     HPRQP_HIP(hipGetLastError());
 }
 
