@@ -190,10 +190,10 @@ struct hprqp_model {
     // Q pattern symmetric completion and transpose position map (CSR Q only)
     hprqp::HostCsr Qsym;
     std::vector<int32_t> q_tpos;
-    // SpMV lanes chosen by the autotuner (0 = not tuned yet), per matrix
-    // value precision (FP64, FP32) for Q, A, A^T and a LASSO operator's
-    // data matrix and its transpose
-    mutable int tuned_lanes[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+    // SpMV lanes chosen by the autotuner (0 = not tuned yet), per precision
+    // mode (HPRQP_PREC_FP64 / MIXED / FP32) for Q, A, A^T and a LASSO
+    // operator's data matrix and its transpose
+    mutable int tuned_lanes[3][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
 };
 
 struct hprqp_batch {

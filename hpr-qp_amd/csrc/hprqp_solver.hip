@@ -1695,8 +1695,10 @@ struct EventTimer {
 };
 
 // Rows-per-group autotuner for one matrix: times the plain product for every
-// lane count (replaces determine_spmv_mode's cuSPARSE-vs-custom choice).
-template <class VT>
+// lane count (replaces determine_spmv_mode's cuSPARSE-vs-custom choice), with
+// the iteration's own types: matrix values MT, vectors and accumulation VT
+// (FP64 / FP64 plain, FP32 / FP64 mixed, FP32 / FP32 fp32).
+template <class MT, class VT>
 static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
     DevVec<VT> x(std::max<int64_t>(1, a.ncol)), y(std::max<int64_t>(1, a.nrow));
     {
@@ -1712,7 +1714,7 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
     if (mean >= 64.0) cand.push_back(kBlock);  // one block per row
     auto time_of = [&](int L) {
         return tm.avg_ms(reps, [&] {
-            launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<VT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
+            launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
         });
     };
     // the heuristic shape stays unless another is clearly faster (10 %), so
@@ -1731,6 +1733,10 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
 }
 
 static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool mixed) {
+    HPRQP_REQUIRE(p->precision == HPRQP_PREC_FP64 || p->precision == HPRQP_PREC_MIXED ||
+                      p->precision == HPRQP_PREC_FP32,
+                  HPRQP_E_ARG, "unknown precision mode");
+    (void)mixed;
     if (p->spmv_lanes != 0) {
         const int L = p->spmv_lanes;
         const bool vec = L > 0 && (L & (L - 1)) == 0 && (L <= 64 || L == kBlock);
@@ -1753,13 +1759,16 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
         if (!mats[k]) continue;
         DevCsr &a = *mats[k];
         if (a.nnz < (1 << 18)) continue;  // heuristic choice is already close
-        int &tuned = md->tuned_lanes[mixed ? 1 : 0][k];
+        int &tuned = md->tuned_lanes[p->precision][k];
         if (tuned == 0) {
-            if (mixed) {
+            if (p->precision == HPRQP_PREC_MIXED) {
                 ensure_v32(a);
-                tuned = tune_lanes<float>(s, a, 20);
+                tuned = tune_lanes<float, double>(s, a, 20);
+            } else if (p->precision == HPRQP_PREC_FP32) {
+                ensure_v32(a);
+                tuned = tune_lanes<float, float>(s, a, 20);
             } else {
-                tuned = tune_lanes<double>(s, a, 20);
+                tuned = tune_lanes<double, double>(s, a, 20);
             }
         }
         a.lanes = tuned;
