@@ -173,10 +173,39 @@ __device__ void lr_apply(int32_t n, int32_t k, const UT *U, const UT *d, const V
     for (int32_t i = threadIdx.x; i < n; i += kBB) out[i] = (double)d[i] * (double)v[i] + lr_row(n, k, U, i, t);
     __syncthreads();
 }
-// y_r = sum A_r. v  for the instance's m rows: one wave per row
+// y_r = sum A_r. v  for the instance's m rows: one wave per row, or, with
+// fewer rows than waves (a budget row of n nonzeros), each row by the whole
+// block with four loads per thread in flight -- one wave walking a long row
+// waits on its loads n/64 times in a row
 template <class AT_, class VT>
 __device__ void csr_rows_wave(int32_t m, const int32_t *rp, const int32_t *ci, const AT_ *v, const VT *x,
                               double *out) {
+    if (m < kBW) {
+        __shared__ double shr[kBW];
+        for (int32_t r = 0; r < m; ++r) {
+            const int32_t s = rp[r], e = rp[r + 1];
+            double a[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int32_t q0 = s + (int32_t)threadIdx.x; q0 < e; q0 += 4 * kBB) {
+                AT_ vv[4];
+                int32_t cc[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {  // past the row's end: its first nonzero, weight zero
+                    const int32_t q = q0 + u * kBB, at = q < e ? q : s;
+                    vv[u] = v[at];
+                    cc[u] = ci[at];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double pr = (double)vv[u] * (double)x[cc[u]];
+                    a[u] += q0 + u * kBB < e ? pr : 0.0;
+                }
+            }
+            const double tot = bsum((a[0] + a[1]) + (a[2] + a[3]), shr);
+            if (threadIdx.x == 0) out[r] = tot;
+        }
+        __syncthreads();
+        return;
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int32_t r = wave; r < m; r += kBB / 64) {
         double acc = 0.0;
