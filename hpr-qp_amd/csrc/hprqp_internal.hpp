@@ -158,7 +158,8 @@ struct hprqp_qop {
     hprqp::DevVec<double> qa, qb, qs, qt, qtemp;
     hprqp::DevVec<float> qtemp32;
     // QAP operands of the MFMA kernels (built by qap_prepare): split-BF16
-    // pieces padded to qnp = nq rounded up to 32 (A, S row-major; B, T
+    // pieces padded to qnp = nq rounded up to 32, or to 128 where the
+    // blocked passes run (nq >= 256) (A, S row-major; B, T
     // column-major); the diagonals of S / T when those are diagonal (the
     // full matrices are then not read); qtemp / qtemp32 hold the two n x n
     // intermediates of the FP64 / split-BF16 passes
