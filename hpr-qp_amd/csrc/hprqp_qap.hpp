@@ -22,7 +22,8 @@
 // in one of three kernel families by size: n < 256, one 16 x 16 tile per
 // block with its K range split over the block's four waves; 256 <= n < 1024
 // (and any full S / T), register-blocked 32 x 32 blocks; n >= 1024 with
-// diagonal S and T, LDS-tiled 64 x 64 blocks.  Staging loads are issued all
+// diagonal S and T, LDS-tiled 64 x 64 blocks (split-BF16 from n = 2048:
+// 64 x 128, swizzled LDS rows).  Staging loads are issued all
 // together before their first use, with out-of-range lanes reading element 0
 // and discarding it, so that no load waits behind a branch.
 #pragma once
@@ -782,31 +783,31 @@ __device__ __forceinline__ floatx4 mma_dc(floatx4 acc, const bf16x8 (&d)[3], bf1
     return acc;
 }
 // acc[a][b] += D_a C_b for all a, b as three rounds, each issuing the
-// TW x TW independent MFMAs back to back (rather than each accumulator's
+// TA x TB independent MFMAs back to back (rather than each accumulator's
 // three dependent MFMAs in a row); per accumulator the terms are added in
 // the order of mma_dc
-template <int TW, bool EXACT>
-__device__ __forceinline__ void mma_rounds_(floatx4 (&acc)[TW][TW], const bf16x8 (&d)[TW][3],
-                                            const bf16x8 (&c0)[TW], const bf16x8 (&c1)[TW]) {
+template <int TA, int TB, bool EXACT>
+__device__ __forceinline__ void mma_rounds_(floatx4 (&acc)[TA][TB], const bf16x8 (&d)[TA][3],
+                                            const bf16x8 (&c0)[TB], const bf16x8 (&c1)[TB]) {
 #pragma unroll
     for (int rd = 0; rd < 3; ++rd)
 #pragma unroll
-        for (int a = 0; a < TW; ++a)
+        for (int a = 0; a < TA; ++a)
 #pragma unroll
-            for (int b = 0; b < TW; ++b) {
+            for (int b = 0; b < TB; ++b) {
                 if constexpr (EXACT)
                     acc[a][b] = mfma_bf16(d[a][2 - rd], c0[b], acc[a][b]);
                 else
                     acc[a][b] = mfma_bf16(d[a][rd == 1 ? 1 : 0], rd == 0 ? c1[b] : c0[b], acc[a][b]);
             }
 }
-template <int TW>
-__device__ __forceinline__ void mma_rounds(floatx4 (&acc)[TW][TW], const bf16x8 (&d)[TW][3], const bf16x8 (&c0)[TW],
-                                           const bf16x8 (&c1)[TW], bool exact) {
+template <int TA, int TB = TA>
+__device__ __forceinline__ void mma_rounds(floatx4 (&acc)[TA][TB], const bf16x8 (&d)[TA][3], const bf16x8 (&c0)[TB],
+                                           const bf16x8 (&c1)[TB], bool exact) {
     if (exact)
-        mma_rounds_<TW, true>(acc, d, c0, c1);
+        mma_rounds_<TA, TB, true>(acc, d, c0, c1);
     else
-        mma_rounds_<TW, false>(acc, d, c0, c1);
+        mma_rounds_<TA, TB, false>(acc, d, c0, c1);
 }
 __device__ __forceinline__ void wave_sum_f(floatx4 (&acc)[kBT][kBT], float *red, float (&out)[kBT * kBT]) {
     constexpr int NC = kBT * kBT * 4;
@@ -988,147 +989,194 @@ __global__ __launch_bounds__(256) void k_qapbfb_p2(QapArgs g, const IT *__restri
 
 // ---------------- split-BF16 x3 passes, n >= 1024, diagonal S and T ---------
 // LDS-tiled GEMM form of the two passes for the QAP relaxation's shifts
-// (diagonal S, T): a block of 2 x 2 waves owns a (32 TW) x (32 TW) output,
-// each wave a (16 TW) x (16 TW) part as TW x TW MFMA tiles.  Per K step of
-// KS (64 for 64-wide blocks, TW = 2, the form the launcher uses; 32 for
-// 128-wide, so LDS stays within 64 KB) the
-// block stages its data rows (three pieces) and coefficient rows (one or two
-// pieces) in LDS with 16-byte loads -- each staged fragment then feeds TW
-// MFMA tiles of two waves, so global traffic per MFMA falls by the tile
-// edge -- and every wave reads its fragments from LDS.  Rows are padded to
-// KS + 8 halfwords to spread the 16 lanes reading one fragment column over
-// the banks.  Piece arrays are qnp x qnp with qnp a multiple of 128, so no
-// access needs a bound check; results outside n x n are not stored.
+// (diagonal S, T): a block of 2 x 2 waves owns a (32 TA) x (32 TB) output
+// (data rows x coefficient rows), each wave a (16 TA) x (16 TB) part as
+// TA x TB MFMA tiles.  Per K step of KS the block stages its data rows
+// (three pieces) and coefficient rows (one or two pieces) in LDS with
+// 16-byte loads -- each staged fragment then feeds the MFMA tiles of two
+// waves -- and every wave reads its fragments from LDS.  Per 32-deep step a
+// wave reads 3 TA + TB (exact coefficients) or 3 TA + 2 TB fragments for
+// 3 TA TB MFMAs, so the wider coefficient side (TB = 4) cuts LDS reads per
+// MFMA.  Rows are padded to KS + 8 halfwords to spread the 16 lanes reading
+// one fragment column over the banks.  Piece arrays are qnp x qnp with qnp
+// a multiple of 128, so no access needs a bound check; results outside
+// n x n are not stored.
 constexpr int kQapLdsMin = 1024;       // n from which these passes run
 
 // The staging of one K step, split into its global loads (into registers,
 // issued before the previous step's MFMAs so they are in flight meanwhile)
-// and its LDS stores: three data pieces of R rows and the coefficient's one
-// or two pieces of R rows, 16-byte chunks, R * KS / 8 chunks per piece.
-template <int TW, int KS>
+// and its LDS stores: three data pieces of RD rows and the coefficient's one
+// or two pieces of RC rows, 16-byte chunks, rows * KS / 8 chunks per piece.
+// KS = 32: LDS rows of 32 halfwords (64 bytes, no padding) with chunk c of
+// row r at position c ^ ((r >> 1) & 3), measured free of bank conflicts both
+// for these stores (four lanes per row) and for the fragment reads (sixteen
+// rows, one chunk each), where padded rows conflict two ways on gfx950.
+// Other KS: rows padded to KS + 8 halfwords.
+template <int KS>
+__device__ __forceinline__ int lds_off(int row, int c) {
+    if constexpr (KS == 32)
+        return row * 32 + ((c ^ (row >> 1)) & 3) * 8;
+    else
+        return row * (KS + 8) + c * 8;
+}
+template <int KS>
+constexpr int lds_row() { return KS == 32 ? KS : KS + 8; }
+
+template <int TA, int TB, int KS>
 struct LdsStage {
-    static constexpr int R = 32 * TW, CR = KS / 8, LS = KS + 8;
-    static constexpr int CP = R * CR / 256;  // chunks per thread per piece
-    static_assert(R * CR % 256 == 0, "chunks per piece must be a multiple of the block");
-    u16x8 d[3][CP], c[2][CP];
+    static constexpr int RD = 32 * TA, RC = 32 * TB, CR = KS / 8, LS = lds_row<KS>();
+    static constexpr int CPD = RD * CR / 256, CPC = RC * CR / 256;  // chunks per thread per piece
+    static_assert(RD * CR % 256 == 0 && RC * CR % 256 == 0, "chunks per piece must be a multiple of the block");
+    u16x8 d[3][CPD], c[2][CPC];
     __device__ __forceinline__ void load(const unsigned short *__restrict__ D, int64_t pp, const unsigned short *C0,
                                          const unsigned short *C1, bool two, int np, int drow0, int crow0, int k0) {
 #pragma unroll
-        for (int t = 0; t < CP; ++t) {
+        for (int t = 0; t < CPD; ++t) {
             const int ch = t * 256 + threadIdx.x, row = ch / CR, kc = (ch % CR) * 8;
-            const int64_t od = (int64_t)(drow0 + row) * np + k0 + kc, oc = (int64_t)(crow0 + row) * np + k0 + kc;
+            const int64_t od = (int64_t)(drow0 + row) * np + k0 + kc;
 #pragma unroll
             for (int p = 0; p < 3; ++p) d[p][t] = *(const u16x8 *)(D + p * pp + od);
+        }
+#pragma unroll
+        for (int t = 0; t < CPC; ++t) {
+            const int ch = t * 256 + threadIdx.x, row = ch / CR, kc = (ch % CR) * 8;
+            const int64_t oc = (int64_t)(crow0 + row) * np + k0 + kc;
             c[0][t] = *(const u16x8 *)(C0 + oc);
             if (two) c[1][t] = *(const u16x8 *)(C1 + oc);
         }
     }
     __device__ __forceinline__ void store(unsigned short *Ds, unsigned short *Cs, bool two) const {
 #pragma unroll
-        for (int t = 0; t < CP; ++t) {
+        for (int t = 0; t < CPD; ++t) {
             const int ch = t * 256 + threadIdx.x, row = ch / CR, kc = (ch % CR) * 8;
 #pragma unroll
-            for (int p = 0; p < 3; ++p) *(u16x8 *)(Ds + (p * R + row) * LS + kc) = d[p][t];
-            *(u16x8 *)(Cs + row * LS + kc) = c[0][t];
-            if (two) *(u16x8 *)(Cs + (R + row) * LS + kc) = c[1][t];
+            for (int p = 0; p < 3; ++p) *(u16x8 *)(Ds + p * RD * LS + lds_off<KS>(row, kc / 8)) = d[p][t];
+        }
+#pragma unroll
+        for (int t = 0; t < CPC; ++t) {
+            const int ch = t * 256 + threadIdx.x, row = ch / CR, kc = (ch % CR) * 8;
+            *(u16x8 *)(Cs + lds_off<KS>(row, kc / 8)) = c[0][t];
+            if (two) *(u16x8 *)(Cs + RC * LS + lds_off<KS>(row, kc / 8)) = c[1][t];
         }
     }
 };
 
 // one K step of a wave: acc[a][b] += D_a C_b over the staged pieces, in
 // 32-deep MFMA steps
-template <int TW, int KS>
-__device__ __forceinline__ void lds_step(floatx4 (&acc)[TW][TW], const unsigned short *Ds, const unsigned short *Cs,
+template <int TA, int TB, int KS>
+__device__ __forceinline__ void lds_step(floatx4 (&acc)[TA][TB], const unsigned short *Ds, const unsigned short *Cs,
                                          bool exact) {
-    constexpr int R = 32 * TW, LS = KS + 8;
+    constexpr int RD = 32 * TA, RC = 32 * TB, LS = lds_row<KS>();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
-    const int dr = (w & 1) * 16 * TW, cr = (w >> 1) * 16 * TW;
+    const int dr = (w & 1) * 16 * TA, cr = (w >> 1) * 16 * TB;
 #pragma unroll
-    for (int kq = 0; kq < KS; kq += 32) {
-        bf16x8 c0[TW], c1[TW];
+    for (int kq = 0; kq < KS / 32; ++kq) {
+        bf16x8 c0[TB], c1[TB];
 #pragma unroll
-        for (int b = 0; b < TW; ++b) {
-            const int o = (cr + 16 * b + li) * LS + kq + 8 * lg;
+        for (int b = 0; b < TB; ++b) {
+            const int o = lds_off<KS>(cr + 16 * b + li, 4 * kq + lg);
             c0[b] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + o));
-            c1[b] = exact ? c0[b] : __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + R * LS + o));
+            c1[b] = exact ? c0[b] : __builtin_bit_cast(bf16x8, *(const u16x8 *)(Cs + RC * LS + o));
         }
-        bf16x8 d[TW][3];
+        bf16x8 d[TA][3];
 #pragma unroll
-        for (int a = 0; a < TW; ++a) {
-            const int o = (dr + 16 * a + li) * LS + kq + 8 * lg;
+        for (int a = 0; a < TA; ++a) {
+            const int o = lds_off<KS>(dr + 16 * a + li, 4 * kq + lg);
 #pragma unroll
-            for (int p = 0; p < 3; ++p) d[a][p] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Ds + p * R * LS + o));
+            for (int p = 0; p < 3; ++p) d[a][p] = __builtin_bit_cast(bf16x8, *(const u16x8 *)(Ds + p * RD * LS + o));
         }
-        mma_rounds<TW>(acc, d, c0, c1, exact);
+        mma_rounds<TA, TB>(acc, d, c0, c1, exact);
     }
 }
 
-// pass 1: Y^T = X^T A^T, stored as row-major Y pieces
-template <int TW, int KS>
-__global__ __launch_bounds__(256) void k_qapbfl_p1(QapArgs g, const unsigned short *__restrict__ XP,
-                                                   unsigned short *__restrict__ YP) {
-    constexpr int R = 32 * TW;
-    __shared__ __attribute__((aligned(16))) unsigned short Ds[3 * R * (KS + 8)], Cs[2 * R * (KS + 8)];
-    const int n = g.n, np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
-    const int64_t pp = (int64_t)np * np;
-    const int j0 = blockIdx.x * R, i0 = blockIdx.y * R;
-    floatx4 acc[TW][TW];
-#pragma unroll
-    for (int a = 0; a < TW; ++a)
-#pragma unroll
-        for (int b = 0; b < TW; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const bool two = !g.a_exact;
-    LdsStage<TW, KS> st;
-    st.load(XP, pp, g.A0, g.A1, two, np, j0, i0, 0);
-    for (int k0 = 0; k0 < np; k0 += KS) {
+// The K loop of one block: acc += D rows drow0.. times C rows crow0.. over
+// all np.  NB = 1: one LDS stage, two barriers per K step; NB = 2: two
+// stages, the next step's pieces are stored into the other stage after this
+// step's MFMAs and one barrier per step separates them (a stage is
+// overwritten only after every wave has passed the barrier that ended the
+// MFMAs reading it).
+template <int TA, int TB, int KS, int NB>
+__device__ __forceinline__ void lds_gemm(floatx4 (&acc)[TA][TB], const unsigned short *__restrict__ D, int64_t pp,
+                                         const unsigned short *C0, const unsigned short *C1, bool exact, int np,
+                                         int drow0, int crow0, unsigned short *Ds, unsigned short *Cs) {
+    constexpr int DSZ = 3 * 32 * TA * lds_row<KS>(), CSZ = 2 * 32 * TB * lds_row<KS>();
+    const bool two = !exact;
+    LdsStage<TA, TB, KS> st;
+    st.load(D, pp, C0, C1, two, np, drow0, crow0, 0);
+    if constexpr (NB == 1) {
+        for (int k0 = 0; k0 < np; k0 += KS) {
+            st.store(Ds, Cs, two);
+            __syncthreads();
+            if (k0 + KS < np) st.load(D, pp, C0, C1, two, np, drow0, crow0, k0 + KS);
+            lds_step<TA, TB, KS>(acc, Ds, Cs, exact);
+            __syncthreads();
+        }
+    } else {
         st.store(Ds, Cs, two);
         __syncthreads();
-        if (k0 + KS < np) st.load(XP, pp, g.A0, g.A1, two, np, j0, i0, k0 + KS);
-        lds_step<TW, KS>(acc, Ds, Cs, g.a_exact);
-        __syncthreads();
+        int cur = 0;
+        for (int k0 = 0; k0 < np; k0 += KS) {
+            const bool more = k0 + KS < np;
+            if (more) st.load(D, pp, C0, C1, two, np, drow0, crow0, k0 + KS);
+            lds_step<TA, TB, KS>(acc, Ds + cur * DSZ, Cs + cur * CSZ, exact);
+            if (more) st.store(Ds + (cur ^ 1) * DSZ, Cs + (cur ^ 1) * CSZ, two);
+            __syncthreads();
+            cur ^= 1;
+        }
     }
+}
+
+// pass 1: Y^T = X^T A^T, stored as row-major Y pieces; block (x, y) owns
+// columns j0.. of Y (data rows, 32 TA) and rows i0.. (coefficient rows, 32 TB)
+template <int TA, int TB, int KS, int NB>
+__global__ __launch_bounds__(256) void k_qapbfl_p1(QapArgs g, const unsigned short *__restrict__ XP,
+                                                   unsigned short *__restrict__ YP) {
+    constexpr int RD = 32 * TA, RC = 32 * TB;
+    __shared__ __attribute__((aligned(16))) unsigned short Ds[NB * 3 * RD * lds_row<KS>()],
+        Cs[NB * 2 * RC * lds_row<KS>()];
+    const int np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
+    const int64_t pp = (int64_t)np * np;
+    const int j0 = blockIdx.x * RD, i0 = blockIdx.y * RC;
+    floatx4 acc[TA][TB];
+#pragma unroll
+    for (int a = 0; a < TA; ++a)
+#pragma unroll
+        for (int b = 0; b < TB; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    lds_gemm<TA, TB, KS, NB>(acc, XP, pp, g.A0, g.A1, g.a_exact, np, j0, i0, Ds, Cs);
     // four consecutive j per lane: one 8-byte store per piece; rows and
     // columns past n are exact zeros, so the padded tile is stored whole
-    const int jr = j0 + (w & 1) * 16 * TW, ic = i0 + (w >> 1) * 16 * TW;
+    const int jr = j0 + (w & 1) * 16 * TA, ic = i0 + (w >> 1) * 16 * TB;
 #pragma unroll
-    for (int a = 0; a < TW; ++a)
+    for (int a = 0; a < TA; ++a)
 #pragma unroll
-        for (int b = 0; b < TW; ++b) {
+        for (int b = 0; b < TB; ++b) {
             const float v[4] = {acc[a][b][0], acc[a][b][1], acc[a][b][2], acc[a][b][3]};
             store_pieces4(YP, pp, (int64_t)(ic + 16 * b + li) * np + jr + 16 * a + 4 * lg, v);
         }
-    (void)n;
 }
 
-// pass 2: R = Y B, then - s_i X_ij - X_ij t_j and the epilogue
-template <int TW, int KS, class IT, class Epi>
+// pass 2: R = Y B, then - s_i X_ij - X_ij t_j and the epilogue; block (x, y)
+// owns rows i0.. (data rows, 32 TA) and columns j0.. (coefficient rows, 32 TB)
+template <int TA, int TB, int KS, int NB, class IT, class Epi>
 __global__ __launch_bounds__(256) void k_qapbfl_p2(QapArgs g, const IT *__restrict__ x,
                                                    const unsigned short *__restrict__ YP, Epi epi) {
-    constexpr int R = 32 * TW;
-    __shared__ __attribute__((aligned(16))) unsigned short Ds[3 * R * (KS + 8)], Cs[2 * R * (KS + 8)];
+    constexpr int RD = 32 * TA, RC = 32 * TB;
+    __shared__ __attribute__((aligned(16))) unsigned short Ds[NB * 3 * RD * lds_row<KS>()],
+        Cs[NB * 2 * RC * lds_row<KS>()];
     const int n = g.n, np = g.np, lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lg = lane >> 4;
     const int64_t pp = (int64_t)np * np;
-    const int i0 = blockIdx.x * R, j0 = blockIdx.y * R;
-    floatx4 acc[TW][TW];
+    const int i0 = blockIdx.x * RD, j0 = blockIdx.y * RC;
+    floatx4 acc[TA][TB];
 #pragma unroll
-    for (int a = 0; a < TW; ++a)
+    for (int a = 0; a < TA; ++a)
 #pragma unroll
-        for (int b = 0; b < TW; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const bool two = !g.b_exact;
-    LdsStage<TW, KS> st;
-    st.load(YP, pp, g.B0, g.B1, two, np, i0, j0, 0);
-    for (int k0 = 0; k0 < np; k0 += KS) {
-        st.store(Ds, Cs, two);
-        __syncthreads();
-        if (k0 + KS < np) st.load(YP, pp, g.B0, g.B1, two, np, i0, j0, k0 + KS);
-        lds_step<TW, KS>(acc, Ds, Cs, g.b_exact);
-        __syncthreads();
-    }
-    const int ir = i0 + (w & 1) * 16 * TW, jc = j0 + (w >> 1) * 16 * TW;
+        for (int b = 0; b < TB; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    lds_gemm<TA, TB, KS, NB>(acc, YP, pp, g.B0, g.B1, g.b_exact, np, i0, j0, Ds, Cs);
+    const int ir = i0 + (w & 1) * 16 * TA, jc = j0 + (w >> 1) * 16 * TB;
 #pragma unroll
-    for (int a = 0; a < TW; ++a)
+    for (int a = 0; a < TA; ++a)
 #pragma unroll
-        for (int b = 0; b < TW; ++b) {
+        for (int b = 0; b < TB; ++b) {
             const int j = jc + 16 * b + li;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -1184,12 +1232,23 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
         launch_k(k_qap_split<IT>, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, g.n, g.np, X, q->qxp.p,
                  (unsigned short *)nullptr);
         HPRQP_HIP(hipGetLastError());
-        // 64-wide blocks (measured faster than 128-wide up to n = 4096)
-        const int nb = g.np / 64;
-        launch_k(k_qapbfl_p1<2, 64>, dim3(nb, nb), dim3(256), 0, s, g, (const unsigned short *)q->qxp.p, q->qyp.p);
-        HPRQP_HIP(hipGetLastError());
-        launch_k(k_qapbfl_p2<2, 64, IT, Epi>, dim3(nb, nb), dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
-                 epi);
+        const unsigned short *xp = q->qxp.p;
+        if ((int64_t)(g.np / 64) * (g.np / 128) >= 512) {
+            // 64 x 128 blocks, each wave 2 x 4 tiles: fewer LDS reads per
+            // MFMA (measured 13 % faster than 64 x 64 at n = 2048), used
+            // once they make two blocks per CU
+            const dim3 grid(g.np / 64, g.np / 128);
+            launch_k(k_qapbfl_p1<2, 4, 32, 1>, grid, dim3(256), 0, s, g, xp, q->qyp.p);
+            HPRQP_HIP(hipGetLastError());
+            launch_k(k_qapbfl_p2<2, 4, 32, 1, IT, Epi>, grid, dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
+                     epi);
+        } else {
+            const dim3 grid(g.np / 64, g.np / 64);
+            launch_k(k_qapbfl_p1<2, 2, 64, 1>, grid, dim3(256), 0, s, g, xp, q->qyp.p);
+            HPRQP_HIP(hipGetLastError());
+            launch_k(k_qapbfl_p2<2, 2, 64, 1, IT, Epi>, grid, dim3(256), 0, s, g, X, (const unsigned short *)q->qyp.p,
+                     epi);
+        }
         HPRQP_HIP(hipGetLastError());
         return;
     }

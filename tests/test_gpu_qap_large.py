@@ -5,9 +5,10 @@ operator.
 
 Sizes cover every large-n path: 256 <= n < 1024 (register-blocked passes,
 four waves splitting K), n >= 1024 with diagonal S and T (LDS-tiled
-split-BF16 passes with 64-wide blocks, and 128-wide at n = 2048), and a full
+split-BF16 passes: 64 x 64 blocks at n = 1100, 64 x 128 blocks with swizzled
+LDS rows at n = 2048 and 2100), and a full
 S, T at n = 1100 (which stays on the register-blocked passes).  Sizes are
-ragged against the block and MFMA tiles (300, 400, 517, 1100) as well as
+ragged against the block and MFMA tiles (300, 400, 517, 1100, 2100) as well as
 aligned (256, 448, 512, 2048); coefficients are Gaussian (two-piece BF16 split of both
 operands) or integer flows and distances (BF16-exact, three-piece data
 split)."""
@@ -27,7 +28,7 @@ def _cases():
 
     rng = np.random.default_rng(21)
     out = []
-    for nq, full in ((300, False), (400, True), (517, False), (1100, False), (1100, True)):
+    for nq, full in ((300, False), (400, True), (517, False), (1100, False), (1100, True), (2100, False)):
         A = rng.standard_normal((nq, nq))
         B = rng.standard_normal((nq, nq))
         if full:
