@@ -625,54 +625,76 @@ struct Solver {
         const double *v;
         __device__ void operator()(int64_t i, double *s, double *) const { s[0] += v[i] * v[i]; }
     };
-    struct Dot2 {
+
+    struct Dot2Sq {
         const double *a, *b;
-        __device__ void operator()(int64_t i, double *s, double *) const { s[0] += a[i] * b[i]; }
+        __device__ void operator()(int64_t i, double *s, double *) const {
+            const double ai = a[i];
+            s[0] += ai * b[i];
+            s[1] += ai * ai;
+        }
     };
 
     // which = 0: A A^T (size m); which = 1: Q (size n).  FP64 throughout.
+    // The reference's iteration (src/utils.jl:1357-1376): q = z / |z|,
+    // z' = A A^T q (or Q q), lambda = q.z', err = |z' - lambda q| / (|z'| +
+    // lambda), stop at the first err < tol.  Here, per iteration, the A^T
+    // product reads z and scales its rows by 1/|z| (q is not stored; for Q
+    // it is), one reduction gives z'.z (or z'.q) and |z'|^2, and
+    // |z' - lambda q|^2 is expanded as |z'|^2 - 2 lambda q.z' + lambda^2
+    // |q|^2 = |z'|^2 - lambda^2 (2 - |q|^2), with |q|^2 = |z|^2 / |z|^2 as
+    // rounded; |z'|^2 is the next iteration's |z|^2.  Five launches per
+    // iteration instead of eight (no normalising pass, one reduction).
     double power_iteration(int which, int64_t max_it, double tol, int64_t *its) {
         const int64_t len = which == 0 ? m : n;
         std::vector<double> z0(len);
         randn_stream(1, len, z0.data());
         for (auto &v : z0) v += 1e-8;
-        DevVec<double> z(len), q(len), ATq(which == 0 ? n : 0), qd_tmp(which == 1 ? n : 0);
-        HPRQP_HIP(hipMemcpyAsync(z.p, z0.data(), sizeof(double) * len, hipMemcpyHostToDevice, s));
-        // device scalars: 0 |z|^2, 1 lambda, 2 |q|^2, 3 |z|^2, 4 done, 5 lambda
-        // at the stop, 6 stop iteration, 7 error.  Every kernel after the
-        // converging iteration is a no-op, so the result is the reference's
-        // (first iteration with err < tol) while the host polls rarely.
+        DevVec<double> za(len), zb(len), q(which == 1 ? len : 0), ATq(which == 0 ? n : 0);
+        HPRQP_HIP(hipMemcpyAsync(za.p, z0.data(), sizeof(double) * len, hipMemcpyHostToDevice, s));
+        // device scalars: 0 |z|^2, 1 1/|z|, 2 z'.z (A A^T) or z'.q (Q),
+        // 3 |z'|^2, 4 done, 5 lambda at the stop, 6 stop iteration, 7 error.
+        // Nothing is recorded after the converging iteration, so the result
+        // is the reference's (first iteration with err < tol) while the host
+        // polls rarely.
         double *d = red.out.p + 32;
         HPRQP_HIP(hipMemsetAsync(d, 0, sizeof(double) * 8, s));
+        red.enqueue<1, 0>(len, SumSq{za.p}, 32);
+        k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { d[1] = 1.0 / sqrt(d[0]); });
+        HPRQP_HIP(hipGetLastError());
         constexpr int kPoll = 16;
         double lambda_max = 1.0;
+        double *zo = za.p, *zn = zb.p;
         for (int64_t it = 1; it <= max_it; ++it) {
-            red.enqueue<1, 0>(len, SumSq{z.p}, 32);
-            {
-                double *qq = q.p;
-                const double *zz = z.p;
-                each(s, len, [=] __device__(int64_t i) {
-                    if (d[4] == 0.0) qq[i] = zz[i] / sqrt(d[0]);
-                });
-            }
             if (which == 0) {
-                csr_apply<double, double>(s, S.AT, (const double *)q.p, EpiStore<double>{ATq.p});
-                csr_apply<double, double>(s, S.A, (const double *)ATq.p, EpiStore<double>{z.p});
+                csr_apply<double, double>(s, S.AT, (const double *)zo, EpiStoreScaled<double>{ATq.p, d + 1});
+                csr_apply<double, double>(s, S.A, (const double *)ATq.p, EpiStore<double>{zn});
+                red.enqueue<2, 0>(len, Dot2Sq{zn, zo}, 34);
             } else {
-                qop_fp64(q.p, z.p);
+                double *qq = q.p;
+                const double *zz = zo;
+                each(s, len, [=] __device__(int64_t i) { qq[i] = zz[i] * d[1]; });
+                qop_fp64(q.p, zn);
+                red.enqueue<2, 0>(len, Dot2Sq{zn, q.p}, 34);
             }
-            red.enqueue<1, 0>(len, Dot2{q.p, z.p}, 33);
-            red.enqueue<2, 0>(len, PowerResid{q.p, z.p, d}, 34);
             const double itd = (double)it, tl = tol;
+            const bool scaled = which == 0;
             k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) {
-                if (d[4] != 0.0) return;
-                const double err = sqrt(d[2]) / (sqrt(d[3]) + d[1]);
-                d[5] = d[1];
-                d[6] = itd;
-                d[7] = err;
-                if (err < tl) d[4] = 1.0;
+                const double lam = scaled ? d[2] * d[1] : d[2];
+                const double qn = d[0] * d[1] * d[1];
+                const double e2 = fmax(d[3] - lam * lam * (2.0 - qn), 0.0);
+                if (d[4] == 0.0) {
+                    const double err = sqrt(e2) / (sqrt(d[3]) + lam);
+                    d[5] = lam;
+                    d[6] = itd;
+                    d[7] = err;
+                    if (err < tl) d[4] = 1.0;
+                }
+                d[0] = d[3];
+                d[1] = 1.0 / sqrt(d[3]);
             });
             HPRQP_HIP(hipGetLastError());
+            std::swap(zo, zn);
             if (it % kPoll == 0 || it == max_it) {
                 HPRQP_HIP(hipMemcpyAsync(red.host, d, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
                 HPRQP_HIP(hipStreamSynchronize(s));
@@ -684,20 +706,6 @@ struct Solver {
         if (p->verbose) std::printf("Power iteration did not converge within %lld iterations\n", (long long)max_it);
         return lambda_max;
     }
-    // q = z - lambda q (skipped after convergence), and |q|^2, |z|^2
-    struct PowerResid {
-        double *q;
-        const double *z, *d;
-        __device__ void operator()(int64_t i, double *s, double *) const {
-            double qi = q[i];
-            if (d[4] == 0.0) {
-                qi = z[i] - d[1] * qi;
-                q[i] = qi;
-            }
-            s[0] += qi * qi;
-            s[1] += z[i] * z[i];
-        }
-    };
 
     // FP64 operator apply on FP64 vectors (power method)
     void qop_fp64(const double *xin, double *out) {

@@ -47,6 +47,12 @@ def _check(res, ref, obj_tol=1e-6):
     # the power method is seeded identically: lambda_max agrees to its tolerance
     assert abs(res.lambda_max_A - ref.lambda_max_A) <= 1e-3 * max(1.0, ref.lambda_max_A)
     assert abs(res.lambda_max_Q - ref.lambda_max_Q) <= 1e-3 * max(1.0, ref.lambda_max_Q)
+    # and it stops on the same iteration (one either way for an error within
+    # rounding of the tolerance)
+    print(f"power iterations: gpu A {res.power_iter_A} Q {res.power_iter_Q}; "
+          f"cpu A {ref.power_iter_A} Q {ref.power_iter_Q}")
+    assert abs(res.power_iter_A - ref.power_iter_A) <= 1, (res.power_iter_A, ref.power_iter_A)
+    assert abs(res.power_iter_Q - ref.power_iter_Q) <= 1, (res.power_iter_Q, ref.power_iter_Q)
 
 
 def test_model_mps():
