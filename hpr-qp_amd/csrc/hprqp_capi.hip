@@ -98,6 +98,9 @@ void upload_csr(hipStream_t s, const HostCsr &h, DevCsr &d) {
         HPRQP_HIP(hipMemcpyAsync(d.v.p, h.v.data(), sizeof(double) * d.nnz, hipMemcpyHostToDevice, s));
     }
     d.lanes = default_lanes(h);
+    d.cseg.release();  // a new pattern: chunk segments are rebuilt on use
+    d.cpart.release();
+    d.nchunk = 0;
     HPRQP_HIP(hipStreamSynchronize(s));
 }
 

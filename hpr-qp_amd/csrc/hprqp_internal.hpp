@@ -82,6 +82,11 @@ struct DevCsr {
     DevVec<double> v;
     DevVec<float> v32;
     int lanes = 1;
+    // column-chunked product (built on its first use): per-chunk row segment
+    // starts, (nchunk + 1) x nrow, and the per-chunk sums, nchunk x nrow
+    mutable DevVec<int32_t> cseg;
+    mutable DevVec<double> cpart;
+    mutable int32_t nchunk = 0;
 };
 
 // Host-side CSR (sorted column indices), used while building models.

@@ -100,6 +100,107 @@ __global__ __launch_bounds__(kBlock) void k_csr_rowblock(int32_t nrows, const in
     if (threadIdx.x == 0) epi(row, (part[0] + part[1]) + (part[2] + part[3]));
 }
 
+// Column-chunked row product for long rows over many columns (the LASSO data
+// matrix: 3,308 rows of ~1,400 nonzeros over 150,360 columns).  There, one
+// block per row gathers x[col] at random from L2, one request per nonzero.
+// Here block (g, c) stages x[c W .. c W + W) in LDS and sums the nonzeros of
+// rows 128 g .. 128 g + 127 that fall in that chunk; a 16-lane group owns a
+// row, and the first 64 nonzeros of four rows are loaded together.  The
+// per-chunk sums go to part[c * nrow + r], and k_chunk_sum adds them in a
+// fixed order, so the result does not depend on scheduling.  seg[c * nrow +
+// r] is the first nonzero of row r with column >= c W (c = 0 .. nchunk).
+constexpr int kChunkW = 4096;       // columns per chunk
+constexpr int kChunkRows = 128;     // rows per block (32 per wave)
+constexpr int kLanesChunked = 1 << 20;  // row-product shape code of this form
+
+static __global__ __launch_bounds__(kBlock) void k_chunk_seg(int32_t nrow, int32_t nchunk, const int32_t *__restrict__ rp,
+                                                             const int32_t *__restrict__ ci, int32_t *__restrict__ seg) {
+    const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= (int64_t)(nchunk + 1) * nrow) return;
+    const int32_t c = (int32_t)(idx / nrow), r = (int32_t)(idx % nrow);
+    // first entry with column >= c W; at c = nchunk (c W >= ncol) that is
+    // the row's end
+    int32_t lo = rp[r], hi = rp[r + 1];
+    const int64_t col = (int64_t)c * kChunkW;
+    while (lo < hi) {
+        const int32_t mid = lo + (hi - lo) / 2;
+        if ((int64_t)ci[mid] < col)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    seg[idx] = lo;
+}
+
+template <class MT, class XT, class AT>
+__global__ __launch_bounds__(kBlock) void k_csr_chunk(int32_t nrow, int32_t ncol, const int32_t *__restrict__ seg,
+                                                      const int32_t *__restrict__ ci, const MT *__restrict__ v,
+                                                      const XT *__restrict__ x, double *__restrict__ part) {
+    __shared__ XT xs[kChunkW];
+    const int32_t c = blockIdx.y, c0 = c * kChunkW, cw = min(kChunkW, ncol - c0);
+    for (int i = threadIdx.x; i < cw; i += kBlock) xs[i] = x[c0 + i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, sub = lane >> 4, l16 = lane & 15;
+    for (int32_t rg = blockIdx.x * kChunkRows + 16 * w; rg < blockIdx.x * kChunkRows + kChunkRows; rg += 64) {
+        const int32_t rbase = rg + sub;  // rows rbase + 4 p, p < 4
+        int32_t sg[4], eg[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int32_t r = rbase + 4 * p;
+            sg[p] = r < nrow ? seg[(int64_t)c * nrow + r] : 0;
+            eg[p] = r < nrow ? seg[(int64_t)(c + 1) * nrow + r] : 0;
+        }
+        AT acc[4];
+        {
+            // the first 64 nonzeros of the four rows, all loads issued
+            // together; lanes past a segment's end load entry 0 and add nothing
+            MT vv[4][4];
+            int32_t cc[4][4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int32_t q = sg[p] + l16 + 16 * u, at = q < eg[p] ? q : 0;
+                    vv[p][u] = v[at];
+                    cc[p][u] = ci[at];
+                }
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                acc[p] = (AT)0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool ok = sg[p] + l16 + 16 * u < eg[p];
+                    const AT pr = (AT)vv[p][u] * (AT)xs[ok ? cc[p][u] - c0 : 0];
+                    acc[p] += ok ? pr : (AT)0;
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            for (int32_t q = sg[p] + 64 + l16; q < eg[p]; q += 16) acc[p] += (AT)v[q] * (AT)xs[ci[q] - c0];
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) acc[p] += __shfl_xor(acc[p], off);
+            const int32_t r = rbase + 4 * p;
+            if (l16 == 0 && r < nrow) part[(int64_t)c * nrow + r] = (double)acc[p];
+        }
+    }
+}
+
+// 16 lanes per row: lane l adds chunks l, l + 16, ..., then the 16 sums are
+// combined by a fixed shuffle tree
+template <class AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_chunk_sum(int32_t nrow, int32_t nchunk, const double *__restrict__ part,
+                                                      Epi epi) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int32_t r = (int32_t)(t >> 4), l16 = (int32_t)(t & 15);
+    double s = 0.0;
+    if (r < nrow)
+        for (int32_t c = l16; c < nchunk; c += 16) s += part[(int64_t)c * nrow + r];
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (l16 == 0 && r < nrow) epi(r, (AT)s);
+}
+
 // CSR-stream row product for short rows: a block owns RB consecutive rows;
 // its nonzeros are read once, fully coalesced, and their products staged in
 // LDS; then thread t sums row r0+t in order and runs the epilogue, so every

@@ -72,9 +72,35 @@ const double *csr_vals<double>(const DevCsr &a) { return a.v.p; }
 template <>
 const float *csr_vals<float>(const DevCsr &a) { return a.v32.p; }
 
+// the column-chunked form: segment starts built on first use (the pattern
+// is fixed for the matrix's life), then the chunk sums and their total
+template <class AT, class MT, class XT, class Epi>
+static void launch_chunked(hipStream_t s, const DevCsr &a, const MT *v, const XT *x, const Epi &epi) {
+    if (a.nrow <= 0) return;
+    if (a.nchunk == 0) {
+        const int32_t nc = (int32_t)((a.ncol + kChunkW - 1) / kChunkW);
+        a.cseg.alloc((int64_t)(nc + 1) * a.nrow);
+        a.cpart.alloc((int64_t)nc * a.nrow);
+        const int64_t cnt = (int64_t)(nc + 1) * a.nrow;
+        k_chunk_seg<<<(unsigned)((cnt + kBlock - 1) / kBlock), kBlock, 0, s>>>(a.nrow, nc, a.rp.p, a.ci.p,
+                                                                               a.cseg.p);
+        HPRQP_HIP(hipGetLastError());
+        a.nchunk = nc;
+    }
+    launch_k(k_csr_chunk<MT, XT, AT>, dim3((a.nrow + kChunkRows - 1) / kChunkRows, a.nchunk), dim3(kBlock), 0, s,
+             a.nrow, a.ncol, (const int32_t *)a.cseg.p, a.ci.p, v, x, a.cpart.p);
+    HPRQP_HIP(hipGetLastError());
+    launch_k(k_chunk_sum<AT, Epi>, dim3(nblk((int64_t)a.nrow * 16)), dim3(kBlock), 0, s, a.nrow, a.nchunk,
+             (const double *)a.cpart.p, epi);
+    HPRQP_HIP(hipGetLastError());
+}
+
 template <class AT, class MT, class XT, class Epi>
 static void csr_apply(hipStream_t s, const DevCsr &a, const XT *x, const Epi &epi) {
-    launch_csr<AT>(s, a.lanes, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), x, epi);
+    if (a.lanes == kLanesChunked)
+        launch_chunked<AT>(s, a, csr_vals<MT>(a), x, epi);
+    else
+        launch_csr<AT>(s, a.lanes, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), x, epi);
 }
 
 template <class F>
@@ -1707,7 +1733,7 @@ struct EventTimer {
 // the iteration's own types: matrix values MT, vectors and accumulation VT
 // (FP64 / FP64 plain, FP32 / FP64 mixed, FP32 / FP32 fp32).
 template <class MT, class VT>
-static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
+static int tune_lanes(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
     DevVec<VT> x(std::max<int64_t>(1, a.ncol)), y(std::max<int64_t>(1, a.nrow));
     {
         VT *xp = x.p;
@@ -1720,18 +1746,27 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps) {
         if (mean * rb <= 0.75 * kStreamCap) cand.push_back(-rb);
     for (int L = 1; L <= 64; L *= 2) cand.push_back(L);
     if (mean >= 64.0) cand.push_back(kBlock);  // one block per row
+    // long rows over many columns: x staged in LDS chunk by chunk
+    if (mean >= 256.0 && a.ncol >= 4 * (int64_t)kChunkW) cand.push_back(kLanesChunked);
     auto time_of = [&](int L) {
         return tm.avg_ms(reps, [&] {
-            launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
+            if (L == kLanesChunked)
+                launch_chunked<VT>(s, a, csr_vals<MT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
+            else
+                launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
         });
     };
     // the heuristic shape stays unless another is clearly faster (10 %), so
     // shapes within timing noise of each other do not flip from run to run
     int best = a.lanes;
     double best_ms = time_of(best);
+    if (verbose)
+        std::printf("row-product autotune %d x %d (%lld nnz): shape %d %.4f ms (heuristic)\n", a.nrow, a.ncol,
+                    (long long)a.nnz, best, best_ms);
     for (int L : cand) {
         if (L == a.lanes) continue;
         const double ms = time_of(L);
+        if (verbose) std::printf("row-product autotune   shape %d %.4f ms\n", L, ms);
         if (ms < best_ms * 0.90) {
             best_ms = ms;
             best = L;
@@ -1749,9 +1784,9 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
         const int L = p->spmv_lanes;
         const bool vec = L > 0 && (L & (L - 1)) == 0 && (L <= 64 || L == kBlock);
         const bool stream = L == -64 || L == -128 || L == -256;
-        HPRQP_REQUIRE(vec || stream, HPRQP_E_ARG,
-                      "spmv_lanes must be 0, a power of two <= 64, 256 (a block per row) or -64/-128/-256 "
-                      "(rows per block)");
+        HPRQP_REQUIRE(vec || stream || L == kLanesChunked, HPRQP_E_ARG,
+                      "spmv_lanes must be 0, a power of two <= 64, 256 (a block per row), -64/-128/-256 "
+                      "(rows per block) or 1048576 (column-chunked)");
         S.Q.lanes = S.A.lanes = S.AT.lanes = p->spmv_lanes;
         if (S.qkind == QOP_LASSO) S.qop->la.lanes = S.qop->lat.lanes = p->spmv_lanes;
         return;
@@ -1771,12 +1806,12 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
         if (tuned == 0) {
             if (p->precision == HPRQP_PREC_MIXED) {
                 ensure_v32(a);
-                tuned = tune_lanes<float, double>(s, a, 20);
+                tuned = tune_lanes<float, double>(s, a, 20, p->verbose);
             } else if (p->precision == HPRQP_PREC_FP32) {
                 ensure_v32(a);
-                tuned = tune_lanes<float, float>(s, a, 20);
+                tuned = tune_lanes<float, float>(s, a, 20, p->verbose);
             } else {
-                tuned = tune_lanes<double, double>(s, a, 20);
+                tuned = tune_lanes<double, double>(s, a, 20, p->verbose);
             }
         }
         a.lanes = tuned;

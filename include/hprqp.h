@@ -92,7 +92,8 @@ typedef struct {
     int32_t spmv_lanes;         /* 0 = autotune; 1..64 (power of two) = lanes
                                    per row; 256 = one block per row;
                                    -64/-128/-256 = CSR-stream with that many
-                                   rows per block */
+                                   rows per block; 1048576 = column-chunked
+                                   (x staged in LDS 4,096 columns at a time) */
     int32_t time_kernels;       /* 1 = time the dominant fused kernel with
                                    HIP events during the solve (result.kernel_ms);
                                    one launch in eight carries the events */
@@ -139,7 +140,9 @@ typedef struct {
     int64_t kernel_calls; /* launches averaged into kernel_ms (every eighth) */
     int32_t lanes_Q;      /* row-product shape used for Q, A, A^T (spmv_lanes */
     int32_t lanes_A;      /* codes: 1..64 lanes per row, 256 a block per row, */
-    int32_t lanes_AT;     /* -64/-128/-256 CSR-stream rows per block) */
+    int32_t lanes_AT;     /* -64/-128/-256 CSR-stream rows per block, 1048576
+                             column-chunked: x staged in LDS 4,096 columns at a
+                             time, for long rows over many columns) */
     int32_t lanes_pad;
 } hprqp_result;
 

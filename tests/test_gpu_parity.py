@@ -149,6 +149,24 @@ def test_lasso_small():
     _check(res, ref)
 
 
+def test_lasso_long_rows_column_chunked():
+    """Rows of ~500 nonzeros over 30,000 columns (eight column chunks, the
+    last ragged; log-normal row lengths, so some rows pass 64 nonzeros in a
+    chunk and some are empty there) with the column-chunked row product forced
+    for every matrix: parity with the oracle, and the same solve as the
+    autotuned shapes."""
+    from hprqp import HPRQP_parameters, instances, optimize
+
+    model = instances.lasso(120, 30_000, mean_nnz=500, seed=3)
+    res, ref = _solve_both(model, problem_type="LASSO", spmv_lanes=1 << 20)
+    _check(res, ref)
+    assert res.lanes[0] == 1 << 20
+    auto = optimize(model, HPRQP_parameters(verbose=False, problem_type="LASSO"))
+    rel = abs(res.primal_obj - auto.primal_obj) / (1.0 + abs(auto.primal_obj))
+    print(f"autotuned shapes {auto.lanes}: iter={auto.iter} rel obj {rel:.2e}")
+    assert rel < 1e-9 and abs(res.iter - auto.iter) <= 10
+
+
 def test_qap_esc64a():
     from hprqp import build_from_mat
 
