@@ -1742,7 +1742,7 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
     EventTimer tm(s);
     const double mean = a.nrow > 0 ? (double)a.nnz / (double)a.nrow : 0.0;
     std::vector<int> cand;
-    for (int rb : {256, 128, 64})  // CSR-stream when a block's nonzeros fit in LDS
+    for (int rb : {256, 128, 64, 32, 16})  // CSR-stream when a block's nonzeros fit in LDS
         if (mean * rb <= 0.75 * kStreamCap) cand.push_back(-rb);
     for (int L = 1; L <= 64; L *= 2) cand.push_back(L);
     if (mean >= 64.0) cand.push_back(kBlock);  // one block per row

@@ -140,7 +140,7 @@ typedef struct {
     int64_t kernel_calls; /* launches averaged into kernel_ms (every eighth) */
     int32_t lanes_Q;      /* row-product shape used for Q, A, A^T (spmv_lanes */
     int32_t lanes_A;      /* codes: 1..64 lanes per row, 256 a block per row, */
-    int32_t lanes_AT;     /* -64/-128/-256 CSR-stream rows per block, 1048576
+    int32_t lanes_AT;     /* -16 .. -256 CSR-stream rows per block, 1048576
                              column-chunked: x staged in LDS 4,096 columns at a
                              time, for long rows over many columns) */
     int32_t lanes_pad;
