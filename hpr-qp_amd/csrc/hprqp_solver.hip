@@ -1728,17 +1728,37 @@ struct EventTimer {
     }
 };
 
-// Rows-per-group autotuner for one matrix: times the plain product for every
+// Epilogue the autotuner times the shapes with: like the iteration's fused
+// half-steps it reads four row vectors and writes two, so a shape whose
+// epilogue runs on one lane of each row group (its vector accesses poorly
+// coalesced) pays for that as it does in the solve
+template <class VT>
+struct EpiTune {
+    VT *y, *z;
+    const VT *a, *b, *c, *d;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT acc) const {
+        const double v = (double)acc;
+        y[i] = (VT)(v + (double)a[i] * (double)b[i]);
+        z[i] = (VT)(v * (double)c[i] - (double)d[i]);
+    }
+};
+
+// Rows-per-group autotuner for one matrix: times the product for every
 // lane count (replaces determine_spmv_mode's cuSPARSE-vs-custom choice), with
 // the iteration's own types: matrix values MT, vectors and accumulation VT
-// (FP64 / FP64 plain, FP32 / FP64 mixed, FP32 / FP32 fp32).
+// (FP64 / FP64 plain, FP32 / FP64 mixed, FP32 / FP32 fp32), and an epilogue
+// with the row-vector traffic of the fused half-steps (EpiTune).
 template <class MT, class VT>
 static int tune_lanes(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
-    DevVec<VT> x(std::max<int64_t>(1, a.ncol)), y(std::max<int64_t>(1, a.nrow));
+    const int64_t nr = std::max<int64_t>(1, a.nrow);
+    DevVec<VT> x(std::max<int64_t>(1, a.ncol)), y(nr), z(nr), rv(4 * nr);
     {
-        VT *xp = x.p;
+        VT *xp = x.p, *rp = rv.p;
         each(s, a.ncol, [=] __device__(int64_t i) { xp[i] = (VT)1; });
+        each(s, 4 * nr, [=] __device__(int64_t i) { rp[i] = (VT)0.5; });
     }
+    const EpiTune<VT> epi{y.p, z.p, rv.p, rv.p + nr, rv.p + 2 * nr, rv.p + 3 * nr};
     EventTimer tm(s);
     const double mean = a.nrow > 0 ? (double)a.nnz / (double)a.nrow : 0.0;
     std::vector<int> cand;
@@ -1751,9 +1771,9 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
     auto time_of = [&](int L) {
         return tm.avg_ms(reps, [&] {
             if (L == kLanesChunked)
-                launch_chunked<VT>(s, a, csr_vals<MT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
+                launch_chunked<VT>(s, a, csr_vals<MT>(a), (const VT *)x.p, epi);
             else
-                launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), (const VT *)x.p, EpiStore<VT>{y.p});
+                launch_csr<VT>(s, L, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), (const VT *)x.p, epi);
         });
     };
     // the heuristic shape stays unless another is clearly faster (10 %), so
