@@ -113,12 +113,13 @@ def test_every_spmv_kernel_shape_gives_the_same_solution(lanes):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp32"])
-@pytest.mark.parametrize("case", ["grid", "random", "aug3dcqp", "lasso"])
+@pytest.mark.parametrize("case", ["grid", "random", "aug3dcqp", "lasso", "lasso_chunked"])
 def test_reduced_precision_reaches_fp64_accuracy(precision, case):
     """Mixed (FP32 matrices and data, FP64 iterates) and FP32 (+FP64 polish)
     forms: the KKT residual is evaluated in FP64 against the FP64 data, so
     the same 1e-6 relative KKT bar applies; the objective must agree with
-    the FP64 oracle within 1e-6 relative."""
+    the FP64 oracle within 1e-6 relative.  lasso_chunked forces the
+    column-chunked row product (FP32 values on FP64 vectors, and all-FP32)."""
     from hprqp import HPRQP_parameters, build_from_mps, instances, optimize
     from oracle import oracle
 
@@ -129,10 +130,14 @@ def test_reduced_precision_reaches_fp64_accuracy(precision, case):
         model = instances.random_sparse_qp(2000, 1000, seed=4)
     elif case == "aug3dcqp":
         model = build_from_mps(fixture_path("AUG3DCQP.mps"), verbose=False)
-    else:
+    elif case == "lasso":
         model = instances.lasso(200, 4000, mean_nnz=60, seed=1)
         ptype = "LASSO"
-    res = optimize(model, HPRQP_parameters(verbose=False, precision=precision, problem_type=ptype))
+    else:
+        model = instances.lasso(120, 30_000, mean_nnz=500, seed=3)
+        ptype = "LASSO"
+    lanes = 1 << 20 if case == "lasso_chunked" else 0
+    res = optimize(model, HPRQP_parameters(verbose=False, precision=precision, problem_type=ptype, spmv_lanes=lanes))
     ref = oracle.solve(model, problem_type=ptype)
     rel = abs(res.primal_obj - ref.primal_obj) / (1.0 + abs(ref.primal_obj))
     print(f"{precision} {case}: {res.status} iter={res.iter} polish={res.polish_iter} kkt={res.residuals:.2e} "
