@@ -1443,7 +1443,108 @@ struct LoopState {
     int64_t nh = 0;
     double best_kkt = INFINITY;
     int stall = 0;
+    // auto-save (src/algorithm.jl:97-112): the best iterate so far by
+    // max(err_Rp, err_Rd, rel_gap), scaled space, FP64
+    DevVec<double> sx, sy, sz, sw;
+    bool saved = false;
+    int64_t save_iter = 0;
+    double save_sigma = 0.0, save_Rp = 0.0, save_Rd = 0.0, save_pobj = 0.0, save_dobj = 0.0, save_gap = 0.0;
 };
+
+template <class T>
+static void save_copy(hipStream_t s, const T *src, double *dst, int64_t len) {
+    if (len > 0) each(s, len, [=] __device__(int64_t i) { dst[i] = (double)src[i]; });
+}
+
+// at each residual evaluation: keep the iterate if it is the best so far
+template <class VT, class MT>
+static void auto_save_track(Solver<VT, MT> &W, LoopState &L, int64_t iter,
+                            const typename Solver<VT, MT>::Resid &res) {
+    if (L.saved && !(res.kkt < std::max(std::max(L.save_Rp, L.save_Rd), L.save_gap))) return;
+    const int32_t n = W.n, m = W.m;
+    if (!L.saved) {
+        L.sx.alloc(n);
+        L.sz.alloc(n);
+        L.sw.alloc(n);
+        L.sy.alloc(m);
+    }
+    save_copy(W.s, (const VT *)W.x_bar.p, L.sx.p, n);
+    save_copy(W.s, (const VT *)W.z_bar.p, L.sz.p, n);
+    save_copy(W.s, (const VT *)W.w_bar.p, L.sw.p, n);
+    save_copy(W.s, (const VT *)W.y_bar.p, L.sy.p, m);
+    L.saved = true;
+    L.save_iter = iter;
+    L.save_sigma = W.sigma;
+    L.save_Rp = res.err_Rp;
+    L.save_Rd = res.err_Rd;
+    L.save_pobj = res.primal_obj;
+    L.save_dobj = res.dual_obj;
+    L.save_gap = res.rel_gap;
+}
+
+// v -> sc * v / nrm (the reference's save formulas), downloaded into out
+template <class T>
+static void save_download(hipStream_t s, const T *src, const double *nrm, int64_t len, double sc, double *tmp,
+                          double *out) {
+    if (len <= 0) return;
+    each(s, len, [=] __device__(int64_t i) { tmp[i] = sc * ((double)src[i] / nrm[i]); });
+    HPRQP_HIP(hipMemcpyAsync(out, tmp, sizeof(double) * len, hipMemcpyDeviceToHost, s));
+    HPRQP_HIP(hipStreamSynchronize(s));
+}
+
+// at print steps and at termination: the current and the best iterate in
+// the original space to the caller's callback (save_state_to_hdf5!,
+// src/algorithm.jl:183-301)
+template <class VT, class MT>
+static void auto_save_write(Solver<VT, MT> &W, LoopState &L, int64_t iter,
+                            const typename Solver<VT, MT>::Resid &res, double elapsed) {
+    const hprqp_params *p = W.p;
+    if (!p->save_cb || !L.saved) return;
+    hipStream_t s = W.s;
+    const int32_t n = W.n, m = W.m;
+    const double bs = W.S.b_scale, cs = W.S.c_scale;
+    const double *cn = W.S.col_norm.p, *rn = W.S.row_norm.p;
+    DevVec<double> t(std::max<int64_t>(1, std::max(n, m)));
+    std::vector<double> cur(3 * (size_t)n + (size_t)m), best(3 * (size_t)n + (size_t)m);
+    double *cx = cur.data(), *cz = cx + n, *cw = cz + n, *cy = cw + n;
+    double *bx = best.data(), *bz = bx + n, *bw = bz + n, *by = bw + n;
+    save_download(s, (const VT *)W.x_bar.p, cn, n, bs, t.p, cx);
+    save_download(s, (const VT *)W.w_bar.p, cn, n, bs, t.p, cw);
+    save_download(s, (const VT *)W.z_bar.p, cn, n, cs, t.p, cz);
+    save_download(s, (const VT *)W.y_bar.p, rn, m, cs, t.p, cy);
+    save_download(s, (const double *)L.sx.p, cn, n, bs, t.p, bx);
+    save_download(s, (const double *)L.sw.p, cn, n, bs, t.p, bw);
+    save_download(s, (const double *)L.sz.p, cn, n, cs, t.p, bz);
+    save_download(s, (const double *)L.sy.p, rn, m, cs, t.p, by);
+    hprqp_save_state st{};
+    st.n = n;
+    st.m = m;
+    st.iteration = iter;
+    st.time_elapsed = elapsed;
+    st.sigma = W.sigma;
+    st.err_Rp = res.err_Rp;
+    st.err_Rd = res.err_Rd;
+    st.primal_obj = res.primal_obj;
+    st.dual_obj = res.dual_obj;
+    st.rel_gap = res.rel_gap;
+    st.kkt = res.kkt;
+    st.x = cx;
+    st.y = cy;
+    st.z = cz;
+    st.w = cw;
+    st.best_iteration = L.save_iter;
+    st.best_sigma = L.save_sigma;
+    st.best_err_Rp = L.save_Rp;
+    st.best_err_Rd = L.save_Rd;
+    st.best_primal_obj = L.save_pobj;
+    st.best_dual_obj = L.save_dobj;
+    st.best_rel_gap = L.save_gap;
+    st.best_x = bx;
+    st.best_y = by;
+    st.best_z = bz;
+    st.best_w = bw;
+    HPRQP_REQUIRE(p->save_cb(&st, p->save_user) == 0, HPRQP_E_ARG, "auto-save callback failed");
+}
 
 template <class VT, class MT>
 static int run_loop(Solver<VT, MT> &W, LoopState &L, int64_t iter_begin, double t_start, double obj_constant,
@@ -1460,6 +1561,7 @@ static int run_loop(Solver<VT, MT> &W, LoopState &L, int64_t iter_begin, double 
         if (do_res) {
             W.compute_residuals(iter, obj_constant, lasso, res);
             has_res = true;
+            if (p->auto_save) auto_save_track(W, L, iter, res);
             // reduced precision: hand over to the FP64 polish once the
             // FP64-measured KKT residual is within polish_switch of the
             // tolerance (the FP32 data / iterates floor lies near there), or
@@ -1506,6 +1608,7 @@ static int run_loop(Solver<VT, MT> &W, LoopState &L, int64_t iter_begin, double 
         if (p->verbose && (print_yes || st != -2))
             std::printf("%7lld  %3.2e  %3.2e  %+.8e  %+.8e  %3.2e  %3.2e  %7.2f\n", (long long)iter, res.err_Rp,
                         res.err_Rd, res.primal_obj, res.dual_obj, res.rel_gap, W.sigma, now_s() - t_start);
+        if (p->auto_save && (print_yes || st != -2)) auto_save_write(W, L, iter, res, now_s() - t_start);
         if (res.kkt < 1e-4 && L.first_4) {
             L.time_4 = now_s() - t_start;
             L.iter_4 = iter;

@@ -46,7 +46,43 @@ class hprqp_params(C.Structure):
         ("time_kernels", C.c_int32),
         ("polish_stall_checks", C.c_int32),
         ("polish_switch", C.c_double),
+        ("auto_save", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("save_cb", C.c_void_p),
+        ("save_user", C.c_void_p),
     ]
+
+
+_D = C.POINTER(C.c_double)
+
+
+class hprqp_save_state(C.Structure):
+    """State the solver hands to the auto-save callback (include/hprqp.h)."""
+    _fields_ = [
+        ("n", C.c_int32),
+        ("m", C.c_int32),
+        ("iteration", C.c_int64),
+        ("time_elapsed", C.c_double),
+        ("sigma", C.c_double),
+        ("err_Rp", C.c_double),
+        ("err_Rd", C.c_double),
+        ("primal_obj", C.c_double),
+        ("dual_obj", C.c_double),
+        ("rel_gap", C.c_double),
+        ("kkt", C.c_double),
+        ("x", _D), ("y", _D), ("z", _D), ("w", _D),
+        ("best_iteration", C.c_int64),
+        ("best_sigma", C.c_double),
+        ("best_err_Rp", C.c_double),
+        ("best_err_Rd", C.c_double),
+        ("best_primal_obj", C.c_double),
+        ("best_dual_obj", C.c_double),
+        ("best_rel_gap", C.c_double),
+        ("best_x", _D), ("best_y", _D), ("best_z", _D), ("best_w", _D),
+    ]
+
+
+SAVE_FN = C.CFUNCTYPE(C.c_int, C.POINTER(hprqp_save_state), C.c_void_p)
 
 
 class hprqp_result(C.Structure):

@@ -83,7 +83,71 @@ class HPRQP_parameters:
         p.time_kernels = int(bool(self.time_kernels))
         p.polish_stall_checks = int(self.polish_stall_checks)
         p.polish_switch = float(self.polish_switch)
+        p.auto_save = int(bool(self.auto_save))
         return p
+
+
+def autosave_path(filename: str) -> str:
+    """Where auto_save writes: a NumPy .npz archive (no HDF5 library is
+    available here), at save_filename with ".npz" appended unless present."""
+    return filename if filename.endswith(".npz") else filename + ".npz"
+
+
+def _autosave_writer(params: HPRQP_parameters):
+    """The auto-save callback: writes the reference's groups (current/,
+    best/, parameters/; save_state_to_hdf5!, src/algorithm.jl:183-301) as
+    "group/name" entries of an .npz archive, replacing the file each time."""
+    import datetime
+
+    path = autosave_path(params.save_filename)
+
+    def write(st_p, _user):
+        try:
+            st = st_p.contents
+            n, m = st.n, st.m
+
+            def vec(ptr, k):
+                return np.ctypeslib.as_array(ptr, shape=(k,)).copy() if k > 0 else np.zeros(0)
+
+            d = {
+                "current/iteration": np.int64(st.iteration),
+                "current/time_elapsed": np.float64(st.time_elapsed),
+                "current/timestamp": np.str_(datetime.datetime.now().isoformat()),
+                "current/x_org": vec(st.x, n), "current/w_org": vec(st.w, n), "current/z_org": vec(st.z, n),
+                "current/sigma": np.float64(st.sigma),
+                "current/err_Rp": np.float64(st.err_Rp), "current/err_Rd": np.float64(st.err_Rd),
+                "current/primal_obj": np.float64(st.primal_obj), "current/dual_obj": np.float64(st.dual_obj),
+                "current/rel_gap": np.float64(st.rel_gap), "current/KKTx_and_gap": np.float64(st.kkt),
+                "best/x_org": vec(st.best_x, n), "best/w_org": vec(st.best_w, n), "best/z_org": vec(st.best_z, n),
+                "best/sigma": np.float64(st.best_sigma), "best/iteration": np.int64(st.best_iteration),
+                "best/err_Rp": np.float64(st.best_err_Rp), "best/err_Rd": np.float64(st.best_err_Rd),
+                "best/primal_obj": np.float64(st.best_primal_obj), "best/dual_obj": np.float64(st.best_dual_obj),
+                "best/rel_gap": np.float64(st.best_rel_gap),
+                "best/KKTx_and_gap": np.float64(max(st.best_err_Rp, st.best_err_Rd, st.best_rel_gap)),
+            }
+            if m > 0:
+                d["current/y_org"] = vec(st.y, m)
+                d["best/y_org"] = vec(st.best_y, m)
+            for k in ("stoptol", "sigma", "max_iter", "sigma_fixed", "time_limit", "eig_factor", "check_iter",
+                      "warm_up", "spmv_mode_Q", "spmv_mode_A", "print_frequency", "device_number",
+                      "use_Ruiz_scaling", "use_bc_scaling", "use_l2_scaling", "use_Pock_Chambolle_scaling",
+                      "problem_type", "auto_save"):
+                d["parameters/" + k] = np.asarray(getattr(params, k))
+            d["parameters/lambda"] = np.float64(params.lam)
+            if params.initial_x is not None:
+                d["parameters/initial_x"] = np.asarray(params.initial_x, dtype=np.float64)
+            if params.initial_y is not None:
+                d["parameters/initial_y"] = np.asarray(params.initial_y, dtype=np.float64)
+            with open(path, "wb") as f:
+                np.savez(f, **d)
+            if params.verbose:
+                print(f"State saved to {path} at iteration {st.iteration}")
+            return 0
+        except Exception as e:  # an exception must not cross the C ABI
+            print(f"auto-save failed: {e}")
+            return 1
+
+    return _lib.SAVE_FN(write)
 
 
 @dataclass
@@ -228,6 +292,10 @@ class DeviceModel:
 
     def solve(self, params: HPRQP_parameters, hist_cap: int = 100000) -> HPRQP_results:
         p = params.to_c()
+        cb = None
+        if params.auto_save:
+            cb = _autosave_writer(params)  # kept alive for the call
+            p.save_cb = C.cast(cb, C.c_void_p)
         n, m = self.n, self.m
         x0 = None if params.initial_x is None else np.ascontiguousarray(params.initial_x, dtype=np.float64)
         y0 = None if params.initial_y is None else np.ascontiguousarray(params.initial_y, dtype=np.float64)

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define HPRQP_ABI_VERSION 1
+#define HPRQP_ABI_VERSION 2  /* 2: hprqp_params gained the auto-save fields */
 
 /* ---- error codes ---------------------------------------------------- */
 #define HPRQP_OK 0
@@ -72,6 +72,25 @@ typedef struct hprqp_csr hprqp_csr;
 typedef struct hprqp_qop hprqp_qop;
 typedef struct hprqp_model hprqp_model;
 
+/* State handed to the auto-save callback at print steps and at termination
+ * (save_state_to_hdf5!, src/algorithm.jl:183-301): the current iterate and
+ * the best one so far -- lowest max(err_Rp, err_Rd, rel_gap) over the
+ * residual checks (src/algorithm.jl:97-112) -- in the original space with
+ * the reference's save formulas (x, w: b_scale v / col_norm; y: c_scale v /
+ * row_norm; z: c_scale v / col_norm).  The arrays are host memory valid only
+ * during the call; y and best_y are empty when m = 0. */
+typedef struct hprqp_save_state {
+    int32_t n, m;
+    int64_t iteration;
+    double time_elapsed, sigma, err_Rp, err_Rd, primal_obj, dual_obj, rel_gap, kkt;
+    const double *x, *y, *z, *w;
+    int64_t best_iteration;
+    double best_sigma, best_err_Rp, best_err_Rd, best_primal_obj, best_dual_obj, best_rel_gap;
+    const double *best_x, *best_y, *best_z, *best_w;
+} hprqp_save_state;
+/* returns 0 on success; anything else ends the solve with HPRQP_E_ARG */
+typedef int (*hprqp_save_fn)(const hprqp_save_state *state, void *user);
+
 /* Solver parameters.  Field meanings and defaults follow HPRQP_parameters
  * (src/structs.jl:195-275); hprqp_params_default fills the defaults. */
 typedef struct {
@@ -105,6 +124,12 @@ typedef struct {
                                     takes over once the KKT residual drops
                                     below polish_switch * stoptol (default 5;
                                     0 = only on a stall) */
+    int32_t auto_save;           /* 0; 1 = track the best iterate and hand it
+                                    with the current one to save_cb at print
+                                    steps and at termination (auto_save) */
+    int32_t reserved0;
+    hprqp_save_fn save_cb;       /* NULL: tracked but not handed over */
+    void *save_user;             /* passed to save_cb */
 } hprqp_params;
 
 /* Results (HPRQP_results, src/structs.jl:292-309) plus diagnostics. */

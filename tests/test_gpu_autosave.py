@@ -1,0 +1,57 @@
+"""auto_save (src/algorithm.jl:97-112, 183-301): the best iterate by
+max(err_Rp, err_Rd, rel_gap) over the residual checks and the current one,
+written with the parameters at print steps and at termination.  The archive
+is NumPy .npz with the reference's "group/name" keys (no HDF5 library here)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _load(path):
+    with np.load(path, allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autosave_current_and_best(tmp_path, precision):
+    from hprqp import HPRQP_parameters, instances, optimize
+
+    model = instances.random_sparse_qp(400, 200, seed=0)
+    fn = str(tmp_path / "state")
+    res = optimize(model, HPRQP_parameters(verbose=False, auto_save=True, save_filename=fn, precision=precision))
+    d = _load(fn + ".npz")
+    print(f"{precision}: iter={res.iter} saved current {int(d['current/iteration'])} best "
+          f"{int(d['best/iteration'])} kkt {float(d['best/KKTx_and_gap']):.2e}")
+    assert res.status == "OPTIMAL"
+    # the last write is at termination: the current iterate is the result
+    assert int(d["current/iteration"]) == res.iter
+    np.testing.assert_array_equal(d["current/x_org"], res.x)
+    np.testing.assert_array_equal(d["current/y_org"], res.y)
+    np.testing.assert_array_equal(d["current/w_org"], res.w)
+    assert abs(float(d["current/primal_obj"]) - res.primal_obj) <= 1e-12 * (1 + abs(res.primal_obj))
+    # the best iterate is never worse than the current one and comes no later
+    assert float(d["best/KKTx_and_gap"]) <= float(d["current/KKTx_and_gap"])
+    assert int(d["best/iteration"]) <= res.iter
+    if int(d["best/iteration"]) == res.iter:
+        np.testing.assert_array_equal(d["best/x_org"], res.x)
+    assert d["best/x_org"].shape == (model.n,) and d["best/y_org"].shape == (model.m,)
+    assert float(d["parameters/stoptol"]) == 1e-6 and str(d["parameters/problem_type"]) == "QP"
+    assert bool(d["parameters/auto_save"])
+
+
+def test_autosave_unscaled_z_and_failed_write(tmp_path):
+    """With scaling off (col_norm = 1) the saved z is the returned z; a write
+    that fails ends the solve with an error instead of being ignored."""
+    from hprqp import HPRQP_parameters, instances, optimize
+
+    model = instances.random_sparse_qp(300, 150, seed=1)
+    fn = str(tmp_path / "plain.npz")
+    p = HPRQP_parameters(verbose=False, auto_save=True, save_filename=fn, use_Ruiz_scaling=False,
+                         use_Pock_Chambolle_scaling=False)
+    res = optimize(model, p)
+    d = _load(fn)
+    np.testing.assert_array_equal(d["current/z_org"], res.z)
+    bad = HPRQP_parameters(verbose=False, auto_save=True, save_filename=str(tmp_path / "missing" / "x.npz"))
+    with pytest.raises(Exception):
+        optimize(model, bad)

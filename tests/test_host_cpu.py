@@ -193,7 +193,38 @@ def test_library_exports_every_declared_symbol():
     # the ctypes binding covers every declared entry point
     assert set(_declared_symbols()) <= set(_lib.exported_symbols())
     lib.hprqp_abi_version.restype = C.c_int
-    assert lib.hprqp_abi_version() == 1
+    assert lib.hprqp_abi_version() == 2
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of hprqp_params, hprqp_result and hprqp_save_state
+    have the sizes and field offsets the C header gives (a C program built
+    with gcc against include/hprqp.h prints them)."""
+    import shutil
+    import subprocess
+
+    from hprqp import _lib
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"hprqp_params": _lib.hprqp_params, "hprqp_result": _lib.hprqp_result,
+               "hprqp_save_state": _lib.hprqp_save_state}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "hprqp.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'printf("{name} sizeof %zu\\n", sizeof({name}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for name, cls in structs.items():
+        assert got[(name, "sizeof")] == C.sizeof(cls), name
+        for f, _ in cls._fields_:
+            assert got[(name, f)] == getattr(cls, f).offset, (name, f)
 
 
 def test_params_default_matches_reference_defaults():
