@@ -18,7 +18,7 @@ from .model import (
     build_from_QAbc,
     qp_formulation,
 )
-from .solver import DeviceModel, DeviceQOperator, HPRQP_parameters, HPRQP_results, optimize, to_gpu
+from .solver import DeviceModel, DeviceQOperator, HPRQP_parameters, HPRQP_results, load_autosave, optimize, to_gpu
 
 __all__ = [
     "AbstractQOperator",
@@ -36,6 +36,7 @@ __all__ = [
     "DeviceQOperator",
     "HPRQP_parameters",
     "HPRQP_results",
+    "load_autosave",
     "optimize",
     "to_gpu",
 ]

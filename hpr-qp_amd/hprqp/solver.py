@@ -93,6 +93,18 @@ def autosave_path(filename: str) -> str:
     return filename if filename.endswith(".npz") else filename + ".npz"
 
 
+def load_autosave(filename: str, which: str = "best"):
+    """(x, y) of a saved state ("best" or "current"), in the original space,
+    for HPRQP_parameters.initial_x / initial_y -- resuming as the reference
+    does from its saved file.  y is None when the problem has no rows."""
+    if which not in ("best", "current"):
+        raise ValueError("which must be 'best' or 'current'")
+    with np.load(autosave_path(filename), allow_pickle=False) as f:
+        x = f[f"{which}/x_org"].copy()
+        y = f[f"{which}/y_org"].copy() if f"{which}/y_org" in f.files else None
+    return x, y
+
+
 def _autosave_writer(params: HPRQP_parameters):
     """The auto-save callback: writes the reference's groups (current/,
     best/, parameters/; save_state_to_hdf5!, src/algorithm.jl:183-301) as

@@ -55,3 +55,19 @@ def test_autosave_unscaled_z_and_failed_write(tmp_path):
     bad = HPRQP_parameters(verbose=False, auto_save=True, save_filename=str(tmp_path / "missing" / "x.npz"))
     with pytest.raises(Exception):
         optimize(model, bad)
+
+
+def test_resume_from_the_saved_best_iterate(tmp_path):
+    """The saved best iterate as a warm start (initial_x / initial_y in the
+    original space) resumes at the optimum: the restarted solve needs far
+    fewer iterations and reaches the same objective."""
+    from hprqp import HPRQP_parameters, instances, load_autosave, optimize
+
+    model = instances.random_sparse_qp(400, 200, seed=0)
+    fn = str(tmp_path / "resume")
+    first = optimize(model, HPRQP_parameters(verbose=False, auto_save=True, save_filename=fn))
+    x0, y0 = load_autosave(fn)
+    again = optimize(model, HPRQP_parameters(verbose=False, initial_x=x0, initial_y=y0))
+    rel = abs(again.primal_obj - first.primal_obj) / (1.0 + abs(first.primal_obj))
+    print(f"first iter={first.iter} resumed iter={again.iter} rel obj {rel:.2e}")
+    assert again.status == "OPTIMAL" and again.iter < first.iter // 4 and rel < 1e-6
