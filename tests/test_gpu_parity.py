@@ -194,6 +194,44 @@ def test_lp_empty_Q():
     _check(res, ref)
 
 
+def _box_qp_no_constraints():
+    from hprqp import build_from_QAbc
+
+    rng = np.random.default_rng(5)
+    n = 80
+    R = sp.random(40, n, density=0.1, random_state=6, format="csr")
+    Q = (R.T @ R + sp.diags(rng.uniform(0.01, 1.0, n))).tocsr()
+    c = rng.normal(size=n)
+    return build_from_QAbc(Q, c, sp.csr_matrix((0, n)), np.zeros(0), np.zeros(0),
+                           np.full(n, -1.0), np.full(n, 1.0), verbose=False)
+
+
+def test_box_qp_without_constraints():
+    """m = 0: the y half-step, the A/Aᵀ products and the primal residual are
+    skipped (src/kernels.jl:2246, 2586, 2686); only the box projection and
+    Q remain."""
+    model = _box_qp_no_constraints()
+    assert model.m == 0
+    res, ref = _solve_both(model)
+    _check(res, ref)
+    assert res.err_Rp == 0.0
+    np.testing.assert_allclose(res.x, ref.x, atol=1e-4)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp32"])
+def test_box_qp_without_constraints_reduced_precision(precision):
+    from hprqp import HPRQP_parameters, optimize
+    from oracle import oracle
+
+    model = _box_qp_no_constraints()
+    res = optimize(model, HPRQP_parameters(verbose=False, precision=precision))
+    ref = oracle.solve(model)
+    rel = abs(res.primal_obj - ref.primal_obj) / (1.0 + abs(ref.primal_obj))
+    print(f"{precision} box QP m=0: {res.status} iter={res.iter} kkt={res.residuals:.2e} rel_obj_err={rel:.2e}")
+    assert res.status == "OPTIMAL" and res.residuals < 1e-6
+    assert rel < 1e-6
+
+
 def test_portfolio_small():
     from hprqp import instances
 

@@ -323,3 +323,31 @@ def test_lp_matches_scipy_linprog():
     ref = linprog(c, A_eq=A, b_eq=b, bounds=[(0, 3)] * n, method="highs")
     assert r.status == "OPTIMAL"
     assert abs(r.primal_obj - ref.fun) / (1 + abs(ref.fun)) < 1e-6
+
+
+def test_box_qp_without_constraints_matches_lbfgsb():
+    """m = 0: no linear constraints, only the box (the reference skips the y
+    half-step when ws.m == 0, src/kernels.jl:2246).  The optimum of a strictly
+    convex box QP is unique, so L-BFGS-B's answer pins the oracle's."""
+    from scipy.optimize import minimize
+
+    from hprqp import build_from_QAbc
+
+    o = _oracle()
+    rng = np.random.default_rng(5)
+    n = 80
+    R = sp.random(40, n, density=0.1, random_state=6, format="csr")
+    Q = (R.T @ R + sp.diags(rng.uniform(0.01, 1.0, n))).tocsr()
+    c = rng.normal(size=n)
+    lo, hi = np.full(n, -1.0), np.full(n, 1.0)
+    model = build_from_QAbc(Q, c, sp.csr_matrix((0, n)), np.zeros(0), np.zeros(0), lo, hi, verbose=False)
+    assert model.m == 0
+    r = o.solve(model, stoptol=1e-8)
+    ref = minimize(lambda x: 0.5 * x @ (Q @ x) + c @ x, np.zeros(n), jac=lambda x: Q @ x + c,
+                   bounds=list(zip(lo, hi)), method="L-BFGS-B",
+                   options=dict(ftol=1e-15, gtol=1e-12, maxiter=10000))
+    assert r.status == "OPTIMAL"
+    assert abs(r.primal_obj - ref.fun) / (1 + abs(ref.fun)) < 1e-8
+    np.testing.assert_allclose(r.x, ref.x, atol=1e-5)
+    # some bounds are active, so the box projection is exercised
+    assert np.sum(np.abs(np.abs(r.x) - 1.0) < 1e-6) > 0
