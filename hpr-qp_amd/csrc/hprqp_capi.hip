@@ -27,10 +27,12 @@ HostCsr host_csr_from(int32_t nrow, int32_t ncol, int64_t nnz, const int32_t *ro
     if (nrow > 0) HPRQP_REQUIRE(rowptr != nullptr, HPRQP_E_ARG, "NULL rowptr");
     for (int32_t i = 0; i <= nrow && nrow > 0; ++i) h.rp[i] = rowptr[i] - index_base;
     HPRQP_REQUIRE(nrow == 0 || (h.rp[0] == 0 && h.rp[nrow] == nnz), HPRQP_E_ARG, "rowptr inconsistent with nnz");
+    // the whole rowptr first: with rp[0] = 0, rp[nrow] = nnz and no descent,
+    // every row's range lies inside colidx / vals
+    for (int32_t i = 0; i < nrow; ++i) HPRQP_REQUIRE(h.rp[i] <= h.rp[i + 1], HPRQP_E_ARG, "rowptr not monotone");
     h.ci.resize((size_t)nnz);
     h.v.resize((size_t)nnz);
     for (int32_t i = 0; i < nrow; ++i) {
-        HPRQP_REQUIRE(h.rp[i] <= h.rp[i + 1], HPRQP_E_ARG, "rowptr not monotone");
         std::vector<std::pair<int32_t, double>> row;
         row.reserve((size_t)(h.rp[i + 1] - h.rp[i]));
         for (int32_t k = h.rp[i]; k < h.rp[i + 1]; ++k) {

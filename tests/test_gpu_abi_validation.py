@@ -1,0 +1,85 @@
+"""The C ABI's own input checks (what a Julia ccall caller meets when it
+skips the host package's checks): hprqp_model_create rejects bad problem
+data with the reference's messages (qp_formulation, src/utils.jl:78-146) and
+bad CSR structure with HPRQP_E_ARG, before anything reaches the device; the
+context stays usable afterwards."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+HPRQP_E_ARG = -1
+HPRQP_E_DATA = -6
+
+
+def _valid():
+    n, m = 4, 2
+    A = sp.csr_matrix(np.array([[1.0, 1.0, 0.0, 0.0], [0.0, 1.0, -1.0, 2.0]]))
+    return dict(n=n, m=m, rp=A.indptr.astype(np.int32), ci=A.indices.astype(np.int32), v=A.data.copy(),
+                nnz=A.nnz, c=np.array([1.0, -1.0, 0.5, 0.0]), AL=np.array([0.0, -1.0]), AU=np.array([1.0, 1.0]),
+                l=np.full(n, -2.0), u=np.full(n, 2.0))
+
+
+def _create(d):
+    from hprqp import _lib
+    from hprqp.solver import DeviceQOperator
+
+    ctx = _lib.context(0)
+    qop = DeviceQOperator(ctx, sp.identity(d["n"], format="csr"), d["n"])
+    h = C.c_void_p()
+    arrs = [np.ascontiguousarray(d[k], dtype=np.float64) for k in ("c", "AL", "AU", "l", "u")]
+    rc = ctx.lib.hprqp_model_create(ctx.h, d["n"], d["m"], qop.h, d["nnz"], _lib.ptr(d["rp"]), _lib.ptr(d["ci"]),
+                                    _lib.ptr(d["v"]), 0, *(_lib.ptr(a) for a in arrs), 0.0, 0.0, C.byref(h))
+    msg = ctx.lib.hprqp_last_error(ctx.h).decode() if rc else ""
+    if rc == 0:
+        ctx.lib.hprqp_model_destroy(ctx.h, h)
+    else:
+        assert not h.value, "a failed create must not hand out a model"
+    qop.close()
+    return rc, msg
+
+
+def _bad(key, idx, val):
+    d = _valid()
+    d[key] = d[key].copy()
+    d[key][idx] = val
+    return d
+
+
+@pytest.mark.parametrize("case,code,text", [
+    (("c", 1, np.nan), HPRQP_E_DATA, "c vector contains NaN or Inf"),
+    (("c", 0, np.inf), HPRQP_E_DATA, "c vector contains NaN or Inf"),
+    (("l", 2, np.nan), HPRQP_E_DATA, "variable bounds l or u contain NaN"),
+    (("l", 3, 3.0), HPRQP_E_DATA, "infeasible variable bounds: l > u"),
+    (("AL", 0, np.nan), HPRQP_E_DATA, "constraint bounds AL or AU contain NaN"),
+    (("AL", 1, 5.0), HPRQP_E_DATA, "infeasible constraint bounds: AL > AU"),
+    (("v", 2, np.inf), HPRQP_E_DATA, "matrix contains NaN or Inf"),
+    (("ci", 3, 4), HPRQP_E_ARG, "column index out of range"),
+    (("ci", 0, -1), HPRQP_E_ARG, "column index out of range"),
+    (("rp", 2, 4), HPRQP_E_ARG, "rowptr inconsistent with nnz"),
+    # row 0 would claim entries 0..5 of a 5-entry array: rejected before any is read
+    (("rp", 1, 6), HPRQP_E_ARG, "rowptr not monotone"),
+    (("ci", 1, 0), HPRQP_E_ARG, "duplicate entry in CSR row"),
+], ids=["c_nan", "c_inf", "l_nan", "l_gt_u", "AL_nan", "AL_gt_AU", "A_inf", "col_past_n", "col_negative",
+        "rowptr_end", "rowptr_order", "duplicate"])
+def test_model_create_rejects(case, code, text):
+    rc, msg = _create(_bad(*case))
+    print(f"{case}: rc={rc} msg={msg!r}")
+    assert rc == code
+    assert text in msg
+
+
+def test_context_usable_after_rejections():
+    from hprqp import HPRQP_parameters, build_from_QAbc, optimize
+
+    assert _create(_bad("c", 1, np.nan))[0] == HPRQP_E_DATA
+    assert _create(_valid()) == (0, "")
+    d = _valid()
+    A = sp.csr_matrix((d["v"], d["ci"], d["rp"]), shape=(d["m"], d["n"]))
+    model = build_from_QAbc(sp.identity(d["n"], format="csr"), d["c"], A, d["AL"], d["AU"], d["l"], d["u"],
+                            verbose=False)
+    res = optimize(model, HPRQP_parameters(verbose=False))
+    assert res.status == "OPTIMAL" and res.residuals < 1e-6
