@@ -232,6 +232,31 @@ def test_box_qp_without_constraints_reduced_precision(precision):
     assert rel < 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32"])
+def test_every_bound_kind(precision):
+    """Free, lower-only, upper-only, boxed and fixed variables with =, <=, >=
+    and ranged rows in one instance: the projections onto [l, u] and
+    [AL, AU] with infinite ends, against the oracle and against the exact
+    minimiser on the active set."""
+    from bounds_case import active_set_certificate, every_bound_kind_qp
+
+    from hprqp import build_from_QAbc
+
+    Q, c, A, AL, AU, l, u = every_bound_kind_qp()
+    model = build_from_QAbc(Q, c, A, AL, AU, l, u, verbose=False)
+    res, ref = _solve_both(model, precision=precision)
+    if precision == "fp64":
+        _check(res, ref)
+    else:
+        assert res.status == "OPTIMAL" and res.residuals < 1e-6
+        assert abs(res.primal_obj - ref.primal_obj) / (1.0 + abs(ref.primal_obj)) < 1e-6
+    xs, ok = active_set_certificate(Q, c, A, AL, AU, l, u, res.x, tol=1e-4)
+    assert ok
+    fstar = 0.5 * xs @ (Q @ xs) + c @ xs
+    assert abs(res.primal_obj - fstar) / (1 + abs(fstar)) < 1e-6
+    np.testing.assert_allclose(res.x, xs, atol=1e-3)
+
+
 def test_portfolio_small():
     from hprqp import instances
 

@@ -351,3 +351,22 @@ def test_box_qp_without_constraints_matches_lbfgsb():
     np.testing.assert_allclose(r.x, ref.x, atol=1e-5)
     # some bounds are active, so the box projection is exercised
     assert np.sum(np.abs(np.abs(r.x) - 1.0) < 1e-6) > 0
+
+
+def test_every_bound_kind_against_active_set_certificate():
+    """Free, lower-only, upper-only, boxed and fixed variables with =, <=, >=
+    and ranged rows: the oracle's x agrees with the exact minimiser on its
+    active set, and that minimiser is certified optimal."""
+    from bounds_case import active_set_certificate, every_bound_kind_qp
+
+    from hprqp import build_from_QAbc
+
+    Q, c, A, AL, AU, l, u = every_bound_kind_qp()
+    model = build_from_QAbc(Q, c, A, AL, AU, l, u, verbose=False)
+    r = _oracle().solve(model, stoptol=1e-9)
+    assert r.status == "OPTIMAL"
+    xs, ok = active_set_certificate(Q, c, A, AL, AU, l, u, r.x)
+    assert ok
+    np.testing.assert_allclose(r.x, xs, atol=1e-6)
+    fstar = 0.5 * xs @ (Q @ xs) + c @ xs
+    assert abs(r.primal_obj - fstar) / (1 + abs(fstar)) < 1e-9
