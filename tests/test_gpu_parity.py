@@ -154,6 +154,26 @@ def test_lasso_small():
     _check(res, ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("lam_factor", [1.01, 0.995])
+def test_lasso_closed_form_at_the_zero_threshold(lam_factor, precision):
+    """lam just above ||A^T b||_inf: x = 0; just below: one nonzero with a
+    closed form.  The soft-threshold prox must land on exact zeros."""
+    from lasso_closed_form import closed_form
+
+    from hprqp import instances
+
+    model = instances.lasso(60, 900, mean_nnz=40, seed=7, lam_factor=lam_factor)
+    x, f = closed_form(model.Q.A, model.Q.b, model.lam)
+    res, ref = _solve_both(model, problem_type="LASSO", precision=precision)
+    if precision == "fp64":
+        _check(res, ref)
+    assert res.status == "OPTIMAL" and res.residuals < 1e-6
+    assert abs(res.primal_obj - f) / (1 + abs(f)) < 1e-6
+    np.testing.assert_allclose(res.x, x, atol=1e-4)
+    assert np.count_nonzero(res.x) == np.count_nonzero(x)
+
+
 def test_lasso_long_rows_column_chunked():
     """Rows of ~500 nonzeros over 30,000 columns (eight column chunks, the
     last ragged; log-normal row lengths, so some rows pass 64 nonzeros in a

@@ -370,3 +370,17 @@ def test_every_bound_kind_against_active_set_certificate():
     np.testing.assert_allclose(r.x, xs, atol=1e-6)
     fstar = 0.5 * xs @ (Q @ xs) + c @ xs
     assert abs(r.primal_obj - fstar) / (1 + abs(fstar)) < 1e-9
+
+
+@pytest.mark.parametrize("lam_factor", [1.01, 0.995])
+def test_lasso_closed_form_at_the_zero_threshold(lam_factor):
+    from lasso_closed_form import closed_form
+
+    from hprqp import instances
+
+    model = instances.lasso(60, 900, mean_nnz=40, seed=7, lam_factor=lam_factor)
+    x, f = closed_form(model.Q.A, model.Q.b, model.lam)
+    r = _oracle().solve(model, problem_type="LASSO", stoptol=1e-9)
+    assert r.status == "OPTIMAL"
+    assert abs(r.primal_obj - f) / (1 + abs(f)) < 1e-9
+    np.testing.assert_allclose(r.x, x, atol=1e-6)
