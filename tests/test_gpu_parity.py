@@ -298,6 +298,31 @@ def test_warm_start_at_optimum_gives_iter_0():
     assert xonly.status == "OPTIMAL" and xonly.iter <= cold.iter
 
 
+@pytest.mark.parametrize("start", ["x_only", "y_only", "perturbed_xy"])
+def test_warm_start_matches_oracle(start):
+    """The reference's three warm starts (test/runtests.jl:190-245), each
+    given to the GPU solve and to the oracle: the starts are mapped into the
+    scaled space the same way, so the runs agree in iterations and
+    objective.  Perturbed starts keep the run away from iteration 0."""
+    from hprqp import HPRQP_parameters, instances, optimize
+    from oracle import oracle
+
+    model = instances.random_sparse_qp(1500, 700, seed=17)
+    cold = oracle.solve(model)
+    rng = np.random.default_rng(1)
+    x0 = cold.x + 1e-2 * rng.standard_normal(model.n)
+    y0 = cold.y + 1e-2 * rng.standard_normal(model.m)
+    kw = {"x_only": dict(initial_x=x0), "y_only": dict(initial_y=y0),
+          "perturbed_xy": dict(initial_x=x0, initial_y=y0)}[start]
+    res = optimize(model, HPRQP_parameters(verbose=False, **kw))
+    ref = oracle.solve(model, **kw)
+    print(f"{start}: gpu iter={res.iter} obj={res.primal_obj:.12e}; oracle iter={ref.iter} "
+          f"obj={ref.primal_obj:.12e}; cold iter={cold.iter}")
+    _check(res, ref)
+    if start == "perturbed_xy":
+        assert res.iter < cold.iter
+
+
 def test_csr_spmv_matches_scipy():
     import ctypes as C
 
