@@ -1,5 +1,5 @@
-"""A small strictly convex QP with every kind of bound, and an exact
-certificate of its optimum.
+"""Small QPs with structure the solver must handle (every kind of bound;
+a Q with empty rows), and an exact certificate of an optimum.
 
 Variables: free, lower-only, upper-only, boxed and fixed (l = u); rows:
 equality, <=, >= and ranged (qp_formulation keeps each kind,
@@ -32,6 +32,21 @@ def every_bound_kind_qp():
     AL[16:24] = b[16:24] - r[16:24]                      # >=
     AL[24:], AU[24:] = b[24:] - r[24:], b[24:] + r[24:]  # ranged
     return Q, c, A, AL, AU, l, u
+
+
+def partially_quadratic_qp():
+    """Half the variables enter only linearly: Q's last 200 rows are empty,
+    as in many Maros-Meszaros problems; box [0, 2], ranged rows."""
+    rng = np.random.default_rng(31)
+    n, m = 400, 150
+    R = sp.random(80, 200, density=0.05, random_state=32, format="csr")
+    Qh = (R.T @ R + sp.diags(rng.uniform(0.05, 1.0, 200))).tocsr()
+    Q = sp.block_diag([Qh, sp.csr_matrix((200, 200))], format="csr")
+    Q.eliminate_zeros()
+    c = rng.normal(size=n)
+    A = (sp.random(m, n, density=0.03, random_state=33, format="csr") + sp.eye(m, n, format="csr")).tocsr()
+    b = A @ rng.uniform(0, 1, n)
+    return Q, c, A, b - 0.5, b + 0.5, np.zeros(n), np.full(n, 2.0)
 
 
 def active_set_certificate(Q, c, A, AL, AU, l, u, x, tol=1e-6):

@@ -277,6 +277,21 @@ def test_every_bound_kind(precision):
     np.testing.assert_allclose(res.x, xs, atol=1e-3)
 
 
+@pytest.mark.parametrize("lanes", [0, 1, 64, -256])
+def test_partially_quadratic_empty_Q_rows(lanes):
+    """Q with 200 empty rows out of 400 (variables that enter only linearly)
+    under the autotuned and forced row-product shapes."""
+    from bounds_case import partially_quadratic_qp
+
+    from hprqp import build_from_QAbc
+
+    Q, c, A, AL, AU, l, u = partially_quadratic_qp()
+    model = build_from_QAbc(Q, c, A, AL, AU, l, u, verbose=False)
+    res, ref = _solve_both(model, spmv_lanes=lanes)
+    _check(res, ref)
+    assert not res.Q_is_diag
+
+
 def test_portfolio_small():
     from hprqp import instances
 

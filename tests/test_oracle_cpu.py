@@ -384,3 +384,19 @@ def test_lasso_closed_form_at_the_zero_threshold(lam_factor):
     assert r.status == "OPTIMAL"
     assert abs(r.primal_obj - f) / (1 + abs(f)) < 1e-9
     np.testing.assert_allclose(r.x, x, atol=1e-6)
+
+
+def test_partially_quadratic_against_active_set_certificate():
+    """Q with 200 empty rows (linear variables): the oracle's x agrees with
+    the certified exact minimiser on its active set."""
+    from bounds_case import active_set_certificate, partially_quadratic_qp
+
+    from hprqp import build_from_QAbc
+
+    Q, c, A, AL, AU, l, u = partially_quadratic_qp()
+    assert np.sum(np.diff(Q.indptr) == 0) == 200
+    r = _oracle().solve(build_from_QAbc(Q, c, A, AL, AU, l, u, verbose=False), stoptol=1e-9)
+    assert r.status == "OPTIMAL"
+    xs, ok = active_set_certificate(Q, c, A, AL, AU, l, u, r.x)
+    assert ok
+    np.testing.assert_allclose(r.x, xs, atol=1e-6)
