@@ -292,6 +292,19 @@ def test_partially_quadratic_empty_Q_rows(lanes):
     assert not res.Q_is_diag
 
 
+def test_bound_edges_mps():
+    """An MPS file with every RANGES sign and bound type (tests/golden/
+    bound_edges.mps), read, formulated and solved: against the oracle and the
+    hand-derived optimum -21/16 (x5 and x7 are not unique)."""
+    from hprqp import build_from_mps
+
+    model = build_from_mps(fixture_path("bound_edges.mps"), verbose=False)
+    res, ref = _solve_both(model)
+    _check(res, ref)
+    assert abs(res.primal_obj - (-21 / 16)) < 1e-5
+    np.testing.assert_allclose(res.x[[0, 1, 2, 3, 5, 7]], [-1, 1.75, 2.5, -0.75, 3.25, 0], atol=1e-4)
+
+
 def test_portfolio_small():
     from hprqp import instances
 

@@ -256,3 +256,28 @@ def test_no_gpu_raises_instead_of_falling_back():
         pass
     with pytest.raises(_lib.HPRQPError):
         _lib.Context(0)
+
+
+def test_mps_ranges_bound_types_and_objective_constant():
+    """tests/golden/bound_edges.mps (written for this test): every RANGES sign per row type, every bound type (UP below zero on a
+    default lower bound, LO, FX, FR, MI, PL after UP, BV, default), the
+    objective-row RHS as a constant, and the lower-triangle QUADOBJ, with
+    the expected arrays written out by hand."""
+    from hprqp.mps import read_mps
+
+    Q, c, A, lcon, ucon, lvar, uvar, c0 = read_mps(fixture_path("bound_edges.mps"))
+    inf = np.inf
+    np.testing.assert_array_equal(c, [1, -1, 0, 0.5, 0, 2, 0, 1])
+    assert c0 == -7.0
+    np.testing.assert_array_equal(A.toarray(), [[1, 1, 0, 0, 0, 1, 0, 0],
+                                                [1, 0, 2, 0, 0, 0, 1, 0],
+                                                [0, 1, 0, -1, 0, 0, 0, 1],
+                                                [0, 0, 1, 0, 1, 0, 0, 0]])
+    # E with R > 0: [rhs, rhs+|R|]; L: [rhs-|R|, rhs]; G: [rhs, rhs+|R|]; E with R < 0: [rhs-|R|, rhs]
+    np.testing.assert_array_equal(lcon, [4, 2, 1, -2])
+    np.testing.assert_array_equal(ucon, [6, 5, 2.5, 0])
+    np.testing.assert_array_equal(lvar, [-inf, -3, 2.5, -inf, -inf, 0, 0, 0])
+    np.testing.assert_array_equal(uvar, [-1, inf, 2.5, inf, inf, inf, 1, inf])
+    Qd = np.zeros((8, 8))
+    Qd[:2, :2] = [[2, 1], [1, 2]]
+    np.testing.assert_array_equal(Q.toarray(), Qd)

@@ -400,3 +400,17 @@ def test_partially_quadratic_against_active_set_certificate():
     xs, ok = active_set_certificate(Q, c, A, AL, AU, l, u, r.x)
     assert ok
     np.testing.assert_allclose(r.x, xs, atol=1e-6)
+
+
+def test_bound_edges_mps_hand_derived_optimum():
+    """tests/golden/bound_edges.mps by hand: x3 is fixed at 2.5, x8 = 0 and
+    x4 = x2 - 2.5 (r3 at its lower end), x6 = 4 - x1 - x2 (r1 at its lower
+    end), x1 at its upper bound -1; what is left is x2^2 - 3.5 x2 + 1.75,
+    minimised at x2 = 1.75 with value -21/16 (x5 and x7 are not unique)."""
+    from hprqp import build_from_mps
+
+    model = build_from_mps(fixture_path("bound_edges.mps"), verbose=False)
+    r = _oracle().solve(model, stoptol=1e-9)
+    assert r.status == "OPTIMAL"
+    assert abs(r.primal_obj - (-21 / 16)) < 1e-7
+    np.testing.assert_allclose(r.x[[0, 1, 2, 3, 5, 7]], [-1, 1.75, 2.5, -0.75, 3.25, 0], atol=1e-6)
