@@ -351,6 +351,32 @@ def test_warm_start_matches_oracle(start):
         assert res.iter < cold.iter
 
 
+@pytest.mark.parametrize("case", ["random", "grid", "lasso"])
+def test_trajectory_matches_oracle_mid_solve(case):
+    """Stopped part-way (max_iter = 300, before convergence), the GPU's
+    iterates equal the oracle's to rounding: the same steps, restarts and
+    sigma updates, not only the same end point.  Summation order differs
+    (row products, reductions), so rounding drifts over 300 iterations:
+    measured up to 1.3e-9 relative (random, grid) and 2e-14 (LASSO); a wrong
+    term or sign moves the iterates by orders of magnitude more."""
+    from hprqp import HPRQP_parameters, instances, optimize
+    from oracle import oracle
+
+    ptype = "LASSO" if case == "lasso" else "QP"
+    model = {"random": lambda: instances.random_sparse_qp(2000, 1000, seed=0),
+             "grid": lambda: instances.grid_qp(60, seed=0),
+             "lasso": lambda: instances.lasso(200, 4000, mean_nnz=60, seed=1)}[case]()
+    res = optimize(model, HPRQP_parameters(verbose=False, max_iter=300, problem_type=ptype))
+    ref = oracle.solve(model, max_iter=300, problem_type=ptype)
+    assert res.status == ref.status == "MAX_ITER" and res.iter == ref.iter == 300
+    assert res.restarts == ref.restarts
+    dx = np.max(np.abs(res.x - ref.x)) / (1 + np.max(np.abs(ref.x)))
+    dy = np.max(np.abs(res.y - ref.y)) / (1 + np.max(np.abs(ref.y))) if model.m else 0.0
+    print(f"{case}: restarts {res.restarts}, x diff {dx:.2e}, y diff {dy:.2e}, kkt {res.residuals:.3e} / {ref.residuals:.3e}")
+    assert dx < 1e-7 and dy < 1e-7
+    assert abs(res.residuals - ref.residuals) <= 1e-5 * ref.residuals
+
+
 def test_csr_spmv_matches_scipy():
     import ctypes as C
 
