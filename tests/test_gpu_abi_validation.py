@@ -83,3 +83,36 @@ def test_context_usable_after_rejections():
                             verbose=False)
     res = optimize(model, HPRQP_parameters(verbose=False))
     assert res.status == "OPTIMAL" and res.residuals < 1e-6
+
+
+def test_products_reject_short_buffers_and_missing_transpose():
+    """hprqp_csr_spmv and hprqp_qop_apply check buffer sizes (and that a
+    transpose was built) on the host before launching anything."""
+    from hprqp import _lib
+    from hprqp.solver import DeviceQOperator
+
+    HPRQP_E_STATE = -4
+    ctx = _lib.context(0)
+    M = sp.csr_matrix(np.arange(1.0, 16.0).reshape(3, 5))
+    rp, ci, v = M.indptr.astype(np.int32), M.indices.astype(np.int32), M.data.copy()
+    h = C.c_void_p()
+    _lib.check(ctx.h, ctx.lib.hprqp_csr_create(ctx.h, 3, 5, M.nnz, _lib.ptr(rp), _lib.ptr(ci), _lib.ptr(v), 0, 0,
+                                               C.byref(h)))
+    x = np.arange(5.0)
+    cases = [(0, 4, 3, HPRQP_E_ARG, "x buffer too small"), (0, 5, 2, HPRQP_E_ARG, "y buffer too small"),
+             (1, 3, 5, HPRQP_E_STATE, "without its transpose")]
+    for transpose, nx, ny, code, text in cases:
+        xb, yb = _lib.DeviceBuffer(ctx, nx), _lib.DeviceBuffer(ctx, ny)
+        rc = ctx.lib.hprqp_csr_spmv(ctx.h, h, transpose, xb.h, yb.h)
+        assert rc == code and text in ctx.lib.hprqp_last_error(ctx.h).decode()
+    xb, yb = _lib.DeviceBuffer.from_array(ctx, x), _lib.DeviceBuffer(ctx, 3)
+    _lib.check(ctx.h, ctx.lib.hprqp_csr_spmv(ctx.h, h, 0, xb.h, yb.h))
+    np.testing.assert_allclose(yb.download(), M @ x, rtol=1e-15)
+    ctx.lib.hprqp_csr_destroy(ctx.h, h)
+
+    qop = DeviceQOperator(ctx, sp.identity(6, format="csr"), 6)
+    short, full = _lib.DeviceBuffer(ctx, 5), _lib.DeviceBuffer(ctx, 6)
+    for a, b in ((short, full), (full, short)):
+        rc = ctx.lib.hprqp_qop_apply(ctx.h, qop.h, a.h, b.h)
+        assert rc == HPRQP_E_ARG and "smaller than the operator size" in ctx.lib.hprqp_last_error(ctx.h).decode()
+    qop.close()
