@@ -598,7 +598,8 @@ int hprqp_solve(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *pa
 }
 
 int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const int32_t *m, const int32_t *k,
-                       const double *U, const double *diag, const int32_t *a_rowptr, const int32_t *a_colidx,
+                       const double *U, const double *diag, int64_t a_nnz, const int32_t *a_rowptr,
+                       const int32_t *a_colidx,
                        const double *a_vals, const double *c, const double *l, const double *u, const double *AL,
                        const double *AU, const double *obj_constant, hprqp_batch **out) {
     CTX_CHECK(ctx);
@@ -626,15 +627,32 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
                 arp_off[b] = ARP;
                 atrp_off[b] = ATRP;
                 nnz_off[b] = NZ;
-                const int32_t nnz_b = m[b] > 0 ? a_rowptr[ARP + m[b]] : 0;
+                int32_t nnz_b = 0;
+                if (m[b] > 0) {
+                    // the instance's whole rowptr is checked before its last
+                    // entry is taken as the instance's nonzero count
+                    HPRQP_REQUIRE(a_rowptr != nullptr, HPRQP_E_ARG, "NULL a_rowptr with m > 0");
+                    const int32_t *rp = a_rowptr + ARP;
+                    HPRQP_REQUIRE(rp[0] == 0, HPRQP_E_ARG, "rowptr inconsistent with nnz");
+                    for (int32_t r = 0; r < m[b]; ++r)
+                        HPRQP_REQUIRE(rp[r] <= rp[r + 1], HPRQP_E_ARG, "rowptr not monotone");
+                    nnz_b = rp[m[b]];
+                }
                 N += n[b];
                 M += m[b];
                 UT += (int64_t)n[b] * k[b];
                 NZ += nnz_b;
                 ARP += m[b] + 1;
                 ATRP += n[b] + 1;
+                HPRQP_REQUIRE(NZ < INT32_MAX && N < INT32_MAX, HPRQP_E_ARG, "batch exceeds int32 indexing");
             }
-            HPRQP_REQUIRE(NZ < INT32_MAX && N < INT32_MAX, HPRQP_E_ARG, "batch exceeds int32 indexing");
+            // the instances' nonzeros together fill a_colidx / a_vals exactly
+            HPRQP_REQUIRE(NZ == a_nnz, HPRQP_E_ARG, "rowptr inconsistent with nnz");
+            HPRQP_REQUIRE(NZ == 0 || (a_colidx && a_vals), HPRQP_E_ARG, "NULL a_colidx or a_vals with nnz > 0");
+            HPRQP_REQUIRE(M == 0 || (AL && AU), HPRQP_E_ARG, "NULL AL or AU with m > 0");
+            HPRQP_REQUIRE(UT == 0 || U, HPRQP_E_ARG, "NULL U with k > 0");
+            for (int64_t i = 0; i < UT; ++i)
+                HPRQP_REQUIRE(std::isfinite(U[i]), HPRQP_E_DATA, "U has NaN/Inf");
             B->Ntot = N;
             B->Mtot = M;
             B->Utot = UT;
@@ -672,6 +690,8 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
                     const int64_t g = B->noff[b] + i;
                     HPRQP_REQUIRE(std::isfinite(c[g]) && std::isfinite(diag[g]), HPRQP_E_DATA,
                                   "c or diag has NaN/Inf");
+                    HPRQP_REQUIRE(!std::isnan(l[g]) && !std::isnan(u[g]), HPRQP_E_DATA,
+                                  "variable bounds l or u contain NaN values");
                     HPRQP_REQUIRE(!(l[g] > u[g]), HPRQP_E_DATA, "infeasible variable bounds: l > u");
                     s2 += c[g] * c[g];
                     si = std::max(si, std::fabs(c[g]));
@@ -681,6 +701,8 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
                 s2 = si = 0;
                 for (int32_t r = 0; r < m[b]; ++r) {
                     const int64_t g = B->moff[b] + r;
+                    HPRQP_REQUIRE(!std::isnan(AL[g]) && !std::isnan(AU[g]), HPRQP_E_DATA,
+                                  "constraint bounds AL or AU contain NaN values");
                     HPRQP_REQUIRE(!(AL[g] > AU[g]), HPRQP_E_DATA, "infeasible constraint bounds: AL > AU");
                     const double a = AL[g] == -INFINITY ? 0.0 : AL[g], bb = AU[g] == INFINITY ? 0.0 : AU[g];
                     const double t = std::max(std::fabs(a), std::fabs(bb));

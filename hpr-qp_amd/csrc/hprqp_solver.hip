@@ -5,12 +5,16 @@
 // the fused kernels of hprqp_kernels.hpp.  Restart and sigma logic stay on
 // the host and read back a few scalars per check step, as in the reference.
 //
-// Precision: Solver<double> is the plain FP64 form.  Solver<float> is the
-// mixed form: FP32 matrices, FP32 bulk vectors and FP32 accumulation in the
-// SpMVs and fused updates; FP64 Halpern anchors (last_*); FP64 matrices and
-// FP64 accumulation for the residual/KKT evaluation and all dot products of
-// the sigma/restart tests.  A mixed solve finishes with an FP64 polish that
-// continues the same iteration in Solver<double> from the FP32 state.
+// Precision: Solver<VT, MT> keeps its bulk vectors in VT and reads matrix
+// values and problem data in MT.  Solver<double> is the plain FP64 form.
+// Solver<double, float> is the mixed form: FP32 matrix values and data, FP64
+// vectors and FP64 accumulation.  Solver<float, float> is the FP32 form: FP32
+// matrices, FP32 bulk vectors and FP32 accumulation in the row products and
+// fused updates.  In every form the Halpern anchors (last_*) are FP64, and the
+// residual/KKT evaluation and the dot products of the sigma/restart tests use
+// FP64 matrices and FP64 accumulation.  A reduced-precision solve finishes
+// with an FP64 polish that continues the same iteration in Solver<double>
+// from its state once it stalls near its floor.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -95,9 +99,15 @@ static void launch_chunked(hipStream_t s, const DevCsr &a, const MT *v, const XT
     HPRQP_HIP(hipGetLastError());
 }
 
+// A matrix without nonzeros has no value or index arrays (nothing is
+// allocated for them): it takes the one-lane shape, whose row loop reads
+// nothing when every row is empty, whatever shape was chosen (the chunked
+// shape loads entry 0 of the value and index arrays speculatively).
 template <class AT, class MT, class XT, class Epi>
 static void csr_apply(hipStream_t s, const DevCsr &a, const XT *x, const Epi &epi) {
-    if (a.lanes == kLanesChunked)
+    if (a.nnz == 0)
+        launch_csr<AT>(s, 1, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), x, epi);
+    else if (a.lanes == kLanesChunked)
         launch_chunked<AT>(s, a, csr_vals<MT>(a), x, epi);
     else
         launch_csr<AT>(s, a.lanes, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), x, epi);
@@ -1543,7 +1553,11 @@ static void auto_save_write(Solver<VT, MT> &W, LoopState &L, int64_t iter,
     st.best_y = by;
     st.best_z = bz;
     st.best_w = bw;
-    HPRQP_REQUIRE(p->save_cb(&st, p->save_user) == 0, HPRQP_E_ARG, "auto-save callback failed");
+    // a failed save is reported and the solve goes on (the reference catches
+    // the failure of save_state_to_hdf5!, src/algorithm.jl:2005-2012)
+    const int rc = p->save_cb(&st, p->save_user);
+    if (rc != 0 && p->verbose)
+        std::printf("Warning: auto-save callback failed (code %d) at iteration %lld\n", rc, (long long)iter);
 }
 
 template <class VT, class MT>
@@ -1662,10 +1676,32 @@ static void init_solver(Solver<VT, MT> &W, const hprqp_model *md) {
 
 static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool mixed);
 
+// The row-product shapes of a LASSO operator's matrices live in the operator,
+// which other solves and hprqp_qop_apply share: a solve sets them (forced or
+// tuned) for its own duration, and this guard puts back what they were when
+// the solve ends, however it ends.
+struct OperatorLanesGuard {
+    hprqp_qop *q;
+    int la = 0, lat = 0;
+    explicit OperatorLanesGuard(hprqp_qop *op) : q(op->kind == QOP_LASSO ? op : nullptr) {
+        if (q) {
+            la = q->la.lanes;
+            lat = q->lat.lanes;
+        }
+    }
+    ~OperatorLanesGuard() {
+        if (q) {
+            q->la.lanes = la;
+            q->lat.lanes = lat;
+        }
+    }
+};
+
 void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const double *x0, const double *y0,
            double *x_out, double *y_out, double *z_out, double *w_out, double *hist, int64_t hist_cap,
            hprqp_result *res) {
     std::memset(res, 0, sizeof(*res));
+    OperatorLanesGuard lanes_guard(md->Q);
     HPRQP_REQUIRE(p->check_iter > 0, HPRQP_E_ARG, "check_iter must be positive");
     HPRQP_REQUIRE(p->print_frequency == -1 || p->print_frequency > 0, HPRQP_E_ARG,
                   "print_frequency must be -1 or positive");
@@ -1964,6 +2000,7 @@ static void profile_impl(Solver<VT, MT> &W, int reps, double *ms, int *nout) {
 
 void profile_step(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, int reps, double *ms_out,
                   int *n_out) {
+    OperatorLanesGuard lanes_guard(md->Q);
     Reducer red(ctx);
     Scaled S;
     build_scaled(ctx, md, p, red, S);

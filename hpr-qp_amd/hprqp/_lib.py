@@ -181,8 +181,8 @@ SIGNATURES = [
     ("hprqp_solve", C.c_int, [_P, _P, C.POINTER(hprqp_params), _P, _P, _P, _P, _P, _P, _P, C.c_int64,
                               C.POINTER(hprqp_result)]),
     ("hprqp_profile_step", C.c_int, [_P, _P, C.POINTER(hprqp_params), C.c_int, _P, C.POINTER(C.c_int)]),
-    ("hprqp_batch_create", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-                                     C.POINTER(_P)]),
+    ("hprqp_batch_create", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, C.c_int64, _P, _P, _P, _P, _P, _P, _P,
+                                     _P, _P, C.POINTER(_P)]),
     ("hprqp_batch_destroy", C.c_int, [_P, _P]),
     ("hprqp_mlp_create", C.c_int, [_P, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_P)]),
     ("hprqp_mlp_destroy", C.c_int, [_P, _P]),

@@ -66,8 +66,9 @@ class BatchModel:
         h = C.c_void_p()
         P = _lib.ptr
         _lib.check(self.ctx.h, self.ctx.lib.hprqp_batch_create(
-            self.ctx.h, cnt, P(a["n"]), P(a["m"]), P(a["k"]), P(a["U"]), P(a["d"]), P(a["rp"]), P(a["ci"]),
-            P(a["av"]), P(a["c"]), P(a["l"]), P(a["u"]), P(a["AL"]), P(a["AU"]), P(a["c0"]), C.byref(h)))
+            self.ctx.h, cnt, P(a["n"]), P(a["m"]), P(a["k"]), P(a["U"]), P(a["d"]), a["ci"].size, P(a["rp"]),
+            P(a["ci"]), P(a["av"]), P(a["c"]), P(a["l"]), P(a["u"]), P(a["AL"]), P(a["AU"]), P(a["c0"]),
+            C.byref(h)))
         self.h = h
         self.count = cnt
         self.noff = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)

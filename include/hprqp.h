@@ -37,7 +37,8 @@
 extern "C" {
 #endif
 
-#define HPRQP_ABI_VERSION 2  /* 2: hprqp_params gained the auto-save fields */
+#define HPRQP_ABI_VERSION 3  /* 2: hprqp_params gained the auto-save fields;
+                                3: hprqp_batch_create takes the total nnz of A */
 
 /* ---- error codes ---------------------------------------------------- */
 #define HPRQP_OK 0
@@ -88,7 +89,9 @@ typedef struct hprqp_save_state {
     double best_sigma, best_err_Rp, best_err_Rd, best_primal_obj, best_dual_obj, best_rel_gap;
     const double *best_x, *best_y, *best_z, *best_w;
 } hprqp_save_state;
-/* returns 0 on success; anything else ends the solve with HPRQP_E_ARG */
+/* returns 0 on success; a non-zero return is reported (when verbose) and the
+ * solve continues, as the reference catches a failed save
+ * (src/algorithm.jl:2005-2012) */
 typedef int (*hprqp_save_fn)(const hprqp_save_state *state, void *user);
 
 /* Solver parameters.  Field meanings and defaults follow HPRQP_parameters
@@ -294,7 +297,8 @@ typedef struct {
 
 int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n,
                        const int32_t *m, const int32_t *k, const double *U,
-                       const double *diag, const int32_t *a_rowptr,
+                       const double *diag, int64_t a_nnz,
+                       const int32_t *a_rowptr,
                        const int32_t *a_colidx, const double *a_vals,
                        const double *c, const double *l, const double *u,
                        const double *AL, const double *AU,

@@ -2,7 +2,9 @@
 skips the host package's checks): hprqp_model_create rejects bad problem
 data with the reference's messages (qp_formulation, src/utils.jl:78-146) and
 bad CSR structure with HPRQP_E_ARG, before anything reaches the device; the
-context stays usable afterwards."""
+context stays usable afterwards.  hprqp_batch_create checks each instance's
+row pointers before it sums them, and the sum against the total nnz it is
+given, so no instance can claim entries past the packed arrays."""
 import ctypes as C
 
 import numpy as np
@@ -116,3 +118,69 @@ def test_products_reject_short_buffers_and_missing_transpose():
         rc = ctx.lib.hprqp_qop_apply(ctx.h, qop.h, a.h, b.h)
         assert rc == HPRQP_E_ARG and "smaller than the operator size" in ctx.lib.hprqp_last_error(ctx.h).decode()
     qop.close()
+
+
+def _batch_valid():
+    """Two instances packed as hprqp_batch_create takes them: (n, m, k) =
+    (3, 1, 2) and (2, 2, 1); row pointers local to each instance."""
+    return dict(n=np.array([3, 2], np.int32), m=np.array([1, 2], np.int32), k=np.array([2, 1], np.int32),
+                U=np.arange(1.0, 9.0) * 0.1, d=np.full(5, 0.5), rp=np.array([0, 3, 0, 1, 2], np.int32),
+                ci=np.array([0, 1, 2, 0, 1], np.int32), av=np.ones(5), c=np.linspace(-1.0, 1.0, 5),
+                l=np.zeros(5), u=np.ones(5), AL=np.array([1.0, 0.0, 0.5]), AU=np.array([1.0, 1.0, 0.5]),
+                c0=np.zeros(2), nnz=5)
+
+
+def _batch_create(d):
+    from hprqp import _lib
+
+    ctx = _lib.context(0)
+    P = lambda key: None if d[key] is None else _lib.ptr(np.ascontiguousarray(d[key]))
+    h = C.c_void_p()
+    rc = ctx.lib.hprqp_batch_create(ctx.h, 2, P("n"), P("m"), P("k"), P("U"), P("d"), d["nnz"], P("rp"), P("ci"),
+                                    P("av"), P("c"), P("l"), P("u"), P("AL"), P("AU"), P("c0"), C.byref(h))
+    msg = ctx.lib.hprqp_last_error(ctx.h).decode() if rc else ""
+    if rc == 0:
+        ctx.lib.hprqp_batch_destroy(ctx.h, h)
+    else:
+        assert not h.value, "a failed create must not hand out a batch"
+    return rc, msg
+
+
+def _batch_bad(key, idx, val):
+    d = _batch_valid()
+    if idx is None:
+        d[key] = val
+    else:
+        d[key] = d[key].copy()
+        d[key][idx] = val
+    return d
+
+
+@pytest.mark.parametrize("case,code,text", [
+    # instance 0's last row pointer claims 9 nonzeros where a_colidx holds 5
+    (("rp", 1, 9), HPRQP_E_ARG, "rowptr inconsistent with nnz"),
+    (("rp", 2, 1), HPRQP_E_ARG, "rowptr inconsistent with nnz"),
+    (("nnz", None, 6), HPRQP_E_ARG, "rowptr inconsistent with nnz"),
+    (("rp", 3, 3), HPRQP_E_ARG, "rowptr not monotone"),
+    (("rp", None, None), HPRQP_E_ARG, "NULL a_rowptr"),
+    (("ci", None, None), HPRQP_E_ARG, "NULL a_colidx or a_vals"),
+    (("AL", None, None), HPRQP_E_ARG, "NULL AL or AU"),
+    (("U", None, None), HPRQP_E_ARG, "NULL U"),
+    (("U", 4, np.inf), HPRQP_E_DATA, "U has NaN/Inf"),
+    (("ci", 4, 2), HPRQP_E_ARG, "column index out of range"),
+    (("l", 1, np.nan), HPRQP_E_DATA, "variable bounds l or u contain NaN"),
+    (("u", 4, np.nan), HPRQP_E_DATA, "variable bounds l or u contain NaN"),
+    (("l", 0, 2.0), HPRQP_E_DATA, "infeasible variable bounds: l > u"),
+    (("AU", 2, np.nan), HPRQP_E_DATA, "constraint bounds AL or AU contain NaN"),
+    (("AL", 1, 3.0), HPRQP_E_DATA, "infeasible constraint bounds: AL > AU"),
+], ids=["rowptr_last_oversized", "rowptr_start", "nnz_total", "rowptr_order", "rowptr_null", "colidx_null",
+        "AL_null", "U_null", "U_inf", "col_past_n", "l_nan", "u_nan", "l_gt_u", "AU_nan", "AL_gt_AU"])
+def test_batch_create_rejects(case, code, text):
+    rc, msg = _batch_create(_batch_bad(*case))
+    print(f"{case}: rc={rc} msg={msg!r}")
+    assert rc == code
+    assert text in msg
+
+
+def test_batch_create_accepts_the_valid_batch():
+    assert _batch_create(_batch_valid()) == (0, "")

@@ -511,3 +511,38 @@ def test_qap_reduced_precision_solve(precision):
               f"kkt={res.residuals:.2e} rel_obj_err_vs_fp64={rel:.2e} (oracle iter={ref.iter})")
         assert res.status == "OPTIMAL" and res.residuals < 1e-6
         assert rel < 1e-6
+
+
+@pytest.mark.parametrize("lanes", [1 << 20, 0])
+def test_empty_A_rows_with_the_column_chunked_shape(lanes):
+    """A model handed to the C ABI with m > 0 rows and no nonzeros in A (the
+    host package's qp_formulation would delete such rows; hprqp_model_create
+    does not).  The column-chunked shape loads entry 0 of the value and index
+    arrays speculatively, and an empty matrix has none, so an empty matrix
+    takes the one-lane shape whatever shape is forced.  The reference's power
+    method gives NaN for lambda_max(A A^T) = 0 (0/0 in its error test), and so
+    do the oracle and this solve; A^T y stays exactly 0, so x follows the box
+    QP's trajectory, which the oracle's matches for the 20 iterations before
+    the first sigma update."""
+    from hprqp import HPRQP_parameters
+    from hprqp.model import QP_info_cpu
+    from hprqp.solver import DeviceModel
+    from oracle import oracle
+
+    n, rng = 40, np.random.default_rng(0)
+    R = sp.random(20, n, density=0.2, random_state=1, format="csr")
+    model = QP_info_cpu(Q=(R.T @ R + sp.identity(n)).tocsr(), c=rng.normal(size=n), A=sp.csc_matrix((3, n)),
+                        AL=-np.ones(3), AU=np.ones(3), l=-np.ones(n), u=np.ones(n))
+    dm = DeviceModel(model)
+    try:
+        res = dm.solve(HPRQP_parameters(verbose=False, max_iter=20, spmv_lanes=lanes))
+    finally:
+        dm.close()
+    ref = oracle.solve(model, max_iter=20)
+    print(f"lanes {lanes}: gpu {res.status} iter={res.iter} lamA={res.lambda_max_A} x[:3]={res.x[:3]}; "
+          f"oracle {ref.status} iter={ref.iter} lamA={ref.lambda_max_A}")
+    assert res.status == ref.status == "MAX_ITER" and res.iter == ref.iter == 20
+    assert np.isnan(res.lambda_max_A) and np.isnan(ref.lambda_max_A)
+    assert res.power_iter_A == ref.power_iter_A == 5000
+    assert np.all(np.isfinite(res.x))
+    np.testing.assert_allclose(res.x, ref.x, rtol=0, atol=1e-12)
