@@ -10,7 +10,12 @@
  * @simd loops, which this file restates as plain sequential C loops.
  *
  * Parity status of each part (what pins it) is listed in the tests under
- * tests/test_oracle_*.py.  One part is "parity unpinned": the power-method
+ * tests/test_oracle_*.py.  scaling and compute_M_norm, which only shape the
+ * trajectory, are pinned on their own through oracle_scale / oracle_mnorm
+ * (tests/test_oracle_pinning_cpu.py): the Ruiz entry bound and
+ * equilibration, the Pock-Chambolle spectral bound, the scaling bookkeeping,
+ * and the M-norm as a sum of squares, each check shown to fail on a copy of
+ * this file with a planted slip.  One part is "parity unpinned": the power-method
  * start vector.  The reference draws it from Julia's MersenneTwister(1)
  * randn stream (src/utils.jl:1339-1343), which cannot be regenerated without
  * Julia; this oracle uses its own counter-based N(0,1) stream instead
@@ -930,8 +935,9 @@ static void compute_residuals(ws_t *ws, const scaling_t *sc, resid_t *res,
     res->kkt = fmax_j(fmax_j(res->err_Rp, res->err_Rd), res->rel_gap);
 }
 
-/* compute_M_norm! (src/algorithm.jl:689-739) */
-static double compute_M_norm(ws_t *ws) {
+/* compute_M_norm! (src/algorithm.jl:689-739); terms (may be NULL) receives
+ * the unclamped M_1, M_2, M_3 */
+static double compute_M_norm(ws_t *ws, double *terms) {
     int32_t n = ws->n, m = ws->m;
     double M1 = 0.0, M3 = 0.0;
     double M2 = (1.0 / ws->sigma) * dot(n, ws->dx, ws->dx);
@@ -954,6 +960,11 @@ static double compute_M_norm(ws_t *ws) {
         }
     } else if (!ws->Q_is_diag) {
         M1 = ws->sigma * ws->lambda_max_Q * dot(n, ws->dw, ws->dQw);
+    }
+    if (terms) {
+        terms[0] = M1;
+        terms[1] = M2;
+        terms[2] = M3;
     }
     M2 += fmax_j(M1, 0.0);
     double Mn = fmax_j(M2, 0.0) + fmax_j(M3, 0.0);
@@ -1290,8 +1301,8 @@ int oracle_solve(int32_t n, int32_t m, const oracle_qop *qin, int64_t a_nnz,
         ws.to_check = (next % p->check_iter == 0) || (r.restart_flag > 0) || (next % ps == 0);
         /* perform_iteration_step! (src/algorithm.jl:1881-1900) */
         main_update(&ws, &r);
-        if (r.restart_flag > 0) r.last_gap = compute_M_norm(&ws);
-        if ((iter + 1) % p->check_iter == 0) r.current_gap = compute_M_norm(&ws);
+        if (r.restart_flag > 0) r.last_gap = compute_M_norm(&ws, NULL);
+        if ((iter + 1) % p->check_iter == 0) r.current_gap = compute_M_norm(&ws, NULL);
         r.inner++;
     }
     if (iter > p->max_iter) iter = p->max_iter;
@@ -1330,4 +1341,91 @@ int oracle_solve(int32_t n, int32_t m, const oracle_qop *qin, int64_t a_nnz,
     free(sc.row_norm);
     free(sc.col_norm);
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* scaling! and compute_M_norm! on their own, for the pinning tests     */
+/* ------------------------------------------------------------------ */
+
+int oracle_scale(int32_t n, int32_t m, const oracle_qop *qin, int64_t a_nnz,
+                 const int64_t *a_colptr, const int32_t *a_rowidx,
+                 const double *a_val, const double *c_in, const double *AL_in,
+                 const double *AU_in, const double *l_in, const double *u_in,
+                 const oracle_params *p, double *Q_out, double *A_out,
+                 double *c_out, double *AL_out, double *AU_out, double *l_out,
+                 double *u_out, double *row_norm, double *col_norm,
+                 double *norms) {
+    (void)a_nnz;
+    qop_t Q;
+    if (qop_build(n, qin, &Q) != 0) return -1;
+    csc_t A = csc_copy(m, n, a_colptr, a_rowidx, a_val);
+    memcpy(c_out, c_in, sizeof(double) * (size_t)n);
+    memcpy(l_out, l_in, sizeof(double) * (size_t)n);
+    memcpy(u_out, u_in, sizeof(double) * (size_t)n);
+    if (m > 0) {
+        memcpy(AL_out, AL_in, sizeof(double) * (size_t)m);
+        memcpy(AU_out, AU_in, sizeof(double) * (size_t)m);
+    }
+    scaling_t sc;
+    scaling(n, m, &Q, &A, c_out, AL_out, AU_out, l_out, u_out, p, &sc);
+    if (Q_out && Q.kind == ORACLE_Q_SPARSE) {
+        memset(Q_out, 0, sizeof(double) * (size_t)n * (size_t)n);
+        for (int32_t j = 0; j < n; ++j)
+            for (int64_t k = Q.Q.cp[j]; k < Q.Q.cp[j + 1]; ++k)
+                Q_out[Q.Q.ri[k] + (int64_t)j * n] = Q.Q.v[k];
+    }
+    if (A_out) {
+        memset(A_out, 0, sizeof(double) * (size_t)m * (size_t)n);
+        for (int32_t j = 0; j < n; ++j)
+            for (int64_t k = A.cp[j]; k < A.cp[j + 1]; ++k) A_out[A.ri[k] + (int64_t)j * m] = A.v[k];
+    }
+    memcpy(row_norm, sc.row_norm, sizeof(double) * (size_t)m);
+    memcpy(col_norm, sc.col_norm, sizeof(double) * (size_t)n);
+    norms[0] = sc.norm_b;
+    norms[1] = sc.norm_c;
+    norms[2] = sc.norm_b_org;
+    norms[3] = sc.norm_c_org;
+    norms[4] = sc.b_scale;
+    norms[5] = sc.c_scale;
+    csc_free(&A);
+    qop_free(&Q);
+    free(sc.row_norm);
+    free(sc.col_norm);
+    return 0;
+}
+
+double oracle_mnorm(int32_t n, int32_t m, const oracle_qop *qin, int64_t a_nnz,
+                    const int64_t *a_colptr, const int32_t *a_rowidx,
+                    const double *a_val, const double *dx, const double *dy,
+                    const double *dw, double sigma, double lambda_max_A,
+                    double lambda_max_Q, int Q_is_diag, double *terms) {
+    (void)a_nnz;
+    qop_t Q;
+    if (qop_build(n, qin, &Q) != 0) return NAN;
+    csc_t A = csc_copy(m, n, a_colptr, a_rowidx, a_val);
+    csc_t AT = csc_transpose(&A);
+    ws_t ws;
+    ws_alloc(&ws, n, m);
+    ws.A = &A;
+    ws.AT = &AT;
+    ws.Q = &Q;
+    ws.sigma = sigma;
+    ws.lambda_max_A = lambda_max_A;
+    ws.lambda_max_Q = lambda_max_Q;
+    ws.Q_is_diag = Q_is_diag;
+    ws.diag_Q = dalloc(n);
+    if (Q_is_diag && Q.kind == ORACLE_Q_SPARSE)
+        for (int32_t j = 0; j < n; ++j)
+            for (int64_t k = Q.Q.cp[j]; k < Q.Q.cp[j + 1]; ++k)
+                if (Q.Q.ri[k] == j) ws.diag_Q[j] = Q.Q.v[k];
+    if (Q_is_diag) update_Q_factors(&ws);
+    memcpy(ws.dx, dx, sizeof(double) * (size_t)n);
+    memcpy(ws.dw, dw, sizeof(double) * (size_t)n);
+    if (m > 0) memcpy(ws.dy, dy, sizeof(double) * (size_t)m);
+    const double r = compute_M_norm(&ws, terms);
+    ws_free(&ws);
+    csc_free(&A);
+    csc_free(&AT);
+    qop_free(&Q);
+    return r;
 }

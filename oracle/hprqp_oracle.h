@@ -130,6 +130,30 @@ double oracle_golden_Q_diag(double a, double b, int32_t n, const double *Q,
                             const double *c, const double *d, double lo,
                             double hi, double tol, int maxiter);
 
+/* scaling! on its own (src/utils.jl:350-804): the scaled Q (sparse kind
+ * only) and A as dense column-major n x n and m x n arrays, the scaled c,
+ * AL, AU, l, u, the cumulative row_norm (m) and col_norm (n), and norms[6] =
+ * norm_b, norm_c (scaled data), norm_b_org, norm_c_org, b_scale, c_scale. */
+int oracle_scale(int32_t n, int32_t m, const oracle_qop *q, int64_t a_nnz,
+                 const int64_t *a_colptr, const int32_t *a_rowidx,
+                 const double *a_val, const double *c, const double *AL,
+                 const double *AU, const double *l, const double *u,
+                 const oracle_params *p, double *Q_out, double *A_out,
+                 double *c_out, double *AL_out, double *AU_out, double *l_out,
+                 double *u_out, double *row_norm, double *col_norm,
+                 double *norms);
+
+/* compute_M_norm! on its own (src/algorithm.jl:689-739) for given dx (n),
+ * dy (m), dw (n), sigma, lambda_max_A, lambda_max_Q and the diagonal-Q flag
+ * (the factor fact_M = sigma^2 / (1 + sigma Q_ii) is formed as the solver
+ * forms it).  Returns the M-norm; terms (may be NULL) receives the
+ * unclamped M_1, M_2, M_3. */
+double oracle_mnorm(int32_t n, int32_t m, const oracle_qop *q, int64_t a_nnz,
+                    const int64_t *a_colptr, const int32_t *a_rowidx,
+                    const double *a_val, const double *dx, const double *dy,
+                    const double *dw, double sigma, double lambda_max_A,
+                    double lambda_max_Q, int Q_is_diag, double *terms);
+
 #ifdef __cplusplus
 }
 #endif
