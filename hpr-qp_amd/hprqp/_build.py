@@ -37,5 +37,21 @@ def build_library(verbose: bool = False) -> Path:
     return out
 
 
+def build_test_plugin(verbose: bool = False) -> Path:
+    """The tests' Q-operator plugin (tests/plugin/user_csr_qop.hip): a HIP
+    library that a caller registers through hprqp_qop_create_user."""
+    src = ROOT / "tests" / "plugin" / "user_csr_qop.hip"
+    out = src.parent / "libuser_csr_qop.so"
+    tmp = src.parent / "libuser_csr_qop.so.tmp"
+    cmd = [HIPCC, *FLAGS, f"-I{INCLUDE}", str(src), "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + " ".join(cmd) + "\n" + r.stderr[-4000:])
+    if verbose and r.stderr:
+        print(r.stderr)
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))

@@ -134,12 +134,19 @@ class hprqp_batch_result(C.Structure):
     ]
 
 
+QOP_APPLY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
+QOP_SIZE_FN = C.CFUNCTYPE(C.c_int64, C.c_void_p)
+# const char *(*name)(void *): declared as returning a plain pointer so that a
+# Python callback can hand back the address of a buffer it keeps alive
+QOP_NAME_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p)
+
+
 class hprqp_qop_vtable(C.Structure):
     _fields_ = [
-        ("apply", C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)),
-        ("problem_size", C.CFUNCTYPE(C.c_int64, C.c_void_p)),
-        ("temp_size", C.CFUNCTYPE(C.c_int64, C.c_void_p)),
-        ("name", C.CFUNCTYPE(C.c_char_p, C.c_void_p)),
+        ("apply", QOP_APPLY_FN),
+        ("problem_size", QOP_SIZE_FN),
+        ("temp_size", QOP_SIZE_FN),
+        ("name", QOP_NAME_FN),
     ]
 
 

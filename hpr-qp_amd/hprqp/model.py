@@ -25,7 +25,16 @@ from .mps import read_mps
 # Q operators (AbstractQOperatorCPU, src/Q_operators/Q_operator_interface.jl)
 # --------------------------------------------------------------------------
 class AbstractQOperator:
-    """Plugin interface: Qmap / get_problem_size / get_temp_size / name."""
+    """Plugin interface (src/Q_operators/Q_operator_interface.jl:5-92):
+    get_problem_size / get_temp_size / name, the CPU apply Qmap, and the
+    device apply Qmap_device that the solver calls.
+
+    A user operator implements Qmap_device(x_ptr, Qx_ptr, stream): enqueue
+    Qx = Q x on the HIP stream `stream` for device arrays of n doubles at the
+    integer addresses x_ptr and Qx_ptr (the GPU Qmap! of the reference), for
+    example by calling its own HIP library.  Such an operator is registered
+    with hprqp_qop_create_user through ctypes callbacks.  The built-in
+    operators below are created natively instead."""
 
     name = "USER"
 
@@ -37,6 +46,10 @@ class AbstractQOperator:
 
     def Qmap(self, x: np.ndarray) -> np.ndarray:
         """Reference-semantics CPU apply (used only to describe the operator)."""
+        raise NotImplementedError
+
+    def Qmap_device(self, x_ptr: int, Qx_ptr: int, stream: int) -> None:
+        """Device apply on `stream`; a user operator must provide it."""
         raise NotImplementedError
 
 

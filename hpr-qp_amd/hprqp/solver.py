@@ -233,8 +233,7 @@ class DeviceQOperator:
             _lib.check(ctx.h, lib.hprqp_qop_create_lowrank(ctx.h, Q.diag.size, Q.U.shape[1], _lib.ptr(U),
                                                            _lib.ptr(d), C.byref(h)))
         elif isinstance(Q, AbstractQOperator):
-            raise NotImplementedError("Python-side user operators are bound through hprqp_qop_create_user "
-                                      "from native code; see INTEGRATION.md")
+            keep += self._bind_user(ctx, Q, n, h)
         else:
             rp, ci, v, M = _csr_arrays(Q)
             if M.shape != (n, n):
@@ -244,6 +243,38 @@ class DeviceQOperator:
                                                        C.byref(h)))
         self.h = h
         self.n = n
+        self._keep = keep
+
+    @staticmethod
+    def _bind_user(ctx, Q, n, h):
+        """A user AbstractQOperator through hprqp_qop_create_user: its
+        Qmap_device, get_problem_size, get_temp_size and name become the
+        vtable's C callbacks.  The callbacks (and the name buffer) are kept
+        alive with the device operator; an exception in Qmap_device becomes
+        a non-zero return, which the solver reports as a failed apply."""
+        if type(Q).Qmap_device is AbstractQOperator.Qmap_device:
+            raise NotImplementedError(f"{type(Q).__name__} has no Qmap_device: a user Q operator must enqueue "
+                                      "Qx = Q x on the device (there is no CPU path)")
+        size = int(Q.get_problem_size())
+        if size != n:
+            raise ValueError(f"operator problem size {size} differs from n = {n}")
+        name_buf = C.create_string_buffer(str(getattr(Q, "name", type(Q).__name__)).encode())
+
+        def apply(_user, x, qx, stream):
+            try:
+                Q.Qmap_device(x or 0, qx or 0, stream or 0)
+                return 0
+            except Exception as e:  # nothing may cross the C ABI
+                print(f"user Q operator {type(Q).__name__}: Qmap_device failed: {e}")
+                return 1
+
+        vt = _lib.hprqp_qop_vtable()
+        vt.apply = _lib.QOP_APPLY_FN(apply)
+        vt.problem_size = _lib.QOP_SIZE_FN(lambda _u: size)
+        vt.temp_size = _lib.QOP_SIZE_FN(lambda _u: int(Q.get_temp_size()))
+        vt.name = _lib.QOP_NAME_FN(lambda _u: C.addressof(name_buf))
+        _lib.check(ctx.h, ctx.lib.hprqp_qop_create_user(ctx.h, C.byref(vt), None, C.byref(h)))
+        return [vt, name_buf, Q]
 
     def Qmap(self, x: np.ndarray, precision: str = "fp64") -> np.ndarray:
         """Qmap!(x, Qx, Q) through the device.  precision "fp64" (the

@@ -1,0 +1,73 @@
+"""GPU parity at the workloads' own sizes wherever the oracle finishes in
+seconds: the random sparse QP at n = 10,000, m = 5,000 (seed 0, the
+reference-runnable configuration), the 128 x 128 QAP relaxation (n = 16,384,
+doubly-stochastic constraints, flow/distance data, diagonal shifts) and the
+grid QP at 400 x 400 (n = 160,000; the 1,414 x 1,414 grid takes the oracle
+about 12 minutes).  The bar is the one of test_gpu_parity: OPTIMAL at 1e-6
+relative KKT, objective within 1e-6 relative of the oracle, iterations within
+10 %, lambda_max within 1e-3 and the power method stopping on the oracle's
+iteration.  The reduced-precision forms are held to 1e-6 KKT and an objective
+within 1e-6 of the FP64 solve."""
+import time
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import _check, _solve_both
+
+pytestmark = pytest.mark.gpu
+
+
+def _timed_both(model, problem_type="QP"):
+    t = time.time()
+    res, ref = _solve_both(model, problem_type=problem_type)
+    print(f"gpu solve {res.time:.3f} s, oracle + gpu wall {time.time() - t:.1f} s")
+    return res, ref
+
+
+def _reduced(model, fp64, problem_type="QP", precision="mixed"):
+    from hprqp import HPRQP_parameters, optimize
+
+    res = optimize(model, HPRQP_parameters(verbose=False, precision=precision, problem_type=problem_type))
+    rel = abs(res.primal_obj - fp64.primal_obj) / (1.0 + abs(fp64.primal_obj))
+    print(f"{precision}: {res.status} iter={res.iter} polish={res.polish_iter} kkt={res.residuals:.2e} "
+          f"time={res.time:.3f} s rel_obj_err_vs_fp64={rel:.2e}")
+    assert res.status == "OPTIMAL" and res.residuals < 1e-6
+    assert rel < 1e-6
+    return res
+
+
+def test_random_sparse_qp_n10000_m5000_seed0():
+    from hprqp import instances
+
+    model = instances.random_sparse_qp(10_000, 5_000, seed=0)
+    assert model.n == 10_000 and model.m == 5_000 and model.A.nnz == 25_000
+    res, ref = _timed_both(model)
+    _check(res, ref)
+    assert not res.Q_is_diag and not res.noC
+    _reduced(model, res, precision="mixed")
+
+
+def test_qap_128_flow_distance():
+    from hprqp import instances
+
+    model = instances.qap(128, seed=0)[0]
+    assert model.n == 128 * 128 and model.m == 256
+    res, ref = _timed_both(model, problem_type="QAP")
+    _check(res, ref)
+    # split-BF16 x3 MFMA products in the iteration, FP64 KKT
+    mixed = _reduced(model, res, problem_type="QAP", precision="mixed")
+    # the relaxation's doubly-stochastic constraints hold at the returned x
+    X = mixed.x.reshape(128, 128, order="F")
+    assert np.max(np.abs(X.sum(axis=0) - 1.0)) < 1e-4 and np.max(np.abs(X.sum(axis=1) - 1.0)) < 1e-4
+    assert X.min() >= -1e-9
+
+
+def test_grid_qp_400():
+    from hprqp import instances
+
+    model = instances.grid_qp(400, seed=0)
+    assert model.n == 400 * 400
+    res, ref = _timed_both(model)
+    _check(res, ref)
+    _reduced(model, res, precision="mixed")
