@@ -597,6 +597,114 @@ int hprqp_solve(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *pa
     });
 }
 
+// ---- step-level interface ----------------------------------------------
+#define WS_CHECK(ctx, ws)                                                                            \
+    do {                                                                                             \
+        CTX_CHECK(ctx);                                                                              \
+        if (!(ws)) return HPRQP_E_ARG;                                                               \
+        if (hprqp::ws_ctx(ws) != (ctx)) {                                                            \
+            (ctx)->err = "workspace belongs to another context";                                     \
+            return HPRQP_E_STATE;                                                                    \
+        }                                                                                            \
+    } while (0)
+
+int hprqp_ws_create(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, const double *x0,
+                    const double *y0, hprqp_ws **out, hprqp_ws_info *info) {
+    CTX_CHECK(ctx);
+    if (!model || !out) return HPRQP_E_ARG;
+    *out = nullptr;
+    hprqp_params def;
+    if (!params) {
+        hprqp_params_default(&def);
+        params = &def;
+    }
+    return guarded(ctx, [&] {
+        hprqp_ws *w = hprqp::ws_create(ctx, model, params, x0, y0);
+        if (info) hprqp::ws_info(w, info);
+        *out = w;
+    });
+}
+
+int hprqp_ws_destroy(hprqp_ctx *ctx, hprqp_ws *ws) {
+    if (!ws) return HPRQP_OK;
+    WS_CHECK(ctx, ws);
+    return guarded(ctx, [&] {
+        HPRQP_HIP(hipStreamSynchronize(ctx->stream));
+        hprqp::ws_destroy(ws);
+    });
+}
+
+static int ws_step_entry(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp, int kind) {
+    WS_CHECK(ctx, ws);
+    if (!sp) return HPRQP_E_ARG;
+    return guarded(ctx, [&] { hprqp::ws_step(ws, kind, sp); });
+}
+
+int hprqp_step(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) { return ws_step_entry(ctx, ws, sp, 0); }
+int hprqp_step_zxw1(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) {
+    return ws_step_entry(ctx, ws, sp, 1);
+}
+int hprqp_step_tempv(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) {
+    return ws_step_entry(ctx, ws, sp, 2);
+}
+int hprqp_step_y(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) { return ws_step_entry(ctx, ws, sp, 3); }
+int hprqp_step_w2(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) {
+    return ws_step_entry(ctx, ws, sp, 4);
+}
+int hprqp_step_noc_zxw(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) {
+    return ws_step_entry(ctx, ws, sp, 5);
+}
+int hprqp_step_noc_tempv(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) {
+    return ws_step_entry(ctx, ws, sp, 6);
+}
+int hprqp_step_lasso(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) {
+    return ws_step_entry(ctx, ws, sp, 7);
+}
+int hprqp_step_lp_zx(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) {
+    return ws_step_entry(ctx, ws, sp, 8);
+}
+int hprqp_step_lp_y(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp) {
+    return ws_step_entry(ctx, ws, sp, 9);
+}
+
+int hprqp_kkt_reduce(hprqp_ctx *ctx, hprqp_ws *ws, int64_t iter, hprqp_kkt_out *out) {
+    WS_CHECK(ctx, ws);
+    if (!out || iter < 0) return HPRQP_E_ARG;
+    return guarded(ctx, [&] { hprqp::ws_kkt(ws, iter, out); });
+}
+
+int hprqp_mnorm(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp, double *out) {
+    WS_CHECK(ctx, ws);
+    if (!sp || !out) return HPRQP_E_ARG;
+    return guarded(ctx, [&] { *out = hprqp::ws_mnorm(ws, sp); });
+}
+
+int hprqp_sigma_dots(hprqp_ctx *ctx, hprqp_ws *ws, double *out) {
+    WS_CHECK(ctx, ws);
+    if (!out) return HPRQP_E_ARG;
+    return guarded(ctx, [&] { hprqp::ws_sigma_dots(ws, out); });
+}
+
+int hprqp_update_Q_factors(hprqp_ctx *ctx, hprqp_ws *ws, double sigma) {
+    WS_CHECK(ctx, ws);
+    return guarded(ctx, [&] { hprqp::ws_q_factors(ws, sigma); });
+}
+
+int hprqp_restart(hprqp_ctx *ctx, hprqp_ws *ws) {
+    WS_CHECK(ctx, ws);
+    return guarded(ctx, [&] { hprqp::ws_restart(ws); });
+}
+
+int hprqp_ws_collect(hprqp_ctx *ctx, hprqp_ws *ws, double *x, double *y, double *z, double *w) {
+    WS_CHECK(ctx, ws);
+    return guarded(ctx, [&] { hprqp::ws_collect(ws, x, y, z, w); });
+}
+
+int hprqp_ws_download(hprqp_ctx *ctx, hprqp_ws *ws, int which, double *host, int64_t len) {
+    WS_CHECK(ctx, ws);
+    return guarded(ctx, [&] { hprqp::ws_download(ws, which, host, len); });
+}
+
 int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const int32_t *m, const int32_t *k,
                        const double *U, const double *diag, int64_t a_nnz, const int32_t *a_rowptr,
                        const int32_t *a_colidx,

@@ -249,6 +249,20 @@ constexpr int kWarmFeatures = 12;
 void solve(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, const double *x0,
            const double *y0, double *x_out, double *y_out, double *z_out, double *w_out,
            double *hist, int64_t hist_cap, hprqp_result *res);
+// step-level workspace (hprqp_ws_*, hprqp_step_*; defined in hprqp_solver.hip)
+hprqp_ws *ws_create(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const double *x0,
+                    const double *y0);
+void ws_info(const hprqp_ws *w, hprqp_ws_info *info);
+hprqp_ctx *ws_ctx(const hprqp_ws *w);
+void ws_step(hprqp_ws *w, int kind, const hprqp_step_params *sp);
+void ws_kkt(hprqp_ws *w, int64_t iter, hprqp_kkt_out *out);
+double ws_mnorm(hprqp_ws *w, const hprqp_step_params *sp);
+void ws_sigma_dots(hprqp_ws *w, double *out);
+void ws_q_factors(hprqp_ws *w, double sigma);
+void ws_restart(hprqp_ws *w);
+void ws_collect(hprqp_ws *w, double *x, double *y, double *z, double *wv);
+void ws_download(hprqp_ws *w, int which, double *host, int64_t len);
+void ws_destroy(hprqp_ws *w);
 // per-kernel timing of the iteration (hprqp_profile_step)
 void profile_step(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, int reps,
                   double *ms_out, int *n_out);

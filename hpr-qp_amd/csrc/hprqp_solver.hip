@@ -835,8 +835,16 @@ struct Solver {
         e.h2 = h2;
         csr_apply<VT, MT>(s, S.AT, (const VT *)y_bar.p, e);
     }
+    // noC half-steps: z/x/w from A^T y, then Q w_bar with the Halpern Qw
+    // update and tempv, then the y step on tempv
     template <bool FULL, bool DIAG>
     void step_noc(double h1, double h2) {
+        step_noc_zxw<FULL, DIAG>(h1, h2);
+        step_noc_tempv(h1, h2);
+        step_y<FULL>(tempv.p, h1, h2);
+    }
+    template <bool FULL, bool DIAG>
+    void step_noc_zxw(double h1, double h2) {
         EpiNocZxw<VT, MT, FULL, DIAG> e;
         e.ATy = ATy.p;
         e.Qw = Qw.p;
@@ -865,9 +873,10 @@ struct Solver {
         kbegin();
         csr_apply<VT, MT>(s, S.AT, (const VT *)y.p, e);
         kend();
+    }
+    void step_noc_tempv(double h1, double h2) {
         EpiNocTempv<VT> e2{x_hat.p, last_Qw.p, Qw.p, Qw_bar.p, tempv.p, sigma, h1, h2};
         q_fused(w_bar.p, e2, Qw_bar.p);
-        step_y<FULL>(tempv.p, h1, h2);
     }
     template <bool FULL>
     void step_lasso(double h1, double h2) {
@@ -895,8 +904,14 @@ struct Solver {
         q_fused(w.p, e, Qw.p);
         kend();
     }
+    // LP half-steps: z/x from A^T y, then the y step on x_hat
     template <bool FULL>
     void step_lp(double h1, double h2) {
+        step_lp_zx<FULL>(h1, h2);
+        step_y<FULL>(x_hat.p, h1, h2);
+    }
+    template <bool FULL>
+    void step_lp_zx(double h1, double h2) {
         EpiLpZx<VT, MT, FULL> e;
         e.ATy = ATy.p;
         e.c = cV;
@@ -916,7 +931,6 @@ struct Solver {
         kbegin();
         csr_apply<VT, MT>(s, S.AT, (const VT *)y.p, e);
         kend();
-        step_y<FULL>(x_hat.p, h1, h2);
     }
 
     template <bool FULL, bool DIAG>
@@ -1206,14 +1220,14 @@ struct SigmaDots {
     }
 };
 
-// update_sigma! (src/algorithm.jl:323-423)
+// the device half of update_sigma! (src/algorithm.jl:323-360): the
+// differences to the anchors, Q dw and Q A^T dy, and six dot products:
+// h = <dx,dx>, <Q dw, A^T dy>, <Q dw, Q dw>, <dw, Q dw>, <A^T dy, Q A^T dy>,
+// <dy,dy>
 template <class VT, class MT>
-static void update_sigma(Solver<VT, MT> &W, Restart &r, double err_Rd, double err_Rp, double rel_gap) {
-    const hprqp_params *p = W.p;
-    if (p->sigma_fixed || r.restart_flag < 1) return;
+static void sigma_dots(Solver<VT, MT> &W, double *h) {
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
-    const double sigma_old = W.sigma;
     {
         VT *dx = W.dx.p, *dw = W.dw.p;
         const VT *xb = W.x_bar.p, *wb = W.w_bar.p;
@@ -1237,7 +1251,20 @@ static void update_sigma(Solver<VT, MT> &W, Restart &r, double err_Rd, double er
                                                m > 0 ? W.QATdy.p : nullptr, W.dw.p},
                                  0);
     W.red.template enqueue<1, 0>(m, typename Solver<VT, MT>::SumSqV{W.dy.p}, 5);
-    const double *h = W.red.fetch(6);
+    const double *f = W.red.fetch(6);
+    for (int k = 0; k < 6; ++k) h[k] = f[k];
+}
+
+// update_sigma! (src/algorithm.jl:323-423)
+template <class VT, class MT>
+static void update_sigma(Solver<VT, MT> &W, Restart &r, double err_Rd, double err_Rp, double rel_gap) {
+    const hprqp_params *p = W.p;
+    if (p->sigma_fixed || r.restart_flag < 1) return;
+    hipStream_t s = W.s;
+    const int32_t n = W.n, m = W.m;
+    const double sigma_old = W.sigma;
+    double h[6];
+    sigma_dots(W, h);
     const double dxdx = h[0], dQwATdy = h[1], dQwdQw = h[2], dwdQw = h[3], ATdyQATdy = h[4], dydy = h[5];
     double a = 0.0, b = dxdx, c = 0.0, d = 0.0;
     if (m > 0) a = W.lambda_max_A * dydy - 2 * dQwATdy;
@@ -1282,10 +1309,10 @@ static void update_sigma(Solver<VT, MT> &W, Restart &r, double err_Rd, double er
     if (W.S.Q_is_diag && std::fabs(sigma_old - W.sigma) > 1e-15) W.update_Q_factors();
 }
 
-// do_restart! (src/algorithm.jl:522-555)
+// the device half of do_restart! (src/algorithm.jl:522-555): the anchors
+// and iterates from the averaged iterates, A^T y_bar
 template <class VT, class MT>
-static void do_restart(Solver<VT, MT> &W, Restart &r) {
-    if (r.restart_flag <= 0) return;
+static void restart_vectors(Solver<VT, MT> &W) {
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
     if (m > 0) {
@@ -1318,6 +1345,13 @@ static void do_restart(Solver<VT, MT> &W, Restart &r) {
             }
         });
     }
+}
+
+// do_restart! (src/algorithm.jl:522-555)
+template <class VT, class MT>
+static void do_restart(Solver<VT, MT> &W, Restart &r) {
+    if (r.restart_flag <= 0) return;
+    restart_vectors(W);
     r.last_gap = r.current_gap;
     r.save_gap = INFINITY;
     r.times++;
@@ -2097,5 +2131,287 @@ void qop_apply_mixed(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx) 
         }
     }
 }
+
+}  // namespace hprqp
+
+// ---------------------------------------------------------------------
+// step-level workspace: the state of solve() before its loop, with each
+// device operation of the loop as its own call (include/hprqp.h)
+// ---------------------------------------------------------------------
+struct hprqp_ws {
+    hprqp_ws_info info{};
+    virtual ~hprqp_ws() = default;
+    virtual hprqp_ctx *ctx() const = 0;
+    virtual void step(int kind, const hprqp_step_params *sp) = 0;
+    virtual void kkt(int64_t iter, hprqp_kkt_out *out) = 0;
+    virtual double mnorm(const hprqp_step_params *sp) = 0;
+    virtual void sigma_dots(double *out) = 0;
+    virtual void q_factors(double sigma) = 0;
+    virtual void restart() = 0;
+    virtual void collect(double *x, double *y, double *z, double *w) = 0;
+    virtual void download(int which, double *host, int64_t len) = 0;
+};
+
+namespace hprqp {
+
+// the half-steps by entry point; kWsMain runs main_update along the path
+enum WsStep {
+    kWsMain = 0,
+    kWsZxw1,
+    kWsTempv,
+    kWsY,
+    kWsW2,
+    kWsNocZxw,
+    kWsNocTempv,
+    kWsLasso,
+    kWsLpZx,
+    kWsLpY
+};
+
+template <class VT, class MT>
+struct WsImpl final : hprqp_ws {
+    hprqp_ctx *c_;
+    const hprqp_model *md;
+    hprqp_params p;
+    Reducer red;
+    Scaled S;
+    std::unique_ptr<Solver<VT, MT>> W;
+    int la_lanes = 0, lat_lanes = 0;  // the LASSO operator's shapes for this workspace
+
+    WsImpl(hprqp_ctx *c, const hprqp_model *m, const hprqp_params *pp) : c_(c), md(m), p(*pp), red(c) {
+        p.save_cb = nullptr;  // a step-level host saves for itself
+        p.auto_save = 0;
+    }
+
+    hprqp_ctx *ctx() const override { return c_; }
+
+    // the shared LASSO operator carries its row-product shapes; a workspace
+    // sets its own for each call and puts the previous ones back after it
+    struct Lanes {
+        WsImpl *w;
+        OperatorLanesGuard g;
+        explicit Lanes(WsImpl *ws) : w(ws), g(ws->md->Q) {
+            if (w->S.qkind == QOP_LASSO) {
+                w->S.qop->la.lanes = w->la_lanes;
+                w->S.qop->lat.lanes = w->lat_lanes;
+            }
+        }
+    };
+
+    // solve() up to its loop (src/algorithm.jl:1904-1976)
+    void create(const double *x0, const double *y0) {
+        OperatorLanesGuard guard(md->Q);
+        const double setup_start = now_s();
+        build_scaled(c_, md, &p, red, S);
+        check_Q_diagonal(red, S);
+        apply_lanes(c_->stream, md, &p, S, p.precision != HPRQP_PREC_FP64);
+        if (S.qkind == QOP_LASSO) {
+            la_lanes = S.qop->la.lanes;
+            lat_lanes = S.qop->lat.lanes;
+        }
+        W.reset(new Solver<VT, MT>(c_, S, red, &p));
+        Solver<VT, MT> &D = *W;
+        init_solver(D, md);
+        double sigma0 = 1.0;
+        if (p.sigma == -1) {
+            if (S.norm_c > 1e-16 && S.norm_b > 1e-16 && S.norm_b < 1e16 && S.norm_c < 1e16)
+                sigma0 = S.norm_b / S.norm_c;
+        } else {
+            sigma0 = p.sigma;
+        }
+        D.sigma = sigma0;
+        HPRQP_HIP(hipStreamSynchronize(c_->stream));
+        info.setup_time = now_s() - setup_start;
+        const double t0 = now_s();
+        D.lambda_max_A = md->m > 0 ? D.power_iteration(0, 5000, 1e-4, &info.power_iter_A) * p.eig_factor : 0.0;
+        if (S.qkind != QOP_CSR)
+            D.lambda_max_Q = D.power_iteration(1, 5000, 1e-4, &info.power_iter_Q) * p.eig_factor;
+        else if (!S.q_empty)
+            D.lambda_max_Q =
+                S.Q_is_diag ? S.q_max_val : D.power_iteration(1, 5000, 1e-4, &info.power_iter_Q) * p.eig_factor;
+        else
+            D.lambda_max_Q = 0.0;
+        info.power_time = now_s() - t0;
+        int64_t nfree = 0;
+        for (int32_t i = 0; i < md->n; ++i)
+            if (md->l[i] == -INFINITY && md->u[i] == INFINITY) nfree++;
+        D.noC = (double)nfree > 0.8 * (double)md->n;
+        process_initial_points(D, x0, y0);
+        if (S.Q_is_diag) D.update_Q_factors();
+        HPRQP_HIP(hipStreamSynchronize(c_->stream));
+        info.n = S.n;
+        info.m = S.m;
+        info.Q_is_diag = S.Q_is_diag;
+        info.noC = D.noC;
+        info.path = S.qkind == QOP_LASSO ? HPRQP_PATH_LASSO
+                    : S.q_empty          ? HPRQP_PATH_LP
+                    : D.noC              ? HPRQP_PATH_NOC
+                                         : HPRQP_PATH_STANDARD;
+        info.precision = p.precision;
+        info.lambda_max_A = D.lambda_max_A;
+        info.lambda_max_Q = D.lambda_max_Q;
+        info.sigma0 = sigma0;
+        info.b_scale = S.b_scale;
+        info.c_scale = S.c_scale;
+        info.norm_b = S.norm_b;
+        info.norm_c = S.norm_c;
+    }
+
+    void set_scalars(const hprqp_step_params *sp) {
+        W->sigma = sp->sigma;
+        W->lambda_max_A = sp->lambda_max_A;
+        W->lambda_max_Q = sp->lambda_max_Q;
+    }
+
+    template <bool F, bool DG>
+    void run(int kind, double h1, double h2) {
+        Solver<VT, MT> &D = *W;
+        switch (kind) {
+            case kWsMain: D.template update_impl<F, DG>(h1, h2); break;
+            case kWsZxw1: D.template step_zxw1<F, DG>(h1, h2); break;
+            case kWsTempv: D.step_tempv(); break;
+            case kWsY: D.template step_y<F>(D.tempv.p, h1, h2); break;
+            case kWsW2: D.template step_w2<F, DG>(h1, h2); break;
+            case kWsNocZxw: D.template step_noc_zxw<F, DG>(h1, h2); break;
+            case kWsNocTempv: D.step_noc_tempv(h1, h2); break;
+            case kWsLasso: D.template step_lasso<F>(h1, h2); break;
+            case kWsLpZx: D.template step_lp_zx<F>(h1, h2); break;
+            default: D.template step_y<F>(D.x_hat.p, h1, h2); break;  // kWsLpY
+        }
+    }
+
+    void step(int kind, const hprqp_step_params *sp) override {
+        static const int path_of[] = {-1,
+                                      HPRQP_PATH_STANDARD,
+                                      HPRQP_PATH_STANDARD,
+                                      HPRQP_PATH_STANDARD,
+                                      HPRQP_PATH_STANDARD,
+                                      HPRQP_PATH_NOC,
+                                      HPRQP_PATH_NOC,
+                                      HPRQP_PATH_LASSO,
+                                      HPRQP_PATH_LP,
+                                      HPRQP_PATH_LP};
+        HPRQP_REQUIRE(kind >= kWsMain && kind <= kWsLpY, HPRQP_E_ARG, "unknown half-step");
+        // the y step is shared by the standard and noC paths
+        const bool on_path = kind == kWsMain || path_of[kind] == info.path ||
+                             (kind == kWsY && info.path == HPRQP_PATH_NOC);
+        HPRQP_REQUIRE(on_path, HPRQP_E_STATE, "half-step is not on this problem's path (hprqp_ws_info.path)");
+        HPRQP_REQUIRE(sp->sigma > 0 && sp->h1 >= 0 && sp->h1 <= 1, HPRQP_E_ARG, "step scalars out of range");
+        Lanes scope(this);
+        set_scalars(sp);
+        const bool full = sp->full != 0, diag = S.Q_is_diag;
+        if (full)
+            diag ? run<true, true>(kind, sp->h1, sp->h2) : run<true, false>(kind, sp->h1, sp->h2);
+        else
+            diag ? run<false, true>(kind, sp->h1, sp->h2) : run<false, false>(kind, sp->h1, sp->h2);
+    }
+
+    void kkt(int64_t iter, hprqp_kkt_out *o) override {
+        Lanes scope(this);
+        typename Solver<VT, MT>::Resid r;
+        W->compute_residuals(iter, md->obj_constant, p.problem_type == HPRQP_PROBLEM_LASSO, r);
+        o->err_Rp = r.err_Rp;
+        o->err_Rd = r.err_Rd;
+        o->rel_gap = r.rel_gap;
+        o->primal_obj = r.primal_obj;
+        o->dual_obj = r.dual_obj;
+        o->kkt = r.kkt;
+    }
+
+    double mnorm(const hprqp_step_params *sp) override {
+        Lanes scope(this);
+        set_scalars(sp);
+        return W->compute_M_norm();
+    }
+
+    void sigma_dots(double *out) override {
+        Lanes scope(this);
+        hprqp::sigma_dots(*W, out);
+    }
+
+    void q_factors(double sigma) override {
+        HPRQP_REQUIRE(S.Q_is_diag, HPRQP_E_STATE, "Q is not diagonal");
+        HPRQP_REQUIRE(sigma > 0, HPRQP_E_ARG, "sigma must be positive");
+        W->sigma = sigma;
+        W->update_Q_factors();
+    }
+
+    void restart() override {
+        Lanes scope(this);
+        restart_vectors(*W);
+    }
+
+    void collect(double *x, double *y, double *z, double *w) override { hprqp::collect(*W, x, y, z, w); }
+
+    void download(int which, double *host, int64_t len) override {
+        Solver<VT, MT> &D = *W;
+        const VT *src = nullptr;
+        const double *srcd = nullptr;
+        int64_t want = D.n;
+        switch (which) {
+            case HPRQP_VEC_X_BAR: src = D.x_bar.p; break;
+            case HPRQP_VEC_Y_BAR: src = D.y_bar.p, want = D.m; break;
+            case HPRQP_VEC_Z_BAR: src = D.z_bar.p; break;
+            case HPRQP_VEC_W_BAR: src = D.w_bar.p; break;
+            case HPRQP_VEC_ATDY: src = D.ATdy.p; break;
+            case HPRQP_VEC_QATDY: src = D.QATdy.p; break;
+            case HPRQP_VEC_DIAG_Q: srcd = S.diag_Q.p; break;
+            default: throw Error(HPRQP_E_ARG, "unknown vector");
+        }
+        HPRQP_REQUIRE(host != nullptr && len == want, HPRQP_E_ARG, "host buffer length differs from the vector's");
+        if (want == 0) return;
+        if (!srcd) {
+            DevVec<double> t(want);
+            double *tp = t.p;
+            each(c_->stream, want, [=] __device__(int64_t i) { tp[i] = (double)src[i]; });
+            HPRQP_HIP(hipMemcpyAsync(host, t.p, sizeof(double) * want, hipMemcpyDeviceToHost, c_->stream));
+            HPRQP_HIP(hipStreamSynchronize(c_->stream));
+        } else {
+            HPRQP_HIP(hipMemcpyAsync(host, srcd, sizeof(double) * want, hipMemcpyDeviceToHost, c_->stream));
+            HPRQP_HIP(hipStreamSynchronize(c_->stream));
+        }
+    }
+};
+
+hprqp_ws *ws_create(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const double *x0,
+                    const double *y0) {
+    HPRQP_REQUIRE(p->check_iter > 0, HPRQP_E_ARG, "check_iter must be positive");
+    HPRQP_REQUIRE(p->sigma == -1 || p->sigma > 0, HPRQP_E_ARG, "sigma must be positive or -1");
+    std::unique_ptr<hprqp_ws> w;
+    switch (p->precision) {
+        case HPRQP_PREC_FP64: {
+            auto *i = new WsImpl<double, double>(ctx, md, p);
+            w.reset(i);
+            i->create(x0, y0);
+            break;
+        }
+        case HPRQP_PREC_MIXED: {
+            auto *i = new WsImpl<double, float>(ctx, md, p);
+            w.reset(i);
+            i->create(x0, y0);
+            break;
+        }
+        case HPRQP_PREC_FP32: {
+            auto *i = new WsImpl<float, float>(ctx, md, p);
+            w.reset(i);
+            i->create(x0, y0);
+            break;
+        }
+        default: throw Error(HPRQP_E_ARG, "unknown precision mode");
+    }
+    return w.release();
+}
+
+void ws_info(const hprqp_ws *w, hprqp_ws_info *info) { *info = w->info; }
+hprqp_ctx *ws_ctx(const hprqp_ws *w) { return w->ctx(); }
+void ws_step(hprqp_ws *w, int kind, const hprqp_step_params *sp) { w->step(kind, sp); }
+void ws_kkt(hprqp_ws *w, int64_t iter, hprqp_kkt_out *out) { w->kkt(iter, out); }
+double ws_mnorm(hprqp_ws *w, const hprqp_step_params *sp) { return w->mnorm(sp); }
+void ws_sigma_dots(hprqp_ws *w, double *out) { w->sigma_dots(out); }
+void ws_q_factors(hprqp_ws *w, double sigma) { w->q_factors(sigma); }
+void ws_restart(hprqp_ws *w) { w->restart(); }
+void ws_collect(hprqp_ws *w, double *x, double *y, double *z, double *wv) { w->collect(x, y, z, wv); }
+void ws_download(hprqp_ws *w, int which, double *host, int64_t len) { w->download(which, host, len); }
+void ws_destroy(hprqp_ws *w) { delete w; }
 
 }  // namespace hprqp

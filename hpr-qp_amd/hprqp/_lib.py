@@ -141,6 +141,55 @@ QOP_SIZE_FN = C.CFUNCTYPE(C.c_int64, C.c_void_p)
 QOP_NAME_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p)
 
 
+class hprqp_ws_info(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("m", C.c_int32),
+        ("Q_is_diag", C.c_int32),
+        ("noC", C.c_int32),
+        ("path", C.c_int32),
+        ("precision", C.c_int32),
+        ("lambda_max_A", C.c_double),
+        ("lambda_max_Q", C.c_double),
+        ("sigma0", C.c_double),
+        ("b_scale", C.c_double),
+        ("c_scale", C.c_double),
+        ("norm_b", C.c_double),
+        ("norm_c", C.c_double),
+        ("power_iter_A", C.c_int64),
+        ("power_iter_Q", C.c_int64),
+        ("setup_time", C.c_double),
+        ("power_time", C.c_double),
+    ]
+
+
+class hprqp_step_params(C.Structure):
+    _fields_ = [
+        ("sigma", C.c_double),
+        ("h1", C.c_double),
+        ("h2", C.c_double),
+        ("lambda_max_A", C.c_double),
+        ("lambda_max_Q", C.c_double),
+        ("full", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class hprqp_kkt_out(C.Structure):
+    _fields_ = [
+        ("err_Rp", C.c_double),
+        ("err_Rd", C.c_double),
+        ("rel_gap", C.c_double),
+        ("primal_obj", C.c_double),
+        ("dual_obj", C.c_double),
+        ("kkt", C.c_double),
+    ]
+
+
+PATH_STANDARD, PATH_NOC, PATH_LASSO, PATH_LP = 0, 1, 2, 3
+VEC_X_BAR, VEC_Y_BAR, VEC_Z_BAR, VEC_W_BAR, VEC_ATDY, VEC_QATDY, VEC_DIAG_Q = range(7)
+
+
 class hprqp_qop_vtable(C.Structure):
     _fields_ = [
         ("apply", QOP_APPLY_FN),
@@ -188,6 +237,26 @@ SIGNATURES = [
     ("hprqp_solve", C.c_int, [_P, _P, C.POINTER(hprqp_params), _P, _P, _P, _P, _P, _P, _P, C.c_int64,
                               C.POINTER(hprqp_result)]),
     ("hprqp_profile_step", C.c_int, [_P, _P, C.POINTER(hprqp_params), C.c_int, _P, C.POINTER(C.c_int)]),
+    ("hprqp_ws_create", C.c_int, [_P, _P, C.POINTER(hprqp_params), _P, _P, C.POINTER(_P),
+                                  C.POINTER(hprqp_ws_info)]),
+    ("hprqp_ws_destroy", C.c_int, [_P, _P]),
+    ("hprqp_step", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_zxw1", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_tempv", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_y", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_w2", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_noc_zxw", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_noc_tempv", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_lasso", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_lp_zx", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_step_lp_y", C.c_int, [_P, _P, C.POINTER(hprqp_step_params)]),
+    ("hprqp_kkt_reduce", C.c_int, [_P, _P, C.c_int64, C.POINTER(hprqp_kkt_out)]),
+    ("hprqp_mnorm", C.c_int, [_P, _P, C.POINTER(hprqp_step_params), C.POINTER(C.c_double)]),
+    ("hprqp_sigma_dots", C.c_int, [_P, _P, _P]),
+    ("hprqp_update_Q_factors", C.c_int, [_P, _P, C.c_double]),
+    ("hprqp_restart", C.c_int, [_P, _P]),
+    ("hprqp_ws_collect", C.c_int, [_P, _P, _P, _P, _P, _P]),
+    ("hprqp_ws_download", C.c_int, [_P, _P, C.c_int, _P, C.c_int64]),
     ("hprqp_batch_create", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, C.c_int64, _P, _P, _P, _P, _P, _P, _P,
                                      _P, _P, C.POINTER(_P)]),
     ("hprqp_batch_destroy", C.c_int, [_P, _P]),

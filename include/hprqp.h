@@ -273,6 +273,107 @@ int hprqp_solve(hprqp_ctx *ctx, const hprqp_model *model,
                 double *x_out, double *y_out, double *z_out, double *w_out,
                 double *hist, int64_t hist_cap, hprqp_result *result);
 
+/* ---- step-level interface: the host runs the iteration loop ------------
+ * HPRQP.jl keeps its own loop (solve, src/algorithm.jl:1904-2039): the
+ * restart tests, the sigma update and the termination tests run on the
+ * host, and each device operation of an iteration is one call below.  A
+ * workspace (HPRQP_workspace_gpu, src/structs.jl:394-459) holds the scaled
+ * problem and every iteration vector on the device.  hprqp_solve runs the
+ * same device operations in the same order, so a host loop that follows
+ * solve()'s cadence reproduces its iterates exactly.  All calls are
+ * asynchronous on the context stream except those returning host data. */
+typedef struct hprqp_ws hprqp_ws;
+
+#define HPRQP_PATH_STANDARD 0 /* zxw1, tempv (m > 0), y, w2 */
+#define HPRQP_PATH_NOC 1      /* noc_zxw, noc_tempv, y  (> 80 % free variables) */
+#define HPRQP_PATH_LASSO 2    /* lasso */
+#define HPRQP_PATH_LP 3       /* lp_zx, lp_y  (Q = 0) */
+
+typedef struct {
+    int32_t n, m;
+    int32_t Q_is_diag;    /* check_Q_diagonal after scaling */
+    int32_t noC;          /* > 80 % free variables (unscaled bounds) */
+    int32_t path;         /* HPRQP_PATH_*: the half-steps main_update runs */
+    int32_t precision;    /* the workspace's HPRQP_PREC_* */
+    double lambda_max_A;  /* power method x eig_factor, as solve uses it */
+    double lambda_max_Q;  /* (diagonal Q: max Q_ii, no eig_factor) */
+    double sigma0;        /* src/algorithm.jl:839-851 */
+    double b_scale, c_scale, norm_b, norm_c;
+    int64_t power_iter_A, power_iter_Q;
+    double setup_time;    /* scaling + allocation */
+    double power_time;
+} hprqp_ws_info;
+
+/* scalars of one half-step (main_update_gpu!, src/algorithm.jl:1250-1279):
+ * h1 = 1/(inner + 2), h2 = 1 - h1; full = 1 writes the full set (dx, x_bar,
+ * z_bar, dy, s, ...) that residual and restart steps read */
+typedef struct {
+    double sigma;
+    double h1, h2;
+    double lambda_max_A, lambda_max_Q;
+    int32_t full;
+    int32_t reserved;
+} hprqp_step_params;
+
+/* compute_residuals! (src/algorithm.jl:37-133), in the original space */
+typedef struct {
+    double err_Rp, err_Rd, rel_gap, primal_obj, dual_obj, kkt;
+} hprqp_kkt_out;
+
+/* the vectors hprqp_ws_download returns (scaled space) */
+#define HPRQP_VEC_X_BAR 0   /* n */
+#define HPRQP_VEC_Y_BAR 1   /* m */
+#define HPRQP_VEC_Z_BAR 2   /* n */
+#define HPRQP_VEC_W_BAR 3   /* n */
+#define HPRQP_VEC_ATDY 4    /* n: A^T dy after hprqp_sigma_dots */
+#define HPRQP_VEC_QATDY 5   /* n: Q A^T dy after hprqp_sigma_dots */
+#define HPRQP_VEC_DIAG_Q 6  /* n: diagonal of the scaled Q */
+
+/* Scaling, check_Q_diagonal, row-product shapes (autotuned as in solve),
+ * allocation, power method, sigma_0, the noC decision, then
+ * process_initial_points! with x0 / y0 (optional, original space):
+ * everything solve() does before its loop (src/algorithm.jl:1904-1976). */
+int hprqp_ws_create(hprqp_ctx *ctx, const hprqp_model *model,
+                    const hprqp_params *params, const double *x0,
+                    const double *y0, hprqp_ws **out, hprqp_ws_info *info);
+int hprqp_ws_destroy(hprqp_ctx *ctx, hprqp_ws *ws);
+/* one whole iteration along info.path (main_update_gpu!) */
+int hprqp_step(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+/* the half-steps themselves (src/kernels.jl:261-1426) */
+int hprqp_step_zxw1(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+int hprqp_step_tempv(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+int hprqp_step_y(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+int hprqp_step_w2(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+int hprqp_step_noc_zxw(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+int hprqp_step_noc_tempv(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+int hprqp_step_lasso(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+int hprqp_step_lp_zx(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+int hprqp_step_lp_y(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp);
+/* residuals of the current (x_bar, y_bar, z_bar, s) at iteration `iter`
+ * (iter 0 adds the bound violation, src/algorithm.jl:118-124); the one
+ * readback of a check step */
+int hprqp_kkt_reduce(hprqp_ctx *ctx, hprqp_ws *ws, int64_t iter,
+                     hprqp_kkt_out *out);
+/* compute_M_norm! (src/algorithm.jl:689-739) of the current (dx, dy, dw)
+ * with sp's sigma and lambdas */
+int hprqp_mnorm(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp,
+                double *out);
+/* the device half of update_sigma! (src/algorithm.jl:323-360): dx, dw, dy,
+ * A^T dy from the anchors, Q dw, Q A^T dy, then out[6] = <dx,dx>,
+ * <Q dw, A^T dy>, <Q dw, Q dw>, <dw, Q dw>, <A^T dy, Q A^T dy>, <dy,dy> */
+int hprqp_sigma_dots(hprqp_ctx *ctx, hprqp_ws *ws, double *out);
+/* unified_update_Q_factors! for a new sigma (diagonal Q only) */
+int hprqp_update_Q_factors(hprqp_ctx *ctx, hprqp_ws *ws, double sigma);
+/* the device half of do_restart! (src/algorithm.jl:522-555): anchors and
+ * iterates from the averaged iterates, A^T y_bar */
+int hprqp_restart(hprqp_ctx *ctx, hprqp_ws *ws);
+/* collect_results! (src/algorithm.jl:611-643): x, y, z, w in the original
+ * space (each optional) */
+int hprqp_ws_collect(hprqp_ctx *ctx, hprqp_ws *ws, double *x, double *y,
+                     double *z, double *w);
+int hprqp_ws_download(hprqp_ctx *ctx, hprqp_ws *ws, int which, double *host,
+                      int64_t len);
+
 /* ---- batched solves (many instances, one launch per check interval) ----
  * Instance i: Q_i = diag(d_i) + U_i U_i^T (U_i n_i x k_i column-major),
  * sparse A_i (m_i x n_i CSR), c_i, l_i, u_i, AL_i, AU_i.  Arrays are packed

@@ -208,7 +208,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     structs = {"hprqp_params": _lib.hprqp_params, "hprqp_result": _lib.hprqp_result,
-               "hprqp_save_state": _lib.hprqp_save_state}
+               "hprqp_save_state": _lib.hprqp_save_state, "hprqp_ws_info": _lib.hprqp_ws_info,
+               "hprqp_step_params": _lib.hprqp_step_params, "hprqp_kkt_out": _lib.hprqp_kkt_out}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "hprqp.h"', "int main(void) {"]
     for name, cls in structs.items():
         lines.append(f'printf("{name} sizeof %zu\\n", sizeof({name}));')

@@ -1,8 +1,9 @@
 """The Julia ccall binding printed in INTEGRATION.md, checked as text against
 include/hprqp.h (Julia is not installed here, so the binding is never run):
 
-- CParams, CResult and SaveState list the fields of hprqp_params,
-  hprqp_result and hprqp_save_state in order, each with a Julia type of the
+- CParams, CResult, SaveState, WsInfo, StepParams and KKT list the fields
+  of hprqp_params, hprqp_result, hprqp_save_state, hprqp_ws_info,
+  hprqp_step_params and hprqp_kkt_out in order, each with a Julia type of the
   C field's size;
 - every field the binding assigns on a CParams exists;
 - every ccall names an entry point the header declares, with as many
@@ -47,7 +48,8 @@ def _ctypes_kind(t):
 
 
 @pytest.mark.parametrize("jl,cname", [("CParams", "hprqp_params"), ("CResult", "hprqp_result"),
-                                      ("SaveState", "hprqp_save_state")])
+                                      ("SaveState", "hprqp_save_state"), ("WsInfo", "hprqp_ws_info"),
+                                      ("StepParams", "hprqp_step_params"), ("KKT", "hprqp_kkt_out")])
 def test_julia_structs_mirror_the_header(jl, cname):
     from hprqp import _lib
 
