@@ -242,11 +242,14 @@ def _update_sigma(ws, r, res, sigma, lamA, lamQ, params, diag_Q):
     return new
 
 
-def solve_by_steps(model, params: Optional[HPRQP_parameters] = None, half_steps: bool = False) -> HPRQP_results:
+def solve_by_steps(model, params: Optional[HPRQP_parameters] = None, half_steps: bool = False,
+                   device_model: Optional[DeviceModel] = None) -> HPRQP_results:
     """solve()'s loop (src/algorithm.jl:1977-2039) on the host over the step
-    ABI.  Returns the same HPRQP_results as optimize()."""
+    ABI.  Returns the same HPRQP_results as optimize().  device_model, when
+    given, is used instead of uploading `model` again; it then shares the
+    row-product shapes the autotuner cached in it with its own solves."""
     params = params or HPRQP_parameters()
-    dm = DeviceModel(model, params.device_number)
+    dm = device_model or DeviceModel(model, params.device_number)
     ws = Workspace(dm, params)
     try:
         info = ws.info
@@ -299,4 +302,5 @@ def solve_by_steps(model, params: Optional[HPRQP_parameters] = None, half_steps:
                              noC=bool(info.noC), power_iter_A=info.power_iter_A, power_iter_Q=info.power_iter_Q)
     finally:
         ws.close()
-        dm.close()
+        if device_model is None:
+            dm.close()

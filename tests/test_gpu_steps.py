@@ -3,7 +3,8 @@ hprqp_mnorm, hprqp_sigma_dots, hprqp_restart): solve()'s loop run on the host
 (hprqp.steploop, the restart and sigma logic of src/algorithm.jl:323-555 in
 Python) over the device operations must give hprqp_solve's result bit for
 bit -- the same iteration count, restarts, final sigma, and identical x and
-y -- on every path main_update takes (standard, noC with diagonal Q, LASSO,
+y, with the row-product shapes the autotuner chose for the model shared --
+on every path main_update takes (standard, noC with diagonal Q, LASSO,
 LP, the QAP operator, no constraints), both through hprqp_step and through
 the individual half-step entry points."""
 import ctypes as C
@@ -48,13 +49,20 @@ def _model(case):
 @pytest.mark.parametrize("half_steps", [False, True], ids=["hprqp_step", "half_steps"])
 @pytest.mark.parametrize("case", ["random", "aug2d_noC_diagQ", "lasso", "qap_esc64a", "lp", "box_m0"])
 def test_host_loop_reproduces_solve_bit_for_bit(case, half_steps):
-    from hprqp import HPRQP_parameters, optimize
+    from hprqp import HPRQP_parameters
+    from hprqp.solver import DeviceModel
     from hprqp.steploop import solve_by_steps
 
     model, ptype = _model(case)
     params = HPRQP_parameters(verbose=False, problem_type=ptype)
-    ref = optimize(model, params)
-    got = solve_by_steps(model, params, half_steps=half_steps)
+    # one uploaded model: the row-product shapes the autotuner picks (by
+    # timing) are cached in it, so both solves sum in the same order
+    dm = DeviceModel(model)
+    try:
+        ref = dm.solve(params)
+        got = solve_by_steps(model, params, half_steps=half_steps, device_model=dm)
+    finally:
+        dm.close()
     print(f"{case}: solve iter={ref.iter} restarts={ref.restarts} sigma={ref.sigma!r}; host loop iter={got.iter} "
           f"restarts={got.restarts} sigma={got.sigma!r} obj {got.primal_obj:.12e}")
     assert got.status == ref.status == "OPTIMAL"
@@ -70,13 +78,13 @@ def test_workspace_info_and_path_checks():
     """hprqp_ws_info reports what solve decides before its loop; a half-step
     that is not on the problem's path is refused with HPRQP_E_STATE, and a
     workspace is tied to its context."""
-    from hprqp import HPRQP_parameters, _lib, optimize
+    from hprqp import HPRQP_parameters, _lib
     from hprqp.solver import DeviceModel
     from hprqp.steploop import Workspace
 
     model, _ = _model("aug2d_noC_diagQ")
-    ref = optimize(model, HPRQP_parameters(verbose=False))
     dm = DeviceModel(model)
+    ref = dm.solve(HPRQP_parameters(verbose=False))
     ws = Workspace(dm, HPRQP_parameters(verbose=False))
     try:
         info = ws.info
