@@ -5,6 +5,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 #include "hprqp_internal.hpp"
@@ -109,6 +110,35 @@ void upload_csr(hipStream_t s, const HostCsr &h, DevCsr &d) {
 }  // namespace hprqp
 
 // ---------------------------------------------------------------------
+// Static scratch for hprqp_qop_apply_noalloc: a file-scope device array, so
+// that entry point allocates nothing.  Each GPU the library runs on holds
+// its own copy (8 MiB) until the process ends.  In host code the array's
+// name is only a stand-in; its device address on each device comes from
+// hipGetSymbolAddress, resolved once per device when the first context is
+// created there and kept in g_scratch.
+constexpr int64_t kScratchDoubles = int64_t(1) << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// with `device` current: this device's address of the scratch array
+double *scratch_on(int device) {
+    HPRQP_REQUIRE(device >= 0 && device < kMaxDevices, HPRQP_E_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        HPRQP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        HPRQP_REQUIRE(p != nullptr, HPRQP_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch[device] = static_cast<double *>(p);
+    }
+    return g_scratch[device];
+}
+}  // namespace
+
+// ---------------------------------------------------------------------
 template <class F>
 static int guarded(hprqp_ctx *ctx, F &&f) {
     try {
@@ -176,6 +206,7 @@ int hprqp_ctx_create(int device, hprqp_ctx **out) {
         HPRQP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HPRQP_HIP(hipHostMalloc((void **)&c->pinned, sizeof(double) * 256, hipHostMallocDefault));
         c->pinned_count = 256;
+        c->scratch = scratch_on(device);
     });
     if (rc != HPRQP_OK) {
         if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -445,6 +476,21 @@ int hprqp_qop_apply(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x, hprqp_buf 
         HPRQP_REQUIRE(x->bytes >= sizeof(double) * (size_t)q->n && Qx->bytes >= sizeof(double) * (size_t)q->n,
                       HPRQP_E_ARG, "buffer smaller than the operator size");
         hprqp::qop_apply_fp64(ctx, q, (const double *)x->p, (double *)Qx->p);
+    });
+}
+
+int hprqp_qop_apply_noalloc(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x, hprqp_buf *Qx) {
+    CTX_CHECK(ctx);
+    if (!q || !x || !Qx) return HPRQP_E_ARG;
+    return guarded(ctx, [&] {
+        HPRQP_REQUIRE(x->bytes >= sizeof(double) * (size_t)q->n && Qx->bytes >= sizeof(double) * (size_t)q->n,
+                      HPRQP_E_ARG, "buffer smaller than the operator size");
+        const int64_t need = hprqp::qop_scratch_doubles(q);
+        HPRQP_REQUIRE(need <= kScratchDoubles, HPRQP_E_ARG,
+                      "operator temporaries (" + std::to_string(need) +
+                          " doubles) exceed the static scratch of 1048576 doubles");
+        HPRQP_REQUIRE(ctx->scratch != nullptr, HPRQP_E_STATE, "no scratch address for this device");
+        hprqp::qop_apply_fp64(ctx, q, (const double *)x->p, (double *)Qx->p, ctx->scratch);
     });
 }
 

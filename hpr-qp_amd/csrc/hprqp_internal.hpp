@@ -133,6 +133,7 @@ struct hprqp_ctx {
     std::string err;
     double *pinned = nullptr;  // small pinned staging area for scalar readback
     size_t pinned_count = 0;
+    double *scratch = nullptr;  // this device's static scratch array (hprqp_qop_apply_noalloc)
 };
 
 struct hprqp_buf {
@@ -266,14 +267,18 @@ void ws_destroy(hprqp_ws *w);
 // per-kernel timing of the iteration (hprqp_profile_step)
 void profile_step(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, int reps,
                   double *ms_out, int *n_out);
-// operator applies exposed through the ABI (FP64)
-void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx);
+// operator applies exposed through the ABI (FP64).  scratch (optional)
+// replaces the operator's own temporaries (qop_scratch_doubles of them) and
+// then no shape that builds its own buffers is used
+void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx, double *scratch = nullptr);
+int64_t qop_scratch_doubles(const hprqp_qop *q);
 void csr_spmv_fp64(hprqp_ctx *ctx, const DevCsr &a, const double *x, double *y);
 // QAP dense products (hprqp_qap.hip).  qap_prepare builds the MFMA operands
 // from qa/qb/qs/qt (host copies given); qap_apply_f64 is the FP64 product,
 // qap_apply_bf16x3 the split-BF16 three-term product accumulated in FP32.
 void qap_prepare(hipStream_t s, hprqp_qop *q, const double *A, const double *B, const double *S, const double *T);
-void qap_apply_f64(hipStream_t s, const hprqp_qop *q, const double *X, double *QX);
+// tmp (optional) replaces the operator's intermediates (2 n^2 doubles)
+void qap_apply_f64(hipStream_t s, const hprqp_qop *q, const double *X, double *QX, double *tmp = nullptr);
 template <class IT, class OT>
 void qap_apply_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, OT *QX);
 // the reduced-precision operator apply of the MIXED path (FP32 matrix data;

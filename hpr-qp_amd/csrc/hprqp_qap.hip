@@ -86,8 +86,8 @@ void qap_prepare(hipStream_t s, hprqp_qop *q, const double *A, const double *B, 
     }
 }
 
-void qap_apply_f64(hipStream_t s, const hprqp_qop *q, const double *X, double *QX) {
-    qap_launch_f64(s, q, X, EpiStore<double>{QX});
+void qap_apply_f64(hipStream_t s, const hprqp_qop *q, const double *X, double *QX, double *tmp) {
+    qap_launch_f64(s, q, X, EpiStore<double>{QX}, tmp);
 }
 
 template <class IT, class OT>

@@ -1191,12 +1191,15 @@ __global__ __launch_bounds__(256) void k_qapbfl_p2(QapArgs g, const IT *__restri
 }
 
 // Q x with the result handed to epi(i, (Q x)_i): FP64 / split-BF16 x3
+// (tmp, when given, replaces the operator's two n x n intermediates: 2 n^2
+// doubles)
 template <class Epi>
-void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Epi &epi) {
+void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Epi &epi, double *tmp = nullptr) {
     const QapArgs g = qap_args(q);
     const int nt = (g.n + 15) / 16;
     const int64_t nn = (int64_t)g.n * g.n;
-    double *Y = q->qtemp.p, *Z = q->qtemp.p + nn;
+    double *const base = tmp ? tmp : q->qtemp.p;
+    double *Y = base, *Z = base + nn;
     if (g.n >= kQapLdsMin && !g.s_full && !g.t_full) {
         // 64-wide blocks (measured faster than 128-wide up to n = 4096:
         // four blocks per CU hide each other's K-step barriers)

@@ -2076,24 +2076,56 @@ void csr_spmv_fp64(hprqp_ctx *ctx, const DevCsr &a, const double *x, double *y) 
     csr_apply<double, double>(ctx->stream, a, x, EpiStore<double>{y});
 }
 
-void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx) {
+// the temporaries of an operator's FP64 apply: LASSO A x (rows), QAP A X
+// and S X (2 n^2), low rank U^T x (k); CSR and user operators none
+int64_t qop_scratch_doubles(const hprqp_qop *q) {
+    switch (q->kind) {
+        case QOP_LASSO: return q->la.nrow;
+        case QOP_QAP: return 2 * (int64_t)q->nq * q->nq;
+        case QOP_LOWRANK: return q->lr_k;
+        default: return 0;
+    }
+}
+
+// a product into caller-given scratch must not build the chunked shape's
+// segment and partial-sum buffers: that shape gives way to a block per row
+static void csr_apply_noalloc(hipStream_t s, const DevCsr &a, const double *x, double *y) {
+    if (a.lanes == kLanesChunked && a.nchunk == 0)
+        launch_csr<double>(s, kBlock, a.nrow, a.rp.p, a.ci.p, a.v.p, x, EpiStore<double>{y});
+    else
+        csr_apply<double, double>(s, a, x, EpiStore<double>{y});
+}
+
+void qop_apply_fp64(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx, double *scratch) {
     hipStream_t s = ctx->stream;
     switch (q->kind) {
         case QOP_CSR:
-            if (q->q.nnz == 0)
-                HPRQP_HIP(hipMemsetAsync(Qx, 0, sizeof(double) * q->n, s));
-            else
+            if (q->q.nnz == 0) {
+                // a fill kernel rather than a memset node: graph-capture safe
+                each(s, q->n, [=] __device__(int64_t i) { Qx[i] = 0.0; });
+            } else if (scratch) {
+                csr_apply_noalloc(s, q->q, x, Qx);
+            } else {
                 csr_apply<double, double>(s, q->q, x, EpiStore<double>{Qx});
+            }
             break;
-        case QOP_LASSO:
-            csr_apply<double, double>(s, q->la, x, EpiStore<double>{q->ltemp.p});
-            csr_apply<double, double>(s, q->lat, (const double *)q->ltemp.p, EpiStore<double>{Qx});
+        case QOP_LASSO: {
+            double *t = scratch ? scratch : q->ltemp.p;
+            if (scratch) {
+                csr_apply_noalloc(s, q->la, x, t);
+                csr_apply_noalloc(s, q->lat, (const double *)t, Qx);
+            } else {
+                csr_apply<double, double>(s, q->la, x, EpiStore<double>{t});
+                csr_apply<double, double>(s, q->lat, (const double *)t, EpiStore<double>{Qx});
+            }
             break;
+        }
         case QOP_QAP:
-            qap_apply_f64(s, q, x, Qx);
+            qap_apply_f64(s, q, x, Qx, scratch);
             break;
         case QOP_LOWRANK:
-            lowrank_apply<double, double, double>(s, q->n, q->lr_k, q->lr_U.p, q->lr_d.p, x, q->lr_t.p, Qx);
+            lowrank_apply<double, double, double>(s, q->n, q->lr_k, q->lr_U.p, q->lr_d.p, x,
+                                                  scratch ? scratch : q->lr_t.p, Qx);
             break;
         default: {
             int rc = q->vt.apply(q->user, x, Qx, (void *)s);

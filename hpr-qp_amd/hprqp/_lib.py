@@ -230,6 +230,7 @@ SIGNATURES = [
     ("hprqp_qop_destroy", C.c_int, [_P, _P]),
     ("hprqp_qop_apply", C.c_int, [_P, _P, _P, _P]),
     ("hprqp_qop_apply_prec", C.c_int, [_P, _P, _P, _P, C.c_int]),
+    ("hprqp_qop_apply_noalloc", C.c_int, [_P, _P, _P, _P]),
     ("hprqp_qop_info", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_char_p)]),
     ("hprqp_model_create", C.c_int, [_P, C.c_int32, C.c_int32, _P, C.c_int64, _P, _P, _P, C.c_int,
                                      _P, _P, _P, _P, _P, C.c_double, C.c_double, C.POINTER(_P)]),

@@ -240,6 +240,20 @@ int hprqp_qop_destroy(hprqp_ctx *ctx, hprqp_qop *q);
 /* Qmap!(x, Qx, Q): FP64 in, FP64 out, on the context stream */
 int hprqp_qop_apply(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x,
                     hprqp_buf *Qx);
+/* The hprqp_qop_apply product with no allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (HIP graph
+ * capture, tight serving loops).  The operator's temporaries -- LASSO A x
+ * (rows doubles), QAP A X and S X (2 nq^2 doubles), low rank U^T x (k
+ * doubles) -- go to a static device array of 1,048,576 doubles (8 MiB) that
+ * each GPU holds from the first context created on it until the process
+ * ends; operators needing more are refused with HPRQP_E_ARG before any
+ * launch.  Every call on one GPU shares that array (every temporary is fully
+ * written before it is read, so nothing carries over between calls), so
+ * calls that overlap on one GPU -- concurrent streams, contexts or threads --
+ * race.  Enqueues only: no hipMalloc, hipFree, memset node or host
+ * synchronisation. */
+int hprqp_qop_apply_noalloc(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x,
+                            hprqp_buf *Qx);
 /* The same product in the precision the solver's iteration uses:
  * HPRQP_PREC_FP64 as hprqp_qop_apply; HPRQP_PREC_MIXED with FP32 matrix
  * data (CSR, LASSO, low rank) and FP64 accumulation, and for QAP the
