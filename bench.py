@@ -65,6 +65,32 @@ def lasso_step_bytes(nnz, m, n, prec):
     return 2 * nnz * (v + 4) + 4 * (m + n + 2) + v * n + 2 * v * m + 9 * v * n + 16 * n
 
 
+def batch_iter_bytes(n, k, m, nnz_a):
+    """Algorithmic HBM bytes of one partial iteration of one batched instance
+    (mixed: FP32 U, d, c, l, u, A values; FP64 iterates; 4-byte indices):
+    two passes over U (Q w = d.*w + U (U'w) with U'w kept in k-space, then
+    U' x_hat) = 8 n k; the z/x/w1 row pass reads w x ATy last_x (FP64), d c l
+    u (FP32) and writes x x_hat w_bar v = 80 n; U' x_hat reads x_hat = 8 n;
+    the y step reads A's values and indices and gathers v = 8 nnz + 8 n; the
+    w2 step reads A^T's row pointers, indices and values, w w_bar ATy last_w
+    last_ATy and writes w ATy = 4 n + 8 nnz + 56 n; the m-vectors and A U
+    (m x k) are negligible beside these."""
+    return 8.0 * n * k + 156.0 * n + 16.0 * nnz_a
+
+
+def measured_batch_traffic():
+    """HBM bytes of the batched leg's k_batch_interval launches over their
+    algorithmic bytes, from the FETCH_SIZE / WRITE_SIZE counter runs
+    summarised under profiles/ (None when not measured)."""
+    f = ROOT / "profiles" / "portfolio_batch4096_interval_traffic.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_over_algorithmic:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def measured_traffic(workload_name, grid, prec):
     """HBM bytes per launch of the dominant kernel, from the FETCH_SIZE /
     WRITE_SIZE counter runs summarised under profiles/ (None when that
@@ -141,13 +167,10 @@ def batched_leg(count, plain_sample):
     warm = bm.solve(mixed, warm=net)  # prediction on the device, then the solve
     warm_s = time.perf_counter() - t
     t_pred = warm.predict_time
-    # algorithmic HBM bytes of the batched iteration (mixed: FP32 U, d, c, l,
-    # u, A; FP64 iterates): four passes over U (Q w, Q w_bar, each a U^T v
-    # and a U t pass) = 16 n k, plus 216 n of vector streams per iteration
-    nk = np.array([m.n * m.Q.U.shape[1] for m in models], dtype=np.float64)
-    nn = np.array([m.n for m in models], dtype=np.float64)
-    per_it = 16.0 * nk + 216.0 * nn
+    per_it = np.array([batch_iter_bytes(m.n, m.Q.U.shape[1], m.m, m.A.nnz) for m in models])
     cold_bytes = float(np.sum(per_it * cold.iter))
+    warm_bytes = float(np.sum(per_it * warm.iter))
+    ratio = measured_batch_traffic()
     cold_ok = int(np.sum((np.array(cold.status) == "OPTIMAL") & (cold.residuals < 1e-6)))
     warm_ok = int(np.sum((np.array(warm.status) == "OPTIMAL") & (warm.residuals < 1e-6)))
     out = {"workload": f"{count} portfolio QPs, n~U{{200..5000}}, k~U{{10..50}}, 1e-6 relative KKT",
@@ -159,7 +182,19 @@ def batched_leg(count, plain_sample):
            "cold_max_iter": int(cold.iter.max()), "warm_max_iter": int(warm.iter.max()),
            "mlp_inference_s": t_pred, "all_solved": cold_ok == count and warm_ok == count,
            "cold_iteration_hbm_gbs": cold_bytes / (cold.time - cold.power_time) / 1e9,
-           "cold_iteration_bytes": cold_bytes, "cold_lambda_max_s": cold.power_time}
+           "cold_iteration_bytes": cold_bytes, "warm_iteration_bytes": warm_bytes,
+           "cold_lambda_max_s": cold.power_time, "cold_interval_kernel_s": cold.interval_time,
+           "intervals": cold.intervals,
+           "roofline": {"bound": "hbm", "kernel": "k_batch_interval<float> (one workgroup per instance, "
+                                                  "one launch per check interval)",
+                        "achieved": cold_bytes / cold.interval_time / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": cold_bytes / cold.interval_time / 1e9 / HBM_PEAK_GBS,
+                        "traffic": None if ratio is None else ratio * cold_bytes,
+                        "traffic_over_algorithmic": ratio,
+                        "bytes_per_solve": cold_bytes,
+                        "traffic_unit": "HBM bytes per cold solve (over all its interval launches), from the "
+                                        "counter ratio measured on the same workload",
+                        "timing": "HIP events around each check-interval replay of the cold solve"}}
     log("batched: " + json.dumps(out))
     bm.close()
     net.close()

@@ -74,6 +74,10 @@ struct BArgs {
     double *y, *y_bar, *s, *dy;
     double *last_x, *last_w, *last_y, *last_ATy;
     double *pq, *pz, *pq2, *pz2;  // eigenvalue scratch (n-packed and m-packed)
+    // A_i U_i (m_i x k_i row-major, FP64 sums of the iteration's A and U
+    // values), packed at sum_j m_j k_j
+    double *AUk;
+    const int64_t *auoff;
     BCtl *ctl;
     double *rec;
 };
@@ -241,6 +245,7 @@ struct IPtr {
     double *x, *w, *w_bar, *x_hat, *x_bar, *z_bar, *dx, *dw, *Qw, *Qw_bar, *tempv, *ATy, *ATy_bar, *ATdy, *QATdy,
         *dQw;
     double *y, *y_bar, *s, *dy, *last_x, *last_w, *last_y, *last_ATy, *pq, *pz, *pq2, *pz2;
+    const double *AUk;  // A U (m x k row-major)
 };
 template <class MT>
 struct Inst {
@@ -293,23 +298,52 @@ __device__ Inst<MT> inst_of(const BArgs<MT> &a, int b, IPtr<MT> *tab) {
         T.last_y = a.last_y + mo;
         T.pz = a.pz + mo;
         T.pz2 = a.pz2 + mo;
+        T.AUk = a.AUk ? a.AUk + a.auoff[b] : nullptr;
     }
     __syncthreads();
     return I;
 }
 
-// one HPR step of the standard path (Q non-diagonal operator, noC false)
+// The projections of the iteration's n-vectors on the columns of U, kept in
+// k-space for a block's whole check interval: tw = U' w, taty = U' ATy and
+// those of the Halpern anchors, tlw = U' last_w, tlaty = U' last_ATy; tx =
+// U' x_hat of the current step.  They start exact at every interval (the
+// anchors change only at restarts, which happen between intervals) and
+// follow the iteration's own linear updates inside it.
+struct KSpace {
+    double tw[64], taty[64], tlw[64], tlaty[64], tx[64];
+};
+
 template <class MT>
-__device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, double h1, double h2, bool full,
-                      double *t, double *sh) {
+__device__ void bkspace_init(Inst<MT> &I, KSpace &ks, double *sh) {
+    const int32_t n = I.n, k = I.k;
+    lr_tvec(n, k, I.P->U, I.P->w, ks.tw, sh);
+    lr_tvec(n, k, I.P->U, I.P->ATy, ks.taty, sh);
+    lr_tvec(n, k, I.P->U, I.P->last_w, ks.tlw, sh);
+    lr_tvec(n, k, I.P->U, I.P->last_ATy, ks.tlaty, sh);
+}
+
+// One HPR step of the standard path (Q = diag(d) + U U' an operator, noC
+// false; src/kernels.jl:2301-2832) with two passes over U instead of four:
+//  * z/x/w1 takes Q w = d.*w + U tw from tw in k-space and leaves x_hat;
+//    one pass over U then gives tx = U' x_hat;
+//  * w_bar = f1 w + f2 x_hat gives U' w_bar = f1 tw + f2 tx, so
+//    tempv = x_hat + sigma (Q w - Q w_bar)
+//          = x_hat + sigma f2 (d.*(w - x_hat) + U (tw - tx)),
+//    and the y step needs only A tempv = A v + sigma f2 (A U)(tw - tx) with
+//    v = x_hat + sigma f2 d.*(w - x_hat): no pass over U;
+//  * the w2 step's updates of w and ATy are linear in A^T y_bar, whose
+//    projection is U' A^T y_bar = (A U)' y_bar, so tw and taty follow in
+//    k-space.
+template <class MT>
+__device__ void bstep(Inst<MT> &I, KSpace &ks, double sigma, double lamA, double lamQ, double h1, double h2,
+                      bool full, double *sh) {
     const int32_t n = I.n, m = I.m, k = I.k;
-    const double f2s = 1.0 / (1.0 + sigma * lamQ), f1s = 1.0 - f2s;
-    // z/x/w1: Qw = Q w
-    lr_tvec(n, k, I.P->U, I.P->w, t, sh);
+    const double f2s = 1.0 / (1.0 + sigma * lamQ), f1s = 1.0 - f2s, sf2 = sigma * f2s;
+    // z/x/w1: Qw = d.*w + U tw
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
-        const double wi = I.P->w[i], xi = I.P->x[i];
-        const double qw = (double)I.P->d[i] * wi + lr_row(n, k, I.P->U, i, t);
-        I.P->Qw[i] = qw;
+        const double wi = I.P->w[i], xi = I.P->x[i], di = (double)I.P->d[i];
+        const double qw = di * wi + lr_row(n, k, I.P->U, i, ks.tw);
         const double z_raw = xi + sigma * (-qw + I.P->ATy[i] - (double)I.P->cM[i]);
         const double xb = proj(z_raw, (double)I.P->lM[i], (double)I.P->uM[i]);
         const double xh = 2.0 * xb - xi;
@@ -321,22 +355,20 @@ __device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, doubl
         I.P->x[i] = fma(h2, xh, h1 * I.P->last_x[i]);
         I.P->x_hat[i] = xh;
         I.P->w_bar[i] = fma(f1s, wi, f2s * xh);
+        if (m > 0) I.P->tempv[i] = fma(sf2, di * (wi - xh), xh);  // v
     }
     __syncthreads();
+    lr_tvec(n, k, I.P->U, I.P->x_hat, ks.tx, sh);
     if (m > 0) {
-        // tempv = x_hat + sigma (Qw - Q w_bar)
-        lr_tvec(n, k, I.P->U, I.P->w_bar, t, sh);
-        for (int32_t i = threadIdx.x; i < n; i += kBB) {
-            const double qwb = (double)I.P->d[i] * I.P->w_bar[i] + lr_row(n, k, I.P->U, i, t);
-            I.P->tempv[i] = I.P->x_hat[i] + sigma * (I.P->Qw[i] - qwb);
-        }
-        __syncthreads();
-        // y step: A tempv
+        // y step: A tempv = A v + sigma f2 (A U)(tw - tx)
         csr_rows_wave(m, I.P->arp, I.P->aci, I.P->av, I.P->tempv, I.P->pz);
         const double fa = lamA * sigma, fb = 1.0 / fa;
         for (int32_t r = threadIdx.x; r < m; r += kBB) {
+            const double *au = I.P->AUk + (int64_t)r * k;
+            double corr = 0.0;
+            for (int32_t j = 0; j < k; ++j) corr += au[j] * (ks.tw[j] - ks.tx[j]);
             const double yi = I.P->y[r];
-            const double s_raw = I.P->pz[r] - fa * yi;
+            const double s_raw = fma(sf2, corr, I.P->pz[r]) - fa * yi;
             const double sp = proj(s_raw, (double)I.P->ALM[r], (double)I.P->AUM[r]);
             const double yb = fb * (sp - s_raw);
             if (full) {
@@ -362,6 +394,15 @@ __device__ void bstep(Inst<MT> &I, double sigma, double lamA, double lamQ, doubl
             I.P->ATdy[i] = atyb - atyi;
             I.P->ATy_bar[i] = atyb;
         }
+    }
+    // the same updates in k-space: ta = (A U)' y_bar = U' A^T y_bar
+    for (int32_t j = threadIdx.x; j < k; j += kBB) {
+        double ta = 0.0;
+        for (int32_t r = 0; r < m; ++r) ta += I.P->AUk[(int64_t)r * k + j] * I.P->y_bar[r];
+        const double twj = ks.tw[j], tatyj = ks.taty[j];
+        const double twbn = fma(f1s, twj, f2s * ks.tx[j]) + fs * (ta - tatyj);
+        ks.tw[j] = h1 * ks.tlw[j] + h2 * (2.0 * twbn - twj);
+        ks.taty[j] = h1 * ks.tlaty[j] + h2 * (2.0 * ta - tatyj);
     }
     __syncthreads();
 }
@@ -548,6 +589,7 @@ template <class MT>
 __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
     __shared__ double t[64];
     __shared__ double sh[kBW];
+    __shared__ KSpace ks;
     const int b = a.order[blockIdx.x];
     const BCtl c = a.ctl[b];
     if (!c.active) return;
@@ -555,6 +597,7 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
     Inst<MT> I = inst_of(a, b, &tab);
     double *rec = a.rec + (int64_t)b * R_COUNT;
     if (c.restart) brestart(I);
+    bkspace_init(I, ks, sh);
     if (threadIdx.x == 0) rec[R_DONE] = -1.0;
     double inner = c.h_inner;
     for (int s = 0; s < c.K; ++s) {
@@ -562,7 +605,7 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
         const bool print_yes = next % (a.print_freq > 0 ? a.print_freq : bprint_step(next)) == 0;
         const bool full = (s == c.K - 1) || (s == 0 && c.restart) || (next % c.check_iter == 0) || print_yes;
         const double h1 = 1.0 / (inner + 2.0);
-        bstep(I, c.sigma, c.lamA, c.lamQ, h1, 1.0 - h1, full, t, sh);
+        bstep(I, ks, c.sigma, c.lamA, c.lamQ, h1, 1.0 - h1, full, sh);
         if (s == 0 && c.want_last_gap) {
             const double g = bmnorm(I, c.sigma, c.lamA, c.lamQ, t, sh);
             if (threadIdx.x == 0) rec[R_LASTGAP] = g;
@@ -586,6 +629,23 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
     }
     bresid(I, false, rec, t, sh);
     bsigma_dots(I, rec, t, sh);
+}
+
+// A_i U_i of every instance (row-major m_i x k_i), from the A and U values
+// the iteration reads, summed in FP64
+template <class MT>
+__global__ __launch_bounds__(kBB) void k_batch_au(BArgs<MT> a) {
+    const int b = blockIdx.x;
+    const int32_t n = a.n[b], m = a.m[b], k = a.k[b];
+    const int32_t *rp = a.a_rp + a.arp_off[b];
+    const MT *U = a.U + a.uoff[b];
+    double *AU = a.AUk + a.auoff[b];
+    for (int64_t t = threadIdx.x; t < (int64_t)m * k; t += kBB) {
+        const int32_t r = (int32_t)(t / k), j = (int32_t)(t % k);
+        double acc = 0.0;
+        for (int32_t q = rp[r]; q < rp[r + 1]; ++q) acc += (double)a.a_v[q] * (double)U[a.a_ci[q] + (int64_t)j * n];
+        AU[t] = acc;
+    }
 }
 
 // iteration-0 residuals (with the bound-violation term) and warm start
@@ -847,6 +907,7 @@ static double bgolden(double ap, double bp, double cp, double dp) {
 
 struct BatchWork {
     DevVec<double> vn[21], vm[7];
+    DevVec<double> au;  // A_i U_i of every instance
     DevVec<BCtl> ctl;
     DevVec<double> rec;
 };
@@ -907,6 +968,8 @@ static BArgs<MT> make_args(hprqp_batch *B, BatchWork &W) {
     for (int i = 0; i < 21; ++i) *np[i] = W.vn[i].p;
     double **mp[] = {&a.y, &a.y_bar, &a.s, &a.dy, &a.last_y, &a.pz, &a.pz2};
     for (int i = 0; i < 7; ++i) *mp[i] = W.vm[i].p;
+    a.AUk = W.au.p;
+    a.auoff = B->dauoff.p;
     a.ctl = W.ctl.p;
     a.rec = W.rec.p;
     return a;
@@ -940,10 +1003,13 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
         v.alloc(std::max<int64_t>(M, 1));
         HPRQP_HIP(hipMemsetAsync(v.p, 0, sizeof(double) * v.n, s));
     }
+    W.au.alloc(std::max<int64_t>(B->AUtot, 1));
     W.ctl.alloc(cnt);
     W.rec.alloc((int64_t)cnt * R_COUNT);
     HPRQP_HIP(hipMemsetAsync(W.rec.p, 0, sizeof(double) * W.rec.n, s));
     BArgs<MT> a = make_args<MT>(B, W);
+    k_batch_au<MT><<<cnt, kBB, 0, s>>>(a);
+    HPRQP_HIP(hipGetLastError());
     a.tol = p->stoptol;
     a.print_freq = p->print_frequency;
     // pinned staging for the per-instance control and records
@@ -1036,6 +1102,20 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
             if (g) (void)hipGraphDestroy(g);
         }
     } gf{graph, gexec};
+
+    // each replay carries an event pair: the summed spans are the time of
+    // the interval kernels (with the two small copies of the replay)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    HPRQP_HIP(hipEventCreateWithFlags(&ev0, hipEventDisableSystemFence));
+    HPRQP_HIP(hipEventCreateWithFlags(&ev1, hipEventDisableSystemFence));
+    struct EventFree {
+        hipEvent_t a, b;
+        ~EventFree() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } evf{ev0, ev1};
+    double interval_ms = 0.0;
 
     int64_t it = 0, intervals = 0, solved = 0;
     int active = cnt;
@@ -1137,8 +1217,15 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
         while (!(((it + K) % p->check_iter) == 0 || it + K >= p->max_iter)) ++K;
         for (int b = 0; b < cnt; ++b)
             if (hctl[b].active) hctl[b].K = (int32_t)K;
+        HPRQP_HIP(hipEventRecord(ev0, s));
         HPRQP_HIP(hipGraphLaunch(gexec, s));
+        HPRQP_HIP(hipEventRecord(ev1, s));
         HPRQP_HIP(hipStreamSynchronize(s));
+        {
+            float ms = 0.f;
+            HPRQP_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+            interval_ms += ms;
+        }
         ++intervals;
         for (int b = 0; b < cnt; ++b) {
             if (!hctl[b].active) continue;
@@ -1156,6 +1243,7 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
     res->time = bnow() - t_start;
     res->solved = solved;
     res->intervals = intervals;
+    res->interval_time = 1e-3 * interval_ms;
     res->max_iter_reached = it;
     int64_t tot_it = 0;
     for (int b = 0; b < cnt; ++b) {

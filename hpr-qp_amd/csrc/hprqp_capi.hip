@@ -770,6 +770,7 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
             B->noff.resize(count);
             B->moff.resize(count);
             B->uoff.resize(count);
+            B->auoff.resize(count);
             std::vector<int64_t> arp_off(count), atrp_off(count), nnz_off(count);
             int64_t N = 0, M = 0, UT = 0, NZ = 0, ARP = 0, ATRP = 0;
             for (int32_t b = 0; b < count; ++b) {
@@ -778,6 +779,8 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
                 B->noff[b] = N;
                 B->moff[b] = M;
                 B->uoff[b] = UT;
+                B->auoff[b] = B->AUtot;
+                B->AUtot += (int64_t)m[b] * k[b];
                 arp_off[b] = ARP;
                 atrp_off[b] = ATRP;
                 nnz_off[b] = NZ;
@@ -881,6 +884,7 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
             up(B->dnoff, B->noff.data(), count);
             up(B->dmoff, B->moff.data(), count);
             up(B->duoff, B->uoff.data(), count);
+            up(B->dauoff, B->auoff.data(), count);
             up(B->darp_off, arp_off.data(), count);
             up(B->datrp_off, atrp_off.data(), count);
             up(B->U, U, UT);

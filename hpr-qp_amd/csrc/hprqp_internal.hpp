@@ -208,6 +208,9 @@ struct hprqp_batch {
     int64_t Ntot = 0, Mtot = 0, Utot = 0, nnzA = 0;
     std::vector<int32_t> n, m, k;
     std::vector<int64_t> noff, moff, uoff;
+    std::vector<int64_t> auoff;  // A_i U_i (m_i x k_i) at sum_j m_j k_j
+    int64_t AUtot = 0;
+    hprqp::DevVec<int64_t> dauoff;
     std::vector<double> obj_constant, norm_b2, norm_c2, norm_b_inf, norm_c_inf;
     hprqp::DevVec<int32_t> dn, dm, dk;
     hprqp::DevVec<int64_t> dnoff, dmoff, duoff, darp_off, datrp_off;

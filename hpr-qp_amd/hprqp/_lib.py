@@ -131,6 +131,7 @@ class hprqp_batch_result(C.Structure):
         ("max_iter_reached", C.c_int64),
         ("total_iter", C.c_int64),
         ("predict_time", C.c_double),
+        ("interval_time", C.c_double),
     ]
 
 

@@ -32,6 +32,7 @@ class BatchResults:
     intervals: int
     total_iter: int
     predict_time: float = 0.0
+    interval_time: float = 0.0  # seconds of the check-interval launches (HIP events)
 
 
 class BatchModel:
@@ -137,7 +138,8 @@ class BatchModel:
         xs = [x[self.noff[i]:self.noff[i + 1]].copy() for i in range(cnt)]
         ys = [y[self.moff[i]:self.moff[i + 1]].copy() for i in range(cnt)]
         return BatchResults([STATUS.get(int(s), str(int(s))) for s in st], it, obj, kkt, xs, ys, r.time,
-                            r.power_time, r.solved, r.intervals, r.total_iter, r.predict_time)
+                            r.power_time, r.solved, r.intervals, r.total_iter, r.predict_time,
+                            r.interval_time)
 
     def close(self):
         if self.h:

@@ -37,8 +37,9 @@
 extern "C" {
 #endif
 
-#define HPRQP_ABI_VERSION 3  /* 2: hprqp_params gained the auto-save fields;
-                                3: hprqp_batch_create takes the total nnz of A */
+#define HPRQP_ABI_VERSION 4  /* 2: hprqp_params gained the auto-save fields;
+                                3: hprqp_batch_create takes the total nnz of A;
+                                4: hprqp_batch_result gained interval_time */
 
 /* ---- error codes ---------------------------------------------------- */
 #define HPRQP_OK 0
@@ -408,6 +409,9 @@ typedef struct {
     int64_t total_iter;       /* sum of per-instance iteration counts */
     double predict_time;      /* hprqp_batch_solve_warm: seconds computing the
                                  learned warm start (not part of time) */
+    double interval_time;     /* seconds of the check-interval launches (HIP
+                                 events around each replay: the interval
+                                 kernel and its control / record copies) */
 } hprqp_batch_result;
 
 int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n,

@@ -193,7 +193,7 @@ def test_library_exports_every_declared_symbol():
     # the ctypes binding covers every declared entry point
     assert set(_declared_symbols()) <= set(_lib.exported_symbols())
     lib.hprqp_abi_version.restype = C.c_int
-    assert lib.hprqp_abi_version() == 3
+    assert lib.hprqp_abi_version() == 4
 
 
 def test_struct_layouts_match_the_header(tmp_path):
@@ -209,7 +209,8 @@ def test_struct_layouts_match_the_header(tmp_path):
         pytest.skip("gcc not available")
     structs = {"hprqp_params": _lib.hprqp_params, "hprqp_result": _lib.hprqp_result,
                "hprqp_save_state": _lib.hprqp_save_state, "hprqp_ws_info": _lib.hprqp_ws_info,
-               "hprqp_step_params": _lib.hprqp_step_params, "hprqp_kkt_out": _lib.hprqp_kkt_out}
+               "hprqp_step_params": _lib.hprqp_step_params, "hprqp_kkt_out": _lib.hprqp_kkt_out,
+               "hprqp_batch_result": _lib.hprqp_batch_result}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "hprqp.h"', "int main(void) {"]
     for name, cls in structs.items():
         lines.append(f'printf("{name} sizeof %zu\\n", sizeof({name}));')
