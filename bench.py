@@ -36,10 +36,16 @@ HBM_PEAK_GBS = 8000.0
 
 
 def zxw1_bytes(nnz_q, n, prec, bound_streams=2):
-    """Algorithmic HBM bytes of one z/x/w1 launch (partial write set):
-    Q values+indices, row pointers, gathered w, streamed ATy c x, the bound
-    vectors l u that are not a single value (0 or +-inf for every variable
-    is passed as a scalar), the FP64 anchor last_x; written Qw x x_hat w_bar."""
+    """Algorithmic HBM bytes of one z/x/w1 launch (partial write set).
+    fp64 / mixed: Q values+indices, row pointers, gathered w, streamed ATy c x,
+    the bound vectors l u that are not a single value (0 or +-inf for every
+    variable is passed as a scalar), the FP64 anchor last_x; written Qw x x_hat
+    w_bar.  fp32 (the offset form): FP32 Q values + indices, row pointers, the
+    gathered FP32 offset dW, the row's dX dT dW (FP32), the FP64 anchor
+    vectors G0 X0, the FP32 anchor difference D0 and the FP64 bounds that are
+    not one value; written Q dW, dX, dX_hat, dW_bar (FP32)."""
+    if prec == "fp32":
+        return nnz_q * 8 + 4 * (n + 1) + 4 * n + 12 * n + 20 * n + 8 * bound_streams * n + 16 * n
     v = 8 if prec == "fp64" else 4
     return nnz_q * (v + 4) + 4 * (n + 1) + v * n + (3 + bound_streams) * v * n + 8 * n + 4 * v * n
 
@@ -255,7 +261,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=1414)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "fp32"])
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=40)
@@ -381,24 +387,36 @@ def main():
                     "launch_sample": "every 8th partial-step launch of the timed solves carries the event pair",
                     "isolated_kernels_ms": prof}
 
-    # the same workload in the mixed form (FP32 matrices and data, FP64
-    # iterates, anchors and KKT, FP64 polish near the tolerance): solves/s and
-    # its error against the FP64 solves above
-    other = None
+    # the same workload in the reduced-precision forms, with solves/s and the
+    # error against the FP64 solves above: fp32 (FP32 matrices, FP32 offsets
+    # of the iterates from the FP64 Halpern anchors, FP32 accumulation, FP64
+    # anchor products, residuals and KKT) and mixed (FP32 matrices and data on
+    # FP64 iterates with FP64 accumulation, FP64 polish near the tolerance)
+    reduced = {}
     if args.precision == "fp64" and not args.no_mixed:
-        mp = HPRQP_parameters(verbose=False, precision="mixed", problem_type=ptype)
-        dm.solve(mp, hist_cap=16)
-        t = time.perf_counter()
-        mr = [dm.solve(mp, hist_cap=16) for _ in range(max(1, args.steps))]
-        mt = time.perf_counter() - t
-        ok = sum(1 for r in mr if r.status == "OPTIMAL" and r.residuals < 1e-6)
-        m0 = mr[-1]
-        other = {"precision": "mixed", "solves_per_s": ok / mt, "iterations": m0.iter,
-                 "fp64_polish_iterations": m0.polish_iter, "kkt": m0.residuals,
-                 "rel_obj_err_vs_fp64": abs(m0.primal_obj - r0.primal_obj) / (1.0 + abs(r0.primal_obj)),
-                 "rel_x_err_vs_fp64": float(np.max(np.abs(m0.x - r0.x)) / (1.0 + np.max(np.abs(r0.x))))}
-        log(f"mixed: {ok}/{len(mr)} solved, {other['solves_per_s']:.3f} solves/s, iter={m0.iter} "
-            f"(polish {m0.polish_iter}), rel obj err vs fp64 {other['rel_obj_err_vs_fp64']:.2e}")
+        for prec in ("fp32", "mixed"):
+            mp = HPRQP_parameters(verbose=False, precision=prec, problem_type=ptype, time_kernels=True)
+            dm.solve(mp, hist_cap=16)
+            t = time.perf_counter()
+            mr = [dm.solve(mp, hist_cap=16) for _ in range(max(1, args.steps))]
+            mt = time.perf_counter() - t
+            ok = sum(1 for r in mr if r.status == "OPTIMAL" and r.residuals < 1e-6)
+            m0 = mr[-1]
+            kc = sum(r.kernel_calls for r in mr)
+            km = sum(r.kernel_ms * r.kernel_calls for r in mr) / max(1, kc)
+            reduced[prec] = {"precision": prec, "solves_per_s": ok / mt, "vs_fp64": (ok / mt) / value,
+                             "iterations": m0.iter, "iterations_vs_fp64": m0.iter / r0.iter,
+                             "fp64_polish_iterations": m0.polish_iter, "kkt": m0.residuals,
+                             "rel_obj_err_vs_fp64": abs(m0.primal_obj - r0.primal_obj) / (1.0 + abs(r0.primal_obj)),
+                             "rel_x_err_vs_fp64": float(np.max(np.abs(m0.x - r0.x)) / (1.0 + np.max(np.abs(r0.x)))),
+                             "zxw1_kernel_ms": km}
+            if ptype == "QP":
+                zb = zxw1_bytes(model.Q.nnz, model.n, prec, bound_streams(model))
+                reduced[prec]["zxw1_bytes_per_launch"] = zb
+                reduced[prec]["zxw1_hbm_frac"] = zb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS if km > 0 else None
+            log(f"{prec}: {ok}/{len(mr)} solved, {reduced[prec]['solves_per_s']:.3f} solves/s "
+                f"({reduced[prec]['vs_fp64']:.2f}x fp64), iter={m0.iter} (polish {m0.polish_iter}), "
+                f"rel obj err vs fp64 {reduced[prec]['rel_obj_err_vs_fp64']:.2e}, z/x/w1 {km * 1e3:.1f} us")
 
     cpu = None
     if not args.no_cpu:
@@ -444,7 +462,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "mixed": other,
+        "fp32": reduced.get("fp32"),
+        "mixed": reduced.get("mixed"),
         "batched": (None if args.no_batch or args.workload != "grid"
                     else batched_leg(args.batch_count, args.plain_sample)),
     }

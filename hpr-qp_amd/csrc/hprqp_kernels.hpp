@@ -501,6 +501,116 @@ struct EpiLpZx {
 };
 
 // ---------------------------------------------------------------------
+// Offset form of the standard half-steps (FP32 bulk).  The bulk iterates
+// are FP32 offsets from the FP64 Halpern anchors X0 = last_x, W0 = last_w,
+// Y0 = last_y, T0 = last_ATy: dX = x - X0, dW = w - W0, dY = y - Y0,
+// dT = ATy - T0, and likewise for x_hat, w_bar, y_bar, A^T y_bar and
+// tempv.  Between two restarts the anchors and sigma are fixed, so the
+// anchors' products are FP64 vectors formed once per restart:
+// G0 = -Q W0 + T0 - c, H0 = A X0 - lambda_A sigma Y0, and D0 = X0 - W0
+// (FP32: it enters only as f2 D0, and shrinks with the anchors).  The
+// row products act on offsets only (FP32 values, FP32 accumulation), so
+// their rounding shrinks with the offsets as the iteration converges.
+// Full steps also write the FP64 iterates the residuals, the M-norm, the
+// sigma update and restarts read (x_bar, z_bar, dx; y_bar, s, dy; w_bar,
+// dw, ATy_bar, ATdy).  Formulas: update_zxw1 / update_y / update_w2
+// (src/kernels.jl:422-537, 760-862, 878-991) rewritten for offsets.
+// ---------------------------------------------------------------------
+
+// z/x/w1, product Q dW: z = X0 + dX + sigma (G0 - Q dW + dT),
+// x_bar = proj(z), dXh = 2 (x_bar - X0) - dX, dX <- h2 dXh,
+// dWb = f1 dW + f2 (dXh + D0)
+template <bool FULL>
+struct EpiOffZxw1 {
+    float *qd;  // Q dW, read again by the tempv step
+    const float *dW, *dT, *D0;
+    const double *G0, *X0, *l, *u;
+    float *dX, *dXh, *dWb;
+    double *x_bar, *z_bar, *dx;
+    double sigma, f1s, f2s, h2;
+    double lS = 0.0, uS = 0.0;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT q_) const {
+        const double q = (double)q_, dxi = (double)dX[i], x0 = X0[i];
+        qd[i] = (float)q_;
+        const double z_raw = x0 + dxi + sigma * (G0[i] - q + (double)dT[i]);
+        const double xb = proj(z_raw, l ? l[i] : lS, u ? u[i] : uS);
+        const double dxb = xb - x0;
+        const double dxh = 2.0 * dxb - dxi;
+        if constexpr (FULL) {
+            x_bar[i] = xb;
+            z_bar[i] = (xb - z_raw) / sigma;
+            dx[i] = dxb - dxi;
+        }
+        dX[i] = (float)(h2 * dxh);
+        dXh[i] = (float)dxh;
+        dWb[i] = (float)fma(f1s, (double)dW[i], f2s * (dxh + (double)D0[i]));
+    }
+};
+
+// tempv offset: dTv = dXh + sigma (Q dW - Q dWb), product Q dWb
+struct EpiOffTempv {
+    const float *dXh, *qd;
+    float *dTv;
+    double sigma;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT q_) const {
+        dTv[i] = (float)((double)dXh[i] + sigma * ((double)qd[i] - (double)q_));
+    }
+};
+
+// y step, product A dTv: s = H0 + A dTv - lambda_A sigma dY,
+// y_bar = (proj(s) - s) / (lambda_A sigma), dYb = y_bar - Y0,
+// dY <- h2 (2 dYb - dY)
+template <bool FULL>
+struct EpiOffY {
+    const double *H0, *Y0, *AL, *AU;
+    float *dY, *dYb;
+    double *y_bar, *s, *dy;
+    double fact1, fact2, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT a_) const {
+        const double dyi = (double)dY[i];
+        const double s_raw = H0[i] + (double)a_ - fact1 * dyi;
+        const double lo = AL[i];
+        const double sp = proj(s_raw, lo, AU ? AU[i] : lo);  // AU null: AL = AU
+        const double yb = fact2 * (sp - s_raw);
+        const double dyb = yb - Y0[i];
+        if constexpr (FULL) {
+            y_bar[i] = yb;
+            s[i] = sp;
+            dy[i] = dyb - dyi;
+        }
+        dYb[i] = (float)dyb;
+        dY[i] = (float)(h2 * (2.0 * dyb - dyi));
+    }
+};
+
+// w2 step, product A^T dYb = A^T y_bar - T0 (= dTb):
+// dWbn = dWb + fs (dTb - dT), dW <- h2 (2 dWbn - dW), dT <- h2 (2 dTb - dT)
+template <bool FULL>
+struct EpiOffW2 {
+    const double *W0, *T0;
+    float *dW, *dWb, *dT;
+    double *w_bar, *dw, *ATy_bar, *ATdy;
+    double fs, h2;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT a_) const {
+        const double dtb = (double)a_, dwi = (double)dW[i], dti = (double)dT[i];
+        const double dwbn = (double)dWb[i] + fs * (dtb - dti);
+        if constexpr (FULL) {
+            dWb[i] = (float)dwbn;
+            w_bar[i] = W0[i] + dwbn;
+            dw[i] = dwbn - dwi;
+            ATy_bar[i] = T0[i] + dtb;
+            ATdy[i] = dtb - dti;
+        }
+        dW[i] = (float)(h2 * (2.0 * dwbn - dwi));
+        dT[i] = (float)(h2 * (2.0 * dtb - dti));
+    }
+};
+
+// ---------------------------------------------------------------------
 // multi-value reductions: NS sums and NM maxima, two passes
 // ---------------------------------------------------------------------
 template <int NS, int NM, class F>

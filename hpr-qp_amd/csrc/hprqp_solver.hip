@@ -433,6 +433,11 @@ static void randn_stream(uint64_t seed, int64_t count, double *out) {
     }
 }
 
+// residuals of one evaluation (compute_residuals!, src/algorithm.jl:37-133)
+struct Resid {
+    double err_Rp = 0, err_Rd = 0, kkt = 0, primal_obj = 0, rel_gap = 0, dual_obj = 0;
+};
+
 // ---------------------------------------------------------------------
 // Solver<VT>
 // ---------------------------------------------------------------------
@@ -1002,9 +1007,7 @@ struct Solver {
         }
     };
 
-    struct Resid {
-        double err_Rp = 0, err_Rd = 0, kkt = 0, primal_obj = 0, rel_gap = 0, dual_obj = 0;
-    };
+    using Resid = hprqp::Resid;
 
     void compute_residuals(int64_t iter, double obj_constant, bool lasso, Resid &r) {
         q_apply_d(x_bar.p, Qx_d.p);
@@ -1502,8 +1505,7 @@ static void save_copy(hipStream_t s, const T *src, double *dst, int64_t len) {
 
 // at each residual evaluation: keep the iterate if it is the best so far
 template <class VT, class MT>
-static void auto_save_track(Solver<VT, MT> &W, LoopState &L, int64_t iter,
-                            const typename Solver<VT, MT>::Resid &res) {
+static void auto_save_track(Solver<VT, MT> &W, LoopState &L, int64_t iter, const Resid &res) {
     if (L.saved && !(res.kkt < std::max(std::max(L.save_Rp, L.save_Rd), L.save_gap))) return;
     const int32_t n = W.n, m = W.m;
     if (!L.saved) {
@@ -1540,8 +1542,7 @@ static void save_download(hipStream_t s, const T *src, const double *nrm, int64_
 // the original space to the caller's callback (save_state_to_hdf5!,
 // src/algorithm.jl:183-301)
 template <class VT, class MT>
-static void auto_save_write(Solver<VT, MT> &W, LoopState &L, int64_t iter,
-                            const typename Solver<VT, MT>::Resid &res, double elapsed) {
+static void auto_save_write(Solver<VT, MT> &W, LoopState &L, int64_t iter, const Resid &res, double elapsed) {
     const hprqp_params *p = W.p;
     if (!p->save_cb || !L.saved) return;
     hipStream_t s = W.s;
@@ -1594,10 +1595,183 @@ static void auto_save_write(Solver<VT, MT> &W, LoopState &L, int64_t iter,
         std::printf("Warning: auto-save callback failed (code %d) at iteration %lld\n", rc, (long long)iter);
 }
 
-template <class VT, class MT>
-static int run_loop(Solver<VT, MT> &W, LoopState &L, int64_t iter_begin, double t_start, double obj_constant,
-                    bool lasso, double *hist, int64_t hist_cap, bool mixed, typename Solver<VT, MT>::Resid &res,
-                    bool has_res) {
+// ---------------------------------------------------------------------
+// FP32 bulk in offset form (HPRQP_PREC_FP32 on the standard path with a
+// non-diagonal sparse Q): the bulk iterates are FP32 offsets from the FP64
+// Halpern anchors, the row products act on offsets only (FP32 values and
+// accumulation), and the anchors' products are FP64 vectors formed once per
+// restart (see the EpiOff* epilogues).  D, an FP64 Solver, holds the
+// anchors and the FP64 iterates the full steps write, and does the
+// residuals, the M-norm, the sigma update and the restarts exactly as in
+// the FP64 form.
+// ---------------------------------------------------------------------
+
+// G0 = -Q W0 + T0 - c
+struct EpiAnchorG {
+    double *G0;
+    const double *T0, *c;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT acc) const { G0[i] = -(double)acc + T0[i] - c[i]; }
+};
+// H0 = A X0 - fa Y0
+struct EpiAnchorH {
+    double *H0;
+    const double *Y0;
+    double fa;
+    template <class AT>
+    __device__ void operator()(int32_t i, AT acc) const { H0[i] = (double)acc - fa * Y0[i]; }
+};
+
+struct OffsetSolver {
+    Solver<double> &D;
+    hipStream_t s;
+    const hprqp_params *p;
+    Scaled &S;
+    int32_t n, m;
+    double &sigma;
+    DevVec<float> dX, dXh, dW, dWb, dT, qd, dTv, dY, dYb, D0;
+    DevVec<double> G0, H0;
+    KernelTimer *kt = nullptr;
+
+    explicit OffsetSolver(Solver<double> &d) : D(d), s(d.s), p(d.p), S(d.S), n(d.n), m(d.m), sigma(d.sigma) {}
+
+    void alloc() {
+        for (DevVec<float> *v : {&dX, &dXh, &dW, &dWb, &dT, &qd, &dTv}) v->alloc(n);
+        for (DevVec<float> *v : {&dY, &dYb}) v->alloc(m);
+        G0.alloc(n);
+        D0.alloc(n);
+        H0.alloc(m);
+        ensure_v32(S.Q);
+        ensure_v32(S.A);
+        ensure_v32(S.AT);
+    }
+
+    // after the anchors (or sigma) change: the offsets from the new anchors
+    // are zero, and the anchors' products are formed in FP64
+    void reanchor() {
+        auto zero = [&](DevVec<float> &v) {
+            float *q = v.p;
+            each(s, v.n, [=] __device__(int64_t i) { q[i] = 0.f; });
+        };
+        for (DevVec<float> *v : {&dX, &dW, &dT, &dWb, &dY}) zero(*v);
+        csr_apply<double, double>(s, S.Q, (const double *)D.last_w.p, EpiAnchorG{G0.p, D.last_ATy.p, S.c.p});
+        if (m > 0)
+            csr_apply<double, double>(s, S.A, (const double *)D.last_x.p,
+                                      EpiAnchorH{H0.p, D.last_y.p, D.lambda_max_A * sigma});
+        float *d0 = D0.p;
+        const double *x0 = D.last_x.p, *w0 = D.last_w.p;
+        each(s, n, [=] __device__(int64_t i) { d0[i] = (float)(x0[i] - w0[i]); });
+    }
+
+    template <bool FULL>
+    void steps(double h2) {
+        const double f2s = 1.0 / (1.0 + sigma * D.lambda_max_Q), f1s = 1.0 - f2s;
+        EpiOffZxw1<FULL> e;
+        e.qd = qd.p;
+        e.dW = dW.p;
+        e.dT = dT.p;
+        e.G0 = G0.p;
+        e.X0 = D.last_x.p;
+        e.D0 = D0.p;
+        e.l = D.lE;
+        e.u = D.uE;
+        e.lS = D.lS;
+        e.uS = D.uS;
+        e.dX = dX.p;
+        e.dXh = dXh.p;
+        e.dWb = dWb.p;
+        e.x_bar = D.x_bar.p;
+        e.z_bar = D.z_bar.p;
+        e.dx = D.dx.p;
+        e.sigma = sigma;
+        e.f1s = f1s;
+        e.f2s = f2s;
+        e.h2 = h2;
+        const bool timed = kt && !FULL;
+        if (timed) kt->begin();
+        csr_apply<float, float>(s, S.Q, (const float *)dW.p, e);
+        if (timed) kt->end();
+        if (m > 0) {
+            csr_apply<float, float>(s, S.Q, (const float *)dWb.p, EpiOffTempv{dXh.p, qd.p, dTv.p, sigma});
+            EpiOffY<FULL> ey;
+            ey.H0 = H0.p;
+            ey.Y0 = D.last_y.p;
+            ey.AL = S.AL.p;
+            ey.AU = S.all_equal_rows ? nullptr : S.AU.p;
+            ey.dY = dY.p;
+            ey.dYb = dYb.p;
+            ey.y_bar = D.y_bar.p;
+            ey.s = D.s_.p;
+            ey.dy = D.dy.p;
+            ey.fact1 = D.lambda_max_A * sigma;
+            ey.fact2 = 1.0 / ey.fact1;
+            ey.h2 = h2;
+            csr_apply<float, float>(s, S.A, (const float *)dTv.p, ey);
+        }
+        EpiOffW2<FULL> ew;
+        ew.W0 = D.last_w.p;
+        ew.T0 = D.last_ATy.p;
+        ew.dW = dW.p;
+        ew.dWb = dWb.p;
+        ew.dT = dT.p;
+        ew.w_bar = D.w_bar.p;
+        ew.dw = D.dw.p;
+        ew.ATy_bar = D.ATy_bar.p;
+        ew.ATdy = D.ATdy.p;
+        ew.fs = sigma / (1.0 + sigma * D.lambda_max_Q);
+        ew.h2 = h2;
+        csr_apply<float, float>(s, S.AT, (const float *)dYb.p, ew);
+    }
+
+    // main_update_gpu! (src/algorithm.jl:1250-1279) in offset form
+    void main_update(int64_t inner, bool full) {
+        const double h1 = 1.0 / ((double)inner + 2.0), h2 = 1.0 - h1;
+        full ? steps<true>(h2) : steps<false>(h2);
+    }
+    void compute_residuals(int64_t iter, double obj_constant, bool lasso, Resid &r) {
+        D.compute_residuals(iter, obj_constant, lasso, r);
+    }
+    double compute_M_norm() { return D.compute_M_norm(); }
+
+    // hand the iteration to the FP64 form: its bulk vectors from the
+    // anchors and offsets (the full-step vectors are FP64 already)
+    void promote() {
+        auto rebuild = [&](DevVec<double> &v, const DevVec<double> &a, const DevVec<float> &d) {
+            double *o = v.p;
+            const double *ap = a.p;
+            const float *dp = d.p;
+            each(s, v.n, [=] __device__(int64_t i) { o[i] = ap[i] + (double)dp[i]; });
+        };
+        rebuild(D.x, D.last_x, dX);
+        rebuild(D.w, D.last_w, dW);
+        rebuild(D.ATy, D.last_ATy, dT);
+        if (m > 0) rebuild(D.y, D.last_y, dY);
+        HPRQP_HIP(hipStreamSynchronize(s));
+    }
+};
+
+static void update_sigma(OffsetSolver &O, Restart &r, double err_Rd, double err_Rp, double rel_gap) {
+    update_sigma(O.D, r, err_Rd, err_Rp, rel_gap);
+}
+static void do_restart(OffsetSolver &O, Restart &r) {
+    if (r.restart_flag <= 0) return;
+    do_restart(O.D, r);
+    O.reanchor();
+}
+static void auto_save_track(OffsetSolver &O, LoopState &L, int64_t iter, const Resid &res) {
+    auto_save_track(O.D, L, iter, res);
+}
+static void auto_save_write(OffsetSolver &O, LoopState &L, int64_t iter, const Resid &res, double elapsed) {
+    auto_save_write(O.D, L, iter, res, elapsed);
+}
+
+// W is a Solver<VT, MT> or the FP32 offset form (OffsetSolver).  mixed: a
+// reduced-precision run that may hand over to the FP64 polish, on a stall
+// and, with use_switch, once the KKT residual is within polish_switch of the
+// tolerance.
+template <class WS>
+static int run_loop(WS &W, LoopState &L, int64_t iter_begin, double t_start, double obj_constant, bool lasso,
+                    double *hist, int64_t hist_cap, bool mixed, Resid &res, bool has_res, bool use_switch = true) {
     const hprqp_params *p = W.p;
     Restart &r = L.r;
     for (int64_t iter = iter_begin; iter <= p->max_iter; ++iter) {
@@ -1613,18 +1787,22 @@ static int run_loop(Solver<VT, MT> &W, LoopState &L, int64_t iter_begin, double 
             // reduced precision: hand over to the FP64 polish once the
             // FP64-measured KKT residual is within polish_switch of the
             // tolerance (the FP32 data / iterates floor lies near there), or
-            // once the FP32 iteration stops improving it
+            // once the FP32 iteration stops improving it.  The offset form
+            // (use_switch false) has no such floor: its error shrinks with
+            // the distance to the anchors, and on slow grids the FP64
+            // iteration itself runs 40 checks without halving the residual,
+            // so its stall window is 400 checks
             if (mixed && res.kkt >= p->stoptol && (iter % p->check_iter == 0)) {
-                if (res.kkt < p->polish_switch * p->stoptol) {
+                if (use_switch && res.kkt < p->polish_switch * p->stoptol) {
                     L.iter = iter;
                     return -1;
                 }
                 if (res.kkt < 0.5 * L.best_kkt) {
                     L.best_kkt = res.kkt;
                     L.stall = 0;
-                } else if (++L.stall >= (p->polish_stall_checks > 0
-                                             ? p->polish_stall_checks
-                                             : (p->precision == HPRQP_PREC_MIXED ? 100 : 40)) &&
+                } else if (++L.stall >= (p->polish_stall_checks > 0 ? p->polish_stall_checks
+                                         : !use_switch                    ? 400
+                                         : (p->precision == HPRQP_PREC_MIXED ? 100 : 40)) &&
                            res.kkt < 1e-3) {
                     L.iter = iter;
                     return -1;
@@ -1793,10 +1971,29 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     Solver<double>::Resid R;
     int status;
     int64_t polish_from = -1;
+    // FP32 on the standard path with a non-diagonal sparse Q: the offset form
+    const bool offset = p->precision == HPRQP_PREC_FP32 && S.qkind == QOP_CSR && !S.q_empty && !noC &&
+                        !S.Q_is_diag && lasso == false;
+    bool from_D = !mixed;
     if (!mixed) {
         process_initial_points(D, x0, y0);
         if (S.Q_is_diag) D.update_Q_factors();
         status = run_loop(D, L, 0, t_start, md->obj_constant, lasso, hist, hist_cap, false, R, false);
+    } else if (offset) {
+        process_initial_points(D, x0, y0);
+        OffsetSolver O(D);
+        O.alloc();
+        O.kt = ktimer.get();
+        O.reanchor();
+        // the offsets carry no fixed FP32 floor: the FP64 polish takes over
+        // only if the KKT residual stalls, not near the tolerance
+        status = run_loop(O, L, 0, t_start, md->obj_constant, lasso, hist, hist_cap, true, R, false, false);
+        if (status == -1) {
+            O.promote();
+            polish_from = L.iter;
+            status = run_loop(D, L, L.iter, t_start, md->obj_constant, lasso, hist, hist_cap, false, R, false);
+        }
+        from_D = true;
     } else {
         // reduced-precision run, then (only if it stalls) the FP64 polish
         // continuing the same iteration from its state
@@ -1836,7 +2033,7 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     }
     HPRQP_HIP(hipStreamSynchronize(ctx->stream));
     res->time = now_s() - t_start;
-    if (!mixed || polish_from >= 0) {
+    if (from_D || polish_from >= 0) {
         collect(D, x_out, y_out, z_out, w_out);
         res->sigma = D.sigma;
     }

@@ -52,8 +52,8 @@ class HPRQP_parameters:
     precision: str = "fp64"      # "fp64" (plain), "mixed" or "fp32"
     spmv_lanes: int = 0          # 0 = automatic
     time_kernels: bool = False   # HIP-event timing of the first fused step kernel
-    polish_stall_checks: int = 0  # checks without KKT halving before FP64 polish (0: 100 mixed, 40 fp32)
-    polish_switch: float = 5.0  # reduced precision: FP64 polish once KKT < polish_switch * stoptol
+    polish_stall_checks: int = 0  # checks without KKT halving before FP64 polish (0: 100 mixed, 40 fp32, 400 fp32 offset form)
+    polish_switch: float = 5.0  # mixed / plain fp32: FP64 polish once KKT < polish_switch * stoptol
 
     def to_c(self) -> _lib.hprqp_params:
         p = _lib.hprqp_params()

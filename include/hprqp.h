@@ -66,7 +66,13 @@ extern "C" {
                                byte streams); FP64 iterates, anchors,
                                accumulation and KKT */
 #define HPRQP_PREC_FP32 2   /* FP32 matrices, iterates and accumulation; FP64
-                               anchors and KKT; FP64 polish when FP32 stalls */
+                               anchors and KKT; FP64 polish when FP32 stalls.
+                               On the standard path with a non-diagonal sparse
+                               Q the bulk iterates are FP32 offsets from the
+                               FP64 Halpern anchors, and the anchors' products
+                               are FP64, formed once per restart: that form
+                               tracks the FP64 iteration to the tolerance
+                               and switches to FP64 only on a long stall */
 
 typedef struct hprqp_ctx hprqp_ctx;
 typedef struct hprqp_buf hprqp_buf;
@@ -123,11 +129,13 @@ typedef struct {
     int32_t polish_stall_checks; /* reduced precision: residual checks without
                                     the KKT residual halving before the FP64
                                     polish takes over (0 = automatic: 100 for
-                                    HPRQP_PREC_MIXED, 40 for HPRQP_PREC_FP32) */
+                                    HPRQP_PREC_MIXED, 40 for HPRQP_PREC_FP32,
+                                    400 for its offset form) */
     double polish_switch;        /* reduced precision: the FP64 polish also
                                     takes over once the KKT residual drops
                                     below polish_switch * stoptol (default 5;
-                                    0 = only on a stall) */
+                                    0 = only on a stall; not used by the FP32
+                                    offset form) */
     int32_t auto_save;           /* 0; 1 = track the best iterate and hand it
                                     with the current one to save_cb at print
                                     steps and at termination (auto_save) */

@@ -7,7 +7,8 @@ about 12 minutes).  The bar is the one of test_gpu_parity: OPTIMAL at 1e-6
 relative KKT, objective within 1e-6 relative of the oracle, iterations within
 10 %, lambda_max within 1e-3 and the power method stopping on the oracle's
 iteration.  The reduced-precision forms are held to 1e-6 KKT and an objective
-within 1e-6 of the FP64 solve."""
+within 1e-6 of the FP64 solve; the FP32 offset form (FP32 offsets from FP64
+anchors) also to FP64's iteration count within 10 % and x within 1e-6."""
 import time
 
 import numpy as np
@@ -37,6 +38,16 @@ def _reduced(model, fp64, problem_type="QP", precision="mixed"):
     return res
 
 
+def _offset_form(model, fp64):
+    """FP32 bulk iterates as offsets from FP64 anchors: tracks the FP64 run."""
+    res = _reduced(model, fp64, precision="fp32")
+    xerr = np.linalg.norm(res.x - fp64.x) / (1.0 + np.linalg.norm(fp64.x))
+    print(f"fp32 offset form: iter {res.iter} vs fp64 {fp64.iter}, polish {res.polish_iter}, rel x err {xerr:.2e}")
+    assert res.iter <= 1.1 * fp64.iter
+    assert xerr < 1e-6
+    return res
+
+
 def test_random_sparse_qp_n10000_m5000_seed0():
     from hprqp import instances
 
@@ -46,6 +57,7 @@ def test_random_sparse_qp_n10000_m5000_seed0():
     _check(res, ref)
     assert not res.Q_is_diag and not res.noC
     _reduced(model, res, precision="mixed")
+    _offset_form(model, res)
 
 
 def test_qap_128_flow_distance():
@@ -71,3 +83,4 @@ def test_grid_qp_400():
     res, ref = _timed_both(model)
     _check(res, ref)
     _reduced(model, res, precision="mixed")
+    _offset_form(model, res)
