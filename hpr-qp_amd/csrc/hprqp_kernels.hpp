@@ -25,6 +25,10 @@ namespace hprqp {
 
 constexpr int kBlock = 256;
 constexpr int kMaxReduceBlocks = 1024;
+// k_pow_* partial pairs one final pass takes: the final block reads them
+// all (1.1 M-row grid QP, 7,810 pairs: 8.1 us, and the fused stream product
+// 6 us slower than the plain one, so that size keeps the separate reduction)
+constexpr int64_t kPowMaxBlocks = 4096;
 
 __device__ __forceinline__ int xcd_block(int b, int nblocks) {
     constexpr int X = 8;
@@ -650,8 +654,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, F f, double *__res
 
 // final pass: one block; out[k] for k < NS + NM
 template <int NS, int NM>
-__global__ __launch_bounds__(kBlock) void k_reduce_final(int nb, const double *__restrict__ part,
-                                                         double *__restrict__ out) {
+__device__ __forceinline__ void reduce_final_body(int nb, const double *__restrict__ part,
+                                                  double *__restrict__ out) {
     constexpr int NT = NS + NM;
     __shared__ double sh[kBlock / 64][NT > 0 ? NT : 1];
     double r[NT > 0 ? NT : 1];
@@ -683,6 +687,182 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(int nb, const double *_
         double a = sh[0][k];
         for (int w = 1; w < kBlock / 64; ++w) a = k < NS ? a + sh[w][k] : dmax(a, sh[w][k]);
         out[k] = a;
+    }
+}
+
+template <int NS, int NM>
+__global__ __launch_bounds__(kBlock) void k_reduce_final(int nb, const double *__restrict__ part,
+                                                         double *__restrict__ out) {
+    reduce_final_body<NS, NM>(nb, part, out);
+}
+
+// the same, with the totals handed to post(tot) on thread 0 through LDS:
+// scalar bookkeeping that would otherwise take a launch of its own.  The
+// totals must not make a round trip through global memory inside the
+// kernel: post's loads from a uniform address are scalar loads, which are
+// not ordered after this kernel's vector stores and would read the old
+// value.  post writes wherever the totals are wanted next.
+template <int NS, int NM, class Post>
+__global__ __launch_bounds__(kBlock) void k_reduce_final_post(int nb, const double *__restrict__ part, Post post) {
+    __shared__ double tot[NS + NM > 0 ? NS + NM : 1];
+    reduce_final_body<NS, NM>(nb, part, tot);
+    __syncthreads();
+    if (threadIdx.x == 0) post((const double *)tot);
+}
+
+// a block's sums of s0 and s1 (every thread's) into dst[0], dst[1], in a
+// fixed order; every thread of the block calls it
+__device__ __forceinline__ void pow_block_partials(double s0, double s1, double *dst) {
+    static_assert(kBlock == 256, "the block reduction sums four waves");
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+    }
+    __shared__ double sh[kBlock / 64][2];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[wave][0] = s0;
+        sh[wave][1] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int q = threadIdx.x;
+        dst[q] = (sh[0][q] + sh[1][q]) + (sh[2][q] + sh[3][q]);
+    }
+}
+
+// Power-method row product with its reduction's first pass: y = M (g x) for
+// the rows of M (L lanes per row, the summation of k_csr), g = *gs when SC
+// and 1 otherwise, and each block's partial sums of y_r (g t_r) and y_r^2 in
+// part[2 b], part[2 b + 1] (b = the block's row range, so the partials are
+// in row order).  g x_j is formed before the product, so y is bitwise the
+// product of M with a stored g x.  Every thread reaches the block reduction.
+template <int L, bool SC>
+__global__ __launch_bounds__(kBlock) void k_pow_csr(int32_t nrows, const int32_t *__restrict__ rp,
+                                                    const int32_t *__restrict__ ci,
+                                                    const double *__restrict__ v, const double *__restrict__ x,
+                                                    const double *__restrict__ t, const double *__restrict__ gs,
+                                                    double *__restrict__ y, double *__restrict__ part) {
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t tid = (int64_t)b * kBlock + threadIdx.x;
+    const int32_t row = (int32_t)(tid / L);
+    const int lane = threadIdx.x & (L - 1);
+    const double g = SC ? *gs : 1.0;
+    double s0 = 0.0, s1 = 0.0;
+    if (row < nrows) {  // whole L-group in or out together
+        const int32_t s = rp[row], e = rp[row + 1];
+        double acc0 = 0.0, acc1 = 0.0;
+        int32_t k = s + lane;
+        for (; k + L < e; k += 2 * L) {
+            const double xa = SC ? x[ci[k]] * g : x[ci[k]];
+            const double xb = SC ? x[ci[k + L]] * g : x[ci[k + L]];
+            acc0 += v[k] * xa;
+            acc1 += v[k + L] * xb;
+        }
+        if (k < e) acc0 += v[k] * (SC ? x[ci[k]] * g : x[ci[k]]);
+        double acc = acc0 + acc1;
+        if constexpr (L > 1) {
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+        }
+        if (lane == 0) {
+            y[row] = acc;
+            s0 = acc * (SC ? t[row] * g : t[row]);
+            s1 = acc * acc;
+        }
+    }
+    pow_block_partials(s0, s1, part + 2 * (int64_t)b);
+}
+
+// the same over the CSR-stream shape (rb rows per block, the summation of
+// k_csr_stream): one partial pair per block of rows
+template <bool SC>
+__global__ __launch_bounds__(kBlock) void k_pow_stream(int32_t nrows, int rb, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci,
+                                                       const double *__restrict__ v, const double *__restrict__ x,
+                                                       const double *__restrict__ t, const double *__restrict__ gs,
+                                                       double *__restrict__ y, double *__restrict__ part) {
+    __shared__ double prod[kStreamCap];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t r0 = b * rb;
+    const int32_t r1 = min(r0 + rb, nrows);
+    const double g = SC ? *gs : 1.0;
+    const int tt = threadIdx.x;
+    const int32_t row = r0 + tt;
+    double s0 = 0.0, s1 = 0.0;
+    // r0 < nrows for every block of the grid (nrows / rb rounded up), so
+    // the block is never empty; all its threads reach the barriers
+    const int32_t base = rp[r0];
+    const int32_t cnt = rp[r1] - base;
+    double acc = 0.0;
+    if (cnt <= kStreamCap) {
+        for (int32_t k0 = 0; k0 < cnt; k0 += kStreamU * kBlock) {
+            double vv[kStreamU];
+            int32_t cc[kStreamU];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) {
+                const int32_t k = k0 + j * kBlock + tt;
+                const int32_t at = base + (k < cnt ? k : 0);
+                vv[j] = v[at];
+                cc[j] = ci[at];
+            }
+            double pr[kStreamU];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) pr[j] = vv[j] * (SC ? x[cc[j]] * g : x[cc[j]]);
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) {
+                const int32_t k = k0 + j * kBlock + tt;
+                if (k < cnt) prod[k] = pr[j];
+            }
+        }
+        __syncthreads();
+        if (row < r1) {
+            const int32_t s = rp[row] - base, e = rp[row + 1] - base;
+            for (int32_t k = s; k < e; ++k) acc += prod[k];
+        }
+    } else if (row < r1) {
+        for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += v[k] * (SC ? x[ci[k]] * g : x[ci[k]]);
+    }
+    if (row < r1) {
+        y[row] = acc;
+        s0 = acc * (SC ? t[row] * g : t[row]);
+        s1 = acc * acc;
+    }
+    pow_block_partials(s0, s1, part + 2 * (int64_t)b);
+}
+
+// the same over the one-block-per-row shape (the summation of
+// k_csr_rowblock): one partial pair per row
+template <bool SC>
+__global__ __launch_bounds__(kBlock) void k_pow_rowblock(int32_t nrows, const int32_t *__restrict__ rp,
+                                                         const int32_t *__restrict__ ci,
+                                                         const double *__restrict__ v, const double *__restrict__ x,
+                                                         const double *__restrict__ t, const double *__restrict__ gs,
+                                                         double *__restrict__ y, double *__restrict__ part) {
+    __shared__ double pw[kBlock / 64];
+    const int32_t row = blockIdx.x;
+    const double g = SC ? *gs : 1.0;
+    const int32_t s = rp[row], e = rp[row + 1];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int32_t k = s + threadIdx.x;
+    for (; k + 3 * kBlock < e; k += 4 * kBlock) {
+        a0 += v[k] * (SC ? x[ci[k]] * g : x[ci[k]]);
+        a1 += v[k + kBlock] * (SC ? x[ci[k + kBlock]] * g : x[ci[k + kBlock]]);
+        a2 += v[k + 2 * kBlock] * (SC ? x[ci[k + 2 * kBlock]] * g : x[ci[k + 2 * kBlock]]);
+        a3 += v[k + 3 * kBlock] * (SC ? x[ci[k + 3 * kBlock]] * g : x[ci[k + 3 * kBlock]]);
+    }
+    for (; k < e; k += kBlock) a0 += v[k] * (SC ? x[ci[k]] * g : x[ci[k]]);
+    double acc = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) pw[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double tot = (pw[0] + pw[1]) + (pw[2] + pw[3]);
+        y[row] = tot;
+        part[2 * (int64_t)row] = tot * (SC ? t[row] * g : t[row]);
+        part[2 * (int64_t)row + 1] = tot * tot;
     }
 }
 

@@ -78,19 +78,22 @@ const float *csr_vals<float>(const DevCsr &a) { return a.v32.p; }
 
 // the column-chunked form: segment starts built on first use (the pattern
 // is fixed for the matrix's life), then the chunk sums and their total
+// the column-chunked shape's segment table, built on its first use
+static void chunk_prepare(hipStream_t s, const DevCsr &a) {
+    if (a.nrow <= 0 || a.nchunk != 0) return;
+    const int32_t nc = (int32_t)((a.ncol + kChunkW - 1) / kChunkW);
+    a.cseg.alloc((int64_t)(nc + 1) * a.nrow);
+    a.cpart.alloc((int64_t)nc * a.nrow);
+    const int64_t cnt = (int64_t)(nc + 1) * a.nrow;
+    k_chunk_seg<<<(unsigned)((cnt + kBlock - 1) / kBlock), kBlock, 0, s>>>(a.nrow, nc, a.rp.p, a.ci.p, a.cseg.p);
+    HPRQP_HIP(hipGetLastError());
+    a.nchunk = nc;
+}
+
 template <class AT, class MT, class XT, class Epi>
 static void launch_chunked(hipStream_t s, const DevCsr &a, const MT *v, const XT *x, const Epi &epi) {
     if (a.nrow <= 0) return;
-    if (a.nchunk == 0) {
-        const int32_t nc = (int32_t)((a.ncol + kChunkW - 1) / kChunkW);
-        a.cseg.alloc((int64_t)(nc + 1) * a.nrow);
-        a.cpart.alloc((int64_t)nc * a.nrow);
-        const int64_t cnt = (int64_t)(nc + 1) * a.nrow;
-        k_chunk_seg<<<(unsigned)((cnt + kBlock - 1) / kBlock), kBlock, 0, s>>>(a.nrow, nc, a.rp.p, a.ci.p,
-                                                                               a.cseg.p);
-        HPRQP_HIP(hipGetLastError());
-        a.nchunk = nc;
-    }
+    chunk_prepare(s, a);
     launch_k(k_csr_chunk<MT, XT, AT>, dim3((a.nrow + kChunkRows - 1) / kChunkRows, a.nchunk), dim3(kBlock), 0, s,
              a.nrow, a.ncol, (const int32_t *)a.cseg.p, a.ci.p, v, x, a.cpart.p);
     HPRQP_HIP(hipGetLastError());
@@ -676,6 +679,52 @@ struct Solver {
         }
     };
 
+    // blocks of the power-method product over a in its row-product shape
+    // (one partial pair each), 0 where no fused form exists (column-chunked
+    // shape, no nonzeros) or its partials are too many for one final pass
+    static int64_t pow_blocks(const DevCsr &a) {
+        if (a.nnz == 0 || a.nrow <= 0) return 0;
+        int64_t nb = 0;
+        if (a.lanes < 0)
+            nb = ((int64_t)a.nrow - a.lanes - 1) / (-a.lanes);
+        else if (a.lanes > 64 && a.lanes != kLanesChunked)
+            nb = a.nrow;
+        else if (a.lanes >= 1 && a.lanes <= 64 && (a.lanes & (a.lanes - 1)) == 0)
+            nb = ((int64_t)a.nrow * a.lanes + kBlock - 1) / kBlock;
+        return nb <= kPowMaxBlocks ? nb : 0;
+    }
+
+    // the power-method product y = a (g x) with its block partials, nb =
+    // pow_blocks(a)
+    template <bool SC>
+    void pow_csr(const DevCsr &a, int64_t nb, const double *x, const double *t, const double *gs, double *y,
+                 double *part) {
+        const int32_t nr = a.nrow;
+        const int32_t *rp = a.rp.p, *ci = a.ci.p;
+        const double *v = a.v.p;
+        const dim3 grid((unsigned)nb), blk(kBlock);
+        if (a.lanes < 0) {
+            k_pow_stream<SC><<<grid, blk, 0, s>>>(nr, -a.lanes, rp, ci, v, x, t, gs, y, part);
+            HPRQP_HIP(hipGetLastError());
+            return;
+        }
+        if (a.lanes > 64) {
+            k_pow_rowblock<SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part);
+            HPRQP_HIP(hipGetLastError());
+            return;
+        }
+        switch (a.lanes) {
+            case 1: k_pow_csr<1, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
+            case 2: k_pow_csr<2, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
+            case 4: k_pow_csr<4, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
+            case 8: k_pow_csr<8, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
+            case 16: k_pow_csr<16, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
+            case 32: k_pow_csr<32, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
+            default: k_pow_csr<64, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
+        }
+        HPRQP_HIP(hipGetLastError());
+    }
+
     // which = 0: A A^T (size m); which = 1: Q (size n).  FP64 throughout.
     // The reference's iteration (src/utils.jl:1357-1376): q = z / |z|,
     // z' = A A^T q (or Q q), lambda = q.z', err = |z' - lambda q| / (|z'| +
@@ -684,8 +733,16 @@ struct Solver {
     // it is), one reduction gives z'.z (or z'.q) and |z'|^2, and
     // |z' - lambda q|^2 is expanded as |z'|^2 - 2 lambda q.z' + lambda^2
     // |q|^2 = |z'|^2 - lambda^2 (2 - |q|^2), with |q|^2 = |z|^2 / |z|^2 as
-    // rounded; |z'|^2 is the next iteration's |z|^2.  Five launches per
-    // iteration instead of eight (no normalising pass, one reduction).
+    // rounded; |z'|^2 is the next iteration's |z|^2.  For A A^T and a
+    // sparse Q the last product also writes its block partials of both sums
+    // (k_pow_csr / k_pow_stream / k_pow_rowblock, the summation of the
+    // solve's own row-product shape; for Q it gathers z scaled by 1/|z| on
+    // the fly, bitwise the product with a stored q) and the final pass does
+    // the scalar update: three launches per iteration for A A^T, two for a
+    // sparse Q.  Other operators take five (product, reduction, scalars).
+    // The host polls every 16 iterations.  (Replaying those 16 from a
+    // hipGraph was slower: the random QP's 197 iterations took 4.1 ms with
+    // the capture and instantiation each call needs, 2.8 ms as launches.)
     double power_iteration(int which, int64_t max_it, double tol, int64_t *its) {
         const int64_t len = which == 0 ? m : n;
         std::vector<double> z0(len);
@@ -694,55 +751,83 @@ struct Solver {
         DevVec<double> za(len), zb(len), q(which == 1 ? len : 0), ATq(which == 0 ? n : 0);
         HPRQP_HIP(hipMemcpyAsync(za.p, z0.data(), sizeof(double) * len, hipMemcpyHostToDevice, s));
         // device scalars: 0 |z|^2, 1 1/|z|, 2 z'.z (A A^T) or z'.q (Q),
-        // 3 |z'|^2, 4 done, 5 lambda at the stop, 6 stop iteration, 7 error.
-        // Nothing is recorded after the converging iteration, so the result
-        // is the reference's (first iteration with err < tol) while the host
-        // polls rarely.
+        // 3 |z'|^2, 4 done, 5 lambda at the stop, 6 stop iteration, 7 error,
+        // 8 iterations run.  Nothing is recorded after the converging
+        // iteration, so the result is the reference's (first iteration with
+        // err < tol) while the host polls rarely.
         double *d = red.out.p + 32;
-        HPRQP_HIP(hipMemsetAsync(d, 0, sizeof(double) * 8, s));
+        HPRQP_HIP(hipMemsetAsync(d, 0, sizeof(double) * 9, s));
         red.enqueue<1, 0>(len, SumSq{za.p}, 32);
         k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { d[1] = 1.0 / sqrt(d[0]); });
         HPRQP_HIP(hipGetLastError());
+        // the fused form: the product whose sums are reduced (A for A A^T,
+        // Q) in a lanes-per-row shape, and few enough blocks for one final
+        // pass over their partials
+        const DevCsr *fm = which == 0 ? &S.A : (S.qkind == QOP_CSR && !S.q_empty ? &S.Q : nullptr);
+        const int64_t fblocks = fm && fm->nrow == len ? pow_blocks(*fm) : 0;
+        DevVec<double> fpart(2 * fblocks);
+        const double tl = tol;
+        const bool scaled = which == 0;
+        // tot = {z'.z (or z'.q), |z'|^2}; d[0], d[1], d[4], d[8] were
+        // written by earlier launches
+        auto post = [=] __device__(const double *tot) {
+            const double zz = d[0], rz = d[1], done = d[4], itn = d[8] + 1.0;
+            const double dot = tot[0], nn = tot[1];
+            const double lam = scaled ? dot * rz : dot;
+            const double qn = zz * rz * rz;
+            const double e2 = fmax(nn - lam * lam * (2.0 - qn), 0.0);
+            if (done == 0.0) {
+                const double err = sqrt(e2) / (sqrt(nn) + lam);
+                d[5] = lam;
+                d[6] = itn;
+                d[7] = err;
+                if (err < tl) d[4] = 1.0;
+            }
+            d[2] = dot;
+            d[3] = nn;
+            d[0] = nn;
+            d[1] = 1.0 / sqrt(nn);
+            d[8] = itn;
+        };
+        double *zo = za.p, *zn = zb.p;
+        auto iteration = [&]() {
+            if (fblocks > 0) {
+                if (which == 0) {
+                    csr_apply<double, double>(s, S.AT, (const double *)zo, EpiStoreScaled<double>{ATq.p, d + 1});
+                    pow_csr<false>(*fm, fblocks, ATq.p, zo, d + 1, zn, fpart.p);
+                } else {
+                    pow_csr<true>(*fm, fblocks, zo, zo, d + 1, zn, fpart.p);
+                }
+                k_reduce_final_post<2, 0><<<1, kBlock, 0, s>>>((int)fblocks, fpart.p, post);
+                HPRQP_HIP(hipGetLastError());
+            } else {
+                if (which == 0) {
+                    csr_apply<double, double>(s, S.AT, (const double *)zo, EpiStoreScaled<double>{ATq.p, d + 1});
+                    csr_apply<double, double>(s, S.A, (const double *)ATq.p, EpiStore<double>{zn});
+                    red.enqueue<2, 0>(len, Dot2Sq{zn, zo}, 34);
+                } else {
+                    double *qq = q.p;
+                    const double *zz = zo;
+                    each(s, len, [=] __device__(int64_t i) { qq[i] = zz[i] * d[1]; });
+                    qop_fp64(q.p, zn);
+                    red.enqueue<2, 0>(len, Dot2Sq{zn, q.p}, 34);
+                }
+                k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { post(d + 2); });
+                HPRQP_HIP(hipGetLastError());
+            }
+            std::swap(zo, zn);
+        };
         constexpr int kPoll = 16;
         double lambda_max = 1.0;
-        double *zo = za.p, *zn = zb.p;
-        for (int64_t it = 1; it <= max_it; ++it) {
-            if (which == 0) {
-                csr_apply<double, double>(s, S.AT, (const double *)zo, EpiStoreScaled<double>{ATq.p, d + 1});
-                csr_apply<double, double>(s, S.A, (const double *)ATq.p, EpiStore<double>{zn});
-                red.enqueue<2, 0>(len, Dot2Sq{zn, zo}, 34);
-            } else {
-                double *qq = q.p;
-                const double *zz = zo;
-                each(s, len, [=] __device__(int64_t i) { qq[i] = zz[i] * d[1]; });
-                qop_fp64(q.p, zn);
-                red.enqueue<2, 0>(len, Dot2Sq{zn, q.p}, 34);
-            }
-            const double itd = (double)it, tl = tol;
-            const bool scaled = which == 0;
-            k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) {
-                const double lam = scaled ? d[2] * d[1] : d[2];
-                const double qn = d[0] * d[1] * d[1];
-                const double e2 = fmax(d[3] - lam * lam * (2.0 - qn), 0.0);
-                if (d[4] == 0.0) {
-                    const double err = sqrt(e2) / (sqrt(d[3]) + lam);
-                    d[5] = lam;
-                    d[6] = itd;
-                    d[7] = err;
-                    if (err < tl) d[4] = 1.0;
-                }
-                d[0] = d[3];
-                d[1] = 1.0 / sqrt(d[3]);
-            });
-            HPRQP_HIP(hipGetLastError());
-            std::swap(zo, zn);
-            if (it % kPoll == 0 || it == max_it) {
-                HPRQP_HIP(hipMemcpyAsync(red.host, d, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
-                HPRQP_HIP(hipStreamSynchronize(s));
-                lambda_max = red.host[5];
-                *its = (int64_t)red.host[6];
-                if (red.host[4] != 0.0) return lambda_max;
-            }
+        for (int64_t it = 0; it < max_it;) {
+            const int64_t chunk = std::min<int64_t>(kPoll, max_it - it);
+            for (int64_t k = 0; k < chunk; ++k) iteration();
+            it += chunk;
+            HPRQP_HIP(hipMemcpyAsync(red.host, d, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
+            HPRQP_HIP(hipStreamSynchronize(s));
+            lambda_max = red.host[5];
+            *its = (int64_t)red.host[6];
+            if (red.host[4] != 0.0) return lambda_max;
         }
         if (p->verbose) std::printf("Power iteration did not converge within %lld iterations\n", (long long)max_it);
         return lambda_max;
