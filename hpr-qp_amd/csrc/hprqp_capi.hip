@@ -190,6 +190,7 @@ void hprqp_params_default(hprqp_params *p) {
     p->time_kernels = 0;
     p->polish_stall_checks = 0;
     p->polish_switch = 5.0;
+    p->qap_recompute_Qw = 0;
 }
 
 int hprqp_ctx_create(int device, hprqp_ctx **out) {

@@ -29,6 +29,7 @@
 #pragma once
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "hprqp_internal.hpp"
 
@@ -220,6 +221,47 @@ __device__ __forceinline__ doublex4 wave_sum1(doublex4 a, double *red) {
     return r;
 }
 
+// Epilogues that set kTileSums return the value they produce (tempv) and
+// carry tsum: the small-n pass 2 then writes each 16 x 16 tile's row sums
+// (tsum[by * n + i]) and column sums (tsum[(nt + bx) * n + j]) of it, from
+// which the next kernel forms A tempv of the doubly stochastic constraints.
+template <class E, class = void>
+struct TileSums : std::false_type {};
+template <class E>
+struct TileSums<E, std::void_t<decltype(E::kTileSums)>> : std::bool_constant<E::kTileSums> {};
+
+// row and column sums of the tile held by wave 0: lane (g = lane / 16,
+// c = lane % 16) has t[e] at column c0 + c and row r0 + g + 4e (ROWS4 false,
+// the FP64 MFMA layout) or r0 + 4g + e (true, the BF16 layout); entries
+// outside the matrix are 0.  Butterfly sums: every lane ends with the same
+// bits.
+template <bool ROWS4>
+__device__ __forceinline__ void qap_tile_sums(const double (&t)[4], int n, int r0, int c0, double *__restrict__ tsum) {
+    const int lane = threadIdx.x & 63, c = lane & 15, gq = lane >> 4;
+    double cs = (t[0] + t[1]) + (t[2] + t[3]);
+    cs += __shfl_xor(cs, 16);
+    cs += __shfl_xor(cs, 32);
+    double rs[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        double v = t[e];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        rs[e] = v;
+    }
+    const int64_t nt = gridDim.x;
+    if (gq == 0 && c0 + c < n) tsum[(nt + blockIdx.x) * n + c0 + c] = cs;
+    if (c == 0) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + (ROWS4 ? 4 * gq + e : gq + 4 * e);
+            if (r < n) tsum[(int64_t)blockIdx.y * n + r] = rs[e];
+        }
+    }
+}
+
 static __global__ __launch_bounds__(256) void k_qap64s_p1(QapArgs g, const double *__restrict__ x,
                                                           double *__restrict__ Y, double *__restrict__ Z) {
     __shared__ double As[4][kSK64 * 16], Ss[4][kSK64 * 16], Xs[4][16 * kSLd64];
@@ -284,20 +326,27 @@ __global__ __launch_bounds__(256) void k_qap64s_p2(QapArgs g, const double *__re
     }
     acc = wave_sum1(acc, red);
     const int j = c0 + li;
-    if (w != 0 || j >= n) return;
+    if (w != 0) return;
+    double tv[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < n) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int r = r0 + lg + 4 * e;
-        if (r >= n) continue;
-        const int64_t o = r + (int64_t)j * n;
-        double v = acc[e];
-        if (g.s_full)
-            v -= Z[o];
-        else
-            v -= g.sd[r] * x[o];
-        if (!g.t_full) v -= x[o] * g.td[j];
-        epi((int32_t)o, 2.0 * v);
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + lg + 4 * e;
+            if (r >= n) continue;
+            const int64_t o = r + (int64_t)j * n;
+            double v = acc[e];
+            if (g.s_full)
+                v -= Z[o];
+            else
+                v -= g.sd[r] * x[o];
+            if (!g.t_full) v -= x[o] * g.td[j];
+            if constexpr (TileSums<Epi>::value)
+                tv[e] = epi((int32_t)o, 2.0 * v);
+            else
+                epi((int32_t)o, 2.0 * v);
+        }
     }
+    if constexpr (TileSums<Epi>::value) qap_tile_sums<false>(tv, n, r0, c0, epi.tsum);
 }
 
 // ---------------- FP64 passes, large n ----------------------------------------
@@ -740,20 +789,27 @@ __global__ __launch_bounds__(256) void k_qapbfs_p2(QapArgs g, const IT *__restri
     }
     acc = wave_sum1f(acc, red);
     const int j = c0 + li;
-    if (w != 0 || j >= n) return;
+    if (w != 0) return;
+    double tv[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < n) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int r = r0 + 4 * lg + e;
-        if (r >= n) continue;
-        const int64_t o = r + (int64_t)j * n;
-        double v = (double)acc[e];
-        if (g.s_full)
-            v -= (double)Z[o];
-        else
-            v -= g.sd[r] * (double)x[o];
-        if (!g.t_full) v -= (double)x[o] * g.td[j];
-        epi((int32_t)o, 2.0 * v);
+        for (int e = 0; e < 4; ++e) {
+            const int r = r0 + 4 * lg + e;
+            if (r >= n) continue;
+            const int64_t o = r + (int64_t)j * n;
+            double v = (double)acc[e];
+            if (g.s_full)
+                v -= (double)Z[o];
+            else
+                v -= g.sd[r] * (double)x[o];
+            if (!g.t_full) v -= (double)x[o] * g.td[j];
+            if constexpr (TileSums<Epi>::value)
+                tv[e] = epi((int32_t)o, 2.0 * v);
+            else
+                epi((int32_t)o, 2.0 * v);
+        }
     }
+    if constexpr (TileSums<Epi>::value) qap_tile_sums<true>(tv, n, r0, c0, epi.tsum);
 }
 
 // ---------------- split-BF16 x3 passes, large n ------------------------------

@@ -26,6 +26,7 @@
 #include "hprqp_internal.hpp"
 #include "hprqp_kernels.hpp"
 #include "hprqp_qap.hpp"
+#include "hprqp_qapds.hpp"
 
 namespace hprqp {
 
@@ -237,7 +238,29 @@ struct Scaled {
     // every constraint row an equality
     bool l_uniform = false, u_uniform = false, all_equal_rows = false;
     double l_value = 0.0, u_value = 0.0;
+    // QAP operator with exactly the relaxation's doubly stochastic
+    // constraints (rows: X 1 = 1, then X^T 1 = 1; every value 1)
+    bool qap_ds = false;
 };
+
+// A is build_from_ABST's constraint matrix [kron(1', I); kron(I, 1')] for
+// the operator's nq (src/utils.jl:1160-1163), values exactly 1
+static bool qap_doubly_stochastic(const hprqp_model *md) {
+    const int32_t nq = md->Q->nq;
+    const HostCsr &A = md->A;
+    if (nq <= 0 || A.nrow != 2 * nq || (int64_t)A.ncol != (int64_t)nq * nq ||
+        (int64_t)A.v.size() != 2 * (int64_t)nq * nq || (int64_t)A.rp.size() != 2 * (int64_t)nq + 1)
+        return false;
+    for (int32_t r = 0; r < 2 * nq; ++r) {
+        if (A.rp[r] != (int64_t)r * nq) return false;
+        for (int32_t t = 0; t < nq; ++t) {
+            const int64_t k = (int64_t)r * nq + t;
+            const int64_t col = r < nq ? r + (int64_t)nq * t : (int64_t)(r - nq) * nq + t;
+            if (A.ci[k] != col || A.v[k] != 1.0) return false;
+        }
+    }
+    return A.rp[2 * nq] == 2 * (int64_t)nq * nq;
+}
 
 static void upload_vec(hipStream_t s, DevVec<double> &d, const std::vector<double> &h) {
     d.alloc((int64_t)h.size());
@@ -314,6 +337,7 @@ static void build_scaled(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_para
     const double *h = red.fetch(4);
     S.norm_b_org = m > 0 ? h[1] : 0.0;
     S.norm_c_org = h[3];
+    S.qap_ds = S.qkind == QOP_QAP && qap_doubly_stochastic(md);
     if (S.qkind != QOP_CSR) {
         S.norm_b = m > 0 ? std::sqrt(h[0]) : 0.0;
         S.norm_c = std::sqrt(h[2]);
@@ -471,6 +495,17 @@ struct Solver {
     double sigma = 1.0, lambda_max_A = 0.0, lambda_max_Q = 0.0, lambda = 0.0;
     bool noC = false;
     KernelTimer *kt = nullptr;  // times the first fused kernel of partial steps
+    // QAP with doubly stochastic constraints (qds): one apply per iteration.
+    // qv_ok: Qw, QATy, last_Qw, last_QATy are the products of w, ATy,
+    // last_w, last_ATy; pend: the last step's second w update is still to
+    // run (it runs fused with the next z/x/w_bar update, or in flush())
+    // qds_small (nq < 256): the y update is pending too and runs in the
+    // fused kernel from pass 2's tile sums (tsum); y_alt is the other buffer
+    // of the y pair
+    bool qds = false, qds_small = false, qv_ok = false, pend = false, pend_full = false, flushed_full = false;
+    double pend_h1 = 0.0, pend_h2 = 0.0, pend_fs = 0.0, pend_fact1 = 1.0;
+    DevVec<VT> QATy, Qwb_new, Qg, y_alt, qdv;
+    DevVec<double> last_QATy, qvec, qBt, qTt, tsum;
     bool timing_now = false;
     void kbegin() {
         if (timing_now) kt->begin();
@@ -553,6 +588,212 @@ struct Solver {
             }
         }
         set_epilogue_bounds();
+        if (S.qap_ds && !p->qap_recompute_Qw) alloc_qds();
+    }
+
+    // the carried products and the constants of Q A^T y (hprqp_qapds.hpp)
+    void alloc_qds() {
+        hprqp_qop *q = S.qop;
+        const int32_t nq = q->nq;
+        const int64_t nn = (int64_t)nq * nq;
+        qds = true;
+        for (DevVec<VT> *v : {&QATy, &Qwb_new, &Qg, &qdv}) {
+            v->alloc(nn);
+            HPRQP_HIP(hipMemsetAsync(v->p, 0, sizeof(VT) * nn, s));
+        }
+        last_QATy.alloc(nn);
+        HPRQP_HIP(hipMemsetAsync(last_QATy.p, 0, sizeof(double) * nn, s));
+        qvec.alloc(8 * (int64_t)nq);
+        auto transpose = [&](DevVec<double> &dst, const double *src) {
+            dst.alloc(nn);
+            double *d = dst.p;
+            each(s, nn, [=] __device__(int64_t o) { d[o] = src[(o / nq) + (o % nq) * (int64_t)nq]; });
+        };
+        transpose(qBt, q->qb.p);
+        if (!q->qt_diag) transpose(qTt, q->qt.p);
+        double *c = qvec.p + 4 * (int64_t)nq;  // rA, cB, rS, cT
+        qap_ds_gemv<double>(s, nq, q->qa.p, qBt.p, q->qs_diag ? nullptr : q->qs.p, q->qt_diag ? nullptr : qTt.p,
+                            nullptr, c, c + nq, c + 2 * nq, c + 3 * nq);
+        qds_small = nq < kQapBlockedMin;
+        if (qds_small) {
+            y_alt.alloc(m);
+            tsum.alloc(2 * (int64_t)((nq + 15) / 16) * nq);
+        }
+        qv_ok = false;
+        pend = false;
+    }
+
+    // the products of the iterates by applies (Q w in the mode's precision,
+    // the anchors' in FP64)
+    void qds_init() {
+        q_apply(w.p, Qw.p);
+        q_apply(ATy.p, QATy.p);
+        qop_fp64(last_w.p, last_Qw.p);
+        qop_fp64(last_ATy.p, last_QATy.p);
+        qv_ok = true;
+    }
+
+    QapDsStep<VT, MT> qds_args() {
+        hprqp_qop *q = S.qop;
+        const int32_t nq = q->nq;
+        const double *v = qvec.p, *c = qvec.p + 4 * (int64_t)nq;
+        QapDsStep<VT, MT> e{};
+        e.n = nq;
+        e.yb = y_bar.p;
+        e.P = v;
+        e.R = v + nq;
+        e.Sa = q->qs_diag ? nullptr : v + 2 * nq;
+        e.Tb = q->qt_diag ? nullptr : v + 3 * nq;
+        e.rA = c;
+        e.cB = c + nq;
+        e.rS = c + 2 * nq;
+        e.cT = c + 3 * nq;
+        e.sd = q->qsd.p;
+        e.td = q->qtd.p;
+        e.gA = q->qa.p;
+        e.gBt = qBt.p;
+        e.gS = q->qs_diag ? nullptr : q->qs.p;
+        e.gTt = q->qt_diag ? nullptr : qTt.p;
+        e.last_w = last_w.p;
+        e.last_ATy = last_ATy.p;
+        e.last_Qw = last_Qw.p;
+        e.last_QATy = last_QATy.p;
+        e.w = w.p;
+        e.w_bar = w_bar.p;
+        e.ATy = ATy.p;
+        e.Qw = Qw.p;
+        e.QATy = QATy.p;
+        e.Qw_bar = Qw_bar.p;
+        e.dw = dw.p;
+        e.ATdy = ATdy.p;
+        e.ATy_bar = ATy_bar.p;
+        e.Qwb_new = Qwb_new.p;
+        e.Qg = Qg.p;
+        e.fs = pend_fs;
+        e.wh1 = pend_h1;
+        e.wh2 = pend_h2;
+        e.c = cV;
+        e.l = lE;
+        e.u = uE;
+        e.lS = lS;
+        e.uS = uS;
+        e.last_x = last_x.p;
+        e.x = x.p;
+        e.x_hat = x_hat.p;
+        e.dx = dx.p;
+        e.x_bar = x_bar.p;
+        e.z_bar = z_bar.p;
+        e.d = qdv.p;
+        e.sigma = sigma;
+        e.f2s = 1.0 / (1.0 + sigma * lambda_max_Q);
+        e.f1s = 1.0 - e.f2s;
+        return e;
+    }
+
+    // the pending y update (EpiY's scalars of its step)
+    QapDsY<VT, MT> qds_yargs() {
+        QapDsY<VT, MT> q{};
+        q.tsum = tsum.p;
+        q.AL = ALV;
+        q.AU = AUE;
+        q.last_y = last_y.p;
+        q.y = y.p;
+        q.y_next = y_alt.p;
+        q.y_bar = y_bar.p;
+        q.s = s_.p;
+        q.dy = dy.p;
+        q.fact1 = pend_fact1;
+        q.fact2 = 1.0 / pend_fact1;
+        q.h1 = pend_h1;
+        q.h2 = pend_h2;
+        return q;
+    }
+
+    void qds_gemv() {
+        hprqp_qop *q = S.qop;
+        const int32_t nq = q->nq;
+        double *v = qvec.p;
+        qap_ds_gemv<VT>(s, nq, q->qa.p, qBt.p, q->qs_diag ? nullptr : q->qs.p, q->qt_diag ? nullptr : qTt.p,
+                        y_bar.p, v, v + nq, v + 2 * nq, v + 3 * nq);
+    }
+
+    // run the pending second w update on its own
+    void flush() {
+        if (!pend) return;
+        QapDsStep<VT, MT> e = qds_args();
+        if (qds_small) {
+            if (pend_full)
+                qap_ds_tile<VT, MT, 2, 0>(s, e, qds_yargs());
+            else
+                qap_ds_tile<VT, MT, 1, 0>(s, e, qds_yargs());
+            std::swap(y, y_alt);
+        } else {
+            qds_gemv();
+            if (pend_full)
+                qap_ds_step<VT, MT, 2, 0>(s, e);
+            else
+                qap_ds_step<VT, MT, 1, 0>(s, e);
+        }
+        flushed_full = pend_full;
+        pend = false;
+    }
+
+    // the iterates were set from outside the iteration: products by applies
+    // at the next step
+    void invalidate_products() {
+        flush();
+        qv_ok = false;
+    }
+
+    // one HPR step of the QAP relaxation: [second w update of the pending
+    // step +] z/x/w_bar update on the carried Q w, then Q w_bar with the
+    // tempv epilogue (the only apply), then the y update; the second w
+    // update is left pending
+    template <bool FULL>
+    void qds_update(double h1, double h2) {
+        if (!qv_ok) {
+            flush();
+            qds_init();
+        }
+        if (pend && pend_full) flush();
+        QapDsStep<VT, MT> e = qds_args();
+        e.h1 = h1;
+        e.h2 = h2;
+        kbegin();
+        if (qds_small) {
+            if (pend) {
+                flushed_full = false;
+                qap_ds_tile<VT, MT, 1, FULL ? 2 : 1>(s, e, qds_yargs());
+                std::swap(y, y_alt);
+            } else {
+                qap_ds_tile<VT, MT, 0, FULL ? 2 : 1>(s, e, qds_yargs());
+            }
+        } else if (pend) {
+            qds_gemv();
+            flushed_full = false;
+            qap_ds_step<VT, MT, 1, FULL ? 2 : 1>(s, e);
+        } else {
+            qap_ds_step<VT, MT, 0, FULL ? 2 : 1>(s, e);
+        }
+        kend();
+        pend = false;
+        // the apply, of d = w - x_hat
+        if (qds_small) {
+            // pass 2 leaves tempv's tile sums; the y update waits for the
+            // next fused kernel
+            EpiTempvDiff<VT, true> et{x_hat.p, Qw.p, Qw_bar.p, nullptr, tsum.p, sigma, e.f2s};
+            q_fused(qdv.p, et, Qw_bar.p);
+        } else {
+            EpiTempvDiff<VT, false> et{x_hat.p, Qw.p, Qw_bar.p, tempv.p, nullptr, sigma, e.f2s};
+            q_fused(qdv.p, et, Qw_bar.p);
+            step_y<FULL>(tempv.p, h1, h2);
+        }
+        pend = true;
+        pend_full = FULL;
+        pend_h1 = h1;
+        pend_h2 = h2;
+        pend_fs = sigma / (1.0 + sigma * lambda_max_Q);
+        pend_fact1 = lambda_max_A * sigma;
     }
 
     void set_epilogue_bounds() {
@@ -1025,6 +1266,10 @@ struct Solver {
 
     template <bool FULL, bool DIAG>
     void update_impl(double h1, double h2) {
+        if (qds) {
+            qds_update<FULL>(h1, h2);
+            return;
+        }
         if (S.qkind == QOP_LASSO) {
             step_lasso<FULL>(h1, h2);
             return;
@@ -1095,6 +1340,7 @@ struct Solver {
     using Resid = hprqp::Resid;
 
     void compute_residuals(int64_t iter, double obj_constant, bool lasso, Resid &r) {
+        flush();
         q_apply_d(x_bar.p, Qx_d.p);
         if (m > 0) {
             csr_apply<double, double>(s, S.AT, (const VT *)y_bar.p, EpiStore<double>{ATdy_d.p});
@@ -1146,6 +1392,7 @@ struct Solver {
     };
 
     double compute_M_norm() {
+        flush();
         q_apply(dw.p, dQw.p);
         if (m > 0) {
             csr_apply<VT, MT>(s, S.AT, (const VT *)dy.p, EpiStore<VT>{ATdy.p});
@@ -1314,6 +1561,7 @@ struct SigmaDots {
 // <dy,dy>
 template <class VT, class MT>
 static void sigma_dots(Solver<VT, MT> &W, double *h) {
+    W.flush();
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
     {
@@ -1401,6 +1649,7 @@ static void update_sigma(Solver<VT, MT> &W, Restart &r, double err_Rd, double er
 // and iterates from the averaged iterates, A^T y_bar
 template <class VT, class MT>
 static void restart_vectors(Solver<VT, MT> &W) {
+    W.flush();
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
     if (m > 0) {
@@ -1433,6 +1682,23 @@ static void restart_vectors(Solver<VT, MT> &W) {
             }
         });
     }
+    // QAP one-apply form: w = w_bar' and ATy = A^T y_bar, whose products the
+    // full step's second w update has formed
+    if (W.qds) {
+        if (W.qv_ok && W.flushed_full) {
+            VT *qw = W.Qw.p, *qaty = W.QATy.p;
+            const VT *qwn = W.Qwb_new.p, *qg = W.Qg.p;
+            double *lqw = W.last_Qw.p, *lqaty = W.last_QATy.p;
+            each(s, n, [=] __device__(int64_t i) {
+                qw[i] = qwn[i];
+                lqw[i] = (double)qwn[i];
+                qaty[i] = qg[i];
+                lqaty[i] = (double)qg[i];
+            });
+        } else {
+            W.qv_ok = false;
+        }
+    }
 }
 
 // do_restart! (src/algorithm.jl:522-555)
@@ -1449,6 +1715,7 @@ static void do_restart(Solver<VT, MT> &W, Restart &r) {
 // process_initial_points! (src/algorithm.jl:1783-1836)
 template <class VT, class MT>
 static void process_initial_points(Solver<VT, MT> &W, const double *x0, const double *y0) {
+    W.invalidate_products();
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
     const double bs = W.S.b_scale, cs = W.S.c_scale;
@@ -1524,6 +1791,8 @@ static int64_t print_step(int64_t k) {
 // copy the FP32 state of a mixed solve into the FP64 solver (polish start)
 template <class VT, class MT>
 static void promote_state(Solver<VT, MT> &F, Solver<double> &D) {
+    F.flush();
+    D.invalidate_products();
     hipStream_t s = D.s;
     auto cp = [&](DevVec<double> &dst, const DevVec<VT> &src) {
         const VT *a = src.p;
@@ -1591,6 +1860,7 @@ static void save_copy(hipStream_t s, const T *src, double *dst, int64_t len) {
 // at each residual evaluation: keep the iterate if it is the best so far
 template <class VT, class MT>
 static void auto_save_track(Solver<VT, MT> &W, LoopState &L, int64_t iter, const Resid &res) {
+    W.flush();
     if (L.saved && !(res.kkt < std::max(std::max(L.save_Rp, L.save_Rd), L.save_gap))) return;
     const int32_t n = W.n, m = W.m;
     if (!L.saved) {
@@ -1628,6 +1898,7 @@ static void save_download(hipStream_t s, const T *src, const double *nrm, int64_
 // src/algorithm.jl:183-301)
 template <class VT, class MT>
 static void auto_save_write(Solver<VT, MT> &W, LoopState &L, int64_t iter, const Resid &res, double elapsed) {
+    W.flush();
     const hprqp_params *p = W.p;
     if (!p->save_cb || !L.saved) return;
     hipStream_t s = W.s;
@@ -1945,6 +2216,7 @@ static int run_loop(WS &W, LoopState &L, int64_t iter_begin, double t_start, dou
 
 template <class VT, class MT>
 static void collect(Solver<VT, MT> &W, double *x_out, double *y_out, double *z_out, double *w_out) {
+    W.flush();
     hipStream_t s = W.s;
     const int32_t n = W.n, m = W.m;
     const double bs = W.S.b_scale, cs = W.S.c_scale;
@@ -2612,6 +2884,9 @@ struct WsImpl final : hprqp_ws {
         HPRQP_REQUIRE(on_path, HPRQP_E_STATE, "half-step is not on this problem's path (hprqp_ws_info.path)");
         HPRQP_REQUIRE(sp->sigma > 0 && sp->h1 >= 0 && sp->h1 <= 1, HPRQP_E_ARG, "step scalars out of range");
         Lanes scope(this);
+        // a half-step on its own leaves the QAP one-apply form's products
+        // behind the iterates
+        if (kind != kWsMain) W->invalidate_products();
         set_scalars(sp);
         const bool full = sp->full != 0, diag = S.Q_is_diag;
         if (full)
@@ -2659,6 +2934,7 @@ struct WsImpl final : hprqp_ws {
 
     void download(int which, double *host, int64_t len) override {
         Solver<VT, MT> &D = *W;
+        D.flush();
         const VT *src = nullptr;
         const double *srcd = nullptr;
         int64_t want = D.n;

@@ -47,7 +47,7 @@ class hprqp_params(C.Structure):
         ("polish_stall_checks", C.c_int32),
         ("polish_switch", C.c_double),
         ("auto_save", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("qap_recompute_Qw", C.c_int32),
         ("save_cb", C.c_void_p),
         ("save_user", C.c_void_p),
     ]

@@ -54,6 +54,7 @@ class HPRQP_parameters:
     time_kernels: bool = False   # HIP-event timing of the first fused step kernel
     polish_stall_checks: int = 0  # checks without KKT halving before FP64 polish (0: 100 mixed, 40 fp32, 400 fp32 offset form)
     polish_switch: float = 5.0  # mixed / plain fp32: FP64 polish once KKT < polish_switch * stoptol
+    qap_recompute_Qw: bool = False  # QAP with doubly stochastic rows: True = two applies per iteration as the reference
 
     def to_c(self) -> _lib.hprqp_params:
         p = _lib.hprqp_params()
@@ -84,6 +85,7 @@ class HPRQP_parameters:
         p.polish_stall_checks = int(self.polish_stall_checks)
         p.polish_switch = float(self.polish_switch)
         p.auto_save = int(bool(self.auto_save))
+        p.qap_recompute_Qw = int(bool(self.qap_recompute_Qw))
         return p
 
 

@@ -37,9 +37,11 @@
 extern "C" {
 #endif
 
-#define HPRQP_ABI_VERSION 4  /* 2: hprqp_params gained the auto-save fields;
+#define HPRQP_ABI_VERSION 5  /* 2: hprqp_params gained the auto-save fields;
                                 3: hprqp_batch_create takes the total nnz of A;
-                                4: hprqp_batch_result gained interval_time */
+                                4: hprqp_batch_result gained interval_time;
+                                5: hprqp_params.reserved0 became
+                                   qap_recompute_Qw */
 
 /* ---- error codes ---------------------------------------------------- */
 #define HPRQP_OK 0
@@ -139,7 +141,13 @@ typedef struct {
     int32_t auto_save;           /* 0; 1 = track the best iterate and hand it
                                     with the current one to save_cb at print
                                     steps and at termination (auto_save) */
-    int32_t reserved0;
+    int32_t qap_recompute_Qw;    /* 0: for the QAP operator with the relaxation's
+                                    doubly stochastic constraints, one operator
+                                    apply per iteration -- Q w and Q ATy are
+                                    carried by their linear recurrences and
+                                    Q A^T y_bar is formed from y_bar in O(nq^2);
+                                    1: Q w is recomputed every iteration, as
+                                    the reference does (two applies) */
     hprqp_save_fn save_cb;       /* NULL: tracked but not handed over */
     void *save_user;             /* passed to save_cb */
 } hprqp_params;

@@ -55,6 +55,10 @@ def test_host_loop_reproduces_solve_bit_for_bit(case, half_steps):
 
     model, ptype = _model(case)
     params = HPRQP_parameters(verbose=False, problem_type=ptype)
+    if case == "qap_esc64a" and half_steps:
+        # the half-step entries are the reference's four half-steps; the
+        # one-apply QAP form fuses them differently, so compare like with like
+        params.qap_recompute_Qw = True
     # one uploaded model: the row-product shapes the autotuner picks (by
     # timing) are cached in it, so both solves sum in the same order
     dm = DeviceModel(model)
