@@ -365,7 +365,8 @@ def main():
         fl = 4.0 * nq ** 3  # two n^3 GEMM pairs per apply (diagonal S, T)
         peak = 78.6 if args.precision == "fp64" else 2500.0 / 3.0
         achieved = fl / (kms * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "kernel": "k_qap_f64 / k_qap_bf16x3 + z/x/w1 epilogue (timed together)",
+        roofline = {"bound": "mfma", "kernel": "the step's one operator apply: MFMA passes 1 and 2 of Q d, d = w - "
+                    "x_hat, with the tempv epilogue and its tile sums (timed together)",
                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                     "traffic": None, "flops_per_launch": fl, "avg_ms": kms, "launches": calls,
                     "launch_sample": "every 8th partial-step launch of the timed solves carries the event pair",

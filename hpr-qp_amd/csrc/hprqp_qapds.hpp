@@ -249,14 +249,53 @@ struct QapDsY {
 template <class VT, class DT, int YW, int ZX>
 __global__ __launch_bounds__(256) void k_qap_ds_tile(QapDsStep<VT, DT> e, QapDsY<VT, DT> q) {
     __shared__ double ys[512];          // y_bar (2n <= 510)
-    __shared__ double red[4][16][17];   // GEMV partials: [product][k segment][entry]
+    __shared__ double red[4][16][17];   // products' partials: [product][k segment][entry]
     __shared__ double pv[4][16];        // P, R, Sa, Tb of the tile's rows / columns
     const int n = e.n, t = threadIdx.x, li = t & 15, lj = t >> 4;
     const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16, i = i0 + li, j = j0 + lj;
     const bool in = i < n && j < n;
-    const int64_t o = (int64_t)i + (int64_t)j * n;
-    double wi = 0.0, atyi = 0.0, qwi = 0.0;
+    const int64_t o = in ? (int64_t)i + (int64_t)j * n : 0;
+    // everything that does not depend on y_bar is loaded first, so its
+    // latency overlaps the tile sums and the products
+    double w0 = (double)e.w[o], t0 = (double)e.ATy[o], qw0 = (double)e.Qw[o];
+    double wb = 0.0, qt0 = 0.0, qwb = 0.0, lw = 0.0, lt = 0.0, lqw = 0.0, lqt = 0.0;
     if constexpr (YW != 0) {
+        wb = (double)e.w_bar[o];
+        qt0 = (double)e.QATy[o];
+        qwb = (double)e.Qw_bar[o];
+        lw = e.last_w[o];
+        lt = e.last_ATy[o];
+        lqw = e.last_Qw[o];
+        lqt = e.last_QATy[o];
+    }
+    double xi = 0.0, ci = 0.0, lx = 0.0, lo_b = e.lS, hi_b = e.uS;
+    if constexpr (ZX != 0) {
+        xi = (double)e.x[o];
+        ci = (double)e.c[o];
+        lx = e.last_x[o];
+        if (e.l) lo_b = (double)e.l[o];
+        if (e.u) hi_b = (double)e.u[o];
+    }
+    double wi = w0, atyi = t0, qwi = qw0;
+    if constexpr (YW != 0) {
+        // the products' operands: thread (li, lj) takes k in segment lj of
+        // row i0 + li of A and S and of column j0 + li of B and T (as rows of
+        // their transposes); kseg <= 16 for n < 256
+        const int kseg = (n + 15) / 16, ka = lj * kseg;
+        const int ri = min(i0 + li, n - 1), rj = min(j0 + li, n - 1);
+        const bool fs_ = e.Sa != nullptr, ft_ = e.Tb != nullptr;
+        double ma[16], mb[16], ms[16], mt[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            ma[u] = mb[u] = ms[u] = mt[u] = 0.0;
+            if (u < kseg) {
+                const int k = min(ka + u, n - 1);
+                ma[u] = e.gA[ri + (int64_t)k * n];
+                mb[u] = e.gBt[rj + (int64_t)k * n];
+                if (fs_) ms[u] = e.gS[ri + (int64_t)k * n];
+                if (ft_) mt[u] = e.gTt[rj + (int64_t)k * n];
+            }
+        }
         const int m = 2 * n, nt = (int)gridDim.x;
         const bool lead = blockIdx.x == 0 && blockIdx.y == 0;
         for (int r = t; r < m; r += 256) {
@@ -268,7 +307,7 @@ __global__ __launch_bounds__(256) void k_qap_ds_tile(QapDsStep<VT, DT> e, QapDsY
             const double lo = (double)q.AL[r];
             const double sp = proj(s_raw, lo, q.AU ? (double)q.AU[r] : lo);
             const double yb = q.fact2 * (sp - s_raw);
-            ys[r] = yb;
+            ys[r] = (double)(VT)yb;  // y_bar as the iteration stores it
             if (lead) {
                 const double yh = 2.0 * yb - yi;
                 q.y_next[r] = (VT)fma(q.h2, yh, q.h1 * q.last_y[r]);
@@ -280,18 +319,18 @@ __global__ __launch_bounds__(256) void k_qap_ds_tile(QapDsStep<VT, DT> e, QapDsY
             }
         }
         __syncthreads();
-        // the products of y_bar this tile needs: P_i = (A a)_i, R_j = (B^T b)_j
-        // and, for a full S / T, (S a)_i, (T^T b)_j; a = y_bar[0:n] as the
-        // iteration stores it (VT), b = y_bar[n:2n]
-        const int kseg = (n + 15) / 16, ka = lj * kseg, kb = min(n, ka + kseg);
-        const int ri = min(i0 + li, n - 1), rj = min(j0 + li, n - 1);
+        // P_i = (A a)_i, R_j = (B^T b)_j and, for a full S / T, (S a)_i,
+        // (T^T b)_j, with a = y_bar[0:n], b = y_bar[n:2n]
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int k = ka; k < kb; ++k) {
-            const double a = (double)(VT)ys[k], b = (double)(VT)ys[n + k];
-            acc[0] = fma(e.gA[ri + (int64_t)k * n], a, acc[0]);
-            acc[1] = fma(e.gBt[rj + (int64_t)k * n], b, acc[1]);
-            if (e.Sa) acc[2] = fma(e.gS[ri + (int64_t)k * n], a, acc[2]);
-            if (e.Tb) acc[3] = fma(e.gTt[rj + (int64_t)k * n], b, acc[3]);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = ka + u;
+            const bool kin = u < kseg && k < n;
+            const double a = kin ? ys[k] : 0.0, b = kin ? ys[n + k] : 0.0;
+            acc[0] = fma(ma[u], a, acc[0]);
+            acc[1] = fma(mb[u], b, acc[1]);
+            acc[2] = fma(ms[u], a, acc[2]);
+            acc[3] = fma(mt[u], b, acc[3]);
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) red[c][lj][li] = acc[c];
@@ -304,19 +343,17 @@ __global__ __launch_bounds__(256) void k_qap_ds_tile(QapDsStep<VT, DT> e, QapDsY
         }
         __syncthreads();
         if (in) {
-            const double a = (double)(VT)ys[i], b = (double)(VT)ys[n + j];
+            const double a = ys[i], b = ys[n + j];
             const double g = a + b;
-            const double sa = e.Sa ? pv[2][li] : e.sd[i] * a, rs = e.Sa ? e.rS[i] : e.sd[i];
-            const double tb = e.Tb ? pv[3][lj] : e.td[j] * b, ct = e.Tb ? e.cT[j] : e.td[j];
+            const double sa = fs_ ? pv[2][li] : e.sd[i] * a, rs = fs_ ? e.rS[i] : e.sd[i];
+            const double tb = ft_ ? pv[3][lj] : e.td[j] * b, ct = ft_ ? e.cT[j] : e.td[j];
             const double qg = 2.0 * (pv[0][li] * e.cB[j] + e.rA[i] * pv[1][lj] - sa - rs * b - a * ct - tb);
-            const double w0 = (double)e.w[o], wb = (double)e.w_bar[o], t0 = (double)e.ATy[o];
-            const double qw0 = (double)e.Qw[o], qt0 = (double)e.QATy[o];
             const double wbn = wb + e.fs * (g - t0);
-            const double wn = e.wh1 * e.last_w[o] + e.wh2 * (2.0 * wbn - w0);
-            const double atyn = e.wh1 * e.last_ATy[o] + e.wh2 * (2.0 * g - t0);
-            const double qwbn = (double)e.Qw_bar[o] + e.fs * (qg - qt0);
-            const double qwn = e.wh1 * e.last_Qw[o] + e.wh2 * (2.0 * qwbn - qw0);
-            const double qtn = e.wh1 * e.last_QATy[o] + e.wh2 * (2.0 * qg - qt0);
+            const double wn = e.wh1 * lw + e.wh2 * (2.0 * wbn - w0);
+            const double atyn = e.wh1 * lt + e.wh2 * (2.0 * g - t0);
+            const double qwbn = qwb + e.fs * (qg - qt0);
+            const double qwn = e.wh1 * lqw + e.wh2 * (2.0 * qwbn - qw0);
+            const double qtn = e.wh1 * lqt + e.wh2 * (2.0 * qg - qt0);
             e.w[o] = (VT)wn;
             e.ATy[o] = (VT)atyn;
             e.Qw[o] = (VT)qwn;
@@ -333,18 +370,13 @@ __global__ __launch_bounds__(256) void k_qap_ds_tile(QapDsStep<VT, DT> e, QapDsY
             atyi = (double)(VT)atyn;
             qwi = (double)(VT)qwn;
         }
-    } else if (in) {
-        wi = (double)e.w[o];
-        atyi = (double)e.ATy[o];
-        qwi = (double)e.Qw[o];
     }
     if constexpr (ZX != 0) {
         if (!in) return;
-        const double xi = (double)e.x[o];
-        const double z_raw = xi + e.sigma * (-qwi + atyi - (double)e.c[o]);
-        const double xb = proj(z_raw, e.l ? (double)e.l[o] : e.lS, e.u ? (double)e.u[o] : e.uS);
+        const double z_raw = xi + e.sigma * (-qwi + atyi - ci);
+        const double xb = proj(z_raw, lo_b, hi_b);
         const double xh = 2.0 * xb - xi;
-        const double xn = fma(e.h2, xh, e.h1 * e.last_x[o]);
+        const double xn = fma(e.h2, xh, e.h1 * lx);
         const double wbz = fma(e.f1s, wi, e.f2s * xh);
         if constexpr (ZX == 2) {
             e.dx[o] = (VT)(xb - xi);

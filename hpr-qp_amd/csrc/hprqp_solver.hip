@@ -759,7 +759,6 @@ struct Solver {
         QapDsStep<VT, MT> e = qds_args();
         e.h1 = h1;
         e.h2 = h2;
-        kbegin();
         if (qds_small) {
             if (pend) {
                 flushed_full = false;
@@ -775,17 +774,20 @@ struct Solver {
         } else {
             qap_ds_step<VT, MT, 0, FULL ? 2 : 1>(s, e);
         }
-        kend();
         pend = false;
-        // the apply, of d = w - x_hat
+        // the apply, of d = w - x_hat (its two MFMA passes are the timed
+        // kernels of the step)
+        kbegin();
         if (qds_small) {
             // pass 2 leaves tempv's tile sums; the y update waits for the
             // next fused kernel
             EpiTempvDiff<VT, true> et{x_hat.p, Qw.p, Qw_bar.p, nullptr, tsum.p, sigma, e.f2s};
             q_fused(qdv.p, et, Qw_bar.p);
+            kend();
         } else {
             EpiTempvDiff<VT, false> et{x_hat.p, Qw.p, Qw_bar.p, tempv.p, nullptr, sigma, e.f2s};
             q_fused(qdv.p, et, Qw_bar.p);
+            kend();
             step_y<FULL>(tempv.p, h1, h2);
         }
         pend = true;
