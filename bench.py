@@ -264,7 +264,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "fp32"])
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=40)
+    ap.add_argument("--cpu-iters", type=int, default=300,
+                    help="HPR iterations of the CPU oracle sample (about 12 s on the 1414^2 grid)")
     ap.add_argument("--cpu-power-iters", type=int, default=100)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=64)

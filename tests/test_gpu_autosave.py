@@ -77,3 +77,58 @@ def test_resume_from_the_saved_best_iterate(tmp_path):
     rel = abs(again.primal_obj - first.primal_obj) / (1.0 + abs(first.primal_obj))
     print(f"first iter={first.iter} resumed iter={again.iter} rel obj {rel:.2e}")
     assert again.status == "OPTIMAL" and again.iter < first.iter // 4 and rel < 1e-6
+
+
+def test_saved_best_state_recomputed_independently(tmp_path):
+    """The saved best iterate checked against quantities formed without the
+    solver: its iteration is the first minimiser of max(err_Rp, err_Rd,
+    rel_gap) over the run's residual history (the reference replaces the
+    best only on a strict decrease, src/algorithm.jl:97-112), its recorded
+    residuals are that history row, and the primal objective and the primal
+    and dual residuals recomputed by scipy from the saved (x, y, z) in the
+    original space (scaling off, so the saved z is the result's z) equal the
+    recorded ones.  The run is stopped at a residual check whose merit is
+    above an earlier one (found from a first run's history), so the best
+    iterate is not the current one."""
+    import scipy.sparse as sp
+
+    from hprqp import HPRQP_parameters, instances, optimize
+
+    model = instances.random_sparse_qp(400, 200, seed=0)  # its merit rises once, at iteration 510
+    kw = dict(verbose=False, use_Ruiz_scaling=False, use_Pock_Chambolle_scaling=False)
+    h0 = optimize(model, HPRQP_parameters(**kw)).history
+    m0 = np.maximum(np.maximum(h0[:, 1], h0[:, 2]), h0[:, 3])
+    # a check whose merit rose above an earlier check's after the start
+    above = [r for r in range(2, len(h0)) if m0[r] > np.min(m0[1:r]) and np.argmin(m0[:r]) > 0]
+    assert above, "the merit never rose"
+    stop = int(h0[above[-1], 0])
+    fn = str(tmp_path / "best.npz")
+    res = optimize(model, HPRQP_parameters(auto_save=True, save_filename=fn, max_iter=stop, **kw))
+    assert res.iter == stop
+    d = _load(fn)
+    h = res.history
+    merit = np.maximum(np.maximum(h[:, 1], h[:, 2]), h[:, 3])
+    k = int(np.argmin(merit))
+    print(f"status {res.status} iter {res.iter}; {len(h)} residual checks; best at {int(h[k, 0])} "
+          f"(merit {merit[k]:.3e}); saved best {int(d['best/iteration'])}")
+    assert 0 < int(d["best/iteration"]) == int(h[k, 0]) < stop
+    assert float(d["best/err_Rp"]) == h[k, 1] and float(d["best/err_Rd"]) == h[k, 2]
+    assert float(d["best/rel_gap"]) == h[k, 3] and float(d["best/primal_obj"]) == h[k, 4]
+    pobj, dobj = float(d["best/primal_obj"]), float(d["best/dual_obj"])
+    gap = abs(pobj - dobj) / (1.0 + max(abs(pobj), abs(dobj)))
+    assert abs(gap - float(d["best/rel_gap"])) <= 1e-12 * (1.0 + gap)
+    # recomputed from the saved vectors
+    x, y, z = d["best/x_org"], d["best/y_org"], d["best/z_org"]
+    Q, A = sp.csr_matrix(model.Q), sp.csr_matrix(model.A)
+    Qx, Ax, ATy = Q @ x, A @ x, A.T @ y
+    p_re = float(model.c @ x + 0.5 * x @ Qx + model.obj_constant)
+    rp = np.max(np.abs(np.clip(Ax, model.AL, model.AU) - Ax)) / (1.0 + max(np.max(np.abs(np.where(
+        np.isfinite(model.AL), model.AL, 0.0))), np.max(np.abs(np.where(np.isfinite(model.AU), model.AU, 0.0))),
+        np.max(np.abs(Ax))))
+    rd = np.max(np.abs(Qx + model.c - ATy - z)) / (1.0 + max(np.max(np.abs(model.c)), np.max(np.abs(ATy)),
+                                                          np.max(np.abs(Qx))))
+    print(f"recomputed: pobj {p_re:.12e} vs {pobj:.12e}, err_Rp {rp:.6e} vs {float(d['best/err_Rp']):.6e}, "
+          f"err_Rd {rd:.6e} vs {float(d['best/err_Rd']):.6e}")
+    assert abs(p_re - pobj) <= 1e-10 * (1.0 + abs(pobj))
+    assert abs(rp - float(d["best/err_Rp"])) <= 1e-8 * float(d["best/err_Rp"]) + 1e-15
+    assert abs(rd - float(d["best/err_Rd"])) <= 1e-8 * float(d["best/err_Rd"]) + 1e-15

@@ -351,6 +351,38 @@ def test_warm_start_matches_oracle(start):
         assert res.iter < cold.iter
 
 
+def test_noC_warm_start_with_initial_x_leaves_Qw_unset():
+    """The noC path with initial_x (SURVEY hazard B.6): process_initial_points!
+    sets x and w from initial_x but not Qw / last_Qw (src/algorithm.jl:1797-1802),
+    which the noC step carries by its Halpern recurrence from the last restart
+    (:544-547), so the run starts from Qw = 0 until its first restart.  The GPU
+    reproduces that: on AUG2D (noC, diagonal Q) warm-started from a perturbed
+    optimum with x only, it takes the oracle's iterations and restarts, and
+    stopped part-way, before the first restart, its iterates are the
+    oracle's."""
+    from hprqp import HPRQP_parameters, build_from_mps, optimize
+    from oracle import oracle
+
+    model = build_from_mps(fixture_path("AUG2D.mps"), verbose=False)
+    cold = oracle.solve(model)
+    rng = np.random.default_rng(11)
+    x0 = cold.x + 1e-2 * rng.standard_normal(model.n)
+    res = optimize(model, HPRQP_parameters(verbose=False, initial_x=x0))
+    ref = oracle.solve(model, initial_x=x0)
+    print(f"noC + initial_x: gpu iter={res.iter} restarts={res.restarts} obj={res.primal_obj:.12e}; oracle "
+          f"iter={ref.iter} restarts={ref.restarts} obj={ref.primal_obj:.12e}; cold {cold.iter}")
+    assert res.noC and ref.noC and res.Q_is_diag
+    _check(res, ref)
+    assert res.iter == ref.iter and res.restarts == ref.restarts
+    its = 60
+    part = optimize(model, HPRQP_parameters(verbose=False, initial_x=x0, max_iter=its))
+    rpart = oracle.solve(model, initial_x=x0, max_iter=its)
+    dx = np.max(np.abs(part.x - rpart.x)) / (1 + np.max(np.abs(rpart.x)))
+    dy = np.max(np.abs(part.y - rpart.y)) / (1 + np.max(np.abs(rpart.y)))
+    print(f"stopped at {its}: x diff {dx:.2e}, y diff {dy:.2e}, restarts {part.restarts} / {rpart.restarts}")
+    assert part.restarts == rpart.restarts == 0 and dx < 1e-9 and dy < 1e-9
+
+
 @pytest.mark.parametrize("case", ["random", "grid", "lasso", "qap"])
 def test_trajectory_matches_oracle_mid_solve(case):
     """Stopped part-way (max_iter = 300, before convergence), the GPU's
