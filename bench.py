@@ -207,6 +207,68 @@ def batched_leg(count, plain_sample):
     return out
 
 
+def batch_csr_iter_bytes(n, m, nnzq, nnza):
+    """Algorithmic bytes of one HPR iteration of a sparse-Q instance in the
+    batch (FP64): Q w and Q w_bar (12 B per nonzero: value, index, gathered
+    entry ~ 8 + 4, plus the row pointers), A tempv and A^T y_bar likewise,
+    and the elementwise updates' n- and m-vector streams (z/x/w1 reads 7
+    writes 4, tempv 3 + 1, w2 6 + 2; y 4 + 2 per row)."""
+    return 2 * (12 * nnzq + 4 * n) + 12 * nnza + 4 * m + 12 * nnza + 4 * n + 8 * (23 * n + 6 * m)
+
+
+def batched_csr_leg(count, plain_sample):
+    """`count` random sparse QPs of the reference-runnable generator at uneven
+    sizes (n ~ U{200..2000}, m = n/2, Q = R'R + diag with ~160 nonzeros per
+    row, instance seeds 1..count), FP64: plain single-instance GPU solves
+    (sequential, models resident; timed on the first `plain_sample`) against
+    one batched solve of all of them (hprqp_batch_create_csr: every instance
+    scaled and iterated by its own workgroup).  Every solve must reach 1e-6
+    relative KKT."""
+    import numpy as np
+
+    from hprqp import DeviceModel, HPRQP_parameters, instances
+    from hprqp.batch import BatchModel
+
+    rng = np.random.default_rng(2024)
+    sizes = rng.integers(200, 2001, count)
+    t = time.time()
+    models = [instances.random_sparse_qp(int(n), int(n) // 2, seed=1 + i) for i, n in enumerate(sizes)]
+    log(f"batched csr: {count} instances generated in {time.time() - t:.1f}s")
+    plain = HPRQP_parameters(verbose=False)
+    dms = [DeviceModel(m) for m in models[:plain_sample]]
+    dms[0].solve(plain)
+    t = time.perf_counter()
+    ok = 0
+    for dm in dms:
+        r = dm.solve(plain)
+        ok += r.status == "OPTIMAL" and r.residuals < 1e-6
+    plain_sps = ok / (time.perf_counter() - t)
+    for dm in dms:
+        dm.close()
+    bm = BatchModel(models)
+    bm.solve(plain)  # warm-up
+    t = time.perf_counter()
+    res = bm.solve(plain)
+    wall = time.perf_counter() - t
+    solved = int(np.sum((np.array(res.status) == "OPTIMAL") & (res.residuals < 1e-6)))
+    per_it = np.array([batch_csr_iter_bytes(m.n, m.m, m.Q.nnz, m.A.nnz) for m in models])
+    nbytes = float(np.sum(per_it * res.iter))
+    out = {"workload": f"{count} random sparse QPs, n~U{{200..2000}}, m=n/2, FP64, 1e-6 relative KKT",
+           "plain_fp64_solves_per_s": plain_sps, "plain_sample": plain_sample,
+           "batched_fp64_solves_per_s": solved / wall, "batched_vs_plain": (solved / wall) / plain_sps,
+           "all_solved": solved == count, "mean_iter": float(res.iter.mean()), "max_iter": int(res.iter.max()),
+           "lambda_max_s": res.power_time, "interval_kernel_s": res.interval_time, "intervals": res.intervals,
+           "roofline": {"bound": "hbm", "kernel": "k_batch_interval<double, 1> (one workgroup per instance, "
+                                                  "one launch per check interval)",
+                        "achieved": nbytes / res.interval_time / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": nbytes / res.interval_time / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                        "bytes_per_solve": nbytes,
+                        "timing": "HIP events around each check-interval replay"}}
+    log("batched csr: " + json.dumps(out))
+    bm.close()
+    return out
+
+
 def qap_apply_leg(sizes, reps):
     """The QAP operator apply Q(X) = 2(A X B - S X - X T) alone, buffers
     resident in HBM, at several sizes: FP64 MFMA and split-BF16 x3 MFMA,
@@ -268,6 +330,8 @@ def main():
                     help="HPR iterations of the CPU oracle sample (about 12 s on the 1414^2 grid)")
     ap.add_argument("--cpu-power-iters", type=int, default=100)
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-csr-count", type=int, default=512,
+                    help="--workload batch-csr: number of random sparse QPs in the batch")
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -276,7 +340,7 @@ def main():
     ap.add_argument("--qap-sizes", default="128,256,512,1024,2048,4096",
                     help="--workload qap-apply: QAP sizes n (X is n x n)")
     ap.add_argument("--workload", default="grid",
-                    choices=["grid", "random", "lasso", "qap", "batch", "copy", "qap-apply"],
+                    choices=["grid", "random", "lasso", "qap", "batch", "batch-csr", "copy", "qap-apply"],
                     help="grid (default): the headline single-instance workload; random / lasso / qap: the "
                          "other single-instance workloads (for profiling); batch: only the batched leg")
     args = ap.parse_args()
@@ -325,6 +389,9 @@ def main():
         return
     if args.workload == "qap-apply":
         print(json.dumps(qap_apply_leg([int(v) for v in args.qap_sizes.split(",")], args.steps)), flush=True)
+        return
+    if args.workload == "batch-csr":
+        print(json.dumps(batched_csr_leg(args.batch_csr_count, args.plain_sample)), flush=True)
         return
     if args.workload == "batch":
         out = batched_leg(args.batch_count, args.plain_sample)

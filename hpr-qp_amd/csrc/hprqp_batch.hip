@@ -17,6 +17,13 @@
 // Host side: per-instance check_break / check_restart / update_sigma /
 // do_restart decisions from a 32-double record per instance
 // (src/algorithm.jl:323-576), exactly as the single-instance loop.
+//
+// A second kind of batch (hprqp_batch_create_csr) holds sparse Q_i (CSR,
+// symmetric pattern): each solve first scales every instance as the single
+// solve does (k_batch_scale: Ruiz x 10 and Pock-Chambolle,
+// src/utils.jl:350-804, then (Q + Q')/2), the kernels take QK = 1 and form
+// Q v by CSR row products, and the residuals carry the row and column norms
+// (compute_residuals!, src/algorithm.jl:37-133).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -50,7 +57,9 @@ enum {
     R_DXDX, R_DQWATDY, R_DQWDQW, R_DWDQW, R_ATDYQATDY, R_DYDY,          // sigma dots
     R_POW_A, R_POW_Q, R_POW_ITA, R_POW_ITQ,                             // power method
     R_DONE,  // iteration at which the instance met the tolerance inside an interval, -1 if not
-    R_COUNT = 24
+    R_NB,    // ||b||_2, ||c||_2 of the scaled data (sigma_0)
+    R_NC,
+    R_COUNT = 26
 };
 
 template <class MT>
@@ -64,6 +73,13 @@ struct BArgs {
     const int32_t *a_rp, *a_ci, *at_rp, *at_ci;  // packed CSR, row pointers local to the instance
     const MT *a_v, *at_v;
     const double *a_v64, *at_v64;
+    // CSR Q (QK = 1): packed, row pointers global (instance b's at
+    // qrp_off[b]), column indices local; the scaling's column and row norms
+    // (null for the operator kind, which is not scaled)
+    const int32_t *q_rp, *q_ci;
+    const int64_t *qrp_off;
+    const MT *q_v;
+    const double *q_v64, *cn, *rn;
     const double *c, *l, *u, *AL, *AU;
     const double *c0, *nbi, *nci;  // objective constant, |b|_inf, |c|_inf (residual norms)
     double tol;
@@ -240,6 +256,9 @@ struct IPtr {
     const int32_t *arp, *aci, *atrp, *atci;
     const MT *av, *atv;
     const double *av64, *atv64;
+    const int32_t *qrp, *qci;
+    const MT *qv;
+    const double *qv64, *cn, *rn;
     const double *c, *l, *u, *AL, *AU;
     const MT *cM, *lM, *uM, *ALM, *AUM;
     double *x, *w, *w_bar, *x_hat, *x_bar, *z_bar, *dx, *dw, *Qw, *Qw_bar, *tempv, *ATy, *ATy_bar, *ATdy, *QATdy,
@@ -277,6 +296,12 @@ __device__ Inst<MT> inst_of(const BArgs<MT> &a, int b, IPtr<MT> *tab) {
         T.atv = a.at_v;
         T.av64 = a.a_v64;
         T.atv64 = a.at_v64;
+        T.qrp = a.q_rp ? a.q_rp + a.qrp_off[b] : nullptr;
+        T.qci = a.q_ci;
+        T.qv = a.q_v;
+        T.qv64 = a.q_v64;
+        T.cn = a.cn ? a.cn + no : nullptr;
+        T.rn = a.rn ? a.rn + mo : nullptr;
         T.c = a.c + no;
         T.l = a.l + no;
         T.u = a.u + no;
@@ -321,6 +346,152 @@ __device__ void bkspace_init(Inst<MT> &I, KSpace &ks, double *sh) {
     lr_tvec(n, k, I.P->U, I.P->ATy, ks.taty, sh);
     lr_tvec(n, k, I.P->U, I.P->last_w, ks.tlw, sh);
     lr_tvec(n, k, I.P->U, I.P->last_ATy, ks.tlaty, sh);
+}
+
+// out_r = sum A_r. x for `rows` rows with L lanes per row (L divides 64):
+// the block takes kBB / L rows at a time; each lane loads eight of its row's
+// nonzeros (stride L, past the row's end the row's first entry with weight
+// zero), then their eight gathers, so each lane has eight gathers in flight;
+// the row's lanes sum by xor shuffles
+template <int L, class AT_, class VT>
+__device__ void csr_rows_lanes(int32_t rows, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const AT_ *__restrict__ v, const VT *__restrict__ x, double *__restrict__ out) {
+    constexpr int G = kBB / L, U = 8;
+    const int lane = (int)threadIdx.x % L, grp = (int)threadIdx.x / L;
+    for (int32_t r0 = 0; r0 < rows; r0 += G) {
+        const int32_t r = r0 + grp;
+        double acc = 0.0;
+        if (r < rows) {
+            const int32_t s = rp[r], e = rp[r + 1];
+            for (int32_t q0 = s + lane; q0 < e; q0 += U * L) {
+                int32_t cc[U];
+                double vv[U], xx[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int32_t q = q0 + u * L, at = q < e ? q : s;
+                    cc[u] = ci[at];
+                    vv[u] = q < e ? (double)v[at] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) xx[u] = (double)x[cc[u]];
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc += vv[u] * xx[u];
+            }
+        }
+        if constexpr (L > 1) {
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+        }
+        if (lane == 0 && r < rows) out[r] = acc;
+    }
+    __syncthreads();
+}
+// the lanes per row from the matrix's mean row length (block-uniform): one
+// lane for rows of a few nonzeros, four up to a few hundred (a row then
+// takes few rounds of eight gathers per lane), sixteen beyond
+template <class AT_, class VT>
+__device__ void csr_rows_auto(int32_t rows, const int32_t *rp, const int32_t *ci, const AT_ *v, const VT *x,
+                              double *out) {
+    if (rows <= 0) return;
+    const int32_t mean = (rp[rows] - rp[0]) / rows;
+    if (mean <= 6)
+        csr_rows_lanes<1>(rows, rp, ci, v, x, out);
+    else if (mean <= 256)
+        csr_rows_lanes<4>(rows, rp, ci, v, x, out);
+    else
+        csr_rows_lanes<16>(rows, rp, ci, v, x, out);
+}
+
+// out = Q v for a CSR instance (QK = 1), in the iteration's precision (qv)
+// or in FP64 (qv64)
+template <class MT, class AT_, class VT>
+__device__ void qcsr_apply(const Inst<MT> &I, const AT_ *qv, const VT *v, double *out) {
+    csr_rows_auto(I.n, I.P->qrp, I.P->qci, qv, v, out);
+}
+
+// Q v of either kind: out = Q v; t, sh LDS scratch of the low-rank product
+template <int QK, class MT, class VT>
+__device__ void bq_apply(const Inst<MT> &I, const VT *v, double *out, double *t, double *sh, bool fp64 = false) {
+    if constexpr (QK == 0) {
+        if (fp64)
+            lr_apply(I.n, I.k, I.P->U64, I.P->d64, v, out, t, sh);
+        else
+            lr_apply(I.n, I.k, I.P->U, I.P->d, v, out, t, sh);
+    } else {
+        if (fp64)
+            qcsr_apply(I, I.P->qv64, v, out);
+        else
+            qcsr_apply(I, I.P->qv, v, out);
+    }
+}
+
+// One HPR step of the standard path with a sparse Q (QK = 1): the
+// reference's four half-steps (src/kernels.jl:2301-2832), each a row
+// product (Q w, Q w_bar, A tempv, A^T y_bar) by csr_rows_auto into a work
+// vector followed by its elementwise update
+template <class MT>
+__device__ void bstep_csr(Inst<MT> &I, double sigma, double lamA, double lamQ, double h1, double h2, bool full) {
+    const int32_t n = I.n, m = I.m;
+    const double f2s = 1.0 / (1.0 + sigma * lamQ), f1s = 1.0 - f2s;
+    // z/x/w1 on Q w
+    csr_rows_auto(n, I.P->qrp, I.P->qci, I.P->qv, I.P->w, I.P->Qw);
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
+        const double qw = I.P->Qw[i];
+        const double wi = I.P->w[i], xi = I.P->x[i];
+        const double z_raw = xi + sigma * (-qw + I.P->ATy[i] - (double)I.P->cM[i]);
+        const double xb = proj(z_raw, (double)I.P->lM[i], (double)I.P->uM[i]);
+        const double xh = 2.0 * xb - xi;
+        if (full) {
+            I.P->dx[i] = xb - xi;
+            I.P->x_bar[i] = xb;
+            I.P->z_bar[i] = (xb - z_raw) / sigma;
+        }
+        I.P->x[i] = fma(h2, xh, h1 * I.P->last_x[i]);
+        I.P->x_hat[i] = xh;
+        I.P->w_bar[i] = fma(f1s, wi, f2s * xh);
+    }
+    __syncthreads();
+    if (m > 0) {
+        // tempv = x_hat + sigma (Q w - Q w_bar)
+        csr_rows_auto(n, I.P->qrp, I.P->qci, I.P->qv, I.P->w_bar, I.P->Qw_bar);
+        for (int32_t i = threadIdx.x; i < n; i += kBB)
+            I.P->tempv[i] = I.P->x_hat[i] + sigma * (I.P->Qw[i] - I.P->Qw_bar[i]);
+        __syncthreads();
+        // y step on A tempv
+        csr_rows_auto(m, I.P->arp, I.P->aci, I.P->av, I.P->tempv, I.P->pz);
+        const double fa = lamA * sigma, fb = 1.0 / fa;
+        for (int32_t r = threadIdx.x; r < m; r += kBB) {
+            const double yi = I.P->y[r];
+            const double s_raw = I.P->pz[r] - fa * yi;
+            const double sp = proj(s_raw, (double)I.P->ALM[r], (double)I.P->AUM[r]);
+            const double yb = fb * (sp - s_raw);
+            if (full) {
+                I.P->s[r] = sp;
+                I.P->dy[r] = yb - yi;
+            }
+            I.P->y_bar[r] = yb;
+            I.P->y[r] = fma(h2, 2.0 * yb - yi, h1 * I.P->last_y[r]);
+        }
+        __syncthreads();
+        // A^T y_bar into tempv (no longer needed this step)
+        csr_rows_auto(n, I.P->atrp, I.P->atci, I.P->atv, I.P->y_bar, I.P->tempv);
+    }
+    // w2 step
+    const double fs = sigma / (1.0 + sigma * lamQ);
+    for (int32_t i = threadIdx.x; i < n; i += kBB) {
+        const double atyb = m > 0 ? I.P->tempv[i] : 0.0;
+        const double wi = I.P->w[i], wbi = I.P->w_bar[i], atyi = I.P->ATy[i];
+        const double wbn = wbi + fs * (atyb - atyi);
+        I.P->w[i] = h1 * I.P->last_w[i] + h2 * (2.0 * wbn - wi);
+        I.P->ATy[i] = h1 * I.P->last_ATy[i] + h2 * (2.0 * atyb - atyi);
+        if (full) {
+            I.P->w_bar[i] = wbn;
+            I.P->dw[i] = wbn - wi;
+            I.P->ATdy[i] = atyb - atyi;
+            I.P->ATy_bar[i] = atyb;
+        }
+    }
+    __syncthreads();
 }
 
 // One HPR step of the standard path (Q = diag(d) + U U' an operator, noC
@@ -408,14 +579,14 @@ __device__ void bstep(Inst<MT> &I, KSpace &ks, double sigma, double lamA, double
 }
 
 // compute_M_norm! (src/algorithm.jl:689-739), non-diagonal Q
-template <class MT>
+template <int QK, class MT>
 __device__ double bmnorm(Inst<MT> &I, double sigma, double lamA, double lamQ, double *t, double *sh) {
-    const int32_t n = I.n, m = I.m, k = I.k;
-    lr_apply(n, k, I.P->U, I.P->d, I.P->dw, I.P->dQw, t, sh);
+    const int32_t n = I.n, m = I.m;
+    bq_apply<QK>(I, I.P->dw, I.P->dQw, t, sh);
     if (m > 0) {
         for (int32_t i = threadIdx.x; i < n; i += kBB) I.P->ATdy[i] = csr_row(I.P->atrp, I.P->atci, I.P->atv, I.P->dy, i);
         __syncthreads();
-        lr_apply(n, k, I.P->U, I.P->d, I.P->ATdy, I.P->QATdy, t, sh);
+        bq_apply<QK>(I, I.P->ATdy, I.P->QATdy, t, sh);
     }
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s6 = 0, s7 = 0;
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
@@ -450,11 +621,11 @@ __device__ double bmnorm(Inst<MT> &I, double sigma, double lamA, double lamQ, do
 }
 
 // residual parts in FP64 against the FP64 data (src/algorithm.jl:37-133)
-template <class MT>
+template <int QK, class MT>
 __device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *sh) {
-    const int32_t n = I.n, m = I.m, k = I.k;
+    const int32_t n = I.n, m = I.m;
     // Qx (into dQw as scratch), A^T y_bar (into ATdy), A x_bar (into pz)
-    lr_apply(n, k, I.P->U64, I.P->d64, I.P->x_bar, I.P->dQw, t, sh);
+    bq_apply<QK>(I, I.P->x_bar, I.P->dQw, t, sh, true);
     if (m > 0) {
         for (int32_t i = threadIdx.x; i < n; i += kBB) I.P->ATdy[i] = csr_row(I.P->atrp, I.P->atci, I.P->atv64, I.P->y_bar, i);
         csr_rows_wave(m, I.P->arp, I.P->aci, I.P->av64, I.P->x_bar, I.P->pz);
@@ -462,19 +633,22 @@ __device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *
     double cx = 0, xqx = 0, zx = 0, rd = 0, aty = 0, qx = 0, bnd = 0, ys = 0, rp = 0, ax = 0;
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
         const double x = I.P->x_bar[i], q = I.P->dQw[i], z = I.P->z_bar[i], ci = I.P->c[i], a = m > 0 ? I.P->ATdy[i] : 0.0;
+        // scaled instances: the original-space residuals (column norms)
+        const double sf = QK == 1 ? I.P->cn[i] : 1.0;
         cx += ci * x;
         xqx += x * q;
         zx += z * x;
-        rd = dmax(rd, fabs(q + ci - a - z));
-        aty = dmax(aty, fabs(a));
-        qx = dmax(qx, fabs(q));
-        if (iter0) bnd = dmax(bnd, dmax(I.P->l[i] - x, 0.0) + dmax(x - I.P->u[i], 0.0));
+        rd = dmax(rd, fabs((q + ci - a - z) * sf));
+        aty = dmax(aty, fabs(a * sf));
+        qx = dmax(qx, fabs(q * sf));
+        if (iter0) bnd = dmax(bnd, (dmax(I.P->l[i] - x, 0.0) + dmax(x - I.P->u[i], 0.0)) / sf);
     }
     for (int32_t r = threadIdx.x; r < m; r += kBB) {
         ys += I.P->y_bar[r] * I.P->s[r];
         const double v = I.P->pz[r];
-        rp = dmax(rp, fabs(proj(v, I.P->AL[r], I.P->AU[r]) - v));
-        ax = dmax(ax, fabs(v));
+        const double sf = QK == 1 ? I.P->rn[r] : 1.0;
+        rp = dmax(rp, fabs((proj(v, I.P->AL[r], I.P->AU[r]) - v) * sf));
+        ax = dmax(ax, fabs(v * sf));
     }
     cx = bsum(cx, sh);
     xqx = bsum(xqx, sh);
@@ -502,9 +676,9 @@ __device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *
 }
 
 // the dot products of update_sigma! (src/algorithm.jl:332-364)
-template <class MT>
+template <int QK, class MT>
 __device__ void bsigma_dots(Inst<MT> &I, double *rec, double *t, double *sh) {
-    const int32_t n = I.n, m = I.m, k = I.k;
+    const int32_t n = I.n, m = I.m;
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
         I.P->dx[i] = I.P->x_bar[i] - I.P->last_x[i];
         I.P->dw[i] = I.P->w_bar[i] - I.P->last_w[i];
@@ -512,8 +686,8 @@ __device__ void bsigma_dots(Inst<MT> &I, double *rec, double *t, double *sh) {
     }
     for (int32_t r = threadIdx.x; r < m; r += kBB) I.P->dy[r] = I.P->y_bar[r] - I.P->last_y[r];
     __syncthreads();
-    lr_apply(n, k, I.P->U, I.P->d, I.P->dw, I.P->dQw, t, sh);
-    if (m > 0) lr_apply(n, k, I.P->U, I.P->d, I.P->ATdy, I.P->QATdy, t, sh);
+    bq_apply<QK>(I, I.P->dw, I.P->dQw, t, sh);
+    if (m > 0) bq_apply<QK>(I, I.P->ATdy, I.P->QATdy, t, sh);
     double a = 0, b = 0, c = 0, d = 0, e = 0, f = 0;
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
         const double dxi = I.P->dx[i], dq = I.P->dQw[i], at = m > 0 ? I.P->ATdy[i] : 0.0, qat = m > 0 ? I.P->QATdy[i] : 0.0;
@@ -585,7 +759,7 @@ __device__ __forceinline__ double bkkt(const double *r, double c0, double nbi, d
 }
 
 // one check interval for every active instance
-template <class MT>
+template <class MT, int QK>
 __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
     __shared__ double t[64];
     __shared__ double sh[kBW];
@@ -597,7 +771,7 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
     Inst<MT> I = inst_of(a, b, &tab);
     double *rec = a.rec + (int64_t)b * R_COUNT;
     if (c.restart) brestart(I);
-    bkspace_init(I, ks, sh);
+    if constexpr (QK == 0) bkspace_init(I, ks, sh);
     if (threadIdx.x == 0) rec[R_DONE] = -1.0;
     double inner = c.h_inner;
     for (int s = 0; s < c.K; ++s) {
@@ -605,13 +779,16 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
         const bool print_yes = next % (a.print_freq > 0 ? a.print_freq : bprint_step(next)) == 0;
         const bool full = (s == c.K - 1) || (s == 0 && c.restart) || (next % c.check_iter == 0) || print_yes;
         const double h1 = 1.0 / (inner + 2.0);
-        bstep(I, ks, c.sigma, c.lamA, c.lamQ, h1, 1.0 - h1, full, sh);
+        if constexpr (QK == 0)
+            bstep(I, ks, c.sigma, c.lamA, c.lamQ, h1, 1.0 - h1, full, sh);
+        else
+            bstep_csr(I, c.sigma, c.lamA, c.lamQ, h1, 1.0 - h1, full);
         if (s == 0 && c.want_last_gap) {
-            const double g = bmnorm(I, c.sigma, c.lamA, c.lamQ, t, sh);
+            const double g = bmnorm<QK>(I, c.sigma, c.lamA, c.lamQ, t, sh);
             if (threadIdx.x == 0) rec[R_LASTGAP] = g;
         }
         if (next % c.check_iter == 0) {
-            const double g = bmnorm(I, c.sigma, c.lamA, c.lamQ, t, sh);
+            const double g = bmnorm<QK>(I, c.sigma, c.lamA, c.lamQ, t, sh);
             if (threadIdx.x == 0) rec[R_CURGAP] = g;
         }
         inner += 1.0;
@@ -619,7 +796,7 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
         // two check iterations): only the termination test can act there, so
         // it runs here and a converged instance retires without the host
         if (print_yes && s < c.K - 1) {
-            bresid(I, false, rec, t, sh);
+            bresid<QK>(I, false, rec, t, sh);
             __syncthreads();
             if (bkkt(rec, a.c0[b], a.nbi[b], a.nci[b], I.m) < a.tol) {
                 if (threadIdx.x == 0) rec[R_DONE] = (double)next;
@@ -627,8 +804,8 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
             }
         }
     }
-    bresid(I, false, rec, t, sh);
-    bsigma_dots(I, rec, t, sh);
+    bresid<QK>(I, false, rec, t, sh);
+    bsigma_dots<QK>(I, rec, t, sh);
 }
 
 // A_i U_i of every instance (row-major m_i x k_i), from the A and U values
@@ -649,7 +826,7 @@ __global__ __launch_bounds__(kBB) void k_batch_au(BArgs<MT> a) {
 }
 
 // iteration-0 residuals (with the bound-violation term) and warm start
-template <class MT>
+template <class MT, int QK>
 __global__ __launch_bounds__(kBB) void k_batch_init(BArgs<MT> a, int has_x0, int has_y0) {
     __shared__ double t[64];
     __shared__ double sh[kBW];
@@ -661,23 +838,28 @@ __global__ __launch_bounds__(kBB) void k_batch_init(BArgs<MT> a, int has_x0, int
     double *rec = a.rec + (int64_t)b * R_COUNT;
     const int32_t n = I.n, m = I.m;
     // process_initial_points! (src/algorithm.jl:1783-1836): x0/y0 were
-    // copied into x_bar / y_bar by the host
+    // copied into x_bar / y_bar by the host, in the original space (scaled
+    // instances: x col_norm, y row_norm; b_scale = c_scale = 1)
     if (has_x0) {
         for (int32_t i = threadIdx.x; i < n; i += kBB) {
-            const double v = I.P->x_bar[i];
+            const double v = QK == 1 ? I.P->x_bar[i] * I.P->cn[i] : I.P->x_bar[i];
+            I.P->x_bar[i] = v;
             I.P->x[i] = I.P->w[i] = I.P->w_bar[i] = I.P->last_x[i] = I.P->last_w[i] = v;
         }
     }
     __syncthreads();
     if (has_y0) {
-        for (int32_t r = threadIdx.x; r < m; r += kBB) I.P->y[r] = I.P->last_y[r] = I.P->y_bar[r];
+        for (int32_t r = threadIdx.x; r < m; r += kBB) {
+            const double v = QK == 1 ? I.P->y_bar[r] * I.P->rn[r] : I.P->y_bar[r];
+            I.P->y_bar[r] = I.P->y[r] = I.P->last_y[r] = v;
+        }
         __syncthreads();
         for (int32_t i = threadIdx.x; i < n; i += kBB) {
             const double v = m > 0 ? csr_row(I.P->atrp, I.P->atci, I.P->atv, I.P->y_bar, i) : 0.0;
             I.P->ATy_bar[i] = I.P->ATy[i] = I.P->last_ATy[i] = v;
         }
         __syncthreads();
-        lr_apply(n, I.k, I.P->U, I.P->d, I.P->x_bar, I.P->dQw, t, sh);
+        bq_apply<QK>(I, I.P->x_bar, I.P->dQw, t, sh);
         for (int32_t i = threadIdx.x; i < n; i += kBB) {
             const double z_raw = I.P->x_bar[i] + c.sigma * (-I.P->dQw[i] + I.P->ATy_bar[i] - (double)I.P->cM[i]);
             I.P->z_bar[i] = (I.P->x_bar[i] - z_raw) / c.sigma;
@@ -689,7 +871,96 @@ __global__ __launch_bounds__(kBB) void k_batch_init(BArgs<MT> a, int has_x0, int
         }
         __syncthreads();
     }
-    bresid(I, true, rec, t, sh);
+    bresid<QK>(I, true, rec, t, sh);
+}
+
+// ---------------- scaling of CSR instances -------------------------------
+// scaling! (src/utils.jl:350-804) of every instance by its own block, on
+// FP64 working copies of the data: nruiz Ruiz passes (statistic sqrt(max|.|))
+// then, with pc, one Pock-Chambolle pass (sqrt(sum|.|)); the column
+// statistic runs over A's column and Q's row, the row statistic over A's
+// row; then Q = (Q + Q')/2 over the symmetric pattern, and the 2-norms of the
+// scaled b and c for sigma_0.  The formulas are build_scaled's
+// (hprqp_solver.hip), so a batched instance is scaled exactly as a single one.
+struct BScale {
+    const int32_t *n, *m;
+    const int64_t *noff, *moff, *qrp_off, *arp_off, *atrp_off;
+    const int32_t *q_rp, *q_ci, *q_tpos, *a_rp, *a_ci, *at_rp, *at_ci;
+    double *qv, *av, *atv, *c, *l, *u, *AL, *AU, *cn, *rn, *tc, *tr, *qtmp, *rec;
+    int nruiz, pc;
+};
+
+__global__ __launch_bounds__(kBB) void k_batch_scale(BScale g) {
+    __shared__ double sh[kBW];
+    const int b = blockIdx.x;
+    const int32_t n = g.n[b], m = g.m[b];
+    const int64_t no = g.noff[b], mo = g.moff[b];
+    const int32_t *qrp = g.q_rp + g.qrp_off[b], *arp = g.a_rp + g.arp_off[b], *atrp = g.at_rp + g.atrp_off[b];
+    double *cn = g.cn + no, *rn = g.rn + mo, *tc = g.tc + no, *tr = g.tr + mo;
+    for (int32_t i = threadIdx.x; i < n; i += kBB) cn[i] = 1.0;
+    for (int32_t r = threadIdx.x; r < m; r += kBB) rn[r] = 1.0;
+    const int total = g.nruiz + (g.pc ? 1 : 0);
+    for (int pass = 0; pass < total; ++pass) {
+        const bool ruiz = pass < g.nruiz;
+        for (int32_t j = threadIdx.x; j < n; j += kBB) {
+            double sa = 0.0, sq = 0.0;
+            for (int32_t k = atrp[j]; k < atrp[j + 1]; ++k) sa = ruiz ? dmax(sa, fabs(g.atv[k])) : sa + fabs(g.atv[k]);
+            for (int32_t k = qrp[j]; k < qrp[j + 1]; ++k) sq = ruiz ? dmax(sq, fabs(g.qv[k])) : sq + fabs(g.qv[k]);
+            const double t = ruiz ? dmax(sq, sa) : sq + sa;
+            tc[j] = t > 0.0 ? sqrt(t) : 1.0;
+        }
+        for (int32_t r = threadIdx.x; r < m; r += kBB) {
+            double sr = 0.0;
+            for (int32_t k = arp[r]; k < arp[r + 1]; ++k) sr = ruiz ? dmax(sr, fabs(g.av[k])) : sr + fabs(g.av[k]);
+            tr[r] = sr > 0.0 ? sqrt(sr) : 1.0;
+        }
+        __syncthreads();
+        for (int32_t j = threadIdx.x; j < n; j += kBB) {
+            const double t = tc[j];
+            cn[j] *= t;
+            g.c[no + j] /= t;
+            g.l[no + j] *= t;
+            g.u[no + j] *= t;
+        }
+        for (int32_t r = threadIdx.x; r < m; r += kBB) {
+            const double t = tr[r];
+            rn[r] *= t;
+            g.AL[mo + r] /= t;
+            g.AU[mo + r] /= t;
+        }
+        for (int32_t i = threadIdx.x; i < n; i += kBB) {
+            const double inv = 1.0 / tc[i];
+            for (int32_t k = qrp[i]; k < qrp[i + 1]; ++k) g.qv[k] = (g.qv[k] * inv) / tc[g.q_ci[k]];
+        }
+        for (int32_t r = threadIdx.x; r < m; r += kBB) {
+            const double inv = 1.0 / tr[r];
+            for (int32_t k = arp[r]; k < arp[r + 1]; ++k) g.av[k] = (g.av[k] * inv) / tc[g.a_ci[k]];
+        }
+        for (int32_t j = threadIdx.x; j < n; j += kBB) {
+            const double inv = 1.0 / tc[j];
+            for (int32_t k = atrp[j]; k < atrp[j + 1]; ++k) g.atv[k] = (g.atv[k] * inv) / tr[g.at_ci[k]];
+        }
+        __syncthreads();
+    }
+    // Q = (Q + Q')/2 (src/utils.jl:772-777)
+    const int32_t q0 = qrp[0], q1 = qrp[n];
+    for (int32_t k = q0 + (int32_t)threadIdx.x; k < q1; k += kBB) g.qtmp[k] = (g.qv[k] + g.qv[g.q_tpos[k]]) / 2.0;
+    __syncthreads();
+    for (int32_t k = q0 + (int32_t)threadIdx.x; k < q1; k += kBB) g.qv[k] = g.qtmp[k];
+    // norms of the scaled b (max(|AL|, |AU|) of the finite bounds) and c
+    double nb = 0.0, nc = 0.0;
+    for (int32_t r = threadIdx.x; r < m; r += kBB) {
+        const double lo = g.AL[mo + r], hi = g.AU[mo + r];
+        const double t = dmax(fabs(lo == -INFINITY ? 0.0 : lo), fabs(hi == INFINITY ? 0.0 : hi));
+        nb += t * t;
+    }
+    for (int32_t j = threadIdx.x; j < n; j += kBB) nc += g.c[no + j] * g.c[no + j];
+    nb = bsum(nb, sh);
+    nc = bsum(nc, sh);
+    if (threadIdx.x == 0) {
+        g.rec[(int64_t)b * R_COUNT + R_NB] = sqrt(nb);
+        g.rec[(int64_t)b * R_COUNT + R_NC] = sqrt(nc);
+    }
 }
 
 // ---------------- lambda_max by Lanczos ----------------------------------
@@ -825,7 +1096,7 @@ __device__ double blanczos(int32_t dim, Op op, double *v0, double *v1, double *v
 
 // per-instance lambda_max of A A^T and Q in one launch; start vectors were
 // written by the host (m-part in pz, n-part in dQw)
-template <class MT>
+template <class MT, int QK>
 __global__ __launch_bounds__(kBB) void k_batch_power(BArgs<MT> a, double tol, int max_it) {
     __shared__ double t[64];
     __shared__ double sh[kBW];
@@ -850,7 +1121,7 @@ __global__ __launch_bounds__(kBB) void k_batch_power(BArgs<MT> a, double tol, in
         rec[R_POW_A] = lam;
         rec[R_POW_ITA] = (double)steps;
     }
-    auto opQ = [&](const double *v, double *out) { lr_apply(n, k, I.P->U, I.P->d, v, out, t, sh); };
+    auto opQ = [&](const double *v, double *out) { bq_apply<QK>(I, v, out, t, sh); };
     const double lq = blanczos(n, opQ, I.P->dQw, I.P->pq, I.P->pq2, tol, max_it, &steps, sh);
     if (threadIdx.x == 0) {
         rec[R_POW_Q] = lq;
@@ -910,7 +1181,89 @@ struct BatchWork {
     DevVec<double> au;  // A_i U_i of every instance
     DevVec<BCtl> ctl;
     DevVec<double> rec;
+    // CSR kind: the scaled data of this solve (FP64, and FP32 for the mixed
+    // form), the scaling's column and row norms, scratch of the (Q + Q')/2
+    DevVec<double> sq, sa, sat, sc, sl, su, sAL, sAU, cn, rn, qtmp;
+    DevVec<float> sq32, sa32, sat32, sc32, sl32, su32, sAL32, sAU32;
 };
+
+template <class F>
+static void beach(hipStream_t s, int64_t n, const F &f) {
+    if (n <= 0) return;
+    k_each<F><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(n, f);
+    HPRQP_HIP(hipGetLastError());
+}
+
+// the CSR kind's per-solve scaling: working copies of the data, scaled on
+// the device by k_batch_scale (column / row statistics' scratch in tc, tr)
+static void batch_scale_data(hipStream_t s, hprqp_batch *B, BatchWork &W, const hprqp_params *p, bool mixed,
+                             double *tc, double *tr) {
+    const int64_t N = B->Ntot, M = B->Mtot;
+    auto copy = [&](DevVec<double> &dst, const DevVec<double> &src, int64_t len) {
+        dst.alloc(std::max<int64_t>(len, 1));
+        if (len) HPRQP_HIP(hipMemcpyAsync(dst.p, src.p, sizeof(double) * len, hipMemcpyDeviceToDevice, s));
+    };
+    copy(W.sq, B->q_v, B->nnzQ);
+    copy(W.sa, B->a_v, B->nnzA);
+    copy(W.sat, B->at_v, B->nnzA);
+    copy(W.sc, B->c, N);
+    copy(W.sl, B->l, N);
+    copy(W.su, B->u, N);
+    copy(W.sAL, B->AL, M);
+    copy(W.sAU, B->AU, M);
+    W.cn.alloc(std::max<int64_t>(N, 1));
+    W.rn.alloc(std::max<int64_t>(M, 1));
+    W.qtmp.alloc(std::max<int64_t>(B->nnzQ, 1));
+    BScale g{};
+    g.n = B->dn.p;
+    g.m = B->dm.p;
+    g.noff = B->dnoff.p;
+    g.moff = B->dmoff.p;
+    g.qrp_off = B->dqrp_off.p;
+    g.arp_off = B->darp_off.p;
+    g.atrp_off = B->datrp_off.p;
+    g.q_rp = B->q_rp.p;
+    g.q_ci = B->q_ci.p;
+    g.q_tpos = B->q_tpos.p;
+    g.a_rp = B->a_rp.p;
+    g.a_ci = B->a_ci.p;
+    g.at_rp = B->at_rp.p;
+    g.at_ci = B->at_ci.p;
+    g.qv = W.sq.p;
+    g.av = W.sa.p;
+    g.atv = W.sat.p;
+    g.c = W.sc.p;
+    g.l = W.sl.p;
+    g.u = W.su.p;
+    g.AL = W.sAL.p;
+    g.AU = W.sAU.p;
+    g.cn = W.cn.p;
+    g.rn = W.rn.p;
+    g.tc = tc;
+    g.tr = tr;
+    g.qtmp = W.qtmp.p;
+    g.rec = W.rec.p;
+    g.nruiz = p->use_Ruiz_scaling ? 10 : 0;
+    g.pc = p->use_Pock_Chambolle_scaling ? 1 : 0;
+    k_batch_scale<<<B->count, kBB, 0, s>>>(g);
+    HPRQP_HIP(hipGetLastError());
+    if (mixed) {
+        auto to32 = [&](DevVec<float> &dst, const DevVec<double> &src, int64_t len) {
+            dst.alloc(std::max<int64_t>(len, 1));
+            float *d = dst.p;
+            const double *a = src.p;
+            beach(s, len, [=] __device__(int64_t i) { d[i] = (float)a[i]; });
+        };
+        to32(W.sq32, W.sq, B->nnzQ);
+        to32(W.sa32, W.sa, B->nnzA);
+        to32(W.sat32, W.sat, B->nnzA);
+        to32(W.sc32, W.sc, N);
+        to32(W.sl32, W.sl, N);
+        to32(W.su32, W.su, N);
+        to32(W.sAL32, W.sAL, M);
+        to32(W.sAU32, W.sAU, M);
+    }
+}
 
 template <class MT>
 static BArgs<MT> make_args(hprqp_batch *B, BatchWork &W) {
@@ -941,7 +1294,44 @@ static BArgs<MT> make_args(hprqp_batch *B, BatchWork &W) {
     a.c0 = B->dc0.p;
     a.nbi = B->dnbi.p;
     a.nci = B->dnci.p;
-    if constexpr (std::is_same<MT, double>::value) {
+    if (B->qkind == 1) {
+        // the scaled data of this solve
+        a.a_v64 = W.sa.p;
+        a.at_v64 = W.sat.p;
+        a.c = W.sc.p;
+        a.l = W.sl.p;
+        a.u = W.su.p;
+        a.AL = W.sAL.p;
+        a.AU = W.sAU.p;
+        a.q_rp = B->q_rp.p;
+        a.q_ci = B->q_ci.p;
+        a.qrp_off = B->dqrp_off.p;
+        a.q_v64 = W.sq.p;
+        a.cn = W.cn.p;
+        a.rn = W.rn.p;
+        if constexpr (std::is_same<MT, double>::value) {
+            a.q_v = W.sq.p;
+            a.a_v = W.sa.p;
+            a.at_v = W.sat.p;
+            a.cM = W.sc.p;
+            a.lM = W.sl.p;
+            a.uM = W.su.p;
+            a.ALM = W.sAL.p;
+            a.AUM = W.sAU.p;
+        } else {
+            a.q_v = W.sq32.p;
+            a.a_v = W.sa32.p;
+            a.at_v = W.sat32.p;
+            a.cM = W.sc32.p;
+            a.lM = W.sl32.p;
+            a.uM = W.su32.p;
+            a.ALM = W.sAL32.p;
+            a.AUM = W.sAU32.p;
+        }
+    }
+    if (B->qkind == 1) {
+        // filled after the scaling below
+    } else if constexpr (std::is_same<MT, double>::value) {
         a.U = B->U.p;
         a.d = B->d.p;
         a.a_v = B->a_v.p;
@@ -1007,9 +1397,15 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
     W.ctl.alloc(cnt);
     W.rec.alloc((int64_t)cnt * R_COUNT);
     HPRQP_HIP(hipMemsetAsync(W.rec.p, 0, sizeof(double) * W.rec.n, s));
+    const bool csrq = B->qkind == 1;
+    // CSR kind: scale every instance (the statistics' scratch: pq, pz, which
+    // the power method overwrites later)
+    if (csrq) batch_scale_data(s, B, W, p, !std::is_same<MT, double>::value, W.vn[19].p, W.vm[5].p);
     BArgs<MT> a = make_args<MT>(B, W);
-    k_batch_au<MT><<<cnt, kBB, 0, s>>>(a);
-    HPRQP_HIP(hipGetLastError());
+    if (!csrq) {
+        k_batch_au<MT><<<cnt, kBB, 0, s>>>(a);
+        HPRQP_HIP(hipGetLastError());
+    }
     a.tol = p->stoptol;
     a.print_freq = p->print_frequency;
     // pinned staging for the per-instance control and records
@@ -1050,7 +1446,10 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
         HPRQP_HIP(hipMemcpyAsync(a.pz, zm.data(), sizeof(double) * M, hipMemcpyHostToDevice, s));
         HPRQP_HIP(hipMemcpyAsync(a.dQw, zn.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
         put_ctl();
-        k_batch_power<MT><<<cnt, kBB, 0, s>>>(a, 1e-4, 5000);
+        if (csrq)
+            k_batch_power<MT, 1><<<cnt, kBB, 0, s>>>(a, 1e-4, 5000);
+        else
+            k_batch_power<MT, 0><<<cnt, kBB, 0, s>>>(a, 1e-4, 5000);
         HPRQP_HIP(hipGetLastError());
         get_rec();
     }
@@ -1060,8 +1459,9 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
         S.lamA = B->m[b] > 0 ? r[R_POW_A] * p->eig_factor : 0.0;
         S.lamQ = r[R_POW_Q] * p->eig_factor;
         S.power_it = (int64_t)(r[R_POW_ITA] + r[R_POW_ITQ]);
-        // sigma_0 (src/algorithm.jl:839-851), unscaled data
-        const double nb = B->norm_b2[b], nc = B->norm_c2[b];
+        // sigma_0 (src/algorithm.jl:839-851) from the norms of the data the
+        // iteration sees (the operator kind is not scaled)
+        const double nb = csrq ? r[R_NB] : B->norm_b2[b], nc = csrq ? r[R_NC] : B->norm_c2[b];
         S.sigma = (p->sigma == -1) ? ((nc > 1e-16 && nb > 1e-16 && nb < 1e16 && nc < 1e16) ? nb / nc : 1.0)
                                    : p->sigma;
         S.best_sigma = S.sigma;
@@ -1080,7 +1480,13 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
     else if (y0 && M)
         HPRQP_HIP(hipMemcpyAsync(a.y_bar, y0, sizeof(double) * M, hipMemcpyHostToDevice, s));
     put_ctl();
-    k_batch_init<MT><<<cnt, kBB, 0, s>>>(a, x0 != nullptr || x0_dev != nullptr, y0 != nullptr || y0_dev != nullptr);
+    {
+        const int hx = x0 != nullptr || x0_dev != nullptr, hy = y0 != nullptr || y0_dev != nullptr;
+        if (csrq)
+            k_batch_init<MT, 1><<<cnt, kBB, 0, s>>>(a, hx, hy);
+        else
+            k_batch_init<MT, 0><<<cnt, kBB, 0, s>>>(a, hx, hy);
+    }
     HPRQP_HIP(hipGetLastError());
     get_rec();
     // one check interval = ctl upload + interval kernel + record download,
@@ -1090,7 +1496,10 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
     hipGraphExec_t gexec = nullptr;
     HPRQP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     HPRQP_HIP(hipMemcpyAsync(W.ctl.p, hctl, sizeof(BCtl) * cnt, hipMemcpyHostToDevice, s));
-    k_batch_interval<MT><<<cnt, kBB, 0, s>>>(a);
+    if (csrq)
+        k_batch_interval<MT, 1><<<cnt, kBB, 0, s>>>(a);
+    else
+        k_batch_interval<MT, 0><<<cnt, kBB, 0, s>>>(a);
     HPRQP_HIP(hipMemcpyAsync(hrec, W.rec.p, sizeof(double) * R_COUNT * cnt, hipMemcpyDeviceToHost, s));
     HPRQP_HIP(hipStreamEndCapture(s, &graph));
     HPRQP_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
@@ -1255,6 +1664,15 @@ static void batch_solve_impl(hprqp_ctx *ctx, hprqp_batch *B, const hprqp_params 
         if (kkt_out) kkt_out[b] = S.kkt;
     }
     res->total_iter = tot_it;
+    if (csrq) {
+        // results in the original space (src/algorithm.jl:630-640; b_scale =
+        // c_scale = 1): x = x_bar / col_norm, y = y_bar / row_norm
+        double *xb = a.x_bar, *yb = a.y_bar;
+        const double *cn = W.cn.p, *rn = W.rn.p;
+        beach(s, N, [=] __device__(int64_t i) { xb[i] /= cn[i]; });
+        beach(s, M, [=] __device__(int64_t i) { yb[i] /= rn[i]; });
+        HPRQP_HIP(hipStreamSynchronize(s));
+    }
     if (x_out) HPRQP_HIP(hipMemcpy(x_out, a.x_bar, sizeof(double) * N, hipMemcpyDeviceToHost));
     if (y_out && M) HPRQP_HIP(hipMemcpy(y_out, a.y_bar, sizeof(double) * M, hipMemcpyDeviceToHost));
 }

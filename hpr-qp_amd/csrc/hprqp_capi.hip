@@ -53,6 +53,52 @@ HostCsr host_csr_from(int32_t nrow, int32_t ncol, int64_t nnz, const int32_t *ro
     return h;
 }
 
+// the pattern of a square a made symmetric: where only one of (i, j),
+// (j, i) is stored, the other is added as an explicit zero
+HostCsr host_csr_symmetric_pattern(const HostCsr &q) {
+    const int32_t n = q.nrow;
+    HostCsr qt = host_csr_transpose(q);
+    HostCsr s;
+    s.nrow = s.ncol = n;
+    s.rp.assign((size_t)n + 1, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        int32_t a = q.rp[i], ae = q.rp[i + 1], b = qt.rp[i], be = qt.rp[i + 1];
+        while (a < ae || b < be) {
+            const int32_t ca = a < ae ? q.ci[a] : INT32_MAX, cb = b < be ? qt.ci[b] : INT32_MAX;
+            if (ca == cb) {
+                s.ci.push_back(ca);
+                s.v.push_back(q.v[a]);
+                ++a;
+                ++b;
+            } else if (ca < cb) {
+                s.ci.push_back(ca);
+                s.v.push_back(q.v[a]);
+                ++a;
+            } else {
+                s.ci.push_back(cb);
+                s.v.push_back(0.0);
+                ++b;
+            }
+        }
+        s.rp[i + 1] = (int32_t)s.ci.size();
+    }
+    return s;
+}
+
+// for a symmetric pattern: the position of each entry's transpose partner
+std::vector<int32_t> host_csr_transpose_positions(const HostCsr &s) {
+    std::vector<int32_t> tpos(s.ci.size());
+    std::vector<int32_t> next(s.rp.begin(), s.rp.end() - 1);
+    for (int32_t i = 0; i < s.nrow; ++i)
+        for (int32_t k = s.rp[i]; k < s.rp[i + 1]; ++k) {
+            // rows are visited in increasing i, so the entries (j, i) of row
+            // j appear in the same increasing order
+            const int32_t j = s.ci[k];
+            tpos[k] = next[j]++;
+        }
+    return tpos;
+}
+
 HostCsr host_csr_transpose(const HostCsr &a) {
     HostCsr t;
     t.nrow = a.ncol;
@@ -576,44 +622,8 @@ int hprqp_model_create(hprqp_ctx *ctx, int32_t n, int32_t m, hprqp_qop *Q, int64
             if (Q->kind == QOP_CSR) {
                 // symmetric pattern completion (explicit zeros) and the map
                 // from each entry to its transpose partner
-                const HostCsr &q = Q->hq;
-                HostCsr qt = hprqp::host_csr_transpose(q);
-                HostCsr &s = md->Qsym;
-                s.nrow = s.ncol = n;
-                s.rp.assign((size_t)n + 1, 0);
-                s.ci.clear();
-                s.v.clear();
-                for (int32_t i = 0; i < n; ++i) {
-                    int32_t a = q.rp[i], ae = q.rp[i + 1], b = qt.rp[i], be = qt.rp[i + 1];
-                    while (a < ae || b < be) {
-                        const int32_t ca = a < ae ? q.ci[a] : INT32_MAX, cb = b < be ? qt.ci[b] : INT32_MAX;
-                        if (ca == cb) {
-                            s.ci.push_back(ca);
-                            s.v.push_back(q.v[a]);
-                            ++a;
-                            ++b;
-                        } else if (ca < cb) {
-                            s.ci.push_back(ca);
-                            s.v.push_back(q.v[a]);
-                            ++a;
-                        } else {
-                            s.ci.push_back(cb);
-                            s.v.push_back(0.0);
-                            ++b;
-                        }
-                    }
-                    s.rp[i + 1] = (int32_t)s.ci.size();
-                }
-                // transpose positions: entry (i,j) at k <-> entry (j,i)
-                md->q_tpos.resize(s.ci.size());
-                std::vector<int32_t> next(s.rp.begin(), s.rp.end() - 1);
-                for (int32_t i = 0; i < n; ++i)
-                    for (int32_t k = s.rp[i]; k < s.rp[i + 1]; ++k) {
-                        // rows are visited in increasing i, so the entries
-                        // (j, i) of row j appear in the same increasing order
-                        const int32_t j = s.ci[k];
-                        md->q_tpos[k] = next[j]++;
-                    }
+                md->Qsym = hprqp::host_csr_symmetric_pattern(Q->hq);
+                md->q_tpos = hprqp::host_csr_transpose_positions(md->Qsym);
             }
         } catch (...) {
             delete md;
@@ -752,6 +762,160 @@ int hprqp_ws_download(hprqp_ctx *ctx, hprqp_ws *ws, int which, double *host, int
     return guarded(ctx, [&] { hprqp::ws_download(ws, which, host, len); });
 }
 
+}  // extern "C"
+
+static void batch_up32(hprqp::DevVec<float> &dst, const double *src, int64_t len) {
+    std::vector<float> f(src, src + len);
+    dst.alloc(std::max<int64_t>(len, 1));
+    if (len) HPRQP_HIP(hipMemcpy(dst.p, f.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+}
+
+// what the two batch kinds share: the instances' sizes and offsets, their
+// packed CSR A with per-instance transposes, the vectors and their norms,
+// uploaded in FP64 and FP32, and the largest-first dispatch order by
+// `cost` (per-iteration work of instance b given its nnz(A))
+template <class Cost>
+static void batch_pack_common(hprqp_ctx *ctx, hprqp_batch *B, int32_t count, const int32_t *n, const int32_t *m,
+                              int64_t a_nnz, const int32_t *a_rowptr, const int32_t *a_colidx, const double *a_vals,
+                              const double *c, const double *l, const double *u, const double *AL,
+                              const double *AU, const double *obj_constant, Cost cost_of) {
+    hipStream_t s = ctx->stream;
+    B->count = count;
+    B->n.assign(n, n + count);
+    B->m.assign(m, m + count);
+    B->noff.resize(count);
+    B->moff.resize(count);
+    std::vector<int64_t> arp_off(count), atrp_off(count), nnz_off(count);
+    int64_t N = 0, M = 0, NZ = 0, ARP = 0, ATRP = 0;
+    for (int32_t b = 0; b < count; ++b) {
+        HPRQP_REQUIRE(n[b] > 0 && m[b] >= 0, HPRQP_E_ARG, "instance dimensions out of range (n > 0, m >= 0)");
+        B->noff[b] = N;
+        B->moff[b] = M;
+        arp_off[b] = ARP;
+        atrp_off[b] = ATRP;
+        nnz_off[b] = NZ;
+        int32_t nnz_b = 0;
+        if (m[b] > 0) {
+            // the instance's whole rowptr is checked before its last
+            // entry is taken as the instance's nonzero count
+            HPRQP_REQUIRE(a_rowptr != nullptr, HPRQP_E_ARG, "NULL a_rowptr with m > 0");
+            const int32_t *rp = a_rowptr + ARP;
+            HPRQP_REQUIRE(rp[0] == 0, HPRQP_E_ARG, "rowptr inconsistent with nnz");
+            for (int32_t r = 0; r < m[b]; ++r) HPRQP_REQUIRE(rp[r] <= rp[r + 1], HPRQP_E_ARG, "rowptr not monotone");
+            nnz_b = rp[m[b]];
+        }
+        N += n[b];
+        M += m[b];
+        NZ += nnz_b;
+        ARP += m[b] + 1;
+        ATRP += n[b] + 1;
+        HPRQP_REQUIRE(NZ < INT32_MAX && N < INT32_MAX, HPRQP_E_ARG, "batch exceeds int32 indexing");
+    }
+    // the instances' nonzeros together fill a_colidx / a_vals exactly
+    HPRQP_REQUIRE(NZ == a_nnz, HPRQP_E_ARG, "rowptr inconsistent with nnz");
+    HPRQP_REQUIRE(NZ == 0 || (a_colidx && a_vals), HPRQP_E_ARG, "NULL a_colidx or a_vals with nnz > 0");
+    HPRQP_REQUIRE(M == 0 || (AL && AU), HPRQP_E_ARG, "NULL AL or AU with m > 0");
+    B->Ntot = N;
+    B->Mtot = M;
+    B->nnzA = NZ;
+    // packed CSR with row pointers made global, and the per-instance transposes
+    std::vector<int32_t> arp(ARP), aci(NZ), atrp(ATRP), atci(NZ);
+    std::vector<double> av(NZ), atv(NZ);
+    for (int32_t b = 0; b < count; ++b) {
+        if (m[b] > 0) {
+            HostCsr h = hprqp::host_csr_from(m[b], n[b], a_rowptr[arp_off[b] + m[b]], a_rowptr + arp_off[b],
+                                             a_colidx + nnz_off[b], a_vals + nnz_off[b], 0);
+            HostCsr t = hprqp::host_csr_transpose(h);
+            for (int32_t r = 0; r <= m[b]; ++r) arp[arp_off[b] + r] = (int32_t)(h.rp[r] + nnz_off[b]);
+            for (int32_t j = 0; j <= n[b]; ++j) atrp[atrp_off[b] + j] = (int32_t)(t.rp[j] + nnz_off[b]);
+            std::copy(h.ci.begin(), h.ci.end(), aci.begin() + nnz_off[b]);
+            std::copy(h.v.begin(), h.v.end(), av.begin() + nnz_off[b]);
+            std::copy(t.ci.begin(), t.ci.end(), atci.begin() + nnz_off[b]);
+            std::copy(t.v.begin(), t.v.end(), atv.begin() + nnz_off[b]);
+        } else {
+            arp[arp_off[b]] = (int32_t)nnz_off[b];
+            for (int32_t j = 0; j <= n[b]; ++j) atrp[atrp_off[b] + j] = (int32_t)nnz_off[b];
+        }
+    }
+    // per-instance norms of the data as given (src/utils.jl:404-408)
+    B->obj_constant.assign(count, 0.0);
+    B->norm_b2.assign(count, 0.0);
+    B->norm_c2.assign(count, 0.0);
+    B->norm_b_inf.assign(count, 0.0);
+    B->norm_c_inf.assign(count, 0.0);
+    for (int32_t b = 0; b < count; ++b) {
+        if (obj_constant) B->obj_constant[b] = obj_constant[b];
+        double s2 = 0, si = 0;
+        for (int32_t i = 0; i < n[b]; ++i) {
+            const int64_t g = B->noff[b] + i;
+            HPRQP_REQUIRE(std::isfinite(c[g]), HPRQP_E_DATA, "c vector contains NaN or Inf values");
+            HPRQP_REQUIRE(!std::isnan(l[g]) && !std::isnan(u[g]), HPRQP_E_DATA,
+                          "variable bounds l or u contain NaN values");
+            HPRQP_REQUIRE(!(l[g] > u[g]), HPRQP_E_DATA, "infeasible variable bounds: l > u");
+            s2 += c[g] * c[g];
+            si = std::max(si, std::fabs(c[g]));
+        }
+        B->norm_c2[b] = std::sqrt(s2);
+        B->norm_c_inf[b] = si;
+        s2 = si = 0;
+        for (int32_t r = 0; r < m[b]; ++r) {
+            const int64_t g = B->moff[b] + r;
+            HPRQP_REQUIRE(!std::isnan(AL[g]) && !std::isnan(AU[g]), HPRQP_E_DATA,
+                          "constraint bounds AL or AU contain NaN values");
+            HPRQP_REQUIRE(!(AL[g] > AU[g]), HPRQP_E_DATA, "infeasible constraint bounds: AL > AU");
+            const double a = AL[g] == -INFINITY ? 0.0 : AL[g], bb = AU[g] == INFINITY ? 0.0 : AU[g];
+            const double t = std::max(std::fabs(a), std::fabs(bb));
+            s2 += t * t;
+            si = std::max(si, t);
+        }
+        B->norm_b2[b] = std::sqrt(s2);
+        B->norm_b_inf[b] = si;
+    }
+    auto up = [&](auto &dst, const auto *src, int64_t len) {
+        dst.alloc(std::max<int64_t>(len, 1));
+        if (len) HPRQP_HIP(hipMemcpyAsync(dst.p, src, sizeof(*src) * len, hipMemcpyHostToDevice, s));
+    };
+    up(B->dn, n, count);
+    up(B->dm, m, count);
+    up(B->dnoff, B->noff.data(), count);
+    up(B->dmoff, B->moff.data(), count);
+    up(B->darp_off, arp_off.data(), count);
+    up(B->datrp_off, atrp_off.data(), count);
+    up(B->a_rp, arp.data(), ARP);
+    up(B->a_ci, aci.data(), NZ);
+    up(B->a_v, av.data(), NZ);
+    up(B->at_rp, atrp.data(), ATRP);
+    up(B->at_ci, atci.data(), NZ);
+    up(B->at_v, atv.data(), NZ);
+    up(B->c, c, N);
+    up(B->l, l, N);
+    up(B->u, u, N);
+    up(B->AL, AL, M);
+    up(B->AU, AU, M);
+    // largest-first dispatch (longest-processing-time order)
+    std::vector<int32_t> order(count);
+    std::vector<double> cost(count);
+    for (int32_t b = 0; b < count; ++b) {
+        order[b] = b;
+        cost[b] = cost_of(b, (int64_t)arp[arp_off[b] + m[b]] - nnz_off[b]);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
+    up(B->dorder, order.data(), count);
+    up(B->dc0, B->obj_constant.data(), count);
+    up(B->dnbi, B->norm_b_inf.data(), count);
+    up(B->dnci, B->norm_c_inf.data(), count);
+    HPRQP_HIP(hipStreamSynchronize(s));
+    batch_up32(B->a_v32, av.data(), NZ);
+    batch_up32(B->at_v32, atv.data(), NZ);
+    batch_up32(B->c32, c, N);
+    batch_up32(B->l32, l, N);
+    batch_up32(B->u32, u, N);
+    batch_up32(B->AL32, AL, M);
+    batch_up32(B->AU32, AU, M);
+}
+
+extern "C" {
+
 int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const int32_t *m, const int32_t *k,
                        const double *U, const double *diag, int64_t a_nnz, const int32_t *a_rowptr,
                        const int32_t *a_colidx,
@@ -764,166 +928,150 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n, const in
         hprqp_batch *B = new hprqp_batch();
         try {
             hipStream_t s = ctx->stream;
-            B->count = count;
-            B->n.assign(n, n + count);
-            B->m.assign(m, m + count);
+            B->qkind = 0;
             B->k.assign(k, k + count);
-            B->noff.resize(count);
-            B->moff.resize(count);
             B->uoff.resize(count);
             B->auoff.resize(count);
-            std::vector<int64_t> arp_off(count), atrp_off(count), nnz_off(count);
-            int64_t N = 0, M = 0, UT = 0, NZ = 0, ARP = 0, ATRP = 0;
+            int64_t UT = 0;
             for (int32_t b = 0; b < count; ++b) {
                 HPRQP_REQUIRE(n[b] > 0 && m[b] >= 0 && k[b] >= 0 && k[b] <= 64, HPRQP_E_ARG,
                               "instance dimensions out of range (n > 0, m >= 0, 0 <= k <= 64)");
-                B->noff[b] = N;
-                B->moff[b] = M;
                 B->uoff[b] = UT;
                 B->auoff[b] = B->AUtot;
                 B->AUtot += (int64_t)m[b] * k[b];
-                arp_off[b] = ARP;
-                atrp_off[b] = ATRP;
-                nnz_off[b] = NZ;
-                int32_t nnz_b = 0;
-                if (m[b] > 0) {
-                    // the instance's whole rowptr is checked before its last
-                    // entry is taken as the instance's nonzero count
-                    HPRQP_REQUIRE(a_rowptr != nullptr, HPRQP_E_ARG, "NULL a_rowptr with m > 0");
-                    const int32_t *rp = a_rowptr + ARP;
-                    HPRQP_REQUIRE(rp[0] == 0, HPRQP_E_ARG, "rowptr inconsistent with nnz");
-                    for (int32_t r = 0; r < m[b]; ++r)
-                        HPRQP_REQUIRE(rp[r] <= rp[r + 1], HPRQP_E_ARG, "rowptr not monotone");
-                    nnz_b = rp[m[b]];
-                }
-                N += n[b];
-                M += m[b];
                 UT += (int64_t)n[b] * k[b];
-                NZ += nnz_b;
-                ARP += m[b] + 1;
-                ATRP += n[b] + 1;
-                HPRQP_REQUIRE(NZ < INT32_MAX && N < INT32_MAX, HPRQP_E_ARG, "batch exceeds int32 indexing");
             }
-            // the instances' nonzeros together fill a_colidx / a_vals exactly
-            HPRQP_REQUIRE(NZ == a_nnz, HPRQP_E_ARG, "rowptr inconsistent with nnz");
-            HPRQP_REQUIRE(NZ == 0 || (a_colidx && a_vals), HPRQP_E_ARG, "NULL a_colidx or a_vals with nnz > 0");
-            HPRQP_REQUIRE(M == 0 || (AL && AU), HPRQP_E_ARG, "NULL AL or AU with m > 0");
             HPRQP_REQUIRE(UT == 0 || U, HPRQP_E_ARG, "NULL U with k > 0");
-            for (int64_t i = 0; i < UT; ++i)
-                HPRQP_REQUIRE(std::isfinite(U[i]), HPRQP_E_DATA, "U has NaN/Inf");
-            B->Ntot = N;
-            B->Mtot = M;
+            for (int64_t i = 0; i < UT; ++i) HPRQP_REQUIRE(std::isfinite(U[i]), HPRQP_E_DATA, "U has NaN/Inf");
+            int64_t N = 0;
+            for (int32_t b = 0; b < count; ++b) N += n[b];
+            for (int64_t i = 0; i < N; ++i) HPRQP_REQUIRE(std::isfinite(diag[i]), HPRQP_E_DATA, "diag has NaN/Inf");
             B->Utot = UT;
-            B->nnzA = NZ;
-            // packed CSR with row pointers made global, and the per-instance transposes
-            std::vector<int32_t> arp(ARP), aci(NZ), atrp(ATRP), atci(NZ);
-            std::vector<double> av(NZ), atv(NZ);
-            for (int32_t b = 0; b < count; ++b) {
-                if (m[b] > 0) {
-                    HostCsr h = hprqp::host_csr_from(m[b], n[b], a_rowptr[arp_off[b] + m[b]],
-                                                     a_rowptr + arp_off[b], a_colidx + nnz_off[b],
-                                                     a_vals + nnz_off[b], 0);
-                    HostCsr t = hprqp::host_csr_transpose(h);
-                    for (int32_t r = 0; r <= m[b]; ++r) arp[arp_off[b] + r] = (int32_t)(h.rp[r] + nnz_off[b]);
-                    for (int32_t j = 0; j <= n[b]; ++j) atrp[atrp_off[b] + j] = (int32_t)(t.rp[j] + nnz_off[b]);
-                    std::copy(h.ci.begin(), h.ci.end(), aci.begin() + nnz_off[b]);
-                    std::copy(h.v.begin(), h.v.end(), av.begin() + nnz_off[b]);
-                    std::copy(t.ci.begin(), t.ci.end(), atci.begin() + nnz_off[b]);
-                    std::copy(t.v.begin(), t.v.end(), atv.begin() + nnz_off[b]);
-                } else {
-                    arp[arp_off[b]] = (int32_t)nnz_off[b];
-                    for (int32_t j = 0; j <= n[b]; ++j) atrp[atrp_off[b] + j] = (int32_t)nnz_off[b];
-                }
-            }
-            // per-instance norms (operator Q: no scaling, src/utils.jl:364-406)
-            B->obj_constant.assign(count, 0.0);
-            B->norm_b2.assign(count, 0.0);
-            B->norm_c2.assign(count, 0.0);
-            B->norm_b_inf.assign(count, 0.0);
-            B->norm_c_inf.assign(count, 0.0);
-            for (int32_t b = 0; b < count; ++b) {
-                if (obj_constant) B->obj_constant[b] = obj_constant[b];
-                double s2 = 0, si = 0;
-                for (int32_t i = 0; i < n[b]; ++i) {
-                    const int64_t g = B->noff[b] + i;
-                    HPRQP_REQUIRE(std::isfinite(c[g]) && std::isfinite(diag[g]), HPRQP_E_DATA,
-                                  "c or diag has NaN/Inf");
-                    HPRQP_REQUIRE(!std::isnan(l[g]) && !std::isnan(u[g]), HPRQP_E_DATA,
-                                  "variable bounds l or u contain NaN values");
-                    HPRQP_REQUIRE(!(l[g] > u[g]), HPRQP_E_DATA, "infeasible variable bounds: l > u");
-                    s2 += c[g] * c[g];
-                    si = std::max(si, std::fabs(c[g]));
-                }
-                B->norm_c2[b] = std::sqrt(s2);
-                B->norm_c_inf[b] = si;
-                s2 = si = 0;
-                for (int32_t r = 0; r < m[b]; ++r) {
-                    const int64_t g = B->moff[b] + r;
-                    HPRQP_REQUIRE(!std::isnan(AL[g]) && !std::isnan(AU[g]), HPRQP_E_DATA,
-                                  "constraint bounds AL or AU contain NaN values");
-                    HPRQP_REQUIRE(!(AL[g] > AU[g]), HPRQP_E_DATA, "infeasible constraint bounds: AL > AU");
-                    const double a = AL[g] == -INFINITY ? 0.0 : AL[g], bb = AU[g] == INFINITY ? 0.0 : AU[g];
-                    const double t = std::max(std::fabs(a), std::fabs(bb));
-                    s2 += t * t;
-                    si = std::max(si, t);
-                }
-                B->norm_b2[b] = std::sqrt(s2);
-                B->norm_b_inf[b] = si;
-            }
+            // per-iteration work: about two passes over U plus the A and A^T products
+            batch_pack_common(ctx, B, count, n, m, a_nnz, a_rowptr, a_colidx, a_vals, c, l, u, AL, AU, obj_constant,
+                              [&](int32_t b, int64_t nnz_b) {
+                                  return 4.0 * (double)n[b] * (k[b] + 1) + 2.0 * (double)nnz_b + m[b];
+                              });
             auto up = [&](auto &dst, const auto *src, int64_t len) {
                 dst.alloc(std::max<int64_t>(len, 1));
                 if (len) HPRQP_HIP(hipMemcpyAsync(dst.p, src, sizeof(*src) * len, hipMemcpyHostToDevice, s));
             };
-            auto up32 = [&](hprqp::DevVec<float> &dst, const double *src, int64_t len) {
-                std::vector<float> f(src, src + len);
-                dst.alloc(std::max<int64_t>(len, 1));
-                if (len) HPRQP_HIP(hipMemcpy(dst.p, f.data(), sizeof(float) * len, hipMemcpyHostToDevice));
-            };
-            up(B->dn, n, count);
-            up(B->dm, m, count);
             up(B->dk, k, count);
-            up(B->dnoff, B->noff.data(), count);
-            up(B->dmoff, B->moff.data(), count);
             up(B->duoff, B->uoff.data(), count);
             up(B->dauoff, B->auoff.data(), count);
-            up(B->darp_off, arp_off.data(), count);
-            up(B->datrp_off, atrp_off.data(), count);
             up(B->U, U, UT);
             up(B->d, diag, N);
-            up(B->a_rp, arp.data(), ARP);
-            up(B->a_ci, aci.data(), NZ);
-            up(B->a_v, av.data(), NZ);
-            up(B->at_rp, atrp.data(), ATRP);
-            up(B->at_ci, atci.data(), NZ);
-            up(B->at_v, atv.data(), NZ);
-            up(B->c, c, N);
-            up(B->l, l, N);
-            up(B->u, u, N);
-            up(B->AL, AL, M);
-            up(B->AU, AU, M);
-            // largest-first dispatch: per-iteration work of an instance is
-            // about four passes over U plus the A and A^T products
-            std::vector<int32_t> order(count);
-            std::vector<double> cost(count);
-            for (int32_t b = 0; b < count; ++b) {
-                order[b] = b;
-                cost[b] = 4.0 * (double)n[b] * (k[b] + 1) + 2.0 * (arp[arp_off[b] + m[b]] - nnz_off[b]) + m[b];
-            }
-            std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return cost[x] > cost[y]; });
-            up(B->dorder, order.data(), count);
-            up(B->dc0, B->obj_constant.data(), count);
-            up(B->dnbi, B->norm_b_inf.data(), count);
-            up(B->dnci, B->norm_c_inf.data(), count);
             HPRQP_HIP(hipStreamSynchronize(s));
-            up32(B->U32, U, UT);
-            up32(B->d32, diag, N);
-            up32(B->a_v32, av.data(), NZ);
-            up32(B->at_v32, atv.data(), NZ);
-            up32(B->c32, c, N);
-            up32(B->l32, l, N);
-            up32(B->u32, u, N);
-            up32(B->AL32, AL, M);
-            up32(B->AU32, AU, M);
+            batch_up32(B->U32, U, UT);
+            batch_up32(B->d32, diag, N);
+        } catch (...) {
+            delete B;
+            throw;
+        }
+        *out = B;
+    });
+}
+
+int hprqp_batch_create_csr(hprqp_ctx *ctx, int32_t count, const int32_t *n, const int32_t *m, int64_t q_nnz,
+                           const int32_t *q_rowptr, const int32_t *q_colidx, const double *q_vals, int64_t a_nnz,
+                           const int32_t *a_rowptr, const int32_t *a_colidx, const double *a_vals, const double *c,
+                           const double *l, const double *u, const double *AL, const double *AU,
+                           const double *obj_constant, hprqp_batch **out) {
+    CTX_CHECK(ctx);
+    if (!out || count <= 0 || !n || !m || !c || !l || !u || !q_rowptr) return HPRQP_E_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        hprqp_batch *B = new hprqp_batch();
+        try {
+            hipStream_t s = ctx->stream;
+            B->qkind = 1;
+            // per instance: Q's rowptr (n + 1 entries, from 0) checked whole
+            // before its last entry is taken as the instance's nnz(Q)
+            std::vector<int64_t> qnz_off(count), qrp_in(count);
+            int64_t QNZ = 0, QRP = 0;
+            for (int32_t b = 0; b < count; ++b) {
+                HPRQP_REQUIRE(n[b] > 0 && m[b] >= 0, HPRQP_E_ARG, "instance dimensions out of range (n > 0, m >= 0)");
+                const int32_t *rp = q_rowptr + QRP;
+                HPRQP_REQUIRE(rp[0] == 0, HPRQP_E_ARG, "Q rowptr inconsistent with nnz");
+                for (int32_t i = 0; i < n[b]; ++i) HPRQP_REQUIRE(rp[i] <= rp[i + 1], HPRQP_E_ARG, "Q rowptr not monotone");
+                qrp_in[b] = QRP;
+                qnz_off[b] = QNZ;
+                QNZ += rp[n[b]];
+                QRP += n[b] + 1;
+                HPRQP_REQUIRE(QNZ < INT32_MAX, HPRQP_E_ARG, "batch exceeds int32 indexing");
+            }
+            HPRQP_REQUIRE(QNZ == q_nnz, HPRQP_E_ARG, "Q rowptr inconsistent with nnz");
+            HPRQP_REQUIRE(QNZ == 0 || (q_colidx && q_vals), HPRQP_E_ARG, "NULL q_colidx or q_vals with nnz > 0");
+            // each Q as the single-instance model keeps it: the pattern made
+            // symmetric (explicit zeros) with the map from each entry to its
+            // transpose partner, for the post-scaling (Q + Q')/2
+            std::vector<int32_t> srp, sci, stp;
+            std::vector<double> sv;
+            std::vector<int64_t> qrp_off(count);
+            int64_t SNZ = 0, noff_run = 0;
+            for (int32_t b = 0; b < count; ++b) {
+                const int32_t nb = n[b];
+                HostCsr q = hprqp::host_csr_from(nb, nb, q_rowptr[qrp_in[b] + nb], q_rowptr + qrp_in[b],
+                                                 q_colidx + qnz_off[b], q_vals + qnz_off[b], 0);
+                HostCsr sym = hprqp::host_csr_symmetric_pattern(q);
+                std::vector<int32_t> tpos = hprqp::host_csr_transpose_positions(sym);
+                // the batch runs the standard path: a diagonal Q, an empty Q
+                // (LP) or mostly free variables (noC) take other paths
+                bool offdiag = false;
+                for (int32_t i = 0; i < nb && !offdiag; ++i) {
+                    double sum = 0.0, d = 0.0;
+                    for (int32_t t = q.rp[i]; t < q.rp[i + 1]; ++t) {
+                        sum += std::fabs(q.v[t]);
+                        if (q.ci[t] == i) d = q.v[t];
+                    }
+                    const double ad = std::fabs(d);
+                    offdiag = !(sum == ad || std::fabs(sum - ad) <= 1.4901161193847656e-8 * std::max(sum, ad));
+                }
+                HPRQP_REQUIRE(offdiag, HPRQP_E_UNSUP,
+                              "batched CSR instances need a Q with off-diagonal entries (diagonal or empty Q: solve singly)");
+                int64_t nfree = 0;
+                for (int32_t i = 0; i < nb; ++i)
+                    if (l[noff_run + i] == -INFINITY && u[noff_run + i] == INFINITY) nfree++;
+                HPRQP_REQUIRE(!((double)nfree > 0.8 * (double)nb), HPRQP_E_UNSUP,
+                              "batched CSR instances run the standard path: more than 80% free variables (noC) "
+                              "must be solved singly");
+                noff_run += nb;
+                qrp_off[b] = (int64_t)srp.size();
+                for (int32_t i = 0; i <= nb; ++i) srp.push_back((int32_t)(sym.rp[i] + SNZ));
+                sci.insert(sci.end(), sym.ci.begin(), sym.ci.end());
+                sv.insert(sv.end(), sym.v.begin(), sym.v.end());
+                for (int32_t t : tpos) stp.push_back((int32_t)(t + SNZ));
+                SNZ += (int64_t)sym.ci.size();
+                HPRQP_REQUIRE(SNZ < INT32_MAX, HPRQP_E_ARG, "batch exceeds int32 indexing");
+            }
+            B->nnzQ = SNZ;
+            B->qrp_off = qrp_off;
+            // per-iteration work: two Q products plus the A and A^T products
+            batch_pack_common(ctx, B, count, n, m, a_nnz, a_rowptr, a_colidx, a_vals, c, l, u, AL, AU, obj_constant,
+                              [&](int32_t b, int64_t nnz_b) {
+                                  const int64_t qn = srp[qrp_off[b] + n[b]] - srp[qrp_off[b]];
+                                  return 2.0 * (double)qn + 2.0 * (double)nnz_b + 4.0 * n[b] + m[b];
+                              });
+            auto up = [&](auto &dst, const auto *src, int64_t len) {
+                dst.alloc(std::max<int64_t>(len, 1));
+                if (len) HPRQP_HIP(hipMemcpyAsync(dst.p, src, sizeof(*src) * len, hipMemcpyHostToDevice, s));
+            };
+            up(B->dqrp_off, qrp_off.data(), count);
+            up(B->q_rp, srp.data(), (int64_t)srp.size());
+            up(B->q_ci, sci.data(), SNZ);
+            up(B->q_tpos, stp.data(), SNZ);
+            up(B->q_v, sv.data(), SNZ);
+            // unused by this kind, allocated so every pointer is valid
+            B->k.assign(count, 0);
+            std::vector<int64_t> zero(count, 0);
+            up(B->dk, B->k.data(), count);
+            up(B->duoff, zero.data(), count);
+            up(B->dauoff, zero.data(), count);
+            B->U.alloc(1);
+            B->d.alloc(1);
+            B->U32.alloc(1);
+            B->d32.alloc(1);
+            HPRQP_HIP(hipStreamSynchronize(s));
         } catch (...) {
             delete B;
             throw;

@@ -122,6 +122,8 @@ inline void launch_k(void (*kernel)(KA...), dim3 grid, dim3 block, uint32_t shm,
 HostCsr host_csr_from(int32_t nrow, int32_t ncol, int64_t nnz, const int32_t *rowptr,
                       const int32_t *colidx, const double *vals, int index_base);
 HostCsr host_csr_transpose(const HostCsr &a);
+HostCsr host_csr_symmetric_pattern(const HostCsr &q);
+std::vector<int32_t> host_csr_transpose_positions(const HostCsr &s);
 void upload_csr(hipStream_t s, const HostCsr &h, DevCsr &d);
 int default_lanes(const HostCsr &h);
 
@@ -205,7 +207,17 @@ struct hprqp_model {
 
 struct hprqp_batch {
     int32_t count = 0;
-    int64_t Ntot = 0, Mtot = 0, Utot = 0, nnzA = 0;
+    // qkind 0: Q_i = diag(d_i) + U_i U_i' (operator, no scaling); 1: Q_i
+    // sparse CSR (scaled per instance at each solve, as the single solve)
+    int qkind = 0;
+    int64_t Ntot = 0, Mtot = 0, Utot = 0, nnzA = 0, nnzQ = 0;
+    // qkind 1: the instances' Q with symmetric patterns, packed (row
+    // pointers global, instance b's at qrp_off[b]) with each entry's
+    // transpose position
+    std::vector<int64_t> qrp_off;
+    hprqp::DevVec<int64_t> dqrp_off;
+    hprqp::DevVec<int32_t> q_rp, q_ci, q_tpos;
+    hprqp::DevVec<double> q_v;
     std::vector<int32_t> n, m, k;
     std::vector<int64_t> noff, moff, uoff;
     std::vector<int64_t> auoff;  // A_i U_i (m_i x k_i) at sum_j m_j k_j

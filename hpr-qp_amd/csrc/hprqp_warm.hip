@@ -157,6 +157,8 @@ __global__ __launch_bounds__(kWB) void k_warm_finish(const int32_t *nv, const in
 
 void batch_predict(hprqp_ctx *ctx, hprqp_batch *B, hprqp_mlp *M, const float *feat_mean, const float *feat_std,
                    double y_scale, double *x0, double *y0) {
+    HPRQP_REQUIRE(B->qkind == 0, HPRQP_E_UNSUP,
+                  "the learned warm start takes diagonal-plus-low-rank batches (hprqp_batch_create)");
     HPRQP_REQUIRE(M->nin == kWarmFeatures && M->nout == 2, HPRQP_E_ARG,
                   "warm-start MLP must take 12 features and give 2 outputs");
     HPRQP_REQUIRE(y_scale > 0.0, HPRQP_E_ARG, "y_scale must be positive");

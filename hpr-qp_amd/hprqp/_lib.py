@@ -261,6 +261,8 @@ SIGNATURES = [
     ("hprqp_ws_download", C.c_int, [_P, _P, C.c_int, _P, C.c_int64]),
     ("hprqp_batch_create", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, C.c_int64, _P, _P, _P, _P, _P, _P, _P,
                                      _P, _P, C.POINTER(_P)]),
+    ("hprqp_batch_create_csr", C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, _P, _P, _P, _P,
+                                         _P, _P, _P, _P, _P, C.POINTER(_P)]),
     ("hprqp_batch_destroy", C.c_int, [_P, _P]),
     ("hprqp_mlp_create", C.c_int, [_P, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_P)]),
     ("hprqp_mlp_destroy", C.c_int, [_P, _P]),

@@ -1,8 +1,10 @@
 """Batched solves: thousands of instances in one launch per check interval.
 
-Each instance is a QP_info_cpu whose Q is a DiagonalPlusLowRank_Q_operator
-(diag + U U^T) and whose A is sparse.  The batch is packed once on the GPU
-(hprqp_batch_create); solve() runs every instance with its own restart and
+Each instance is a QP_info_cpu whose A is sparse and whose Q is either a
+DiagonalPlusLowRank_Q_operator (diag + U U^T; hprqp_batch_create) or a sparse
+matrix (hprqp_batch_create_csr: every instance scaled as the single solve
+scales it, then the same iteration).  One batch holds one kind.  The batch is
+packed once on the GPU; solve() runs every instance with its own restart and
 sigma state until each one reaches the tolerance (hprqp_batch_solve).
 """
 from __future__ import annotations
@@ -42,15 +44,26 @@ class BatchModel:
         cnt = len(self.models)
         n = np.array([mdl.n for mdl in self.models], dtype=np.int32)
         m = np.array([mdl.m for mdl in self.models], dtype=np.int32)
-        k = np.empty(cnt, dtype=np.int32)
-        U, d, rp, ci, av = [], [], [], [], []
+        lowrank = [isinstance(mdl.Q, DiagonalPlusLowRank_Q_operator) for mdl in self.models]
+        sparse = [sp.issparse(mdl.Q) for mdl in self.models]
+        if not (all(lowrank) or all(sparse)):
+            raise TypeError("batched instances need a DiagonalPlusLowRank_Q_operator Q or a sparse-matrix Q, "
+                            "the same kind for every instance")
+        self.kind = "lowrank" if all(lowrank) else "csr"
+        k = np.zeros(cnt, dtype=np.int32)
+        U, d, rp, ci, av, qrp, qci, qv = [], [], [], [], [], [], [], []
         for i, mdl in enumerate(self.models):
             Q = mdl.Q
-            if not isinstance(Q, DiagonalPlusLowRank_Q_operator):
-                raise TypeError("batched instances need a DiagonalPlusLowRank_Q_operator Q")
-            k[i] = Q.U.shape[1]
-            U.append(np.asarray(Q.U, dtype=np.float64).ravel(order="F"))
-            d.append(Q.diag)
+            if self.kind == "lowrank":
+                k[i] = Q.U.shape[1]
+                U.append(np.asarray(Q.U, dtype=np.float64).ravel(order="F"))
+                d.append(Q.diag)
+            else:
+                Qc = sp.csr_matrix(Q, dtype=np.float64)
+                Qc.sort_indices()
+                qrp.append(Qc.indptr.astype(np.int32))
+                qci.append(Qc.indices.astype(np.int32))
+                qv.append(Qc.data)
             A = sp.csr_matrix(mdl.A, dtype=np.float64)
             A.sort_indices()
             rp.append(A.indptr.astype(np.int32))
@@ -59,17 +72,24 @@ class BatchModel:
         cat = lambda lst, dt: np.ascontiguousarray(np.concatenate(lst) if lst else np.zeros(0), dtype=dt)
         self.arrays = dict(
             n=n, m=m, k=k, U=cat(U, np.float64), d=cat(d, np.float64), rp=cat(rp, np.int32), ci=cat(ci, np.int32),
-            av=cat(av, np.float64), c=cat([mdl.c for mdl in self.models], np.float64),
+            av=cat(av, np.float64), qrp=cat(qrp, np.int32), qci=cat(qci, np.int32), qv=cat(qv, np.float64),
+            c=cat([mdl.c for mdl in self.models], np.float64),
             l=cat([mdl.l for mdl in self.models], np.float64), u=cat([mdl.u for mdl in self.models], np.float64),
             AL=cat([mdl.AL for mdl in self.models], np.float64), AU=cat([mdl.AU for mdl in self.models], np.float64),
             c0=np.array([mdl.obj_constant for mdl in self.models], dtype=np.float64))
         a = self.arrays
         h = C.c_void_p()
         P = _lib.ptr
-        _lib.check(self.ctx.h, self.ctx.lib.hprqp_batch_create(
-            self.ctx.h, cnt, P(a["n"]), P(a["m"]), P(a["k"]), P(a["U"]), P(a["d"]), a["ci"].size, P(a["rp"]),
-            P(a["ci"]), P(a["av"]), P(a["c"]), P(a["l"]), P(a["u"]), P(a["AL"]), P(a["AU"]), P(a["c0"]),
-            C.byref(h)))
+        if self.kind == "lowrank":
+            _lib.check(self.ctx.h, self.ctx.lib.hprqp_batch_create(
+                self.ctx.h, cnt, P(a["n"]), P(a["m"]), P(a["k"]), P(a["U"]), P(a["d"]), a["ci"].size, P(a["rp"]),
+                P(a["ci"]), P(a["av"]), P(a["c"]), P(a["l"]), P(a["u"]), P(a["AL"]), P(a["AU"]), P(a["c0"]),
+                C.byref(h)))
+        else:
+            _lib.check(self.ctx.h, self.ctx.lib.hprqp_batch_create_csr(
+                self.ctx.h, cnt, P(a["n"]), P(a["m"]), a["qci"].size, P(a["qrp"]), P(a["qci"]), P(a["qv"]),
+                a["ci"].size, P(a["rp"]), P(a["ci"]), P(a["av"]), P(a["c"]), P(a["l"]), P(a["u"]), P(a["AL"]),
+                P(a["AU"]), P(a["c0"]), C.byref(h)))
         self.h = h
         self.count = cnt
         self.noff = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)

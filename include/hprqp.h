@@ -438,6 +438,28 @@ int hprqp_batch_create(hprqp_ctx *ctx, int32_t count, const int32_t *n,
                        const double *c, const double *l, const double *u,
                        const double *AL, const double *AU,
                        const double *obj_constant, hprqp_batch **out);
+/* Batched instances whose Q is sparse (Qmap! for the CSR operator,
+ * src/Q_operators/sparse_matrix_operator.jl:27-51): per instance b,
+ *   min 1/2 x'Q_b x + c_b'x  s.t.  AL_b <= A_b x <= AU_b,  l_b <= x <= u_b,
+ * Q_b given as n_b x n_b CSR (0-based, its rowptr starting at 0), packed back
+ * to back in q_rowptr (n_b + 1 entries each) / q_colidx / q_vals (q_nnz in
+ * all); A_b as in hprqp_batch_create.  Each solve scales every instance as
+ * the single solve does (Ruiz and Pock-Chambolle, src/utils.jl:350-804;
+ * (Q + Q')/2 after scaling) and runs the same block-per-instance check
+ * intervals with CSR products of Q.  The batch runs the standard path only:
+ * an instance with a diagonal or empty Q, or with more than 80% free
+ * variables (the noC path), is refused with HPRQP_E_UNSUP.  No counterpart in
+ * the reference (one instance per call); hprqp_batch_predict and
+ * hprqp_batch_solve_warm take only the diagonal-plus-low-rank kind. */
+int hprqp_batch_create_csr(hprqp_ctx *ctx, int32_t count, const int32_t *n,
+                           const int32_t *m, int64_t q_nnz,
+                           const int32_t *q_rowptr, const int32_t *q_colidx,
+                           const double *q_vals, int64_t a_nnz,
+                           const int32_t *a_rowptr, const int32_t *a_colidx,
+                           const double *a_vals, const double *c,
+                           const double *l, const double *u, const double *AL,
+                           const double *AU, const double *obj_constant,
+                           hprqp_batch **out);
 int hprqp_batch_destroy(hprqp_ctx *ctx, hprqp_batch *batch);
 /* precision: HPRQP_PREC_FP64 (all FP64) or HPRQP_PREC_MIXED (FP32 U, d, A,
  * c, l, u, AL, AU; FP64 iterates, anchors, accumulation and KKT).  x0/y0
