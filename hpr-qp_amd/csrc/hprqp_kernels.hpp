@@ -141,11 +141,19 @@ __global__ __launch_bounds__(kBlock) void k_csr_chunk(int32_t nrow, int32_t ncol
                                                       const int32_t *__restrict__ ci, const MT *__restrict__ v,
                                                       const XT *__restrict__ x, double *__restrict__ part) {
     __shared__ XT xs[kChunkW];
-    const int32_t c = blockIdx.y, c0 = c * kChunkW, cw = min(kChunkW, ncol - c0);
+    // 1-D grid of (row group, chunk) pairs, chunk-major over each XCD's
+    // contiguous share of the grid: an XCD's blocks take a few whole chunks,
+    // so the chunk of x each block stages is read from HBM about once and
+    // from that XCD's L2 by the chunk's other row groups (row-group-major
+    // dealing spread every chunk over all eight L2s: 1.6x the algorithmic
+    // bytes fetched)
+    const int32_t ng = (nrow + kChunkRows - 1) / kChunkRows;
+    const int32_t vb = xcd_block(blockIdx.x, gridDim.x), c = vb / ng, grp = vb % ng;
+    const int32_t c0 = c * kChunkW, cw = min(kChunkW, ncol - c0);
     for (int i = threadIdx.x; i < cw; i += kBlock) xs[i] = x[c0 + i];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, sub = lane >> 4, l16 = lane & 15;
-    for (int32_t rg = blockIdx.x * kChunkRows + 16 * w; rg < blockIdx.x * kChunkRows + kChunkRows; rg += 64) {
+    for (int32_t rg = grp * kChunkRows + 16 * w; rg < grp * kChunkRows + kChunkRows; rg += 64) {
         const int32_t rbase = rg + sub;  // rows rbase + 4 p, p < 4
         int32_t sg[4], eg[4];
 #pragma unroll

@@ -95,7 +95,8 @@ template <class AT, class MT, class XT, class Epi>
 static void launch_chunked(hipStream_t s, const DevCsr &a, const MT *v, const XT *x, const Epi &epi) {
     if (a.nrow <= 0) return;
     chunk_prepare(s, a);
-    launch_k(k_csr_chunk<MT, XT, AT>, dim3((a.nrow + kChunkRows - 1) / kChunkRows, a.nchunk), dim3(kBlock), 0, s,
+    launch_k(k_csr_chunk<MT, XT, AT>, dim3((unsigned)(((a.nrow + kChunkRows - 1) / kChunkRows) * a.nchunk)),
+             dim3(kBlock), 0, s,
              a.nrow, a.ncol, (const int32_t *)a.cseg.p, a.ci.p, v, x, a.cpart.p);
     HPRQP_HIP(hipGetLastError());
     launch_k(k_chunk_sum<AT, Epi>, dim3(nblk((int64_t)a.nrow * 16)), dim3(kBlock), 0, s, a.nrow, a.nchunk,
