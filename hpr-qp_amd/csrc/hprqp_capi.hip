@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <unordered_map>
 
 #include "hprqp_internal.hpp"
 
@@ -83,6 +84,33 @@ HostCsr host_csr_symmetric_pattern(const HostCsr &q) {
         s.rp[i + 1] = (int32_t)s.ci.size();
     }
     return s;
+}
+
+// the value table of v: its distinct values (exact comparison, first
+// occurrence order) and one code per entry; false (and nothing kept) when
+// v is empty or has more than 256 distinct values
+bool dict_encode(const std::vector<double> &v, std::vector<double> &dict, std::vector<uint8_t> &code) {
+    dict.clear();
+    code.clear();
+    if (v.empty()) return false;
+    std::unordered_map<uint64_t, uint8_t> idx;
+    code.resize(v.size());
+    for (size_t k = 0; k < v.size(); ++k) {
+        uint64_t key;
+        std::memcpy(&key, &v[k], 8);
+        auto it = idx.find(key);
+        if (it == idx.end()) {
+            if (dict.size() == 256) {
+                dict.clear();
+                code.clear();
+                return false;
+            }
+            it = idx.emplace(key, (uint8_t)dict.size()).first;
+            dict.push_back(v[k]);
+        }
+        code[k] = it->second;
+    }
+    return true;
 }
 
 // for a symmetric pattern: the position of each entry's transpose partner
@@ -624,6 +652,20 @@ int hprqp_model_create(hprqp_ctx *ctx, int32_t n, int32_t m, hprqp_qop *Q, int64
                 // from each entry to its transpose partner
                 md->Qsym = hprqp::host_csr_symmetric_pattern(Q->hq);
                 md->q_tpos = hprqp::host_csr_transpose_positions(md->Qsym);
+                std::vector<double> qs(md->Qsym.v.size());
+                for (size_t k = 0; k < qs.size(); ++k) qs[k] = (md->Qsym.v[k] + md->Qsym.v[md->q_tpos[k]]) / 2.0;
+                if (!hprqp::dict_encode(qs, md->q_dict, md->q_code)) md->q_dict.clear();
+            }
+            if (hprqp::dict_encode(md->A.v, md->a_dict, md->a_code)) {
+                // A^T's codes: the transpose of A with its codes as values
+                HostCsr ac = md->A;
+                for (size_t k = 0; k < ac.v.size(); ++k) ac.v[k] = (double)md->a_code[k];
+                HostCsr tc = hprqp::host_csr_transpose(ac);
+                md->at_code.resize(tc.v.size());
+                for (size_t k = 0; k < tc.v.size(); ++k) md->at_code[k] = (uint8_t)tc.v[k];
+            } else {
+                md->a_dict.clear();
+                md->a_code.clear();
             }
         } catch (...) {
             delete md;

@@ -87,6 +87,16 @@ struct DevCsr {
     mutable DevVec<int32_t> cseg;
     mutable DevVec<double> cpart;
     mutable int32_t nchunk = 0;
+    // value table form (matrices with at most 256 distinct values before
+    // scaling): one byte per nonzero indexing dict, the scaling applied as
+    // the row factor rsc and the column factor csc (entry = dict[code] *
+    // rsc[row] * csc[col]); dict_rb > 0 selects it, rows per block of the
+    // CSR-stream shape
+    int dict_rb = 0;
+    int ndict = 0;
+    DevVec<uint8_t> code;
+    DevVec<double> dict;
+    const double *rsc = nullptr, *csc = nullptr;
 };
 
 // Host-side CSR (sorted column indices), used while building models.
@@ -123,6 +133,7 @@ HostCsr host_csr_from(int32_t nrow, int32_t ncol, int64_t nnz, const int32_t *ro
                       const int32_t *colidx, const double *vals, int index_base);
 HostCsr host_csr_transpose(const HostCsr &a);
 HostCsr host_csr_symmetric_pattern(const HostCsr &q);
+bool dict_encode(const std::vector<double> &v, std::vector<double> &dict, std::vector<uint8_t> &code);
 std::vector<int32_t> host_csr_transpose_positions(const HostCsr &s);
 void upload_csr(hipStream_t s, const HostCsr &h, DevCsr &d);
 int default_lanes(const HostCsr &h);
@@ -203,6 +214,13 @@ struct hprqp_model {
     // mode (HPRQP_PREC_FP64 / MIXED / FP32) for Q, A, A^T and a LASSO
     // operator's data matrix and its transpose
     mutable int tuned_lanes[3][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+    // value tables of A (A^T's codes in its own order) and of Q's
+    // symmetrised values (Qsym, (q_ij + q_ji) / 2), empty when a matrix has
+    // more than 256 distinct values; the FP64 value-table shape the
+    // autotuner chose for Q, A, A^T (0 not tuned, -1 none, else rows per block)
+    std::vector<double> a_dict, q_dict;
+    std::vector<uint8_t> a_code, at_code, q_code;
+    mutable int tuned_dict[3] = {0, 0, 0};
 };
 
 struct hprqp_batch {

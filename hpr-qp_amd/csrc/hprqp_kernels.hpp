@@ -273,6 +273,66 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream(int32_t nrows, int rb, co
     }
 }
 
+// The CSR-stream product of a matrix in value-table form: entry k of row r
+// is dict[code[k]] * rsc[r] * csc[col] -- the unscaled value (one byte per
+// nonzero instead of eight) with the row and column scaling factors
+// (src/utils.jl:350-804 scales by diagonal factors, so the scaled matrix is
+// exactly this product).  The table (at most 256 values) is staged in LDS;
+// csc[col] is gathered beside x[col], and rsc[r] multiplies the row's sum.
+template <class XT, class AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_csr_stream_dict(int32_t nrows, int rb, const int32_t *__restrict__ rp,
+                                                            const int32_t *__restrict__ ci,
+                                                            const uint8_t *__restrict__ code,
+                                                            const double *__restrict__ dict, int ndict,
+                                                            const double *__restrict__ rsc,
+                                                            const double *__restrict__ csc, const XT *__restrict__ x,
+                                                            Epi epi) {
+    __shared__ AT prod[kStreamCap];
+    __shared__ double dl[256];
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t r0 = b * rb;
+    if (r0 >= nrows) return;
+    if ((int)threadIdx.x < ndict) dl[threadIdx.x] = dict[threadIdx.x];
+    const int32_t r1 = min(r0 + rb, nrows);
+    const int32_t base = rp[r0];
+    const int32_t cnt = rp[r1] - base;
+    const int t = threadIdx.x;
+    const int32_t row = r0 + t;
+    __syncthreads();
+    if (cnt <= kStreamCap) {
+        for (int32_t k0 = 0; k0 < cnt; k0 += kStreamU * kBlock) {
+            uint8_t cd[kStreamU];
+            int32_t cc[kStreamU];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) {
+                const int32_t k = k0 + j * kBlock + t;
+                const int32_t at = base + (k < cnt ? k : 0);
+                cd[j] = code[at];
+                cc[j] = ci[at];
+            }
+            AT pr[kStreamU];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) pr[j] = (AT)(dl[cd[j]] * csc[cc[j]]) * (AT)x[cc[j]];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) {
+                const int32_t k = k0 + j * kBlock + t;
+                if (k < cnt) prod[k] = pr[j];
+            }
+        }
+        __syncthreads();
+        if (row < r1) {
+            const int32_t s = rp[row] - base, e = rp[row + 1] - base;
+            AT acc = 0;
+            for (int32_t k = s; k < e; ++k) acc += prod[k];
+            epi(row, acc * (AT)rsc[row]);
+        }
+    } else if (row < r1) {
+        AT acc = 0;
+        for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += (AT)(dl[code[k]] * csc[ci[k]]) * (AT)x[ci[k]];
+        epi(row, acc * (AT)rsc[row]);
+    }
+}
+
 // elementwise form of the same epilogue (operator Q: product precomputed)
 template <class ST, class AT, class Epi>
 __global__ __launch_bounds__(kBlock) void k_vec(int32_t n, const ST *__restrict__ src, Epi epi) {

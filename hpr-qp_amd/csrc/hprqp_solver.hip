@@ -108,8 +108,23 @@ static void launch_chunked(hipStream_t s, const DevCsr &a, const MT *v, const XT
 // allocated for them): it takes the one-lane shape, whose row loop reads
 // nothing when every row is empty, whatever shape was chosen (the chunked
 // shape loads entry 0 of the value and index arrays speculatively).
+template <class AT, class XT, class Epi>
+static void launch_dict(hipStream_t s, const DevCsr &a, int rb, const XT *x, const Epi &epi) {
+    const int grid = (int)(((int64_t)a.nrow + rb - 1) / rb);
+    launch_k(k_csr_stream_dict<XT, AT, Epi>, dim3(grid), dim3(kBlock), 0, s, a.nrow, rb, a.rp.p, a.ci.p,
+             (const uint8_t *)a.code.p, (const double *)a.dict.p, a.ndict, a.rsc, a.csc, x, epi);
+    HPRQP_HIP(hipGetLastError());
+}
+
 template <class AT, class MT, class XT, class Epi>
 static void csr_apply(hipStream_t s, const DevCsr &a, const XT *x, const Epi &epi) {
+    if constexpr (std::is_same<MT, double>::value && std::is_same<AT, double>::value) {
+        // the FP64 iteration's value-table form, where the autotuner chose it
+        if (a.dict_rb > 0 && a.nnz > 0 && a.nrow > 0) {
+            launch_dict<AT>(s, a, a.dict_rb, x, epi);
+            return;
+        }
+    }
     if (a.nnz == 0)
         launch_csr<AT>(s, 1, a.nrow, a.rp.p, a.ci.p, csr_vals<MT>(a), x, epi);
     else if (a.lanes == kLanesChunked)
@@ -231,6 +246,7 @@ struct Scaled {
     bool q_empty = true;
     DevCsr Q, A, AT;
     DevVec<double> c, l, u, AL, AU, row_norm, col_norm, diag_Q;
+    DevVec<double> inv_rn, inv_cn;  // 1 / row_norm, 1 / col_norm (value-table products)
     std::vector<double> diag_Q_host;
     double b_scale = 1.0, c_scale = 1.0, norm_b = 0, norm_c = 0, norm_b_org = 0, norm_c_org = 0;
     bool Q_is_diag = false;
@@ -292,8 +308,8 @@ struct VecNorm {
     }
 };
 
-static void build_scaled(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, Reducer &red,
-                         Scaled &S) {
+static void build_scaled_data(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, Reducer &red,
+                              Scaled &S) {
     hipStream_t s = ctx->stream;
     const int32_t n = md->n, m = md->m;
     S.n = n;
@@ -403,6 +419,43 @@ static void build_scaled(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_para
     h = red.fetch(4);
     S.norm_b = m > 0 ? std::sqrt(h[0]) : 0.0;
     S.norm_c = std::sqrt(h[2]);
+}
+
+// the value-table form of Q, A, A^T (model value tables, the scaling as
+// row and column factors 1 / row_norm, 1 / col_norm); used where the
+// autotuner finds it faster (apply_lanes)
+static void attach_dicts(hipStream_t s, const hprqp_model *md, Scaled &S) {
+    const int32_t n = S.n, m = S.m;
+    S.inv_rn.alloc(m);
+    S.inv_cn.alloc(n);
+    {
+        double *ir = S.inv_rn.p, *ic = S.inv_cn.p;
+        const double *rn = S.row_norm.p, *cn = S.col_norm.p;
+        each(s, m, [=] __device__(int64_t i) { ir[i] = 1.0 / rn[i]; });
+        each(s, n, [=] __device__(int64_t j) { ic[j] = 1.0 / cn[j]; });
+    }
+    auto attach = [&](DevCsr &a, const std::vector<double> &dict, const std::vector<uint8_t> &code,
+                      const double *rsc, const double *csc) {
+        if (dict.empty() || (int64_t)code.size() != a.nnz || a.nnz == 0) return;
+        a.code.alloc(a.nnz);
+        HPRQP_HIP(hipMemcpyAsync(a.code.p, code.data(), a.nnz, hipMemcpyHostToDevice, s));
+        a.dict.alloc((int64_t)dict.size());
+        HPRQP_HIP(hipMemcpyAsync(a.dict.p, dict.data(), sizeof(double) * dict.size(), hipMemcpyHostToDevice, s));
+        a.ndict = (int)dict.size();
+        a.rsc = rsc;
+        a.csc = csc;
+    };
+    if (m > 0) {
+        attach(S.A, md->a_dict, md->a_code, S.inv_rn.p, S.inv_cn.p);
+        attach(S.AT, md->a_dict, md->at_code, S.inv_cn.p, S.inv_rn.p);
+    }
+    if (S.qkind == QOP_CSR && !S.q_empty) attach(S.Q, md->q_dict, md->q_code, S.inv_cn.p, S.inv_cn.p);
+    HPRQP_HIP(hipStreamSynchronize(s));
+}
+
+static void build_scaled(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, Reducer &red, Scaled &S) {
+    build_scaled_data(ctx, md, p, red, S);
+    attach_dicts(ctx->stream, md, S);
 }
 
 // check_Q_diagonal after scaling (src/utils.jl:827-855)
@@ -2417,9 +2470,10 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     res->noC = noC;
     res->n_history = L.nh;
     res->polish_iter = polish_from >= 0 ? L.iter - polish_from : 0;
-    res->lanes_Q = S.Q.lanes;
-    res->lanes_A = S.A.lanes;
-    res->lanes_AT = S.AT.lanes;
+    // value-table shapes reported as -1000 - rows per block
+    res->lanes_Q = S.Q.dict_rb > 0 ? -1000 - S.Q.dict_rb : S.Q.lanes;
+    res->lanes_A = S.A.dict_rb > 0 ? -1000 - S.A.dict_rb : S.A.lanes;
+    res->lanes_AT = S.AT.dict_rb > 0 ? -1000 - S.AT.dict_rb : S.AT.lanes;
     if (S.qkind == QOP_LASSO) {
         res->lanes_Q = S.qop->la.lanes;  // the LASSO operator's A (its A^T in lanes_AT)
         res->lanes_AT = S.qop->lat.lanes;
@@ -2525,11 +2579,50 @@ static int tune_lanes(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
     return best;
 }
 
+// The value-table form against the matrix's tuned shape (FP64): CSR-stream
+// blocks of 256, 128, 64 and 32 rows; taken when 5 % faster.  Returns the rows
+// per block, or -1.
+static int tune_dict(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
+    const int64_t nr = std::max<int64_t>(1, a.nrow);
+    DevVec<double> x(std::max<int64_t>(1, a.ncol)), y(nr), z(nr), rv(4 * nr);
+    {
+        double *xp = x.p, *rp = rv.p;
+        each(s, a.ncol, [=] __device__(int64_t i) { xp[i] = 1.0; });
+        each(s, 4 * nr, [=] __device__(int64_t i) { rp[i] = 0.5; });
+    }
+    const EpiTune<double> epi{y.p, z.p, rv.p, rv.p + nr, rv.p + 2 * nr, rv.p + 3 * nr};
+    EventTimer tm(s);
+    const double plain = tm.avg_ms(reps, [&] { csr_apply<double, double>(s, a, (const double *)x.p, epi); });
+    const double mean = a.nrow > 0 ? (double)a.nnz / (double)a.nrow : 0.0;
+    int best = -1;
+    double best_ms = plain * 0.95;
+    for (int rb : {256, 128, 64, 32}) {
+        if (mean * rb > 0.75 * kStreamCap) continue;
+        const double ms = tm.avg_ms(reps, [&] { launch_dict<double>(s, a, rb, (const double *)x.p, epi); });
+        if (verbose) std::printf("row-product autotune   value table, %d rows per block %.4f ms (shape %d %.4f ms)\n",
+                                 rb, ms, a.lanes, plain);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = rb;
+        }
+    }
+    return best;
+}
+
 static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool mixed) {
     HPRQP_REQUIRE(p->precision == HPRQP_PREC_FP64 || p->precision == HPRQP_PREC_MIXED ||
                       p->precision == HPRQP_PREC_FP32,
                   HPRQP_E_ARG, "unknown precision mode");
     (void)mixed;
+    if (p->spmv_lanes <= -1000) {
+        // the value-table form forced (rows per block -spmv_lanes - 1000)
+        const int rb = -p->spmv_lanes - 1000;
+        HPRQP_REQUIRE(rb == 32 || rb == 64 || rb == 128 || rb == 256, HPRQP_E_ARG,
+                      "spmv_lanes below -1000 selects the value-table form: -1032, -1064, -1128 or -1256");
+        for (DevCsr *a : {&S.Q, &S.A, &S.AT})
+            if (a->ndict > 0) a->dict_rb = rb;
+        return;
+    }
     if (p->spmv_lanes != 0) {
         const int L = p->spmv_lanes;
         const bool vec = L > 0 && (L & (L - 1)) == 0 && (L <= 64 || L == kBlock);
@@ -2565,6 +2658,17 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
             }
         }
         a.lanes = tuned;
+    }
+    // the FP64 iteration may take Q, A, A^T in value-table form
+    if (p->precision == HPRQP_PREC_FP64) {
+        DevCsr *dm[3] = {&S.Q, &S.A, &S.AT};
+        for (int k = 0; k < 3; ++k) {
+            DevCsr &a = *dm[k];
+            if (a.ndict == 0 || a.nnz < (1 << 18)) continue;
+            int &td = md->tuned_dict[k];
+            if (td == 0) td = tune_dict(s, a, 20, p->verbose);
+            a.dict_rb = td > 0 ? td : 0;
+        }
     }
 }
 

@@ -112,6 +112,36 @@ def test_every_spmv_kernel_shape_gives_the_same_solution(lanes):
     _check(res, ref)
 
 
+@pytest.mark.parametrize("case", ["grid", "aug2d", "aug3dcqp", "random"])
+def test_value_table_form_matches_oracle(case):
+    """The FP64 iteration with Q, A and A^T in value-table form (one byte per
+    nonzero into at most 256 unscaled values, the scaling as row and column
+    factors), forced: the oracle's solution, iterations and restarts; a
+    matrix with more than 256 distinct values keeps its plain shape (the
+    random QP's Q), and stopped part-way the iterates are the plain form's
+    to rounding."""
+    from hprqp import HPRQP_parameters, build_from_mps, instances, optimize
+
+    model = {"grid": lambda: instances.grid_qp(60, seed=0),
+             "aug2d": lambda: build_from_mps(fixture_path("AUG2D.mps"), verbose=False),
+             "aug3dcqp": lambda: build_from_mps(fixture_path("AUG3DCQP.mps"), verbose=False),
+             "random": lambda: instances.random_sparse_qp(1500, 700, seed=11)}[case]()
+    res, ref = _solve_both(model, spmv_lanes=-1256)
+    print(f"{case}: shapes {res.lanes}")
+    _check(res, ref)
+    assert abs(res.iter - ref.iter) <= 10 and res.restarts == ref.restarts
+    if case == "grid":
+        assert res.lanes[0] == res.lanes[1] == res.lanes[2] == -1256
+    if case == "random":
+        assert res.lanes[0] != -1256
+    its = 200
+    a = optimize(model, HPRQP_parameters(verbose=False, max_iter=its, spmv_lanes=-1256))
+    b = optimize(model, HPRQP_parameters(verbose=False, max_iter=its, spmv_lanes=-256))
+    dx = np.max(np.abs(a.x - b.x)) / (1 + np.max(np.abs(b.x)))
+    print(f"{case}: stopped at {its}: x diff value-table vs plain {dx:.2e}")
+    assert a.restarts == b.restarts and dx < 1e-9
+
+
 @pytest.mark.parametrize("precision", ["mixed", "fp32"])
 @pytest.mark.parametrize("case", ["grid", "random", "aug3dcqp", "lasso", "lasso_chunked"])
 def test_reduced_precision_reaches_fp64_accuracy(precision, case):
