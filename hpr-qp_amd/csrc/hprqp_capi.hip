@@ -804,6 +804,11 @@ int hprqp_ws_download(hprqp_ctx *ctx, hprqp_ws *ws, int which, double *host, int
     return guarded(ctx, [&] { hprqp::ws_download(ws, which, host, len); });
 }
 
+int hprqp_ws_upload(hprqp_ctx *ctx, hprqp_ws *ws, int which, const double *host, int64_t len) {
+    WS_CHECK(ctx, ws);
+    return guarded(ctx, [&] { hprqp::ws_upload(ws, which, host, len); });
+}
+
 }  // extern "C"
 
 static void batch_up32(hprqp::DevVec<float> &dst, const double *src, int64_t len) {

@@ -296,6 +296,7 @@ void ws_q_factors(hprqp_ws *w, double sigma);
 void ws_restart(hprqp_ws *w);
 void ws_collect(hprqp_ws *w, double *x, double *y, double *z, double *wv);
 void ws_download(hprqp_ws *w, int which, double *host, int64_t len);
+void ws_upload(hprqp_ws *w, int which, const double *host, int64_t len);
 void ws_destroy(hprqp_ws *w);
 // per-kernel timing of the iteration (hprqp_profile_step)
 void profile_step(hprqp_ctx *ctx, const hprqp_model *model, const hprqp_params *params, int reps,

@@ -2843,6 +2843,7 @@ struct hprqp_ws {
     virtual void restart() = 0;
     virtual void collect(double *x, double *y, double *z, double *w) = 0;
     virtual void download(int which, double *host, int64_t len) = 0;
+    virtual void upload(int which, const double *host, int64_t len) = 0;
 };
 
 namespace hprqp {
@@ -3039,34 +3040,88 @@ struct WsImpl final : hprqp_ws {
 
     void collect(double *x, double *y, double *z, double *w) override { hprqp::collect(*W, x, y, z, w); }
 
+    // the workspace vector `which` (HPRQP_VEC_*): iterate-precision pointer
+    // or FP64 pointer (anchors, diagonal of Q), and its length
+    void vec_of(int which, VT *&v, double *&vd, int64_t &len) {
+        Solver<VT, MT> &D = *W;
+        v = nullptr;
+        vd = nullptr;
+        len = D.n;
+        switch (which) {
+            case HPRQP_VEC_X_BAR: v = D.x_bar.p; break;
+            case HPRQP_VEC_Y_BAR: v = D.y_bar.p, len = D.m; break;
+            case HPRQP_VEC_Z_BAR: v = D.z_bar.p; break;
+            case HPRQP_VEC_W_BAR: v = D.w_bar.p; break;
+            case HPRQP_VEC_ATDY: v = D.ATdy.p; break;
+            case HPRQP_VEC_QATDY: v = D.QATdy.p; break;
+            case HPRQP_VEC_DIAG_Q: vd = S.diag_Q.p; break;
+            case HPRQP_VEC_X: v = D.x.p; break;
+            case HPRQP_VEC_W: v = D.w.p; break;
+            case HPRQP_VEC_Y: v = D.y.p, len = D.m; break;
+            case HPRQP_VEC_ATY: v = D.ATy.p; break;
+            case HPRQP_VEC_QW: v = D.Qw.p; break;
+            case HPRQP_VEC_X_HAT: v = D.x_hat.p; break;
+            case HPRQP_VEC_DX: v = D.dx.p; break;
+            case HPRQP_VEC_DW: v = D.dw.p; break;
+            case HPRQP_VEC_DY: v = D.dy.p, len = D.m; break;
+            case HPRQP_VEC_S: v = D.s_.p, len = D.m; break;
+            case HPRQP_VEC_ATY_BAR: v = D.ATy_bar.p; break;
+            case HPRQP_VEC_TEMPV: v = D.tempv.p; break;
+            case HPRQP_VEC_QW_BAR: v = D.Qw_bar.p; break;
+            case HPRQP_VEC_LAST_X: vd = D.last_x.p; break;
+            case HPRQP_VEC_LAST_Y: vd = D.last_y.p, len = D.m; break;
+            case HPRQP_VEC_LAST_W: vd = D.last_w.p; break;
+            case HPRQP_VEC_LAST_ATY: vd = D.last_ATy.p; break;
+            case HPRQP_VEC_LAST_QW: vd = D.last_Qw.p; break;
+            case HPRQP_VEC_FACT1: v = D.fact1.p; break;
+            case HPRQP_VEC_FACT2: v = D.fact2.p; break;
+            case HPRQP_VEC_FACT: v = D.fact.p; break;
+            default: throw Error(HPRQP_E_ARG, "unknown vector");
+        }
+    }
+
     void download(int which, double *host, int64_t len) override {
         Solver<VT, MT> &D = *W;
         D.flush();
-        const VT *src = nullptr;
-        const double *srcd = nullptr;
-        int64_t want = D.n;
-        switch (which) {
-            case HPRQP_VEC_X_BAR: src = D.x_bar.p; break;
-            case HPRQP_VEC_Y_BAR: src = D.y_bar.p, want = D.m; break;
-            case HPRQP_VEC_Z_BAR: src = D.z_bar.p; break;
-            case HPRQP_VEC_W_BAR: src = D.w_bar.p; break;
-            case HPRQP_VEC_ATDY: src = D.ATdy.p; break;
-            case HPRQP_VEC_QATDY: src = D.QATdy.p; break;
-            case HPRQP_VEC_DIAG_Q: srcd = S.diag_Q.p; break;
-            default: throw Error(HPRQP_E_ARG, "unknown vector");
-        }
+        VT *src = nullptr;
+        double *srcd = nullptr;
+        int64_t want = 0;
+        vec_of(which, src, srcd, want);
         HPRQP_REQUIRE(host != nullptr && len == want, HPRQP_E_ARG, "host buffer length differs from the vector's");
         if (want == 0) return;
         if (!srcd) {
             DevVec<double> t(want);
             double *tp = t.p;
-            each(c_->stream, want, [=] __device__(int64_t i) { tp[i] = (double)src[i]; });
+            const VT *sp = src;
+            each(c_->stream, want, [=] __device__(int64_t i) { tp[i] = (double)sp[i]; });
             HPRQP_HIP(hipMemcpyAsync(host, t.p, sizeof(double) * want, hipMemcpyDeviceToHost, c_->stream));
             HPRQP_HIP(hipStreamSynchronize(c_->stream));
         } else {
             HPRQP_HIP(hipMemcpyAsync(host, srcd, sizeof(double) * want, hipMemcpyDeviceToHost, c_->stream));
             HPRQP_HIP(hipStreamSynchronize(c_->stream));
         }
+    }
+
+    void upload(int which, const double *host, int64_t len) override {
+        Solver<VT, MT> &D = *W;
+        D.invalidate_products();
+        HPRQP_REQUIRE(which != HPRQP_VEC_DIAG_Q, HPRQP_E_ARG, "the diagonal of Q is problem data, not state");
+        VT *dst = nullptr;
+        double *dstd = nullptr;
+        int64_t want = 0;
+        vec_of(which, dst, dstd, want);
+        HPRQP_REQUIRE(host != nullptr && len == want, HPRQP_E_ARG, "host buffer length differs from the vector's");
+        if (want == 0) return;
+        DevVec<double> t(want);
+        HPRQP_HIP(hipMemcpyAsync(t.p, host, sizeof(double) * want, hipMemcpyHostToDevice, c_->stream));
+        if (dstd) {
+            HPRQP_HIP(hipMemcpyAsync(dstd, t.p, sizeof(double) * want, hipMemcpyDeviceToDevice, c_->stream));
+        } else {
+            const double *tp = t.p;
+            VT *dp = dst;
+            each(c_->stream, want, [=] __device__(int64_t i) { dp[i] = (VT)tp[i]; });
+        }
+        HPRQP_HIP(hipStreamSynchronize(c_->stream));
     }
 };
 
@@ -3109,6 +3164,7 @@ void ws_q_factors(hprqp_ws *w, double sigma) { w->q_factors(sigma); }
 void ws_restart(hprqp_ws *w) { w->restart(); }
 void ws_collect(hprqp_ws *w, double *x, double *y, double *z, double *wv) { w->collect(x, y, z, wv); }
 void ws_download(hprqp_ws *w, int which, double *host, int64_t len) { w->download(which, host, len); }
+void ws_upload(hprqp_ws *w, int which, const double *host, int64_t len) { w->upload(which, host, len); }
 void ws_destroy(hprqp_ws *w) { delete w; }
 
 }  // namespace hprqp

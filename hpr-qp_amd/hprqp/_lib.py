@@ -188,7 +188,9 @@ class hprqp_kkt_out(C.Structure):
 
 
 PATH_STANDARD, PATH_NOC, PATH_LASSO, PATH_LP = 0, 1, 2, 3
-VEC_X_BAR, VEC_Y_BAR, VEC_Z_BAR, VEC_W_BAR, VEC_ATDY, VEC_QATDY, VEC_DIAG_Q = range(7)
+(VEC_X_BAR, VEC_Y_BAR, VEC_Z_BAR, VEC_W_BAR, VEC_ATDY, VEC_QATDY, VEC_DIAG_Q, VEC_X, VEC_W, VEC_Y, VEC_ATY, VEC_QW,
+ VEC_X_HAT, VEC_DX, VEC_DW, VEC_DY, VEC_S, VEC_ATY_BAR, VEC_TEMPV, VEC_QW_BAR, VEC_LAST_X, VEC_LAST_Y, VEC_LAST_W,
+ VEC_LAST_ATY, VEC_LAST_QW, VEC_FACT1, VEC_FACT2, VEC_FACT) = range(28)
 
 
 class hprqp_qop_vtable(C.Structure):
@@ -259,6 +261,7 @@ SIGNATURES = [
     ("hprqp_restart", C.c_int, [_P, _P]),
     ("hprqp_ws_collect", C.c_int, [_P, _P, _P, _P, _P, _P]),
     ("hprqp_ws_download", C.c_int, [_P, _P, C.c_int, _P, C.c_int64]),
+    ("hprqp_ws_upload", C.c_int, [_P, _P, C.c_int, _P, C.c_int64]),
     ("hprqp_batch_create", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, C.c_int64, _P, _P, _P, _P, _P, _P, _P,
                                      _P, _P, C.POINTER(_P)]),
     ("hprqp_batch_create_csr", C.c_int, [_P, C.c_int32, _P, _P, C.c_int64, _P, _P, _P, C.c_int64, _P, _P, _P, _P,

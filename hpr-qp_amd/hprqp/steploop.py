@@ -176,6 +176,10 @@ class Workspace:
         _lib.check(self.ctx.h, self.lib.hprqp_ws_download(self.ctx.h, self.h, which, _lib.ptr(out), length))
         return out
 
+    def upload(self, which, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _lib.check(self.ctx.h, self.lib.hprqp_ws_upload(self.ctx.h, self.h, which, _lib.ptr(v), v.size))
+
     def collect(self):
         n, m = self.info.n, self.info.m
         x, y, z, w = np.zeros(n), np.zeros(m), np.zeros(n), np.zeros(n)

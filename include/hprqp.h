@@ -351,14 +351,36 @@ typedef struct {
     double err_Rp, err_Rd, rel_gap, primal_obj, dual_obj, kkt;
 } hprqp_kkt_out;
 
-/* the vectors hprqp_ws_download returns (scaled space) */
-#define HPRQP_VEC_X_BAR 0   /* n */
-#define HPRQP_VEC_Y_BAR 1   /* m */
-#define HPRQP_VEC_Z_BAR 2   /* n */
-#define HPRQP_VEC_W_BAR 3   /* n */
-#define HPRQP_VEC_ATDY 4    /* n: A^T dy after hprqp_sigma_dots */
-#define HPRQP_VEC_QATDY 5   /* n: Q A^T dy after hprqp_sigma_dots */
-#define HPRQP_VEC_DIAG_Q 6  /* n: diagonal of the scaled Q */
+/* the workspace vectors (HPRQP_workspace_gpu's fields, src/structs.jl:394-459)
+ * hprqp_ws_download returns and hprqp_ws_upload sets (scaled space) */
+#define HPRQP_VEC_X_BAR 0     /* n */
+#define HPRQP_VEC_Y_BAR 1     /* m */
+#define HPRQP_VEC_Z_BAR 2     /* n */
+#define HPRQP_VEC_W_BAR 3     /* n */
+#define HPRQP_VEC_ATDY 4      /* n: A^T dy after hprqp_sigma_dots */
+#define HPRQP_VEC_QATDY 5     /* n: Q A^T dy after hprqp_sigma_dots */
+#define HPRQP_VEC_DIAG_Q 6    /* n: diagonal of the scaled Q (download only) */
+#define HPRQP_VEC_X 7         /* n */
+#define HPRQP_VEC_W 8         /* n */
+#define HPRQP_VEC_Y 9         /* m */
+#define HPRQP_VEC_ATY 10      /* n: Halpern-averaged A^T y */
+#define HPRQP_VEC_QW 11       /* n */
+#define HPRQP_VEC_X_HAT 12    /* n */
+#define HPRQP_VEC_DX 13       /* n */
+#define HPRQP_VEC_DW 14       /* n */
+#define HPRQP_VEC_DY 15       /* m */
+#define HPRQP_VEC_S 16        /* m */
+#define HPRQP_VEC_ATY_BAR 17  /* n */
+#define HPRQP_VEC_TEMPV 18    /* n */
+#define HPRQP_VEC_QW_BAR 19   /* n */
+#define HPRQP_VEC_LAST_X 20   /* n: Halpern anchors */
+#define HPRQP_VEC_LAST_Y 21   /* m */
+#define HPRQP_VEC_LAST_W 22   /* n */
+#define HPRQP_VEC_LAST_ATY 23 /* n */
+#define HPRQP_VEC_LAST_QW 24  /* n */
+#define HPRQP_VEC_FACT1 25    /* n: diagonal-Q factors (update_Q_factors) */
+#define HPRQP_VEC_FACT2 26    /* n */
+#define HPRQP_VEC_FACT 27     /* n */
 
 /* Scaling, check_Q_diagonal, row-product shapes (autotuned as in solve),
  * allocation, power method, sigma_0, the noC decision, then
@@ -404,6 +426,10 @@ int hprqp_ws_collect(hprqp_ctx *ctx, hprqp_ws *ws, double *x, double *y,
                      double *z, double *w);
 int hprqp_ws_download(hprqp_ctx *ctx, hprqp_ws *ws, int which, double *host,
                       int64_t len);
+/* set a workspace vector (a host loop's ws.x .= ..., for instance); the
+ * QAP one-apply form's carried products are formed again at the next step */
+int hprqp_ws_upload(hprqp_ctx *ctx, hprqp_ws *ws, int which, const double *host,
+                    int64_t len);
 
 /* ---- batched solves (many instances, one launch per check interval) ----
  * Instance i: Q_i = diag(d_i) + U_i U_i^T (U_i n_i x k_i column-major),

@@ -1429,3 +1429,69 @@ double oracle_mnorm(int32_t n, int32_t m, const oracle_qop *qin, int64_t a_nnz,
     qop_free(&Q);
     return r;
 }
+
+/* ------------------------------------------------------------------ */
+/* one half-step on a given state (per-kernel check)                   */
+/* ------------------------------------------------------------------ */
+
+int oracle_halfstep(int kind, int32_t n, int32_t m, const oracle_qop *qin,
+                    int64_t a_nnz, const int64_t *a_colptr,
+                    const int32_t *a_rowidx, const double *a_val,
+                    const double *c, const double *AL, const double *AU,
+                    const double *l, const double *u, oracle_state *st) {
+    (void)a_nnz;
+    qop_t q;
+    if (qop_build(n, qin, &q) != 0) return -1;
+    csc_t A = csc_copy(m, n, a_colptr, a_rowidx, a_val);
+    csc_t AT = csc_transpose(&A);
+    ws_t ws;
+    memset(&ws, 0, sizeof(ws));
+    ws.n = n;
+    ws.m = m;
+    ws.w = st->w; ws.w_bar = st->w_bar; ws.dw = st->dw; ws.x = st->x;
+    ws.x_hat = st->x_hat; ws.x_bar = st->x_bar; ws.dx = st->dx;
+    ws.z_bar = st->z_bar; ws.ATy = st->ATy; ws.ATy_bar = st->ATy_bar;
+    ws.ATdy = st->ATdy; ws.Qw = st->Qw; ws.Qw_bar = st->Qw_bar;
+    ws.tempv = st->tempv; ws.last_x = st->last_x; ws.last_w = st->last_w;
+    ws.last_ATy = st->last_ATy; ws.last_Qw = st->last_Qw;
+    ws.fact1 = st->fact1; ws.fact2 = st->fact2; ws.fact = st->fact;
+    ws.y = st->y; ws.y_bar = st->y_bar; ws.dy = st->dy; ws.s = st->s;
+    ws.last_y = st->last_y;
+    /* the step functions' own temporaries */
+    ws.Ax = dalloc(m);
+    ws.Rp = dalloc(m);
+    ws.Rd = dalloc(n);
+    ws.Qx = dalloc(n);
+    ws.dQw = dalloc(n);
+    ws.QATdy = dalloc(n);
+    ws.fact_M = dalloc(n);
+    ws.lambda = dalloc(n);
+    for (int32_t i = 0; i < n; ++i) ws.lambda[i] = st->lambda;
+    ws.c = c; ws.l = l; ws.u = u; ws.AL = AL; ws.AU = AU;
+    ws.A = &A;
+    ws.AT = &AT;
+    ws.Q = &q;
+    ws.sigma = st->sigma;
+    ws.lambda_max_A = st->lambda_max_A;
+    ws.lambda_max_Q = st->lambda_max_Q;
+    ws.Q_is_diag = st->Q_is_diag;
+    ws.to_check = st->full;
+    int rc = 0;
+    switch (kind) {
+        case 1: step_zxw1(&ws, st->h1, st->h2); break;
+        case 2: step_y(&ws, st->h1, st->h2); break;
+        case 3: step_w2(&ws, st->h1, st->h2); break;
+        case 4: step_noc_zxw(&ws, st->h1, st->h2); break;
+        case 5: step_y_from(&ws, ws.tempv, st->h1, st->h2, 1); break;
+        case 6: step_lasso(&ws, st->h1, st->h2); break;
+        case 7: step_lp_zx(&ws, st->h1, st->h2); break;
+        case 8: step_y_from(&ws, ws.x_hat, st->h1, st->h2, 0); break;
+        default: rc = -2;
+    }
+    free(ws.Ax); free(ws.Rp); free(ws.Rd); free(ws.Qx); free(ws.dQw);
+    free(ws.QATdy); free(ws.fact_M); free(ws.lambda);
+    csc_free(&A);
+    csc_free(&AT);
+    qop_free(&q);
+    return rc;
+}

@@ -154,6 +154,33 @@ double oracle_mnorm(int32_t n, int32_t m, const oracle_qop *q, int64_t a_nnz,
                     const double *dw, double sigma, double lambda_max_A,
                     double lambda_max_Q, int Q_is_diag, double *terms);
 
+/* One half-step of the iteration on a given state (test infrastructure:
+ * the check of each GPU half-step kernel on its own).  kind:
+ *   1 z/x/w1 (update_zxw1_cpu!, src/kernels.jl:2301-2397)
+ *   2 tempv and y (update_y_cpu!, :2682-2743)
+ *   3 w2 (update_w2_cpu!, :2747-2832)
+ *   4 noC z/x/w with the Halpern Qw update and tempv (:2117-2238)
+ *   5 the noC y step on tempv (:2242-2298)
+ *   6 LASSO z/x/w (update_zxw_LASSO_cpu!, :2400-2496)
+ *   7 LP z/x (unified_update_zx_cpu!, :2504-2578)
+ *   8 the LP y step on x_hat (:2582-2658; the CPU loop's partial write set,
+ *     without y_bar)
+ * The data are taken as given (no scaling); full selects the full write
+ * set.  Every vector must be allocated with its length (n or m). */
+typedef struct {
+    double *w, *w_bar, *dw, *x, *x_hat, *x_bar, *dx, *z_bar, *ATy, *ATy_bar,
+        *ATdy, *Qw, *Qw_bar, *tempv, *last_x, *last_w, *last_ATy, *last_Qw,
+        *fact1, *fact2, *fact;        /* n */
+    double *y, *y_bar, *dy, *s, *last_y; /* m */
+    double sigma, lambda_max_A, lambda_max_Q, lambda, h1, h2;
+    int32_t Q_is_diag, full;
+} oracle_state;
+int oracle_halfstep(int kind, int32_t n, int32_t m, const oracle_qop *q,
+                    int64_t a_nnz, const int64_t *a_colptr,
+                    const int32_t *a_rowidx, const double *a_val,
+                    const double *c, const double *AL, const double *AU,
+                    const double *l, const double *u, oracle_state *st);
+
 #ifdef __cplusplus
 }
 #endif
