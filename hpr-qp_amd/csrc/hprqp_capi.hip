@@ -265,6 +265,7 @@ void hprqp_params_default(hprqp_params *p) {
     p->polish_stall_checks = 0;
     p->polish_switch = 5.0;
     p->qap_recompute_Qw = 0;
+    p->qap_family = 0;
 }
 
 int hprqp_ctx_create(int device, hprqp_ctx **out) {
@@ -580,6 +581,20 @@ int hprqp_qop_apply_prec(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x, hprqp
             hprqp::qop_apply_fp64(ctx, q, (const double *)x->p, (double *)Qx->p);
         else
             hprqp::qop_apply_mixed(ctx, q, (const double *)x->p, (double *)Qx->p);
+    });
+}
+
+int hprqp_qop_set_qap_family(hprqp_ctx *ctx, hprqp_qop *q, int32_t family, int32_t *fp64_family,
+                             int32_t *bf16_family) {
+    CTX_CHECK(ctx);
+    if (!q) return HPRQP_E_ARG;
+    return guarded(ctx, [&] {
+        HPRQP_REQUIRE(q->kind == QOP_QAP, HPRQP_E_ARG, "not a QAP operator");
+        HPRQP_REQUIRE(family == 0 || family == 2 || family == 3 || family == 4, HPRQP_E_ARG,
+                      "qap_family must be 0, 2, 3 or 4");
+        q->qfam64 = q->qfam16 = family;
+        if (fp64_family) *fp64_family = hprqp::qap_family_of(q, false);
+        if (bf16_family) *bf16_family = hprqp::qap_family_of(q, true);
     });
 }
 

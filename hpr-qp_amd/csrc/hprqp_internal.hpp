@@ -191,6 +191,13 @@ struct hprqp_qop {
     // qnp x qnp with zero padding, three pieces back to back: X transposed
     // (qxp), Y row-major (qyp, written by pass 1), X row-major (qxr, full T)
     hprqp::DevVec<uint16_t> qxp, qyp, qxr;
+    // kernel family of the FP64 and split-BF16 passes (QapFamily): 0 by
+    // size (n < 256 one-tile, then blocked, LDS-tiled from n = 1024 with
+    // diagonal S and T).  qfam* is the family the launches take, set per
+    // solve from the caller's choice or from qtuned*, the fastest family
+    // the solver's timing found for this operator (0 = not timed yet)
+    int qfam64 = 0, qfam16 = 0;
+    mutable int qtuned64 = 0, qtuned16 = 0;
     // diagonal plus low rank: Q = diag(d) + U U^T, U n x k column-major
     int32_t lr_k = 0;
     hprqp::DevVec<double> lr_U, lr_d, lr_t;
@@ -315,6 +322,11 @@ void qap_prepare(hipStream_t s, hprqp_qop *q, const double *A, const double *B, 
 void qap_apply_f64(hipStream_t s, const hprqp_qop *q, const double *X, double *QX, double *tmp = nullptr);
 template <class IT, class OT>
 void qap_apply_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, OT *QX);
+// kernel family of the passes: the fastest of those that can run on the
+// operator, timed (qap_tune), and the one its launches take now
+// (qap_family_of; hprqp_params.qap_family codes)
+int qap_tune(hipStream_t s, hprqp_qop *q, bool bf16, bool verbose);
+int qap_family_of(const hprqp_qop *q, bool bf16);
 // the reduced-precision operator apply of the MIXED path (FP32 matrix data;
 // split-BF16 x3 MFMA for QAP), FP64 in and out
 void qop_apply_mixed(hprqp_ctx *ctx, hprqp_qop *q, const double *x, double *Qx);

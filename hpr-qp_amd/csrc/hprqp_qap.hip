@@ -2,6 +2,7 @@
 // (BF16 splitting, diagonal detection) and the plain applies.  The kernels
 // are in hprqp_qap.hpp, where the solver also instantiates them with its
 // fused half-step epilogues.
+#include <cstdio>
 #include <vector>
 
 #include "hprqp_kernels.hpp"
@@ -97,5 +98,68 @@ void qap_apply_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, OT *QX) {
 
 template void qap_apply_bf16x3<double, double>(hipStream_t, const hprqp_qop *, const double *, double *);
 template void qap_apply_bf16x3<float, float>(hipStream_t, const hprqp_qop *, const float *, float *);
+
+__global__ void k_qap_tune_fill(int64_t nn, int n, double *x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) x[i] = (1.0 + 0.25 * (double)((i * 7919) % 13)) / n;
+}
+
+int qap_family_of(const hprqp_qop *q, bool bf16) {
+    return qap_family(qap_args(q), bf16 ? q->qfam16 : q->qfam64, bf16);
+}
+
+// Each family that can run on the operator, timed on a plain apply (the
+// iteration's epilogues add the same per-element work to every family): 2
+// warm-up and 10 timed applies of an n x n input, the fastest kept.  Below
+// n = 256, or with a full S or T (blocked only), nothing is timed.
+int qap_tune(hipStream_t s, hprqp_qop *q, bool bf16, bool verbose) {
+    const QapArgs g = qap_args(q);
+    const bool diag = !g.s_full && !g.t_full;
+    if (g.n < kQapBlockedMin) return kQapFamSmall;
+    if (!diag) return kQapFamBlocked;
+    std::vector<int> cand = {kQapFamBlocked, kQapFamLds};
+    if (bf16 && (int64_t)(g.np / 64) * (g.np / 128) >= 512) cand.push_back(kQapFamLdsWide);
+    const int64_t nn = (int64_t)g.n * g.n;
+    DevVec<double> x(nn), y(nn);
+    // a doubly stochastic-like iterate: entries near 1/n
+    launch_k(k_qap_tune_fill, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, nn, g.n, x.p);
+    HPRQP_HIP(hipGetLastError());
+    int &fam = bf16 ? q->qfam16 : q->qfam64;
+    const int saved = fam;
+    hipEvent_t a, b;
+    HPRQP_HIP(hipEventCreate(&a));
+    HPRQP_HIP(hipEventCreate(&b));
+    int best = kQapBySize;
+    float best_ms = 0.f;
+    for (int c : cand) {
+        fam = c;
+        auto run = [&] {
+            if (bf16)
+                qap_apply_bf16x3<double, double>(s, q, (const double *)x.p, y.p);
+            else
+                qap_apply_f64(s, q, (const double *)x.p, y.p);
+        };
+        run();
+        run();
+        HPRQP_HIP(hipEventRecord(a, s));
+        for (int r = 0; r < 10; ++r) run();
+        HPRQP_HIP(hipEventRecord(b, s));
+        HPRQP_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        HPRQP_HIP(hipEventElapsedTime(&ms, a, b));
+        ms /= 10.f;
+        if (verbose)
+            std::printf("QAP kernel autotune    %s family %d: %.4f ms per apply\n", bf16 ? "split-BF16" : "FP64", c,
+                        (double)ms);
+        if (best == kQapBySize || ms < best_ms) {
+            best = c;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    fam = saved;
+    return best;
+}
 
 }  // namespace hprqp

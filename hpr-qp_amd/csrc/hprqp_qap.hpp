@@ -1249,6 +1249,26 @@ __global__ __launch_bounds__(256) void k_qapbfl_p2(QapArgs g, const IT *__restri
 // Q x with the result handed to epi(i, (Q x)_i): FP64 / split-BF16 x3
 // (tmp, when given, replaces the operator's two n x n intermediates: 2 n^2
 // doubles)
+// kernel families of the passes (hprqp_qop::qfam64 / qfam16)
+enum QapFamily { kQapBySize = 0, kQapFamSmall = 1, kQapFamBlocked = 2, kQapFamLds = 3, kQapFamLdsWide = 4 };
+
+// the family a pass runs: the operator's chosen one where it applies, else
+// by size.  The one-tile passes serve n < 256 (their pieces are padded to 32,
+// and the solver's one-apply step needs their tile sums there); the
+// LDS-tiled ones need diagonal S and T; the 64 x 128 split-BF16 blocks need
+// two blocks per CU (the grid np/64 x np/128 >= 512).
+static inline int qap_family(const QapArgs &g, int chosen, bool bf16) {
+    const bool diag = !g.s_full && !g.t_full;
+    if (g.n < kQapBlockedMin) return kQapFamSmall;
+    if (chosen == kQapFamBlocked) return kQapFamBlocked;
+    if (chosen == kQapFamLds && diag) return kQapFamLds;
+    if (chosen == kQapFamLdsWide && diag && bf16 && (int64_t)(g.np / 64) * (g.np / 128) >= 512)
+        return kQapFamLdsWide;
+    if (g.n >= kQapLdsMin && diag)
+        return bf16 && (int64_t)(g.np / 64) * (g.np / 128) >= 512 ? kQapFamLdsWide : kQapFamLds;
+    return kQapFamBlocked;
+}
+
 template <class Epi>
 void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Epi &epi, double *tmp = nullptr) {
     const QapArgs g = qap_args(q);
@@ -1256,7 +1276,8 @@ void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Ep
     const int64_t nn = (int64_t)g.n * g.n;
     double *const base = tmp ? tmp : q->qtemp.p;
     double *Y = base, *Z = base + nn;
-    if (g.n >= kQapLdsMin && !g.s_full && !g.t_full) {
+    const int fam = qap_family(g, q->qfam64, false);
+    if (fam == kQapFamLds) {
         // 64-wide blocks (measured faster than 128-wide up to n = 4096:
         // four blocks per CU hide each other's K-step barriers)
         const int nb = (g.n + 63) / 64;
@@ -1266,7 +1287,7 @@ void qap_launch_f64(hipStream_t s, const hprqp_qop *q, const double *X, const Ep
         HPRQP_HIP(hipGetLastError());
         return;
     }
-    if (g.n >= kQapBlockedMin) {
+    if (fam == kQapFamBlocked) {
         const int nb = (g.n + kBE - 1) / kBE;
         launch_k(k_qap64b_p1, dim3(nb, nb), dim3(256), 0, s, g, X, Y, Z);
         HPRQP_HIP(hipGetLastError());
@@ -1287,12 +1308,13 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
     const int nt = (g.n + 15) / 16;
     const int64_t nn = (int64_t)g.n * g.n;
     float *Y = q->qtemp32.p, *Z = q->qtemp32.p + nn;
-    if (g.n >= kQapLdsMin && !g.s_full && !g.t_full) {
+    const int fam = qap_family(g, q->qfam16, true);
+    if (fam == kQapFamLds || fam == kQapFamLdsWide) {
         launch_k(k_qap_split<IT>, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, g.n, g.np, X, q->qxp.p,
                  (unsigned short *)nullptr);
         HPRQP_HIP(hipGetLastError());
         const unsigned short *xp = q->qxp.p;
-        if ((int64_t)(g.np / 64) * (g.np / 128) >= 512) {
+        if (fam == kQapFamLdsWide) {
             // 64 x 128 blocks, each wave 2 x 4 tiles: fewer LDS reads per
             // MFMA (measured 13 % faster than 64 x 64 at n = 2048), used
             // once they make two blocks per CU
@@ -1311,7 +1333,7 @@ void qap_launch_bf16x3(hipStream_t s, const hprqp_qop *q, const IT *X, const Epi
         HPRQP_HIP(hipGetLastError());
         return;
     }
-    if (g.n >= kQapBlockedMin) {
+    if (fam == kQapFamBlocked) {
         const int nb = g.np / kBE;
         launch_k(k_qap_split<IT>, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, g.n, g.np, X, q->qxp.p,
                  g.t_full ? q->qxr.p : (unsigned short *)nullptr);

@@ -2478,6 +2478,7 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
         res->lanes_Q = S.qop->la.lanes;  // the LASSO operator's A (its A^T in lanes_AT)
         res->lanes_AT = S.qop->lat.lanes;
     }
+    if (S.qkind == QOP_QAP) res->qap_family = qap_family_of(S.qop, p->precision != HPRQP_PREC_FP64);
     if (ktimer) {
         ktimer->flush();
         res->kernel_calls = ktimer->count;
@@ -2609,11 +2610,33 @@ static int tune_dict(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
     return best;
 }
 
+// the QAP operator's pass family: the caller's, or the one timed fastest at
+// the operator's first solve in that precision (the reduced precisions run
+// the split-BF16 passes and the FP64 ones in the polish and residuals)
+static void apply_qap_family(hipStream_t s, const hprqp_params *p, Scaled &S) {
+    const int f = p->qap_family;
+    HPRQP_REQUIRE(f == 0 || f == 2 || f == 3 || f == 4, HPRQP_E_ARG, "qap_family must be 0, 2, 3 or 4");
+    if (S.qkind != QOP_QAP) return;
+    hprqp_qop *q = S.qop;
+    const bool reduced = p->precision != HPRQP_PREC_FP64;
+    if (f != 0) {
+        q->qfam64 = q->qfam16 = f;
+        return;
+    }
+    if (q->qtuned64 == 0) q->qtuned64 = qap_tune(s, q, false, p->verbose);
+    q->qfam64 = q->qtuned64;
+    if (reduced) {
+        if (q->qtuned16 == 0) q->qtuned16 = qap_tune(s, q, true, p->verbose);
+        q->qfam16 = q->qtuned16;
+    }
+}
+
 static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool mixed) {
     HPRQP_REQUIRE(p->precision == HPRQP_PREC_FP64 || p->precision == HPRQP_PREC_MIXED ||
                       p->precision == HPRQP_PREC_FP32,
                   HPRQP_E_ARG, "unknown precision mode");
     (void)mixed;
+    apply_qap_family(s, p, S);
     if (p->spmv_lanes <= -1000) {
         // the value-table form forced (rows per block -spmv_lanes - 1000)
         const int rb = -p->spmv_lanes - 1000;

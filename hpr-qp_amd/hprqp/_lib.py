@@ -48,6 +48,7 @@ class hprqp_params(C.Structure):
         ("polish_switch", C.c_double),
         ("auto_save", C.c_int32),
         ("qap_recompute_Qw", C.c_int32),
+        ("qap_family", C.c_int32),
         ("save_cb", C.c_void_p),
         ("save_user", C.c_void_p),
     ]
@@ -118,7 +119,7 @@ class hprqp_result(C.Structure):
         ("lanes_Q", C.c_int32),
         ("lanes_A", C.c_int32),
         ("lanes_AT", C.c_int32),
-        ("lanes_pad", C.c_int32),
+        ("qap_family", C.c_int32),
     ]
 
 
@@ -232,6 +233,7 @@ SIGNATURES = [
     ("hprqp_qop_create_user", C.c_int, [_P, C.POINTER(hprqp_qop_vtable), _P, C.POINTER(_P)]),
     ("hprqp_qop_destroy", C.c_int, [_P, _P]),
     ("hprqp_qop_apply", C.c_int, [_P, _P, _P, _P]),
+    ("hprqp_qop_set_qap_family", C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     ("hprqp_qop_apply_prec", C.c_int, [_P, _P, _P, _P, C.c_int]),
     ("hprqp_qop_apply_noalloc", C.c_int, [_P, _P, _P, _P]),
     ("hprqp_qop_info", C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_char_p)]),

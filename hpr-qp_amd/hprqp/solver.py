@@ -55,6 +55,7 @@ class HPRQP_parameters:
     polish_stall_checks: int = 0  # checks without KKT halving before FP64 polish (0: 100 mixed, 40 fp32, 400 fp32 offset form)
     polish_switch: float = 5.0  # mixed / plain fp32: FP64 polish once KKT < polish_switch * stoptol
     qap_recompute_Qw: bool = False  # QAP with doubly stochastic rows: True = two applies per iteration as the reference
+    qap_family: int = 0  # QAP n >= 256: 0 autotune, 2 blocked, 3 LDS 64x64, 4 LDS 64x128 (split-BF16)
 
     def to_c(self) -> _lib.hprqp_params:
         p = _lib.hprqp_params()
@@ -86,6 +87,9 @@ class HPRQP_parameters:
         p.polish_switch = float(self.polish_switch)
         p.auto_save = int(bool(self.auto_save))
         p.qap_recompute_Qw = int(bool(self.qap_recompute_Qw))
+        if self.qap_family not in (0, 2, 3, 4):
+            raise ValueError(f"qap_family must be 0, 2, 3 or 4, got {self.qap_family}")
+        p.qap_family = int(self.qap_family)
         return p
 
 
@@ -199,6 +203,7 @@ class HPRQP_results:
     kernel_ms: float = 0.0
     kernel_calls: int = 0
     lanes: tuple = (0, 0, 0)  # row-product shapes used for Q, A, A^T
+    qap_family: int = 0  # QAP operator: kernel family of the iteration's passes (qap_family codes)
     history: np.ndarray = field(default_factory=lambda: np.zeros((0, 8)))
 
 
@@ -292,6 +297,16 @@ class DeviceQOperator:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         return qb.download()
 
+    def set_qap_family(self, family: int):
+        """QAP operator: the kernel family of its applies outside a solve
+        (0 = size rule, 2 blocked, 3 LDS 64x64, 4 LDS 64x128 split-BF16);
+        returns the (FP64, split-BF16) families the applies then run."""
+        f64 = C.c_int32()
+        f16 = C.c_int32()
+        _lib.check(self.ctx.h, self.ctx.lib.hprqp_qop_set_qap_family(self.ctx.h, self.h, int(family), C.byref(f64),
+                                                                     C.byref(f16)))
+        return f64.value, f16.value
+
     def info(self):
         n = C.c_int64()
         t = C.c_int64()
@@ -365,7 +380,7 @@ class DeviceModel:
             sigma0=r.sigma0, restarts=r.restarts, Q_is_diag=bool(r.Q_is_diag), noC=bool(r.noC),
             setup_time=r.setup_time, polish_iter=r.polish_iter, power_iter_A=r.power_iter_A,
             power_iter_Q=r.power_iter_Q, kernel_ms=r.kernel_ms, kernel_calls=r.kernel_calls,
-            lanes=(r.lanes_Q, r.lanes_A, r.lanes_AT),
+            lanes=(r.lanes_Q, r.lanes_A, r.lanes_AT), qap_family=r.qap_family,
             history=hist[: r.n_history].copy())
 
     def profile(self, params: HPRQP_parameters, reps: int = 50):

@@ -37,11 +37,13 @@
 extern "C" {
 #endif
 
-#define HPRQP_ABI_VERSION 5  /* 2: hprqp_params gained the auto-save fields;
+#define HPRQP_ABI_VERSION 6  /* 2: hprqp_params gained the auto-save fields;
                                 3: hprqp_batch_create takes the total nnz of A;
                                 4: hprqp_batch_result gained interval_time;
                                 5: hprqp_params.reserved0 became
-                                   qap_recompute_Qw */
+                                   qap_recompute_Qw;
+                                6: hprqp_params gained qap_family and
+                                   hprqp_result.lanes_pad became qap_family */
 
 /* ---- error codes ---------------------------------------------------- */
 #define HPRQP_OK 0
@@ -148,6 +150,20 @@ typedef struct {
                                     Q A^T y_bar is formed from y_bar in O(nq^2);
                                     1: Q w is recomputed every iteration, as
                                     the reference does (two applies) */
+    int32_t qap_family;          /* QAP operator, n >= 256: kernel family of
+                                    its two GEMM passes.  0 = autotune (each
+                                    family that can run is timed at the
+                                    operator's first solve in each precision
+                                    and the fastest kept); 2 = blocked 32 x 32
+                                    tiles from global memory; 3 = LDS-staged
+                                    64 x 64 blocks (diagonal S and T only);
+                                    4 = LDS-staged 64 x 128 blocks (split-BF16
+                                    passes, diagonal S and T, a grid of at
+                                    least 512 blocks).  A family that cannot
+                                    run on the operator falls back to the size
+                                    rule (n < 256: one-tile passes, family 1;
+                                    blocked below n = 1024; LDS-staged from
+                                    n = 1024 with diagonal S and T) */
     hprqp_save_fn save_cb;       /* NULL: tracked but not handed over */
     void *save_user;             /* passed to save_cb */
 } hprqp_params;
@@ -188,7 +204,9 @@ typedef struct {
     int32_t lanes_AT;     /* -16 .. -256 CSR-stream rows per block, 1048576
                              column-chunked: x staged in LDS 4,096 columns at a
                              time, for long rows over many columns) */
-    int32_t lanes_pad;
+    int32_t qap_family;   /* QAP operator: family of the iteration's passes
+                             (params.qap_family codes, 1 = one-tile; the
+                             split-BF16 one in the reduced precisions) */
 } hprqp_result;
 
 /* User Q operator plugin (AbstractQOperator, Q_operator_interface.jl:33-92).
@@ -278,6 +296,16 @@ int hprqp_qop_apply_noalloc(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x,
  * buffers of n doubles.  No counterpart in the reference (FP64 only). */
 int hprqp_qop_apply_prec(hprqp_ctx *ctx, hprqp_qop *q, const hprqp_buf *x,
                          hprqp_buf *Qx, int precision);
+/* QAP operator: the kernel family its applies outside a solve take
+ * (hprqp_params.qap_family codes; 0 = the size rule), and in *fp64_family /
+ * *bf16_family (either may be NULL) the family its FP64 and split-BF16
+ * applies then run, which is the size rule's where the one asked for cannot
+ * run on the operator.  A solve replaces the setting with its own
+ * (params.qap_family, or the autotuned family).  HPRQP_E_ARG for another
+ * kind of operator or an unknown code.  No counterpart in the reference (one
+ * CPU/GPU product). */
+int hprqp_qop_set_qap_family(hprqp_ctx *ctx, hprqp_qop *q, int32_t family,
+                             int32_t *fp64_family, int32_t *bf16_family);
 /* get_problem_size / get_temp_size / get_operator_name */
 int hprqp_qop_info(const hprqp_qop *q, int64_t *n, int64_t *temp_size,
                    const char **name);

@@ -193,7 +193,7 @@ def test_library_exports_every_declared_symbol():
     # the ctypes binding covers every declared entry point
     assert set(_declared_symbols()) <= set(_lib.exported_symbols())
     lib.hprqp_abi_version.restype = C.c_int
-    assert lib.hprqp_abi_version() == 5
+    assert lib.hprqp_abi_version() == 6
 
 
 def test_struct_layouts_match_the_header(tmp_path):
