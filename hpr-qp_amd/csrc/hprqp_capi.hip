@@ -280,11 +280,15 @@ int hprqp_ctx_create(int device, hprqp_ctx **out) {
     int rc = guarded(c, [&] {
         HPRQP_HIP(hipSetDevice(device));
         HPRQP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HPRQP_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        HPRQP_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
         HPRQP_HIP(hipHostMalloc((void **)&c->pinned, sizeof(double) * 256, hipHostMallocDefault));
         c->pinned_count = 256;
         c->scratch = scratch_on(device);
     });
     if (rc != HPRQP_OK) {
+        if (c->fork) (void)hipEventDestroy(c->fork);
+        if (c->stream2) (void)hipStreamDestroy(c->stream2);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return rc;
@@ -300,6 +304,11 @@ int hprqp_ctx_destroy(hprqp_ctx *ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamDestroy(ctx->stream);
     }
+    if (ctx->stream2) {
+        (void)hipStreamSynchronize(ctx->stream2);
+        (void)hipStreamDestroy(ctx->stream2);
+    }
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     delete ctx;
     return HPRQP_OK;

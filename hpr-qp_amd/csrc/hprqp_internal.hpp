@@ -143,6 +143,10 @@ int default_lanes(const HostCsr &h);
 struct hprqp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) for work independent of the
+    // solve's stream, and the event that orders it after that stream
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork = nullptr;
     std::string err;
     double *pinned = nullptr;  // small pinned staging area for scalar readback
     size_t pinned_count = 0;

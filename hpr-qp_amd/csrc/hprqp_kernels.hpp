@@ -807,16 +807,13 @@ __device__ __forceinline__ void pow_block_partials(double s0, double s1, double 
 // in row order).  g x_j is formed before the product, so y is bitwise the
 // product of M with a stored g x.  Every thread reaches the block reduction.
 template <int L, bool SC>
-__global__ __launch_bounds__(kBlock) void k_pow_csr(int32_t nrows, const int32_t *__restrict__ rp,
-                                                    const int32_t *__restrict__ ci,
-                                                    const double *__restrict__ v, const double *__restrict__ x,
-                                                    const double *__restrict__ t, const double *__restrict__ gs,
-                                                    double *__restrict__ y, double *__restrict__ part) {
-    const int b = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void pow_csr_body(int b, int32_t nrows, const int32_t *__restrict__ rp,
+                                             const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                             const double *__restrict__ x, const double *__restrict__ t, double g,
+                                             double *__restrict__ y, double *__restrict__ part) {
     const int64_t tid = (int64_t)b * kBlock + threadIdx.x;
     const int32_t row = (int32_t)(tid / L);
     const int lane = threadIdx.x & (L - 1);
-    const double g = SC ? *gs : 1.0;
     double s0 = 0.0, s1 = 0.0;
     if (row < nrows) {  // whole L-group in or out together
         const int32_t s = rp[row], e = rp[row + 1];
@@ -843,19 +840,27 @@ __global__ __launch_bounds__(kBlock) void k_pow_csr(int32_t nrows, const int32_t
     pow_block_partials(s0, s1, part + 2 * (int64_t)b);
 }
 
+template <int L, bool SC>
+__global__ __launch_bounds__(kBlock) void k_pow_csr(int32_t nrows, const int32_t *__restrict__ rp,
+                                                    const int32_t *__restrict__ ci,
+                                                    const double *__restrict__ v, const double *__restrict__ x,
+                                                    const double *__restrict__ t, const double *__restrict__ gs,
+                                                    double *__restrict__ y, double *__restrict__ part,
+                                                    const double *__restrict__ stop) {
+    if (*stop != 0.0) return;  // the run has stopped (a whole grid returns)
+    pow_csr_body<L, SC>(xcd_block(blockIdx.x, gridDim.x), nrows, rp, ci, v, x, t, SC ? *gs : 1.0, y, part);
+}
+
 // the same over the CSR-stream shape (rb rows per block, the summation of
 // k_csr_stream): one partial pair per block of rows
 template <bool SC>
-__global__ __launch_bounds__(kBlock) void k_pow_stream(int32_t nrows, int rb, const int32_t *__restrict__ rp,
-                                                       const int32_t *__restrict__ ci,
-                                                       const double *__restrict__ v, const double *__restrict__ x,
-                                                       const double *__restrict__ t, const double *__restrict__ gs,
-                                                       double *__restrict__ y, double *__restrict__ part) {
+__device__ __forceinline__ void pow_stream_body(int b, int32_t nrows, int rb, const int32_t *__restrict__ rp,
+                                                const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                                const double *__restrict__ x, const double *__restrict__ t, double g,
+                                                double *__restrict__ y, double *__restrict__ part) {
     __shared__ double prod[kStreamCap];
-    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int32_t r0 = b * rb;
     const int32_t r1 = min(r0 + rb, nrows);
-    const double g = SC ? *gs : 1.0;
     const int tt = threadIdx.x;
     const int32_t row = r0 + tt;
     double s0 = 0.0, s1 = 0.0;
@@ -900,6 +905,212 @@ __global__ __launch_bounds__(kBlock) void k_pow_stream(int32_t nrows, int rb, co
     pow_block_partials(s0, s1, part + 2 * (int64_t)b);
 }
 
+template <bool SC>
+__global__ __launch_bounds__(kBlock) void k_pow_stream(int32_t nrows, int rb, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci,
+                                                       const double *__restrict__ v, const double *__restrict__ x,
+                                                       const double *__restrict__ t, const double *__restrict__ gs,
+                                                       double *__restrict__ y, double *__restrict__ part,
+                                                       const double *__restrict__ stop) {
+    if (*stop != 0.0) return;
+    pow_stream_body<SC>(xcd_block(blockIdx.x, gridDim.x), nrows, rb, rp, ci, v, x, t, SC ? *gs : 1.0, y, part);
+}
+
+// Power method on a sparse Q in one launch per iteration.  The launch of
+// iteration t totals iteration t-1's partial pairs -- every block reads all
+// of them in the same fixed order, so every block holds the same totals --
+// evaluates t-1's stop test from them and, unless the run has stopped,
+// stores z_{t+1} = Q (z_t / |z_t|) with its own partials.  So the scalar
+// update needs no launch of its own and nothing has to cross the XCDs inside
+// a launch: block 0 only records what every block computed.  The loads of
+// the totals, of res[0] and of the row product are all issued before the
+// first barrier, so their latencies overlap; the product is formed unscaled
+// and multiplied by 1/|z_t| once that is known (y = g (Q z) rather than
+// Q (g z): a rounding apart).
+//   prev, pnb: iteration t-1's pairs (for t = 0 the pair (0, |z_0|^2));
+//   part: this iteration's pairs; zz: ring of |z_t|^2 (zz[t & 3], written
+//   by block 0 of launch t, read by launch t + 1); res: {stopped, lambda,
+//   iterations, err} of the last test.
+// A block that reads res[0] as set by block 0 of its own launch returns as
+// it would after computing the same test; thread 0 reads res[0] for the
+// whole block, so the block returns as one.
+constexpr int kPowQPairs = 1024;  // most partial pairs of an iteration in this form
+static_assert(kPowQPairs == 4 * kBlock, "four pairs per thread");
+struct PowQArgs {
+    const double *prev;
+    int pnb;
+    double *part;
+    double *zz;
+    double *res;
+    int64_t t;
+    double tol;
+};
+
+// first half: res[0] into *stopped (thread 0), this thread's share of the
+// previous pairs summed over its wave into sh (the caller's barrier follows)
+__device__ __forceinline__ void pow_q_issue(const PowQArgs &a, double *stopped, double (*sh)[2]) {
+    static_assert(kBlock == 256, "the totals sum four waves");
+    if (threadIdx.x == 0) *stopped = a.res[0];
+    const double2 *p2 = (const double2 *)a.prev;
+    double2 u[kPowQPairs / kBlock];
+#pragma unroll
+    for (int k = 0; k < kPowQPairs / kBlock; ++k) {
+        const int i = threadIdx.x + k * kBlock;
+        u[k] = i < a.pnb ? p2[i] : double2{0.0, 0.0};
+    }
+    double s0 = (u[0].x + u[1].x) + (u[2].x + u[3].x);
+    double s1 = (u[0].y + u[1].y) + (u[2].y + u[3].y);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6][0] = s0;
+        sh[threadIdx.x >> 6][1] = s1;
+    }
+}
+
+// second half, after a barrier: false when the run has stopped (the launch
+// stores nothing); else g = 1/|z_t|
+__device__ __forceinline__ bool pow_q_test(const PowQArgs &a, double stopped, const double (*sh)[2], double &g) {
+    if (stopped != 0.0) return false;
+    const double dot = (sh[0][0] + sh[1][0]) + (sh[2][0] + sh[3][0]);
+    const double nn = (sh[0][1] + sh[1][1]) + (sh[2][1] + sh[3][1]);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (a.t > 0) {
+        // the reference's test for iteration t-1 (src/utils.jl:1357-1376):
+        // lambda = q.z', err = |z' - lambda q| / (|z'| + lambda), with
+        // |z' - lambda q|^2 = |z'|^2 - lambda^2 (2 - |q|^2)
+        const double zz = a.zz[(a.t - 1) & 3], rz = 1.0 / sqrt(zz);
+        const double lam = dot, qn = zz * rz * rz;
+        const double e2 = fmax(nn - lam * lam * (2.0 - qn), 0.0);
+        const double err = sqrt(e2) / (sqrt(nn) + lam);
+        if (lead) {
+            a.res[1] = lam;
+            a.res[2] = (double)a.t;
+            a.res[3] = err;
+            if (err < a.tol) a.res[0] = 1.0;
+        }
+        if (err < a.tol) return false;
+    }
+    if (lead) a.zz[a.t & 3] = nn;
+    g = 1.0 / sqrt(nn);
+    return true;
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_pow_q_csr(int32_t nrows, const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                                      const double *__restrict__ x, double *__restrict__ y,
+                                                      PowQArgs a) {
+    __shared__ double sh[kBlock / 64][2];
+    __shared__ double stopped;
+    pow_q_issue(a, &stopped, sh);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t tid = (int64_t)b * kBlock + threadIdx.x;
+    const int32_t row = (int32_t)(tid / L);
+    const int lane = threadIdx.x & (L - 1);
+    double acc = 0.0;
+    if (row < nrows) {  // whole L-group in or out together
+        const int32_t s = rp[row], e = rp[row + 1];
+        double acc0 = 0.0, acc1 = 0.0;
+        int32_t k = s + lane;
+        for (; k + L < e; k += 2 * L) {
+            acc0 += v[k] * x[ci[k]];
+            acc1 += v[k + L] * x[ci[k + L]];
+        }
+        if (k < e) acc0 += v[k] * x[ci[k]];
+        acc = acc0 + acc1;
+        if constexpr (L > 1) {
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+        }
+    }
+    __syncthreads();
+    double g;
+    if (!pow_q_test(a, stopped, sh, g)) return;
+    double s0 = 0.0, s1 = 0.0;
+    if (row < nrows && lane == 0) {
+        const double yr = acc * g;
+        y[row] = yr;
+        s0 = yr * (x[row] * g);
+        s1 = yr * yr;
+    }
+    pow_block_partials(s0, s1, a.part + 2 * (int64_t)b);
+}
+
+static __global__ __launch_bounds__(kBlock) void k_pow_q_stream(int32_t nrows, int rb, const int32_t *__restrict__ rp,
+                                                                const int32_t *__restrict__ ci,
+                                                                const double *__restrict__ v,
+                                                                const double *__restrict__ x, double *__restrict__ y,
+                                                                PowQArgs a) {
+    __shared__ double sh[kBlock / 64][2];
+    __shared__ double stopped;
+    __shared__ double prod[kStreamCap];
+    pow_q_issue(a, &stopped, sh);
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t r0 = b * rb;
+    const int32_t r1 = min(r0 + rb, nrows);
+    const int tt = threadIdx.x;
+    const int32_t row = r0 + tt;
+    // r0 < nrows for every block of the grid (nrows / rb rounded up), so
+    // the block is never empty; all its threads reach the barriers
+    const int32_t base = rp[r0];
+    const int32_t cnt = rp[r1] - base;
+    double acc = 0.0;
+    if (cnt <= kStreamCap) {
+        for (int32_t k0 = 0; k0 < cnt; k0 += kStreamU * kBlock) {
+            double vv[kStreamU];
+            int32_t cc[kStreamU];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) {
+                const int32_t k = k0 + j * kBlock + tt;
+                const int32_t at = base + (k < cnt ? k : 0);
+                vv[j] = v[at];
+                cc[j] = ci[at];
+            }
+            double pr[kStreamU];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) pr[j] = vv[j] * x[cc[j]];
+#pragma unroll
+            for (int j = 0; j < kStreamU; ++j) {
+                const int32_t k = k0 + j * kBlock + tt;
+                if (k < cnt) prod[k] = pr[j];
+            }
+        }
+        __syncthreads();
+        if (row < r1) {
+            const int32_t s = rp[row] - base, e = rp[row + 1] - base;
+            for (int32_t k = s; k < e; ++k) acc += prod[k];
+        }
+    } else if (row < r1) {
+        for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += v[k] * x[ci[k]];
+    }
+    __syncthreads();
+    double g;
+    if (!pow_q_test(a, stopped, sh, g)) return;
+    double s0 = 0.0, s1 = 0.0;
+    if (row < r1) {
+        const double yr = acc * g;
+        y[row] = yr;
+        s0 = yr * (x[row] * g);
+        s1 = yr * yr;
+    }
+    pow_block_partials(s0, s1, a.part + 2 * (int64_t)b);
+}
+
+// the test of the last iteration enqueued (one block), when the run ends
+// at its iteration limit
+static __global__ __launch_bounds__(kBlock) void k_pow_q_tail(PowQArgs a) {
+    __shared__ double sh[kBlock / 64][2];
+    __shared__ double stopped;
+    pow_q_issue(a, &stopped, sh);
+    __syncthreads();
+    double g;
+    (void)pow_q_test(a, stopped, sh, g);
+}
+
 // the same over the one-block-per-row shape (the summation of
 // k_csr_rowblock): one partial pair per row
 template <bool SC>
@@ -907,8 +1118,10 @@ __global__ __launch_bounds__(kBlock) void k_pow_rowblock(int32_t nrows, const in
                                                          const int32_t *__restrict__ ci,
                                                          const double *__restrict__ v, const double *__restrict__ x,
                                                          const double *__restrict__ t, const double *__restrict__ gs,
-                                                         double *__restrict__ y, double *__restrict__ part) {
+                                                         double *__restrict__ y, double *__restrict__ part,
+                                                         const double *__restrict__ stop) {
     __shared__ double pw[kBlock / 64];
+    if (*stop != 0.0) return;
     const int32_t row = blockIdx.x;
     const double g = SC ? *gs : 1.0;
     const int32_t s = rp[row], e = rp[row + 1];
@@ -932,6 +1145,48 @@ __global__ __launch_bounds__(kBlock) void k_pow_rowblock(int32_t nrows, const in
         part[2 * (int64_t)row] = tot * (SC ? t[row] * g : t[row]);
         part[2 * (int64_t)row + 1] = tot * tot;
     }
+}
+
+// The power method's final pass: the totals of nb <= kPowMaxBlocks partial
+// pairs, handed to post(tot) on thread 0 through LDS (see
+// k_reduce_final_post).  1,024 threads, each loading up to four pairs at
+// once (16-byte loads), so the pass is one round of loads.  Nothing runs once
+// the run has stopped.
+constexpr int kPowFinalThreads = 1024;
+static_assert(kPowMaxBlocks <= 4 * kPowFinalThreads, "four partial pairs per thread");
+template <class Post>
+__global__ __launch_bounds__(kPowFinalThreads) void k_pow_final(int nb, const double *__restrict__ part,
+                                                                const double *__restrict__ stop, Post post) {
+    if (*stop != 0.0) return;
+    const double2 *p2 = (const double2 *)part;
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int i = threadIdx.x + u * kPowFinalThreads;
+        v[u] = i < nb ? p2[i] : double2{0.0, 0.0};
+    }
+    double s0 = (v[0].x + v[1].x) + (v[2].x + v[3].x);
+    double s1 = (v[0].y + v[1].y) + (v[2].y + v[3].y);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+    }
+    __shared__ double sh[kPowFinalThreads / 64][2];
+    __shared__ double tot[2];
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6][0] = s0;
+        sh[threadIdx.x >> 6][1] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double a = 0.0;
+#pragma unroll
+        for (int w = 0; w < kPowFinalThreads / 64; ++w) a += sh[w][threadIdx.x];
+        tot[threadIdx.x] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) post((const double *)tot);
 }
 
 // ---------------------------------------------------------------------

@@ -992,79 +992,143 @@ struct Solver {
     }
 
     // the power-method product y = a (g x) with its block partials, nb =
-    // pow_blocks(a)
+    // pow_blocks(a), on stream ps; nothing once *stop is set
     template <bool SC>
-    void pow_csr(const DevCsr &a, int64_t nb, const double *x, const double *t, const double *gs, double *y,
-                 double *part) {
+    static void pow_csr(hipStream_t ps, const DevCsr &a, int64_t nb, const double *x, const double *t,
+                        const double *gs, double *y, double *part, const double *stop) {
         const int32_t nr = a.nrow;
         const int32_t *rp = a.rp.p, *ci = a.ci.p;
         const double *v = a.v.p;
         const dim3 grid((unsigned)nb), blk(kBlock);
         if (a.lanes < 0) {
-            k_pow_stream<SC><<<grid, blk, 0, s>>>(nr, -a.lanes, rp, ci, v, x, t, gs, y, part);
+            k_pow_stream<SC><<<grid, blk, 0, ps>>>(nr, -a.lanes, rp, ci, v, x, t, gs, y, part, stop);
             HPRQP_HIP(hipGetLastError());
             return;
         }
         if (a.lanes > 64) {
-            k_pow_rowblock<SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part);
+            k_pow_rowblock<SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop);
             HPRQP_HIP(hipGetLastError());
             return;
         }
         switch (a.lanes) {
-            case 1: k_pow_csr<1, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
-            case 2: k_pow_csr<2, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
-            case 4: k_pow_csr<4, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
-            case 8: k_pow_csr<8, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
-            case 16: k_pow_csr<16, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
-            case 32: k_pow_csr<32, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
-            default: k_pow_csr<64, SC><<<grid, blk, 0, s>>>(nr, rp, ci, v, x, t, gs, y, part); break;
+            case 1: k_pow_csr<1, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
+            case 2: k_pow_csr<2, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
+            case 4: k_pow_csr<4, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
+            case 8: k_pow_csr<8, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
+            case 16: k_pow_csr<16, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
+            case 32: k_pow_csr<32, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
+            default: k_pow_csr<64, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
         }
         HPRQP_HIP(hipGetLastError());
     }
 
-    // which = 0: A A^T (size m); which = 1: Q (size n).  FP64 throughout.
-    // The reference's iteration (src/utils.jl:1357-1376): q = z / |z|,
-    // z' = A A^T q (or Q q), lambda = q.z', err = |z' - lambda q| / (|z'| +
-    // lambda), stop at the first err < tol.  Here, per iteration, the A^T
-    // product reads z and scales its rows by 1/|z| (q is not stored; for Q
-    // it is), one reduction gives z'.z (or z'.q) and |z'|^2, and
-    // |z' - lambda q|^2 is expanded as |z'|^2 - 2 lambda q.z' + lambda^2
-    // |q|^2 = |z'|^2 - lambda^2 (2 - |q|^2), with |q|^2 = |z|^2 / |z|^2 as
-    // rounded; |z'|^2 is the next iteration's |z|^2.  For A A^T and a
-    // sparse Q the last product also writes its block partials of both sums
-    // (k_pow_csr / k_pow_stream / k_pow_rowblock, the summation of the
-    // solve's own row-product shape; for Q it gathers z scaled by 1/|z| on
-    // the fly, bitwise the product with a stored q) and the final pass does
-    // the scalar update: three launches per iteration for A A^T, two for a
-    // sparse Q.  Other operators take five (product, reduction, scalars).
-    // The host polls every 16 iterations.  (Replaying those 16 from a
+    // One power-method run: which = 0: A A^T (size m); which = 1: Q (size
+    // n).  FP64 throughout.  The reference's iteration
+    // (src/utils.jl:1357-1376): q = z / |z|, z' = A A^T q (or Q q), lambda =
+    // q.z', err = |z' - lambda q| / (|z'| + lambda), stop at the first err <
+    // tol.  Here, per iteration, the A^T product reads z and scales its rows
+    // by 1/|z| (q is not stored; for Q it is), one reduction gives z'.z (or
+    // z'.q) and |z'|^2, and |z' - lambda q|^2 is expanded as |z'|^2 - 2
+    // lambda q.z' + lambda^2 |q|^2 = |z'|^2 - lambda^2 (2 - |q|^2), with
+    // |q|^2 = |z|^2 / |z|^2 as rounded; |z'|^2 is the next iteration's
+    // |z|^2.  For A A^T and a sparse Q the last product also writes its
+    // block partials of both sums (k_pow_csr / k_pow_stream /
+    // k_pow_rowblock, the summation of the solve's own row-product shape;
+    // for Q it gathers z scaled by 1/|z| on the fly, bitwise the product
+    // with a stored q) and the final pass does the scalar update: three
+    // launches per iteration for A A^T, two for a sparse Q ("fused").  Other
+    // operators take five (product, reduction, scalars).  Iterations are
+    // enqueued in chunks with a copy of the scalars behind each; in the fused
+    // forms the host reads one chunk's copy while the next chunk (8
+    // iterations) runs, and the launches of iterations past the stop store
+    // nothing; the others are read after every chunk of 16.  A sparse Q whose product makes at most kPowQPairs partial
+    // pairs takes one launch per iteration (k_pow_q_csr / k_pow_q_stream:
+    // each launch totals the previous iteration's pairs itself, evaluates its
+    // stop test and scales the gather by the new 1/|z|), its test recorded in
+    // res = {stopped, lambda, iterations, err}.  (Replaying the 16 from a
     // hipGraph was slower: the random QP's 197 iterations took 4.1 ms with
     // the capture and instantiation each call needs, 2.8 ms as launches.)
-    double power_iteration(int which, int64_t max_it, double tol, int64_t *its) {
-        const int64_t len = which == 0 ? m : n;
-        std::vector<double> z0(len);
-        randn_stream(1, len, z0.data());
+    struct PowerRun {
+        int which = 0;
+        int64_t len = 0, it = 0, its = 0;
+        hipStream_t ps = nullptr;
+        DevVec<double> za, zb, q, ATq, fpart;
+        // one-launch Q form: its partial pairs in two slots (fpart), and
+        // qaux = {0, |z_0|^2 | zz ring of 4 | res of 4}
+        bool oneq = false;
+        DevVec<double> qaux;
+        const DevCsr *fm = nullptr;
+        int64_t fblocks = 0;
+        int slot = 32;                       // device scalars at red.out + slot
+        double *d = nullptr, *host = nullptr;  // device scalars; two pinned copies at host, host + 16
+        double *zo = nullptr, *zn = nullptr;
+        double lambda = 1.0;
+        bool done = false;
+        // chunks enqueued and read; iterations enqueued up to each copy; the
+        // event behind each copy
+        int64_t launched = 0, polled = 0, it_at[2] = {0, 0};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        PowerRun() = default;
+        PowerRun(const PowerRun &) = delete;
+        ~PowerRun() {
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+        }
+    };
+
+    // z0 (the reference's randn(1) stream + 1e-8) and |z0|^2, enqueued on s
+    void power_start(PowerRun &r, int which, int slot, double *host) {
+        r.which = which;
+        r.len = which == 0 ? m : n;
+        r.ps = s;
+        r.slot = slot;
+        r.host = host;
+        std::vector<double> z0(r.len);
+        randn_stream(1, r.len, z0.data());
         for (auto &v : z0) v += 1e-8;
-        DevVec<double> za(len), zb(len), q(which == 1 ? len : 0), ATq(which == 0 ? n : 0);
-        HPRQP_HIP(hipMemcpyAsync(za.p, z0.data(), sizeof(double) * len, hipMemcpyHostToDevice, s));
+        r.za.alloc(r.len);
+        r.zb.alloc(r.len);
+        if (which == 1) r.q.alloc(r.len);
+        if (which == 0) r.ATq.alloc(n);
+        HPRQP_HIP(hipMemcpyAsync(r.za.p, z0.data(), sizeof(double) * r.len, hipMemcpyHostToDevice, s));
         // device scalars: 0 |z|^2, 1 1/|z|, 2 z'.z (A A^T) or z'.q (Q),
         // 3 |z'|^2, 4 done, 5 lambda at the stop, 6 stop iteration, 7 error,
         // 8 iterations run.  Nothing is recorded after the converging
         // iteration, so the result is the reference's (first iteration with
         // err < tol) while the host polls rarely.
-        double *d = red.out.p + 32;
+        double *d = red.out.p + slot;
+        r.d = d;
         HPRQP_HIP(hipMemsetAsync(d, 0, sizeof(double) * 9, s));
-        red.enqueue<1, 0>(len, SumSq{za.p}, 32);
+        red.enqueue<1, 0>(r.len, SumSq{r.za.p}, slot);
         k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { d[1] = 1.0 / sqrt(d[0]); });
         HPRQP_HIP(hipGetLastError());
         // the fused form: the product whose sums are reduced (A for A A^T,
         // Q) in a lanes-per-row shape, and few enough blocks for one final
         // pass over their partials
-        const DevCsr *fm = which == 0 ? &S.A : (S.qkind == QOP_CSR && !S.q_empty ? &S.Q : nullptr);
-        const int64_t fblocks = fm && fm->nrow == len ? pow_blocks(*fm) : 0;
-        DevVec<double> fpart(2 * fblocks);
+        r.fm = which == 0 ? &S.A : (S.qkind == QOP_CSR && !S.q_empty ? &S.Q : nullptr);
+        r.fblocks = r.fm && r.fm->nrow == r.len ? pow_blocks(*r.fm) : 0;
+        r.oneq = which == 1 && r.fblocks > 0 && r.fblocks <= kPowQPairs && r.fm->lanes != kBlock;
+        r.fpart.alloc((r.oneq ? 4 : 2) * r.fblocks);
+        if (r.oneq) {
+            r.qaux.alloc(10);
+            double *qa = r.qaux.p;
+            HPRQP_HIP(hipMemsetAsync(qa, 0, sizeof(double) * 10, s));
+            k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { qa[1] = d[0]; });
+            HPRQP_HIP(hipGetLastError());
+        }
+        r.zo = r.za.p;
+        r.zn = r.zb.p;
+        for (hipEvent_t &e : r.ev) HPRQP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+
+    // up to 16 iterations and the scalars' readback, enqueued on r.ps.  Only
+    // the fused form may run on a stream other than s: the others use the
+    // shared reduction scratch and the operator's own temporaries.
+    void power_chunk(PowerRun &r, int64_t max_it, double tol) {
+        double *d = r.d;
         const double tl = tol;
-        const bool scaled = which == 0;
+        const bool scaled = r.which == 0;
+        const hipStream_t ps = r.ps;
         // tot = {z'.z (or z'.q), |z'|^2}; d[0], d[1], d[4], d[8] were
         // written by earlier launches
         auto post = [=] __device__(const double *tot) {
@@ -1086,48 +1150,166 @@ struct Solver {
             d[1] = 1.0 / sqrt(nn);
             d[8] = itn;
         };
-        double *zo = za.p, *zn = zb.p;
-        auto iteration = [&]() {
-            if (fblocks > 0) {
-                if (which == 0) {
-                    csr_apply<double, double>(s, S.AT, (const double *)zo, EpiStoreScaled<double>{ATq.p, d + 1});
-                    pow_csr<false>(*fm, fblocks, ATq.p, zo, d + 1, zn, fpart.p);
+        // fused runs are read one chunk behind, so a chunk is what runs past
+        // the stop: 8 iterations there, 16 where each chunk is read at once
+        const int64_t kPoll = r.fblocks > 0 ? 8 : 16;
+        const int64_t chunk = std::min<int64_t>(kPoll, max_it - r.it);
+        if (r.oneq) {
+            const DevCsr &a = *r.fm;
+            const int64_t nb = r.fblocks;
+            double *qa = r.qaux.p;
+            auto args = [&](int64_t t) {
+                return PowQArgs{t == 0 ? (const double *)qa : (const double *)r.fpart.p + 2 * nb * ((t - 1) & 1),
+                                t == 0 ? 1 : (int)nb, r.fpart.p + 2 * nb * (t & 1), qa + 2, qa + 6, t, tol};
+            };
+            const dim3 grid((unsigned)nb), blk(kBlock);
+            for (int64_t k = 0; k < chunk; ++k) {
+                const PowQArgs pa = args(r.it + k);
+                const double *zo = r.zo;
+                if (a.lanes < 0) {
+                    k_pow_q_stream<<<grid, blk, 0, ps>>>(a.nrow, -a.lanes, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa);
                 } else {
-                    pow_csr<true>(*fm, fblocks, zo, zo, d + 1, zn, fpart.p);
+                    switch (a.lanes) {
+                        case 1: k_pow_q_csr<1><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
+                        case 2: k_pow_q_csr<2><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
+                        case 4: k_pow_q_csr<4><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
+                        case 8: k_pow_q_csr<8><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
+                        case 16: k_pow_q_csr<16><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
+                        case 32: k_pow_q_csr<32><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
+                        default: k_pow_q_csr<64><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
+                    }
                 }
-                k_reduce_final_post<2, 0><<<1, kBlock, 0, s>>>((int)fblocks, fpart.p, post);
+                HPRQP_HIP(hipGetLastError());
+                std::swap(r.zo, r.zn);
+            }
+            r.it += chunk;
+            // at the limit, the last iteration's test has no next launch
+            if (r.it >= max_it) {
+                k_pow_q_tail<<<1, kBlock, 0, ps>>>(args(r.it));
+                HPRQP_HIP(hipGetLastError());
+            }
+            const int k = (int)(r.launched & 1);
+            HPRQP_HIP(hipMemcpyAsync(r.host + 16 * k, qa + 6, sizeof(double) * 4, hipMemcpyDeviceToHost, ps));
+            HPRQP_HIP(hipEventRecord(r.ev[k], ps));
+            r.it_at[k] = r.it;
+            r.launched++;
+            return;
+        }
+        for (int64_t k = 0; k < chunk; ++k) {
+            const double *zo = r.zo;
+            double *zn = r.zn;
+            if (r.fblocks > 0) {
+                if (r.which == 0) {
+                    csr_apply<double, double>(ps, S.AT, zo, EpiStoreScaled<double>{r.ATq.p, d + 1});
+                    pow_csr<false>(ps, *r.fm, r.fblocks, r.ATq.p, zo, d + 1, zn, r.fpart.p, d + 4);
+                } else {
+                    pow_csr<true>(ps, *r.fm, r.fblocks, zo, zo, d + 1, zn, r.fpart.p, d + 4);
+                }
+                k_pow_final<<<1, kPowFinalThreads, 0, ps>>>((int)r.fblocks, r.fpart.p, d + 4, post);
                 HPRQP_HIP(hipGetLastError());
             } else {
-                if (which == 0) {
-                    csr_apply<double, double>(s, S.AT, (const double *)zo, EpiStoreScaled<double>{ATq.p, d + 1});
-                    csr_apply<double, double>(s, S.A, (const double *)ATq.p, EpiStore<double>{zn});
-                    red.enqueue<2, 0>(len, Dot2Sq{zn, zo}, 34);
+                HPRQP_REQUIRE(ps == s, HPRQP_E_STATE, "unfused power method off the solve's stream");
+                if (r.which == 0) {
+                    csr_apply<double, double>(s, S.AT, zo, EpiStoreScaled<double>{r.ATq.p, d + 1});
+                    csr_apply<double, double>(s, S.A, (const double *)r.ATq.p, EpiStore<double>{zn});
+                    red.enqueue<2, 0>(r.len, Dot2Sq{zn, zo}, r.slot + 2);
                 } else {
-                    double *qq = q.p;
-                    const double *zz = zo;
-                    each(s, len, [=] __device__(int64_t i) { qq[i] = zz[i] * d[1]; });
-                    qop_fp64(q.p, zn);
-                    red.enqueue<2, 0>(len, Dot2Sq{zn, q.p}, 34);
+                    double *qq = r.q.p;
+                    each(s, r.len, [=] __device__(int64_t i) { qq[i] = zo[i] * d[1]; });
+                    qop_fp64(r.q.p, zn);
+                    red.enqueue<2, 0>(r.len, Dot2Sq{zn, r.q.p}, r.slot + 2);
                 }
                 k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { post(d + 2); });
                 HPRQP_HIP(hipGetLastError());
             }
-            std::swap(zo, zn);
-        };
-        constexpr int kPoll = 16;
-        double lambda_max = 1.0;
-        for (int64_t it = 0; it < max_it;) {
-            const int64_t chunk = std::min<int64_t>(kPoll, max_it - it);
-            for (int64_t k = 0; k < chunk; ++k) iteration();
-            it += chunk;
-            HPRQP_HIP(hipMemcpyAsync(red.host, d, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
-            HPRQP_HIP(hipStreamSynchronize(s));
-            lambda_max = red.host[5];
-            *its = (int64_t)red.host[6];
-            if (red.host[4] != 0.0) return lambda_max;
+            std::swap(r.zo, r.zn);
         }
-        if (p->verbose) std::printf("Power iteration did not converge within %lld iterations\n", (long long)max_it);
-        return lambda_max;
+        r.it += chunk;
+        const int k = (int)(r.launched & 1);
+        HPRQP_HIP(hipMemcpyAsync(r.host + 16 * k, d, sizeof(double) * 9, hipMemcpyDeviceToHost, ps));
+        HPRQP_HIP(hipEventRecord(r.ev[k], ps));
+        r.it_at[k] = r.it;
+        r.launched++;
+    }
+
+    // read the oldest unread chunk's copy (waiting for it); sets done once
+    // the run has stopped or reached max_it
+    void power_poll(PowerRun &r, int64_t max_it) {
+        const int k = (int)(r.polled & 1);
+        HPRQP_HIP(hipEventSynchronize(r.ev[k]));
+        r.polled++;
+        const double *h = r.host + 16 * k;
+        // {stopped, lambda, iterations} at h[0..2] (one-launch Q form) or
+        // h[4..6]
+        const int o = r.oneq ? 0 : 4;
+        r.lambda = h[o + 1];
+        r.its = (int64_t)h[o + 2];
+        if (h[o] != 0.0) r.done = true;
+        if (!r.done && r.it_at[k] >= max_it) {
+            r.done = true;
+            if (p->verbose)
+                std::printf("Power iteration did not converge within %lld iterations\n", (long long)max_it);
+        }
+    }
+
+    // one run to its stop: the next chunk is enqueued before the previous
+    // one's copy is read (at most two chunks unread), so the GPU does not
+    // drain while the host polls; chunks still in flight at the stop
+    // finish (returning at once in the fused form) before the run's buffers go
+    void power_step(PowerRun &r, int64_t max_it, double tol) {
+        if (r.launched - r.polled < 2 && r.it < max_it) power_chunk(r, max_it, tol);
+    }
+
+    // lambda_max(A A^T) and lambda_max(Q) (estimate_eigenvalues,
+    // src/algorithm.jl:1407-1440), by one run each.  The two runs are
+    // independent: when the A A^T run has the fused form it
+    // goes on the context's second stream, its 16-iteration chunks
+    // interleaved with the Q run's on s, so the small launches of the two
+    // overlap on the GPU.  Either may be skipped (need_A / need_Q false).
+    void power_pair(bool need_A, bool need_Q, int64_t max_it, double tol, double *lam_A, int64_t *its_A,
+                    double *lam_Q, int64_t *its_Q) {
+        PowerRun ra, rq;
+        if (need_A) power_start(ra, 0, 32, red.host);
+        if (need_Q) power_start(rq, 1, 48, red.host + 128);
+        const bool both = need_A && need_Q && ra.fblocks > 0 && ctx->stream2;
+        // a run whose form is not fused enqueues nothing past its stop test
+        // that would return at once: it is read after every chunk
+        auto run_to_stop = [&](PowerRun &r) {
+            const bool ahead = r.fblocks > 0;
+            while (!r.done) {
+                power_step(r, max_it, tol);
+                if (ahead) power_step(r, max_it, tol);
+                power_poll(r, max_it);
+            }
+            HPRQP_HIP(hipStreamSynchronize(r.ps));
+        };
+        if (both) {
+            ra.ps = ctx->stream2;
+            HPRQP_HIP(hipEventRecord(ctx->fork, s));
+            HPRQP_HIP(hipStreamWaitEvent(ra.ps, ctx->fork, 0));
+            while (!ra.done || !rq.done) {
+                for (PowerRun *r : {&ra, &rq})
+                    if (!r->done) {
+                        power_step(*r, max_it, tol);
+                        power_step(*r, max_it, tol);
+                    }
+                for (PowerRun *r : {&ra, &rq})
+                    if (!r->done) power_poll(*r, max_it);
+            }
+            HPRQP_HIP(hipStreamSynchronize(ra.ps));
+            HPRQP_HIP(hipStreamSynchronize(rq.ps));
+        } else {
+            if (need_A) run_to_stop(ra);
+            if (need_Q) run_to_stop(rq);
+        }
+        if (need_A) {
+            *lam_A = ra.lambda;
+            *its_A = ra.its;
+        }
+        if (need_Q) {
+            *lam_Q = rq.lambda;
+            *its_Q = rq.its;
+        }
     }
 
     // FP64 operator apply on FP64 vectors (power method)
@@ -2322,6 +2504,25 @@ struct OperatorLanesGuard {
     }
 };
 
+// estimate_eigenvalues (src/algorithm.jl:1407-1440): lambda_max(A A^T) when
+// there are constraints; lambda_max(Q) by the power method for a non-empty,
+// non-diagonal Q or any other operator (the largest entry for a diagonal Q,
+// 0 for an empty one); both scaled by eig_factor
+template <class VT, class MT>
+static void estimate_lambdas(Solver<VT, MT> &D, const hprqp_model *md, double eig_factor, int64_t *its_A,
+                             int64_t *its_Q) {
+    const Scaled &S = D.S;
+    const bool need_A = md->m > 0;
+    const bool need_Q = S.qkind != QOP_CSR || (!S.q_empty && !S.Q_is_diag);
+    double lam_A = 0.0, lam_Q = 0.0;
+    D.power_pair(need_A, need_Q, 5000, 1e-4, &lam_A, its_A, &lam_Q, its_Q);
+    D.lambda_max_A = need_A ? lam_A * eig_factor : 0.0;
+    if (need_Q)
+        D.lambda_max_Q = lam_Q * eig_factor;
+    else
+        D.lambda_max_Q = S.qkind == QOP_CSR && !S.q_empty ? S.q_max_val : 0.0;
+}
+
 void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const double *x0, const double *y0,
            double *x_out, double *y_out, double *z_out, double *w_out, double *hist, int64_t hist_cap,
            hprqp_result *res) {
@@ -2363,14 +2564,7 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
 
     const double t_start = now_s();
     // estimate_eigenvalues (src/algorithm.jl:1407-1440)
-    int64_t *pa = &res->power_iter_A, *pq = &res->power_iter_Q;
-    D.lambda_max_A = md->m > 0 ? D.power_iteration(0, 5000, 1e-4, pa) * p->eig_factor : 0.0;
-    if (S.qkind != QOP_CSR)
-        D.lambda_max_Q = D.power_iteration(1, 5000, 1e-4, pq) * p->eig_factor;
-    else if (!S.q_empty)
-        D.lambda_max_Q = S.Q_is_diag ? S.q_max_val : D.power_iteration(1, 5000, 1e-4, pq) * p->eig_factor;
-    else
-        D.lambda_max_Q = 0.0;
+    estimate_lambdas(D, md, p->eig_factor, &res->power_iter_A, &res->power_iter_Q);
     res->power_time = now_s() - t_start;
     // noC decision uses the unscaled bounds (src/algorithm.jl:1969-1974)
     int64_t nfree = 0;
@@ -2940,14 +3134,7 @@ struct WsImpl final : hprqp_ws {
         HPRQP_HIP(hipStreamSynchronize(c_->stream));
         info.setup_time = now_s() - setup_start;
         const double t0 = now_s();
-        D.lambda_max_A = md->m > 0 ? D.power_iteration(0, 5000, 1e-4, &info.power_iter_A) * p.eig_factor : 0.0;
-        if (S.qkind != QOP_CSR)
-            D.lambda_max_Q = D.power_iteration(1, 5000, 1e-4, &info.power_iter_Q) * p.eig_factor;
-        else if (!S.q_empty)
-            D.lambda_max_Q =
-                S.Q_is_diag ? S.q_max_val : D.power_iteration(1, 5000, 1e-4, &info.power_iter_Q) * p.eig_factor;
-        else
-            D.lambda_max_Q = 0.0;
+        estimate_lambdas(D, md, p.eig_factor, &info.power_iter_A, &info.power_iter_Q);
         info.power_time = now_s() - t0;
         int64_t nfree = 0;
         for (int32_t i = 0; i < md->n; ++i)
