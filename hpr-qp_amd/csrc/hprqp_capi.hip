@@ -808,6 +808,12 @@ int hprqp_sigma_dots(hprqp_ctx *ctx, hprqp_ws *ws, double *out) {
     return guarded(ctx, [&] { hprqp::ws_sigma_dots(ws, out); });
 }
 
+int hprqp_sigma_diag_estimate(hprqp_ctx *ctx, hprqp_ws *ws, double a, double b, double *est) {
+    WS_CHECK(ctx, ws);
+    if (!est) return HPRQP_E_ARG;
+    return guarded(ctx, [&] { *est = hprqp::ws_sigma_diag(ws, a, b); });
+}
+
 int hprqp_update_Q_factors(hprqp_ctx *ctx, hprqp_ws *ws, double sigma) {
     WS_CHECK(ctx, ws);
     return guarded(ctx, [&] { hprqp::ws_q_factors(ws, sigma); });

@@ -303,6 +303,7 @@ void ws_step(hprqp_ws *w, int kind, const hprqp_step_params *sp);
 void ws_kkt(hprqp_ws *w, int64_t iter, hprqp_kkt_out *out);
 double ws_mnorm(hprqp_ws *w, const hprqp_step_params *sp);
 void ws_sigma_dots(hprqp_ws *w, double *out);
+double ws_sigma_diag(hprqp_ws *w, double a, double b);
 void ws_q_factors(hprqp_ws *w, double sigma);
 void ws_restart(hprqp_ws *w);
 void ws_collect(hprqp_ws *w, double *x, double *y, double *z, double *wv);

@@ -539,6 +539,7 @@ struct Solver {
     DevVec<double> last_x, last_y, last_Qw, last_w, last_ATy;
     // FP64 residual temporaries
     DevVec<double> Qx_d, ATdy_d, Ax_d, xd_tmp, qtemp_d, lr_t;
+    DevVec<double> sig_buf;  // the diagonal-Q sigma search: bracket and partials
     // data in VT (aliases of the FP64 arrays when VT == double)
     DevVec<MT> c_own, l_own, u_own, AL_own, AU_own;
     const MT *cV, *lV, *uV, *ALV, *AUV;
@@ -1779,6 +1780,91 @@ static double golden_Q_diag(double a, double b, const std::vector<double> &Q, co
     return (f1 < f2) ? x1 : x2;
 }
 
+// The same minimisation on the device, for large n: f(x) = a x + b/x +
+// x^2 sum_i c_i (d_i / (1 + x Q_i)) with c_i d_i = Q_i c_i^2 >= 0 is convex
+// on x > 0, so its minimiser in [1e-12, 1e12] is where f'(x) = a - b/x^2 +
+// sum_i c_i d_i x (2 + x Q_i) / (1 + x Q_i)^2 changes sign (or an end of the
+// interval).  Each pass evaluates f' at 64 points spaced evenly in log x over
+// the bracket (one point per lane: a wave loads 64 elements at once and
+// walks them by shuffles) and keeps the two points around the first
+// non-negative value.  A pass shrinks the bracket's log width by 63; ten
+// take [1e-12, 1e12] (log width 55) below 1e-13 relative, the golden
+// search's tolerance.  The sign of f' is fixed to rounding where f itself
+// is flat (near the minimum f varies by ~x^2 f''), so the point found moves
+// by about 1e-16 relative under a rounding change of the inputs, where
+// comparisons of f values move by about 1e-8.  br = {lo, hi, best}.
+constexpr int kSigPoints = 64;
+constexpr int kSigPasses = 10;
+// n from which the device search runs (below it the golden search on two
+// host copies of length n costs less than the 20 launches)
+constexpr int32_t kSigmaDeviceMin = 2048;
+
+__device__ __forceinline__ double sig_point(double llo, double lhi, int k) {
+    return exp(llo + (lhi - llo) * ((double)k / (double)(kSigPoints - 1)));
+}
+
+template <class VT>
+__global__ __launch_bounds__(kBlock) void k_sigma_terms(int32_t n, const double *__restrict__ Q,
+                                                        const VT *__restrict__ c, const VT *__restrict__ d,
+                                                        const double *__restrict__ br, double *__restrict__ part) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double x = sig_point(log(br[0]), log(br[1]), lane);
+    const int64_t tiles = ((int64_t)n + 63) / 64, waves = (int64_t)gridDim.x * (kBlock / 64);
+    double acc = 0.0;
+    for (int64_t tile = (int64_t)blockIdx.x * (kBlock / 64) + wave; tile < tiles; tile += waves) {
+        const int64_t i = tile * 64 + lane;
+        const double qi = i < n ? Q[i] : 0.0;
+        const double cd = i < n ? (double)c[i] * (double)d[i] : 0.0;
+        const int cnt = (int)min<int64_t>(64, (int64_t)n - tile * 64);
+        for (int j = 0; j < cnt; ++j) {
+            const double q = __shfl(qi, j), w = __shfl(cd, j);
+            const double r = 1.0 / (1.0 + x * q);
+            acc += w * (x * (2.0 + x * q) * r * r);
+        }
+    }
+    __shared__ double sh[kBlock / 64][kSigPoints];
+    sh[wave][lane] = acc;
+    __syncthreads();
+    if (threadIdx.x < kSigPoints)
+        part[(int64_t)blockIdx.x * kSigPoints + lane] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
+}
+
+// the sums of the nb blocks' partials per point, f' at the 64 points, and
+// the new bracket (one block of 1,024 threads; wave 0 picks)
+__global__ __launch_bounds__(1024) void k_sigma_pick(int nb, const double *__restrict__ part, double a, double b,
+                                                     double *__restrict__ br) {
+    const int k = threadIdx.x & 63, j = threadIdx.x >> 6;
+    double s = 0.0;
+    for (int blk = j; blk < nb; blk += 16) s += part[(int64_t)blk * kSigPoints + k];
+    __shared__ double sh[16][kSigPoints];
+    sh[j][k] = s;
+    __syncthreads();
+    if (threadIdx.x >= 64) return;  // wave 0 only from here
+    double S = 0.0;
+    for (int w = 0; w < 16; ++w) S += sh[w][k];
+    const double llo = log(br[0]), lhi = log(br[1]);
+    const double x = sig_point(llo, lhi, k);
+    const double fp = a - b / (x * x) + S;
+    // the first point where f' >= 0 (a NaN counts as >= 0); 64 if none
+    const unsigned long long up = __ballot(!(fp < 0.0));
+    const int k0 = up ? __ffsll((long long)up) - 1 : kSigPoints;
+    // every lane of the wave has read br before lane 0 writes it
+    if (k == 0) {
+        double nlo, nhi;
+        if (k0 == 0) {
+            nlo = nhi = sig_point(llo, lhi, 0);  // f' >= 0 at the lower end
+        } else if (k0 == kSigPoints) {
+            nlo = nhi = sig_point(llo, lhi, kSigPoints - 1);  // f' < 0 up to the upper end
+        } else {
+            nlo = sig_point(llo, lhi, k0 - 1);
+            nhi = sig_point(llo, lhi, k0);
+        }
+        br[0] = nlo;
+        br[1] = nhi;
+        br[2] = exp(0.5 * (log(nlo) + log(nhi)));
+    }
+}
+
 template <class VT>
 struct SigmaDots {
     const VT *dx, *dQw, *ATdy, *QATdy, *dw;
@@ -1829,6 +1915,43 @@ static void sigma_dots(Solver<VT, MT> &W, double *h) {
     for (int k = 0; k < 6; ++k) h[k] = f[k];
 }
 
+// The sigma estimate of a diagonal Q with constraints: the minimiser of
+// unified_golden_Q_diag's function (src/unified_operations.jl:348-386) over
+// A^T dy and Q A^T dy as sigma_dots left them -- the reference's golden
+// search on host copies below kSigmaDeviceMin, the device's search on the
+// sign of f' from there.
+template <class VT, class MT>
+static double sigma_estimate_diag(Solver<VT, MT> &W, double a, double b) {
+    hipStream_t s = W.s;
+    const int32_t n = W.n;
+    if (n >= kSigmaDeviceMin) {
+        const int64_t tiles = ((int64_t)n + 63) / 64;
+        const int nb = (int)std::min<int64_t>(1024, (tiles + 3) / 4);
+        if (W.sig_buf.n < 3 + (int64_t)nb * kSigPoints) W.sig_buf.alloc(3 + (int64_t)nb * kSigPoints);
+        double *br = W.sig_buf.p, *part = W.sig_buf.p + 3;
+        k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) {
+            br[0] = 1e-12;
+            br[1] = 1e12;
+            br[2] = 0.0;
+        });
+        HPRQP_HIP(hipGetLastError());
+        for (int pass = 0; pass < kSigPasses; ++pass) {
+            k_sigma_terms<VT><<<nb, kBlock, 0, s>>>(n, W.S.diag_Q.p, W.ATdy.p, W.QATdy.p, br, part);
+            k_sigma_pick<<<1, 1024, 0, s>>>(nb, part, a, b, br);
+        }
+        HPRQP_HIP(hipGetLastError());
+        HPRQP_HIP(hipMemcpyAsync(W.red.host, br + 2, sizeof(double), hipMemcpyDeviceToHost, s));
+        HPRQP_HIP(hipStreamSynchronize(s));
+        return W.red.host[0];
+    }
+    std::vector<VT> hc(n), hd(n);
+    HPRQP_HIP(hipMemcpyAsync(hc.data(), W.ATdy.p, sizeof(VT) * n, hipMemcpyDeviceToHost, s));
+    HPRQP_HIP(hipMemcpyAsync(hd.data(), W.QATdy.p, sizeof(VT) * n, hipMemcpyDeviceToHost, s));
+    HPRQP_HIP(hipStreamSynchronize(s));
+    std::vector<double> cc(hc.begin(), hc.end()), dd(hd.begin(), hd.end());
+    return golden_Q_diag(a, b, W.S.diag_Q_host, cc, dd, 1e-12, 1e12, 1e-13, 200);
+}
+
 // update_sigma! (src/algorithm.jl:323-423)
 template <class VT, class MT>
 static void update_sigma(Solver<VT, MT> &W, Restart &r, double err_Rd, double err_Rp, double rel_gap) {
@@ -1855,16 +1978,7 @@ static void update_sigma(Solver<VT, MT> &W, Restart &r, double err_Rd, double er
     b = std::max(b, 1e-12);
     double est;
     if (W.S.Q_is_diag) {
-        if (m > 0) {
-            std::vector<VT> hc(n), hd(n);
-            HPRQP_HIP(hipMemcpyAsync(hc.data(), W.ATdy.p, sizeof(VT) * n, hipMemcpyDeviceToHost, s));
-            HPRQP_HIP(hipMemcpyAsync(hd.data(), W.QATdy.p, sizeof(VT) * n, hipMemcpyDeviceToHost, s));
-            HPRQP_HIP(hipStreamSynchronize(s));
-            std::vector<double> cc(hc.begin(), hc.end()), dd(hd.begin(), hd.end());
-            est = golden_Q_diag(a, b, W.S.diag_Q_host, cc, dd, 1e-12, 1e12, 1e-13, 200);
-        } else {
-            est = std::sqrt(b / a);
-        }
+        est = m > 0 ? sigma_estimate_diag(W, a, b) : std::sqrt(b / a);
     } else {
         est = m > 0 ? golden(a, b, c, d, 1e-12, 1e12, 1e-13, 200) : std::sqrt(b / a);
     }
@@ -3056,6 +3170,7 @@ struct hprqp_ws {
     virtual void kkt(int64_t iter, hprqp_kkt_out *out) = 0;
     virtual double mnorm(const hprqp_step_params *sp) = 0;
     virtual void sigma_dots(double *out) = 0;
+    virtual double sigma_diag(double a, double b) = 0;
     virtual void q_factors(double sigma) = 0;
     virtual void restart() = 0;
     virtual void collect(double *x, double *y, double *z, double *w) = 0;
@@ -3236,6 +3351,12 @@ struct WsImpl final : hprqp_ws {
         hprqp::sigma_dots(*W, out);
     }
 
+    double sigma_diag(double a, double b) override {
+        HPRQP_REQUIRE(S.Q_is_diag && S.m > 0, HPRQP_E_STATE, "Q is not diagonal or there are no constraints");
+        HPRQP_REQUIRE(a > 0 && b > 0, HPRQP_E_ARG, "a and b must be positive");
+        return sigma_estimate_diag(*W, a, b);
+    }
+
     void q_factors(double sigma) override {
         HPRQP_REQUIRE(S.Q_is_diag, HPRQP_E_STATE, "Q is not diagonal");
         HPRQP_REQUIRE(sigma > 0, HPRQP_E_ARG, "sigma must be positive");
@@ -3370,6 +3491,7 @@ void ws_step(hprqp_ws *w, int kind, const hprqp_step_params *sp) { w->step(kind,
 void ws_kkt(hprqp_ws *w, int64_t iter, hprqp_kkt_out *out) { w->kkt(iter, out); }
 double ws_mnorm(hprqp_ws *w, const hprqp_step_params *sp) { return w->mnorm(sp); }
 void ws_sigma_dots(hprqp_ws *w, double *out) { w->sigma_dots(out); }
+double ws_sigma_diag(hprqp_ws *w, double a, double b) { return w->sigma_diag(a, b); }
 void ws_q_factors(hprqp_ws *w, double sigma) { w->q_factors(sigma); }
 void ws_restart(hprqp_ws *w) { w->restart(); }
 void ws_collect(hprqp_ws *w, double *x, double *y, double *z, double *wv) { w->collect(x, y, z, wv); }

@@ -259,6 +259,7 @@ SIGNATURES = [
     ("hprqp_kkt_reduce", C.c_int, [_P, _P, C.c_int64, C.POINTER(hprqp_kkt_out)]),
     ("hprqp_mnorm", C.c_int, [_P, _P, C.POINTER(hprqp_step_params), C.POINTER(C.c_double)]),
     ("hprqp_sigma_dots", C.c_int, [_P, _P, _P]),
+    ("hprqp_sigma_diag_estimate", C.c_int, [_P, _P, C.c_double, C.c_double, C.POINTER(C.c_double)]),
     ("hprqp_update_Q_factors", C.c_int, [_P, _P, C.c_double]),
     ("hprqp_restart", C.c_int, [_P, _P]),
     ("hprqp_ws_collect", C.c_int, [_P, _P, _P, _P, _P, _P]),

@@ -2,7 +2,9 @@
 
 HPRQP.jl's own loop (src/algorithm.jl:1977-2039) keeps the restart tests
 (check_restart, :441-520), the sigma update (update_sigma!, :323-423, with
-its golden-section searches) and the termination tests on the host and
+its golden-section search; a diagonal Q's estimate comes from the library,
+hprqp_sigma_diag_estimate, as hprqp_solve takes it) and the termination
+tests on the host and
 calls the device for each operation of an iteration: the half-steps
 (main_update_gpu!, :1250-1279), the residuals (compute_residuals!, :37-133),
 the M-norm (compute_M_norm!, :689-739), the device half of update_sigma!
@@ -48,31 +50,6 @@ def golden(ap, bp, cp, dp, lo, hi, tol, maxiter):
         if (b - a) < tol:
             break
     return 0.5 * (a + b)
-
-
-def golden_Q_diag(a, b, Q, cv, dv, lo, hi, tol, maxiter):
-    """unified_golden_Q_diag (src/unified_operations.jl:348-386); the sum runs
-    in index order (a cumulative sum), as the sequential loop does."""
-
-    def f(x):
-        acc = np.cumsum(cv * (dv / (1.0 + x * Q)))[-1] if Q.size else 0.0
-        return a * x + b / x + x * x * float(acc)
-
-    x1 = hi - _PHI * (hi - lo)
-    x2 = lo + _PHI * (hi - lo)
-    f1, f2 = f(x1), f(x2)
-    it = 0
-    while abs(hi - lo) > tol * max(1.0, abs(lo)) and it < maxiter:
-        if f1 > f2:
-            lo, x1, f1 = x1, x2, f2
-            x2 = lo + _PHI * (hi - lo)
-            f2 = f(x2)
-        else:
-            hi, x2, f2 = x2, x1, f1
-            x1 = hi - _PHI * (hi - lo)
-            f1 = f(x1)
-        it += 1
-    return x1 if f1 < f2 else x2
 
 
 def print_step(k):
@@ -165,6 +142,14 @@ class Workspace:
         _lib.check(self.ctx.h, self.lib.hprqp_sigma_dots(self.ctx.h, self.h, _lib.ptr(out)))
         return out
 
+    def sigma_diag_estimate(self, a, b):
+        """hprqp_sigma_diag_estimate: the diagonal-Q sigma estimate over the
+        last sigma_dots' A^T dy and Q A^T dy."""
+        out = C.c_double()
+        _lib.check(self.ctx.h, self.lib.hprqp_sigma_diag_estimate(self.ctx.h, self.h, float(a), float(b),
+                                                                  C.byref(out)))
+        return out.value
+
     def update_Q_factors(self, sigma):
         _lib.check(self.ctx.h, self.lib.hprqp_update_Q_factors(self.ctx.h, self.h, sigma))
 
@@ -202,7 +187,7 @@ HALF_STEPS = {
 }
 
 
-def _update_sigma(ws, r, res, sigma, lamA, lamQ, params, diag_Q):
+def _update_sigma(ws, r, res, sigma, lamA, lamQ, params):
     """update_sigma! (src/algorithm.jl:323-423): device dots, then the
     golden-section estimate, the log blend with the best sigma and the
     kappa correction on the host."""
@@ -222,12 +207,9 @@ def _update_sigma(ws, r, res, sigma, lamA, lamQ, params, diag_Q):
             c, d = ATdyQATdy, lamQ
     a, b = max(a, 1e-12), max(b, 1e-12)
     if diag:
-        if m > 0:
-            cc = ws.download(_lib.VEC_ATDY, n)
-            dd = ws.download(_lib.VEC_QATDY, n)
-            est = golden_Q_diag(a, b, diag_Q, cc, dd, 1e-12, 1e12, 1e-13, 200)
-        else:
-            est = math.sqrt(b / a)
+        # the library's estimate, as hprqp_solve takes it (the golden search
+        # on host copies for small n, the device's search from n = 2,048)
+        est = ws.sigma_diag_estimate(a, b) if m > 0 else math.sqrt(b / a)
     else:
         est = golden(a, b, c, d, 1e-12, 1e12, 1e-13, 200) if m > 0 else math.sqrt(b / a)
     fact = math.exp(-0.05 * (r.current_gap / r.best_gap))
@@ -258,7 +240,6 @@ def solve_by_steps(model, params: Optional[HPRQP_parameters] = None, half_steps:
     try:
         info = ws.info
         lamA, lamQ, sigma = info.lambda_max_A, info.lambda_max_Q, info.sigma0
-        diag_Q = ws.download(_lib.VEC_DIAG_Q, info.n) if info.Q_is_diag else None
         r = Restart(sigma)
         check_iter, pf, max_iter = int(params.check_iter), int(params.print_frequency), int(params.max_iter)
         steps = HALF_STEPS[info.path]
@@ -273,7 +254,7 @@ def solve_by_steps(model, params: Optional[HPRQP_parameters] = None, half_steps:
             elif it == max_iter:
                 status = 1
             r.check(it, check_iter, sigma)
-            sigma = _update_sigma(ws, r, res, sigma, lamA, lamQ, params, diag_Q)
+            sigma = _update_sigma(ws, r, res, sigma, lamA, lamQ, params)
             if r.restart_flag > 0:  # do_restart! (src/algorithm.jl:522-555)
                 ws.restart()
                 r.last_gap, r.save_gap = r.current_gap, math.inf

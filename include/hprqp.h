@@ -443,6 +443,15 @@ int hprqp_mnorm(hprqp_ctx *ctx, hprqp_ws *ws, const hprqp_step_params *sp,
  * A^T dy from the anchors, Q dw, Q A^T dy, then out[6] = <dx,dx>,
  * <Q dw, A^T dy>, <Q dw, Q dw>, <dw, Q dw>, <A^T dy, Q A^T dy>, <dy,dy> */
 int hprqp_sigma_dots(hprqp_ctx *ctx, hprqp_ws *ws, double *out);
+/* the sigma estimate of a diagonal Q with constraints (the minimiser in
+ * [1e-12, 1e12] of unified_golden_Q_diag's a x + b/x + x^2 sum_i c_i d_i /
+ * (1 + x Q_i), src/unified_operations.jl:348-386) over A^T dy and Q A^T dy
+ * of the last hprqp_sigma_dots, as hprqp_solve takes it: the golden-section
+ * search on host copies for n < 2,048, from there a search for the sign
+ * change of the derivative on the device (ten passes over 64 points).
+ * a, b > 0; HPRQP_E_STATE when Q is not diagonal or m = 0. */
+int hprqp_sigma_diag_estimate(hprqp_ctx *ctx, hprqp_ws *ws, double a, double b,
+                              double *est);
 /* unified_update_Q_factors! for a new sigma (diagonal Q only) */
 int hprqp_update_Q_factors(hprqp_ctx *ctx, hprqp_ws *ws, double sigma);
 /* the device half of do_restart! (src/algorithm.jl:522-555): anchors and

@@ -413,22 +413,26 @@ def test_noC_warm_start_with_initial_x_leaves_Qw_unset():
     assert part.restarts == rpart.restarts == 0 and dx < 1e-9 and dy < 1e-9
 
 
-@pytest.mark.parametrize("case", ["random", "grid", "lasso", "qap"])
+@pytest.mark.parametrize("case", ["random", "grid", "lasso", "qap", "aug2d"])
 def test_trajectory_matches_oracle_mid_solve(case):
     """Stopped part-way (max_iter = 300, before convergence), the GPU's
     iterates equal the oracle's to rounding: the same steps, restarts and
     sigma updates, not only the same end point.  Summation order differs
     (row products, reductions), so rounding drifts over 300 iterations:
     measured up to 1.3e-9 relative (random, grid) and 2e-14 (LASSO); a wrong
-    term or sign moves the iterates by orders of magnitude more."""
-    from hprqp import HPRQP_parameters, build_from_mat, instances, optimize
+    term or sign moves the iterates by orders of magnitude more.  AUG2D
+    (diagonal Q, n = 20,200) takes its sigma updates from the device's
+    log-grid search, the oracle from the reference's golden-section search
+    on the host: the same minimiser to 1e-13."""
+    from hprqp import HPRQP_parameters, build_from_mat, build_from_mps, instances, optimize
     from oracle import oracle
 
     ptype = {"lasso": "LASSO", "qap": "QAP"}.get(case, "QP")
     model = {"random": lambda: instances.random_sparse_qp(2000, 1000, seed=0),
              "grid": lambda: instances.grid_qp(60, seed=0),
              "lasso": lambda: instances.lasso(200, 4000, mean_nnz=60, seed=1),
-             "qap": lambda: build_from_mat(fixture_path("esc64a.mat"), "QAP", verbose=False)}[case]()
+             "qap": lambda: build_from_mat(fixture_path("esc64a.mat"), "QAP", verbose=False),
+             "aug2d": lambda: build_from_mps(fixture_path("AUG2D.mps"), verbose=False)}[case]()
     # esc64a is within 2% of the tolerance by iteration 300: stop it earlier
     its = 200 if case == "qap" else 300
     res = optimize(model, HPRQP_parameters(verbose=False, max_iter=its, problem_type=ptype))
