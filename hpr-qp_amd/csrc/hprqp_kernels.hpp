@@ -25,10 +25,6 @@ namespace hprqp {
 
 constexpr int kBlock = 256;
 constexpr int kMaxReduceBlocks = 1024;
-// k_pow_* partial pairs one final pass takes: the final block reads them
-// all (1.1 M-row grid QP, 7,810 pairs: 8.1 us, and the fused stream product
-// 6 us slower than the plain one, so that size keeps the separate reduction)
-constexpr int64_t kPowMaxBlocks = 4096;
 
 __device__ __forceinline__ int xcd_block(int b, int nblocks) {
     constexpr int X = 8;
@@ -764,20 +760,6 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(int nb, const double *_
     reduce_final_body<NS, NM>(nb, part, out);
 }
 
-// the same, with the totals handed to post(tot) on thread 0 through LDS:
-// scalar bookkeeping that would otherwise take a launch of its own.  The
-// totals must not make a round trip through global memory inside the
-// kernel: post's loads from a uniform address are scalar loads, which are
-// not ordered after this kernel's vector stores and would read the old
-// value.  post writes wherever the totals are wanted next.
-template <int NS, int NM, class Post>
-__global__ __launch_bounds__(kBlock) void k_reduce_final_post(int nb, const double *__restrict__ part, Post post) {
-    __shared__ double tot[NS + NM > 0 ? NS + NM : 1];
-    reduce_final_body<NS, NM>(nb, part, tot);
-    __syncthreads();
-    if (threadIdx.x == 0) post((const double *)tot);
-}
-
 // a block's sums of s0 and s1 (every thread's) into dst[0], dst[1], in a
 // fixed order; every thread of the block calls it
 __device__ __forceinline__ void pow_block_partials(double s0, double s1, double *dst) {
@@ -800,143 +782,39 @@ __device__ __forceinline__ void pow_block_partials(double s0, double s1, double 
     }
 }
 
-// Power-method row product with its reduction's first pass: y = M (g x) for
-// the rows of M (L lanes per row, the summation of k_csr), g = *gs when SC
-// and 1 otherwise, and each block's partial sums of y_r (g t_r) and y_r^2 in
-// part[2 b], part[2 b + 1] (b = the block's row range, so the partials are
-// in row order).  g x_j is formed before the product, so y is bitwise the
-// product of M with a stored g x.  Every thread reaches the block reduction.
-template <int L, bool SC>
-__device__ __forceinline__ void pow_csr_body(int b, int32_t nrows, const int32_t *__restrict__ rp,
-                                             const int32_t *__restrict__ ci, const double *__restrict__ v,
-                                             const double *__restrict__ x, const double *__restrict__ t, double g,
-                                             double *__restrict__ y, double *__restrict__ part) {
-    const int64_t tid = (int64_t)b * kBlock + threadIdx.x;
-    const int32_t row = (int32_t)(tid / L);
-    const int lane = threadIdx.x & (L - 1);
-    double s0 = 0.0, s1 = 0.0;
-    if (row < nrows) {  // whole L-group in or out together
-        const int32_t s = rp[row], e = rp[row + 1];
-        double acc0 = 0.0, acc1 = 0.0;
-        int32_t k = s + lane;
-        for (; k + L < e; k += 2 * L) {
-            const double xa = SC ? x[ci[k]] * g : x[ci[k]];
-            const double xb = SC ? x[ci[k + L]] * g : x[ci[k + L]];
-            acc0 += v[k] * xa;
-            acc1 += v[k + L] * xb;
-        }
-        if (k < e) acc0 += v[k] * (SC ? x[ci[k]] * g : x[ci[k]]);
-        double acc = acc0 + acc1;
-        if constexpr (L > 1) {
-#pragma unroll
-            for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
-        }
-        if (lane == 0) {
-            y[row] = acc;
-            s0 = acc * (SC ? t[row] * g : t[row]);
-            s1 = acc * acc;
-        }
-    }
-    pow_block_partials(s0, s1, part + 2 * (int64_t)b);
-}
+// ---------------------------------------------------------------------
+// power method on CSR matrices (src/utils.jl:1335-1461)
+// ---------------------------------------------------------------------
+// Every product of the power method is one launch over a grid of at most
+// kPowGrid blocks; a block takes a contiguous range of row groups (L lanes
+// per row: 256 / L rows a group; L = 0: CSR-stream groups of rb rows) and,
+// where the product feeds the stop test, writes one partial pair of
+// (z'.t, |z'|^2).  So a product makes at most kPowGrid pairs, and the next
+// launch totals them itself: every block reads all of them in the same
+// fixed order, so every block holds the same totals, evaluates the previous
+// iteration's stop test and scales its own product by the new 1/|z|.  The
+// scalar update needs no launch of its own and nothing crosses the XCDs
+// inside a launch: block 0 only records what every block computed.
+// The loads of the totals, of res[0] and of the block's first row group
+// are issued before the first barrier, so their latencies overlap; the
+// product is formed unscaled and multiplied by 1/|z| once that is known
+// (y = g (M x) rather than M (g x): a rounding apart).
+//   Q run, one launch per iteration (kPowQ): z_{t+1} = Q (z_t / |z_t|);
+//   A A^T run, two (kPowAT: q = A^T z_t / |z_t| with the stop test of
+//   iteration t-1; kPowA: z_{t+1} = A q with its pairs (z_{t+1}.z_t, ...)).
+constexpr int kPowGrid = 1024;
+static_assert(kPowGrid == 4 * kBlock, "four pairs per thread in the totals");
+enum PowMode { kPowQ = 0, kPowAT = 1, kPowA = 2 };
 
-template <int L, bool SC>
-__global__ __launch_bounds__(kBlock) void k_pow_csr(int32_t nrows, const int32_t *__restrict__ rp,
-                                                    const int32_t *__restrict__ ci,
-                                                    const double *__restrict__ v, const double *__restrict__ x,
-                                                    const double *__restrict__ t, const double *__restrict__ gs,
-                                                    double *__restrict__ y, double *__restrict__ part,
-                                                    const double *__restrict__ stop) {
-    if (*stop != 0.0) return;  // the run has stopped (a whole grid returns)
-    pow_csr_body<L, SC>(xcd_block(blockIdx.x, gridDim.x), nrows, rp, ci, v, x, t, SC ? *gs : 1.0, y, part);
-}
-
-// the same over the CSR-stream shape (rb rows per block, the summation of
-// k_csr_stream): one partial pair per block of rows
-template <bool SC>
-__device__ __forceinline__ void pow_stream_body(int b, int32_t nrows, int rb, const int32_t *__restrict__ rp,
-                                                const int32_t *__restrict__ ci, const double *__restrict__ v,
-                                                const double *__restrict__ x, const double *__restrict__ t, double g,
-                                                double *__restrict__ y, double *__restrict__ part) {
-    __shared__ double prod[kStreamCap];
-    const int32_t r0 = b * rb;
-    const int32_t r1 = min(r0 + rb, nrows);
-    const int tt = threadIdx.x;
-    const int32_t row = r0 + tt;
-    double s0 = 0.0, s1 = 0.0;
-    // r0 < nrows for every block of the grid (nrows / rb rounded up), so
-    // the block is never empty; all its threads reach the barriers
-    const int32_t base = rp[r0];
-    const int32_t cnt = rp[r1] - base;
-    double acc = 0.0;
-    if (cnt <= kStreamCap) {
-        for (int32_t k0 = 0; k0 < cnt; k0 += kStreamU * kBlock) {
-            double vv[kStreamU];
-            int32_t cc[kStreamU];
-#pragma unroll
-            for (int j = 0; j < kStreamU; ++j) {
-                const int32_t k = k0 + j * kBlock + tt;
-                const int32_t at = base + (k < cnt ? k : 0);
-                vv[j] = v[at];
-                cc[j] = ci[at];
-            }
-            double pr[kStreamU];
-#pragma unroll
-            for (int j = 0; j < kStreamU; ++j) pr[j] = vv[j] * (SC ? x[cc[j]] * g : x[cc[j]]);
-#pragma unroll
-            for (int j = 0; j < kStreamU; ++j) {
-                const int32_t k = k0 + j * kBlock + tt;
-                if (k < cnt) prod[k] = pr[j];
-            }
-        }
-        __syncthreads();
-        if (row < r1) {
-            const int32_t s = rp[row] - base, e = rp[row + 1] - base;
-            for (int32_t k = s; k < e; ++k) acc += prod[k];
-        }
-    } else if (row < r1) {
-        for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += v[k] * (SC ? x[ci[k]] * g : x[ci[k]]);
-    }
-    if (row < r1) {
-        y[row] = acc;
-        s0 = acc * (SC ? t[row] * g : t[row]);
-        s1 = acc * acc;
-    }
-    pow_block_partials(s0, s1, part + 2 * (int64_t)b);
-}
-
-template <bool SC>
-__global__ __launch_bounds__(kBlock) void k_pow_stream(int32_t nrows, int rb, const int32_t *__restrict__ rp,
-                                                       const int32_t *__restrict__ ci,
-                                                       const double *__restrict__ v, const double *__restrict__ x,
-                                                       const double *__restrict__ t, const double *__restrict__ gs,
-                                                       double *__restrict__ y, double *__restrict__ part,
-                                                       const double *__restrict__ stop) {
-    if (*stop != 0.0) return;
-    pow_stream_body<SC>(xcd_block(blockIdx.x, gridDim.x), nrows, rb, rp, ci, v, x, t, SC ? *gs : 1.0, y, part);
-}
-
-// Power method on a sparse Q in one launch per iteration.  The launch of
-// iteration t totals iteration t-1's partial pairs -- every block reads all
-// of them in the same fixed order, so every block holds the same totals --
-// evaluates t-1's stop test from them and, unless the run has stopped,
-// stores z_{t+1} = Q (z_t / |z_t|) with its own partials.  So the scalar
-// update needs no launch of its own and nothing has to cross the XCDs inside
-// a launch: block 0 only records what every block computed.  The loads of
-// the totals, of res[0] and of the row product are all issued before the
-// first barrier, so their latencies overlap; the product is formed unscaled
-// and multiplied by 1/|z_t| once that is known (y = g (Q z) rather than
-// Q (g z): a rounding apart).
-//   prev, pnb: iteration t-1's pairs (for t = 0 the pair (0, |z_0|^2));
-//   part: this iteration's pairs; zz: ring of |z_t|^2 (zz[t & 3], written
-//   by block 0 of launch t, read by launch t + 1); res: {stopped, lambda,
-//   iterations, err} of the last test.
+//   prev, pnb: the previous product's pairs (for t = 0 the pair (0, |z_0|^2));
+//   part: this launch's pairs; zz: ring of |z_t|^2 (zz[t & 3], written by
+//   block 0 of the launch that tests iteration t-1); res: {stopped, lambda,
+//   iterations, err} of the last test; scaled: lambda = z'.z / |z| (A A^T,
+//   whose pairs are taken against the unscaled z) rather than z'.q (Q).
 // A block that reads res[0] as set by block 0 of its own launch returns as
 // it would after computing the same test; thread 0 reads res[0] for the
 // whole block, so the block returns as one.
-constexpr int kPowQPairs = 1024;  // most partial pairs of an iteration in this form
-static_assert(kPowQPairs == 4 * kBlock, "four pairs per thread");
-struct PowQArgs {
+struct PowArgs {
     const double *prev;
     int pnb;
     double *part;
@@ -944,17 +822,18 @@ struct PowQArgs {
     double *res;
     int64_t t;
     double tol;
+    int scaled;
 };
 
 // first half: res[0] into *stopped (thread 0), this thread's share of the
 // previous pairs summed over its wave into sh (the caller's barrier follows)
-__device__ __forceinline__ void pow_q_issue(const PowQArgs &a, double *stopped, double (*sh)[2]) {
+__device__ __forceinline__ void pow_issue(const PowArgs &a, double *stopped, double (*sh)[2]) {
     static_assert(kBlock == 256, "the totals sum four waves");
     if (threadIdx.x == 0) *stopped = a.res[0];
     const double2 *p2 = (const double2 *)a.prev;
-    double2 u[kPowQPairs / kBlock];
+    double2 u[kPowGrid / kBlock];
 #pragma unroll
-    for (int k = 0; k < kPowQPairs / kBlock; ++k) {
+    for (int k = 0; k < kPowGrid / kBlock; ++k) {
         const int i = threadIdx.x + k * kBlock;
         u[k] = i < a.pnb ? p2[i] : double2{0.0, 0.0};
     }
@@ -973,7 +852,7 @@ __device__ __forceinline__ void pow_q_issue(const PowQArgs &a, double *stopped, 
 
 // second half, after a barrier: false when the run has stopped (the launch
 // stores nothing); else g = 1/|z_t|
-__device__ __forceinline__ bool pow_q_test(const PowQArgs &a, double stopped, const double (*sh)[2], double &g) {
+__device__ __forceinline__ bool pow_test(const PowArgs &a, double stopped, const double (*sh)[2], double &g) {
     if (stopped != 0.0) return false;
     const double dot = (sh[0][0] + sh[1][0]) + (sh[2][0] + sh[3][0]);
     const double nn = (sh[0][1] + sh[1][1]) + (sh[2][1] + sh[3][1]);
@@ -983,7 +862,7 @@ __device__ __forceinline__ bool pow_q_test(const PowQArgs &a, double stopped, co
         // lambda = q.z', err = |z' - lambda q| / (|z'| + lambda), with
         // |z' - lambda q|^2 = |z'|^2 - lambda^2 (2 - |q|^2)
         const double zz = a.zz[(a.t - 1) & 3], rz = 1.0 / sqrt(zz);
-        const double lam = dot, qn = zz * rz * rz;
+        const double lam = a.scaled ? dot * rz : dot, qn = zz * rz * rz;
         const double e2 = fmax(nn - lam * lam * (2.0 - qn), 0.0);
         const double err = sqrt(e2) / (sqrt(nn) + lam);
         if (lead) {
@@ -999,194 +878,116 @@ __device__ __forceinline__ bool pow_q_test(const PowQArgs &a, double stopped, co
     return true;
 }
 
-template <int L>
-__global__ __launch_bounds__(kBlock) void k_pow_q_csr(int32_t nrows, const int32_t *__restrict__ rp,
-                                                      const int32_t *__restrict__ ci, const double *__restrict__ v,
-                                                      const double *__restrict__ x, double *__restrict__ y,
-                                                      PowQArgs a) {
+// y = g (M x) over the block's row groups, with (y.t', |y|^2) pairs in
+// kPowQ (t' = x g) and kPowA (t' = t, g = 1) modes
+template <int L, int MODE>
+__global__ __launch_bounds__(kBlock) void k_pow(int32_t nrows, int rb, int32_t ngroups,
+                                                const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                const double *__restrict__ v, const double *__restrict__ x,
+                                                const double *__restrict__ t, double *__restrict__ y, PowArgs a) {
+    constexpr bool TEST = MODE != kPowA;
     __shared__ double sh[kBlock / 64][2];
     __shared__ double stopped;
-    pow_q_issue(a, &stopped, sh);
+    __shared__ double prod[L == 0 ? kStreamCap : 1];
+    if constexpr (TEST)
+        pow_issue(a, &stopped, sh);
+    else if (threadIdx.x == 0)
+        stopped = a.res[0];
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t tid = (int64_t)b * kBlock + threadIdx.x;
-    const int32_t row = (int32_t)(tid / L);
-    const int lane = threadIdx.x & (L - 1);
-    double acc = 0.0;
-    if (row < nrows) {  // whole L-group in or out together
-        const int32_t s = rp[row], e = rp[row + 1];
-        double acc0 = 0.0, acc1 = 0.0;
-        int32_t k = s + lane;
-        for (; k + L < e; k += 2 * L) {
-            acc0 += v[k] * x[ci[k]];
-            acc1 += v[k + L] * x[ci[k + L]];
-        }
-        if (k < e) acc0 += v[k] * x[ci[k]];
-        acc = acc0 + acc1;
-        if constexpr (L > 1) {
-#pragma unroll
-            for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
-        }
-    }
-    __syncthreads();
-    double g;
-    if (!pow_q_test(a, stopped, sh, g)) return;
-    double s0 = 0.0, s1 = 0.0;
-    if (row < nrows && lane == 0) {
-        const double yr = acc * g;
-        y[row] = yr;
-        s0 = yr * (x[row] * g);
-        s1 = yr * yr;
-    }
-    pow_block_partials(s0, s1, a.part + 2 * (int64_t)b);
-}
-
-static __global__ __launch_bounds__(kBlock) void k_pow_q_stream(int32_t nrows, int rb, const int32_t *__restrict__ rp,
-                                                                const int32_t *__restrict__ ci,
-                                                                const double *__restrict__ v,
-                                                                const double *__restrict__ x, double *__restrict__ y,
-                                                                PowQArgs a) {
-    __shared__ double sh[kBlock / 64][2];
-    __shared__ double stopped;
-    __shared__ double prod[kStreamCap];
-    pow_q_issue(a, &stopped, sh);
-    const int b = xcd_block(blockIdx.x, gridDim.x);
-    const int32_t r0 = b * rb;
-    const int32_t r1 = min(r0 + rb, nrows);
-    const int tt = threadIdx.x;
-    const int32_t row = r0 + tt;
-    // r0 < nrows for every block of the grid (nrows / rb rounded up), so
-    // the block is never empty; all its threads reach the barriers
-    const int32_t base = rp[r0];
-    const int32_t cnt = rp[r1] - base;
-    double acc = 0.0;
-    if (cnt <= kStreamCap) {
-        for (int32_t k0 = 0; k0 < cnt; k0 += kStreamU * kBlock) {
-            double vv[kStreamU];
-            int32_t cc[kStreamU];
-#pragma unroll
-            for (int j = 0; j < kStreamU; ++j) {
-                const int32_t k = k0 + j * kBlock + tt;
-                const int32_t at = base + (k < cnt ? k : 0);
-                vv[j] = v[at];
-                cc[j] = ci[at];
-            }
-            double pr[kStreamU];
-#pragma unroll
-            for (int j = 0; j < kStreamU; ++j) pr[j] = vv[j] * x[cc[j]];
-#pragma unroll
-            for (int j = 0; j < kStreamU; ++j) {
-                const int32_t k = k0 + j * kBlock + tt;
-                if (k < cnt) prod[k] = pr[j];
-            }
-        }
+    const int32_t per = (ngroups + (int32_t)gridDim.x - 1) / (int32_t)gridDim.x;
+    const int32_t g0 = min(b * per, ngroups), g1 = min(g0 + per, ngroups);
+    double g = 1.0, s0 = 0.0, s1 = 0.0;
+    // the block's barrier before the stop decision (block-uniform)
+    auto settle = [&]() -> bool {
         __syncthreads();
-        if (row < r1) {
-            const int32_t s = rp[row] - base, e = rp[row + 1] - base;
-            for (int32_t k = s; k < e; ++k) acc += prod[k];
+        if constexpr (TEST) return pow_test(a, stopped, sh, g);
+        return stopped == 0.0;
+    };
+    if (g1 == g0 && !settle()) return;
+    for (int32_t grp = g0; grp < g1; ++grp) {
+        double acc = 0.0;
+        int32_t row;
+        bool lead;
+        if constexpr (L > 0) {
+            row = grp * (kBlock / L) + (int32_t)(threadIdx.x / L);
+            const int lane = threadIdx.x & (L - 1);
+            lead = lane == 0 && row < nrows;
+            if (row < nrows) {  // whole L-group in or out together
+                const int32_t s = rp[row], e = rp[row + 1];
+                double acc0 = 0.0, acc1 = 0.0;
+                int32_t k = s + lane;
+                for (; k + L < e; k += 2 * L) {
+                    acc0 += v[k] * x[ci[k]];
+                    acc1 += v[k + L] * x[ci[k + L]];
+                }
+                if (k < e) acc0 += v[k] * x[ci[k]];
+                acc = acc0 + acc1;
+                if constexpr (L > 1) {
+#pragma unroll
+                    for (int off = L / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, L);
+                }
+            }
+        } else {
+            // CSR-stream: grp < ngroups, so the group has a row and every
+            // thread of the block reaches its barriers
+            const int32_t r0 = grp * rb, r1 = min(r0 + rb, nrows);
+            const int tt = threadIdx.x;
+            row = r0 + tt;
+            lead = row < r1;
+            const int32_t base = rp[r0], cnt = rp[r1] - base;
+            if (cnt <= kStreamCap) {
+                for (int32_t k0 = 0; k0 < cnt; k0 += kStreamU * kBlock) {
+                    double vv[kStreamU];
+                    int32_t cc[kStreamU];
+#pragma unroll
+                    for (int j = 0; j < kStreamU; ++j) {
+                        const int32_t k = k0 + j * kBlock + tt;
+                        const int32_t at = base + (k < cnt ? k : 0);
+                        vv[j] = v[at];
+                        cc[j] = ci[at];
+                    }
+                    double pr[kStreamU];
+#pragma unroll
+                    for (int j = 0; j < kStreamU; ++j) pr[j] = vv[j] * x[cc[j]];
+#pragma unroll
+                    for (int j = 0; j < kStreamU; ++j) {
+                        const int32_t k = k0 + j * kBlock + tt;
+                        if (k < cnt) prod[k] = pr[j];
+                    }
+                }
+                __syncthreads();
+                if (lead) {
+                    const int32_t s = rp[row] - base, e = rp[row + 1] - base;
+                    for (int32_t k = s; k < e; ++k) acc += prod[k];
+                }
+            } else if (lead) {
+                for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += v[k] * x[ci[k]];
+            }
         }
-    } else if (row < r1) {
-        for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += v[k] * x[ci[k]];
+        if (grp == g0 && !settle()) return;
+        if (lead) {
+            const double yr = acc * g;
+            y[row] = yr;
+            if constexpr (MODE != kPowAT) {
+                const double tr = MODE == kPowQ ? x[row] * g : t[row];
+                s0 += yr * tr;
+                s1 += yr * yr;
+            }
+        }
+        if constexpr (L == 0) __syncthreads();  // prod is rewritten by the next group
     }
-    __syncthreads();
-    double g;
-    if (!pow_q_test(a, stopped, sh, g)) return;
-    double s0 = 0.0, s1 = 0.0;
-    if (row < r1) {
-        const double yr = acc * g;
-        y[row] = yr;
-        s0 = yr * (x[row] * g);
-        s1 = yr * yr;
-    }
-    pow_block_partials(s0, s1, a.part + 2 * (int64_t)b);
+    if constexpr (MODE != kPowAT) pow_block_partials(s0, s1, a.part + 2 * (int64_t)b);
 }
 
 // the test of the last iteration enqueued (one block), when the run ends
 // at its iteration limit
-static __global__ __launch_bounds__(kBlock) void k_pow_q_tail(PowQArgs a) {
+static __global__ __launch_bounds__(kBlock) void k_pow_tail(PowArgs a) {
     __shared__ double sh[kBlock / 64][2];
     __shared__ double stopped;
-    pow_q_issue(a, &stopped, sh);
+    pow_issue(a, &stopped, sh);
     __syncthreads();
     double g;
-    (void)pow_q_test(a, stopped, sh, g);
-}
-
-// the same over the one-block-per-row shape (the summation of
-// k_csr_rowblock): one partial pair per row
-template <bool SC>
-__global__ __launch_bounds__(kBlock) void k_pow_rowblock(int32_t nrows, const int32_t *__restrict__ rp,
-                                                         const int32_t *__restrict__ ci,
-                                                         const double *__restrict__ v, const double *__restrict__ x,
-                                                         const double *__restrict__ t, const double *__restrict__ gs,
-                                                         double *__restrict__ y, double *__restrict__ part,
-                                                         const double *__restrict__ stop) {
-    __shared__ double pw[kBlock / 64];
-    if (*stop != 0.0) return;
-    const int32_t row = blockIdx.x;
-    const double g = SC ? *gs : 1.0;
-    const int32_t s = rp[row], e = rp[row + 1];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int32_t k = s + threadIdx.x;
-    for (; k + 3 * kBlock < e; k += 4 * kBlock) {
-        a0 += v[k] * (SC ? x[ci[k]] * g : x[ci[k]]);
-        a1 += v[k + kBlock] * (SC ? x[ci[k + kBlock]] * g : x[ci[k + kBlock]]);
-        a2 += v[k + 2 * kBlock] * (SC ? x[ci[k + 2 * kBlock]] * g : x[ci[k + 2 * kBlock]]);
-        a3 += v[k + 3 * kBlock] * (SC ? x[ci[k + 3 * kBlock]] * g : x[ci[k + 3 * kBlock]]);
-    }
-    for (; k < e; k += kBlock) a0 += v[k] * (SC ? x[ci[k]] * g : x[ci[k]]);
-    double acc = (a0 + a1) + (a2 + a3);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if ((threadIdx.x & 63) == 0) pw[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const double tot = (pw[0] + pw[1]) + (pw[2] + pw[3]);
-        y[row] = tot;
-        part[2 * (int64_t)row] = tot * (SC ? t[row] * g : t[row]);
-        part[2 * (int64_t)row + 1] = tot * tot;
-    }
-}
-
-// The power method's final pass: the totals of nb <= kPowMaxBlocks partial
-// pairs, handed to post(tot) on thread 0 through LDS (see
-// k_reduce_final_post).  1,024 threads, each loading up to four pairs at
-// once (16-byte loads), so the pass is one round of loads.  Nothing runs once
-// the run has stopped.
-constexpr int kPowFinalThreads = 1024;
-static_assert(kPowMaxBlocks <= 4 * kPowFinalThreads, "four partial pairs per thread");
-template <class Post>
-__global__ __launch_bounds__(kPowFinalThreads) void k_pow_final(int nb, const double *__restrict__ part,
-                                                                const double *__restrict__ stop, Post post) {
-    if (*stop != 0.0) return;
-    const double2 *p2 = (const double2 *)part;
-    double2 v[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int i = threadIdx.x + u * kPowFinalThreads;
-        v[u] = i < nb ? p2[i] : double2{0.0, 0.0};
-    }
-    double s0 = (v[0].x + v[1].x) + (v[2].x + v[3].x);
-    double s1 = (v[0].y + v[1].y) + (v[2].y + v[3].y);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off);
-        s1 += __shfl_xor(s1, off);
-    }
-    __shared__ double sh[kPowFinalThreads / 64][2];
-    __shared__ double tot[2];
-    if ((threadIdx.x & 63) == 0) {
-        sh[threadIdx.x >> 6][0] = s0;
-        sh[threadIdx.x >> 6][1] = s1;
-    }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        double a = 0.0;
-#pragma unroll
-        for (int w = 0; w < kPowFinalThreads / 64; ++w) a += sh[w][threadIdx.x];
-        tot[threadIdx.x] = a;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) post((const double *)tot);
+    (void)pow_test(a, stopped, sh, g);
 }
 
 // ---------------------------------------------------------------------

@@ -977,48 +977,52 @@ struct Solver {
         }
     };
 
-    // blocks of the power-method product over a in its row-product shape
-    // (one partial pair each), 0 where no fused form exists (column-chunked
-    // shape, no nonzeros) or its partials are too many for one final pass
-    static int64_t pow_blocks(const DevCsr &a) {
-        if (a.nnz == 0 || a.nrow <= 0) return 0;
-        int64_t nb = 0;
-        if (a.lanes < 0)
-            nb = ((int64_t)a.nrow - a.lanes - 1) / (-a.lanes);
-        else if (a.lanes > 64 && a.lanes != kLanesChunked)
-            nb = a.nrow;
-        else if (a.lanes >= 1 && a.lanes <= 64 && (a.lanes & (a.lanes - 1)) == 0)
-            nb = ((int64_t)a.nrow * a.lanes + kBlock - 1) / kBlock;
-        return nb <= kPowMaxBlocks ? nb : 0;
+    // The power method's row product over a: the matrix's own row-product
+    // shape (CSR-stream rows per block, or lanes per row; a block per row and
+    // the column-chunked shape take 64 lanes), a block per row group.  The
+    // fused form needs at most kPowGrid groups per product (so that every
+    // block of the next launch can total the pairs); dealing more groups to
+    // kPowGrid blocks was measured slower than the other form on the
+    // 1414^2 grid (A^T 20.5 -> 28.3 us, Q 31.9 -> 69.6 us per product: the
+    // blocks' groups run one after another with too few blocks in flight)
+    struct PowShape {
+        int L = 64, rb = 0;
+        int32_t ngroups = 0;
+        int grid = 0;
+    };
+    static PowShape pow_shape(const DevCsr &a) {
+        PowShape ps;
+        if (a.lanes < 0) {
+            ps.L = 0;
+            ps.rb = -a.lanes;
+            ps.ngroups = (int32_t)(((int64_t)a.nrow + ps.rb - 1) / ps.rb);
+        } else {
+            const int L = a.lanes;
+            ps.L = L >= 1 && L <= 64 && (L & (L - 1)) == 0 ? L : 64;
+            ps.ngroups = (int32_t)(((int64_t)a.nrow * ps.L + kBlock - 1) / kBlock);
+        }
+        ps.grid = (int)std::max<int64_t>(1, std::min<int64_t>(ps.ngroups, kPowGrid));
+        return ps;
+    }
+    static bool pow_fits(const PowShape &ps) { return ps.ngroups >= 1 && ps.ngroups <= kPowGrid;
     }
 
-    // the power-method product y = a (g x) with its block partials, nb =
-    // pow_blocks(a), on stream ps; nothing once *stop is set
-    template <bool SC>
-    static void pow_csr(hipStream_t ps, const DevCsr &a, int64_t nb, const double *x, const double *t,
-                        const double *gs, double *y, double *part, const double *stop) {
-        const int32_t nr = a.nrow;
+    template <int MODE>
+    static void pow_launch(hipStream_t ps, const DevCsr &a, const PowShape &sh, const double *x, const double *t,
+                           double *y, const PowArgs &pa) {
+        const dim3 grid((unsigned)sh.grid), blk(kBlock);
+        const int32_t nr = a.nrow, ng = sh.ngroups;
         const int32_t *rp = a.rp.p, *ci = a.ci.p;
         const double *v = a.v.p;
-        const dim3 grid((unsigned)nb), blk(kBlock);
-        if (a.lanes < 0) {
-            k_pow_stream<SC><<<grid, blk, 0, ps>>>(nr, -a.lanes, rp, ci, v, x, t, gs, y, part, stop);
-            HPRQP_HIP(hipGetLastError());
-            return;
-        }
-        if (a.lanes > 64) {
-            k_pow_rowblock<SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop);
-            HPRQP_HIP(hipGetLastError());
-            return;
-        }
-        switch (a.lanes) {
-            case 1: k_pow_csr<1, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
-            case 2: k_pow_csr<2, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
-            case 4: k_pow_csr<4, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
-            case 8: k_pow_csr<8, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
-            case 16: k_pow_csr<16, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
-            case 32: k_pow_csr<32, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
-            default: k_pow_csr<64, SC><<<grid, blk, 0, ps>>>(nr, rp, ci, v, x, t, gs, y, part, stop); break;
+        switch (sh.L) {
+            case 0: k_pow<0, MODE><<<grid, blk, 0, ps>>>(nr, sh.rb, ng, rp, ci, v, x, t, y, pa); break;
+            case 1: k_pow<1, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
+            case 2: k_pow<2, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
+            case 4: k_pow<4, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
+            case 8: k_pow<8, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
+            case 16: k_pow<16, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
+            case 32: k_pow<32, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
+            default: k_pow<64, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
         }
         HPRQP_HIP(hipGetLastError());
     }
@@ -1027,41 +1031,36 @@ struct Solver {
     // n).  FP64 throughout.  The reference's iteration
     // (src/utils.jl:1357-1376): q = z / |z|, z' = A A^T q (or Q q), lambda =
     // q.z', err = |z' - lambda q| / (|z'| + lambda), stop at the first err <
-    // tol.  Here, per iteration, the A^T product reads z and scales its rows
-    // by 1/|z| (q is not stored; for Q it is), one reduction gives z'.z (or
-    // z'.q) and |z'|^2, and |z' - lambda q|^2 is expanded as |z'|^2 - 2
-    // lambda q.z' + lambda^2 |q|^2 = |z'|^2 - lambda^2 (2 - |q|^2), with
-    // |q|^2 = |z|^2 / |z|^2 as rounded; |z'|^2 is the next iteration's
-    // |z|^2.  For A A^T and a sparse Q the last product also writes its
-    // block partials of both sums (k_pow_csr / k_pow_stream /
-    // k_pow_rowblock, the summation of the solve's own row-product shape;
-    // for Q it gathers z scaled by 1/|z| on the fly, bitwise the product
-    // with a stored q) and the final pass does the scalar update: three
-    // launches per iteration for A A^T, two for a sparse Q ("fused").  Other
-    // operators take five (product, reduction, scalars).  Iterations are
-    // enqueued in chunks with a copy of the scalars behind each; in the fused
-    // forms the host reads one chunk's copy while the next chunk (8
-    // iterations) runs, and the launches of iterations past the stop store
-    // nothing; the others are read after every chunk of 16.  A sparse Q whose product makes at most kPowQPairs partial
-    // pairs takes one launch per iteration (k_pow_q_csr / k_pow_q_stream:
-    // each launch totals the previous iteration's pairs itself, evaluates its
-    // stop test and scales the gather by the new 1/|z|), its test recorded in
-    // res = {stopped, lambda, iterations, err}.  (Replaying the 16 from a
-    // hipGraph was slower: the random QP's 197 iterations took 4.1 ms with
-    // the capture and instantiation each call needs, 2.8 ms as launches.)
+    // tol; |z' - lambda q|^2 is expanded as |z'|^2 - lambda^2 (2 - |q|^2),
+    // with |q|^2 = |z|^2 / |z|^2 as rounded.  Fused form (A A^T, sparse Q,
+    // at most kPowGrid row groups per product; k_pow): each product totals
+    // the partial pairs of the one before it, so A A^T takes two launches
+    // per iteration (A^T with the stop test, A with its pairs) and Q one,
+    // the test recorded in res = {stopped, lambda, iterations, err}.  The
+    // other form (larger matrices, other operators) takes the solve's own
+    // products (A^T scaled by 1/|z| in its epilogue, then A; or the
+    // operator on q = z/|z|), a two-pass reduction and a scalar update, with
+    // the scalars in d.  Iterations are enqueued in chunks with a copy of the scalars
+    // behind each; in the fused form the host reads one chunk's copy while
+    // the next chunk (8 iterations) runs, and the launches of iterations
+    // past the stop store nothing; the other form is read after every chunk
+    // of 16.  (Replaying the 16 from a hipGraph was slower: the random QP's
+    // 197 iterations took 4.1 ms with the capture and instantiation each call
+    // needs, 2.8 ms as launches; one persistent kernel would need grid-wide
+    // barriers, 7.5 us each at 64 blocks and 25 us at 256, against 2.6 us
+    // for a dependent launch.)
     struct PowerRun {
         int which = 0;
         int64_t len = 0, it = 0, its = 0;
         hipStream_t ps = nullptr;
-        DevVec<double> za, zb, q, ATq, fpart;
-        // one-launch Q form: its partial pairs in two slots (fpart), and
-        // qaux = {0, |z_0|^2 | zz ring of 4 | res of 4}
-        bool oneq = false;
-        DevVec<double> qaux;
-        const DevCsr *fm = nullptr;
-        int64_t fblocks = 0;
-        int slot = 32;                       // device scalars at red.out + slot
-        double *d = nullptr, *host = nullptr;  // device scalars; two pinned copies at host, host + 16
+        DevVec<double> za, zb, q, ATq;
+        bool fused = false;
+        PowShape sp, sp_at;  // Q or A; A^T
+        // fused form: pairs in two slots of kPowGrid, and qaux = {0, |z_0|^2
+        // | zz ring of 4 | res of 4}
+        DevVec<double> fpart, qaux;
+        int slot = 32;                         // other form: scalars at red.out + slot
+        double *d = nullptr, *host = nullptr;  // two pinned copies at host, host + 16
         double *zo = nullptr, *zn = nullptr;
         double lambda = 1.0;
         bool done = false;
@@ -1089,127 +1088,104 @@ struct Solver {
         for (auto &v : z0) v += 1e-8;
         r.za.alloc(r.len);
         r.zb.alloc(r.len);
-        if (which == 1) r.q.alloc(r.len);
         if (which == 0) r.ATq.alloc(n);
         HPRQP_HIP(hipMemcpyAsync(r.za.p, z0.data(), sizeof(double) * r.len, hipMemcpyHostToDevice, s));
-        // device scalars: 0 |z|^2, 1 1/|z|, 2 z'.z (A A^T) or z'.q (Q),
-        // 3 |z'|^2, 4 done, 5 lambda at the stop, 6 stop iteration, 7 error,
-        // 8 iterations run.  Nothing is recorded after the converging
-        // iteration, so the result is the reference's (first iteration with
-        // err < tol) while the host polls rarely.
+        // d: 0 |z|^2, 1 1/|z|, 2 z'.q, 3 |z'|^2, 4 done, 5 lambda at the
+        // stop, 6 stop iteration, 7 error, 8 iterations run (the other form;
+        // the fused form takes |z_0|^2 from d[0])
         double *d = red.out.p + slot;
         r.d = d;
         HPRQP_HIP(hipMemsetAsync(d, 0, sizeof(double) * 9, s));
         red.enqueue<1, 0>(r.len, SumSq{r.za.p}, slot);
         k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { d[1] = 1.0 / sqrt(d[0]); });
         HPRQP_HIP(hipGetLastError());
-        // the fused form: the product whose sums are reduced (A for A A^T,
-        // Q) in a lanes-per-row shape, and few enough blocks for one final
-        // pass over their partials
-        r.fm = which == 0 ? &S.A : (S.qkind == QOP_CSR && !S.q_empty ? &S.Q : nullptr);
-        r.fblocks = r.fm && r.fm->nrow == r.len ? pow_blocks(*r.fm) : 0;
-        r.oneq = which == 1 && r.fblocks > 0 && r.fblocks <= kPowQPairs && r.fm->lanes != kBlock;
-        r.fpart.alloc((r.oneq ? 4 : 2) * r.fblocks);
-        if (r.oneq) {
+        if (which == 0) {
+            r.sp = pow_shape(S.A);
+            r.sp_at = pow_shape(S.AT);
+            r.fused = pow_fits(r.sp) && pow_fits(r.sp_at);
+        } else if (S.qkind == QOP_CSR && !S.q_empty) {
+            r.sp = pow_shape(S.Q);
+            r.fused = pow_fits(r.sp);
+        }
+        if (r.fused) {
+            r.fpart.alloc(4 * kPowGrid);
             r.qaux.alloc(10);
             double *qa = r.qaux.p;
             HPRQP_HIP(hipMemsetAsync(qa, 0, sizeof(double) * 10, s));
             k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { qa[1] = d[0]; });
             HPRQP_HIP(hipGetLastError());
+        } else if (which == 1) {
+            r.q.alloc(r.len);
         }
         r.zo = r.za.p;
         r.zn = r.zb.p;
         for (hipEvent_t &e : r.ev) HPRQP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
 
-    // up to 16 iterations and the scalars' readback, enqueued on r.ps.  Only
-    // the fused form may run on a stream other than s: the others use the
-    // shared reduction scratch and the operator's own temporaries.
+    // up to one chunk of iterations and the scalars' readback, enqueued on
+    // r.ps.  Only the fused form may run on a stream other than s: the other
+    // uses the shared reduction scratch and the operator's own temporaries.
     void power_chunk(PowerRun &r, int64_t max_it, double tol) {
-        double *d = r.d;
-        const double tl = tol;
-        const bool scaled = r.which == 0;
-        const hipStream_t ps = r.ps;
-        // tot = {z'.z (or z'.q), |z'|^2}; d[0], d[1], d[4], d[8] were
-        // written by earlier launches
-        auto post = [=] __device__(const double *tot) {
-            const double zz = d[0], rz = d[1], done = d[4], itn = d[8] + 1.0;
-            const double dot = tot[0], nn = tot[1];
-            const double lam = scaled ? dot * rz : dot;
-            const double qn = zz * rz * rz;
-            const double e2 = fmax(nn - lam * lam * (2.0 - qn), 0.0);
-            if (done == 0.0) {
-                const double err = sqrt(e2) / (sqrt(nn) + lam);
-                d[5] = lam;
-                d[6] = itn;
-                d[7] = err;
-                if (err < tl) d[4] = 1.0;
-            }
-            d[2] = dot;
-            d[3] = nn;
-            d[0] = nn;
-            d[1] = 1.0 / sqrt(nn);
-            d[8] = itn;
-        };
-        // fused runs are read one chunk behind, so a chunk is what runs past
-        // the stop: 8 iterations there, 16 where each chunk is read at once
-        const int64_t kPoll = r.fblocks > 0 ? 8 : 16;
+        const int64_t kPoll = r.fused ? 8 : 16;
         const int64_t chunk = std::min<int64_t>(kPoll, max_it - r.it);
-        if (r.oneq) {
-            const DevCsr &a = *r.fm;
-            const int64_t nb = r.fblocks;
-            double *qa = r.qaux.p;
-            auto args = [&](int64_t t) {
-                return PowQArgs{t == 0 ? (const double *)qa : (const double *)r.fpart.p + 2 * nb * ((t - 1) & 1),
-                                t == 0 ? 1 : (int)nb, r.fpart.p + 2 * nb * (t & 1), qa + 2, qa + 6, t, tol};
+        const hipStream_t ps = r.ps;
+        const double *src;
+        if (r.fused) {
+            double *qa = r.qaux.p, *fp = r.fpart.p;
+            // the pairs of iteration t's product go to slot t & 1; the
+            // product that writes them has prod_grid blocks
+            const int prod_grid = r.sp.grid;
+            auto args = [&](int64_t t, bool pairs_out) {
+                return PowArgs{t == 0 ? (const double *)qa : (const double *)fp + 2 * kPowGrid * ((t - 1) & 1),
+                               t == 0 ? 1 : prod_grid, pairs_out ? fp + 2 * kPowGrid * (t & 1) : nullptr,
+                               qa + 2, qa + 6, t, tol, r.which == 0 ? 1 : 0};
             };
-            const dim3 grid((unsigned)nb), blk(kBlock);
             for (int64_t k = 0; k < chunk; ++k) {
-                const PowQArgs pa = args(r.it + k);
-                const double *zo = r.zo;
-                if (a.lanes < 0) {
-                    k_pow_q_stream<<<grid, blk, 0, ps>>>(a.nrow, -a.lanes, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa);
+                const int64_t t = r.it + k;
+                if (r.which == 1) {
+                    pow_launch<kPowQ>(ps, S.Q, r.sp, r.zo, nullptr, r.zn, args(t, true));
                 } else {
-                    switch (a.lanes) {
-                        case 1: k_pow_q_csr<1><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
-                        case 2: k_pow_q_csr<2><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
-                        case 4: k_pow_q_csr<4><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
-                        case 8: k_pow_q_csr<8><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
-                        case 16: k_pow_q_csr<16><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
-                        case 32: k_pow_q_csr<32><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
-                        default: k_pow_q_csr<64><<<grid, blk, 0, ps>>>(a.nrow, a.rp.p, a.ci.p, a.v.p, zo, r.zn, pa); break;
-                    }
+                    pow_launch<kPowAT>(ps, S.AT, r.sp_at, r.zo, nullptr, r.ATq.p, args(t, false));
+                    pow_launch<kPowA>(ps, S.A, r.sp, r.ATq.p, r.zo, r.zn, args(t, true));
                 }
-                HPRQP_HIP(hipGetLastError());
                 std::swap(r.zo, r.zn);
             }
             r.it += chunk;
             // at the limit, the last iteration's test has no next launch
             if (r.it >= max_it) {
-                k_pow_q_tail<<<1, kBlock, 0, ps>>>(args(r.it));
+                k_pow_tail<<<1, kBlock, 0, ps>>>(args(r.it, false));
                 HPRQP_HIP(hipGetLastError());
             }
-            const int k = (int)(r.launched & 1);
-            HPRQP_HIP(hipMemcpyAsync(r.host + 16 * k, qa + 6, sizeof(double) * 4, hipMemcpyDeviceToHost, ps));
-            HPRQP_HIP(hipEventRecord(r.ev[k], ps));
-            r.it_at[k] = r.it;
-            r.launched++;
-            return;
-        }
-        for (int64_t k = 0; k < chunk; ++k) {
-            const double *zo = r.zo;
-            double *zn = r.zn;
-            if (r.fblocks > 0) {
-                if (r.which == 0) {
-                    csr_apply<double, double>(ps, S.AT, zo, EpiStoreScaled<double>{r.ATq.p, d + 1});
-                    pow_csr<false>(ps, *r.fm, r.fblocks, r.ATq.p, zo, d + 1, zn, r.fpart.p, d + 4);
-                } else {
-                    pow_csr<true>(ps, *r.fm, r.fblocks, zo, zo, d + 1, zn, r.fpart.p, d + 4);
+            src = qa + 6;
+        } else {
+            HPRQP_REQUIRE(ps == s, HPRQP_E_STATE, "the power method's other form off the solve's stream");
+            double *d = r.d;
+            const double tl = tol;
+            const bool scaled = r.which == 0;
+            // tot = {z'.z (A A^T) or z'.q (Q), |z'|^2}; d[0], d[1], d[4], d[8]
+            // were written by earlier launches
+            auto post = [=] __device__(const double *tot) {
+                const double zz = d[0], rz = d[1], done = d[4], itn = d[8] + 1.0;
+                const double dot = tot[0], nn = tot[1];
+                const double lam = scaled ? dot * rz : dot;
+                const double qn = zz * rz * rz;
+                const double e2 = fmax(nn - lam * lam * (2.0 - qn), 0.0);
+                if (done == 0.0) {
+                    const double err = sqrt(e2) / (sqrt(nn) + lam);
+                    d[5] = lam;
+                    d[6] = itn;
+                    d[7] = err;
+                    if (err < tl) d[4] = 1.0;
                 }
-                k_pow_final<<<1, kPowFinalThreads, 0, ps>>>((int)r.fblocks, r.fpart.p, d + 4, post);
-                HPRQP_HIP(hipGetLastError());
-            } else {
-                HPRQP_REQUIRE(ps == s, HPRQP_E_STATE, "unfused power method off the solve's stream");
+                d[2] = dot;
+                d[3] = nn;
+                d[0] = nn;
+                d[1] = 1.0 / sqrt(nn);
+                d[8] = itn;
+            };
+            for (int64_t k = 0; k < chunk; ++k) {
+                const double *zo = r.zo;
+                double *zn = r.zn;
                 if (r.which == 0) {
                     csr_apply<double, double>(s, S.AT, zo, EpiStoreScaled<double>{r.ATq.p, d + 1});
                     csr_apply<double, double>(s, S.A, (const double *)r.ATq.p, EpiStore<double>{zn});
@@ -1222,12 +1198,13 @@ struct Solver {
                 }
                 k_each<<<1, 1, 0, s>>>(1, [=] __device__(int64_t) { post(d + 2); });
                 HPRQP_HIP(hipGetLastError());
+                std::swap(r.zo, r.zn);
             }
-            std::swap(r.zo, r.zn);
+            r.it += chunk;
+            src = d + 4;
         }
-        r.it += chunk;
         const int k = (int)(r.launched & 1);
-        HPRQP_HIP(hipMemcpyAsync(r.host + 16 * k, d, sizeof(double) * 9, hipMemcpyDeviceToHost, ps));
+        HPRQP_HIP(hipMemcpyAsync(r.host + 16 * k, src, sizeof(double) * 4, hipMemcpyDeviceToHost, ps));
         HPRQP_HIP(hipEventRecord(r.ev[k], ps));
         r.it_at[k] = r.it;
         r.launched++;
@@ -1239,13 +1216,11 @@ struct Solver {
         const int k = (int)(r.polled & 1);
         HPRQP_HIP(hipEventSynchronize(r.ev[k]));
         r.polled++;
+        // {stopped, lambda, iterations, err}
         const double *h = r.host + 16 * k;
-        // {stopped, lambda, iterations} at h[0..2] (one-launch Q form) or
-        // h[4..6]
-        const int o = r.oneq ? 0 : 4;
-        r.lambda = h[o + 1];
-        r.its = (int64_t)h[o + 2];
-        if (h[o] != 0.0) r.done = true;
+        r.lambda = h[1];
+        r.its = (int64_t)h[2];
+        if (h[0] != 0.0) r.done = true;
         if (!r.done && r.it_at[k] >= max_it) {
             r.done = true;
             if (p->verbose)
@@ -1253,55 +1228,52 @@ struct Solver {
         }
     }
 
-    // one run to its stop: the next chunk is enqueued before the previous
-    // one's copy is read (at most two chunks unread), so the GPU does not
-    // drain while the host polls; chunks still in flight at the stop
-    // finish (returning at once in the fused form) before the run's buffers go
+    // the next chunk, if fewer than two are unread and the limit is not
+    // reached
     void power_step(PowerRun &r, int64_t max_it, double tol) {
         if (r.launched - r.polled < 2 && r.it < max_it) power_chunk(r, max_it, tol);
     }
 
     // lambda_max(A A^T) and lambda_max(Q) (estimate_eigenvalues,
     // src/algorithm.jl:1407-1440), by one run each.  The two runs are
-    // independent: when the A A^T run has the fused form it
-    // goes on the context's second stream, its 16-iteration chunks
-    // interleaved with the Q run's on s, so the small launches of the two
-    // overlap on the GPU.  Either may be skipped (need_A / need_Q false).
+    // independent: when at least one is in the fused form (which uses no
+    // shared scratch) that one goes on the context's second stream, its
+    // chunks interleaved with the other run's on s, so the small launches of
+    // the two overlap on the GPU.  Either may be skipped (need_A / need_Q
+    // false).
     void power_pair(bool need_A, bool need_Q, int64_t max_it, double tol, double *lam_A, int64_t *its_A,
                     double *lam_Q, int64_t *its_Q) {
         PowerRun ra, rq;
         if (need_A) power_start(ra, 0, 32, red.host);
         if (need_Q) power_start(rq, 1, 48, red.host + 128);
-        const bool both = need_A && need_Q && ra.fblocks > 0 && ctx->stream2;
-        // a run whose form is not fused enqueues nothing past its stop test
-        // that would return at once: it is read after every chunk
-        auto run_to_stop = [&](PowerRun &r) {
-            const bool ahead = r.fblocks > 0;
-            while (!r.done) {
-                power_step(r, max_it, tol);
-                if (ahead) power_step(r, max_it, tol);
-                power_poll(r, max_it);
-            }
-            HPRQP_HIP(hipStreamSynchronize(r.ps));
+        // a run in the other form enqueues nothing past its stop test that
+        // would return at once: it is read after every chunk
+        auto advance = [&](PowerRun &r) {
+            power_step(r, max_it, tol);
+            if (r.fused) power_step(r, max_it, tol);
         };
-        if (both) {
-            ra.ps = ctx->stream2;
+        if (need_A && need_Q && ctx->stream2 && (ra.fused || rq.fused)) {
+            PowerRun &side = ra.fused ? ra : rq;
+            side.ps = ctx->stream2;
             HPRQP_HIP(hipEventRecord(ctx->fork, s));
-            HPRQP_HIP(hipStreamWaitEvent(ra.ps, ctx->fork, 0));
+            HPRQP_HIP(hipStreamWaitEvent(side.ps, ctx->fork, 0));
             while (!ra.done || !rq.done) {
                 for (PowerRun *r : {&ra, &rq})
-                    if (!r->done) {
-                        power_step(*r, max_it, tol);
-                        power_step(*r, max_it, tol);
-                    }
+                    if (!r->done) advance(*r);
                 for (PowerRun *r : {&ra, &rq})
                     if (!r->done) power_poll(*r, max_it);
             }
             HPRQP_HIP(hipStreamSynchronize(ra.ps));
             HPRQP_HIP(hipStreamSynchronize(rq.ps));
         } else {
-            if (need_A) run_to_stop(ra);
-            if (need_Q) run_to_stop(rq);
+            for (PowerRun *r : {&ra, &rq}) {
+                if (!(r == &ra ? need_A : need_Q)) continue;
+                while (!r->done) {
+                    advance(*r);
+                    power_poll(*r, max_it);
+                }
+                HPRQP_HIP(hipStreamSynchronize(r->ps));
+            }
         }
         if (need_A) {
             *lam_A = ra.lambda;
