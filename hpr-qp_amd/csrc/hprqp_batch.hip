@@ -353,9 +353,9 @@ __device__ void bkspace_init(Inst<MT> &I, KSpace &ks, double *sh) {
 // nonzeros (stride L, past the row's end the row's first entry with weight
 // zero), then their eight gathers, so each lane has eight gathers in flight;
 // the row's lanes sum by xor shuffles
-template <int L, class AT_, class VT>
-__device__ void csr_rows_lanes(int32_t rows, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                               const AT_ *__restrict__ v, const VT *__restrict__ x, double *__restrict__ out) {
+template <int L, class AT_, class XF>
+__device__ void csr_rows_lanes_f(int32_t rows, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                 const AT_ *__restrict__ v, const XF &xf, double *__restrict__ out) {
     constexpr int G = kBB / L, U = 8;
     const int lane = (int)threadIdx.x % L, grp = (int)threadIdx.x / L;
     for (int32_t r0 = 0; r0 < rows; r0 += G) {
@@ -373,7 +373,7 @@ __device__ void csr_rows_lanes(int32_t rows, const int32_t *__restrict__ rp, con
                     vv[u] = q < e ? (double)v[at] : 0.0;
                 }
 #pragma unroll
-                for (int u = 0; u < U; ++u) xx[u] = (double)x[cc[u]];
+                for (int u = 0; u < U; ++u) xx[u] = xf(cc[u]);
 #pragma unroll
                 for (int u = 0; u < U; ++u) acc += vv[u] * xx[u];
             }
@@ -385,6 +385,11 @@ __device__ void csr_rows_lanes(int32_t rows, const int32_t *__restrict__ rp, con
         if (lane == 0 && r < rows) out[r] = acc;
     }
     __syncthreads();
+}
+template <int L, class AT_, class VT>
+__device__ void csr_rows_lanes(int32_t rows, const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                               const AT_ *__restrict__ v, const VT *__restrict__ x, double *__restrict__ out) {
+    csr_rows_lanes_f<L>(rows, rp, ci, v, [=](int32_t c) { return (double)x[c]; }, out);
 }
 // the lanes per row from the matrix's mean row length (block-uniform): one
 // lane for rows of a few nonzeros, four up to a few hundred (a row then
@@ -402,11 +407,36 @@ __device__ void csr_rows_auto(int32_t rows, const int32_t *rp, const int32_t *ci
         csr_rows_lanes<16>(rows, rp, ci, v, x, out);
 }
 
+// Q's row products gather from an LDS copy of the vector when it fits
+// (kQxs doubles): Q has ~100+ nonzeros per row, and a block's gathers from
+// L1 bound the product (every 8-byte gather is its own cache-line access)
+constexpr int32_t kQxs = 4096;
+template <class AT_, class VT>
+__device__ void qcsr_rows(int32_t n, const int32_t *rp, const int32_t *ci, const AT_ *v, const VT *x,
+                          double *out) {
+    if (n <= 0) return;
+    if (n > kQxs) {
+        csr_rows_auto(n, rp, ci, v, x, out);
+        return;
+    }
+    __shared__ double xs[kQxs];
+    for (int32_t i = threadIdx.x; i < n; i += kBB) xs[i] = (double)x[i];
+    __syncthreads();
+    const int32_t mean = (rp[n] - rp[0]) / n;
+    auto xf = [&](int32_t c) { return xs[c]; };
+    if (mean <= 6)
+        csr_rows_lanes_f<1>(n, rp, ci, v, xf, out);
+    else if (mean <= 256)
+        csr_rows_lanes_f<4>(n, rp, ci, v, xf, out);
+    else
+        csr_rows_lanes_f<16>(n, rp, ci, v, xf, out);
+}
+
 // out = Q v for a CSR instance (QK = 1), in the iteration's precision (qv)
 // or in FP64 (qv64)
 template <class MT, class AT_, class VT>
 __device__ void qcsr_apply(const Inst<MT> &I, const AT_ *qv, const VT *v, double *out) {
-    csr_rows_auto(I.n, I.P->qrp, I.P->qci, qv, v, out);
+    qcsr_rows(I.n, I.P->qrp, I.P->qci, qv, v, out);
 }
 
 // Q v of either kind: out = Q v; t, sh LDS scratch of the low-rank product
@@ -434,7 +464,7 @@ __device__ void bstep_csr(Inst<MT> &I, double sigma, double lamA, double lamQ, d
     const int32_t n = I.n, m = I.m;
     const double f2s = 1.0 / (1.0 + sigma * lamQ), f1s = 1.0 - f2s;
     // z/x/w1 on Q w
-    csr_rows_auto(n, I.P->qrp, I.P->qci, I.P->qv, I.P->w, I.P->Qw);
+    qcsr_rows(n, I.P->qrp, I.P->qci, I.P->qv, I.P->w, I.P->Qw);
     for (int32_t i = threadIdx.x; i < n; i += kBB) {
         const double qw = I.P->Qw[i];
         const double wi = I.P->w[i], xi = I.P->x[i];
@@ -453,7 +483,7 @@ __device__ void bstep_csr(Inst<MT> &I, double sigma, double lamA, double lamQ, d
     __syncthreads();
     if (m > 0) {
         // tempv = x_hat + sigma (Q w - Q w_bar)
-        csr_rows_auto(n, I.P->qrp, I.P->qci, I.P->qv, I.P->w_bar, I.P->Qw_bar);
+        qcsr_rows(n, I.P->qrp, I.P->qci, I.P->qv, I.P->w_bar, I.P->Qw_bar);
         for (int32_t i = threadIdx.x; i < n; i += kBB)
             I.P->tempv[i] = I.P->x_hat[i] + sigma * (I.P->Qw[i] - I.P->Qw_bar[i]);
         __syncthreads();
