@@ -1004,44 +1004,64 @@ __global__ __launch_bounds__(kBlock) void k_each(int64_t n, F f) {
 // ---------------------------------------------------------------------
 // column statistic over A's columns (rows of the stored transpose) and Q's
 // rows: sqrt(max) (Ruiz) or sqrt(sum) (Pock-Chambolle), 0 -> 1.
+// L lanes per row (L divides 64; the row's lanes stride over its nonzeros
+// and combine by xor shuffles, so a wave reads its rows coalesced), a whole
+// L-group in or out of range together
+template <int L>
+__device__ __forceinline__ double row_stat_lanes(const int32_t *__restrict__ rp, const double *__restrict__ v,
+                                                 int64_t row, int lane, int ruiz) {
+    double s = 0.0;
+    for (int32_t k = rp[row] + lane; k < rp[row + 1]; k += L) s = ruiz ? dmax(s, fabs(v[k])) : s + fabs(v[k]);
+    if constexpr (L > 1) {
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) {
+            const double o = __shfl_xor(s, off, L);
+            s = ruiz ? dmax(s, o) : s + o;
+        }
+    }
+    return s;
+}
+
+// column statistic over A's columns (rows of the stored transpose) and Q's
+// rows: sqrt(max) (Ruiz) or sqrt(sum) (Pock-Chambolle), 0 -> 1.
+template <int L>
 static __global__ __launch_bounds__(kBlock) void k_col_stat(int32_t n, const int32_t *__restrict__ at_rp,
                                                      const double *__restrict__ at_v,
                                                      const int32_t *__restrict__ q_rp,
                                                      const double *__restrict__ q_v, int ruiz,
                                                      double *__restrict__ out) {
-    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t j = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L;
+    const int lane = threadIdx.x & (L - 1);
     if (j >= n) return;
-    double sa = 0.0, sq = 0.0;
-    if (at_rp)
-        for (int32_t k = at_rp[j]; k < at_rp[j + 1]; ++k)
-            sa = ruiz ? dmax(sa, fabs(at_v[k])) : sa + fabs(at_v[k]);
-    if (q_rp)
-        for (int32_t k = q_rp[j]; k < q_rp[j + 1]; ++k)
-            sq = ruiz ? dmax(sq, fabs(q_v[k])) : sq + fabs(q_v[k]);
+    const double sa = at_rp ? row_stat_lanes<L>(at_rp, at_v, j, lane, ruiz) : 0.0;
+    const double sq = q_rp ? row_stat_lanes<L>(q_rp, q_v, j, lane, ruiz) : 0.0;
     const double t = ruiz ? dmax(sq, sa) : sq + sa;
-    out[j] = t > 0.0 ? sqrt(t) : 1.0;
+    if (lane == 0) out[j] = t > 0.0 ? sqrt(t) : 1.0;
 }
 
+template <int L>
 static __global__ __launch_bounds__(kBlock) void k_row_stat(int32_t m, const int32_t *__restrict__ rp,
                                                      const double *__restrict__ v, int ruiz,
                                                      double *__restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L;
+    const int lane = threadIdx.x & (L - 1);
     if (i >= m) return;
-    double s = 0.0;
-    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s = ruiz ? dmax(s, fabs(v[k])) : s + fabs(v[k]);
-    out[i] = s > 0.0 ? sqrt(s) : 1.0;
+    const double s = row_stat_lanes<L>(rp, v, i, lane, ruiz);
+    if (lane == 0) out[i] = s > 0.0 ? sqrt(s) : 1.0;
 }
 
 // v_ij = (v_ij * (1/r_i)) / c_j
+template <int L>
 static __global__ __launch_bounds__(kBlock) void k_scale_csr(int32_t nrow, const int32_t *__restrict__ rp,
                                                       const int32_t *__restrict__ ci,
                                                       double *__restrict__ v,
                                                       const double *__restrict__ r,
                                                       const double *__restrict__ c) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t i = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / L;
+    const int lane = threadIdx.x & (L - 1);
     if (i >= nrow) return;
     const double inv = 1.0 / r[i];
-    for (int32_t k = rp[i]; k < rp[i + 1]; ++k) v[k] = (v[k] * inv) / c[ci[k]];
+    for (int32_t k = rp[i] + lane; k < rp[i + 1]; k += L) v[k] = (v[k] * inv) / c[ci[k]];
 }
 
 }  // namespace hprqp

@@ -361,16 +361,42 @@ static void build_scaled_data(hprqp_ctx *ctx, const hprqp_model *md, const hprqp
         return;
     }
     DevVec<double> tr(m), tc(n);
+    // lanes per row of the scaling kernels from the mean row length
+    auto lanes_of = [](int64_t nnz, int64_t rows) { const int64_t mean = rows > 0 ? nnz / rows : 0;
+                                                    return mean >= 32 ? 16 : mean >= 8 ? 4 : 1; };
+    auto launch_rows = [&](int L, int64_t rows, auto &&f1, auto &&f4, auto &&f16) {
+        const dim3 grid((unsigned)(((int64_t)rows * L + kBlock - 1) / kBlock)), blk(kBlock);
+        if (L == 16)
+            f16(grid, blk);
+        else if (L == 4)
+            f4(grid, blk);
+        else
+            f1(grid, blk);
+        HPRQP_HIP(hipGetLastError());
+    };
+    const int Lq = lanes_of(S.Q.nnz, n), La = lanes_of(S.A.nnz, m), Lat = lanes_of(S.AT.nnz, n);
+    const int Lc = std::max(Lq, Lat);
     const int nruiz = p->use_Ruiz_scaling ? 10 : 0;
     const int total = nruiz + (p->use_Pock_Chambolle_scaling ? 1 : 0);
     for (int pass = 0; pass < total; ++pass) {
         const int ruiz = pass < nruiz;
-        k_col_stat<<<nblk(n), kBlock, 0, s>>>(n, m > 0 ? S.AT.rp.p : nullptr, S.AT.v.p,
-                                               S.Q.nnz > 0 ? S.Q.rp.p : nullptr, S.Q.v.p, ruiz, tc.p);
-        HPRQP_HIP(hipGetLastError());
+        {
+            const int32_t *atrp = m > 0 ? S.AT.rp.p : nullptr, *qrp = S.Q.nnz > 0 ? S.Q.rp.p : nullptr;
+            const double *atv = S.AT.v.p, *qv = S.Q.v.p;
+            double *o = tc.p;
+            launch_rows(
+                Lc, n, [&](dim3 g, dim3 b) { k_col_stat<1><<<g, b, 0, s>>>(n, atrp, atv, qrp, qv, ruiz, o); },
+                [&](dim3 g, dim3 b) { k_col_stat<4><<<g, b, 0, s>>>(n, atrp, atv, qrp, qv, ruiz, o); },
+                [&](dim3 g, dim3 b) { k_col_stat<16><<<g, b, 0, s>>>(n, atrp, atv, qrp, qv, ruiz, o); });
+        }
         if (m > 0) {
-            k_row_stat<<<nblk(m), kBlock, 0, s>>>(m, S.A.rp.p, S.A.v.p, ruiz, tr.p);
-            HPRQP_HIP(hipGetLastError());
+            const int32_t *arp = S.A.rp.p;
+            const double *av = S.A.v.p;
+            double *o = tr.p;
+            launch_rows(
+                La, m, [&](dim3 g, dim3 b) { k_row_stat<1><<<g, b, 0, s>>>(m, arp, av, ruiz, o); },
+                [&](dim3 g, dim3 b) { k_row_stat<4><<<g, b, 0, s>>>(m, arp, av, ruiz, o); },
+                [&](dim3 g, dim3 b) { k_row_stat<16><<<g, b, 0, s>>>(m, arp, av, ruiz, o); });
         }
         {
             double *cn = S.col_norm.p, *c = S.c.p, *l = S.l.p, *u = S.u.p;
@@ -391,14 +417,18 @@ static void build_scaled_data(hprqp_ctx *ctx, const hprqp_model *md, const hprqp
                 au[i] /= t[i];
             });
         }
-        if (S.Q.nnz > 0) {
-            k_scale_csr<<<nblk(n), kBlock, 0, s>>>(n, S.Q.rp.p, S.Q.ci.p, S.Q.v.p, tc.p, tc.p);
-            HPRQP_HIP(hipGetLastError());
-        }
+        auto scale = [&](int L, int32_t rows, const DevCsr &a, const double *r, const double *c) {
+            const int32_t *rp = a.rp.p, *ci = a.ci.p;
+            double *v = a.v.p;
+            launch_rows(
+                L, rows, [&](dim3 g, dim3 b) { k_scale_csr<1><<<g, b, 0, s>>>(rows, rp, ci, v, r, c); },
+                [&](dim3 g, dim3 b) { k_scale_csr<4><<<g, b, 0, s>>>(rows, rp, ci, v, r, c); },
+                [&](dim3 g, dim3 b) { k_scale_csr<16><<<g, b, 0, s>>>(rows, rp, ci, v, r, c); });
+        };
+        if (S.Q.nnz > 0) scale(Lq, n, S.Q, tc.p, tc.p);
         if (m > 0 && S.A.nnz > 0) {
-            k_scale_csr<<<nblk(m), kBlock, 0, s>>>(m, S.A.rp.p, S.A.ci.p, S.A.v.p, tr.p, tc.p);
-            k_scale_csr<<<nblk(n), kBlock, 0, s>>>(n, S.AT.rp.p, S.AT.ci.p, S.AT.v.p, tc.p, tr.p);
-            HPRQP_HIP(hipGetLastError());
+            scale(La, m, S.A, tr.p, tc.p);
+            scale(Lat, n, S.AT, tc.p, tr.p);
         }
     }
     // Q = (Q + Q')/2 (src/utils.jl:772-777) over the symmetric pattern
