@@ -232,6 +232,17 @@ struct hprqp_model {
     std::vector<double> a_dict, q_dict;
     std::vector<uint8_t> a_code, at_code, q_code;
     mutable int tuned_dict[3] = {0, 0, 0};
+    // device copies of the raw data (A, its transpose, Qsym, the vectors
+    // and the value-table codes), made at the first solve; each solve
+    // copies them device to device into its scaled data instead of
+    // uploading the host copies again (and transposing A on the host)
+    struct Raw {
+        bool ready = false;
+        hprqp::DevCsr A, AT, Q;
+        hprqp::DevVec<double> c, l, u, AL, AU, a_dict, q_dict;
+        hprqp::DevVec<uint8_t> a_code, at_code, q_code;
+    };
+    mutable Raw raw;
 };
 
 struct hprqp_batch {

@@ -285,6 +285,52 @@ static void upload_vec(hipStream_t s, DevVec<double> &d, const std::vector<doubl
         HPRQP_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
 }
 
+template <class T>
+static void copy_vec(hipStream_t s, DevVec<T> &d, const DevVec<T> &src) {
+    d.alloc(src.n);
+    if (src.n > 0) HPRQP_HIP(hipMemcpyAsync(d.p, src.p, sizeof(T) * src.n, hipMemcpyDeviceToDevice, s));
+}
+
+// a device-to-device copy of a CSR matrix (pattern, FP64 values, shape)
+static void copy_csr(hipStream_t s, DevCsr &d, const DevCsr &src) {
+    d.nrow = src.nrow;
+    d.ncol = src.ncol;
+    d.nnz = src.nnz;
+    copy_vec(s, d.rp, src.rp);
+    copy_vec(s, d.ci, src.ci);
+    copy_vec(s, d.v, src.v);
+    d.v32.release();
+    d.lanes = src.lanes;
+    d.cseg.release();  // a new pattern: chunk segments are rebuilt on use
+    d.cpart.release();
+    d.nchunk = 0;
+}
+
+// the model's raw data on the device, once per model
+static void ensure_raw(hipStream_t s, const hprqp_model *md) {
+    hprqp_model::Raw &r = md->raw;
+    if (r.ready) return;
+    upload_csr(s, md->A, r.A);
+    upload_csr(s, host_csr_transpose(md->A), r.AT);
+    if (md->Q->kind == QOP_CSR) upload_csr(s, md->Qsym, r.Q);
+    upload_vec(s, r.c, md->c);
+    upload_vec(s, r.l, md->l);
+    upload_vec(s, r.u, md->u);
+    upload_vec(s, r.AL, md->AL);
+    upload_vec(s, r.AU, md->AU);
+    upload_vec(s, r.a_dict, md->a_dict);
+    upload_vec(s, r.q_dict, md->q_dict);
+    auto up8 = [&](DevVec<uint8_t> &d, const std::vector<uint8_t> &h) {
+        d.alloc((int64_t)h.size());
+        if (!h.empty()) HPRQP_HIP(hipMemcpyAsync(d.p, h.data(), h.size(), hipMemcpyHostToDevice, s));
+    };
+    up8(r.a_code, md->a_code);
+    up8(r.at_code, md->at_code);
+    up8(r.q_code, md->q_code);
+    HPRQP_HIP(hipStreamSynchronize(s));
+    r.ready = true;
+}
+
 static void fill(hipStream_t s, double *p, int64_t n, double v) {
     each(s, n, [=] __device__(int64_t i) { p[i] = v; });
 }
@@ -329,21 +375,23 @@ static void build_scaled_data(hprqp_ctx *ctx, const hprqp_model *md, const hprqp
     S.u_uniform = uniform(md->u, S.u_value);
     S.all_equal_rows = m > 0;
     for (int32_t r = 0; r < m && S.all_equal_rows; ++r) S.all_equal_rows = md->AL[r] == md->AU[r];
-    // raw data
-    upload_csr(s, md->A, S.A);
-    upload_csr(s, host_csr_transpose(md->A), S.AT);
+    // raw data, from the model's device copies
+    ensure_raw(s, md);
+    const hprqp_model::Raw &raw = md->raw;
+    copy_csr(s, S.A, raw.A);
+    copy_csr(s, S.AT, raw.AT);
     if (S.qkind == QOP_CSR) {
-        upload_csr(s, md->Qsym, S.Q);
+        copy_csr(s, S.Q, raw.Q);
         S.q_empty = md->Qsym.v.empty();
         S.Q.lanes = md->Q->q.lanes;
     } else {
         S.q_empty = false;
     }
-    upload_vec(s, S.c, md->c);
-    upload_vec(s, S.l, md->l);
-    upload_vec(s, S.u, md->u);
-    upload_vec(s, S.AL, md->AL);
-    upload_vec(s, S.AU, md->AU);
+    copy_vec(s, S.c, raw.c);
+    copy_vec(s, S.l, raw.l);
+    copy_vec(s, S.u, raw.u);
+    copy_vec(s, S.AL, raw.AL);
+    copy_vec(s, S.AU, raw.AU);
     S.row_norm.alloc(m);
     S.col_norm.alloc(n);
     fill(s, S.row_norm.p, m, 1.0);
@@ -464,22 +512,22 @@ static void attach_dicts(hipStream_t s, const hprqp_model *md, Scaled &S) {
         each(s, m, [=] __device__(int64_t i) { ir[i] = 1.0 / rn[i]; });
         each(s, n, [=] __device__(int64_t j) { ic[j] = 1.0 / cn[j]; });
     }
-    auto attach = [&](DevCsr &a, const std::vector<double> &dict, const std::vector<uint8_t> &code,
-                      const double *rsc, const double *csc) {
-        if (dict.empty() || (int64_t)code.size() != a.nnz || a.nnz == 0) return;
-        a.code.alloc(a.nnz);
-        HPRQP_HIP(hipMemcpyAsync(a.code.p, code.data(), a.nnz, hipMemcpyHostToDevice, s));
-        a.dict.alloc((int64_t)dict.size());
-        HPRQP_HIP(hipMemcpyAsync(a.dict.p, dict.data(), sizeof(double) * dict.size(), hipMemcpyHostToDevice, s));
-        a.ndict = (int)dict.size();
+    // the codes and tables from the model's device copies
+    const hprqp_model::Raw &raw = md->raw;
+    auto attach = [&](DevCsr &a, const DevVec<double> &dict, const DevVec<uint8_t> &code, const double *rsc,
+                      const double *csc) {
+        if (dict.n == 0 || code.n != a.nnz || a.nnz == 0) return;
+        copy_vec(s, a.code, code);
+        copy_vec(s, a.dict, dict);
+        a.ndict = (int)dict.n;
         a.rsc = rsc;
         a.csc = csc;
     };
     if (m > 0) {
-        attach(S.A, md->a_dict, md->a_code, S.inv_rn.p, S.inv_cn.p);
-        attach(S.AT, md->a_dict, md->at_code, S.inv_cn.p, S.inv_rn.p);
+        attach(S.A, raw.a_dict, raw.a_code, S.inv_rn.p, S.inv_cn.p);
+        attach(S.AT, raw.a_dict, raw.at_code, S.inv_cn.p, S.inv_rn.p);
     }
-    if (S.qkind == QOP_CSR && !S.q_empty) attach(S.Q, md->q_dict, md->q_code, S.inv_cn.p, S.inv_cn.p);
+    if (S.qkind == QOP_CSR && !S.q_empty) attach(S.Q, raw.q_dict, raw.q_code, S.inv_cn.p, S.inv_cn.p);
     HPRQP_HIP(hipStreamSynchronize(s));
 }
 
