@@ -785,16 +785,15 @@ __device__ __forceinline__ void pow_block_partials(double s0, double s1, double 
 // ---------------------------------------------------------------------
 // power method on CSR matrices (src/utils.jl:1335-1461)
 // ---------------------------------------------------------------------
-// Every product of the power method is one launch over a grid of at most
-// kPowGrid blocks; a block takes a contiguous range of row groups (L lanes
-// per row: 256 / L rows a group; L = 0: CSR-stream groups of rb rows) and,
-// where the product feeds the stop test, writes one partial pair of
-// (z'.t, |z'|^2).  So a product makes at most kPowGrid pairs, and the next
-// launch totals them itself: every block reads all of them in the same
-// fixed order, so every block holds the same totals, evaluates the previous
-// iteration's stop test and scales its own product by the new 1/|z|.  The
-// scalar update needs no launch of its own and nothing crosses the XCDs
-// inside a launch: block 0 only records what every block computed.
+// Every product of the power method is one launch, a block per row group
+// (L lanes per row: 256 / L rows a group; L = 0: CSR-stream groups of rb
+// rows), and where the product feeds the stop test each block writes one
+// partial pair of (z'.t, |z'|^2).  The next launch totals them itself
+// (hence at most kPowGrid of them): every block reads all of them in the
+// same fixed order, so every block holds the same totals, evaluates the
+// previous iteration's stop test and scales its own product by the new
+// 1/|z|.  The scalar update needs no launch of its own and nothing crosses
+// the XCDs inside a launch: block 0 only records what every block computed.
 // The loads of the totals, of res[0] and of the block's first row group
 // are issued before the first barrier, so their latencies overlap; the
 // product is formed unscaled and multiplied by 1/|z| once that is known
@@ -878,17 +877,39 @@ __device__ __forceinline__ bool pow_test(const PowArgs &a, double stopped, const
     return true;
 }
 
+// a matrix as the power method's products read it: CSR (rp, ci, v) in
+// row groups (L lanes per row: 256 / L rows; L = 0: CSR-stream groups of rb
+// rows), or the value-table form (code, dict of ndict entries, row and
+// column factors rsc, csc) in CSR-stream groups
+struct PowMat {
+    int32_t nrows, rb, ngroups;
+    const int32_t *rp, *ci;
+    const double *v;
+    const uint8_t *code;
+    const double *dict;
+    int ndict;
+    const double *rsc, *csc;
+};
+
 // y = g (M x) over the block's row groups, with (y.t', |y|^2) pairs in
 // kPowQ (t' = x g) and kPowA (t' = t, g = 1) modes
-template <int L, int MODE>
-__global__ __launch_bounds__(kBlock) void k_pow(int32_t nrows, int rb, int32_t ngroups,
-                                                const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                const double *__restrict__ v, const double *__restrict__ x,
+template <int L, int MODE, bool DICT>
+__global__ __launch_bounds__(kBlock) void k_pow(PowMat M, const double *__restrict__ x,
                                                 const double *__restrict__ t, double *__restrict__ y, PowArgs a) {
+    static_assert(!DICT || L == 0, "the value-table form runs in CSR-stream groups");
     constexpr bool TEST = MODE != kPowA;
     __shared__ double sh[kBlock / 64][2];
     __shared__ double stopped;
     __shared__ double prod[L == 0 ? kStreamCap : 1];
+    __shared__ double dl[DICT ? 256 : 1];
+    const int32_t nrows = M.nrows, rb = M.rb, ngroups = M.ngroups;
+    const int32_t *__restrict__ rp = M.rp;
+    const int32_t *__restrict__ ci = M.ci;
+    const double *__restrict__ v = M.v;
+    if constexpr (DICT) {
+        if ((int)threadIdx.x < M.ndict) dl[threadIdx.x] = M.dict[threadIdx.x];
+        __syncthreads();
+    }
     if constexpr (TEST)
         pow_issue(a, &stopped, sh);
     else if (threadIdx.x == 0)
@@ -943,12 +964,20 @@ __global__ __launch_bounds__(kBlock) void k_pow(int32_t nrows, int rb, int32_t n
                     for (int j = 0; j < kStreamU; ++j) {
                         const int32_t k = k0 + j * kBlock + tt;
                         const int32_t at = base + (k < cnt ? k : 0);
-                        vv[j] = v[at];
                         cc[j] = ci[at];
+                        if constexpr (DICT)
+                            vv[j] = dl[M.code[at]];
+                        else
+                            vv[j] = v[at];
                     }
                     double pr[kStreamU];
 #pragma unroll
-                    for (int j = 0; j < kStreamU; ++j) pr[j] = vv[j] * x[cc[j]];
+                    for (int j = 0; j < kStreamU; ++j) {
+                        if constexpr (DICT)
+                            pr[j] = (vv[j] * M.csc[cc[j]]) * x[cc[j]];
+                        else
+                            pr[j] = vv[j] * x[cc[j]];
+                    }
 #pragma unroll
                     for (int j = 0; j < kStreamU; ++j) {
                         const int32_t k = k0 + j * kBlock + tt;
@@ -961,8 +990,15 @@ __global__ __launch_bounds__(kBlock) void k_pow(int32_t nrows, int rb, int32_t n
                     for (int32_t k = s; k < e; ++k) acc += prod[k];
                 }
             } else if (lead) {
-                for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += v[k] * x[ci[k]];
+                for (int32_t k = rp[row]; k < rp[row + 1]; ++k) {
+                    if constexpr (DICT)
+                        acc += (dl[M.code[k]] * M.csc[ci[k]]) * x[ci[k]];
+                    else
+                        acc += v[k] * x[ci[k]];
+                }
             }
+            if constexpr (DICT)
+                if (lead) acc *= M.rsc[row];
         }
         if (grp == g0 && !settle()) return;
         if (lead) {

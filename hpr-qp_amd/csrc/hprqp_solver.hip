@@ -1057,20 +1057,21 @@ struct Solver {
 
     // The power method's row product over a: the matrix's own row-product
     // shape (CSR-stream rows per block, or lanes per row; a block per row and
-    // the column-chunked shape take 64 lanes), a block per row group.  The
-    // fused form needs at most kPowGrid groups per product (so that every
-    // block of the next launch can total the pairs); dealing more groups to
-    // kPowGrid blocks was measured slower than the other form on the
-    // 1414^2 grid (A^T 20.5 -> 28.3 us, Q 31.9 -> 69.6 us per product: the
-    // blocks' groups run one after another with too few blocks in flight)
+    // the column-chunked shape take 64 lanes; the FP64 value-table form where
+    // the autotuner chose it), a block per row group
     struct PowShape {
         int L = 64, rb = 0;
+        bool dict = false;
         int32_t ngroups = 0;
-        int grid = 0;
     };
     static PowShape pow_shape(const DevCsr &a) {
         PowShape ps;
-        if (a.lanes < 0) {
+        if (a.dict_rb > 0 && a.nnz > 0) {
+            ps.L = 0;
+            ps.rb = a.dict_rb;
+            ps.dict = true;
+            ps.ngroups = (int32_t)(((int64_t)a.nrow + ps.rb - 1) / ps.rb);
+        } else if (a.lanes < 0) {
             ps.L = 0;
             ps.rb = -a.lanes;
             ps.ngroups = (int32_t)(((int64_t)a.nrow + ps.rb - 1) / ps.rb);
@@ -1079,28 +1080,40 @@ struct Solver {
             ps.L = L >= 1 && L <= 64 && (L & (L - 1)) == 0 ? L : 64;
             ps.ngroups = (int32_t)(((int64_t)a.nrow * ps.L + kBlock - 1) / kBlock);
         }
-        ps.grid = (int)std::max<int64_t>(1, std::min<int64_t>(ps.ngroups, kPowGrid));
+        ps.ngroups = std::max<int32_t>(1, ps.ngroups);
         return ps;
     }
-    static bool pow_fits(const PowShape &ps) { return ps.ngroups >= 1 && ps.ngroups <= kPowGrid;
+    // the fused form: every block of the next launch totals the product's
+    // pairs, so they must be few (<= kPowGrid, and pairs x next grid <= 2^20).
+    // Larger matrices take the solve's own products: on the 1414^2 grid the
+    // fused launches were slower than the plain row products by 8-10 us each
+    // (A^T 20.5 -> 30.2 us, A 20.1 -> 28.0 us, Q 31.9 -> 57.4 us; power
+    // method 0.193 -> 0.219 s with the pairs summed by one block first), and
+    // dealing the row groups to 1,024 blocks was slower still.
+    static bool pow_fits(const PowShape &prod, const PowShape &next) {
+        return prod.ngroups <= kPowGrid && (int64_t)prod.ngroups * next.ngroups <= (int64_t)1 << 20;
     }
 
     template <int MODE>
     static void pow_launch(hipStream_t ps, const DevCsr &a, const PowShape &sh, const double *x, const double *t,
                            double *y, const PowArgs &pa) {
-        const dim3 grid((unsigned)sh.grid), blk(kBlock);
-        const int32_t nr = a.nrow, ng = sh.ngroups;
-        const int32_t *rp = a.rp.p, *ci = a.ci.p;
-        const double *v = a.v.p;
+        const dim3 grid((unsigned)sh.ngroups), blk(kBlock);
+        const PowMat M{a.nrow,  sh.rb,          sh.ngroups, a.rp.p, a.ci.p, a.v.p, a.code.p, a.dict.p,
+                       a.ndict, a.rsc,          a.csc};
+        if (sh.dict) {
+            k_pow<0, MODE, true><<<grid, blk, 0, ps>>>(M, x, t, y, pa);
+            HPRQP_HIP(hipGetLastError());
+            return;
+        }
         switch (sh.L) {
-            case 0: k_pow<0, MODE><<<grid, blk, 0, ps>>>(nr, sh.rb, ng, rp, ci, v, x, t, y, pa); break;
-            case 1: k_pow<1, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
-            case 2: k_pow<2, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
-            case 4: k_pow<4, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
-            case 8: k_pow<8, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
-            case 16: k_pow<16, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
-            case 32: k_pow<32, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
-            default: k_pow<64, MODE><<<grid, blk, 0, ps>>>(nr, 0, ng, rp, ci, v, x, t, y, pa); break;
+            case 0: k_pow<0, MODE, false><<<grid, blk, 0, ps>>>(M, x, t, y, pa); break;
+            case 1: k_pow<1, MODE, false><<<grid, blk, 0, ps>>>(M, x, t, y, pa); break;
+            case 2: k_pow<2, MODE, false><<<grid, blk, 0, ps>>>(M, x, t, y, pa); break;
+            case 4: k_pow<4, MODE, false><<<grid, blk, 0, ps>>>(M, x, t, y, pa); break;
+            case 8: k_pow<8, MODE, false><<<grid, blk, 0, ps>>>(M, x, t, y, pa); break;
+            case 16: k_pow<16, MODE, false><<<grid, blk, 0, ps>>>(M, x, t, y, pa); break;
+            case 32: k_pow<32, MODE, false><<<grid, blk, 0, ps>>>(M, x, t, y, pa); break;
+            default: k_pow<64, MODE, false><<<grid, blk, 0, ps>>>(M, x, t, y, pa); break;
         }
         HPRQP_HIP(hipGetLastError());
     }
@@ -1110,33 +1123,34 @@ struct Solver {
     // (src/utils.jl:1357-1376): q = z / |z|, z' = A A^T q (or Q q), lambda =
     // q.z', err = |z' - lambda q| / (|z'| + lambda), stop at the first err <
     // tol; |z' - lambda q|^2 is expanded as |z'|^2 - lambda^2 (2 - |q|^2),
-    // with |q|^2 = |z|^2 / |z|^2 as rounded.  Fused form (A A^T, sparse Q,
-    // at most kPowGrid row groups per product; k_pow): each product totals
-    // the partial pairs of the one before it, so A A^T takes two launches
-    // per iteration (A^T with the stop test, A with its pairs) and Q one,
-    // the test recorded in res = {stopped, lambda, iterations, err}.  The
-    // other form (larger matrices, other operators) takes the solve's own
-    // products (A^T scaled by 1/|z| in its epilogue, then A; or the
-    // operator on q = z/|z|), a two-pass reduction and a scalar update, with
-    // the scalars in d.  Iterations are enqueued in chunks with a copy of the scalars
-    // behind each; in the fused form the host reads one chunk's copy while
-    // the next chunk (8 iterations) runs, and the launches of iterations
-    // past the stop store nothing; the other form is read after every chunk
-    // of 16.  (Replaying the 16 from a hipGraph was slower: the random QP's
-    // 197 iterations took 4.1 ms with the capture and instantiation each call
+    // with |q|^2 = |z|^2 / |z|^2 as rounded.  Fused form (A A^T and a sparse
+    // Q whose products make few pairs, pow_fits; k_pow): each product totals
+    // the partial pairs of the one before it, so A A^T takes two launches per
+    // iteration (A^T with the stop test, A with its pairs) and Q one, the
+    // test recorded in res = {stopped, lambda, iterations, err}.  The other
+    // form (larger matrices, other operators) takes the solve's own products
+    // (A^T scaled by 1/|z| in its epilogue, then A; or the operator on q =
+    // z/|z|), a two-pass reduction and a scalar update, with the scalars in
+    // d.  Iterations are enqueued in chunks with a copy of the scalars behind
+    // each; in the fused form the host reads one chunk's copy while the next
+    // chunk (8 iterations) runs, and the launches of iterations past the stop
+    // store nothing; the other form is read after every chunk of 16.
+    // (Replaying the 16 from a hipGraph was slower: the random QP's 197
+    // iterations took 4.1 ms with the capture and instantiation each call
     // needs, 2.8 ms as launches; one persistent kernel would need grid-wide
-    // barriers, 7.5 us each at 64 blocks and 25 us at 256, against 2.6 us
-    // for a dependent launch.)
+    // barriers, 7.5 us each at 64 blocks and 25 us at 256, against 2.6 us for
+    // a dependent launch.)
     struct PowerRun {
         int which = 0;
         int64_t len = 0, it = 0, its = 0;
         hipStream_t ps = nullptr;
         DevVec<double> za, zb, q, ATq;
         bool fused = false;
-        PowShape sp, sp_at;  // Q or A; A^T
-        // fused form: pairs in two slots of kPowGrid, and qaux = {0, |z_0|^2
-        // | zz ring of 4 | res of 4}
+        PowShape sp, sp_at;  // Q or A (the product with pairs); A^T
+        // fused form: pairs in two slots of slot_pairs, and qaux = {0,
+        // |z_0|^2 | zz ring of 4 | res of 4}
         DevVec<double> fpart, qaux;
+        int64_t slot_pairs = 0;
         int slot = 32;                         // other form: scalars at red.out + slot
         double *d = nullptr, *host = nullptr;  // two pinned copies at host, host + 16
         double *zo = nullptr, *zn = nullptr;
@@ -1180,13 +1194,14 @@ struct Solver {
         if (which == 0) {
             r.sp = pow_shape(S.A);
             r.sp_at = pow_shape(S.AT);
-            r.fused = pow_fits(r.sp) && pow_fits(r.sp_at);
+            r.fused = pow_fits(r.sp, r.sp_at);
         } else if (S.qkind == QOP_CSR && !S.q_empty) {
             r.sp = pow_shape(S.Q);
-            r.fused = pow_fits(r.sp);
+            r.fused = pow_fits(r.sp, r.sp);
         }
         if (r.fused) {
-            r.fpart.alloc(4 * kPowGrid);
+            r.slot_pairs = r.sp.ngroups;
+            r.fpart.alloc(4 * r.slot_pairs);
             r.qaux.alloc(10);
             double *qa = r.qaux.p;
             HPRQP_HIP(hipMemsetAsync(qa, 0, sizeof(double) * 10, s));
@@ -1210,13 +1225,13 @@ struct Solver {
         const double *src;
         if (r.fused) {
             double *qa = r.qaux.p, *fp = r.fpart.p;
-            // the pairs of iteration t's product go to slot t & 1; the
-            // product that writes them has prod_grid blocks
-            const int prod_grid = r.sp.grid;
+            const int64_t sp2 = 2 * r.slot_pairs;
+            // iteration t's product writes its pairs to slot t & 1
             auto args = [&](int64_t t, bool pairs_out) {
-                return PowArgs{t == 0 ? (const double *)qa : (const double *)fp + 2 * kPowGrid * ((t - 1) & 1),
-                               t == 0 ? 1 : prod_grid, pairs_out ? fp + 2 * kPowGrid * (t & 1) : nullptr,
-                               qa + 2, qa + 6, t, tol, r.which == 0 ? 1 : 0};
+                const double *prev = t == 0 ? (const double *)qa : (const double *)fp + sp2 * ((t - 1) & 1);
+                const int pnb = t == 0 ? 1 : r.sp.ngroups;
+                return PowArgs{prev, pnb, pairs_out ? fp + sp2 * (t & 1) : nullptr, qa + 2, qa + 6, t, tol,
+                               r.which == 0 ? 1 : 0};
             };
             for (int64_t k = 0; k < chunk; ++k) {
                 const int64_t t = r.it + k;
