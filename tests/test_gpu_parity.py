@@ -382,7 +382,7 @@ def test_warm_start_matches_oracle(start):
 
 
 def test_noC_warm_start_with_initial_x_leaves_Qw_unset():
-    """The noC path with initial_x (SURVEY hazard B.6): process_initial_points!
+    """The noC path with initial_x: process_initial_points!
     sets x and w from initial_x but not Qw / last_Qw (src/algorithm.jl:1797-1802),
     which the noC step carries by its Halpern recurrence from the last restart
     (:544-547), so the run starts from Qw = 0 until its first restart.  The GPU

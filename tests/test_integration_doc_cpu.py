@@ -1,4 +1,4 @@
-"""The Julia ccall binding printed in INTEGRATION.md, checked as text against
+"""The Julia ccall binding printed in the integration guide, checked as text against
 include/hprqp.h (Julia is not installed here, so the binding is never run):
 
 - CParams, CResult, SaveState, WsInfo, StepParams and KKT list the fields
