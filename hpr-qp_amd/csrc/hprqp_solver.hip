@@ -618,6 +618,16 @@ struct Solver {
     // FP64 residual temporaries
     DevVec<double> Qx_d, ATdy_d, Ax_d, xd_tmp, qtemp_d, lr_t;
     DevVec<double> sig_buf;  // the diagonal-Q sigma search: bracket and partials
+    // FP32 iterates: Q dw and Q A^T dy of the sigma update and the M-norm in
+    // FP64 (<dw, Q dw> cancels below FP32 resolution near convergence when
+    // Q has a large null space, as LASSO's A^T A does)
+    DevVec<double> sg_Qw, sg_QATdy;
+    void ensure_sig64() {
+        if (sg_Qw.n < n) {
+            sg_Qw.alloc(n);
+            sg_QATdy.alloc(n);
+        }
+    }
     // data in VT (aliases of the FP64 arrays when VT == double)
     DevVec<MT> c_own, l_own, u_own, AL_own, AU_own;
     const MT *cV, *lV, *uV, *ALV, *AUV;
@@ -1672,8 +1682,11 @@ struct Solver {
     }
 
     // ---------------- M-norm (src/algorithm.jl:689-739) --------------
+    template <class P>
     struct MnN {
-        const VT *dx, *dQw, *ATdy, *QATdy, *dw, *fM;
+        const VT *dx;
+        const P *dQw, *ATdy, *QATdy;
+        const VT *dw, *fM;
         int diag;
         __device__ void operator()(int64_t i, double *s, double *) const {
             const double a = (double)dx[i], b = (double)dQw[i], c = ATdy ? (double)ATdy[i] : 0.0,
@@ -1697,15 +1710,28 @@ struct Solver {
 
     double compute_M_norm() {
         flush();
-        q_apply(dw.p, dQw.p);
-        if (m > 0) {
-            csr_apply<VT, MT>(s, S.AT, (const VT *)dy.p, EpiStore<VT>{ATdy.p});
-            q_apply(ATdy.p, QATdy.p);
+        if constexpr (std::is_same<VT, double>::value) {
+            q_apply(dw.p, dQw.p);
+            if (m > 0) {
+                csr_apply<VT, MT>(s, S.AT, (const VT *)dy.p, EpiStore<VT>{ATdy.p});
+                q_apply(ATdy.p, QATdy.p);
+            }
+            red.enqueue<7, 0>(n,
+                              MnN<VT>{dx.p, dQw.p, m > 0 ? ATdy.p : nullptr, m > 0 ? QATdy.p : nullptr, dw.p,
+                                      fact_M.p, S.Q_is_diag},
+                              0);
+        } else {
+            ensure_sig64();
+            q_apply_d(dw.p, sg_Qw.p);
+            if (m > 0) {
+                csr_apply<double, double>(s, S.AT, (const VT *)dy.p, EpiStore<double>{ATdy_d.p});
+                qop_fp64(ATdy_d.p, sg_QATdy.p);
+            }
+            red.enqueue<7, 0>(n,
+                              MnN<double>{dx.p, sg_Qw.p, m > 0 ? ATdy_d.p : nullptr,
+                                          m > 0 ? sg_QATdy.p : nullptr, dw.p, fact_M.p, S.Q_is_diag},
+                              0);
         }
-        red.enqueue<7, 0>(n,
-                          MnN{dx.p, dQw.p, m > 0 ? ATdy.p : nullptr, m > 0 ? QATdy.p : nullptr, dw.p, fact_M.p,
-                              S.Q_is_diag},
-                          0);
         red.enqueue<1, 0>(m, SumSqV{dy.p}, 7);
         const double *h = red.fetch(8);
         double M1 = 0.0, M3 = 0.0;
@@ -1868,9 +1894,9 @@ __device__ __forceinline__ double sig_point(double llo, double lhi, int k) {
     return exp(llo + (lhi - llo) * ((double)k / (double)(kSigPoints - 1)));
 }
 
-template <class VT>
+template <class VT, class P>
 __global__ __launch_bounds__(kBlock) void k_sigma_terms(int32_t n, const double *__restrict__ Q,
-                                                        const VT *__restrict__ c, const VT *__restrict__ d,
+                                                        const VT *__restrict__ c, const P *__restrict__ d,
                                                         const double *__restrict__ br, double *__restrict__ part) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const double x = sig_point(log(br[0]), log(br[1]), lane);
@@ -1930,9 +1956,13 @@ __global__ __launch_bounds__(1024) void k_sigma_pick(int nb, const double *__res
     }
 }
 
-template <class VT>
+template <class VT, class P>
 struct SigmaDots {
-    const VT *dx, *dQw, *ATdy, *QATdy, *dw;
+    const VT *dx;
+    const P *dQw;
+    const VT *ATdy;
+    const P *QATdy;
+    const VT *dw;
     __device__ void operator()(int64_t i, double *s, double *) const {
         const double a = (double)dx[i], b = (double)dQw[i], c = ATdy ? (double)ATdy[i] : 0.0,
                      d = QATdy ? (double)QATdy[i] : 0.0, e = (double)dw[i];
@@ -1962,19 +1992,29 @@ static void sigma_dots(Solver<VT, MT> &W, double *h) {
             dw[i] = (VT)((double)wb[i] - lw[i]);
         });
     }
-    W.q_apply(W.dw.p, W.dQw.p);
     if (m > 0) {
         VT *dy = W.dy.p, *atdy = W.ATdy.p;
         const VT *yb = W.y_bar.p, *atyb = W.ATy_bar.p;
         const double *ly = W.last_y.p, *laty = W.last_ATy.p;
         each(s, m, [=] __device__(int64_t i) { dy[i] = (VT)((double)yb[i] - ly[i]); });
         each(s, n, [=] __device__(int64_t i) { atdy[i] = (VT)((double)atyb[i] - laty[i]); });
-        W.q_apply(W.ATdy.p, W.QATdy.p);
     }
-    W.red.template enqueue<5, 0>(n,
-                                 SigmaDots<VT>{W.dx.p, W.dQw.p, m > 0 ? W.ATdy.p : nullptr,
-                                               m > 0 ? W.QATdy.p : nullptr, W.dw.p},
-                                 0);
+    if constexpr (std::is_same<VT, double>::value) {
+        W.q_apply(W.dw.p, W.dQw.p);
+        if (m > 0) W.q_apply(W.ATdy.p, W.QATdy.p);
+        W.red.template enqueue<5, 0>(n,
+                                     SigmaDots<VT, VT>{W.dx.p, W.dQw.p, m > 0 ? W.ATdy.p : nullptr,
+                                                       m > 0 ? W.QATdy.p : nullptr, W.dw.p},
+                                     0);
+    } else {
+        W.ensure_sig64();
+        W.q_apply_d(W.dw.p, W.sg_Qw.p);
+        if (m > 0) W.q_apply_d(W.ATdy.p, W.sg_QATdy.p);
+        W.red.template enqueue<5, 0>(n,
+                                     SigmaDots<VT, double>{W.dx.p, W.sg_Qw.p, m > 0 ? W.ATdy.p : nullptr,
+                                                           m > 0 ? W.sg_QATdy.p : nullptr, W.dw.p},
+                                     0);
+    }
     W.red.template enqueue<1, 0>(m, typename Solver<VT, MT>::SumSqV{W.dy.p}, 5);
     const double *f = W.red.fetch(6);
     for (int k = 0; k < 6; ++k) h[k] = f[k];
@@ -2001,7 +2041,10 @@ static double sigma_estimate_diag(Solver<VT, MT> &W, double a, double b) {
         });
         HPRQP_HIP(hipGetLastError());
         for (int pass = 0; pass < kSigPasses; ++pass) {
-            k_sigma_terms<VT><<<nb, kBlock, 0, s>>>(n, W.S.diag_Q.p, W.ATdy.p, W.QATdy.p, br, part);
+            if constexpr (std::is_same<VT, double>::value)
+                k_sigma_terms<VT, VT><<<nb, kBlock, 0, s>>>(n, W.S.diag_Q.p, W.ATdy.p, W.QATdy.p, br, part);
+            else
+                k_sigma_terms<VT, double><<<nb, kBlock, 0, s>>>(n, W.S.diag_Q.p, W.ATdy.p, W.sg_QATdy.p, br, part);
             k_sigma_pick<<<1, 1024, 0, s>>>(nb, part, a, b, br);
         }
         HPRQP_HIP(hipGetLastError());
@@ -2009,11 +2052,16 @@ static double sigma_estimate_diag(Solver<VT, MT> &W, double a, double b) {
         HPRQP_HIP(hipStreamSynchronize(s));
         return W.red.host[0];
     }
-    std::vector<VT> hc(n), hd(n);
+    std::vector<VT> hc(n);
+    std::vector<double> dd(n);
     HPRQP_HIP(hipMemcpyAsync(hc.data(), W.ATdy.p, sizeof(VT) * n, hipMemcpyDeviceToHost, s));
-    HPRQP_HIP(hipMemcpyAsync(hd.data(), W.QATdy.p, sizeof(VT) * n, hipMemcpyDeviceToHost, s));
+    if constexpr (std::is_same<VT, double>::value) {
+        HPRQP_HIP(hipMemcpyAsync(dd.data(), W.QATdy.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    } else {
+        HPRQP_HIP(hipMemcpyAsync(dd.data(), W.sg_QATdy.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    }
     HPRQP_HIP(hipStreamSynchronize(s));
-    std::vector<double> cc(hc.begin(), hc.end()), dd(hd.begin(), hd.end());
+    std::vector<double> cc(hc.begin(), hc.end());
     return golden_Q_diag(a, b, W.S.diag_Q_host, cc, dd, 1e-12, 1e12, 1e-13, 200);
 }
 
@@ -2576,7 +2624,7 @@ static int run_loop(WS &W, LoopState &L, int64_t iter_begin, double t_start, dou
                 } else if (++L.stall >= (p->polish_stall_checks > 0 ? p->polish_stall_checks
                                          : !use_switch                    ? 400
                                          : (p->precision == HPRQP_PREC_MIXED ? 100 : 40)) &&
-                           res.kkt < 1e-3) {
+                           L.best_kkt < 1e-3) {
                     L.iter = iter;
                     return -1;
                 }
@@ -2809,7 +2857,11 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
                 res->sigma = F.sigma;
             }
         };
-        if (p->precision == HPRQP_PREC_MIXED) {
+        // LASSO takes the FP32 data with FP64 iterates in both reduced
+        // precisions: its sigma climbs to 1e4-1e5, where the FP32 rounding of
+        // the prox argument (sigma * eps * |Qw + c|) exceeds the tolerance
+        // and an FP32 iteration drifts away from the solution
+        if (p->precision == HPRQP_PREC_MIXED || S.qkind == QOP_LASSO) {
             Solver<double, float> F(ctx, S, red, p);
             lowp(F);
         } else {

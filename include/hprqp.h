@@ -76,7 +76,10 @@ extern "C" {
                                FP64 Halpern anchors, and the anchors' products
                                are FP64, formed once per restart: that form
                                tracks the FP64 iteration to the tolerance
-                               and switches to FP64 only on a long stall */
+                               and switches to FP64 only on a long stall.
+                               LASSO runs the HPRQP_PREC_MIXED form: its
+                               sigma climbs to 1e4-1e5, where FP32 iterates
+                               drift off the solution */
 
 typedef struct hprqp_ctx hprqp_ctx;
 typedef struct hprqp_buf hprqp_buf;
@@ -132,7 +135,8 @@ typedef struct {
                                    one launch in eight carries the events */
     int32_t polish_stall_checks; /* reduced precision: residual checks without
                                     the KKT residual halving before the FP64
-                                    polish takes over (0 = automatic: 100 for
+                                    polish takes over, once the residual has
+                                    been below 1e-3 (0 = automatic: 100 for
                                     HPRQP_PREC_MIXED, 40 for HPRQP_PREC_FP32,
                                     400 for its offset form) */
     double polish_switch;        /* reduced precision: the FP64 polish also

@@ -143,13 +143,16 @@ def test_value_table_form_matches_oracle(case):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp32"])
-@pytest.mark.parametrize("case", ["grid", "random", "aug3dcqp", "lasso", "lasso_chunked"])
+@pytest.mark.parametrize("case", ["grid", "random", "aug3dcqp", "lasso", "lasso_chunked", "lasso_wide"])
 def test_reduced_precision_reaches_fp64_accuracy(precision, case):
     """Mixed (FP32 matrices and data, FP64 iterates) and FP32 (+FP64 polish)
     forms: the KKT residual is evaluated in FP64 against the FP64 data, so
     the same 1e-6 relative KKT bar applies; the objective must agree with
     the FP64 oracle within 1e-6 relative.  lasso_chunked forces the
-    column-chunked row product (FP32 values on FP64 vectors, and all-FP32)."""
+    column-chunked row product (FP32 values on FP64 vectors, and all-FP32).
+    lasso_wide (95 x 7538) drives sigma to ~1e5, where FP32 iterates drift
+    off the solution: both reduced precisions must converge within twice
+    the oracle's iterations (LASSO runs the FP32-data, FP64-iterate form)."""
     from hprqp import HPRQP_parameters, build_from_mps, instances, optimize
     from oracle import oracle
 
@@ -163,6 +166,9 @@ def test_reduced_precision_reaches_fp64_accuracy(precision, case):
     elif case == "lasso":
         model = instances.lasso(200, 4000, mean_nnz=60, seed=1)
         ptype = "LASSO"
+    elif case == "lasso_wide":
+        model = instances.lasso(95, 7538, mean_nnz=55.0, seed=281397143)
+        ptype = "LASSO"
     else:
         model = instances.lasso(120, 30_000, mean_nnz=500, seed=3)
         ptype = "LASSO"
@@ -174,6 +180,8 @@ def test_reduced_precision_reaches_fp64_accuracy(precision, case):
           f"rel_obj_err_vs_fp64={rel:.2e} (oracle iter={ref.iter})")
     assert res.status == "OPTIMAL" and res.residuals < 1e-6
     assert rel < 1e-6
+    if case == "lasso_wide":
+        assert res.iter <= 2 * ref.iter
 
 
 def test_lasso_small():
