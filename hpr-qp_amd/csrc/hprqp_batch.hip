@@ -705,6 +705,26 @@ __device__ void bresid(Inst<MT> &I, bool iter0, double *rec, double *t, double *
     __syncthreads();
 }
 
+// the relative primal residual alone (compute_residuals!'s err_Rp,
+// src/algorithm.jl:37-133), FP64: A x_bar over the FP64 A, no pass over Q.
+// The KKT residual is the max of err_Rp, err_Rd and the gap, so err_Rp >= tol
+// settles a print step's termination test without the FP64 Q x
+template <int QK, class MT>
+__device__ double bprimal_err(Inst<MT> &I, double nbi, double *sh) {
+    const int32_t m = I.m;
+    csr_rows_wave(m, I.P->arp, I.P->aci, I.P->av64, I.P->x_bar, I.P->pz);
+    double rp = 0, ax = 0;
+    for (int32_t r = threadIdx.x; r < m; r += kBB) {
+        const double v = I.P->pz[r];
+        const double sf = QK == 1 ? I.P->rn[r] : 1.0;
+        rp = dmax(rp, fabs((proj(v, I.P->AL[r], I.P->AU[r]) - v) * sf));
+        ax = dmax(ax, fabs(v * sf));
+    }
+    rp = bmax(rp, sh);
+    ax = bmax(ax, sh);
+    return rp / (1.0 + fmax(nbi, ax));
+}
+
 // the dot products of update_sigma! (src/algorithm.jl:332-364)
 template <int QK, class MT>
 __device__ void bsigma_dots(Inst<MT> &I, double *rec, double *t, double *sh) {
@@ -824,8 +844,11 @@ __global__ __launch_bounds__(kBB) void k_batch_interval(BArgs<MT> a) {
         inner += 1.0;
         // a residual iteration before the interval's end (a print step between
         // two check iterations): only the termination test can act there, so
-        // it runs here and a converged instance retires without the host
-        if (print_yes && s < c.K - 1) {
+        // it runs here and a converged instance retires without the host; the
+        // FP64 residuals run only once the primal part is below the tolerance
+        // (they write scratch vectors the step does not read, so skipping them
+        // leaves the trajectory as it is)
+        if (print_yes && s < c.K - 1 && (I.m == 0 || bprimal_err<QK>(I, a.nbi[b], sh) < a.tol)) {
             bresid<QK>(I, false, rec, t, sh);
             __syncthreads();
             if (bkkt(rec, a.c0[b], a.nbi[b], a.nci[b], I.m) < a.tol) {
