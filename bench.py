@@ -35,19 +35,36 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def zxw1_bytes(nnz_q, n, prec, bound_streams=2):
+def zxw1_bytes(nnz_q, n, prec, bound_streams=2, dia_nd=None):
     """Algorithmic HBM bytes of one z/x/w1 launch (partial write set).
     fp64 / mixed: Q values+indices, row pointers, gathered w, streamed ATy c x,
     the bound vectors l u that are not a single value (0 or +-inf for every
     variable is passed as a scalar), the FP64 anchor last_x; written Qw x x_hat
-    w_bar.  fp32 (the offset form): FP32 Q values + indices, row pointers, the
+    w_bar (mixed: FP32 Q values, c, l, u; FP64 iterates).  fp32 (the offset form): FP32 Q values + indices, row pointers, the
     gathered FP32 offset dW, the row's dX dT dW (FP32), the FP64 anchor
     vectors G0 X0, the FP32 anchor difference D0 and the FP64 bounds that are
-    not one value; written Q dW, dX, dX_hat, dW_bar (FP32)."""
+    not one value; written Q dW, dX, dX_hat, dW_bar (FP32).  With Q in the
+    diagonal form (dia_nd diagonals) the values are the dia_nd x n diagonal
+    arrays, padding included, and there are no indices or row pointers."""
     if prec == "fp32":
-        return nnz_q * 8 + 4 * (n + 1) + 4 * n + 12 * n + 20 * n + 8 * bound_streams * n + 16 * n
-    v = 8 if prec == "fp64" else 4
-    return nnz_q * (v + 4) + 4 * (n + 1) + v * n + (3 + bound_streams) * v * n + 8 * n + 4 * v * n
+        qb = dia_nd * n * 4 if dia_nd else nnz_q * 8 + 4 * (n + 1)
+        return qb + 4 * n + 12 * n + 20 * n + 8 * bound_streams * n + 16 * n
+    v = 8 if prec == "fp64" else 4  # matrix values and the data c, l, u
+    qb = dia_nd * n * v if dia_nd else nnz_q * (v + 4) + 4 * (n + 1)
+    # w, ATy, x, last_x and the four written vectors are FP64 in both forms
+    return qb + 8 * n + 8 * n + v * n + 8 * n + bound_streams * v * n + 8 * n + 4 * 8 * n
+
+
+def q_diagonals(Q):
+    """The number of diagonals (distinct j - i) of Q's symmetric pattern, the
+    dia_nd of its diagonal form."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    P = sp.csr_matrix(Q, copy=True)
+    P = (abs(P) + abs(P.T)).tocsr()
+    rows = np.repeat(np.arange(P.shape[0]), np.diff(P.indptr))
+    return int(np.unique(P.indices - rows).size)
 
 
 def bound_streams(model):
@@ -445,8 +462,14 @@ def main():
             nb = lasso_step_bytes(A.nnz, A.shape[0], A.shape[1], args.precision)
             kname = "LASSO step: k_csr (A w) + k_csr<EpiLasso> (A^T temp), timed together"
         else:
-            nb = zxw1_bytes(model.Q.nnz, model.n, args.precision, bound_streams(model))
-            kname = "z/x/w1 step: CSR row product Q w with the fused EpiZxw1 epilogue (k_csr / k_csr_stream, lanes autotuned)"
+            dia_nd = q_diagonals(model.Q) if r0.lanes[0] == -2000 else None
+            nb = zxw1_bytes(model.Q.nnz, model.n, args.precision, bound_streams(model), dia_nd)
+            if dia_nd:
+                kname = (f"z/x/w1 step: Q w in the diagonal form ({dia_nd} diagonals, k_dia) with the fused "
+                         "EpiZxw1 epilogue (form autotuned)")
+            else:
+                kname = ("z/x/w1 step: CSR row product Q w with the fused EpiZxw1 epilogue (k_csr / k_csr_stream, "
+                         "lanes autotuned)")
         achieved = nb / (kms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": kname,
                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -480,7 +503,8 @@ def main():
                              "rel_x_err_vs_fp64": float(np.max(np.abs(m0.x - r0.x)) / (1.0 + np.max(np.abs(r0.x)))),
                              "zxw1_kernel_ms": km}
             if ptype == "QP":
-                zb = zxw1_bytes(model.Q.nnz, model.n, prec, bound_streams(model))
+                zb = zxw1_bytes(model.Q.nnz, model.n, prec, bound_streams(model),
+                                q_diagonals(model.Q) if m0.lanes[0] == -2000 else None)
                 reduced[prec]["zxw1_bytes_per_launch"] = zb
                 reduced[prec]["zxw1_hbm_frac"] = zb / (km * 1e-3) / 1e9 / HBM_PEAK_GBS if km > 0 else None
             log(f"{prec}: {ok}/{len(mr)} solved, {reduced[prec]['solves_per_s']:.3f} solves/s "

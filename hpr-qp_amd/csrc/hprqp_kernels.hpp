@@ -112,6 +112,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_rowblock(int32_t nrows, const in
 constexpr int kChunkW = 4096;       // columns per chunk
 constexpr int kChunkRows = 128;     // rows per block (32 per wave)
 constexpr int kLanesChunked = 1 << 20;  // row-product shape code of this form
+constexpr int kLanesDiagonal = -2000;   // shape code of Q's diagonal form (k_dia)
 
 static __global__ __launch_bounds__(kBlock) void k_chunk_seg(int32_t nrow, int32_t nchunk, const int32_t *__restrict__ rp,
                                                              const int32_t *__restrict__ ci, int32_t *__restrict__ seg) {
@@ -327,6 +328,42 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream_dict(int32_t nrows, int r
         for (int32_t k = rp[row]; k < rp[row + 1]; ++k) acc += (AT)(dl[code[k]] * csc[ci[k]]) * (AT)x[ci[k]];
         epi(row, acc * (AT)rsc[row]);
     }
+}
+
+// Diagonal form: a square matrix whose nonzeros lie on at most kDiaMax
+// diagonals (column offsets off[d] = j - i, ascending) keeps each diagonal as
+// an array indexed by row, zero where the pattern has no entry.  There are no
+// row pointers or column indices to read, and every load of a wave -- the
+// diagonal values and x[i + off] -- is contiguous (the neighbouring
+// diagonals' x segments are L2 hits).  Row i adds its diagonals in ascending
+// column order, each product rounded before the add, as the CSR-stream
+// kernel does, so the sum is bitwise that kernel's: the padding adds exact
+// zeros.
+constexpr int kDiaMax = 8;
+struct DiaOff {
+    int32_t o[kDiaMax];
+};
+
+template <class MT, class XT, class AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_dia(int32_t n, int nd, DiaOff off, const MT *__restrict__ dv,
+                                               const XT *__restrict__ x, Epi epi) {
+#pragma clang fp contract(off)
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t i = b * kBlock + (int32_t)threadIdx.x;
+    if (i >= n) return;
+    AT acc = 0;
+#pragma unroll
+    for (int d = 0; d < kDiaMax; ++d) {
+        if (d < nd) {
+            const int32_t j = i + off.o[d];
+            const bool in = j >= 0 && j < n;
+            const AT v = (AT)dv[(int64_t)d * n + i];
+            const AT xv = in ? (AT)x[in ? j : i] : (AT)0;
+            const AT pr = v * xv;
+            acc += pr;
+        }
+    }
+    epi(i, acc);
 }
 
 // elementwise form of the same epilogue (operator Q: product precomputed)

@@ -116,8 +116,27 @@ static void launch_dict(hipStream_t s, const DevCsr &a, int rb, const XT *x, con
     HPRQP_HIP(hipGetLastError());
 }
 
+// the diagonal form's product, in the values' precision MT; false when the
+// matrix has no diagonal form in that precision
+template <class AT, class MT, class XT, class Epi>
+static bool launch_dia(hipStream_t s, const DevCsr &a, const XT *x, const Epi &epi) {
+    const MT *dv;
+    if constexpr (std::is_same<MT, double>::value)
+        dv = a.dia_v.p;
+    else
+        dv = a.dia_v32.p;
+    if (a.dia_nd <= 0 || a.nrow <= 0 || dv == nullptr) return false;
+    DiaOff off;
+    for (int d = 0; d < kDiaMax; ++d) off.o[d] = d < a.dia_nd ? a.dia_off[d] : 0;
+    launch_k(k_dia<MT, XT, AT, Epi>, dim3(nblk(a.nrow)), dim3(kBlock), 0, s, a.nrow, a.dia_nd, off, dv, x, epi);
+    HPRQP_HIP(hipGetLastError());
+    return true;
+}
+
 template <class AT, class MT, class XT, class Epi>
 static void csr_apply(hipStream_t s, const DevCsr &a, const XT *x, const Epi &epi) {
+    // the diagonal form, where the autotuner (or the caller) chose it
+    if (a.dia_nd > 0 && launch_dia<AT, MT>(s, a, x, epi)) return;
     if constexpr (std::is_same<MT, double>::value && std::is_same<AT, double>::value) {
         // the FP64 iteration's value-table form, where the autotuner chose it
         if (a.dict_rb > 0 && a.nnz > 0 && a.nrow > 0) {
@@ -2896,7 +2915,7 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     res->n_history = L.nh;
     res->polish_iter = polish_from >= 0 ? L.iter - polish_from : 0;
     // value-table shapes reported as -1000 - rows per block
-    res->lanes_Q = S.Q.dict_rb > 0 ? -1000 - S.Q.dict_rb : S.Q.lanes;
+    res->lanes_Q = S.Q.dia_nd > 0 ? kLanesDiagonal : S.Q.dict_rb > 0 ? -1000 - S.Q.dict_rb : S.Q.lanes;
     res->lanes_A = S.A.dict_rb > 0 ? -1000 - S.A.dict_rb : S.A.lanes;
     res->lanes_AT = S.AT.dict_rb > 0 ? -1000 - S.AT.dict_rb : S.AT.lanes;
     if (S.qkind == QOP_LASSO) {
@@ -3035,6 +3054,127 @@ static int tune_dict(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
     return best;
 }
 
+// The diagonals of a square pattern: their count (offsets j - i ascending in
+// off), or -1 when there are more than kDiaMax or the diagonal arrays would
+// hold more than 1.5 entries per nonzero (padding)
+static int dia_detect(const HostCsr &h, int32_t *off) {
+    if (h.nrow != h.ncol || h.nrow <= 0 || h.rp.empty() || h.rp[h.nrow] <= 0) return -1;
+    int nd = 0;
+    for (int32_t i = 0; i < h.nrow; ++i)
+        for (int32_t k = h.rp[i]; k < h.rp[i + 1]; ++k) {
+            const int32_t o = h.ci[k] - i;
+            int d = 0;
+            while (d < nd && off[d] != o) ++d;
+            if (d == nd) {
+                if (nd == kDiaMax) return -1;
+                off[nd++] = o;
+            }
+        }
+    std::sort(off, off + nd);
+    if ((double)nd * (double)h.nrow > 1.5 * (double)h.rp[h.nrow]) return -1;
+    return nd;
+}
+
+// Fills a's diagonal arrays from its CSR values (FP64, and FP32 when f32: the
+// FP32 values are the FP64 ones rounded, as the CSR form's v32)
+static void build_dia(hipStream_t s, DevCsr &a, int nd, const int32_t *offs, bool f32) {
+    const int32_t n = a.nrow;
+    const int64_t len = (int64_t)nd * n;
+    DiaOff off;
+    for (int d = 0; d < kDiaMax; ++d) off.o[d] = d < nd ? offs[d] : 0;
+    for (int d = 0; d < kDiaMax; ++d) a.dia_off[d] = off.o[d];
+    if (a.dia_v.n != len) {
+        a.dia_v.alloc(len);
+        HPRQP_HIP(hipMemsetAsync(a.dia_v.p, 0, sizeof(double) * len, s));
+        const int32_t *rp = a.rp.p, *ci = a.ci.p;
+        const double *v = a.v.p;
+        double *dv = a.dia_v.p;
+        each(s, n, [=] __device__(int64_t i) {
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const int32_t o = ci[k] - (int32_t)i;
+                int d = 0;
+                while (d < nd - 1 && off.o[d] != o) ++d;
+                dv[(int64_t)d * n + i] = v[k];
+            }
+        });
+    }
+    if (f32 && a.dia_v32.n != len) {
+        a.dia_v32.alloc(len);
+        const double *dv = a.dia_v.p;
+        float *d32 = a.dia_v32.p;
+        each(s, len, [=] __device__(int64_t k) { d32[k] = (float)dv[k]; });
+    }
+    HPRQP_HIP(hipStreamSynchronize(s));
+}
+
+// The diagonal form of Q against the shape chosen for its CSR form, in the
+// iteration's precision (VT vectors, MT values); taken when 5 % faster
+template <class VT, class MT>
+static bool tune_dia(hipStream_t s, DevCsr &a, int nd, int reps, bool verbose) {
+    const int64_t nr = std::max<int64_t>(1, a.nrow);
+    DevVec<VT> x(nr), y(nr), z(nr), rv(4 * nr);
+    {
+        VT *xp = x.p, *rp = rv.p;
+        each(s, nr, [=] __device__(int64_t i) { xp[i] = (VT)1; });
+        each(s, 4 * nr, [=] __device__(int64_t i) { rp[i] = (VT)0.5; });
+    }
+    const EpiTune<VT> epi{y.p, z.p, rv.p, rv.p + nr, rv.p + 2 * nr, rv.p + 3 * nr};
+    EventTimer tm(s);
+    a.dia_nd = 0;
+    const double csr = tm.avg_ms(reps, [&] { csr_apply<VT, MT>(s, a, (const VT *)x.p, epi); });
+    a.dia_nd = nd;
+    const double dia = tm.avg_ms(reps, [&] { csr_apply<VT, MT>(s, a, (const VT *)x.p, epi); });
+    a.dia_nd = 0;
+    if (verbose)
+        std::printf("row-product autotune   diagonal form (%d diagonals) %.4f ms against %.4f ms\n", nd, dia, csr);
+    return dia < 0.95 * csr;
+}
+
+// Q in the diagonal form: forced (spmv_lanes = kLanesDiagonal), or where its
+// pattern has at most kDiaMax diagonals and the autotuner times that form
+// faster.  The reduced precisions keep FP64 diagonals too (the polish and the
+// residuals run in FP64).
+static void apply_dia(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool forced) {
+    DevCsr &a = S.Q;
+    if (S.qkind != QOP_CSR || S.q_empty) {
+        HPRQP_REQUIRE(!forced, HPRQP_E_ARG, "spmv_lanes = -2000 needs a sparse matrix Q");
+        return;
+    }
+    if (md->q_dia_nd == 0) md->q_dia_nd = dia_detect(md->Qsym, md->q_dia_off);
+    const int nd = md->q_dia_nd;
+    if (nd <= 0 || a.nrow != md->n) {
+        HPRQP_REQUIRE(!forced, HPRQP_E_ARG,
+                      "spmv_lanes = -2000: Q has more than 8 diagonals or more padding than 1.5 entries per nonzero");
+        return;
+    }
+    const bool f32 = p->precision != HPRQP_PREC_FP64;
+    if (forced) {
+        build_dia(s, a, nd, md->q_dia_off, f32);
+        a.dia_nd = nd;
+        return;
+    }
+    if (a.nnz < (1 << 18)) return;
+    int &td = md->tuned_dia[p->precision];
+    if (td == 0) {
+        build_dia(s, a, nd, md->q_dia_off, f32);
+        bool take;
+        if (p->precision == HPRQP_PREC_MIXED)
+            take = tune_dia<double, float>(s, a, nd, 20, p->verbose);
+        else if (p->precision == HPRQP_PREC_FP32)
+            take = tune_dia<float, float>(s, a, nd, 20, p->verbose);
+        else
+            take = tune_dia<double, double>(s, a, nd, 20, p->verbose);
+        td = take ? 1 : -1;
+    }
+    if (td > 0) {
+        build_dia(s, a, nd, md->q_dia_off, f32);
+        a.dia_nd = nd;
+    } else {
+        a.dia_v.release();
+        a.dia_v32.release();
+    }
+}
+
 // the QAP operator's pass family: the caller's, or the one timed fastest at
 // the operator's first solve in that precision (the reduced precisions run
 // the split-BF16 passes and the FP64 ones in the polish and residuals)
@@ -3062,6 +3202,11 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
                   HPRQP_E_ARG, "unknown precision mode");
     (void)mixed;
     apply_qap_family(s, p, S);
+    if (p->spmv_lanes == kLanesDiagonal) {
+        // Q in the diagonal form; A and A^T keep their default shapes
+        apply_dia(s, md, p, S, true);
+        return;
+    }
     if (p->spmv_lanes <= -1000) {
         // the value-table form forced (rows per block -spmv_lanes - 1000)
         const int rb = -p->spmv_lanes - 1000;
@@ -3118,6 +3263,7 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
             a.dict_rb = td > 0 ? td : 0;
         }
     }
+    apply_dia(s, md, p, S, false);
 }
 
 template <class VT, class MT>

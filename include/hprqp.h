@@ -129,7 +129,13 @@ typedef struct {
                                    per row; 256 = one block per row;
                                    -64/-128/-256 = CSR-stream with that many
                                    rows per block; 1048576 = column-chunked
-                                   (x staged in LDS 4,096 columns at a time) */
+                                   (x staged in LDS 4,096 columns at a time);
+                                   -1032/-1064/-1128/-1256 = value-table form;
+                                   -2000 = Q in the diagonal form (its nonzeros
+                                   on at most 8 diagonals, each kept as an
+                                   array by row; A and A^T keep their default
+                                   shapes).  The autotuner also times the
+                                   diagonal form for such a Q */
     int32_t time_kernels;       /* 1 = time the dominant fused kernel with
                                    HIP events during the solve (result.kernel_ms);
                                    one launch in eight carries the events */
@@ -207,7 +213,8 @@ typedef struct {
     int32_t lanes_A;      /* codes: 1..64 lanes per row, 256 a block per row, */
     int32_t lanes_AT;     /* -16 .. -256 CSR-stream rows per block, 1048576
                              column-chunked: x staged in LDS 4,096 columns at a
-                             time, for long rows over many columns) */
+                             time, for long rows over many columns, -1000 - rb
+                             value-table form, -2000 Q's diagonal form) */
     int32_t qap_family;   /* QAP operator: family of the iteration's passes
                              (params.qap_family codes, 1 = one-tile; the
                              split-BF16 one in the reduced precisions) */

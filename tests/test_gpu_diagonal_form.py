@@ -1,0 +1,120 @@
+"""Q's diagonal form: a square Q whose nonzeros lie on at most 8 diagonals is
+kept as one array per diagonal, indexed by row (k_dia), forced with
+spmv_lanes = -2000 or taken by the autotuner where it times faster.
+
+* Bitwise: the fused z/x/w1 and tempv half-steps, whose only matrix product
+  is Q's, write bitwise the same vectors with Q in the diagonal form as in
+  the CSR-stream form of 256 rows per block, from the same uploaded state
+  (both add a row's products in ascending column order; the padding adds
+  exact zeros).
+* Solves with the form forced match the oracle (iterations within 10 %,
+  objective within 1e-6 relative, the power method's stop iterations within
+  one) on the grid QP and on a banded QP with diagonals at offsets 0, +-1 and
+  +-37, and reach 1e-6 in the reduced precisions.
+* A Q on more than 8 diagonals is refused when the form is forced and keeps
+  its CSR form under the autotuner."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from test_gpu_parity import _check, _solve_both
+
+pytestmark = pytest.mark.gpu
+
+VEC = {"w_bar": 3, "x": 7, "w": 8, "y": 9, "ATy": 10, "Qw": 11, "x_hat": 12, "dx": 13, "x_bar": 0,
+       "z_bar": 2, "tempv": 18, "Qw_bar": 19, "last_x": 20, "last_w": 22, "last_ATy": 23}
+
+
+def _banded_qp(n=3000, m=700, seed=5):
+    from hprqp import build_from_QAbc
+
+    rng = np.random.default_rng(seed)
+    B = sp.diags([rng.uniform(0.5, 1.5, n), rng.uniform(-0.5, 0.5, n - 1), rng.uniform(-0.3, 0.3, n - 37)],
+                 [0, 1, 37], format="csr")
+    Q = (B.T @ B + 1e-2 * sp.eye(n)).tocsr()  # offsets 0, +-1, +-36, +-37
+    A = sp.random(m, n, density=4.0 / n, random_state=seed, format="csr") + sp.eye(m, n)
+    b = A @ rng.uniform(-1, 1, n)
+    return build_from_QAbc(Q, rng.standard_normal(n), A.tocsr(), b - 0.5, b + 0.5, np.full(n, -2.0),
+                           np.full(n, 2.0), verbose=False)
+
+
+def test_half_steps_bitwise_equal_to_csr_stream():
+    from hprqp import HPRQP_parameters, instances
+    from hprqp.solver import DeviceModel
+    from hprqp.steploop import Workspace
+
+    model = instances.grid_qp(120, seed=3)
+    n = model.n
+    dm = DeviceModel(model)
+    ws = [Workspace(dm, HPRQP_parameters(verbose=False, spmv_lanes=lanes)) for lanes in (-256, -2000)]
+    try:
+        rng = np.random.default_rng(8)
+        state = {k: rng.uniform(-1.0, 1.0, n) for k in ("x", "w", "ATy", "last_x", "last_w", "last_ATy", "w_bar")}
+        for w in ws:
+            for k, v in state.items():
+                w.upload(VEC[k], v)
+        for name, outs in (("zxw1", ("x", "x_hat", "w_bar", "Qw", "dx", "x_bar", "z_bar")),
+                           ("tempv", ("tempv",))):
+            for w in ws:
+                w.step(name, 0.7, 0.25, 2.0, 8.0, True)
+            for k in outs:
+                a, b = (w.download(VEC[k], n) for w in ws)
+                print(f"{name}: {k} max |diff| {np.max(np.abs(a - b)):.3e}")
+                assert np.array_equal(a, b), (name, k)
+            # the product entered: Q w is not zero, tempv = x_hat + sigma (Q w - Q w_bar) is not x_hat
+            if name == "zxw1":
+                assert np.any(ws[0].download(VEC["Qw"], n) != 0.0)
+            else:
+                assert np.any(ws[0].download(VEC["tempv"], n) != ws[0].download(VEC["x_hat"], n))
+    finally:
+        for w in ws:
+            w.close()
+        dm.close()
+
+
+@pytest.mark.parametrize("case", ["grid", "banded"])
+def test_forced_diagonal_form_matches_oracle(case):
+    from hprqp import instances
+
+    model = instances.grid_qp(90, seed=4) if case == "grid" else _banded_qp()
+    res, ref = _solve_both(model, spmv_lanes=-2000)
+    print(f"{case}: shapes {res.lanes}")
+    assert res.lanes[0] == -2000
+    _check(res, ref)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp32"])
+def test_forced_diagonal_form_reduced_precision(precision):
+    from hprqp import HPRQP_parameters, instances, optimize
+    from oracle import oracle
+
+    model = instances.grid_qp(90, seed=4)
+    res = optimize(model, HPRQP_parameters(verbose=False, precision=precision, spmv_lanes=-2000))
+    ref = oracle.solve(model)
+    rel = abs(res.primal_obj - ref.primal_obj) / (1.0 + abs(ref.primal_obj))
+    print(f"{precision}: {res.status} iter={res.iter} polish={res.polish_iter} kkt={res.residuals:.2e} "
+          f"rel={rel:.2e} (oracle iter={ref.iter}) shapes {res.lanes}")
+    assert res.lanes[0] == -2000
+    assert res.status == "OPTIMAL" and res.residuals < 1e-6 and rel < 1e-6
+
+
+def test_more_than_eight_diagonals():
+    from hprqp import HPRQP_parameters, instances, optimize
+
+    model = instances.random_sparse_qp(2000, 800, seed=2)
+    with pytest.raises(RuntimeError, match="more than 8 diagonals"):
+        optimize(model, HPRQP_parameters(verbose=False, spmv_lanes=-2000))
+    res = optimize(model, HPRQP_parameters(verbose=False))
+    assert res.status == "OPTIMAL" and res.lanes[0] != -2000
+
+
+def test_autotuned_grid():
+    """grid 300 x 300 (450k nonzeros in Q): the autotuner times the diagonal
+    form against Q's CSR shape; whichever it keeps, the solve matches the
+    oracle."""
+    from hprqp import instances
+
+    model = instances.grid_qp(300, seed=1)
+    res, ref = _solve_both(model)
+    print(f"autotuned shapes {res.lanes}")
+    _check(res, ref)
