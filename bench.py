@@ -342,6 +342,9 @@ def main():
     ap.add_argument("--grid", type=int, default=1414)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "fp32"])
     ap.add_argument("--profile-reps", type=int, default=50)
+    ap.add_argument("--spmv-lanes", type=int, default=0,
+                    help="row-product shape of the single-instance solves (0 = autotuned; -2000 pins the diagonal "
+                         "form, as counter runs do, whose timings under the profiler can flip the autotuner)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=300,
                     help="HPR iterations of the CPU oracle sample (about 12 s on the 1414^2 grid)")
@@ -421,7 +424,8 @@ def main():
     model, ptype, desc = workload(args.workload, args.grid)
     log(f"{desc}: n={model.n} m={model.m} built in {time.time() - t0:.1f}s")
     dm = DeviceModel(model, 0)
-    params = HPRQP_parameters(verbose=False, precision=args.precision, time_kernels=True, problem_type=ptype)
+    params = HPRQP_parameters(verbose=False, precision=args.precision, time_kernels=True, problem_type=ptype,
+                              spmv_lanes=args.spmv_lanes)
     for w in range(args.warmup):
         r = dm.solve(params, hist_cap=1000)
         log(f"warmup {w}: {r.status} iter={r.iter} kkt={r.residuals:.2e} time={r.time:.3f}s "
@@ -487,7 +491,8 @@ def main():
     reduced = {}
     if args.precision == "fp64" and not args.no_mixed:
         for prec in ("fp32", "mixed"):
-            mp = HPRQP_parameters(verbose=False, precision=prec, problem_type=ptype, time_kernels=True)
+            mp = HPRQP_parameters(verbose=False, precision=prec, problem_type=ptype, time_kernels=True,
+                                  spmv_lanes=args.spmv_lanes)
             dm.solve(mp, hist_cap=16)
             t = time.perf_counter()
             mr = [dm.solve(mp, hist_cap=16) for _ in range(max(1, args.steps))]

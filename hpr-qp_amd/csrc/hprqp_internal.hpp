@@ -97,10 +97,10 @@ struct DevCsr {
     DevVec<uint8_t> code;
     DevVec<double> dict;
     const double *rsc = nullptr, *csc = nullptr;
-    // diagonal form (a square matrix whose nonzeros lie on at most 8
-    // diagonals): dia_nd > 0 selects it; dia_off the column offsets j - i,
-    // ascending; dia_v (FP64) and dia_v32 (FP32) the diagonals, dia_nd x
-    // nrow, zero where the pattern has no entry
+    // diagonal form (a matrix whose nonzeros lie on at most 8 diagonals j =
+    // floor(i * ncol / nrow) + off): dia_nd > 0 selects it; dia_off the
+    // offsets, ascending; dia_v (FP64) and dia_v32 (FP32) the diagonals,
+    // dia_nd x nrow, zero where the pattern has no entry
     int dia_nd = 0;
     int32_t dia_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     DevVec<double> dia_v;
@@ -240,13 +240,13 @@ struct hprqp_model {
     std::vector<double> a_dict, q_dict;
     std::vector<uint8_t> a_code, at_code, q_code;
     mutable int tuned_dict[3] = {0, 0, 0};
-    // diagonal form of Q (Qsym's pattern): q_dia_nd diagonals at offsets
-    // q_dia_off (0 = not examined yet, -1 = more than 8 diagonals or too
-    // much padding); tuned_dia per precision mode (0 not tuned, 1 taken,
-    // -1 not taken)
-    mutable int q_dia_nd = 0;
-    mutable int32_t q_dia_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    mutable int tuned_dia[3] = {0, 0, 0};
+    // diagonal forms of Q (Qsym's pattern), A and A^T: dia_nd diagonals at
+    // offsets dia_off (0 = not examined yet, -1 = more than 8 diagonals or
+    // too much padding); tuned_dia per precision mode and matrix (0 not
+    // tuned, 1 taken, -1 not taken)
+    mutable int dia_nd[3] = {0, 0, 0};
+    mutable int32_t dia_off[3][8] = {};
+    mutable int tuned_dia[3][3] = {};
     // device copies of the raw data (A, its transpose, Qsym, the vectors
     // and the value-table codes), made at the first solve; each solve
     // copies them device to device into its scaled data instead of

@@ -330,35 +330,42 @@ __global__ __launch_bounds__(kBlock) void k_csr_stream_dict(int32_t nrows, int r
     }
 }
 
-// Diagonal form: a square matrix whose nonzeros lie on at most kDiaMax
-// diagonals (column offsets off[d] = j - i, ascending) keeps each diagonal as
-// an array indexed by row, zero where the pattern has no entry.  There are no
-// row pointers or column indices to read, and every load of a wave -- the
-// diagonal values and x[i + off] -- is contiguous (the neighbouring
-// diagonals' x segments are L2 hits).  Row i adds its diagonals in ascending
-// column order, each product rounded before the add, as the CSR-stream
-// kernel does, so the sum is bitwise that kernel's: the padding adds exact
-// zeros.
+// Diagonal form: a matrix whose nonzeros lie on at most kDiaMax diagonals --
+// entry (i, j) on diagonal d when j = base(i) + off[d], base(i) =
+// floor(i * ncol / nrow) (i itself for a square matrix; for a rectangular one
+// the diagonals follow its slope, as an edge-cell operator's rows do on a
+// grid) -- keeps each diagonal as an array indexed by row, zero where the
+// pattern has no entry.  There are no row pointers or column indices to read,
+// and every load of a wave -- the diagonal values and x[base(i) + off] -- is
+// contiguous (the neighbouring diagonals' x segments are L2 hits).  Row i adds
+// its diagonals in ascending column order (off ascending), each product
+// rounded before the add, as the CSR-stream kernel does, so the sum is
+// bitwise that kernel's: the padding adds exact zeros.
 constexpr int kDiaMax = 8;
 struct DiaOff {
     int32_t o[kDiaMax];
 };
 
+__device__ __host__ __forceinline__ int32_t dia_base(int32_t i, int32_t nrow, int32_t ncol) {
+    return nrow == ncol ? i : (int32_t)(((int64_t)i * ncol) / nrow);
+}
+
 template <class MT, class XT, class AT, class Epi>
-__global__ __launch_bounds__(kBlock) void k_dia(int32_t n, int nd, DiaOff off, const MT *__restrict__ dv,
-                                               const XT *__restrict__ x, Epi epi) {
+__global__ __launch_bounds__(kBlock) void k_dia(int32_t nrow, int32_t ncol, int nd, DiaOff off,
+                                               const MT *__restrict__ dv, const XT *__restrict__ x, Epi epi) {
 #pragma clang fp contract(off)
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int32_t i = b * kBlock + (int32_t)threadIdx.x;
-    if (i >= n) return;
+    if (i >= nrow) return;
+    const int32_t base = dia_base(i, nrow, ncol);
     AT acc = 0;
 #pragma unroll
     for (int d = 0; d < kDiaMax; ++d) {
         if (d < nd) {
-            const int32_t j = i + off.o[d];
-            const bool in = j >= 0 && j < n;
-            const AT v = (AT)dv[(int64_t)d * n + i];
-            const AT xv = in ? (AT)x[in ? j : i] : (AT)0;
+            const int32_t j = base + off.o[d];
+            const bool in = j >= 0 && j < ncol;
+            const AT v = (AT)dv[(int64_t)d * nrow + i];
+            const AT xv = in ? (AT)x[in ? j : 0] : (AT)0;
             const AT pr = v * xv;
             acc += pr;
         }

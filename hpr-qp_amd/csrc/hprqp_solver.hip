@@ -128,7 +128,8 @@ static bool launch_dia(hipStream_t s, const DevCsr &a, const XT *x, const Epi &e
     if (a.dia_nd <= 0 || a.nrow <= 0 || dv == nullptr) return false;
     DiaOff off;
     for (int d = 0; d < kDiaMax; ++d) off.o[d] = d < a.dia_nd ? a.dia_off[d] : 0;
-    launch_k(k_dia<MT, XT, AT, Epi>, dim3(nblk(a.nrow)), dim3(kBlock), 0, s, a.nrow, a.dia_nd, off, dv, x, epi);
+    launch_k(k_dia<MT, XT, AT, Epi>, dim3(nblk(a.nrow)), dim3(kBlock), 0, s, a.nrow, a.ncol, a.dia_nd, off, dv, x,
+             epi);
     HPRQP_HIP(hipGetLastError());
     return true;
 }
@@ -2915,9 +2916,12 @@ void solve(hprqp_ctx *ctx, const hprqp_model *md, const hprqp_params *p, const d
     res->n_history = L.nh;
     res->polish_iter = polish_from >= 0 ? L.iter - polish_from : 0;
     // value-table shapes reported as -1000 - rows per block
-    res->lanes_Q = S.Q.dia_nd > 0 ? kLanesDiagonal : S.Q.dict_rb > 0 ? -1000 - S.Q.dict_rb : S.Q.lanes;
-    res->lanes_A = S.A.dict_rb > 0 ? -1000 - S.A.dict_rb : S.A.lanes;
-    res->lanes_AT = S.AT.dict_rb > 0 ? -1000 - S.AT.dict_rb : S.AT.lanes;
+    auto shape = [](const DevCsr &a) {
+        return a.dia_nd > 0 ? kLanesDiagonal : a.dict_rb > 0 ? -1000 - a.dict_rb : a.lanes;
+    };
+    res->lanes_Q = shape(S.Q);
+    res->lanes_A = shape(S.A);
+    res->lanes_AT = shape(S.AT);
     if (S.qkind == QOP_LASSO) {
         res->lanes_Q = S.qop->la.lanes;  // the LASSO operator's A (its A^T in lanes_AT)
         res->lanes_AT = S.qop->lat.lanes;
@@ -3054,15 +3058,18 @@ static int tune_dict(hipStream_t s, const DevCsr &a, int reps, bool verbose) {
     return best;
 }
 
-// The diagonals of a square pattern: their count (offsets j - i ascending in
-// off), or -1 when there are more than kDiaMax or the diagonal arrays would
-// hold more than 1.5 entries per nonzero (padding)
-static int dia_detect(const HostCsr &h, int32_t *off) {
-    if (h.nrow != h.ncol || h.nrow <= 0 || h.rp.empty() || h.rp[h.nrow] <= 0) return -1;
+// The diagonals of a pattern, entries (i, j) with j = floor(i * ncol / nrow)
+// + off: their count (offsets ascending in off), or -1 when there are more
+// than kDiaMax or the diagonal arrays would hold more than 1.5 entries per
+// nonzero (padding).  transposed: the pattern of h's transpose.
+static int dia_detect(const HostCsr &h, bool transposed, int32_t *off) {
+    const int32_t nrow = transposed ? h.ncol : h.nrow, ncol = transposed ? h.nrow : h.ncol;
+    if (nrow <= 0 || ncol <= 0 || h.rp.empty() || h.rp[h.nrow] <= 0) return -1;
     int nd = 0;
-    for (int32_t i = 0; i < h.nrow; ++i)
-        for (int32_t k = h.rp[i]; k < h.rp[i + 1]; ++k) {
-            const int32_t o = h.ci[k] - i;
+    for (int32_t r = 0; r < h.nrow; ++r)
+        for (int32_t k = h.rp[r]; k < h.rp[r + 1]; ++k) {
+            const int32_t i = transposed ? h.ci[k] : r, j = transposed ? r : h.ci[k];
+            const int32_t o = j - dia_base(i, nrow, ncol);
             int d = 0;
             while (d < nd && off[d] != o) ++d;
             if (d == nd) {
@@ -3071,15 +3078,15 @@ static int dia_detect(const HostCsr &h, int32_t *off) {
             }
         }
     std::sort(off, off + nd);
-    if ((double)nd * (double)h.nrow > 1.5 * (double)h.rp[h.nrow]) return -1;
+    if ((double)nd * (double)nrow > 1.5 * (double)h.rp[h.nrow]) return -1;
     return nd;
 }
 
 // Fills a's diagonal arrays from its CSR values (FP64, and FP32 when f32: the
 // FP32 values are the FP64 ones rounded, as the CSR form's v32)
 static void build_dia(hipStream_t s, DevCsr &a, int nd, const int32_t *offs, bool f32) {
-    const int32_t n = a.nrow;
-    const int64_t len = (int64_t)nd * n;
+    const int32_t nrow = a.nrow, ncol = a.ncol;
+    const int64_t len = (int64_t)nd * nrow;
     DiaOff off;
     for (int d = 0; d < kDiaMax; ++d) off.o[d] = d < nd ? offs[d] : 0;
     for (int d = 0; d < kDiaMax; ++d) a.dia_off[d] = off.o[d];
@@ -3089,12 +3096,13 @@ static void build_dia(hipStream_t s, DevCsr &a, int nd, const int32_t *offs, boo
         const int32_t *rp = a.rp.p, *ci = a.ci.p;
         const double *v = a.v.p;
         double *dv = a.dia_v.p;
-        each(s, n, [=] __device__(int64_t i) {
+        each(s, nrow, [=] __device__(int64_t i) {
+            const int32_t base = dia_base((int32_t)i, nrow, ncol);
             for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
-                const int32_t o = ci[k] - (int32_t)i;
+                const int32_t o = ci[k] - base;
                 int d = 0;
                 while (d < nd - 1 && off.o[d] != o) ++d;
-                dv[(int64_t)d * n + i] = v[k];
+                dv[(int64_t)d * nrow + i] = v[k];
             }
         });
     }
@@ -3107,15 +3115,15 @@ static void build_dia(hipStream_t s, DevCsr &a, int nd, const int32_t *offs, boo
     HPRQP_HIP(hipStreamSynchronize(s));
 }
 
-// The diagonal form of Q against the shape chosen for its CSR form, in the
+// A matrix's diagonal form against the shape chosen for its CSR form, in the
 // iteration's precision (VT vectors, MT values); taken when 5 % faster
 template <class VT, class MT>
 static bool tune_dia(hipStream_t s, DevCsr &a, int nd, int reps, bool verbose) {
     const int64_t nr = std::max<int64_t>(1, a.nrow);
-    DevVec<VT> x(nr), y(nr), z(nr), rv(4 * nr);
+    DevVec<VT> x(std::max<int64_t>(1, a.ncol)), y(nr), z(nr), rv(4 * nr);
     {
         VT *xp = x.p, *rp = rv.p;
-        each(s, nr, [=] __device__(int64_t i) { xp[i] = (VT)1; });
+        each(s, a.ncol, [=] __device__(int64_t i) { xp[i] = (VT)1; });
         each(s, 4 * nr, [=] __device__(int64_t i) { rp[i] = (VT)0.5; });
     }
     const EpiTune<VT> epi{y.p, z.p, rv.p, rv.p + nr, rv.p + 2 * nr, rv.p + 3 * nr};
@@ -3126,52 +3134,59 @@ static bool tune_dia(hipStream_t s, DevCsr &a, int nd, int reps, bool verbose) {
     const double dia = tm.avg_ms(reps, [&] { csr_apply<VT, MT>(s, a, (const VT *)x.p, epi); });
     a.dia_nd = 0;
     if (verbose)
-        std::printf("row-product autotune   diagonal form (%d diagonals) %.4f ms against %.4f ms\n", nd, dia, csr);
+        std::printf("row-product autotune %d x %d: diagonal form (%d diagonals) %.4f ms against %.4f ms\n", a.nrow,
+                    a.ncol, nd, dia, csr);
     return dia < 0.95 * csr;
 }
 
-// Q in the diagonal form: forced (spmv_lanes = kLanesDiagonal), or where its
-// pattern has at most kDiaMax diagonals and the autotuner times that form
-// faster.  The reduced precisions keep FP64 diagonals too (the polish and the
-// residuals run in FP64).
+// Q, A and A^T in the diagonal form: forced (spmv_lanes = kLanesDiagonal:
+// Q must have one; A and A^T take theirs where they have one), or where a
+// matrix's pattern has at most kDiaMax diagonals and the autotuner times
+// that form faster.  The reduced precisions keep FP64 diagonals too (the
+// polish and the residuals run in FP64).
 static void apply_dia(hipStream_t s, const hprqp_model *md, const hprqp_params *p, Scaled &S, bool forced) {
-    DevCsr &a = S.Q;
-    if (S.qkind != QOP_CSR || S.q_empty) {
-        HPRQP_REQUIRE(!forced, HPRQP_E_ARG, "spmv_lanes = -2000 needs a sparse matrix Q");
-        return;
-    }
-    if (md->q_dia_nd == 0) md->q_dia_nd = dia_detect(md->Qsym, md->q_dia_off);
-    const int nd = md->q_dia_nd;
-    if (nd <= 0 || a.nrow != md->n) {
-        HPRQP_REQUIRE(!forced, HPRQP_E_ARG,
-                      "spmv_lanes = -2000: Q has more than 8 diagonals or more padding than 1.5 entries per nonzero");
-        return;
-    }
+    const bool q_csr = S.qkind == QOP_CSR && !S.q_empty;
+    HPRQP_REQUIRE(!forced || q_csr, HPRQP_E_ARG, "spmv_lanes = -2000 needs a sparse matrix Q");
+    DevCsr *mats[3] = {&S.Q, &S.A, &S.AT};
     const bool f32 = p->precision != HPRQP_PREC_FP64;
-    if (forced) {
-        build_dia(s, a, nd, md->q_dia_off, f32);
-        a.dia_nd = nd;
-        return;
-    }
-    if (a.nnz < (1 << 18)) return;
-    int &td = md->tuned_dia[p->precision];
-    if (td == 0) {
-        build_dia(s, a, nd, md->q_dia_off, f32);
-        bool take;
-        if (p->precision == HPRQP_PREC_MIXED)
-            take = tune_dia<double, float>(s, a, nd, 20, p->verbose);
-        else if (p->precision == HPRQP_PREC_FP32)
-            take = tune_dia<float, float>(s, a, nd, 20, p->verbose);
-        else
-            take = tune_dia<double, double>(s, a, nd, 20, p->verbose);
-        td = take ? 1 : -1;
-    }
-    if (td > 0) {
-        build_dia(s, a, nd, md->q_dia_off, f32);
-        a.dia_nd = nd;
-    } else {
-        a.dia_v.release();
-        a.dia_v32.release();
+    for (int k = 0; k < 3; ++k) {
+        DevCsr &a = *mats[k];
+        if ((k == 0 && !q_csr) || (k > 0 && md->m == 0) || a.nnz == 0) continue;
+        if (md->dia_nd[k] == 0)
+            md->dia_nd[k] = k == 0 ? dia_detect(md->Qsym, false, md->dia_off[k])
+                                   : dia_detect(md->A, k == 2, md->dia_off[k]);
+        const int nd = md->dia_nd[k];
+        if (nd <= 0) {
+            HPRQP_REQUIRE(!(forced && k == 0), HPRQP_E_ARG,
+                          "spmv_lanes = -2000: Q has more than 8 diagonals or more padding than 1.5 entries per "
+                          "nonzero");
+            continue;
+        }
+        if (forced) {
+            build_dia(s, a, nd, md->dia_off[k], f32);
+            a.dia_nd = nd;
+            continue;
+        }
+        if (a.nnz < (1 << 18)) continue;
+        int &td = md->tuned_dia[p->precision][k];
+        if (td == 0) {
+            build_dia(s, a, nd, md->dia_off[k], f32);
+            bool take;
+            if (p->precision == HPRQP_PREC_MIXED)
+                take = tune_dia<double, float>(s, a, nd, 20, p->verbose);
+            else if (p->precision == HPRQP_PREC_FP32)
+                take = tune_dia<float, float>(s, a, nd, 20, p->verbose);
+            else
+                take = tune_dia<double, double>(s, a, nd, 20, p->verbose);
+            td = take ? 1 : -1;
+        }
+        if (td > 0) {
+            build_dia(s, a, nd, md->dia_off[k], f32);
+            a.dia_nd = nd;
+        } else {
+            a.dia_v.release();
+            a.dia_v32.release();
+        }
     }
 }
 
@@ -3203,7 +3218,8 @@ static void apply_lanes(hipStream_t s, const hprqp_model *md, const hprqp_params
     (void)mixed;
     apply_qap_family(s, p, S);
     if (p->spmv_lanes == kLanesDiagonal) {
-        // Q in the diagonal form; A and A^T keep their default shapes
+        // Q in the diagonal form, and A and A^T where their patterns have
+        // one; the others keep their default shapes
         apply_dia(s, md, p, S, true);
         return;
     }

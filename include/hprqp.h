@@ -131,11 +131,13 @@ typedef struct {
                                    rows per block; 1048576 = column-chunked
                                    (x staged in LDS 4,096 columns at a time);
                                    -1032/-1064/-1128/-1256 = value-table form;
-                                   -2000 = Q in the diagonal form (its nonzeros
-                                   on at most 8 diagonals, each kept as an
-                                   array by row; A and A^T keep their default
-                                   shapes).  The autotuner also times the
-                                   diagonal form for such a Q */
+                                   -2000 = the diagonal form for Q, which must
+                                   have one, and for A and A^T where they have
+                                   one (nonzeros on at most 8 diagonals j =
+                                   floor(i ncol / nrow) + offset, each kept as
+                                   an array by row; the others keep their
+                                   default shapes).  The autotuner also times
+                                   the diagonal form for such matrices */
     int32_t time_kernels;       /* 1 = time the dominant fused kernel with
                                    HIP events during the solve (result.kernel_ms);
                                    one launch in eight carries the events */
@@ -214,7 +216,7 @@ typedef struct {
     int32_t lanes_AT;     /* -16 .. -256 CSR-stream rows per block, 1048576
                              column-chunked: x staged in LDS 4,096 columns at a
                              time, for long rows over many columns, -1000 - rb
-                             value-table form, -2000 Q's diagonal form) */
+                             value-table form, -2000 diagonal form) */
     int32_t qap_family;   /* QAP operator: family of the iteration's passes
                              (params.qap_family codes, 1 = one-tile; the
                              split-BF16 one in the reduced precisions) */

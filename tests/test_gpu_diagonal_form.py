@@ -1,12 +1,14 @@
-"""Q's diagonal form: a square Q whose nonzeros lie on at most 8 diagonals is
-kept as one array per diagonal, indexed by row (k_dia), forced with
-spmv_lanes = -2000 or taken by the autotuner where it times faster.
+"""The diagonal form: a matrix whose nonzeros lie on at most 8 diagonals j =
+floor(i ncol / nrow) + offset is kept as one array per diagonal, indexed by
+row (k_dia), forced with spmv_lanes = -2000 (Q must have one; A and A^T take
+theirs where they have one) or taken by the autotuner where it times faster.
+On an even grid Q (the 5-point Laplacian), A (the pair operator: 8 offsets
+from column 2i) and A^T (5 offsets from column floor(i / 2)) all have one.
 
-* Bitwise: the fused z/x/w1 and tempv half-steps, whose only matrix product
-  is Q's, write bitwise the same vectors with Q in the diagonal form as in
-  the CSR-stream form of 256 rows per block, from the same uploaded state
-  (both add a row's products in ascending column order; the padding adds
-  exact zeros).
+* Bitwise: the four fused half-steps write bitwise the same vectors with Q,
+  A and A^T in the diagonal form as in the CSR-stream form of 256 rows per
+  block, from the same uploaded state (both add a row's products in
+  ascending column order; the padding adds exact zeros).
 * Solves with the form forced match the oracle (iterations within 10 %,
   objective within 1e-6 relative, the power method's stop iterations within
   one) on the grid QP and on a banded QP with diagonals at offsets 0, +-1 and
@@ -21,8 +23,10 @@ from test_gpu_parity import _check, _solve_both
 
 pytestmark = pytest.mark.gpu
 
-VEC = {"w_bar": 3, "x": 7, "w": 8, "y": 9, "ATy": 10, "Qw": 11, "x_hat": 12, "dx": 13, "x_bar": 0,
-       "z_bar": 2, "tempv": 18, "Qw_bar": 19, "last_x": 20, "last_w": 22, "last_ATy": 23}
+VEC = {"x_bar": 0, "y_bar": 1, "z_bar": 2, "w_bar": 3, "ATdy": 4, "x": 7, "w": 8, "y": 9, "ATy": 10, "Qw": 11,
+       "x_hat": 12, "dx": 13, "dw": 14, "dy": 15, "s": 16, "ATy_bar": 17, "tempv": 18, "Qw_bar": 19,
+       "last_x": 20, "last_y": 21, "last_w": 22, "last_ATy": 23}
+M_VECS = {"y", "y_bar", "dy", "s", "last_y"}
 
 
 def _banded_qp(n=3000, m=700, seed=5):
@@ -44,27 +48,30 @@ def test_half_steps_bitwise_equal_to_csr_stream():
     from hprqp.steploop import Workspace
 
     model = instances.grid_qp(120, seed=3)
-    n = model.n
+    n, m = model.n, model.m
     dm = DeviceModel(model)
     ws = [Workspace(dm, HPRQP_parameters(verbose=False, spmv_lanes=lanes)) for lanes in (-256, -2000)]
     try:
         rng = np.random.default_rng(8)
-        state = {k: rng.uniform(-1.0, 1.0, n) for k in ("x", "w", "ATy", "last_x", "last_w", "last_ATy", "w_bar")}
+        state = {k: rng.uniform(-1.0, 1.0, m if k in M_VECS else n)
+                 for k in ("x", "w", "ATy", "last_x", "last_w", "last_ATy", "w_bar", "y", "last_y")}
         for w in ws:
             for k, v in state.items():
                 w.upload(VEC[k], v)
         for name, outs in (("zxw1", ("x", "x_hat", "w_bar", "Qw", "dx", "x_bar", "z_bar")),
-                           ("tempv", ("tempv",))):
+                           ("tempv", ("tempv",)),
+                           ("y", ("y", "y_bar", "s", "dy")),
+                           ("w2", ("w", "ATy", "w_bar", "dw", "ATdy", "ATy_bar"))):
             for w in ws:
                 w.step(name, 0.7, 0.25, 2.0, 8.0, True)
             for k in outs:
-                a, b = (w.download(VEC[k], n) for w in ws)
+                a, b = (w.download(VEC[k], m if k in M_VECS else n) for w in ws)
                 print(f"{name}: {k} max |diff| {np.max(np.abs(a - b)):.3e}")
                 assert np.array_equal(a, b), (name, k)
             # the product entered: Q w is not zero, tempv = x_hat + sigma (Q w - Q w_bar) is not x_hat
             if name == "zxw1":
                 assert np.any(ws[0].download(VEC["Qw"], n) != 0.0)
-            else:
+            elif name == "tempv":
                 assert np.any(ws[0].download(VEC["tempv"], n) != ws[0].download(VEC["x_hat"], n))
     finally:
         for w in ws:
@@ -79,7 +86,11 @@ def test_forced_diagonal_form_matches_oracle(case):
     model = instances.grid_qp(90, seed=4) if case == "grid" else _banded_qp()
     res, ref = _solve_both(model, spmv_lanes=-2000)
     print(f"{case}: shapes {res.lanes}")
-    assert res.lanes[0] == -2000
+    # the grid's A and A^T have diagonal forms, the banded QP's random A not
+    if case == "grid":
+        assert tuple(res.lanes) == (-2000, -2000, -2000)
+    else:
+        assert res.lanes[0] == -2000 and -2000 not in res.lanes[1:]
     _check(res, ref)
 
 
