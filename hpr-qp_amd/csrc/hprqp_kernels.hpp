@@ -362,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_dia(int32_t nrow, int32_t ncol, int 
 #pragma unroll
     for (int d = 0; d < kDiaMax; ++d) {
         if (d < nd) {
-            const int32_t j = base + off.o[d];
+            const int64_t j = (int64_t)base + off.o[d];
             const bool in = j >= 0 && j < ncol;
             const AT v = (AT)dv[(int64_t)d * nrow + i];
             const AT xv = in ? (AT)x[in ? j : 0] : (AT)0;
