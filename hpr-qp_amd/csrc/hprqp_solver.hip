@@ -184,7 +184,7 @@ struct Reducer {
     double *host;
     explicit Reducer(hprqp_ctx *ctx) : s(ctx->stream), host(ctx->pinned) {
         part.alloc((int64_t)kMaxReduceBlocks * 16);
-        out.alloc(64);
+        out.alloc(96);
     }
     // enqueue; results land in out[off .. off+NS+NM)
     template <int NS, int NM, class F>
@@ -1673,7 +1673,8 @@ struct Solver {
 
     using Resid = hprqp::Resid;
 
-    void compute_residuals(int64_t iter, double obj_constant, bool lasso, Resid &r) {
+    // the residual products and reductions, results at red.out + slot
+    void residuals_enqueue(int64_t iter, int slot) {
         flush();
         q_apply_d(x_bar.p, Qx_d.p);
         if (m > 0) {
@@ -1682,9 +1683,62 @@ struct Solver {
         }
         red.enqueue<4, 4>(n, ResN{S.c.p, Qx_d.p, ATdy_d.p, S.col_norm.p, S.l.p, S.u.p, x_bar.p, z_bar.p,
                                   S.c_scale, S.b_scale, lambda, m > 0, iter == 0},
-                          0);
-        red.enqueue<1, 2>(m, ResM{S.AL.p, S.AU.p, Ax_d.p, S.row_norm.p, y_bar.p, s_.p, S.b_scale}, 8);
-        const double *h = red.fetch(11);
+                          slot);
+        red.enqueue<1, 2>(m, ResM{S.AL.p, S.AU.p, Ax_d.p, S.row_norm.p, y_bar.p, s_.p, S.b_scale}, slot + 8);
+    }
+
+    void compute_residuals(int64_t iter, double obj_constant, bool lasso, Resid &r) {
+        residuals_enqueue(iter, 0);
+        residuals_from(red.fetch(11), iter, obj_constant, lasso, r);
+    }
+
+    // A print step's residuals without the wait (between two check
+    // iterations only the termination test acts on them): the reductions go
+    // to their own slot and pinned staging, an event marks them, and the
+    // iteration goes on; residuals_wait finishes them at the next iteration.
+    // The iterate of that step stays in x_bar and z_bar until the next full
+    // step; y_bar and w_bar, which the partial step rewrites, are copied
+    // aside (with x_bar and z_bar, for the paths whose partial steps write
+    // those) for a solve that stops there.  Not for the QAP one-apply form,
+    // whose pending updates and y buffer pair would have to be unwound.
+    static constexpr int kAsyncSlot = 64, kAsyncHost = 224;
+    hipEvent_t res_ev = nullptr;
+    DevVec<VT> snap_x, snap_z, snap_w, snap_y;
+    bool async_ok() const { return !qds; }
+    void residuals_start(int64_t iter) {
+        residuals_enqueue(iter, kAsyncSlot);
+        HPRQP_HIP(hipMemcpyAsync(red.host + kAsyncHost, red.out.p + kAsyncSlot, sizeof(double) * 11,
+                                 hipMemcpyDeviceToHost, s));
+        if (!res_ev) HPRQP_HIP(hipEventCreateWithFlags(&res_ev, hipEventDisableTiming));
+        HPRQP_HIP(hipEventRecord(res_ev, s));
+        if (snap_x.n != n) {
+            snap_x.alloc(n);
+            snap_z.alloc(n);
+            snap_w.alloc(n);
+            snap_y.alloc(m);
+        }
+        HPRQP_HIP(hipMemcpyAsync(snap_x.p, x_bar.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
+        HPRQP_HIP(hipMemcpyAsync(snap_z.p, z_bar.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
+        HPRQP_HIP(hipMemcpyAsync(snap_w.p, w_bar.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
+        if (m > 0) HPRQP_HIP(hipMemcpyAsync(snap_y.p, y_bar.p, sizeof(VT) * m, hipMemcpyDeviceToDevice, s));
+    }
+    void residuals_wait(int64_t iter, double obj_constant, bool lasso, Resid &r) {
+        HPRQP_HIP(hipEventSynchronize(res_ev));
+        residuals_from(red.host + kAsyncHost, iter, obj_constant, lasso, r);
+    }
+    // the solve stops at the print step: its iterate back in place
+    void restore_snapshot() {
+        HPRQP_HIP(hipMemcpyAsync(x_bar.p, snap_x.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
+        HPRQP_HIP(hipMemcpyAsync(z_bar.p, snap_z.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
+        HPRQP_HIP(hipMemcpyAsync(w_bar.p, snap_w.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
+        if (m > 0) HPRQP_HIP(hipMemcpyAsync(y_bar.p, snap_y.p, sizeof(VT) * m, hipMemcpyDeviceToDevice, s));
+        HPRQP_HIP(hipStreamSynchronize(s));
+    }
+    ~Solver() {
+        if (res_ev) (void)hipEventDestroy(res_ev);
+    }
+
+    void residuals_from(const double *h, int64_t iter, double obj_constant, bool lasso, Resid &r) const {
         const double bc = S.b_scale * S.c_scale;
         r.primal_obj = bc * (h[0] + 0.5 * h[1]) + obj_constant;
         r.dual_obj = bc * (-0.5 * h[1] + h[2]) + obj_constant;
@@ -2572,6 +2626,12 @@ struct OffsetSolver {
     void compute_residuals(int64_t iter, double obj_constant, bool lasso, Resid &r) {
         D.compute_residuals(iter, obj_constant, lasso, r);
     }
+    bool async_ok() const { return D.async_ok(); }
+    void residuals_start(int64_t iter) { D.residuals_start(iter); }
+    void residuals_wait(int64_t iter, double obj_constant, bool lasso, Resid &r) {
+        D.residuals_wait(iter, obj_constant, lasso, r);
+    }
+    void restore_snapshot() { D.restore_snapshot(); }
     double compute_M_norm() { return D.compute_M_norm(); }
 
     // hand the iteration to the FP64 form: its bulk vectors from the
@@ -2615,13 +2675,62 @@ static int run_loop(WS &W, LoopState &L, int64_t iter_begin, double t_start, dou
                     double *hist, int64_t hist_cap, bool mixed, Resid &res, bool has_res, bool use_switch = true) {
     const hprqp_params *p = W.p;
     Restart &r = L.r;
+    int64_t pend = -1;  // a print step whose residuals are still on their way
     for (int64_t iter = iter_begin; iter <= p->max_iter; ++iter) {
         L.iter = iter;
+        if (pend >= 0) {
+            // the previous print step's residuals: its record, line and
+            // termination test, as if they had been waited for there
+            const int64_t k = pend;
+            pend = -1;
+            W.residuals_wait(k, obj_constant, lasso, res);
+            if (hist && L.nh < hist_cap) {
+                double *h = hist + L.nh * 8;
+                h[0] = (double)k;
+                h[1] = res.err_Rp;
+                h[2] = res.err_Rd;
+                h[3] = res.rel_gap;
+                h[4] = res.primal_obj;
+                h[5] = res.dual_obj;
+                h[6] = W.sigma;
+                h[7] = (double)r.restart_flag;
+                L.nh++;
+            }
+            if (p->verbose)
+                std::printf("%7lld  %3.2e  %3.2e  %+.8e  %+.8e  %3.2e  %3.2e  %7.2f\n", (long long)k, res.err_Rp,
+                            res.err_Rd, res.primal_obj, res.dual_obj, res.rel_gap, W.sigma, now_s() - t_start);
+            if (res.kkt < 1e-4 && L.first_4) {
+                L.time_4 = now_s() - t_start;
+                L.iter_4 = k;
+                L.first_4 = false;
+            }
+            if (res.kkt < 1e-6 && L.first_6) {
+                L.time_6 = now_s() - t_start;
+                L.iter_6 = k;
+                L.first_6 = false;
+            }
+            if (res.kkt < p->stoptol) {
+                W.restore_snapshot();
+                L.iter = k;
+                return HPRQP_STATUS_OPTIMAL;
+            }
+        }
         const double tnow = now_s() - t_start;
         const int64_t ps = p->print_frequency == -1 ? print_step(iter) : p->print_frequency;
         const bool print_yes = (iter % ps == 0) || iter == p->max_iter || tnow > p->time_limit;
         const bool do_res = (iter % p->check_iter == 0) || print_yes || !has_res;
-        if (do_res) {
+        // a print step between check iterations whose next step is partial:
+        // its residuals run without a wait (only the termination test acts
+        // on them, at the next iteration)
+        const int64_t nx = iter + 1;
+        const int64_t ps_nx = p->print_frequency == -1 ? print_step(nx) : p->print_frequency;
+        const bool async = do_res && has_res && !p->auto_save && W.async_ok() && iter % p->check_iter != 0 &&
+                           iter < p->max_iter && tnow <= p->time_limit && nx % p->check_iter != 0 &&
+                           nx % ps_nx != 0;
+        if (async) {
+            W.residuals_start(iter);
+            pend = iter;
+        } else if (do_res) {
             W.compute_residuals(iter, obj_constant, lasso, res);
             has_res = true;
             if (p->auto_save) auto_save_track(W, L, iter, res);
@@ -2651,7 +2760,9 @@ static int run_loop(WS &W, LoopState &L, int64_t iter_begin, double t_start, dou
             }
         }
         int st = -2;
-        if (res.kkt < p->stoptol)
+        if (async) {
+            // decided at the next iteration, from this step's residuals
+        } else if (res.kkt < p->stoptol)
             st = HPRQP_STATUS_OPTIMAL;
         else if (iter == p->max_iter)
             st = HPRQP_STATUS_MAX_ITER;
@@ -2660,7 +2771,7 @@ static int run_loop(WS &W, LoopState &L, int64_t iter_begin, double t_start, dou
         r.check(iter, p->check_iter, W.sigma);
         update_sigma(W, r, res.err_Rd, res.err_Rp, res.rel_gap);
         do_restart(W, r);
-        if (do_res && hist && L.nh < hist_cap) {
+        if (do_res && !async && hist && L.nh < hist_cap) {
             double *h = hist + L.nh * 8;
             h[0] = (double)iter;
             h[1] = res.err_Rp;
@@ -2672,16 +2783,16 @@ static int run_loop(WS &W, LoopState &L, int64_t iter_begin, double t_start, dou
             h[7] = (double)r.restart_flag;
             L.nh++;
         }
-        if (p->verbose && (print_yes || st != -2))
+        if (p->verbose && !async && (print_yes || st != -2))
             std::printf("%7lld  %3.2e  %3.2e  %+.8e  %+.8e  %3.2e  %3.2e  %7.2f\n", (long long)iter, res.err_Rp,
                         res.err_Rd, res.primal_obj, res.dual_obj, res.rel_gap, W.sigma, now_s() - t_start);
         if (p->auto_save && (print_yes || st != -2)) auto_save_write(W, L, iter, res, now_s() - t_start);
-        if (res.kkt < 1e-4 && L.first_4) {
+        if (!async && res.kkt < 1e-4 && L.first_4) {
             L.time_4 = now_s() - t_start;
             L.iter_4 = iter;
             L.first_4 = false;
         }
-        if (res.kkt < 1e-6 && L.first_6) {
+        if (!async && res.kkt < 1e-6 && L.first_6) {
             L.time_6 = now_s() - t_start;
             L.iter_6 = iter;
             L.first_6 = false;

@@ -620,3 +620,32 @@ def test_empty_A_rows_with_the_column_chunked_shape(lanes):
     assert res.power_iter_A == ref.power_iter_A == 5000
     assert np.all(np.isfinite(res.x))
     np.testing.assert_allclose(res.x, ref.x, rtol=0, atol=1e-12)
+
+
+def test_history_and_stop_at_a_print_step():
+    """Print steps between two check iterations have their residuals read back
+    one iteration late (only the termination test acts on them).  A solve
+    that stops on such a step (660, below 1,000 where every tenth iteration
+    is a print step) must stop there with that step's iterate, and the
+    residual history -- every print and check step, in order -- must match
+    the oracle's row for row."""
+    from hprqp import HPRQP_parameters, instances, optimize
+    from oracle import oracle
+
+    model = instances.random_sparse_qp(400, 200, seed=0)
+    res = optimize(model, HPRQP_parameters(verbose=False))
+    ref = oracle.solve(model)
+    print(f"gpu iter={res.iter} rows={len(res.history)}; cpu iter={ref.iter} rows={len(ref.history)}")
+    assert res.iter == ref.iter and res.iter % 100 != 0 and res.iter < 1000
+    _check(res, ref)
+    h, g = res.history, ref.history
+    assert h.shape == g.shape
+    np.testing.assert_array_equal(h[:, 0], g[:, 0])
+    np.testing.assert_array_equal(h[:, 7], g[:, 7])
+    for col in (1, 2, 3):
+        np.testing.assert_allclose(h[:, col], g[:, col], rtol=1e-6, atol=1e-12)
+    # sigma: golden-section minimisers of functions formed from GPU and CPU
+    # dot products (3e-8 apart)
+    np.testing.assert_allclose(h[:, 6], g[:, 6], rtol=1e-6)
+    np.testing.assert_allclose(res.x, ref.x, atol=1e-8 * (1 + np.max(np.abs(ref.x))))
+    np.testing.assert_allclose(res.y, ref.y, atol=1e-8 * (1 + np.max(np.abs(ref.y))))
