@@ -11,8 +11,10 @@ from column 2i) and A^T (5 offsets from column floor(i / 2)) all have one.
   ascending column order; the padding adds exact zeros).
 * Solves with the form forced match the oracle (iterations within 10 %,
   objective within 1e-6 relative, the power method's stop iterations within
-  one) on the grid QP and on a banded QP with diagonals at offsets 0, +-1 and
-  +-37, and reach 1e-6 in the reduced precisions.
+  one) and report the form for exactly the matrices that have one: even and
+  odd grids, a banded QP (offsets 0, +-1, +-36, +-37), a 3-D 7-point
+  Laplacian and a tridiagonal Q with a restriction operator; the reduced
+  precisions reach 1e-6.
 * A Q on more than 8 diagonals is refused when the form is forced and keeps
   its CSR form under the autotuner."""
 import numpy as np
@@ -40,6 +42,40 @@ def _banded_qp(n=3000, m=700, seed=5):
     b = A @ rng.uniform(-1, 1, n)
     return build_from_QAbc(Q, rng.standard_normal(n), A.tocsr(), b - 0.5, b + 0.5, np.full(n, -2.0),
                            np.full(n, 2.0), verbose=False)
+
+
+def _grid3d_qp(N=24, seed=6):
+    """7-point Laplacian on an N^3 grid (7 diagonals) with a random sparse A."""
+    from hprqp import build_from_QAbc
+
+    rng = np.random.default_rng(seed)
+    n = N ** 3
+    L1 = sp.diags([np.full(N - 1, -1.0), np.full(N, 2.0), np.full(N - 1, -1.0)], [-1, 0, 1])
+    Id = sp.eye(N)
+    Q = (sp.kron(sp.kron(L1, Id), Id) + sp.kron(sp.kron(Id, L1), Id) + sp.kron(sp.kron(Id, Id), L1)
+         + 1e-2 * sp.eye(n)).tocsr()
+    m = n // 3
+    A = (sp.random(m, n, density=4.0 / n, random_state=seed, format="csr") + sp.eye(m, n)).tocsr()
+    b = A @ rng.uniform(0, 1, n)
+    return build_from_QAbc(Q, rng.standard_normal(n), A, b, b + 0.1, np.zeros(n), np.full(n, 2.0), verbose=False)
+
+
+def _restriction_qp(n=40000, seed=7):
+    """Tridiagonal Q and a restriction operator A (row i: 1, 2, 1 at columns
+    2i-1, 2i, 2i+1): A has 3 diagonals from column 2i, A^T 2 from floor(i/2)."""
+    from hprqp import build_from_QAbc
+
+    rng = np.random.default_rng(seed)
+    m = n // 2
+    Q = (sp.diags([np.full(n - 1, -1.0), np.full(n, 2.05), np.full(n - 1, -1.0)], [-1, 0, 1])).tocsr()
+    r = np.arange(m)
+    rows, cols = np.concatenate([r, r, r]), np.concatenate([2 * r - 1, 2 * r, 2 * r + 1])
+    vals = np.concatenate([np.ones(m), 2 * np.ones(m), np.ones(m)])
+    keep = (cols >= 0) & (cols < n)
+    A = sp.csr_matrix((vals[keep], (rows[keep], cols[keep])), shape=(m, n))
+    b = A @ rng.uniform(-1, 1, n)
+    return build_from_QAbc(Q, rng.standard_normal(n), A, b - 0.2, b + 0.2, np.full(n, -3.0), np.full(n, 3.0),
+                           verbose=False)
 
 
 def test_half_steps_bitwise_equal_to_csr_stream():
@@ -79,18 +115,21 @@ def test_half_steps_bitwise_equal_to_csr_stream():
         dm.close()
 
 
-@pytest.mark.parametrize("case", ["grid", "banded"])
+# the shapes each case must report: whose patterns have a diagonal form
+SHAPES = {"grid": (True, True, True), "grid_odd": (True, True, False), "banded": (True, False, False),
+          "grid3d": (True, False, False), "restriction": (True, True, True)}
+
+
+@pytest.mark.parametrize("case", list(SHAPES))
 def test_forced_diagonal_form_matches_oracle(case):
     from hprqp import instances
 
-    model = instances.grid_qp(90, seed=4) if case == "grid" else _banded_qp()
+    model = {"grid": lambda: instances.grid_qp(90, seed=4), "grid_odd": lambda: instances.grid_qp(91, seed=4),
+             "banded": _banded_qp, "grid3d": _grid3d_qp, "restriction": _restriction_qp}[case]()
     res, ref = _solve_both(model, spmv_lanes=-2000)
     print(f"{case}: shapes {res.lanes}")
-    # the grid's A and A^T have diagonal forms, the banded QP's random A not
-    if case == "grid":
-        assert tuple(res.lanes) == (-2000, -2000, -2000)
-    else:
-        assert res.lanes[0] == -2000 and -2000 not in res.lanes[1:]
+    # an odd grid's A^T needs 6 diagonals at 1.52 entries per nonzero (over the padding limit)
+    assert tuple(lanes == -2000 for lanes in res.lanes) == SHAPES[case]
     _check(res, ref)
 
 
