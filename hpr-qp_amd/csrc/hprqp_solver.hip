@@ -1699,12 +1699,11 @@ struct Solver {
     // The iterate of that step stays in x_bar and z_bar until the next full
     // step; y_bar and w_bar, which the partial step rewrites, are copied
     // aside (with x_bar and z_bar, for the paths whose partial steps write
-    // those) for a solve that stops there.  Not for the QAP one-apply form,
-    // whose pending updates and y buffer pair would have to be unwound.
+    // those) for a solve that stops there.
     static constexpr int kAsyncSlot = 64, kAsyncHost = 224;
     hipEvent_t res_ev = nullptr;
     DevVec<VT> snap_x, snap_z, snap_w, snap_y;
-    bool async_ok() const { return !qds; }
+    bool async_ok() const { return true; }
     void residuals_start(int64_t iter) {
         residuals_enqueue(iter, kAsyncSlot);
         HPRQP_HIP(hipMemcpyAsync(red.host + kAsyncHost, red.out.p + kAsyncSlot, sizeof(double) * 11,
@@ -1726,8 +1725,12 @@ struct Solver {
         HPRQP_HIP(hipEventSynchronize(res_ev));
         residuals_from(red.host + kAsyncHost, iter, obj_constant, lasso, r);
     }
-    // the solve stops at the print step: its iterate back in place
+    // the solve stops at the print step: its iterate back in place.  The
+    // QAP one-apply form's update left pending by the step after it (the
+    // second w update, and for nq < 256 the y update) belongs to that later
+    // step and is dropped, as the waiting form never ran it.
     void restore_snapshot() {
+        pend = false;
         HPRQP_HIP(hipMemcpyAsync(x_bar.p, snap_x.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
         HPRQP_HIP(hipMemcpyAsync(z_bar.p, snap_z.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
         HPRQP_HIP(hipMemcpyAsync(w_bar.p, snap_w.p, sizeof(VT) * n, hipMemcpyDeviceToDevice, s));
