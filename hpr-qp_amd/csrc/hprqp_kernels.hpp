@@ -350,22 +350,52 @@ __device__ __host__ __forceinline__ int32_t dia_base(int32_t i, int32_t nrow, in
     return nrow == ncol ? i : (int32_t)(((int64_t)i * ncol) / nrow);
 }
 
-template <class MT, class XT, class AT, class Epi>
-__global__ __launch_bounds__(kBlock) void k_dia(int32_t nrow, int32_t ncol, int nd, DiaOff off,
+// PAIR (an instance of its own, launched only for matrices with pairs):
+// bit d of pairs set when diagonals d and d + 1 read x[j] and x[j + 1] with
+// j even in every row (an even slope ncol / nrow, an even off[d], off[d + 1]
+// = off[d] + 1, x aligned to two elements) -- one two-element load serves
+// both, halving the cache lines a wave's strided x loads touch
+template <class XT>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using T = double2;
+};
+template <>
+struct Vec2<float> {
+    using T = float2;
+};
+
+template <bool PAIR, class MT, class XT, class AT, class Epi>
+__global__ __launch_bounds__(kBlock) void k_dia(int32_t nrow, int32_t ncol, int nd, DiaOff off, unsigned pairs,
                                                const MT *__restrict__ dv, const XT *__restrict__ x, Epi epi) {
 #pragma clang fp contract(off)
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int32_t i = b * kBlock + (int32_t)threadIdx.x;
     if (i >= nrow) return;
     const int32_t base = dia_base(i, nrow, ncol);
-    AT acc = 0;
+    AT acc = 0, carry = 0;
 #pragma unroll
     for (int d = 0; d < kDiaMax; ++d) {
         if (d < nd) {
             const int64_t j = (int64_t)base + off.o[d];
             const bool in = j >= 0 && j < ncol;
             const AT v = (AT)dv[(int64_t)d * nrow + i];
-            const AT xv = in ? (AT)x[in ? j : 0] : (AT)0;
+            AT xv;
+            if (PAIR && d > 0 && ((pairs >> (d - 1)) & 1u)) {
+                xv = carry;  // the second element of the previous diagonal's load
+            } else if (PAIR && ((pairs >> d) & 1u)) {
+                if (in && j + 1 < ncol) {
+                    const typename Vec2<XT>::T t = *reinterpret_cast<const typename Vec2<XT>::T *>(x + j);
+                    xv = (AT)t.x;
+                    carry = (AT)t.y;
+                } else {
+                    xv = in ? (AT)x[j] : (AT)0;
+                    carry = (j + 1 >= 0 && j + 1 < ncol) ? (AT)x[j + 1] : (AT)0;
+                }
+            } else {
+                xv = in ? (AT)x[in ? j : 0] : (AT)0;
+            }
             const AT pr = v * xv;
             acc += pr;
         }

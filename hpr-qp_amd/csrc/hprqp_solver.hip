@@ -128,8 +128,22 @@ static bool launch_dia(hipStream_t s, const DevCsr &a, const XT *x, const Epi &e
     if (a.dia_nd <= 0 || a.nrow <= 0 || dv == nullptr) return false;
     DiaOff off;
     for (int d = 0; d < kDiaMax; ++d) off.o[d] = d < a.dia_nd ? a.dia_off[d] : 0;
-    launch_k(k_dia<MT, XT, AT, Epi>, dim3(nblk(a.nrow)), dim3(kBlock), 0, s, a.nrow, a.ncol, a.dia_nd, off, dv, x,
-             epi);
+    // one two-element load of x for neighbouring diagonals whose pair starts
+    // at an even column in every row, x aligned to two elements
+    unsigned pairs = 0;
+    const bool even_slope = a.ncol % a.nrow == 0 && (a.ncol / a.nrow) % 2 == 0;
+    if (even_slope && ((uintptr_t)x % (2 * sizeof(XT))) == 0)
+        for (int d = 0; d + 1 < a.dia_nd; ++d)
+            if (off.o[d] % 2 == 0 && off.o[d + 1] == off.o[d] + 1) {
+                pairs |= 1u << d;
+                ++d;
+            }
+    if (pairs)
+        launch_k(k_dia<true, MT, XT, AT, Epi>, dim3(nblk(a.nrow)), dim3(kBlock), 0, s, a.nrow, a.ncol, a.dia_nd,
+                 off, pairs, dv, x, epi);
+    else
+        launch_k(k_dia<false, MT, XT, AT, Epi>, dim3(nblk(a.nrow)), dim3(kBlock), 0, s, a.nrow, a.ncol, a.dia_nd,
+                 off, 0u, dv, x, epi);
     HPRQP_HIP(hipGetLastError());
     return true;
 }
